@@ -1,0 +1,453 @@
+/*
+ * oracle.c -- CPU restatement of the reference's 2D RKDG sweep, Pmv/Smv/Lmv, source iteration and
+ * sweep-driven Peierls GMRES (crockatt/idc-transport-solver).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path, never part of it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library.
+ *
+ * Written from the mathematics the reference implements; each function cites the reference lines
+ * whose behaviour (including floating-point operation order where it is cheap to keep) it follows.
+ * Layouts and conventions: see oracle.h.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+
+/* (-1)^k as in global.hpp:490. */
+static inline double sgn_pow(int k) { return (k % 2) ? -1.0 : 1.0; }
+
+static inline size_t psi_idx(int ny, int nq, int p, int q, int i, int j, int d, int e)
+{
+    const size_t P = (size_t)(p + 1);
+    /* OrdinateFlux.hpp:639-700: e + (p+1)*(d + (p+1)*(q + nq*(j + (ny+2)*i))) */
+    return (size_t)e + P * ((size_t)d + P * ((size_t)q + (size_t)nq * ((size_t)j + (size_t)(ny + 2) * (size_t)i)));
+}
+
+static inline size_t phi_idx(int ny, int p, int i, int j, int d, int e)
+{
+    const size_t P = (size_t)(p + 1);
+    return (size_t)e + P * ((size_t)d + P * ((size_t)j + (size_t)(ny + 2) * (size_t)i));
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Angular matrices.  SweepLinearSolveSpec.cpp:98-158 (loop over quadrants, lanes, d, e).          */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_aq(int p, int nq, const double *xi, const double *eta, double dx, double dy, double *Aq)
+{
+    const int P = p + 1, N = P * P;
+    memset(Aq, 0, sizeof(double) * (size_t)nq * N * N);
+    for (int q = 0; q < nq; ++q) {
+        double *A = Aq + (size_t)q * N * N;
+        const double xi_coeff = dy * xi[q];   /* dx(1) * xi, :117 */
+        const double eta_coeff = dx * eta[q]; /* dx(0) * eta, :118 */
+#define AQ(d, e, a, b) A[((e) + P * (d)) + N * ((b) + P * (a))]
+        for (int d = 0; d <= p; ++d) {
+            for (int e = 0; e <= p; ++e) {
+                /* xi direction, :124-138 */
+                for (int a = d - 1; a >= 0; a -= 2) AQ(d, e, a, e) -= 2.0 * xi_coeff * (2 * d + 1);
+                if (xi[q] < 0) {
+                    for (int a = 0; a <= p; ++a) AQ(d, e, a, e) += sgn_pow(d + a + 1) * xi_coeff * (2 * d + 1);
+                } else {
+                    for (int a = 0; a <= p; ++a) AQ(d, e, a, e) += xi_coeff * (2 * d + 1);
+                }
+                /* eta direction, :141-155 */
+                for (int b = e - 1; b >= 0; b -= 2) AQ(d, e, d, b) -= 2.0 * eta_coeff * (2 * e + 1);
+                if (eta[q] < 0) {
+                    for (int b = 0; b <= p; ++b) AQ(d, e, d, b) += sgn_pow(e + b + 1) * eta_coeff * (2 * e + 1);
+                } else {
+                    for (int b = 0; b <= p; ++b) AQ(d, e, d, b) += eta_coeff * (2 * e + 1);
+                }
+            }
+        }
+#undef AQ
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Unpivoted GE.  SIMD_LinearAlgebra.hpp:78-115, one lane.                                        */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_ge_solve(int N, double *A, double *b)
+{
+#define M(i, j) A[(i) + N * (j)]
+    for (int j = 0; j < N; ++j) {
+        for (int i = j + 1; i < N; ++i) {
+            M(i, j) /= -M(j, j);
+            b[i] += M(i, j) * b[j];
+        }
+        for (int k = j + 1; k < N; ++k)
+            for (int i = j + 1; i < N; ++i) M(i, k) += M(i, j) * M(j, k);
+    }
+    for (int i = N - 1; i >= 0; --i) {
+        b[i] /= M(i, i);
+        for (int j = 0; j < i; ++j) b[j] -= M(j, i) * b[i];
+    }
+#undef M
+}
+
+/* One cell, one ordinate: CalcB (SweepLinearSolveSpec.cpp:473-598), CalcA (:284-298),
+ * SolveSystem, StoreResult (SweepCellSolve.cpp:86-99). */
+static void cell_solve(int nx, int ny, int p, int nq, const double *xi, const double *eta, double dx,
+                       double dy, const double *Aq, const double *sigma_t, double inv_dt, int q, int i,
+                       int j, double *psi, double *A, double *B, double *W0, double *W1)
+{
+    (void)nx;
+    const int P = p + 1, N = P * P;
+    const int xneg = xi[q] <= 0.0, yneg = eta[q] <= 0.0; /* :504-505 */
+    const int xoff = xneg ? +1 : -1, yoff = yneg ? +1 : -1;
+
+    for (int d = 0; d <= p; ++d)
+        for (int e = 0; e <= p; ++e) {
+            W0[e + P * d] = psi[psi_idx(ny, nq, p, q, i + xoff, j, d, e)] * (dy * xi[q]);
+            W1[e + P * d] = psi[psi_idx(ny, nq, p, q, i, j + yoff, d, e)] * (dx * eta[q]);
+        }
+    const double dx_coeff = dx * dy;
+    for (int d = 0; d <= p; ++d)
+        for (int e = 0; e <= p; ++e) B[e + P * d] = dx_coeff * psi[psi_idx(ny, nq, p, q, i, j, d, e)];
+    for (int d = 0; d <= p; ++d)
+        for (int e = 0; e <= p; ++e)
+            for (int a = 0; a <= p; ++a) {
+                const double c = xneg ? sgn_pow(a + 1) * (2 * d + 1) : sgn_pow(d) * (2 * d + 1);
+                B[e + P * d] += W0[e + P * a] * c;
+            }
+    for (int d = 0; d <= p; ++d)
+        for (int e = 0; e <= p; ++e)
+            for (int b = 0; b <= p; ++b) {
+                const double c = yneg ? sgn_pow(b + 1) * (2 * e + 1) : sgn_pow(e) * (2 * e + 1);
+                B[e + P * d] += W1[b + P * d] * c;
+            }
+
+    memcpy(A, Aq + (size_t)q * N * N, sizeof(double) * N * N);
+    const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
+    const double coeff = (sig + inv_dt) * dx * dy;
+    for (int r = 0; r < N; ++r) A[r + N * r] += coeff;
+
+    orc_ge_solve(N, A, B);
+
+    for (int d = 0; d <= p; ++d)
+        for (int e = 0; e <= p; ++e) psi[psi_idx(ny, nq, p, q, i, j, d, e)] = B[e + P * d];
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Wavefront traversal.  SweepPatternUniCartWavefront.cpp:253-296 (all quadrants at once),        */
+/* diagonal length SweepPatternUniCart.cpp:238-254, quadrant signs OrdinateSet.cpp:658-700.       */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+               double dx, double dy, const double *sigma_t, double dt, double *psi)
+{
+    const int P = p + 1, N = P * P;
+    double *Aq = (double *)malloc(sizeof(double) * (size_t)nq * N * N);
+    double *A = (double *)malloc(sizeof(double) * N * N);
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *W0 = (double *)malloc(sizeof(double) * N);
+    double *W1 = (double *)malloc(sizeof(double) * N);
+    orc_aq(p, nq, xi, eta, dx, dy, Aq);
+    const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+
+    const int num_diags = nx + ny - 1;
+    for (int diag = 1; diag <= num_diags; ++diag) {
+        /* ComputeDiagLength: cells (i,j), 1<=i<=nx, 1<=j<=ny, i+j-1 == diag */
+        const int i_hi = diag < nx ? diag : nx;
+        const int i_lo = diag - ny + 1 > 1 ? diag - ny + 1 : 1;
+        const int diag_length = i_hi - i_lo + 1;
+        for (int cell = 1; cell <= diag_length; ++cell) {
+            for (int quad = 0; quad < 4; ++quad) {
+                for (int q = quad_off[quad]; q < quad_off[quad + 1]; ++q) {
+                    int i = i_hi - cell + 1;
+                    int j = diag - i + 1;
+                    if (quad >= 2) i = nx - i + 1;      /* xi < 0 */
+                    if (quad & 1) j = ny - j + 1;       /* eta < 0 */
+                    cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, inv_dt, q, i, j, psi, A, B, W0,
+                               W1);
+                }
+            }
+        }
+    }
+    free(Aq); free(A); free(B); free(W0); free(W1);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Moment contraction and scattering source.  TransportOperator.cpp:140-158 and :355-392.         */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_pmv(int nx, int ny, int p, int nq, const double *w, double alpha, double beta, const double *psi,
+             double *phi, int all_cells)
+{
+    const int o = all_cells ? 1 : 0;
+    for (int i = 1 - o; i <= nx + o; ++i)
+        for (int j = 1 - o; j <= ny + o; ++j) {
+            for (int d = 0; d <= p; ++d)
+                for (int e = 0; e <= p; ++e) phi[phi_idx(ny, p, i, j, d, e)] *= beta;
+            for (int q = 0; q < nq; ++q)
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e)
+                        phi[phi_idx(ny, p, i, j, d, e)] += alpha * w[q] * psi[psi_idx(ny, nq, p, q, i, j, d, e)];
+        }
+}
+
+void orc_smv(int nx, int ny, int p, int nq, double alpha, double beta, const double *sigma_s,
+             const double *phi, double *psi, int all_cells)
+{
+    const int o = all_cells ? 1 : 0;
+    const int P = p + 1;
+    double val[16];
+    for (int i = 1 - o; i <= nx + o; ++i)
+        for (int j = 1 - o; j <= ny + o; ++j) {
+            const double s = sigma_s[(size_t)j + (size_t)(ny + 2) * i];
+            for (int d = 0; d <= p; ++d)
+                for (int e = 0; e <= p; ++e)
+                    val[e + P * d] = alpha * (s * phi[phi_idx(ny, p, i, j, d, e)] / (4.0 * ORC_PI));
+            for (int q = 0; q < nq; ++q)
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) {
+                        double *y = &psi[psi_idx(ny, nq, p, q, i, j, d, e)];
+                        *y = beta * *y + val[e + P * d];
+                    }
+        }
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Explicit transport operator.  TransportOperator.cpp:1576-1658 (2D, unblocked).                 */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_lmv(int nx, int ny, int p, int nq, const double *xi, const double *eta, double dx, double dy,
+             double alpha, double beta, const double *sigma_t, const double *x, double *y)
+{
+#define X(q, i, j, d, e) x[psi_idx(ny, nq, p, q, i, j, d, e)]
+#define Y(q, i, j, d, e) y[psi_idx(ny, nq, p, q, i, j, d, e)]
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            for (int q = 0; q < nq; ++q) {
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) *= beta;
+                const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) += alpha * sig * X(q, i, j, d, e);
+
+                double coeff = alpha * xi[q] / dx;
+                if (xi[q] < 0) {
+                    for (int d = 0; d <= p; ++d)
+                        for (int a = 0; a <= p; ++a)
+                            for (int e = 0; e <= p; ++e)
+                                Y(q, i, j, d, e) += coeff * (2 * d + 1) *
+                                                    (sgn_pow(a) * X(q, i + 1, j, a, e) + sgn_pow(d + a + 1) * X(q, i, j, a, e));
+                } else {
+                    for (int d = 0; d <= p; ++d)
+                        for (int a = 0; a <= p; ++a)
+                            for (int e = 0; e <= p; ++e)
+                                Y(q, i, j, d, e) += coeff * (2 * d + 1) * (X(q, i, j, a, e) + sgn_pow(d + 1) * X(q, i - 1, j, a, e));
+                }
+                coeff *= -2.0;
+                for (int d = 0; d <= p; ++d)
+                    for (int a = d - 1; a >= 0; a -= 2)
+                        for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) += coeff * (2 * d + 1) * X(q, i, j, a, e);
+
+                coeff = alpha * eta[q] / dy;
+                if (eta[q] < 0) {
+                    for (int d = 0; d <= p; ++d)
+                        for (int e = 0; e <= p; ++e)
+                            for (int b = 0; b <= p; ++b)
+                                Y(q, i, j, d, e) += coeff * (2 * e + 1) *
+                                                    (sgn_pow(b) * X(q, i, j + 1, d, b) + sgn_pow(e + b + 1) * X(q, i, j, d, b));
+                } else {
+                    for (int d = 0; d <= p; ++d)
+                        for (int e = 0; e <= p; ++e)
+                            for (int b = 0; b <= p; ++b)
+                                Y(q, i, j, d, e) += coeff * (2 * e + 1) * (X(q, i, j, d, b) + sgn_pow(e + 1) * X(q, i, j - 1, d, b));
+                }
+                coeff *= -2.0;
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e)
+                        for (int b = e - 1; b >= 0; b -= 2) Y(q, i, j, d, e) += coeff * (2 * e + 1) * X(q, i, j, d, b);
+            }
+#undef X
+#undef Y
+}
+
+double orc_l2dist(int nx, int ny, int p, const double *a, const double *b)
+{
+    const int N = (p + 1) * (p + 1);
+    double dist = 0.0;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            for (int l = 0; l < N; ++l) {
+                const size_t k = (size_t)l + (size_t)N * ((size_t)j + (size_t)(ny + 2) * i);
+                const double t = a[k] - b[k];
+                dist += t * t;
+            }
+    return sqrt(dist);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Source iteration.  SISolver.cpp:206-249.                                                       */
+/* ---------------------------------------------------------------------------------------------- */
+int orc_si_solve(int nx, int ny, int p, int nq, const double *xi, const double *eta, const double *w,
+                 const int *quad_off, double dx, double dy, const double *sigma_t, const double *sigma_s,
+                 double dt, double abs_tol, int max_its, double *psi, double *source, int *iters,
+                 double *err)
+{
+    const int N = (p + 1) * (p + 1);
+    const size_t npsi = (size_t)(nx + 2) * (ny + 2) * nq * N;
+    const size_t nphi = (size_t)(nx + 2) * (ny + 2) * N;
+    double *phi_a = (double *)calloc(nphi, sizeof(double));
+    double *phi_b = (double *)calloc(nphi, sizeof(double));
+    double *cur = phi_a, *last = phi_b;
+    int iter = 0;
+    double error = 1.79769313486231570e+308;
+
+    orc_pmv(nx, ny, p, nq, w, 1.0, 0.0, psi, cur, 0);
+    while (iter < max_its && error > abs_tol) {
+        ++iter;
+        double *t = cur; cur = last; last = t;
+        memcpy(psi, source, sizeof(double) * npsi);                        /* :226 */
+        orc_smv(nx, ny, p, nq, 1.0, 1.0, sigma_s, last, psi, 0);           /* :229 */
+        orc_sweep(nx, ny, p, nq, xi, eta, quad_off, dx, dy, sigma_t, dt, psi); /* :232 */
+        orc_pmv(nx, ny, p, nq, w, 1.0, 0.0, psi, cur, 0);                 /* :236 */
+        error = orc_l2dist(nx, ny, p, cur, last);                          /* :239 */
+    }
+    memcpy(source, psi, sizeof(double) * npsi);                           /* :249 */
+    free(phi_a); free(phi_b);
+    if (iters) *iters = iter;
+    if (err) *err = error;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Peierls-form GMRES(m).  PETScPeierlsSolver.cpp:458-505 and IPLS :548-555.                      */
+/* ---------------------------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, p, nq;
+    const double *xi, *eta, *w;
+    const int *quad_off;
+    double dx, dy, dt;
+    const double *sigma_t, *sigma_s;
+    double *psi, *phi;
+} peierls_ctx;
+
+static void pack_interior(const peierls_ctx *c, const double *phi, double *v)
+{
+    const int N = (c->p + 1) * (c->p + 1);
+    size_t k = 0;
+    for (int i = 1; i <= c->nx; ++i)
+        for (int j = 1; j <= c->ny; ++j)
+            for (int l = 0; l < N; ++l) v[k++] = phi[(size_t)l + (size_t)N * ((size_t)j + (size_t)(c->ny + 2) * i)];
+}
+
+static void unpack_interior(const peierls_ctx *c, const double *v, double *phi)
+{
+    const int N = (c->p + 1) * (c->p + 1);
+    size_t k = 0;
+    for (int i = 1; i <= c->nx; ++i)
+        for (int j = 1; j <= c->ny; ++j)
+            for (int l = 0; l < N; ++l) phi[(size_t)l + (size_t)N * ((size_t)j + (size_t)(c->ny + 2) * i)] = v[k++];
+}
+
+/* y = (I - P L^{-1} S) x, zero inflow. */
+static void peierls_apply(peierls_ctx *c, const double *x, double *y)
+{
+    const int N = (c->p + 1) * (c->p + 1);
+    const size_t npsi = (size_t)(c->nx + 2) * (c->ny + 2) * c->nq * N;
+    const size_t nphi = (size_t)(c->nx + 2) * (c->ny + 2) * N;
+    memset(c->phi, 0, sizeof(double) * nphi);
+    unpack_interior(c, x, c->phi);
+    memset(c->psi, 0, sizeof(double) * npsi);
+    orc_smv(c->nx, c->ny, c->p, c->nq, 1.0, 0.0, c->sigma_s, c->phi, c->psi, 1);
+    orc_sweep(c->nx, c->ny, c->p, c->nq, c->xi, c->eta, c->quad_off, c->dx, c->dy, c->sigma_t, c->dt, c->psi);
+    orc_pmv(c->nx, c->ny, c->p, c->nq, c->w, -1.0, 1.0, c->psi, c->phi, 0);
+    pack_interior(c, c->phi, y);
+}
+
+static double dot(size_t n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (size_t k = 0; k < n; ++k) s += a[k] * b[k];
+    return s;
+}
+
+int orc_peierls_gmres(int nx, int ny, int p, int nq, const double *xi, const double *eta, const double *w,
+                      const int *quad_off, double dx, double dy, const double *sigma_t,
+                      const double *sigma_s, double dt, double abs_tol, int max_its, int restart,
+                      double *source, double *phi_out, int *iters, double *resid)
+{
+    const int N = (p + 1) * (p + 1);
+    const size_t n = (size_t)nx * ny * N;
+    const size_t npsi = (size_t)(nx + 2) * (ny + 2) * nq * N;
+    const size_t nphi = (size_t)(nx + 2) * (ny + 2) * N;
+    const int m = restart;
+    peierls_ctx c = {nx, ny, p, nq, xi, eta, w, quad_off, dx, dy, dt, sigma_t, sigma_s,
+                     (double *)malloc(sizeof(double) * npsi), (double *)calloc(nphi, sizeof(double))};
+    double *b = (double *)malloc(sizeof(double) * n);
+    double *x = (double *)calloc(n, sizeof(double));
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *V = (double *)malloc(sizeof(double) * n * (m + 1));
+    double *H = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = (double *)malloc(sizeof(double) * m), *sn = (double *)malloc(sizeof(double) * m);
+    double *g = (double *)malloc(sizeof(double) * (m + 1)), *yv = (double *)malloc(sizeof(double) * m);
+
+    /* RHS = P L^{-1} Q (:461-466) */
+    memcpy(c.psi, source, sizeof(double) * npsi);
+    orc_sweep(nx, ny, p, nq, xi, eta, quad_off, dx, dy, sigma_t, dt, c.psi);
+    memset(c.phi, 0, sizeof(double) * nphi);
+    orc_pmv(nx, ny, p, nq, w, 1.0, 0.0, c.psi, c.phi, 0);
+    pack_interior(&c, c.phi, b);
+
+    int it = 0, converged = 0;
+    double beta = 0.0;
+    while (!converged && it < max_its) {
+        peierls_apply(&c, x, r);
+        for (size_t k = 0; k < n; ++k) r[k] = b[k] - r[k];
+        beta = sqrt(dot(n, r, r));
+        if (beta <= abs_tol) { converged = 1; break; }
+        for (size_t k = 0; k < n; ++k) V[k] = r[k] / beta;
+        memset(g, 0, sizeof(double) * (m + 1));
+        g[0] = beta;
+        int jdone = 0;
+        for (int j = 0; j < m && it < max_its; ++j) {
+            double *vj = V + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
+            peierls_apply(&c, vj, vn);
+            ++it;
+            for (int i = 0; i <= j; ++i) { /* modified Gram-Schmidt */
+                const double h = dot(n, vn, V + (size_t)i * n);
+                H[i + (m + 1) * j] = h;
+                for (size_t k = 0; k < n; ++k) vn[k] -= h * V[(size_t)i * n + k];
+            }
+            const double hn = sqrt(dot(n, vn, vn));
+            H[j + 1 + (m + 1) * j] = hn;
+            if (hn > 0.0) for (size_t k = 0; k < n; ++k) vn[k] /= hn;
+            for (int i = 0; i < j; ++i) { /* apply previous rotations */
+                const double h0 = H[i + (m + 1) * j], h1 = H[i + 1 + (m + 1) * j];
+                H[i + (m + 1) * j] = cs[i] * h0 + sn[i] * h1;
+                H[i + 1 + (m + 1) * j] = -sn[i] * h0 + cs[i] * h1;
+            }
+            const double h0 = H[j + (m + 1) * j], h1 = H[j + 1 + (m + 1) * j];
+            const double den = sqrt(h0 * h0 + h1 * h1);
+            cs[j] = h0 / den; sn[j] = h1 / den;
+            H[j + (m + 1) * j] = den; H[j + 1 + (m + 1) * j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jdone = j + 1;
+            if (fabs(g[j + 1]) <= abs_tol) { converged = 1; break; }
+        }
+        for (int i = jdone - 1; i >= 0; --i) { /* back substitution and update */
+            double s = g[i];
+            for (int k = i + 1; k < jdone; ++k) s -= H[i + (m + 1) * k] * yv[k];
+            yv[i] = s / H[i + (m + 1) * i];
+        }
+        for (int i = 0; i < jdone; ++i)
+            for (size_t k = 0; k < n; ++k) x[k] += yv[i] * V[(size_t)i * n + k];
+        beta = fabs(g[jdone]);
+    }
+
+    /* psi = L^{-1}(Q + S phi) (:504-505) */
+    memset(c.phi, 0, sizeof(double) * nphi);
+    unpack_interior(&c, x, c.phi);
+    orc_smv(nx, ny, p, nq, 1.0, 1.0, sigma_s, c.phi, source, 1);
+    orc_sweep(nx, ny, p, nq, xi, eta, quad_off, dx, dy, sigma_t, dt, source);
+    if (phi_out) memcpy(phi_out, c.phi, sizeof(double) * nphi);
+    if (iters) *iters = it;
+    if (resid) *resid = beta;
+
+    free(c.psi); free(c.phi); free(b); free(x); free(r); free(V); free(H); free(cs); free(sn); free(g); free(yv);
+    return converged ? 0 : -1;
+}
