@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: sweep-preconditioned GMRES(30) on the 280x280 Q2 S12 checkerboard lattice, FP64 sweeps
+versus FP32 sweeps inside FP64 iterative refinement, both to the absolute residual 1e-8.
+
+One step = one full transport solve of the lattice (inputs resident in HBM).  `value` is the
+mixed-precision solve's angular unknowns swept per second (unknowns per sweep x sweeps / wall time).
+The roofline entry is for the FP32 sweep kernel; its algorithmic bytes per sweep are
+U*(4+4) + ncell*4 (read the source, write psi, read sigma) and its duration is measured with HIP
+events recorded by the library on the stream the sweep launches run on.
+The CPU baseline is the oracle (a single-thread C restatement of the reference sweep) on a bounded
+sample of sweeps of the same lattice.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "idc-transport-solver_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(P, sweeps):
+    """Times the oracle's FP64 sweep (one thread) on `sweeps` sweeps of the same lattice."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _common import Grid, Ordinates, orc_sweep
+    g = Grid(P.nx, P.ny, P.p, P.ax, P.bx, P.ay, P.by)
+    o = Ordinates(P.xi, P.eta, P.w, P.quad_off)
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        orc_sweep(g, o, P.sigma_t, P.source)
+    dt = time.perf_counter() - t0
+    return {"value": P.unknowns * sweeps / dt, "unit": "angular unknowns swept/s", "cores": 1, "kind": "port",
+            "sample": f"{sweeps} FP64 sweeps of the 280x280 Q2 S12 lattice by the oracle (1 thread, "
+                      f"{dt:.1f} s) on {cpu_model()}"}
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per sweep from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(kernel_tag, {}).get("bytes_per_sweep")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pattern", default="hip-wavefront")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-sweeps", type=int, default=3, help="oracle sweeps timed for cpu_baseline (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import tsw
+    from tsw import problems
+
+    P = problems.lattice()
+    dev = torch.device("cuda:0")
+    d_src = torch.from_numpy(P.source).to(dev)
+    d_st = torch.from_numpy(P.sigma_t).to(dev)
+    d_ss = torch.from_numpy(P.sigma_s).to(dev)
+    work = torch.empty_like(d_src)
+    phi64 = torch.zeros((P.nx + 2, P.ny + 2, P.p + 1, P.p + 1), dtype=torch.float64, device=dev)
+    phi32 = torch.zeros_like(phi64)
+    ctx = tsw.Context(0)
+    P.set_on(ctx)
+    ctx.set_pattern(args.pattern)
+
+    def solve(precision, phi):
+        work.copy_(d_src)
+        torch.cuda.synchronize()
+        return ctx.gmres_solve(work, d_st, d_ss, phi_out=phi, abs_tol=args.tol, restart=30, precision=precision)
+
+    def timed(precision, phi):
+        for _ in range(args.warmup):
+            solve(precision, phi)
+        ctx.sweep_timing(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(args.steps):
+            its.append(solve(precision, phi))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        n_sweeps, sweep_ms = ctx.sweep_timing(reset=True)
+        return wall, n_sweeps, sweep_ms, its
+
+    print(f"bench: lattice {P.nx}x{P.ny} Q{P.p} S12 nq={P.nq} U={P.unknowns}", file=sys.stderr, flush=True)
+    w64, n64, ms64, its64 = timed(tsw.FP64, phi64)
+    print(f"bench: fp64 wall {w64:.3f}s sweeps {n64} sweep-ms {ms64:.1f} its {its64}", file=sys.stderr, flush=True)
+    w32, n32, ms32, its32 = timed(tsw.FP32, phi32)
+    print(f"bench: mixed wall {w32:.3f}s sweeps {n32} sweep-ms {ms32:.1f} its {its32}", file=sys.stderr, flush=True)
+
+    inner = (slice(1, -1), slice(1, -1))
+    err = float((phi32[inner] - phi64[inner]).norm() / phi64[inner].norm())
+    U = P.unknowns
+    value = U * n32 / w32
+    value64 = U * n64 / w64
+    ncell = P.nx * P.ny
+    bytes32 = U * 8 + ncell * 4
+    t_sweep32 = ms32 / max(n32, 1) * 1e-3
+    achieved = bytes32 / t_sweep32 / 1e9
+    t_sweep64 = ms64 / max(n64, 1) * 1e-3
+    achieved64 = (U * 16 + ncell * 8) / t_sweep64 / 1e9
+
+    line = {
+        "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
+        "value": value,
+        "unit": "angular unknowns swept/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": w32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sweep / f64 refinement",
+        "data": "synthetic (lattice cross sections and source generated in-process)",
+        "config": {"workload": "checkerboard lattice 280x280 cells, Q2 DG, S12 (84 ordinates), Peierls GMRES(30) "
+                               "to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
+                   "sweep_pattern": args.pattern, "dsa": False, "unknowns_per_sweep": U},
+        "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
+                 "gmres_its": [i for i, _ in its64], "sweep_ms": ms64 / max(n64, 1),
+                 "sweep_hbm_gbs": achieved64},
+        "mixed": {"sweeps_per_solve": n32 / args.steps, "gmres_its": [i for i, _ in its32],
+                  "sweep_ms": ms32 / max(n32, 1)},
+        "speedup_mixed_vs_fp64": (w64 / w32),
+        "mixed_rel_err_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(f"sweep_fp32_{args.pattern}")},
+    }
+    if args.cpu_sweeps > 0:
+        line["cpu_baseline"] = cpu_baseline(P, args.cpu_sweeps)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,625 @@
+// C-ABI of libtsw.so: context, problem set-up, sweep dispatch and the solver drivers that run on the
+// device (source iteration, Peierls GMRES with FP64 or mixed-precision sweeps).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tsw_internal.hpp"
+
+using namespace tsw;
+
+enum class Pattern { WavefrontGraph, WavefrontEager, Pipelined };
+
+struct tsw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool have_problem = false;
+    Pattern pattern = Pattern::WavefrontGraph;
+    Geom g;
+    std::vector<double> h_xi, h_eta, h_w;
+
+    // ordinates on the device, both precisions
+    DBuf xi64, eta64, w64, xi32, eta32, w32;
+    // staging / work arrays (grown on demand)
+    DBuf psi64, psi32, q64, r64, sig_t64, sig_s64, sig_t32, sig_s32, phi64a, phi64b, phi32, red_part, red_out;
+    DBuf si_state;  // source-iteration angular flux, persists between solves
+    DBuf ir_state;  // FP64 iterate of the mixed-precision refinement
+
+    // sweep timing: events recorded around every sweep
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    long long sweeps_done = 0;
+    double ms_folded = 0.0;  // elapsed time of event pairs already released
+
+    // captured sweep graphs keyed by (precision, psi, sigma, inv_dt)
+    std::map<std::tuple<int, const void *, const void *, double>, hipGraphExec_t> graphs;
+
+    void clear_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    ~tsw_ctx() {
+        clear_graphs();
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename F>
+int guarded(tsw_ctx *ctx, F &&f) {
+    if (!ctx) return TSW_E_ARG;
+    try {
+        ctx->err.clear();
+        TSW_HIP(hipSetDevice(ctx->device));
+        f();
+        return TSW_OK;
+    } catch (const Error &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return TSW_E_ARG;
+    }
+}
+
+void require(bool cond, int code, const std::string &msg) {
+    if (!cond) throw Error(code, msg);
+}
+
+void need_problem(tsw_ctx *c) { require(c->have_problem, TSW_E_STATE, "tsw_problem_set has not been called"); }
+
+// Device view of a caller array: the pointer itself for TSW_MEM_DEVICE, else a staged copy.
+template <typename T>
+T *stage_in(tsw_ctx *c, const T *p, size_t n, int mem, DBuf &buf) {
+    require(p != nullptr, TSW_E_ARG, "null array argument");
+    if (mem == TSW_MEM_DEVICE) return const_cast<T *>(p);
+    require(mem == TSW_MEM_HOST, TSW_E_ARG, "mem must be TSW_MEM_HOST or TSW_MEM_DEVICE");
+    T *d = static_cast<T *>(buf.reserve(n * sizeof(T)));
+    TSW_HIP(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return d;
+}
+template <typename T>
+void stage_out(tsw_ctx *c, T *p, const T *d, size_t n, int mem) {
+    if (mem == TSW_MEM_DEVICE) {
+        if (p != d) TSW_HIP(hipMemcpyAsync(p, d, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        return;
+    }
+    TSW_HIP(hipMemcpyAsync(p, d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+}
+
+void sync(tsw_ctx *c) { TSW_HIP(hipStreamSynchronize(c->stream)); }
+
+// Sum of the recorded (start, stop) event pairs; waits for them.
+double recorded_ms(tsw_ctx *c) {
+    double tot = 0.0;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        TSW_HIP(hipEventSynchronize(c->ev_pool[k + 1]));
+        float ms = 0.f;
+        TSW_HIP(hipEventElapsedTime(&ms, c->ev_pool[k], c->ev_pool[k + 1]));
+        tot += ms;
+    }
+    return tot;
+}
+
+hipEvent_t next_event(tsw_ctx *c) {
+    if (c->ev_used >= 8192) {  // bound the pool: fold finished pairs into the running total
+        c->ms_folded += recorded_ms(c);
+        c->ev_used = 0;
+    }
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        TSW_HIP(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+template <typename T>
+SweepParams<T> params(tsw_ctx *c, T *psi, const T *sigma, double dt);
+template <>
+SweepParams<double> params(tsw_ctx *c, double *psi, const double *sigma, double dt) {
+    const double inv_dt = (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0;
+    return {psi, sigma, c->xi64.as<double>(), c->eta64.as<double>(), c->g.nx, c->g.ny, c->g.nq, c->g.quad_off[1],
+            c->g.quad_off[2], c->g.quad_off[3], c->g.dx, c->g.dy, inv_dt};
+}
+template <>
+SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt) {
+    const double inv_dt = (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0;
+    return {psi, sigma, c->xi32.as<float>(), c->eta32.as<float>(), c->g.nx, c->g.ny, c->g.nq, c->g.quad_off[1],
+            c->g.quad_off[2], c->g.quad_off[3], (float)c->g.dx, (float)c->g.dy, (float)inv_dt};
+}
+
+// One in-place sweep of device array psi with device cell cross sections sigma.
+template <typename T>
+void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
+    const SweepParams<T> sp = params<T>(c, psi, sigma, dt);
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, c->stream));
+    switch (c->pattern) {
+        case Pattern::WavefrontEager: launch_sweep_wavefront<T>(sp, c->g.p, c->stream); break;
+        case Pattern::WavefrontGraph: {
+            const auto key = std::make_tuple((int)sizeof(T), (const void *)psi, (const void *)sigma, (double)sp.inv_dt);
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                hipGraph_t graph;
+                TSW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_sweep_wavefront<T>(sp, c->g.p, c->stream);
+                } catch (...) {
+                    (void)hipStreamEndCapture(c->stream, &graph);
+                    throw;
+                }
+                TSW_HIP(hipStreamEndCapture(c->stream, &graph));
+                hipGraphExec_t exec;
+                TSW_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(graph);
+                it = c->graphs.emplace(key, exec).first;
+            }
+            TSW_HIP(hipGraphLaunch(it->second, c->stream));
+            break;
+        }
+        case Pattern::Pipelined: throw Error(TSW_E_UNSUPPORTED, "hip-pipelined sweep is not built");
+    }
+    TSW_HIP(hipEventRecord(e1, c->stream));
+    ++c->sweeps_done;
+}
+
+// Float copies of a double cell array (cross sections).
+const float *as_f32(tsw_ctx *c, const double *d, size_t n, DBuf &buf) {
+    float *f = static_cast<float *>(buf.reserve(n * sizeof(float)));
+    launch_convert_d2f(d, f, n, c->stream);
+    return f;
+}
+
+// Sweep a double device array in the requested precision.
+void sweep_prec(tsw_ctx *c, double *psi, const double *sigma, double dt, int precision) {
+    if (precision == TSW_FP64) {
+        sweep_dev<double>(c, psi, sigma, dt);
+        return;
+    }
+    require(precision == TSW_FP32, TSW_E_ARG, "precision must be TSW_FP64 or TSW_FP32");
+    const size_t n = c->g.npsi();
+    float *p32 = static_cast<float *>(c->psi32.reserve(n * sizeof(float)));
+    launch_convert_d2f(psi, p32, n, c->stream);
+    const float *s32 = as_f32(c, sigma, c->g.ncell_h(), c->sig_t32);
+    sweep_dev<float>(c, p32, s32, dt);
+    launch_convert_f2d(p32, psi, n, c->stream);
+}
+
+double device_norm2_interior_diff(tsw_ctx *c, const double *a, const double *b) {
+    const int nb = reduction_blocks();
+    double *part = static_cast<double *>(c->red_part.reserve(sizeof(double) * nb * 64));
+    double *out = static_cast<double *>(c->red_out.reserve(sizeof(double) * 64));
+    launch_sqdist_interior(c->g, a, b, out, part, nb, c->stream);
+    double h = 0.0;
+    TSW_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return std::sqrt(h);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Restarted GMRES on phi-shaped FP64 vectors (zero halos), CGS2 orthogonalisation with batched dot
+// products so each Arnoldi step costs two host round trips.
+// ---------------------------------------------------------------------------------------------------
+struct Gmres {
+    tsw_ctx *c;
+    size_t n;
+    int m;
+    DBuf vbuf, hbuf, pbuf;
+    double *V;        // (m+1) x n
+    double *hdev;     // m+2 doubles
+    double *part;
+    int nb;
+
+    Gmres(tsw_ctx *ctx, size_t n_, int m_) : c(ctx), n(n_), m(m_) {
+        V = static_cast<double *>(vbuf.reserve(sizeof(double) * n * (m + 1)));
+        hdev = static_cast<double *>(hbuf.reserve(sizeof(double) * (m + 2)));
+        nb = reduction_blocks();
+        part = static_cast<double *>(pbuf.reserve(sizeof(double) * nb * (m + 2)));
+    }
+    std::vector<double> dots(int k, const double *y) {  // <V_i, y>, i < k
+        std::vector<double> h(k);
+        launch_multi_dot(n, k, V, n, y, hdev, part, nb, c->stream);
+        TSW_HIP(hipMemcpyAsync(h.data(), hdev, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        return h;
+    }
+    double norm(const double *y) {
+        double h = 0.0;
+        launch_multi_dot(n, 1, y, n, y, hdev, part, nb, c->stream);
+        TSW_HIP(hipMemcpyAsync(&h, hdev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        return std::sqrt(h);
+    }
+
+    // Solves A x = b (x holds the initial guess, updated in place). apply(x_dev, y_dev) computes y = A x.
+    // Stops when the residual estimate <= abs_tol.  Returns true when converged.
+    template <typename Apply>
+    bool solve(Apply &&apply, const double *b, double *x, double *r, double abs_tol, int max_its, int &its,
+               double &res) {
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+        its = 0;
+        res = 0.0;
+        while (true) {
+            apply(x, r);
+            launch_axpby(n, 1.0, b, -1.0, r, c->stream);  // r = b - A x
+            const double beta = norm(r);
+            res = beta;
+            if (beta <= abs_tol) return true;
+            if (its >= max_its) return false;
+            launch_axpby(n, 1.0 / beta, r, 0.0, V, c->stream);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int jdone = 0;
+            bool conv = false;
+            for (int j = 0; j < m && its < max_its; ++j) {
+                double *vn = V + (size_t)(j + 1) * n;
+                apply(V + (size_t)j * n, vn);
+                ++its;
+                std::vector<double> h1 = dots(j + 1, vn);
+                TSW_HIP(hipMemcpyAsync(hdev, h1.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+                launch_multi_axpy(n, j + 1, V, n, hdev, vn, c->stream);
+                std::vector<double> h2 = dots(j + 1, vn);
+                TSW_HIP(hipMemcpyAsync(hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+                launch_multi_axpy(n, j + 1, V, n, hdev, vn, c->stream);
+                for (int i = 0; i <= j; ++i) H[i + (m + 1) * j] = h1[i] + h2[i];
+                const double hn = norm(vn);
+                H[j + 1 + (m + 1) * j] = hn;
+                if (hn > 0.0) launch_axpby(n, 1.0 / hn, vn, 0.0, vn, c->stream);
+                for (int i = 0; i < j; ++i) {
+                    const double h0 = H[i + (m + 1) * j], hh = H[i + 1 + (m + 1) * j];
+                    H[i + (m + 1) * j] = cs[i] * h0 + sn[i] * hh;
+                    H[i + 1 + (m + 1) * j] = -sn[i] * h0 + cs[i] * hh;
+                }
+                const double h0 = H[j + (m + 1) * j], hh = H[j + 1 + (m + 1) * j];
+                const double den = std::sqrt(h0 * h0 + hh * hh);
+                cs[j] = h0 / den;
+                sn[j] = hh / den;
+                H[j + (m + 1) * j] = den;
+                H[j + 1 + (m + 1) * j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                jdone = j + 1;
+                res = std::fabs(g[j + 1]);
+                if (res <= abs_tol) {
+                    conv = true;
+                    break;
+                }
+            }
+            for (int i = jdone - 1; i >= 0; --i) {
+                double s = g[i];
+                for (int k = i + 1; k < jdone; ++k) s -= H[i + (m + 1) * k] * y[k];
+                y[i] = s / H[i + (m + 1) * i];
+            }
+            for (int i = 0; i < jdone; ++i) launch_axpby(n, y[i], V + (size_t)i * n, 1.0, x, c->stream);
+            if (conv) return true;
+            if (its >= max_its) {
+                sync(c);
+                return false;
+            }
+        }
+    }
+};
+
+}  // namespace
+
+// =====================================================================================================
+// C-ABI
+// =====================================================================================================
+extern "C" {
+
+int tsw_create(int device, tsw_ctx **out) {
+    if (!out) return TSW_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return TSW_E_HIP;
+    tsw_ctx *c = new (std::nothrow) tsw_ctx();
+    if (!c) return TSW_E_HIP;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return TSW_E_HIP;
+    }
+    *out = c;
+    return TSW_OK;
+}
+
+void tsw_destroy(tsw_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+}
+
+const char *tsw_last_error(const tsw_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
+                    const double *xi, const double *eta, const double *w, const int *quad_off) {
+    return guarded(ctx, [&] {
+        require(nx >= 1 && ny >= 1, TSW_E_ARG, "nx and ny must be >= 1");
+        require(bx > ax && by > ay, TSW_E_ARG, "empty domain");
+        require(p >= 0, TSW_E_ARG, "DG degree must be >= 0");
+        require(p <= 3, TSW_E_UNSUPPORTED, "DG degree " + std::to_string(p) + " not built (0..3)");
+        require(nq >= 1 && xi && eta && w && quad_off, TSW_E_ARG, "bad ordinate set");
+        require(quad_off[0] == 0 && quad_off[4] == nq, TSW_E_ARG, "quad_off must span [0, nq]");
+        for (int k = 0; k < 4; ++k) {
+            require(quad_off[k] <= quad_off[k + 1], TSW_E_ARG, "quad_off must be non-decreasing");
+            for (int q = quad_off[k]; q < quad_off[k + 1]; ++q) {
+                const bool xneg = (k >= 2), yneg = (k & 1);
+                require(xi[q] != 0.0 && eta[q] != 0.0 && (xi[q] < 0) == xneg && (eta[q] < 0) == yneg, TSW_E_ARG,
+                        "ordinate " + std::to_string(q) + " is not in quadrant " + std::to_string(k));
+            }
+        }
+        ctx->clear_graphs();
+        Geom &g = ctx->g;
+        g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
+        g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
+        g.dx = (bx - ax) / nx;
+        g.dy = (by - ay) / ny;
+        for (int k = 0; k < 5; ++k) g.quad_off[k] = quad_off[k];
+        ctx->h_xi.assign(xi, xi + nq);
+        ctx->h_eta.assign(eta, eta + nq);
+        ctx->h_w.assign(w, w + nq);
+        std::vector<float> f(nq);
+        auto put64 = [&](DBuf &b, const double *src) {
+            b.reserve(sizeof(double) * nq);
+            TSW_HIP(hipMemcpy(b.ptr, src, sizeof(double) * nq, hipMemcpyHostToDevice));
+        };
+        auto put32 = [&](DBuf &b, const double *src) {
+            for (int q = 0; q < nq; ++q) f[q] = (float)src[q];
+            b.reserve(sizeof(float) * nq);
+            TSW_HIP(hipMemcpy(b.ptr, f.data(), sizeof(float) * nq, hipMemcpyHostToDevice));
+        };
+        put64(ctx->xi64, xi); put64(ctx->eta64, eta); put64(ctx->w64, w);
+        put32(ctx->xi32, xi); put32(ctx->eta32, eta); put32(ctx->w32, w);
+        double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
+        TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
+        ctx->have_problem = true;
+    });
+}
+
+int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
+    return guarded(ctx, [&] {
+        require(name != nullptr, TSW_E_ARG, "null pattern name");
+        const std::string s(name);
+        if (s == "hip-wavefront") ctx->pattern = Pattern::WavefrontGraph;
+        else if (s == "hip-wavefront-eager") ctx->pattern = Pattern::WavefrontEager;
+        else if (s == "hip-pipelined") throw Error(TSW_E_UNSUPPORTED, "sweep_pattern 'hip-pipelined' is not built");
+        else throw Error(TSW_E_ARG, "unknown sweep_pattern '" + s + "'");
+    });
+}
+
+int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
+        const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        sweep_prec(ctx, psi, sig, dt, precision);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
+int tsw_pmv(tsw_ctx *ctx, double alpha, double beta, const double *psi, double *phi, int all_cells, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        const double *x = stage_in(ctx, psi, g.npsi(), mem, ctx->psi64);
+        double *y = stage_in(ctx, (const double *)phi, g.nphi(), mem, ctx->phi64a);
+        launch_pmv<double, double>(g, x, y, ctx->w64.as<double>(), alpha, beta, all_cells ? 1 : 0, ctx->stream);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, phi, y, g.nphi(), mem);
+        sync(ctx);
+    });
+}
+
+int tsw_smv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_s, const double *phi, double *psi,
+            int all_cells, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        const double *s = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *f = stage_in(ctx, phi, g.nphi(), mem, ctx->phi64a);
+        double *y = stage_in(ctx, (const double *)psi, g.npsi(), mem, ctx->psi64);
+        launch_smv<double>(g, s, f, y, alpha, beta, all_cells ? 1 : 0, ctx->stream);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi, y, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
+int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, const double *x, double *y, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        require(x != y, TSW_E_ARG, "tsw_lmv needs x != y");
+        const Geom &g = ctx->g;
+        const double *s = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *xd = stage_in(ctx, x, g.npsi(), mem, ctx->psi64);
+        double *yd = stage_in(ctx, (const double *)y, g.npsi(), mem, ctx->r64);
+        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), s, xd, yd, alpha, beta, ctx->stream);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, y, yd, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
+int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
+                 double abs_tol, int max_its, int precision, int mem, int *iters, double *err) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        const Geom &g = ctx->g;
+        hipStream_t s = ctx->stream;
+        double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        double *psi = ctx->si_state.as<double>();
+        double *cur = static_cast<double *>(ctx->phi64a.reserve(sizeof(double) * g.nphi()));
+        double *last = static_cast<double *>(ctx->phi64b.reserve(sizeof(double) * g.nphi()));
+        TSW_HIP(hipMemsetAsync(cur, 0, sizeof(double) * g.nphi(), s));
+        TSW_HIP(hipMemsetAsync(last, 0, sizeof(double) * g.nphi(), s));
+        const double *w = ctx->w64.as<double>();
+        launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);  // SISolver.cpp:214
+        int it = 0;
+        double e = 1.79769313486231570e+308;
+        while (it < max_its && e > abs_tol) {
+            ++it;
+            std::swap(cur, last);
+            TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));  // :226
+            launch_smv<double>(g, ss, last, psi, 1.0, 1.0, 0, s);                                     // :229
+            sweep_prec(ctx, psi, st, dt, precision);                                                  // :232
+            launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);                                      // :236
+            e = device_norm2_interior_diff(ctx, cur, last);                                           // :239
+        }
+        TSW_HIP(hipMemcpyAsync(Q, psi, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));  // :249
+        if (mem == TSW_MEM_HOST) stage_out(ctx, source_inout, Q, g.npsi(), mem);
+        else if (Q != source_inout) stage_out(ctx, source_inout, Q, g.npsi(), mem);
+        sync(ctx);
+        if (iters) *iters = it;
+        if (err) *err = e;
+    });
+}
+
+int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
+                    double abs_tol, int max_its, int restart, int precision, int mem, double *phi_out, int *iters,
+                    double *resid) {
+    int its_total = 0;
+    double res_final = 0.0;
+    bool converged = false;
+    const int rc = guarded(ctx, [&] {
+        need_problem(ctx);
+        require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        const Geom &g = ctx->g;
+        hipStream_t s = ctx->stream;
+        const size_t n = g.nphi(), npsi = g.npsi();
+        double *Q = stage_in(ctx, source_inout, npsi, mem, ctx->q64);
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *w = ctx->w64.as<double>();
+        double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
+        // vectors: b, x, r (phi-shaped, zero halos)
+        DBuf vecs;
+        double *b = static_cast<double *>(vecs.reserve(sizeof(double) * n * 3));
+        double *x = b + n, *r = x + n;
+        TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n * 3, s));
+        Gmres solver(ctx, n, restart);
+
+        // matvec y = (I - P L^{-1} S) x with zero inflow (PETScPeierlsSolver.cpp:548-555)
+        auto apply = [&](const double *xv, double *yv) {
+            launch_smv<double>(g, ss, xv, psi, 1.0, 0.0, 1, s);
+            sweep_prec(ctx, psi, st, dt, precision);
+            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            launch_pmv<double, double>(g, psi, yv, w, -1.0, 1.0, 0, s);
+        };
+
+        if (precision == TSW_FP64) {
+            // b = P L^{-1} Q (:461-466)
+            TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+            sweep_prec(ctx, psi, st, dt, precision);
+            launch_pmv<double, double>(g, psi, b, w, 1.0, 0.0, 0, s);
+            converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
+            // source <- L^{-1}(Q + S phi) (:504-505)
+            launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
+            sweep_prec(ctx, Q, st, dt, precision);
+        } else {
+            // Mixed precision: an FP64 iterate psi64 is refined with corrections computed from FP32
+            // sweeps.  The residual r = Q + S P psi - L psi is formed in FP64 with Lmv; the correction
+            // solves (L - S P) d = r on the Peierls form with FP32 sweeps inside FP64 GMRES.
+            const size_t ncell = g.ncell_h();
+            const int N = g.N();
+            double *psi64 = static_cast<double *>(ctx->ir_state.reserve(sizeof(double) * npsi));
+            double *res64 = static_cast<double *>(ctx->r64.reserve(sizeof(double) * npsi));
+            float *p32 = static_cast<float *>(ctx->psi32.reserve(sizeof(float) * npsi));
+            float *phi32 = static_cast<float *>(ctx->phi32.reserve(sizeof(float) * n));
+            const float *st32 = as_f32(ctx, st, ncell, ctx->sig_t32);
+            const float *ss32 = as_f32(ctx, ss, ncell, ctx->sig_s32);
+            const double *xi64 = ctx->xi64.as<double>(), *eta64 = ctx->eta64.as<double>();
+
+            // y = (I - P L^{-1} S) x with FP32 sweeps
+            auto apply32 = [&](const double *xv, double *yv) {
+                launch_convert_d2f(xv, phi32, n, s);
+                launch_smv<float>(g, ss32, phi32, p32, 1.0f, 0.0f, 1, s);
+                sweep_dev<float>(ctx, p32, st32, dt);
+                TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                launch_pmv<float, double>(g, p32, yv, w, -1.0, 1.0, 0, s);
+            };
+
+            // initial iterate: inflow halos from Q, zero interior
+            TSW_HIP(hipMemcpyAsync(psi64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+            launch_fill_region<double>(g, psi64, (size_t)g.nq * N, 0, 0.0, s);
+            const int max_refine = 30;
+            for (int refine = 0;; ++refine) {
+                // r = Q + S P psi - L psi on the interior, zero halos
+                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+                launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
+                TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+                launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s);
+                launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s);
+                launch_fill_region<double>(g, res64, (size_t)g.nq * N, 1, 0.0, s);
+                // b = P L^{-1} r: the Peierls residual of the current iterate
+                launch_convert_d2f(res64, p32, npsi, s);
+                sweep_dev<float>(ctx, p32, st32, dt);
+                TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
+                launch_pmv<float, double>(g, p32, b, w, 1.0, 0.0, 0, s);
+                res_final = solver.norm(b);
+                if (res_final <= abs_tol) {
+                    converged = true;
+                    break;
+                }
+                if (refine >= max_refine || its_total >= max_its) break;
+                // correction: (I - P L^{-1} S) phi_d = b, then d = L^{-1}(r + S phi_d)
+                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+                const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
+                int its = 0;
+                double rr = 0.0;
+                solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
+                its_total += its;
+                launch_convert_d2f(res64, p32, npsi, s);
+                launch_convert_d2f(x, phi32, n, s);
+                launch_smv<float>(g, ss32, phi32, p32, 1.0f, 1.0f, 1, s);
+                sweep_dev<float>(ctx, p32, st32, dt);
+                launch_accumulate_f2d(p32, psi64, npsi, s);
+            }
+            // outputs: psi and phi = P psi
+            TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
+        }
+        if (phi_out) {
+            if (mem == TSW_MEM_HOST) {
+                TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            } else {
+                TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            }
+        }
+        if (mem == TSW_MEM_HOST) stage_out(ctx, source_inout, Q, npsi, mem);
+        sync(ctx);
+    });
+    if (iters) *iters = its_total;
+    if (resid) *resid = res_final;
+    if (rc == TSW_OK && !converged) {
+        ctx->err = "GMRES transport solve failed to converge";
+        return TSW_E_NOCONV;
+    }
+    return rc;
+}
+
+int tsw_sweep_timing(tsw_ctx *ctx, int reset, long long *n_sweeps, double *total_ms) {
+    return guarded(ctx, [&] {
+        sync(ctx);
+        const double tot = ctx->ms_folded + recorded_ms(ctx);
+        if (n_sweeps) *n_sweeps = ctx->sweeps_done;
+        if (total_ms) *total_ms = tot;
+        if (reset) {
+            ctx->ev_used = 0;
+            ctx->sweeps_done = 0;
+            ctx->ms_folded = 0.0;
+        }
+    });
+}
+
+}  // extern "C"

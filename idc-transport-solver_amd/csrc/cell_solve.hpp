@@ -1,0 +1,107 @@
+// Per-cell, per-ordinate upwind-DG element solve, evaluated entirely in registers.
+//
+// For ordinate (xi, eta) on cell (i,j) with sizes dx, dy, Legendre degree p per axis and
+// N = (p+1)^2 unknowns ordered r = e + (p+1)*d (d: x-degree, e: y-degree):
+//   A = Aq + (sigma_t + 1/dt) dx dy I
+//   Aq[(d,e),(a,e)] = cx (2d+1) [ s_x(d,a) - 2*[a<d, d-a odd] ],  cx = dy*xi,
+//       s_x(d,a) = (-1)^(d+a+1) if xi < 0 else 1                       (SweepLinearSolveSpec.cpp:120-138)
+//   Aq[(d,e),(d,b)] += the same in y with cy = dx*eta, (2e+1)         (:141-155)
+//   B(d,e) = dx dy src(d,e) + (2d+1) sum_a c_x(d,a) cx psi_up_x(a,e) + (2e+1) sum_b c_y(e,b) cy psi_up_y(d,b)
+//       c_x(d,a) = (-1)^(a+1) if xi <= 0 else (-1)^d                   (:473-598)
+// then unpivoted Gaussian elimination and back substitution (SIMD_LinearAlgebra.hpp:78-115) with
+// the reference's operation order, and the solution stored back over the source.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tsw {
+
+__host__ __device__ constexpr int ipow_m1(int k) { return (k & 1) ? -1 : 1; }
+
+// Builds B (right-hand side) for one lane from the source and the two upwind neighbours.
+template <typename T, int P>
+__device__ __forceinline__ void assemble_rhs(T (&B)[P * P], const T *__restrict__ src, const T *__restrict__ upx,
+                                             const T *__restrict__ upy, T xi, T eta, T dx, T dy) {
+    constexpr int N = P * P;
+    const bool xneg = xi <= T(0), yneg = eta <= T(0);
+    T W0[N], W1[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        W0[k] = upx[k] * (dy * xi);
+        W1[k] = upy[k] * (dx * eta);
+    }
+    const T area = dx * dy;
+#pragma unroll
+    for (int k = 0; k < N; ++k) B[k] = area * src[k];
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e)
+#pragma unroll
+            for (int a = 0; a < P; ++a) {
+                const T c = xneg ? T(ipow_m1(a + 1) * (2 * d + 1)) : T(ipow_m1(d) * (2 * d + 1));
+                B[e + P * d] += W0[e + P * a] * c;
+            }
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e)
+#pragma unroll
+            for (int b = 0; b < P; ++b) {
+                const T c = yneg ? T(ipow_m1(b + 1) * (2 * e + 1)) : T(ipow_m1(e) * (2 * e + 1));
+                B[e + P * d] += W1[b + P * d] * c;
+            }
+}
+
+// Builds A = Aq + diag_add * I (column-major, A[row + N*col]).
+template <typename T, int P>
+__device__ __forceinline__ void assemble_matrix(T (&A)[P * P * P * P], T xi, T eta, T dx, T dy, T diag_add) {
+    constexpr int N = P * P;
+    const bool xneg = xi < T(0), yneg = eta < T(0);
+    const T cx = dy * xi, cy = dx * eta;
+#pragma unroll
+    for (int k = 0; k < N * N; ++k) A[k] = T(0);
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e) {
+            const int row = e + P * d;
+            const T tx = cx * T(2 * d + 1);
+            const T ty = cy * T(2 * e + 1);
+#pragma unroll
+            for (int a = d - 1; a >= 0; a -= 2) A[row + N * (e + P * a)] -= T(2) * tx;
+#pragma unroll
+            for (int a = 0; a < P; ++a) A[row + N * (e + P * a)] += xneg ? T(ipow_m1(d + a + 1)) * tx : tx;
+#pragma unroll
+            for (int b = e - 1; b >= 0; b -= 2) A[row + N * (b + P * d)] -= T(2) * ty;
+#pragma unroll
+            for (int b = 0; b < P; ++b) A[row + N * (b + P * d)] += yneg ? T(ipow_m1(e + b + 1)) * ty : ty;
+        }
+#pragma unroll
+    for (int r = 0; r < N; ++r) A[r + N * r] += diag_add;
+}
+
+// Unpivoted Gaussian elimination, same operation order as the reference's SIMD kernel.
+template <typename T, int N>
+__device__ __forceinline__ void ge_solve(T (&A)[N * N], T (&b)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            A[i + N * j] /= -A[j + N * j];
+            b[i] += A[i + N * j] * b[j];
+        }
+#pragma unroll
+        for (int k = j + 1; k < N; ++k)
+#pragma unroll
+            for (int i = j + 1; i < N; ++i) A[i + N * k] += A[i + N * j] * A[j + N * k];
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        b[i] /= A[i + N * i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) b[j] -= A[j + N * i] * b[i];
+    }
+}
+
+}  // namespace tsw

@@ -1,0 +1,325 @@
+// Transport operators around the sweep (Pmv, Smv, Lmv) and the FP64 vector kernels used by the
+// Krylov and source-iteration drivers.  All are HBM-bound streaming kernels over the reference
+// layouts; every thread owns whole coefficient vectors so loads of a wavefront are contiguous.
+#include <cmath>
+
+#include "cell_solve.hpp"
+#include "tsw_internal.hpp"
+
+namespace tsw {
+
+namespace {
+constexpr int kBlock = 256;
+constexpr double kInv4Pi = 0.079577471545947667884;  // 1/(4 pi)
+
+inline int grid_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// Cell (i,j) of a linear index over the chosen domain (interior or halo-inclusive).
+struct Domain {
+    int i0, j0, ni, nj;
+    __host__ __device__ size_t count() const { return (size_t)ni * nj; }
+};
+inline Domain domain(const Geom &g, int all_cells) {
+    return all_cells ? Domain{0, 0, g.nx + 2, g.ny + 2} : Domain{1, 1, g.nx, g.ny};
+}
+}  // namespace
+
+// ---- Pmv: phi = beta*phi + alpha*sum_q w_q psi_q (TransportOperator.cpp:140-158) ----
+// TI: storage type of psi; TO: type of phi, w and the accumulation.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_pmv(const TI *__restrict__ psi, TO *__restrict__ phi,
+                                                const TO *__restrict__ w, TO alpha, TO beta, Domain dom, int ny,
+                                                int nq, int N) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dom.count() * N) return;
+    const int k = (int)(t % N);
+    const size_t c = t / N;
+    const int i = dom.i0 + (int)(c / dom.nj), j = dom.j0 + (int)(c % dom.nj);
+    const size_t cell = (size_t)i * (ny + 2) + j;
+    TO acc = beta == TO(0) ? TO(0) : phi[cell * N + k] * beta;
+    const TI *x = psi + cell * nq * N + k;
+    for (int q = 0; q < nq; ++q) acc += alpha * w[q] * TO(x[(size_t)q * N]);
+    phi[cell * N + k] = acc;
+}
+
+template <typename TI, typename TO>
+void launch_pmv(const Geom &g, const TI *psi, TO *phi, const TO *w, TO alpha, TO beta, int all_cells,
+                hipStream_t s) {
+    const Domain dom = domain(g, all_cells);
+    const size_t n = dom.count() * g.N();
+    hipLaunchKernelGGL((k_pmv<TI, TO>), dim3(grid_for(n)), dim3(kBlock), 0, s, psi, phi, w, alpha, beta, dom, g.ny,
+                       g.nq, g.N());
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- Smv: psi_q = beta*psi_q + alpha*sigma_s*phi/(4 pi) (TransportOperator.cpp:369-392) ----
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_smv(const T *__restrict__ sigma_s, const T *__restrict__ phi,
+                                                T *__restrict__ psi, T alpha, T beta, Domain dom, int ny, int nq,
+                                                int N) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_cell = (size_t)nq * N;
+    if (t >= dom.count() * per_cell) return;
+    const size_t c = t / per_cell;
+    const int r = (int)(t - c * per_cell);
+    const int k = r % N;
+    const int i = dom.i0 + (int)(c / dom.nj), j = dom.j0 + (int)(c % dom.nj);
+    const size_t cell = (size_t)i * (ny + 2) + j;
+    const T val = alpha * (sigma_s[cell] * phi[cell * N + k] * T(kInv4Pi));
+    T *y = psi + cell * per_cell + r;
+    *y = beta == T(0) ? val : beta * *y + val;
+}
+
+template <typename T>
+void launch_smv(const Geom &g, const T *sigma_s, const T *phi, T *psi, T alpha, T beta, int all_cells,
+                hipStream_t s) {
+    const Domain dom = domain(g, all_cells);
+    const size_t n = dom.count() * g.nq * g.N();
+    hipLaunchKernelGGL(k_smv<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, sigma_s, phi, psi, alpha, beta, dom, g.ny,
+                       g.nq, g.N());
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- Lmv: y = beta*y + alpha*(Omega.grad + sigma) x on interior cells (TransportOperator.cpp:1576-1658) ----
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_lmv(const T *__restrict__ xi_, const T *__restrict__ eta_,
+                                                const T *__restrict__ sigma, const T *__restrict__ x,
+                                                T *__restrict__ y, T alpha, T beta, int nx, int ny, int nq, T dx,
+                                                T dy) {
+    constexpr int N = P * P;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nx * ny * nq) return;
+    const int q = (int)(t % nq);
+    const size_t c = t / nq;
+    const int i = 1 + (int)(c / ny), j = 1 + (int)(c % ny);
+    const size_t sj = (size_t)nq * N, si = (size_t)(ny + 2) * sj;
+    const size_t base = (size_t)i * si + (size_t)j * sj + (size_t)q * N;
+    const T *xc = x + base;
+    const T xi = xi_[q], eta = eta_[q];
+    const T sig = sigma[(size_t)i * (ny + 2) + j];
+    T out[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = (beta == T(0) ? T(0) : y[base + k] * beta) + alpha * sig * xc[k];
+    T coeff = alpha * xi / dx;
+    if (xi < T(0)) {
+        const T *xn = xc + si;
+#pragma unroll
+        for (int d = 0; d < P; ++d)
+#pragma unroll
+            for (int a = 0; a < P; ++a)
+#pragma unroll
+                for (int e = 0; e < P; ++e)
+                    out[e + P * d] += coeff * T(2 * d + 1) *
+                                      (T(ipow_m1(a)) * xn[e + P * a] + T(ipow_m1(d + a + 1)) * xc[e + P * a]);
+    } else {
+        const T *xn = xc - si;
+#pragma unroll
+        for (int d = 0; d < P; ++d)
+#pragma unroll
+            for (int a = 0; a < P; ++a)
+#pragma unroll
+                for (int e = 0; e < P; ++e)
+                    out[e + P * d] += coeff * T(2 * d + 1) * (xc[e + P * a] + T(ipow_m1(d + 1)) * xn[e + P * a]);
+    }
+    coeff *= T(-2);
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int a = d - 1; a >= 0; a -= 2)
+#pragma unroll
+            for (int e = 0; e < P; ++e) out[e + P * d] += coeff * T(2 * d + 1) * xc[e + P * a];
+    coeff = alpha * eta / dy;
+    if (eta < T(0)) {
+        const T *xn = xc + sj;
+#pragma unroll
+        for (int d = 0; d < P; ++d)
+#pragma unroll
+            for (int e = 0; e < P; ++e)
+#pragma unroll
+                for (int b = 0; b < P; ++b)
+                    out[e + P * d] += coeff * T(2 * e + 1) *
+                                      (T(ipow_m1(b)) * xn[b + P * d] + T(ipow_m1(e + b + 1)) * xc[b + P * d]);
+    } else {
+        const T *xn = xc - sj;
+#pragma unroll
+        for (int d = 0; d < P; ++d)
+#pragma unroll
+            for (int e = 0; e < P; ++e)
+#pragma unroll
+                for (int b = 0; b < P; ++b)
+                    out[e + P * d] += coeff * T(2 * e + 1) * (xc[b + P * d] + T(ipow_m1(e + 1)) * xn[b + P * d]);
+    }
+    coeff *= T(-2);
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e)
+#pragma unroll
+            for (int b = e - 1; b >= 0; b -= 2) out[e + P * d] += coeff * T(2 * e + 1) * xc[b + P * d];
+#pragma unroll
+    for (int k = 0; k < N; ++k) y[base + k] = out[k];
+}
+
+template <typename T>
+void launch_lmv(const Geom &g, const T *xi, const T *eta, const T *sigma_t, const T *x, T *y, T alpha, T beta,
+                hipStream_t s) {
+    const size_t n = (size_t)g.nx * g.ny * g.nq;
+    const dim3 grid(grid_for(n)), block(kBlock);
+    const T dx = T(g.dx), dy = T(g.dy);
+    switch (g.p) {
+        case 0: hipLaunchKernelGGL((k_lmv<T, 1>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
+        case 1: hipLaunchKernelGGL((k_lmv<T, 2>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
+        case 2: hipLaunchKernelGGL((k_lmv<T, 3>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
+        case 3: hipLaunchKernelGGL((k_lmv<T, 4>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
+    }
+    TSW_HIP(hipGetLastError());
+}
+
+template void launch_pmv<double, double>(const Geom &, const double *, double *, const double *, double, double, int, hipStream_t);
+template void launch_pmv<float, float>(const Geom &, const float *, float *, const float *, float, float, int, hipStream_t);
+template void launch_pmv<float, double>(const Geom &, const float *, double *, const double *, double, double, int, hipStream_t);
+template void launch_smv<double>(const Geom &, const double *, const double *, double *, double, double, int, hipStream_t);
+template void launch_smv<float>(const Geom &, const float *, const float *, float *, float, float, int, hipStream_t);
+template void launch_lmv<double>(const Geom &, const double *, const double *, const double *, const double *, double *, double, double, hipStream_t);
+template void launch_lmv<float>(const Geom &, const float *, const float *, const float *, const float *, float *, float, float, hipStream_t);
+
+// ---- precision conversion ----
+__global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = (float)in[t];
+}
+__global__ void k_f2d(const float *__restrict__ in, double *__restrict__ out, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = (double)in[t];
+}
+void launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+    TSW_HIP(hipGetLastError());
+}
+void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- region fill: halo cells (halo=1) or interior cells (halo=0) of a per-cell array get `value` ----
+template <typename T>
+__global__ void k_fill_region(T *__restrict__ a, int nx, int ny, size_t per_cell, int halo, T value) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)(nx + 2) * (ny + 2) * per_cell;
+    if (t >= n) return;
+    const size_t cell = t / per_cell;
+    const int i = (int)(cell / (ny + 2)), j = (int)(cell % (ny + 2));
+    const bool is_halo = i == 0 || j == 0 || i == nx + 1 || j == ny + 1;
+    if (is_halo == (halo != 0)) a[t] = value;
+}
+template <typename T>
+void launch_fill_region(const Geom &g, T *a, size_t per_cell, int halo, T value, hipStream_t s) {
+    const size_t n = g.ncell_h() * per_cell;
+    hipLaunchKernelGGL(k_fill_region<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, a, g.nx, g.ny, per_cell, halo, value);
+    TSW_HIP(hipGetLastError());
+}
+template void launch_fill_region<double>(const Geom &, double *, size_t, int, double, hipStream_t);
+template void launch_fill_region<float>(const Geom &, float *, size_t, int, float, hipStream_t);
+
+// out[k] += (double) in[k]
+__global__ void k_acc_f2d(const float *__restrict__ in, double *__restrict__ out, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] += (double)in[t];
+}
+void launch_accumulate_f2d(const float *in, double *out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_acc_f2d, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- FP64 vector kernels for the Krylov driver ----
+__global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = b == 0.0 ? a * x[t] : a * x[t] + b * y[t];
+}
+void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+    TSW_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlock / 64; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+// partials[i*nblocks + b] = partial dot of X_i with y over block b's grid-stride share
+__global__ __launch_bounds__(kBlock) void k_multi_dot_partial(size_t n, const double *__restrict__ X, size_t ld,
+                                                              const double *__restrict__ y,
+                                                              double *__restrict__ partials) {
+    const int i = blockIdx.y;
+    const double *x = X + (size_t)i * ld;
+    double acc = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        acc += x[t] * y[t];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partials[(size_t)i * gridDim.x + blockIdx.x] = acc;
+}
+__global__ __launch_bounds__(kBlock) void k_finish_partials(const double *__restrict__ partials, int nblocks,
+                                                            double *__restrict__ out) {
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partials[(size_t)i * nblocks + b];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) out[i] = acc;
+}
+
+int reduction_blocks() { return 512; }
+
+void launch_multi_dot(size_t n, int m, const double *X, size_t ld, const double *y, double *out, double *partials,
+                      int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_multi_dot_partial, dim3(nblocks, m), dim3(kBlock), 0, s, n, X, ld, y, partials);
+    hipLaunchKernelGGL(k_finish_partials, dim3(m), dim3(kBlock), 0, s, partials, nblocks, out);
+    TSW_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_axpy(size_t n, int m, const double *__restrict__ X, size_t ld,
+                                                       const double *__restrict__ h, double *__restrict__ y) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double acc = y[t];
+    for (int i = 0; i < m; ++i) acc -= h[i] * X[(size_t)i * ld + t];
+    y[t] = acc;
+}
+void launch_multi_axpy(size_t n, int m, const double *X, size_t ld, const double *h, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_multi_axpy, dim3(grid_for(n)), dim3(kBlock), 0, s, n, m, X, ld, h, y);
+    TSW_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_sqdist_partial(const double *__restrict__ a, const double *__restrict__ b,
+                                                           int nx, int ny, int N, double *__restrict__ partials) {
+    const size_t n = (size_t)nx * ny * N;
+    double acc = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t / N;
+        const int k = (int)(t - c * N);
+        const size_t cell = (size_t)(1 + c / ny) * (ny + 2) + 1 + c % ny;
+        const double d = a[cell * N + k] - b[cell * N + k];
+        acc += d * d;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+void launch_sqdist_interior(const Geom &g, const double *a, const double *b, double *out, double *partials,
+                            int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_sqdist_partial, dim3(nblocks), dim3(kBlock), 0, s, a, b, g.nx, g.ny, g.N(), partials);
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(kBlock), 0, s, partials, nblocks, out);
+    TSW_HIP(hipGetLastError());
+}
+
+}  // namespace tsw

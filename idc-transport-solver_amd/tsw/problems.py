@@ -1,0 +1,96 @@
+"""Problem generators producing the reference-layout inputs of a sweep or solve: cross sections,
+angular source with inflow halos, grid and ordinate set.
+
+The reference defines problems as classes evaluated at cell centres and projected into the DG basis
+(src/utils/init.cpp:422-946, src/utils/Problems/*).  All problems here have cross sections and
+sources constant per cell, so the projection is exact: sigma is the cell value (DG degree 0,
+init.cpp:741) and the angular source is q/(4 pi) in the (0,0) Legendre coefficient
+(init.cpp:853,924)."""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import level_symmetric
+
+
+@dataclass
+class Problem:
+    name: str
+    nx: int
+    ny: int
+    p: int
+    ax: float
+    bx: float
+    ay: float
+    by: float
+    xi: np.ndarray
+    eta: np.ndarray
+    w: np.ndarray
+    quad_off: np.ndarray
+    sigma_t: np.ndarray   # [nx+2][ny+2]
+    sigma_s: np.ndarray   # [nx+2][ny+2]
+    source: np.ndarray    # [nx+2][ny+2][nq][p+1][p+1], halos = inflow
+
+    @property
+    def nq(self):
+        return int(self.xi.size)
+
+    @property
+    def unknowns(self):
+        """Angular unknowns per sweep, nq * nx * ny * (p+1)^2."""
+        return self.nq * self.nx * self.ny * (self.p + 1) ** 2
+
+    def set_on(self, ctx):
+        ctx.problem_set(self.nx, self.ny, self.p, self.xi, self.eta, self.w, self.quad_off, self.ax, self.bx,
+                        self.ay, self.by)
+
+
+def _centres(n, a, b):
+    h = (b - a) / n
+    return a + (np.arange(n + 2) - 0.5) * h  # includes halo cells 0 and n+1
+
+
+def _isotropic_source(q_cell, nq, p):
+    nx2, ny2 = q_cell.shape
+    src = np.zeros((nx2, ny2, nq, p + 1, p + 1))
+    src[:, :, :, 0, 0] = q_cell[:, :, None] / (4.0 * math.pi)
+    return src
+
+
+def lattice(cells_per_unit=40, p=2, order=12) -> Problem:
+    """Checkerboard lattice on [0,7]^2 (LatticeProblem.cpp:173-255): absorber unit blocks with
+    sigma_t = 10, sigma_s = 0; elsewhere sigma_t = sigma_s = 1; q = 1 on (3,4)^2; vacuum inflow.
+    The absorber set is the one TotalCross codes (:231-255): 11 blocks, the centre column holding
+    only the (3,5) block."""
+    n = 7 * cells_per_unit
+    x = _centres(n, 0.0, 7.0)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    absorbers = [(1, 1), (1, 3), (1, 5), (2, 2), (2, 4), (3, 5), (4, 2), (4, 4), (5, 1), (5, 3), (5, 5)]
+    mask = np.zeros_like(X, dtype=bool)
+    for bx_, by_ in absorbers:
+        mask |= (X > bx_) & (X <= bx_ + 1) & (Y > by_) & (Y <= by_ + 1)
+    sigma_t = np.where(mask, 10.0, 1.0)
+    sigma_s = np.where(mask, 0.0, 1.0)
+    q = np.where((X > 3) & (X < 4) & (Y > 3) & (Y < 4), 1.0, 0.0)
+    halo = np.ones_like(mask)
+    halo[1:-1, 1:-1] = False
+    q[halo] = 0.0  # vacuum inflow
+    xi, eta, w, off = level_symmetric(order)
+    return Problem("lattice", n, n, p, 0.0, 7.0, 0.0, 7.0, xi, eta, w, off, sigma_t, sigma_s,
+                   _isotropic_source(q, xi.size, p))
+
+
+def central_source(n=64, p=1, order=8, sigma_t=1.0, sigma_s=0.5) -> Problem:
+    """Unit square, uniform sigma, isotropic q = 1 on [0.375, 0.625]^2, vacuum boundaries."""
+    x = _centres(n, 0.0, 1.0)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    q = np.where((X > 0.375) & (X < 0.625) & (Y > 0.375) & (Y < 0.625), 1.0, 0.0)
+    q[0, :] = q[-1, :] = q[:, 0] = q[:, -1] = 0.0
+    xi, eta, w, off = level_symmetric(order)
+    st = np.full((n + 2, n + 2), sigma_t)
+    ss = np.full((n + 2, n + 2), sigma_s)
+    return Problem("central-source", n, n, p, 0.0, 1.0, 0.0, 1.0, xi, eta, w, off, st, ss,
+                   _isotropic_source(q, xi.size, p))

@@ -1,0 +1,121 @@
+/*
+ * tsw.h -- C-ABI of the MI355X transport-sweep library (libtsw.so).
+ *
+ * The reference's sweep path is C++ (crockatt/idc-transport-solver).  Each entry point below
+ * replaces one member of its operator interface; the citation names the reference declaration and
+ * the behaviour kept.  Conventions:
+ *   - every function returns int status: TSW_OK (0) or a negative TSW_E_* code; no exception
+ *     crosses the ABI; tsw_last_error(ctx) returns the message of the last failure on ctx;
+ *   - arrays use the reference host layouts (1-cell halo on each side, 1-based interior):
+ *       psi   [i][j][q][d][e]  (nx+2)*(ny+2)*nq*(p+1)^2 doubles  (OrdinateFlux.hpp:639-700)
+ *       phi   [i][j][d][e]     (nx+2)*(ny+2)*(p+1)^2 doubles     (DensityFunction.hpp:475-535)
+ *       sigma [i][j]           (nx+2)*(ny+2) doubles, cell means   (CrossSection.hpp:397-478, DG degree 0)
+ *   - `mem` says where the caller's arrays live: TSW_MEM_HOST (pageable or pinned host memory)
+ *     or TSW_MEM_DEVICE (device pointers on the context's device, same layouts);
+ *   - all buffers are caller-owned; the context owns only its scratch and solver state;
+ *   - every call is complete (device work finished) when it returns, as the reference's CPU
+ *     calls are; one context per host thread.
+ */
+#ifndef TSW_H
+#define TSW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSW_OK 0
+#define TSW_E_ARG (-1)         /* invalid argument (std::invalid_argument / std::out_of_range) */
+#define TSW_E_HIP (-2)         /* HIP runtime failure */
+#define TSW_E_STATE (-3)       /* call made before tsw_problem_set */
+#define TSW_E_NOCONV (-4)      /* iterative solve did not converge (PETScPeierlsSolver.cpp:489-497) */
+#define TSW_E_UNSUPPORTED (-5) /* configuration not built (e.g. DG degree > 3) */
+
+#define TSW_FP64 0 /* sweep and element solves in double precision */
+#define TSW_FP32 1 /* sweep and element solves in single precision (outer loops stay FP64) */
+
+#define TSW_MEM_HOST 0
+#define TSW_MEM_DEVICE 1
+
+typedef struct tsw_ctx tsw_ctx;
+
+/* Creates a context on HIP device `device`.  Replaces the construction of SweepOperator
+ * (SweepOperator.cpp:25-49) minus the mesh, which tsw_problem_set supplies. */
+int tsw_create(int device, tsw_ctx **out);
+
+/* Releases the context and its device memory.  Safe on NULL. */
+void tsw_destroy(tsw_ctx *ctx);
+
+/* Message of the last failing call on ctx ("" if none).  Valid until the next call on ctx. */
+const char *tsw_last_error(const tsw_ctx *ctx);
+
+/* Mesh + ordinate set + angular matrices: DomainDecomposition (DomainDecomposition.hpp:35-62,
+ * uniform Cartesian [ax,bx]x[ay,by] with nx*ny cells), OrdinateSet accessors xi/eta/w/Quadrants
+ * (OrdinateSet.hpp:236-418) and SweepLinearSolveSpec::ComputeMatrices (SweepLinearSolveSpec.cpp:47-178).
+ * quad_off[5]: ordinates q in [quad_off[k], quad_off[k+1]) lie in quadrant k
+ * (0: xi>0,eta>0  1: xi>0,eta<0  2: xi<0,eta>0  3: xi<0,eta<0; OrdinateSet.cpp:658-700).
+ * p in 0..3.  Resets solver state. */
+int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
+                    const double *xi, const double *eta, const double *w, const int *quad_off);
+
+/* Selects the traversal, as the `sweep_pattern` factory key does (SweepPattern.cpp:314-359):
+ *   "hip-wavefront"  one launch per diagonal i+j = const, all quadrants per launch;
+ *   "hip-pipelined"  one persistent launch per sweep, row bands pipelined through device flags.
+ * Unknown names fail with TSW_E_ARG as the reference's factory throws std::out_of_range. */
+int tsw_set_pattern(tsw_ctx *ctx, const char *name);
+
+/* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
+ * cells are overwritten.  dt <= 0 or +inf is steady state.  SweepOperator::Apply
+ * (SweepOperator.hpp:126-131, SweepOperator.cpp:118-135).  precision TSW_FP32 converts psi and sigma
+ * to single precision for the sweep and back. */
+int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem);
+
+/* phi <- beta*phi + alpha*sum_q w_q psi_q.  TransportOperator::Pmv (TransportOperator.hpp:25-43,
+ * TransportOperator.cpp:37-186).  all_cells: 0 = OpDomain::Interior, 1 = OpDomain::All. */
+int tsw_pmv(tsw_ctx *ctx, double alpha, double beta, const double *psi, double *phi, int all_cells, int mem);
+
+/* psi_q <- beta*psi_q + alpha*sigma_s*phi/(4 pi).  TransportOperator::Smv (TransportOperator.cpp:203-443). */
+int tsw_smv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_s, const double *phi, double *psi,
+            int all_cells, int mem);
+
+/* y <- beta*y + alpha*(Omega.grad + sigma_t) x on interior cells; x's halos give the inflow.
+ * TransportOperator::Lmv (TransportOperator.cpp:1215-1683), x != y. */
+int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, const double *x, double *y, int mem);
+
+/* Source iteration, ImplicitSolver::Solve under solve_type "source-iteration" (SISolver.cpp:190-254):
+ * source holds Q on entry (halos = inflow) and psi on exit.  The solver's angular-flux state
+ * persists in ctx between calls (zero after tsw_problem_set), as SISolver::psi_ptr does.
+ * Stops when the absolute l2 change of the scalar-flux coefficients is <= abs_tol or after max_its.
+ * iters / err (may be NULL) receive the iteration count and the last change. */
+int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
+                 double abs_tol, int max_its, int precision, int mem, int *iters, double *err);
+
+/* Sweep-preconditioned Krylov solve on the Peierls form (I - P L^{-1} S) phi = P L^{-1} Q, then
+ * source <- L^{-1}(Q + S phi): ImplicitSolver::Solve under solve_type "petsc-peierls-gmres"
+ * (PETScPeierlsSolver.cpp:423-514, matvec IPLS :529-572), restarted GMRES(restart) with the
+ * absolute residual test ||r||_2 <= abs_tol (:139-143).  phi_out (may be NULL): converged scalar
+ * flux, phi layout.  precision TSW_FP64: FP64 sweeps.  TSW_FP32: FP32 sweeps inside an FP64
+ * iterative refinement on the angular residual r = Q + S P psi - L psi (FP64 Lmv), repeated until
+ * ||P r||_2 <= abs_tol.  Returns TSW_E_NOCONV if max_its is reached. */
+int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
+                    double abs_tol, int max_its, int restart, int precision, int mem, double *phi_out,
+                    int *iters, double *resid);
+
+/* Level-symmetric S_order ordinates reduced to the 2D hemisphere (order in {2,4,6,8,10,12,16}):
+ * nq = order*(order+2)/2 directions, weights summing to 4 pi, grouped by quadrant as above.
+ * The reference has no level-symmetric set; this plays the role of OrdinateSet's constructors
+ * (OrdinateSet.cpp:40-91) and SymmetricReduce (:1171-1190).  Host-only, needs no device.
+ * Arrays must hold nq entries, quad_off 5. */
+int tsw_ordinates_level_symmetric(int order, double *xi, double *eta, double *w, int *quad_off);
+
+/* Timing of the sweeps run on ctx since the last reset, measured with HIP events on the context's
+ * stream around each sweep: count, total milliseconds.  Replaces the TMR_AF_Lsv wall timer
+ * (SweepOperator.cpp:129,134). */
+int tsw_sweep_timing(tsw_ctx *ctx, int reset, long long *n_sweeps, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -289,3 +289,30 @@ def dg_weak_form_solve(g: Grid, xi, eta, sigma_t, src_coef, inflow_coef, inv_dt=
     out = inflow_coef.copy()
     out[1:nx + 1, 1:ny + 1] = sol.reshape(nx, ny, P, P)
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Seeded problems shared by the GPU parity tests and the benchmark
+# ------------------------------------------------------------------------------------------------
+def ls_ordinates(order) -> Ordinates:
+    import tsw
+    xi, eta, w, off = tsw.level_symmetric(order)
+    return Ordinates(xi, eta, w, off)
+
+
+def random_problem(nx, ny, p, ords: Ordinates, seed, ax=0.0, bx=1.0, ay=0.0, by=1.0, halo_inflow=True):
+    """Random cross sections (sigma_s < sigma_t), random source and inflow (halo) coefficients."""
+    rng = np.random.default_rng(seed)
+    g = Grid(nx, ny, p, ax, bx, ay, by)
+    sigma_t = rng.uniform(0.3, 4.0, size=(nx + 2, ny + 2))
+    sigma_s = sigma_t * rng.uniform(0.0, 0.9, size=sigma_t.shape)
+    src = rng.uniform(-0.5, 1.0, size=g.psi_shape(ords.nq))
+    if not halo_inflow:
+        src[0], src[-1], src[:, 0], src[:, -1] = 0.0, 0.0, 0.0, 0.0
+    return g, sigma_t, sigma_s, src
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))

@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path behind the C-ABI against the CPU oracle on the same seeded inputs.
+
+Bars: FP64 operator results within 1e-12 relative of the oracle; FP32 sweeps within 2e-6 relative of
+the FP64 oracle sweep; converged FP64 solves within the solver tolerance (1e-11 relative at
+abs_tol 1e-13); mixed-precision GMRES (FP32 sweeps, FP64 refinement to 1e-10 absolute) within 1e-8."""
+import math
+
+import numpy as np
+import pytest
+
+import tsw
+from _common import (Grid, ls_ordinates, orc_gmres, orc_lmv, orc_pmv, orc_si, orc_smv, orc_sweep,
+                     random_ordinates, random_problem, rel_err)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = tsw.Context(0)
+    yield c
+    c.close()
+
+
+def setup(ctx, g, o):
+    ctx.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
+
+
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-wavefront-eager"])
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("dt", [math.inf, 0.1])
+def test_sweep_fp64_matches_oracle(ctx, pattern, p, dt):
+    o = ls_ordinates(6)
+    g, st, ss, src = random_problem(7, 5, p, o, seed=10 + p, ax=-1.0, bx=2.5, ay=0.0, by=1.0)
+    print(f"sweep fp64 p={p} pattern={pattern} dt={dt}", flush=True)
+    setup(ctx, g, o)
+    ctx.set_pattern(pattern)
+    ref = orc_sweep(g, o, st, src, dt)
+    out = src.copy()
+    ctx.sweep(out, st, dt)
+    assert rel_err(out, ref) <= 1e-12
+    np.testing.assert_array_equal(out[0], src[0])       # halos untouched
+    np.testing.assert_array_equal(out[:, -1], src[:, -1])
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (12, 12)])
+def test_sweep_degenerate_shapes(ctx, shape):
+    o = random_ordinates(3, np.random.default_rng(1))
+    g, st, ss, src = random_problem(shape[0], shape[1], 1, o, seed=3)
+    setup(ctx, g, o)
+    ctx.set_pattern("hip-wavefront")
+    ref = orc_sweep(g, o, st, src)
+    out = src.copy()
+    ctx.sweep(out, st)
+    assert rel_err(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_sweep_fp32_within_tolerance(ctx, p):
+    o = ls_ordinates(8)
+    g, st, ss, src = random_problem(9, 8, p, o, seed=20 + p)
+    src[1:-1, 1:-1] = np.abs(src[1:-1, 1:-1])
+    setup(ctx, g, o)
+    ref = orc_sweep(g, o, st, src)
+    out = src.copy()
+    ctx.sweep(out, st, precision=tsw.FP32)
+    assert rel_err(out, ref) <= 2e-6
+
+
+def test_sweep_device_memory_equals_host(ctx):
+    import torch
+    o = ls_ordinates(4)
+    g, st, ss, src = random_problem(10, 6, 2, o, seed=5)
+    setup(ctx, g, o)
+    host = src.copy()
+    ctx.sweep(host, st)
+    d_psi = torch.from_numpy(src.copy()).cuda()
+    d_st = torch.from_numpy(st.copy()).cuda()
+    torch.cuda.synchronize()
+    ctx.sweep(d_psi, d_st)
+    np.testing.assert_array_equal(d_psi.cpu().numpy(), host)
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_pmv_smv_lmv_match_oracle(ctx, p):
+    o = ls_ordinates(4)
+    g, st, ss, src = random_problem(6, 7, p, o, seed=30 + p)
+    setup(ctx, g, o)
+    rng = np.random.default_rng(p)
+    phi0 = rng.normal(size=g.phi_shape())
+    for all_cells in (0, 1):
+        ref = orc_pmv(g, o, src, phi0, alpha=0.7, beta=-1.3, all_cells=all_cells)
+        out = ctx.pmv(src, phi0.copy(), alpha=0.7, beta=-1.3, all_cells=all_cells)
+        assert rel_err(out, ref) <= 1e-13
+        ref = orc_smv(g, o, ss, phi0, src, alpha=1.1, beta=0.4, all_cells=all_cells)
+        out = ctx.smv(ss, phi0, src.copy(), alpha=1.1, beta=0.4, all_cells=all_cells)
+        assert rel_err(out, ref) <= 1e-13
+    y0 = rng.normal(size=src.shape)
+    ref = orc_lmv(g, o, st, src, y0, alpha=0.9, beta=0.5)
+    out = ctx.lmv(st, src, y0.copy(), alpha=0.9, beta=0.5)
+    assert rel_err(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
+def test_source_iteration_matches_oracle(ctx, precision):
+    o = ls_ordinates(8)
+    g = Grid(16, 16, 1)
+    st = np.ones((18, 18))
+    ss = np.full((18, 18), 0.5)
+    src = np.zeros(g.psi_shape(o.nq))
+    src[1 + 6:1 + 10, 1 + 6:1 + 10, :, 0, 0] = 1.0 / (4 * math.pi)  # central block source, vacuum
+    setup(ctx, g, o)
+    ref, it_ref, err_ref = orc_si(g, o, st, ss, src, abs_tol=1e-11, max_its=200)
+    out = src.copy()
+    it, err = ctx.si_solve(out, st, ss, abs_tol=1e-11, max_its=200, precision=precision)
+    if precision == tsw.FP64:
+        assert it == it_ref
+        assert rel_err(out, ref) <= 1e-12
+    else:
+        assert rel_err(out, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_gmres_fp64_matches_oracle(ctx, p):
+    o = ls_ordinates(6)
+    g, st, ss, src = random_problem(10, 9, p, o, seed=40 + p)
+    setup(ctx, g, o)
+    rc, ref_psi, ref_phi, its_ref, res_ref = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
+    assert rc == 0
+    out = src.copy()
+    phi = np.zeros(g.phi_shape())
+    its, res = ctx.gmres_solve(out, st, ss, phi_out=phi, abs_tol=1e-13, restart=30)
+    assert res <= 1e-13
+    assert rel_err(phi, ref_phi) <= 1e-11
+    assert rel_err(out, ref_psi) <= 1e-11
+
+
+def test_gmres_mixed_precision_refinement(ctx):
+    o = ls_ordinates(6)
+    g, st, ss, src = random_problem(12, 12, 2, o, seed=50)
+    setup(ctx, g, o)
+    rc, ref_psi, ref_phi, _, _ = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
+    out = src.copy()
+    phi = np.zeros(g.phi_shape())
+    its, res = ctx.gmres_solve(out, st, ss, phi_out=phi, abs_tol=1e-10, restart=30, precision=tsw.FP32)
+    assert res <= 1e-10
+    assert rel_err(phi, ref_phi) <= 1e-8
+    assert rel_err(out, ref_psi) <= 1e-8
+
+
+def test_errors_are_reported(ctx):
+    o = ls_ordinates(4)
+    with pytest.raises(tsw.TswError) as e:
+        ctx.problem_set(4, 4, 4, o.xi, o.eta, o.w, o.quad_off)
+    assert e.value.code == tsw.TSW_E_UNSUPPORTED
+    bad = o.quad_off.copy()
+    bad[1], bad[2] = bad[2], bad[1]
+    with pytest.raises(tsw.TswError) as e:
+        ctx.problem_set(4, 4, 1, o.xi, o.eta, o.w, bad)
+    assert e.value.code == tsw.TSW_E_ARG
+    with pytest.raises(tsw.TswError) as e:
+        ctx.set_pattern("no-such-pattern")
+    assert e.value.code == tsw.TSW_E_ARG
+    fresh = tsw.Context(0)
+    psi = np.zeros((3, 3, o.nq, 2, 2))
+    with pytest.raises(tsw.TswError) as e:
+        fresh.sweep(psi, np.ones((3, 3)))
+    assert e.value.code == tsw.TSW_E_STATE
+    fresh.close()
+
+
+def test_full_size_lattice_sweep_properties(ctx):
+    """280x280, Q2, S12 (the lattice workload's sweep): L (L^{-1} q) = q with the GPU Lmv, FP64 and FP32,
+    and agreement of the FP32 and FP64 sweeps."""
+    import torch
+    o = ls_ordinates(12)
+    g, st, ss, src = random_problem(280, 280, 2, o, seed=60, bx=7.0, by=7.0)
+    setup(ctx, g, o)
+    d_src = torch.from_numpy(src).cuda()
+    d_st = torch.from_numpy(st).cuda()
+    psi = d_src.clone()
+    torch.cuda.synchronize()
+    ctx.sweep(psi, d_st)
+    back = torch.zeros_like(psi)
+    ctx.lmv(d_st, psi, back)
+    inner = (slice(1, -1), slice(1, -1))
+    r = (back[inner] - d_src[inner]).norm() / d_src[inner].norm()
+    assert float(r) <= 1e-12
+    psi32 = d_src.clone()
+    torch.cuda.synchronize()
+    ctx.sweep(psi32, d_st, precision=tsw.FP32)
+    r32 = (psi32[inner] - psi[inner]).norm() / psi[inner].norm()
+    assert float(r32) <= 2e-6
