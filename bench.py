@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pattern", default="hip-wavefront")
+    ap.add_argument("--pattern", default="hip-staggered")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-sweeps", type=int, default=3, help="oracle sweeps timed for cpu_baseline (0: skip)")
     args = ap.parse_args()

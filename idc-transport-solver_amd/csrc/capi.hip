@@ -10,7 +10,7 @@
 
 using namespace tsw;
 
-enum class Pattern { WavefrontGraph, WavefrontEager, Pipelined };
+enum class Pattern { WavefrontGraph, WavefrontEager, Staggered };
 
 struct tsw_ctx {
     int device = 0;
@@ -134,6 +134,12 @@ SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt)
             c->g.quad_off[2], c->g.quad_off[3], (float)c->g.dx, (float)c->g.dy, (float)inv_dt};
 }
 
+// The staggered kernel keeps one thread per row and the inflow traces of one halo row in LDS.
+template <typename T>
+bool staggered_fits(const Geom &g) {
+    return g.ny <= 512 && (2 * 512 + (size_t)g.nx) * 4 * sizeof(T) <= 64 * 1024;
+}
+
 // One in-place sweep of device array psi with device cell cross sections sigma.
 template <typename T>
 void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
@@ -163,7 +169,10 @@ void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
             TSW_HIP(hipGraphLaunch(it->second, c->stream));
             break;
         }
-        case Pattern::Pipelined: throw Error(TSW_E_UNSUPPORTED, "hip-pipelined sweep is not built");
+        case Pattern::Staggered:
+            if (staggered_fits<T>(c->g)) launch_sweep_staggered<T>(sp, c->g.p, c->stream);
+            else launch_sweep_wavefront<T>(sp, c->g.p, c->stream);
+            break;
     }
     TSW_HIP(hipEventRecord(e1, c->stream));
     ++c->sweeps_done;
@@ -389,7 +398,7 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
         const std::string s(name);
         if (s == "hip-wavefront") ctx->pattern = Pattern::WavefrontGraph;
         else if (s == "hip-wavefront-eager") ctx->pattern = Pattern::WavefrontEager;
-        else if (s == "hip-pipelined") throw Error(TSW_E_UNSUPPORTED, "sweep_pattern 'hip-pipelined' is not built");
+        else if (s == "hip-staggered") ctx->pattern = Pattern::Staggered;
         else throw Error(TSW_E_ARG, "unknown sweep_pattern '" + s + "'");
     });
 }

@@ -105,3 +105,92 @@ __device__ __forceinline__ void ge_solve(T (&A)[N * N], T (&b)[N]) {
 }
 
 }  // namespace tsw
+
+namespace tsw {
+
+// ---------------------------------------------------------------------------------------------
+// Face traces.  The upwind contribution to B only needs the upwind cell's polynomial restricted to
+// the shared face: for xi > 0 the right-face trace Tx(e) = sum_a psi(a,e), for xi <= 0 the
+// left-face trace Tx(e) = sum_a (-1)^a psi(a,e) (P_a(1) = 1, P_a(-1) = (-1)^a); same in y.
+// In terms of traces the reference's CalcB (SweepLinearSolveSpec.cpp:545-598) reads
+//   B(d,e) = dx dy src(d,e) + (2d+1) cx kx(d) Tx(e) + (2e+1) cy ky(e) Ty(d),
+//   kx(d) = (-1)^d if xi > 0 else -1,   ky(e) = (-1)^e if eta > 0 else -1.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int P>
+__device__ __forceinline__ void trace_x(const T *__restrict__ psi, bool xneg, T (&tx)[P]) {
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+        T s = T(0);
+#pragma unroll
+        for (int a = 0; a < P; ++a) s += (xneg && (a & 1)) ? -psi[e + P * a] : psi[e + P * a];
+        tx[e] = s;
+    }
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void trace_y(const T *__restrict__ psi, bool yneg, T (&ty)[P]) {
+#pragma unroll
+    for (int d = 0; d < P; ++d) {
+        T s = T(0);
+#pragma unroll
+        for (int b = 0; b < P; ++b) s += (yneg && (b & 1)) ? -psi[b + P * d] : psi[b + P * d];
+        ty[d] = s;
+    }
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void rhs_from_traces(T (&B)[P * P], const T (&src)[P * P], const T (&tx)[P],
+                                                const T (&ty)[P], T xi, T eta, T dx, T dy) {
+    const bool xneg = xi <= T(0), yneg = eta <= T(0);
+    const T cx = dy * xi, cy = dx * eta, area = dx * dy;
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e) {
+            const T kx = xneg ? T(-(2 * d + 1)) : T(ipow_m1(d) * (2 * d + 1));
+            const T ky = yneg ? T(-(2 * e + 1)) : T(ipow_m1(e) * (2 * e + 1));
+            B[e + P * d] = area * src[e + P * d] + (cx * kx) * tx[e] + (cy * ky) * ty[d];
+        }
+}
+
+}  // namespace tsw
+
+namespace tsw {
+
+// Unpivoted Gaussian elimination in the reference's order, with each pivot inverted once and the
+// divisions of SIMD_LinearAlgebra.hpp:91,105 replaced by multiplications with that reciprocal
+// (N reciprocals instead of N(N+1)/2 divisions; the results differ from the dividing form by
+// rounding only).
+template <typename T, int N>
+__device__ __forceinline__ void ge_solve_rcp(T (&A)[N * N], T (&b)[N]) {
+    T inv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        inv[j] = T(1) / A[j + N * j];
+        const T ninv = -inv[j];
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            A[i + N * j] *= ninv;
+            b[i] += A[i + N * j] * b[j];
+        }
+#pragma unroll
+        for (int k = j + 1; k < N; ++k)
+#pragma unroll
+            for (int i = j + 1; i < N; ++i) A[i + N * k] += A[i + N * j] * A[j + N * k];
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        b[i] *= inv[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) b[j] -= A[j + N * i] * b[i];
+    }
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads (prefetches) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+}  // namespace tsw

@@ -80,6 +80,8 @@ template <typename T> struct SweepParams {
 // ---- launch wrappers (defined in the .hip files) ----
 template <typename T>
 void launch_sweep_wavefront(const SweepParams<T> &sp, int p, hipStream_t s);
+template <typename T>
+void launch_sweep_staggered(const SweepParams<T> &sp, int p, hipStream_t s);
 
 template <typename TI, typename TO>
 void launch_pmv(const Geom &g, const TI *psi, TO *phi, const TO *w, TO alpha, TO beta, int all_cells, hipStream_t s);

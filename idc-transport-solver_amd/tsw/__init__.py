@@ -44,9 +44,17 @@ _vp = ctypes.c_void_p
 
 
 def lib():
-    """Loads libtsw.so (built in-tree by `make -C idc-transport-solver_amd`)."""
+    """Loads libtsw.so (built in-tree by `make -C idc-transport-solver_amd`).
+
+    When torch is importable it is imported first: torch ships its own HIP runtime with the same
+    SONAME (libamdhip64.so.7), and loading it first makes libtsw.so bind to that copy, so a process
+    holds one HIP runtime and device pointers move freely between torch and libtsw."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)

@@ -24,9 +24,13 @@ def ctx():
 
 def setup(ctx, g, o):
     ctx.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
+    ctx.set_pattern("hip-wavefront")
 
 
-@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-wavefront-eager"])
+PATTERNS = ["hip-wavefront", "hip-wavefront-eager", "hip-staggered"]
+
+
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
 @pytest.mark.parametrize("dt", [math.inf, 0.1])
 def test_sweep_fp64_matches_oracle(ctx, pattern, p, dt):
@@ -43,24 +47,27 @@ def test_sweep_fp64_matches_oracle(ctx, pattern, p, dt):
     np.testing.assert_array_equal(out[:, -1], src[:, -1])
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (12, 12)])
-def test_sweep_degenerate_shapes(ctx, shape):
+@pytest.mark.parametrize("pattern", PATTERNS)
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (12, 12), (3, 70), (70, 3)])
+def test_sweep_degenerate_shapes(ctx, shape, pattern):
     o = random_ordinates(3, np.random.default_rng(1))
     g, st, ss, src = random_problem(shape[0], shape[1], 1, o, seed=3)
     setup(ctx, g, o)
-    ctx.set_pattern("hip-wavefront")
+    ctx.set_pattern(pattern)
     ref = orc_sweep(g, o, st, src)
     out = src.copy()
     ctx.sweep(out, st)
     assert rel_err(out, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
-def test_sweep_fp32_within_tolerance(ctx, p):
+def test_sweep_fp32_within_tolerance(ctx, p, pattern):
     o = ls_ordinates(8)
     g, st, ss, src = random_problem(9, 8, p, o, seed=20 + p)
     src[1:-1, 1:-1] = np.abs(src[1:-1, 1:-1])
     setup(ctx, g, o)
+    ctx.set_pattern(pattern)
     ref = orc_sweep(g, o, st, src)
     out = src.copy()
     ctx.sweep(out, st, precision=tsw.FP32)
@@ -101,8 +108,9 @@ def test_pmv_smv_lmv_match_oracle(ctx, p):
     assert rel_err(out, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
 @pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
-def test_source_iteration_matches_oracle(ctx, precision):
+def test_source_iteration_matches_oracle(ctx, precision, pattern):
     o = ls_ordinates(8)
     g = Grid(16, 16, 1)
     st = np.ones((18, 18))
@@ -110,6 +118,7 @@ def test_source_iteration_matches_oracle(ctx, precision):
     src = np.zeros(g.psi_shape(o.nq))
     src[1 + 6:1 + 10, 1 + 6:1 + 10, :, 0, 0] = 1.0 / (4 * math.pi)  # central block source, vacuum
     setup(ctx, g, o)
+    ctx.set_pattern(pattern)
     ref, it_ref, err_ref = orc_si(g, o, st, ss, src, abs_tol=1e-11, max_its=200)
     out = src.copy()
     it, err = ctx.si_solve(out, st, ss, abs_tol=1e-11, max_its=200, precision=precision)
@@ -120,11 +129,13 @@ def test_source_iteration_matches_oracle(ctx, precision):
         assert rel_err(out, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
 @pytest.mark.parametrize("p", [1, 2])
-def test_gmres_fp64_matches_oracle(ctx, p):
+def test_gmres_fp64_matches_oracle(ctx, p, pattern):
     o = ls_ordinates(6)
     g, st, ss, src = random_problem(10, 9, p, o, seed=40 + p)
     setup(ctx, g, o)
+    ctx.set_pattern(pattern)
     rc, ref_psi, ref_phi, its_ref, res_ref = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
     assert rc == 0
     out = src.copy()
@@ -135,10 +146,12 @@ def test_gmres_fp64_matches_oracle(ctx, p):
     assert rel_err(out, ref_psi) <= 1e-11
 
 
-def test_gmres_mixed_precision_refinement(ctx):
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
+def test_gmres_mixed_precision_refinement(ctx, pattern):
     o = ls_ordinates(6)
     g, st, ss, src = random_problem(12, 12, 2, o, seed=50)
     setup(ctx, g, o)
+    ctx.set_pattern(pattern)
     rc, ref_psi, ref_phi, _, _ = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
     out = src.copy()
     phi = np.zeros(g.phi_shape())
@@ -169,13 +182,15 @@ def test_errors_are_reported(ctx):
     fresh.close()
 
 
-def test_full_size_lattice_sweep_properties(ctx):
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
+def test_full_size_lattice_sweep_properties(ctx, pattern):
     """280x280, Q2, S12 (the lattice workload's sweep): L (L^{-1} q) = q with the GPU Lmv, FP64 and FP32,
     and agreement of the FP32 and FP64 sweeps."""
     import torch
     o = ls_ordinates(12)
     g, st, ss, src = random_problem(280, 280, 2, o, seed=60, bx=7.0, by=7.0)
     setup(ctx, g, o)
+    ctx.set_pattern(pattern)
     d_src = torch.from_numpy(src).cuda()
     d_st = torch.from_numpy(st).cuda()
     psi = d_src.clone()
