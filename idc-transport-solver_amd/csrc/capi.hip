@@ -4,13 +4,14 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "tsw_internal.hpp"
 
 using namespace tsw;
 
-enum class Pattern { WavefrontGraph, WavefrontEager, Staggered };
+enum class Pattern { WavefrontGraph, WavefrontEager, Staggered, StaggeredGE };
 
 struct tsw_ctx {
     int device = 0;
@@ -33,6 +34,17 @@ struct tsw_ctx {
     size_t ev_used = 0;
     long long sweeps_done = 0;
     double ms_folded = 0.0;  // elapsed time of event pairs already released
+
+    // Material table: distinct sigma_t values of the interior cells, the per-cell material index and
+    // the element-matrix inverses for the current dt, in both precisions (built on first use).
+    struct Materials {
+        bool valid = false;
+        int M = 0;
+        double inv_dt = -1.0;
+        DBuf idx, sig, inv64, inv32;
+        bool have64 = false, have32 = false;
+    } mat;
+    static constexpr int kMaxMaterials = 64;
 
     // captured sweep graphs keyed by (precision, psi, sigma, inv_dt)
     std::map<std::tuple<int, const void *, const void *, double>, hipGraphExec_t> graphs;
@@ -121,23 +133,79 @@ hipEvent_t next_event(tsw_ctx *c) {
 
 template <typename T>
 SweepParams<T> params(tsw_ctx *c, T *psi, const T *sigma, double dt);
+double inverse_dt(double dt) { return (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0; }
+
+// Attaches the material table to a sweep when the pattern uses it and it was built for this dt.
+template <typename T>
+void attach_materials(tsw_ctx *c, SweepParams<T> &sp, double dt) {
+    auto &m = c->mat;
+    if (c->pattern != Pattern::Staggered || !m.valid || m.inv_dt != inverse_dt(dt)) return;
+    const size_t n = (size_t)m.M * c->g.nq * c->g.N() * c->g.N();
+    bool &have = sizeof(T) == 8 ? m.have64 : m.have32;
+    DBuf &buf = sizeof(T) == 8 ? m.inv64 : m.inv32;
+    if (!have) {
+        T *out = static_cast<T *>(buf.reserve(n * sizeof(T)));
+        launch_material_inverses<T>(c->g, c->xi64.as<double>(), c->eta64.as<double>(), m.sig.as<double>(), m.M,
+                                    m.inv_dt, out, c->stream);
+        have = true;
+    }
+    SweepParams<T> trial = sp;
+    trial.mat = m.idx.as<uint16_t>();
+    trial.ainv = buf.as<T>();
+    trial.M = m.M;
+    if (staggered_lds_bytes<T>(trial, c->g.p) != 0) sp = trial;
+}
+
 template <>
 SweepParams<double> params(tsw_ctx *c, double *psi, const double *sigma, double dt) {
-    const double inv_dt = (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0;
-    return {psi, sigma, c->xi64.as<double>(), c->eta64.as<double>(), c->g.nx, c->g.ny, c->g.nq, c->g.quad_off[1],
-            c->g.quad_off[2], c->g.quad_off[3], c->g.dx, c->g.dy, inv_dt};
+    SweepParams<double> sp{psi, sigma, c->xi64.as<double>(), c->eta64.as<double>(), c->g.nx, c->g.ny, c->g.nq,
+                           c->g.quad_off[1], c->g.quad_off[2], c->g.quad_off[3], c->g.dx, c->g.dy, inverse_dt(dt)};
+    attach_materials<double>(c, sp, dt);
+    return sp;
 }
 template <>
 SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt) {
-    const double inv_dt = (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0;
-    return {psi, sigma, c->xi32.as<float>(), c->eta32.as<float>(), c->g.nx, c->g.ny, c->g.nq, c->g.quad_off[1],
-            c->g.quad_off[2], c->g.quad_off[3], (float)c->g.dx, (float)c->g.dy, (float)inv_dt};
+    SweepParams<float> sp{psi, sigma, c->xi32.as<float>(), c->eta32.as<float>(), c->g.nx, c->g.ny, c->g.nq,
+                          c->g.quad_off[1], c->g.quad_off[2], c->g.quad_off[3], (float)c->g.dx, (float)c->g.dy,
+                          (float)inverse_dt(dt)};
+    attach_materials<float>(c, sp, dt);
+    return sp;
 }
 
-// The staggered kernel keeps one thread per row and the inflow traces of one halo row in LDS.
-template <typename T>
-bool staggered_fits(const Geom &g) {
-    return g.ny <= 512 && (2 * 512 + (size_t)g.nx) * 4 * sizeof(T) <= 64 * 1024;
+// Builds the material table from device sigma_t (double cell array) for time step dt: reads sigma
+// back, numbers its distinct interior values and uploads the per-cell index.  Leaves the table
+// invalid (so sweeps eliminate per cell) when there are more than kMaxMaterials values.
+void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
+    auto &m = c->mat;
+    m.valid = false;
+    m.have64 = m.have32 = false;
+    if (c->pattern != Pattern::Staggered) return;
+    const Geom &g = c->g;
+    std::vector<double> h(g.ncell_h());
+    TSW_HIP(hipMemcpyAsync(h.data(), sigma, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    TSW_HIP(hipStreamSynchronize(c->stream));
+    std::unordered_map<double, int> ids;
+    std::vector<double> vals;
+    std::vector<uint16_t> idx(h.size(), 0);
+    for (int i = 1; i <= g.nx; ++i)
+        for (int j = 1; j <= g.ny; ++j) {
+            const size_t k = (size_t)i * (g.ny + 2) + j;
+            auto it = ids.find(h[k]);
+            if (it == ids.end()) {
+                if ((int)vals.size() == tsw_ctx::kMaxMaterials) return;
+                it = ids.emplace(h[k], (int)vals.size()).first;
+                vals.push_back(h[k]);
+            }
+            idx[k] = (uint16_t)it->second;
+        }
+    m.M = (int)vals.size();
+    m.inv_dt = inverse_dt(dt);
+    TSW_HIP(hipMemcpyAsync(m.idx.reserve(sizeof(uint16_t) * idx.size()), idx.data(), sizeof(uint16_t) * idx.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    TSW_HIP(hipMemcpyAsync(m.sig.reserve(sizeof(double) * vals.size()), vals.data(), sizeof(double) * vals.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    TSW_HIP(hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    m.valid = true;
 }
 
 // One in-place sweep of device array psi with device cell cross sections sigma.
@@ -170,7 +238,8 @@ void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
             break;
         }
         case Pattern::Staggered:
-            if (staggered_fits<T>(c->g)) launch_sweep_staggered<T>(sp, c->g.p, c->stream);
+        case Pattern::StaggeredGE:
+            if (staggered_lds_bytes<T>(sp, c->g.p) != 0) launch_sweep_staggered<T>(sp, c->g.p, c->stream);
             else launch_sweep_wavefront<T>(sp, c->g.p, c->stream);
             break;
     }
@@ -399,6 +468,7 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
         if (s == "hip-wavefront") ctx->pattern = Pattern::WavefrontGraph;
         else if (s == "hip-wavefront-eager") ctx->pattern = Pattern::WavefrontEager;
         else if (s == "hip-staggered") ctx->pattern = Pattern::Staggered;
+        else if (s == "hip-staggered-ge") ctx->pattern = Pattern::StaggeredGE;
         else throw Error(TSW_E_ARG, "unknown sweep_pattern '" + s + "'");
     });
 }
@@ -409,6 +479,7 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
         const Geom &g = ctx->g;
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
         const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        prepare_materials(ctx, sig, dt);
         sweep_prec(ctx, psi, sig, dt, precision);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
         sync(ctx);
@@ -465,6 +536,7 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        prepare_materials(ctx, st, dt);
         double *psi = ctx->si_state.as<double>();
         double *cur = static_cast<double *>(ctx->phi64a.reserve(sizeof(double) * g.nphi()));
         double *last = static_cast<double *>(ctx->phi64b.reserve(sizeof(double) * g.nphi()));
@@ -509,6 +581,7 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
         const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
         const double *w = ctx->w64.as<double>();
+        prepare_materials(ctx, st, dt);
         double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
         // vectors: b, x, r (phi-shaped, zero halos)
         DBuf vecs;

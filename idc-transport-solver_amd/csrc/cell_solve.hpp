@@ -194,3 +194,19 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 }  // namespace tsw
+
+namespace tsw {
+
+// Dense matrix-vector product psi = M b with M row-major in LDS (M[r*N + c]).
+template <typename T, int N>
+__device__ __forceinline__ void matvec_lds(const T *__restrict__ M, const T (&b)[N], T (&psi)[N]) {
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        T acc = T(0);
+#pragma unroll
+        for (int c = 0; c < N; ++c) acc += M[r * N + c] * b[c];
+        psi[r] = acc;
+    }
+}
+
+}  // namespace tsw

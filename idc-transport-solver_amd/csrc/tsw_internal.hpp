@@ -75,6 +75,10 @@ template <typename T> struct SweepParams {
     int nx, ny, nq;
     int qo1, qo2, qo3;  // quadrant offsets 1..3
     T dx, dy, inv_dt;
+    // optional material table: per-cell material index and element-matrix inverses [M][nq][N][N]
+    const uint16_t *mat = nullptr;
+    const T *ainv = nullptr;
+    int M = 0;
 };
 
 // ---- launch wrappers (defined in the .hip files) ----
@@ -82,6 +86,12 @@ template <typename T>
 void launch_sweep_wavefront(const SweepParams<T> &sp, int p, hipStream_t s);
 template <typename T>
 void launch_sweep_staggered(const SweepParams<T> &sp, int p, hipStream_t s);
+// bytes of LDS the staggered kernel needs for this problem (0 if it cannot run)
+template <typename T>
+size_t staggered_lds_bytes(const SweepParams<T> &sp, int p);
+template <typename T>
+void launch_material_inverses(const Geom &g, const double *xi, const double *eta, const double *sig_m, int M,
+                              double inv_dt, T *out, hipStream_t s);
 
 template <typename TI, typename TO>
 void launch_pmv(const Geom &g, const TI *psi, TO *phi, const TO *w, TO alpha, TO beta, int all_cells, hipStream_t s);

@@ -300,11 +300,17 @@ def ls_ordinates(order) -> Ordinates:
     return Ordinates(xi, eta, w, off)
 
 
-def random_problem(nx, ny, p, ords: Ordinates, seed, ax=0.0, bx=1.0, ay=0.0, by=1.0, halo_inflow=True):
-    """Random cross sections (sigma_s < sigma_t), random source and inflow (halo) coefficients."""
+def random_problem(nx, ny, p, ords: Ordinates, seed, ax=0.0, bx=1.0, ay=0.0, by=1.0, halo_inflow=True,
+                   materials=None):
+    """Random cross sections (sigma_s < sigma_t), random source and inflow (halo) coefficients.
+    materials=k draws sigma_t per cell from k distinct values (the sweep's material-table path)."""
     rng = np.random.default_rng(seed)
     g = Grid(nx, ny, p, ax, bx, ay, by)
-    sigma_t = rng.uniform(0.3, 4.0, size=(nx + 2, ny + 2))
+    if materials:
+        vals = rng.uniform(0.3, 4.0, size=materials)
+        sigma_t = vals[rng.integers(0, materials, size=(nx + 2, ny + 2))]
+    else:
+        sigma_t = rng.uniform(0.3, 4.0, size=(nx + 2, ny + 2))
     sigma_s = sigma_t * rng.uniform(0.0, 0.9, size=sigma_t.shape)
     src = rng.uniform(-0.5, 1.0, size=g.psi_shape(ords.nq))
     if not halo_inflow:

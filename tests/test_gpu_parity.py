@@ -27,7 +27,7 @@ def setup(ctx, g, o):
     ctx.set_pattern("hip-wavefront")
 
 
-PATTERNS = ["hip-wavefront", "hip-wavefront-eager", "hip-staggered"]
+PATTERNS = ["hip-wavefront", "hip-wavefront-eager", "hip-staggered", "hip-staggered-ge"]
 
 
 @pytest.mark.parametrize("pattern", PATTERNS)
@@ -45,6 +45,23 @@ def test_sweep_fp64_matches_oracle(ctx, pattern, p, dt):
     assert rel_err(out, ref) <= 1e-12
     np.testing.assert_array_equal(out[0], src[0])       # halos untouched
     np.testing.assert_array_equal(out[:, -1], src[:, -1])
+
+
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("materials", [1, 3, 64])
+def test_sweep_material_table_matches_oracle(ctx, p, materials, precision):
+    """Piecewise-constant sigma_t with few distinct values: hip-staggered multiplies by precomputed
+    element-matrix inverses; results must equal the oracle's elimination."""
+    o = ls_ordinates(8)
+    g, st, ss, src = random_problem(11, 13, p, o, seed=70 + p, materials=materials)
+    setup(ctx, g, o)
+    ctx.set_pattern("hip-staggered")
+    for dt in (math.inf, 0.2):
+        ref = orc_sweep(g, o, st, src, dt)
+        out = src.copy()
+        ctx.sweep(out, st, dt, precision=precision)
+        assert rel_err(out, ref) <= (1e-12 if precision == tsw.FP64 else 2e-6)
 
 
 @pytest.mark.parametrize("pattern", PATTERNS)
@@ -129,11 +146,12 @@ def test_source_iteration_matches_oracle(ctx, precision, pattern):
         assert rel_err(out, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("materials", [None, 2])
 @pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
 @pytest.mark.parametrize("p", [1, 2])
-def test_gmres_fp64_matches_oracle(ctx, p, pattern):
+def test_gmres_fp64_matches_oracle(ctx, p, pattern, materials):
     o = ls_ordinates(6)
-    g, st, ss, src = random_problem(10, 9, p, o, seed=40 + p)
+    g, st, ss, src = random_problem(10, 9, p, o, seed=40 + p, materials=materials)
     setup(ctx, g, o)
     ctx.set_pattern(pattern)
     rc, ref_psi, ref_phi, its_ref, res_ref = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
@@ -146,10 +164,11 @@ def test_gmres_fp64_matches_oracle(ctx, p, pattern):
     assert rel_err(out, ref_psi) <= 1e-11
 
 
+@pytest.mark.parametrize("materials", [None, 3])
 @pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
-def test_gmres_mixed_precision_refinement(ctx, pattern):
+def test_gmres_mixed_precision_refinement(ctx, pattern, materials):
     o = ls_ordinates(6)
-    g, st, ss, src = random_problem(12, 12, 2, o, seed=50)
+    g, st, ss, src = random_problem(12, 12, 2, o, seed=50, materials=materials)
     setup(ctx, g, o)
     ctx.set_pattern(pattern)
     rc, ref_psi, ref_phi, _, _ = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
@@ -182,13 +201,16 @@ def test_errors_are_reported(ctx):
     fresh.close()
 
 
-@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
+@pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered", "hip-staggered-ge"])
 def test_full_size_lattice_sweep_properties(ctx, pattern):
     """280x280, Q2, S12 (the lattice workload's sweep): L (L^{-1} q) = q with the GPU Lmv, FP64 and FP32,
     and agreement of the FP32 and FP64 sweeps."""
     import torch
+    from tsw import problems
+    lat = problems.lattice()
     o = ls_ordinates(12)
-    g, st, ss, src = random_problem(280, 280, 2, o, seed=60, bx=7.0, by=7.0)
+    g, _, ss, src = random_problem(280, 280, 2, o, seed=60, bx=7.0, by=7.0)
+    st = lat.sigma_t  # the lattice's two materials (table path for hip-staggered)
     setup(ctx, g, o)
     ctx.set_pattern(pattern)
     d_src = torch.from_numpy(src).cuda()
