@@ -150,7 +150,7 @@ void attach_materials(tsw_ctx *c, SweepParams<T> &sp, double dt) {
         have = true;
     }
     SweepParams<T> trial = sp;
-    trial.mat = m.idx.as<uint16_t>();
+    trial.mat = m.idx.as<int32_t>();
     trial.ainv = buf.as<T>();
     trial.M = m.M;
     if (staggered_lds_bytes<T>(trial, c->g.p) != 0) sp = trial;
@@ -186,7 +186,7 @@ void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
     TSW_HIP(hipStreamSynchronize(c->stream));
     std::unordered_map<double, int> ids;
     std::vector<double> vals;
-    std::vector<uint16_t> idx(h.size(), 0);
+    std::vector<int32_t> idx(h.size(), 0);
     for (int i = 1; i <= g.nx; ++i)
         for (int j = 1; j <= g.ny; ++j) {
             const size_t k = (size_t)i * (g.ny + 2) + j;
@@ -196,11 +196,11 @@ void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
                 it = ids.emplace(h[k], (int)vals.size()).first;
                 vals.push_back(h[k]);
             }
-            idx[k] = (uint16_t)it->second;
+            idx[k] = (int32_t)it->second;
         }
     m.M = (int)vals.size();
     m.inv_dt = inverse_dt(dt);
-    TSW_HIP(hipMemcpyAsync(m.idx.reserve(sizeof(uint16_t) * idx.size()), idx.data(), sizeof(uint16_t) * idx.size(),
+    TSW_HIP(hipMemcpyAsync(m.idx.reserve(sizeof(int32_t) * idx.size()), idx.data(), sizeof(int32_t) * idx.size(),
                            hipMemcpyHostToDevice, c->stream));
     TSW_HIP(hipMemcpyAsync(m.sig.reserve(sizeof(double) * vals.size()), vals.data(), sizeof(double) * vals.size(),
                            hipMemcpyHostToDevice, c->stream));

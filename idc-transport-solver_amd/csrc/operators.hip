@@ -281,6 +281,20 @@ __global__ __launch_bounds__(kBlock) void k_finish_partials(const double *__rest
 
 int reduction_blocks() { return 512; }
 
+size_t max_lds_per_block() {
+    static thread_local int cached_dev = -1;
+    static thread_local size_t cached = 0;
+    int dev = 0;
+    TSW_HIP(hipGetDevice(&dev));
+    if (dev != cached_dev) {
+        int v = 0;
+        TSW_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cached = (size_t)v;
+        cached_dev = dev;
+    }
+    return cached;
+}
+
 void launch_multi_dot(size_t n, int m, const double *X, size_t ld, const double *y, double *out, double *partials,
                       int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_multi_dot_partial, dim3(nblocks, m), dim3(kBlock), 0, s, n, X, ld, y, partials);

@@ -76,10 +76,13 @@ template <typename T> struct SweepParams {
     int qo1, qo2, qo3;  // quadrant offsets 1..3
     T dx, dy, inv_dt;
     // optional material table: per-cell material index and element-matrix inverses [M][nq][N][N]
-    const uint16_t *mat = nullptr;
+    const int32_t *mat = nullptr;
     const T *ainv = nullptr;
     int M = 0;
 };
+
+// Largest dynamic LDS allocation one workgroup may make on the current device (bytes).
+size_t max_lds_per_block();
 
 // ---- launch wrappers (defined in the .hip files) ----
 template <typename T>
