@@ -1,0 +1,189 @@
+// C++ host layer over the libtsw C-ABI; see tsw_host.hpp for the reference interfaces mirrored.
+#include "tsw_host.hpp"
+
+#include <array>
+#include <cmath>
+#include <sstream>
+#include <stdexcept>
+
+namespace tsw_host {
+
+// ---------------------------------------------------------------------------------------------------
+std::string ParameterList::GetString(const std::string &key) const {
+    auto it = values_.find(key);
+    if (it == values_.end()) throw std::out_of_range("parameter '" + key + "' not present");
+    return it->second;
+}
+
+ParameterList ParameterList::FromText(const std::string &text) {
+    ParameterList p;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        std::istringstream ls(line);
+        std::string key, value;
+        if (ls >> key >> value) p.Set(key, value);
+    }
+    return p;
+}
+
+// ---------------------------------------------------------------------------------------------------
+OrdinateSet OrdinateSet::LevelSymmetric(int order) {
+    if (order <= 0 || order % 2) throw std::invalid_argument("level-symmetric order must be even and positive");
+    const int nq = order * (order + 2) / 2;
+    std::vector<double> xi(nq), eta(nq), w(nq);
+    std::array<int, 5> off{};
+    if (tsw_ordinates_level_symmetric(order, xi.data(), eta.data(), w.data(), off.data()) != TSW_OK)
+        throw std::invalid_argument("no level-symmetric set of order " + std::to_string(order));
+    return OrdinateSet(xi, eta, w, off);
+}
+
+OrdinateSet::OrdinateSet(std::vector<double> xi, std::vector<double> eta, std::vector<double> w,
+                         std::array<int, 5> quadrants)
+    : xi_(std::move(xi)), eta_(std::move(eta)), w_(std::move(w)), quad_(quadrants) {
+    if (xi_.size() != eta_.size() || xi_.size() != w_.size() || quad_[0] != 0 || quad_[4] != (int)xi_.size())
+        throw std::invalid_argument("inconsistent ordinate set");
+}
+
+// ---------------------------------------------------------------------------------------------------
+OrdinateFlux::OrdinateFlux(const DomainDecomposition &dd, const OrdinateSet &os, int degree)
+    : DG_degree(degree), nx_(dd.nx), ny_(dd.ny), nq_(os.nq()),
+      density_((size_t)(dd.nx + 2) * (dd.ny + 2) * os.nq() * (degree + 1) * (degree + 1), 0.0) {}
+
+double &OrdinateFlux::operator()(int64_t q, int64_t i, int64_t j, int64_t d, int64_t e) {
+    const int64_t P = DG_degree + 1;
+    return density_[(size_t)(e + P * (d + P * (q + nq_ * (j + (ny_ + 2) * i))))];
+}
+double OrdinateFlux::operator()(int64_t q, int64_t i, int64_t j, int64_t d, int64_t e) const {
+    return const_cast<OrdinateFlux *>(this)->operator()(q, i, j, d, e);
+}
+
+ScalarFlux::ScalarFlux(const DomainDecomposition &dd, int degree)
+    : DG_degree(degree), ny_(dd.ny), density_((size_t)(dd.nx + 2) * (dd.ny + 2) * (degree + 1) * (degree + 1), 0.0) {}
+
+double &ScalarFlux::operator()(int64_t i, int64_t j, int64_t d, int64_t e) {
+    const int64_t P = DG_degree + 1;
+    return density_[(size_t)(e + P * (d + P * (j + (ny_ + 2) * i)))];
+}
+double ScalarFlux::operator()(int64_t i, int64_t j, int64_t d, int64_t e) const {
+    return const_cast<ScalarFlux *>(this)->operator()(i, j, d, e);
+}
+
+CrossSection::CrossSection(const DomainDecomposition &dd, double value)
+    : ny_(dd.ny), v_((size_t)(dd.nx + 2) * (dd.ny + 2), value) {}
+
+// ---------------------------------------------------------------------------------------------------
+Device::Device(const DomainDecomposition &dd, const OrdinateSet &os, int degree, int device) {
+    if (tsw_create(device, &ctx_) != TSW_OK) throw std::runtime_error("tsw_create failed: no usable HIP device");
+    try {
+        const auto &off = os.quadrant_offsets();
+        Check(tsw_problem_set(ctx_, (int)dd.nx, (int)dd.ny, dd.ax, dd.bx, dd.ay, dd.by, degree, (int)os.nq(),
+                              os.xi().data(), os.eta().data(), os.w().data(), off.data()),
+              "tsw_problem_set");
+    } catch (...) {
+        tsw_destroy(ctx_);
+        throw;
+    }
+}
+
+Device::~Device() { tsw_destroy(ctx_); }
+
+void Device::Check(int status, const char *what) const {
+    if (status == TSW_OK) return;
+    const std::string msg = std::string(what) + ": " + tsw_last_error(ctx_);
+    if (status == TSW_E_ARG) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+namespace {
+int precision_from(const ParameterList &p) {
+    if (!p.Has("sweep_precision")) return TSW_FP64;
+    const std::string v = p.GetString("sweep_precision");
+    if (v == "fp64") return TSW_FP64;
+    if (v == "fp32") return TSW_FP32;
+    throw std::invalid_argument("sweep_precision must be fp64 or fp32, not '" + v + "'");
+}
+int all_cells(OpDomain d) { return ((int)d & (int)OpDomain::Boundary) ? 1 : 0; }
+}  // namespace
+
+SweepOperator::SweepOperator(std::shared_ptr<Device> dev, const ParameterList &params, double)
+    : dev_(std::move(dev)), precision_(precision_from(params)) {
+    const std::string pattern = params.Has("sweep_pattern") ? params.GetString("sweep_pattern") : "hip-staggered";
+    const int rc = tsw_set_pattern(dev_->ctx(), pattern.c_str());
+    if (rc == TSW_E_ARG) throw std::out_of_range("no sweep_pattern '" + pattern + "'");
+    dev_->Check(rc, "tsw_set_pattern");
+}
+
+void SweepOperator::Apply(OrdinateFlux &source, const CrossSection &sigma, double dt, const OrdinateFlux *) const {
+    dev_->Check(tsw_sweep_apply(dev_->ctx(), source.data(), sigma.data(), dt, precision_, TSW_MEM_HOST),
+                "SweepOperator::Apply");
+}
+
+namespace TransportOperator {
+void Pmv(Device &dev, double alpha, double beta, const OrdinateFlux &x, ScalarFlux &y, OpDomain d) {
+    dev.Check(tsw_pmv(dev.ctx(), alpha, beta, x.data(), y.data(), all_cells(d), TSW_MEM_HOST), "Pmv");
+}
+void Smv(Device &dev, double alpha, double beta, const CrossSection &sigma, const ScalarFlux &x, OrdinateFlux &y,
+         OpDomain d) {
+    dev.Check(tsw_smv(dev.ctx(), alpha, beta, sigma.data(), x.data(), y.data(), all_cells(d), TSW_MEM_HOST), "Smv");
+}
+void Lmv(Device &dev, double alpha, double beta, const CrossSection &sigma, const OrdinateFlux &x, OrdinateFlux &y) {
+    dev.Check(tsw_lmv(dev.ctx(), alpha, beta, sigma.data(), x.data(), y.data(), TSW_MEM_HOST), "Lmv");
+}
+}  // namespace TransportOperator
+
+// ---------------------------------------------------------------------------------------------------
+std::unique_ptr<ImplicitSolver> ImplicitSolver::Create(std::shared_ptr<Device> dev, const ParameterList &params,
+                                                       double) {
+    const std::string key = params.GetString("solve_type");
+    if (params.Has("sweep_pattern")) {
+        const std::string pattern = params.GetString("sweep_pattern");
+        const int rc = tsw_set_pattern(dev->ctx(), pattern.c_str());
+        if (rc == TSW_E_ARG) throw std::out_of_range("no sweep_pattern '" + pattern + "'");
+        dev->Check(rc, "tsw_set_pattern");
+    }
+    if (key == "source-iteration") return std::unique_ptr<ImplicitSolver>(new SISolver(dev, params));
+    if (key == "peierls-gmres" || key == "petsc-peierls-gmres")
+        return std::unique_ptr<ImplicitSolver>(new PeierlsGMRESSolver(dev, params));
+    throw std::out_of_range("Failed to access factory for ImplicitSolver '" + key + "'");
+}
+
+SISolver::SISolver(std::shared_ptr<Device> dev, const ParameterList &p)
+    : dev_(std::move(dev)), precision_(precision_from(p)) {
+    if (p.Has("abs_tol")) abs_tol_ = p.GetDouble("abs_tol");
+    if (p.Has("max_its")) max_its_ = p.GetInt("max_its");
+}
+
+void SISolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s, double dt,
+                     const OrdinateFlux *) {
+    int it = 0;
+    double err = 0.0;
+    dev_->Check(tsw_si_solve(dev_->ctx(), source.data(), sigma_t.data(), sigma_s.data(), dt, abs_tol_, (int)max_its_,
+                             precision_, TSW_MEM_HOST, &it, &err),
+                "SISolver::Solve");
+    iterations_ = it;
+    final_error_ = err;
+}
+
+PeierlsGMRESSolver::PeierlsGMRESSolver(std::shared_ptr<Device> dev, const ParameterList &p)
+    : dev_(std::move(dev)), precision_(precision_from(p)) {
+    if (p.Has("abs_tol")) abs_tol_ = p.GetDouble("abs_tol");
+    if (p.Has("max_its")) max_its_ = p.GetInt("max_its");
+    if (p.Has("gmres_restart")) restart_ = p.GetInt("gmres_restart");
+}
+
+void PeierlsGMRESSolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s,
+                               double dt, const OrdinateFlux *) {
+    int it = 0;
+    double res = 0.0;
+    const int rc = tsw_gmres_solve(dev_->ctx(), source.data(), sigma_t.data(), sigma_s.data(), dt, abs_tol_,
+                                   (int)max_its_, (int)restart_, precision_, TSW_MEM_HOST, nullptr, &it, &res);
+    iterations_ = it;
+    final_error_ = res;
+    if (rc == TSW_E_NOCONV) throw std::runtime_error("GMRES transport solve failed to converge");
+    dev_->Check(rc, "PeierlsGMRESSolver::Solve");
+}
+
+}  // namespace tsw_host

@@ -1,0 +1,213 @@
+// C++ host layer over the libtsw C-ABI, mirroring the reference's operator and solver interface for
+// the sweep path (crockatt/idc-transport-solver):
+//   RKDG::OrdinateFlux / ScalarFlux / CrossSection   src/objects/RKDG/*.hpp (layouts, operator())
+//   Quadrule::OrdinateSet                             src/utils/Quadrule/OrdinateSet.hpp:236-418
+//   DomainDecomposition                               src/objects/DomainDecomposition.hpp
+//   ParameterList                                     src/utils/ParameterList.cpp:202-290 (key value)
+//   SweepOperator::Apply                              src/linear_solvers/Abstract/SweepOperator.hpp:126-131
+//   TransportOperator::Pmv / Smv / Lmv                src/operators/RKDG/TransportOperator.hpp:25-43
+//   ImplicitSolver + factory key solve_type           src/linear_solvers/Abstract/ImplicitSolver.hpp:117-123,
+//                                                     ImplicitSolver.cpp:182-229
+// Arrays live on the host in the reference's layouts; every call hands them to the GPU library.
+// Errors follow the reference: std::invalid_argument for bad arguments, std::out_of_range for unknown
+// factory keys, std::runtime_error for failed solves and device errors.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <limits>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tsw.h"
+
+namespace tsw_host {
+
+// Plain "key value" parameters (ParameterList.cpp:202-290).
+class ParameterList {
+  public:
+    void Set(const std::string &key, const std::string &value) { values_[key] = value; }
+    bool Has(const std::string &key) const { return values_.count(key) != 0; }
+    std::string GetString(const std::string &key) const;
+    double GetDouble(const std::string &key) const { return std::stod(GetString(key)); }
+    int64_t GetInt(const std::string &key) const { return std::stoll(GetString(key)); }
+    // Reads "key value" lines; '#' starts a comment.
+    static ParameterList FromText(const std::string &text);
+
+  private:
+    std::map<std::string, std::string> values_;
+};
+
+// Uniform Cartesian mesh [ax,bx] x [ay,by] with nx x ny cells.
+struct DomainDecomposition {
+    int64_t nx = 0, ny = 0;
+    double ax = 0, bx = 1, ay = 0, by = 1;
+    double dx(int dim) const { return dim == 0 ? (bx - ax) / nx : (by - ay) / ny; }
+};
+
+// Ordinates grouped by quadrant (0:++ 1:+- 2:-+ 3:--), weights summing to 4 pi.
+class OrdinateSet {
+  public:
+    static OrdinateSet LevelSymmetric(int order);
+    OrdinateSet(std::vector<double> xi, std::vector<double> eta, std::vector<double> w,
+                std::array<int, 5> quadrants);
+    int64_t nq() const { return (int64_t)xi_.size(); }
+    double xi(int64_t q) const { return xi_[q]; }
+    double eta(int64_t q) const { return eta_[q]; }
+    double w(int64_t q) const { return w_[q]; }
+    int64_t Quadrants(int k) const { return quad_[k]; }
+    const std::vector<double> &xi() const { return xi_; }
+    const std::vector<double> &eta() const { return eta_; }
+    const std::vector<double> &w() const { return w_; }
+    const std::array<int, 5> &quadrant_offsets() const { return quad_; }
+
+  private:
+    std::vector<double> xi_, eta_, w_;
+    std::array<int, 5> quad_;
+};
+
+// Angular flux, density[((i*(ny+2)+j)*nq+q)*(p+1)^2 + d*(p+1) + e] (OrdinateFlux.hpp:639-700).
+class OrdinateFlux {
+  public:
+    OrdinateFlux(const DomainDecomposition &dd, const OrdinateSet &os, int DG_degree);
+    double &operator()(int64_t q, int64_t i, int64_t j, int64_t d, int64_t e);
+    double operator()(int64_t q, int64_t i, int64_t j, int64_t d, int64_t e) const;
+    double *data() { return density_.data(); }
+    const double *data() const { return density_.data(); }
+    size_t size() const { return density_.size(); }
+    int64_t nq() const { return nq_; }
+    const int DG_degree;
+
+  private:
+    int64_t nx_, ny_, nq_;
+    std::vector<double> density_;
+};
+
+// Scalar flux / density, [i][j][d][e] with halo (DensityFunction.hpp:475-535).
+class ScalarFlux {
+  public:
+    ScalarFlux(const DomainDecomposition &dd, int DG_degree);
+    double &operator()(int64_t i, int64_t j, int64_t d, int64_t e);
+    double operator()(int64_t i, int64_t j, int64_t d, int64_t e) const;
+    double *data() { return density_.data(); }
+    const double *data() const { return density_.data(); }
+    size_t size() const { return density_.size(); }
+    const int DG_degree;
+
+  private:
+    int64_t ny_;
+    std::vector<double> density_;
+};
+
+// Cross section with DG degree 0: one value per cell, [i][j] with halo (CrossSection.hpp:397-478).
+class CrossSection {
+  public:
+    explicit CrossSection(const DomainDecomposition &dd, double value = 0.0);
+    double &operator()(int64_t i, int64_t j) { return v_[(size_t)i * (ny_ + 2) + j]; }
+    double operator()(int64_t i, int64_t j) const { return v_[(size_t)i * (ny_ + 2) + j]; }
+    const double *data() const { return v_.data(); }
+    const int DG_degree = 0;
+
+  private:
+    int64_t ny_;
+    std::vector<double> v_;
+};
+
+// Where an operator applies (DomainDecomposition.hpp:35-62).  Single-rank meshes: every halo cell is
+// a boundary cell.
+enum class OpDomain { Interior = 0x01, Boundary = 0x02, Halo = 0x04, All = 0x07 };
+
+// One device context shared by the operators and solvers built on a mesh and ordinate set.
+class Device {
+  public:
+    Device(const DomainDecomposition &dd, const OrdinateSet &os, int DG_degree, int device = 0);
+    ~Device();
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    tsw_ctx *ctx() const { return ctx_; }
+    void Check(int status, const char *what) const;  // throws on status != TSW_OK
+
+  private:
+    tsw_ctx *ctx_ = nullptr;
+};
+
+// In-place sweep psi <- L^{-1} psi, halos hold the inflow (SweepOperator.cpp:118-135).
+// Parameters: sweep_pattern ("hip-staggered" default, "hip-wavefront", ...), sweep_precision
+// ("fp64" default, "fp32").
+class SweepOperator {
+  public:
+    SweepOperator(std::shared_ptr<Device> dev, const ParameterList &params,
+                  double dt = std::numeric_limits<double>::infinity());
+    void Apply(OrdinateFlux &source, const CrossSection &sigma, double dt = std::numeric_limits<double>::infinity(),
+               const OrdinateFlux *initial_condition = nullptr) const;
+
+  private:
+    std::shared_ptr<Device> dev_;
+    int precision_ = TSW_FP64;
+};
+
+namespace TransportOperator {
+void Pmv(Device &dev, double alpha, double beta, const OrdinateFlux &x, ScalarFlux &y,
+         OpDomain op_domain = OpDomain::All);
+void Smv(Device &dev, double alpha, double beta, const CrossSection &sigma, const ScalarFlux &x, OrdinateFlux &y,
+         OpDomain op_domain = OpDomain::All);
+void Lmv(Device &dev, double alpha, double beta, const CrossSection &sigma, const OrdinateFlux &x, OrdinateFlux &y);
+}  // namespace TransportOperator
+
+// ImplicitSolver with the reference's Solve contract: source holds Q on entry and psi on exit.
+class ImplicitSolver {
+  public:
+    virtual ~ImplicitSolver() = default;
+    virtual void Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s,
+                       double dt = std::numeric_limits<double>::infinity(),
+                       const OrdinateFlux *initial = nullptr) = 0;
+    virtual std::string GetDescriptor() const = 0;
+    int64_t iterations() const { return iterations_; }
+    double final_error() const { return final_error_; }
+
+    // Factory on the "solve_type" key (ImplicitSolver.cpp:182-229): "source-iteration",
+    // "peierls-gmres" (also accepted under the reference's "petsc-peierls-gmres").
+    static std::unique_ptr<ImplicitSolver> Create(std::shared_ptr<Device> dev, const ParameterList &params,
+                                                  double dt = std::numeric_limits<double>::infinity());
+
+  protected:
+    int64_t iterations_ = 0;
+    double final_error_ = 0.0;
+};
+
+// Source iteration (SISolver.cpp:190-254); parameters abs_tol (1e-8), max_its (500), sweep_precision.
+class SISolver : public ImplicitSolver {
+  public:
+    SISolver(std::shared_ptr<Device> dev, const ParameterList &params);
+    void Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s, double dt,
+               const OrdinateFlux *initial) override;
+    std::string GetDescriptor() const override { return "source-iteration"; }
+
+  private:
+    std::shared_ptr<Device> dev_;
+    double abs_tol_ = 1e-8;
+    int64_t max_its_ = 500;
+    int precision_ = TSW_FP64;
+};
+
+// Sweep-preconditioned GMRES on the Peierls form (PETScPeierlsSolver.cpp:423-572); parameters
+// abs_tol (1e-8), max_its (3000), gmres_restart (30), sweep_precision ("fp32" selects FP32 sweeps
+// with FP64 refinement).
+class PeierlsGMRESSolver : public ImplicitSolver {
+  public:
+    PeierlsGMRESSolver(std::shared_ptr<Device> dev, const ParameterList &params);
+    void Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s, double dt,
+               const OrdinateFlux *initial) override;
+    std::string GetDescriptor() const override { return "peierls-gmres"; }
+
+  private:
+    std::shared_ptr<Device> dev_;
+    double abs_tol_ = 1e-8;
+    int64_t max_its_ = 3000;
+    int64_t restart_ = 30;
+    int precision_ = TSW_FP64;
+};
+
+}  // namespace tsw_host
