@@ -13,6 +13,8 @@ using namespace tsw;
 
 enum class Pattern { WavefrontGraph, WavefrontEager, Staggered, StaggeredGE };
 
+static bool is_staggered(Pattern p) { return p == Pattern::Staggered || p == Pattern::StaggeredGE; }
+
 struct tsw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -45,6 +47,15 @@ struct tsw_ctx {
         bool have64 = false, have32 = false;
     } mat;
     static constexpr int kMaxMaterials = 64;
+
+    // Inputs and work arrays of the staggered (diagonal-layout) sweep, per precision where needed:
+    // cross sections and material index per quadrant, inflow traces, the source and psi per ordinate,
+    // the scalar flux per quadrant, the quadrant partial moments and the refinement iterate.
+    struct Diag {
+        DBuf sigt64, sigt32, sigs64, sigs32, mat, tinx64, tiny64, tinx32, tiny32, q64, q32, psi64, psi32, res32,
+            phid64, phid32, phiQ, ir64;
+        bool inflow = false;
+    } diag;
 
     // captured sweep graphs keyed by (precision, psi, sigma, inv_dt)
     std::map<std::tuple<int, const void *, const void *, double>, hipGraphExec_t> graphs;
@@ -135,32 +146,10 @@ template <typename T>
 SweepParams<T> params(tsw_ctx *c, T *psi, const T *sigma, double dt);
 double inverse_dt(double dt) { return (dt > 0.0 && std::isfinite(dt)) ? 1.0 / dt : 0.0; }
 
-// Attaches the material table to a sweep when the pattern uses it and it was built for this dt.
-template <typename T>
-void attach_materials(tsw_ctx *c, SweepParams<T> &sp, double dt) {
-    auto &m = c->mat;
-    if (c->pattern != Pattern::Staggered || !m.valid || m.inv_dt != inverse_dt(dt)) return;
-    const size_t n = (size_t)m.M * c->g.nq * c->g.N() * c->g.N();
-    bool &have = sizeof(T) == 8 ? m.have64 : m.have32;
-    DBuf &buf = sizeof(T) == 8 ? m.inv64 : m.inv32;
-    if (!have) {
-        T *out = static_cast<T *>(buf.reserve(n * sizeof(T)));
-        launch_material_inverses<T>(c->g, c->xi64.as<double>(), c->eta64.as<double>(), m.sig.as<double>(), m.M,
-                                    m.inv_dt, out, c->stream);
-        have = true;
-    }
-    SweepParams<T> trial = sp;
-    trial.mat = m.idx.as<int32_t>();
-    trial.ainv = buf.as<T>();
-    trial.M = m.M;
-    if (staggered_lds_bytes<T>(trial, c->g.p) != 0) sp = trial;
-}
-
 template <>
 SweepParams<double> params(tsw_ctx *c, double *psi, const double *sigma, double dt) {
     SweepParams<double> sp{psi, sigma, c->xi64.as<double>(), c->eta64.as<double>(), c->g.nx, c->g.ny, c->g.nq,
                            c->g.quad_off[1], c->g.quad_off[2], c->g.quad_off[3], c->g.dx, c->g.dy, inverse_dt(dt)};
-    attach_materials<double>(c, sp, dt);
     return sp;
 }
 template <>
@@ -168,7 +157,6 @@ SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt)
     SweepParams<float> sp{psi, sigma, c->xi32.as<float>(), c->eta32.as<float>(), c->g.nx, c->g.ny, c->g.nq,
                           c->g.quad_off[1], c->g.quad_off[2], c->g.quad_off[3], (float)c->g.dx, (float)c->g.dy,
                           (float)inverse_dt(dt)};
-    attach_materials<float>(c, sp, dt);
     return sp;
 }
 
@@ -238,9 +226,8 @@ void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
             break;
         }
         case Pattern::Staggered:
-        case Pattern::StaggeredGE:
-            if (staggered_lds_bytes<T>(sp, c->g.p) != 0) launch_sweep_staggered<T>(sp, c->g.p, c->stream);
-            else launch_sweep_wavefront<T>(sp, c->g.p, c->stream);
+        case Pattern::StaggeredGE:  // meshes the staggered kernel cannot hold
+            launch_sweep_wavefront<T>(sp, c->g.p, c->stream);
             break;
     }
     TSW_HIP(hipEventRecord(e1, c->stream));
@@ -384,6 +371,269 @@ struct Gmres {
     }
 };
 
+
+// ---------------------------------------------------------------------------------------------------
+// Staggered sweep engine: every solver of the "hip-staggered" patterns runs on the diagonal-major
+// layout.  Reference-layout arrays are converted once on entry and once on exit of a call.
+// ---------------------------------------------------------------------------------------------------
+template <typename T>
+T *dbuf(DBuf &b, size_t n) {
+    return static_cast<T *>(b.reserve(n * sizeof(T)));
+}
+
+// Builds the per-call inputs: materials or per-cell sigma_t, sigma_s, inflow traces and the source
+// from reference-layout arrays (sigma_s, qref may be null).  need64/need32 select the precisions.
+void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const double *qref, double dt, bool need64,
+                bool need32) {
+    const Geom &g = c->g;
+    auto &d = c->diag;
+    const size_t ncell = (size_t)g.nx * g.ny, N = g.N();
+    hipStream_t s = c->stream;
+    prepare_materials(c, sigma_t, dt);
+    if (c->mat.valid)
+        launch_cell_to_diag<int32_t, int32_t>(g, c->mat.idx.as<int32_t>(), dbuf<int32_t>(d.mat, 4 * ncell), 1, s);
+    if (need64) launch_cell_to_diag<double, double>(g, sigma_t, dbuf<double>(d.sigt64, 4 * ncell), 1, s);
+    if (need32) launch_cell_to_diag<double, float>(g, sigma_t, dbuf<float>(d.sigt32, 4 * ncell), 1, s);
+    if (sigma_s) {
+        if (need64) launch_cell_to_diag<double, double>(g, sigma_s, dbuf<double>(d.sigs64, 4 * ncell), 1, s);
+        if (need32) launch_cell_to_diag<double, float>(g, sigma_s, dbuf<float>(d.sigs32, 4 * ncell), 1, s);
+    }
+    d.inflow = qref != nullptr;
+    if (qref) {
+        const size_t nt_x = (size_t)g.nq * g.ny * (g.p + 1), nt_y = (size_t)g.nq * g.nx * (g.p + 1);
+        if (need64) {
+            launch_inflow<double>(g, qref, dbuf<double>(d.tinx64, nt_x), dbuf<double>(d.tiny64, nt_y), s);
+            launch_psi_convert<double, double>(g, const_cast<double *>(qref), dbuf<double>(d.q64, g.nq * N * ncell), 0, s);
+        }
+        if (need32) {
+            launch_inflow<float>(g, qref, dbuf<float>(d.tinx32, nt_x), dbuf<float>(d.tiny32, nt_y), s);
+            launch_psi_convert<double, float>(g, const_cast<double *>(qref), dbuf<float>(d.q32, g.nq * N * ncell), 0, s);
+        }
+    }
+}
+
+// Element-inverse table for precision T (null when the per-cell elimination is used).
+template <typename T>
+const T *diag_inverses(tsw_ctx *c, double dt) {
+    auto &m = c->mat;
+    if (c->pattern != Pattern::Staggered || !m.valid || m.inv_dt != inverse_dt(dt)) return nullptr;
+    const size_t n = (size_t)m.M * c->g.nq * c->g.N() * c->g.N();
+    bool &have = sizeof(T) == 8 ? m.have64 : m.have32;
+    DBuf &buf = sizeof(T) == 8 ? m.inv64 : m.inv32;
+    if (!have) {
+        launch_material_inverses<T>(c->g, c->xi64.as<double>(), c->eta64.as<double>(), m.sig.as<double>(), m.M,
+                                    m.inv_dt, static_cast<T *>(buf.reserve(n * sizeof(T))), c->stream);
+        have = true;
+    }
+    return buf.as<T>();
+}
+
+// One staggered sweep: psi_out <- L^{-1}(qsrc + sigma_s phi_d / (4 pi)), inflow from the traces set up
+// by diag_setup when `inflow`, else vacuum.  qsrc or phi_d may be null (not both); qsrc may equal
+// psi_out.
+template <typename T>
+void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow, T *psi_out) {
+    const Geom &g = c->g;
+    auto &d = c->diag;
+    const bool f64 = sizeof(T) == 8;
+    DiagSweepParams<T> sp{};
+    sp.xi = f64 ? (const T *)c->xi64.ptr : (const T *)c->xi32.ptr;
+    sp.eta = f64 ? (const T *)c->eta64.ptr : (const T *)c->eta32.ptr;
+    sp.nx = g.nx; sp.ny = g.ny; sp.nq = g.nq;
+    sp.qo1 = g.quad_off[1]; sp.qo2 = g.quad_off[2]; sp.qo3 = g.quad_off[3];
+    sp.dx = T(g.dx); sp.dy = T(g.dy); sp.inv_dt = T(inverse_dt(dt));
+    sp.qsrc = qsrc;
+    sp.phi = phi_d;
+    sp.sig_s = phi_d ? (const T *)(f64 ? d.sigs64.ptr : d.sigs32.ptr) : nullptr;
+    if (inflow) {
+        require(d.inflow, TSW_E_STATE, "staggered sweep asked for inflow traces that were not set up");
+        sp.tin_x = (const T *)(f64 ? d.tinx64.ptr : d.tinx32.ptr);
+        sp.tin_y = (const T *)(f64 ? d.tiny64.ptr : d.tiny32.ptr);
+    }
+    sp.ainv = diag_inverses<T>(c, dt);
+    if (sp.ainv) {
+        sp.mat = d.mat.as<int32_t>();
+        sp.M = c->mat.M;
+        if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) sp.ainv = nullptr;  // table too large: eliminate per cell
+    }
+    if (!sp.ainv) sp.sig_t = (const T *)(f64 ? d.sigt64.ptr : d.sigt32.ptr);
+    sp.psi = psi_out;
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, c->stream));
+    launch_diag_sweep<T>(sp, g.p, c->stream);
+    TSW_HIP(hipEventRecord(e1, c->stream));
+    ++c->sweeps_done;
+}
+
+// phi_d[Q] (precision T) from a phi-layout FP64 array
+template <typename T>
+T *diag_phi(tsw_ctx *c, const double *phi) {
+    const Geom &g = c->g;
+    auto &d = c->diag;
+    const size_t n = 4 * (size_t)g.nx * g.ny * g.N();
+    T *out = dbuf<T>(sizeof(T) == 8 ? d.phid64 : d.phid32, n);
+    launch_cell_to_diag<double, T>(g, phi, out, g.N(), c->stream);
+    return out;
+}
+
+// phi = beta phi + alpha P psi_d
+template <typename T>
+void diag_pmv(tsw_ctx *c, const T *psi_d, double *phi, double alpha, double beta) {
+    const Geom &g = c->g;
+    double *phiQ = dbuf<double>(c->diag.phiQ, 4 * (size_t)g.nx * g.ny * g.N());
+    launch_pmv_diag<T>(g, psi_d, c->w64.as<double>(), phiQ, phi, alpha, beta, c->stream);
+}
+
+// out (reference layout, interior) <- psi_d; halos of out untouched
+template <typename T>
+void diag_store(tsw_ctx *c, const T *psi_d, double *out_ref) {
+    launch_psi_convert<double, T>(c->g, out_ref, const_cast<T *>(psi_d), 1, c->stream);
+}
+
+// True when the pattern is a staggered one and the mesh fits its kernel (else the wavefront kernels run).
+bool staggered_ok(tsw_ctx *c) {
+    if (!is_staggered(c->pattern)) return false;
+    DiagSweepParams<double> probe{};
+    probe.nx = c->g.nx;
+    probe.ny = c->g.ny;
+    return diag_sweep_lds_bytes<double>(probe, c->g.p) != 0;
+}
+
+// In-place sweep of a reference-layout device array in precision T.
+template <typename T>
+void apply_staggered(tsw_ctx *c, double *psi_ref, const double *sigma_t, double dt) {
+    const bool f64 = sizeof(T) == 8;
+    diag_setup(c, sigma_t, nullptr, psi_ref, dt, f64, !f64);
+    T *q = (T *)(f64 ? c->diag.q64.ptr : c->diag.q32.ptr);
+    diag_sweep<T>(c, dt, q, nullptr, true, q);
+    diag_store<T>(c, q, psi_ref);
+}
+
+
+// Source iteration on the staggered engine (SISolver.cpp:206-249).  Q: reference-layout source with
+// inflow halos (device).  The solver's angular flux state (reference layout) seeds phi and receives
+// the last iterate.  Returns iterations and the last change.
+template <typename T>
+void si_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, double dt, double abs_tol, int max_its,
+                  int &iters, double &err) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    const bool f64 = sizeof(T) == 8;
+    diag_setup(c, st, ss, Q, dt, f64, !f64);
+    double *psi_state = c->si_state.as<double>();
+    double *cur = dbuf<double>(c->phi64a, g.nphi());
+    double *last = dbuf<double>(c->phi64b, g.nphi());
+    TSW_HIP(hipMemsetAsync(cur, 0, sizeof(double) * g.nphi(), s));
+    TSW_HIP(hipMemsetAsync(last, 0, sizeof(double) * g.nphi(), s));
+    launch_pmv<double, double>(g, psi_state, cur, c->w64.as<double>(), 1.0, 0.0, 0, s);  // :214
+    const size_t npsi_d = (size_t)g.nq * g.N() * g.nx * g.ny;
+    T *psi_d = dbuf<T>(f64 ? c->diag.psi64 : c->diag.psi32, npsi_d);
+    const T *q_d = (const T *)(f64 ? c->diag.q64.ptr : c->diag.q32.ptr);
+    int it = 0;
+    double e = 1.79769313486231570e+308;
+    while (it < max_its && e > abs_tol) {
+        ++it;
+        std::swap(cur, last);
+        const T *phi_d = diag_phi<T>(c, last);
+        diag_sweep<T>(c, dt, q_d, phi_d, true, psi_d);   // L^{-1}(Q + S phi), :226-232
+        diag_pmv<T>(c, psi_d, cur, 1.0, 0.0);            // :236
+        e = device_norm2_interior_diff(c, cur, last);    // :239
+    }
+    if (it > 0) {
+        TSW_HIP(hipMemcpyAsync(psi_state, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+        diag_store<T>(c, psi_d, psi_state);
+        TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    } else {
+        TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    }
+    iters = it;
+    err = e;
+}
+
+// Peierls GMRES on the staggered engine.  FP64: sweeps in double.  FP32: an FP64 iterate psi64 (diagonal
+// layout) refined with corrections whose sweeps run in float; the residual r = Q + S P psi - (L + 1/dt) psi
+// is formed in FP64.  Q: reference layout with inflow halos, overwritten by psi; phi (phi layout) receives
+// the scalar flux.  Returns convergence.
+bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, double dt, double abs_tol,
+                     int max_its, int restart, int precision, double *phi_out, int &its_total, double &res_final) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t n = g.nphi();
+    const size_t npsi_d = (size_t)g.nq * g.N() * g.nx * g.ny;
+    DBuf vecs;
+    double *b = dbuf<double>(vecs, 3 * n);
+    double *x = b + n, *r = x + n;
+    TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * 3 * n, s));
+    Gmres solver(c, n, restart);
+    bool converged = false;
+    its_total = 0;
+    if (precision == TSW_FP64) {
+        diag_setup(c, st, ss, Q, dt, true, false);
+        double *psi_d = dbuf<double>(c->diag.psi64, npsi_d);
+        const double *q_d = c->diag.q64.as<double>();
+        auto apply = [&](const double *xv, double *yv) {  // (I - P L^{-1} S) x, zero inflow (:548-555)
+            const double *phi_d = diag_phi<double>(c, xv);
+            diag_sweep<double>(c, dt, nullptr, phi_d, false, psi_d);
+            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            diag_pmv<double>(c, psi_d, yv, -1.0, 1.0);
+        };
+        diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);  // b = P L^{-1} Q (:461-466)
+        diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
+        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
+        const double *phi_d = diag_phi<double>(c, x);            // psi = L^{-1}(Q + S phi) (:504-505)
+        diag_sweep<double>(c, dt, q_d, phi_d, true, psi_d);
+        diag_store<double>(c, psi_d, Q);
+    } else {
+        diag_setup(c, st, ss, Q, dt, true, true);
+        double *psi64 = dbuf<double>(c->diag.ir64, npsi_d);
+        float *res32 = dbuf<float>(c->diag.res32, npsi_d);
+        float *psi32 = dbuf<float>(c->diag.psi32, npsi_d);
+        const double *q64 = c->diag.q64.as<double>();
+        TSW_HIP(hipMemsetAsync(psi64, 0, sizeof(double) * npsi_d, s));
+        auto apply32 = [&](const double *xv, double *yv) {
+            const float *phi_d = diag_phi<float>(c, xv);
+            diag_sweep<float>(c, dt, nullptr, phi_d, false, psi32);
+            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            diag_pmv<float>(c, psi32, yv, -1.0, 1.0);
+        };
+        const int max_refine = 30;
+        for (int refine = 0;; ++refine) {
+            // r = Q + S P psi - (L + 1/dt) psi (FP64), stored in float
+            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            diag_pmv<double>(c, psi64, x, 1.0, 0.0);
+            const double *phi64_d = diag_phi<double>(c, x);
+            launch_residual_diag<float>(g, q64, phi64_d, c->diag.sigs64.as<double>(), c->diag.sigt64.as<double>(),
+                                        psi64, c->diag.tinx64.as<double>(), c->diag.tiny64.as<double>(),
+                                        c->xi64.as<double>(), c->eta64.as<double>(), inverse_dt(dt), res32, s);
+            // b = P L^{-1} r, the Peierls residual of the iterate
+            diag_sweep<float>(c, dt, res32, nullptr, false, psi32);
+            TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
+            diag_pmv<float>(c, psi32, b, 1.0, 0.0);
+            res_final = solver.norm(b);
+            if (res_final <= abs_tol) {
+                converged = true;
+                break;
+            }
+            if (refine >= max_refine || its_total >= max_its) break;
+            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
+            int its = 0;
+            double rr = 0.0;
+            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
+            its_total += its;
+            // d = L^{-1}(r + S phi_d); psi += d
+            const float *phi_d = diag_phi<float>(c, x);
+            diag_sweep<float>(c, dt, res32, phi_d, false, psi32);
+            launch_accumulate_f2d(psi32, psi64, npsi_d, s);
+        }
+        diag_store<double>(c, psi64, Q);
+        TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+        diag_pmv<double>(c, psi64, x, 1.0, 0.0);
+    }
+    if (phi_out) TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    return converged;
+}
+
 }  // namespace
 
 // =====================================================================================================
@@ -477,10 +727,15 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
     return guarded(ctx, [&] {
         need_problem(ctx);
         const Geom &g = ctx->g;
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
         const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        prepare_materials(ctx, sig, dt);
-        sweep_prec(ctx, psi, sig, dt, precision);
+        if (staggered_ok(ctx)) {
+            if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
+            else apply_staggered<float>(ctx, psi, sig, dt);
+        } else {
+            sweep_prec(ctx, psi, sig, dt, precision);
+        }
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
         sync(ctx);
     });
@@ -536,7 +791,17 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
-        prepare_materials(ctx, st, dt);
+        if (staggered_ok(ctx)) {
+            int it = 0;
+            double e = 0.0;
+            if (precision == TSW_FP64) si_staggered<double>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+            else si_staggered<float>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+            if (mem == TSW_MEM_HOST || Q != source_inout) stage_out(ctx, source_inout, Q, g.npsi(), mem);
+            sync(ctx);
+            if (iters) *iters = it;
+            if (err) *err = e;
+            return;
+        }
         double *psi = ctx->si_state.as<double>();
         double *cur = static_cast<double *>(ctx->phi64a.reserve(sizeof(double) * g.nphi()));
         double *last = static_cast<double *>(ctx->phi64b.reserve(sizeof(double) * g.nphi()));
@@ -581,7 +846,16 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
         const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
         const double *w = ctx->w64.as<double>();
-        prepare_materials(ctx, st, dt);
+        if (staggered_ok(ctx)) {
+            DBuf phi_dev;
+            double *pd = phi_out ? (mem == TSW_MEM_DEVICE ? phi_out : static_cast<double *>(phi_dev.reserve(sizeof(double) * n))) : nullptr;
+            converged = gmres_staggered(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
+            if (phi_out && mem == TSW_MEM_HOST)
+                TSW_HIP(hipMemcpyAsync(phi_out, pd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            if (mem == TSW_MEM_HOST) stage_out(ctx, source_inout, Q, npsi, mem);
+            sync(ctx);
+            return;
+        }
         double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
         // vectors: b, x, r (phi-shaped, zero halos)
         DBuf vecs;

@@ -1,0 +1,522 @@
+// Staggered sweep and its companion operators on the diagonal-major layout (diag_layout.hpp).
+//
+// Sweep (k_diag_sweep): one workgroup per ordinate, one compute thread per mesh row r plus a storer
+// wave.  At step s thread r solves cell (c = s - r, r), so a step is one diagonal and the workgroup
+// barrier ending it is the diagonal barrier of the reference's wavefront
+// (SweepPatternUniCartWavefront.cpp:253-296).  The x-upwind face trace stays in registers, the
+// y-upwind trace passes through LDS.  Compute waves load the inputs of step s+D branch-free (lanes
+// outside the mesh read a clamped, valid position) and never store to global memory; the storer wave
+// writes step s-1 during step s.  In the diagonal layout all these accesses are contiguous across
+// the lanes of a wavefront.
+//
+// The source of a cell is Q (optional, per ordinate) plus sigma_s * phi / (4 pi) (optional, the
+// isotropic scattering source of Smv, TransportOperator.cpp:369-392, fused into the load).
+#include "cell_solve.hpp"
+#include "diag_layout.hpp"
+#include "tsw_internal.hpp"
+
+namespace tsw {
+
+namespace {
+constexpr double kInv4Pi = 0.079577471545947667884;
+}
+
+template <typename T, int P>
+struct DiagLds {  // LDS carve-up in units of T
+    static constexpr int N = P * P;
+    size_t ex, in, out, inv, total;
+    __host__ __device__ DiagLds(int cthreads, int nx, int ny, int M) {
+        ex = 0;                              // [2][cthreads][P] y-traces between rows
+        in = ex + (size_t)2 * cthreads * P;  // [nx][P] y-inflow traces of row 0
+        out = in + (size_t)nx * P;           // [2][ny][N] solved cells awaiting the store
+        inv = out + (size_t)2 * ny * N;      // [M][N][N] element inverses (material path)
+        total = inv + (size_t)M * N * N;
+    }
+};
+
+// MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
+template <typename T, int P, bool MAT, int MODE, int D>
+__global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp) {
+    constexpr int N = P * P;
+    constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0;
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    const int nx = sp.nx, ny = sp.ny;
+    const int cthreads = (int)blockDim.x - 64;
+    const DiagLds<T, P> L(cthreads, nx, ny, MAT ? sp.M : 0);
+    T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
+    const DiagMap dm{nx, ny};
+    const size_t ncell = (size_t)dm.cells();
+
+    const int q = blockIdx.x;
+    const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
+    const int tid = threadIdx.x;
+    const T xi = sp.xi[q], eta = sp.eta[q];
+    const bool xneg = xi <= T(0), yneg = eta <= T(0);
+
+    if (MAT) {
+        for (int k = tid; k < sp.M * N * N; k += blockDim.x) {
+            const int m = k / (N * N), e = k - m * N * N;
+            linv[k] = sp.ainv[((size_t)m * sp.nq + q) * N * N + e];
+        }
+    }
+    for (int k = tid; k < nx * P; k += blockDim.x) lin[k] = sp.tin_y ? sp.tin_y[(size_t)q * nx * P + k] : T(0);
+
+    const int steps = nx + ny - 1;
+    const int nsteps = ((steps + D - 1) / D) * D;  // padded steps have no active cell
+    T *psi_q = sp.psi + (size_t)q * N * ncell;
+
+    if (tid >= cthreads) {
+        // storer: barrier k (k >= 1) closes compute step k-1, whose cells are written here while
+        // the compute waves fill the other ring slot
+        const int lane = tid - cthreads;
+        lds_barrier();
+        for (int st = 0; st < nsteps; ++st) {
+            lds_barrier();
+            if (st >= steps) continue;
+            const T *slot = lout + (size_t)(st & 1) * ny * N;
+            const int r0 = dm.rlo(st), r1 = st < ny - 1 ? st : ny - 1;
+            const long long p0 = dm.base(st) - r0;
+            for (int r = r0 + lane; r <= r1; r += 64) {
+                const T *src = slot + (size_t)r * N;
+#pragma unroll
+                for (int k = 0; k < N; ++k) psi_q[(size_t)k * ncell + (size_t)(p0 + r)] = src[k];
+            }
+        }
+        return;
+    }
+
+    const int r = tid;
+    const bool row_ok = r < ny;
+    T tx[P];
+#pragma unroll
+    for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? sp.tin_x[((size_t)q * ny + r) * P + d] : T(0);
+
+    // position of this row's cell at step s; lanes whose row has no cell on diagonal s read the
+    // nearest cell of that diagonal, so a wavefront's loads stay within one contiguous run
+    // (diagonal bounds are wave-uniform scalar arithmetic)
+    auto pos_at = [&](int s) -> size_t {
+        if (s >= steps) return 0;
+        const int lo = dm.rlo(s), hi = s < ny - 1 ? s : ny - 1;
+        const int rc = r < lo ? lo : (r > hi ? hi : r);
+        return (size_t)(dm.base(s) + rc - lo);
+    };
+    const T *qsrc_q = HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr;
+    const T *phi_Q = HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr;
+    const T *sigs_Q = HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr;
+    const int *mat_Q = MAT ? sp.mat + (size_t)Q * ncell : nullptr;
+    const T *sigt_Q = MAT ? nullptr : sp.sig_t + (size_t)Q * ncell;
+
+    // prefetch ring: slot u holds the inputs this row meets at a step s with s % D == u
+    T pq[D][HASQ ? N : 1], pf[D][HASPHI ? N : 1], ps[D], pm_sig[D];
+    int pm[D];
+    auto fetch = [&](int u, size_t p) {
+        if (HASQ) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) pq[u][k] = qsrc_q[(size_t)k * ncell + p];
+        }
+        if (HASPHI) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) pf[u][k] = phi_Q[(size_t)k * ncell + p];
+            ps[u] = sigs_Q[p];
+        }
+        if (MAT) pm[u] = mat_Q[p];
+        else pm_sig[u] = sigt_Q[p];
+    };
+#pragma unroll
+    for (int u = 0; u < D; ++u) fetch(u, pos_at(u));
+    lds_barrier();
+
+    // FP32 keeps the current material's inverse in registers and reloads it from LDS only when the
+    // row enters another material (the LDS reads then move data only for the lanes that changed)
+    constexpr bool REGMAT = MAT && sizeof(T) == 4;
+    T Ainv[REGMAT ? N * N : 1];
+    int cur_mat = -1;
+    const T area = sp.dx * sp.dy;
+    for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const int s = s0 + u;
+            const int c = s - r;
+            const bool active = row_ok && c >= 0 && c < nx;
+            // consume slot u, then refill it with step s + D (kept apart by scheduling barriers so
+            // the slot keeps its registers and its wait covers only its own loads)
+            T B[N];
+            {
+                T src[N];
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    T v = T(0);
+                    if (HASQ) v = pq[u][k];
+                    if (HASPHI) v += ps[u] * pf[u][k] * T(kInv4Pi);
+                    src[k] = v;
+                }
+                T ty[P];
+                const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
+                const T *tsrc = r == 0 ? lin + (size_t)cc * P : ex + ((size_t)((s - 1) & 1) * cthreads + (r - 1)) * P;
+#pragma unroll
+                for (int d = 0; d < P; ++d) ty[d] = tsrc[d];
+                rhs_from_traces<T, P>(B, src, tx, ty, xi, eta, sp.dx, sp.dy);
+            }
+            const int mat = pm[u];
+            const T sig = pm_sig[u];
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(u, pos_at(s + D));
+            __builtin_amdgcn_sched_barrier(0);
+            if (active) {
+                T x[N];
+                if (REGMAT) {
+                    if (mat != cur_mat) {
+                        cur_mat = mat;
+                        const T *Mx = linv + (size_t)mat * N * N;
+#pragma unroll
+                        for (int k = 0; k < N * N; ++k) Ainv[k] = Mx[k];
+                    }
+#pragma unroll
+                    for (int row = 0; row < N; ++row) {
+                        T acc = T(0);
+#pragma unroll
+                        for (int col = 0; col < N; ++col) acc += Ainv[row * N + col] * B[col];
+                        x[row] = acc;
+                    }
+                } else if (MAT) {
+                    matvec_lds<T, N>(linv + (size_t)mat * N * N, B, x);
+                } else {
+                    T A[N * N];
+                    assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, (sig + sp.inv_dt) * area);
+                    ge_solve_rcp<T, N>(A, B);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) x[k] = B[k];
+                }
+                T *o = lout + ((size_t)(s & 1) * ny + r) * N;
+#pragma unroll
+                for (int k = 0; k < N; ++k) o[k] = x[k];
+                trace_x<T, P>(x, xneg, tx);
+                T tyo[P];
+                trace_y<T, P>(x, yneg, tyo);
+                T *dst_l = ex + ((size_t)(s & 1) * cthreads + r) * P;
+#pragma unroll
+                for (int d = 0; d < P; ++d) dst_l[d] = tyo[d];
+            }
+            lds_barrier();
+        }
+    }
+}
+
+template <typename T>
+size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
+    if (sp.ny > 448) return 0;
+    const int cthreads = ((sp.ny + 63) / 64) * 64;
+    const int M = sp.ainv ? sp.M : 0;
+    size_t total;
+    switch (p) {
+        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, sp.ny, M).total; break;
+        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, sp.ny, M).total; break;
+        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, sp.ny, M).total; break;
+        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, sp.ny, M).total; break;
+        default: return 0;
+    }
+    const size_t bytes = total * sizeof(T);
+    return bytes <= max_lds_per_block() ? bytes : 0;
+}
+
+template <typename T, int P, bool MAT, int MODE>
+static void launch_ds(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s) {
+    constexpr int D = sizeof(T) == 8 ? 2 : 3;
+    const int threads = ((sp.ny + 63) / 64) * 64 + 64;
+    auto kern = k_diag_sweep<T, P, MAT, MODE, D>;
+    if (lds > 64 * 1024)
+        TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(sp.nq), dim3(threads), lds, s, sp);
+    TSW_HIP(hipGetLastError());
+}
+
+template <typename T, int P, bool MAT>
+static void launch_ds_mode(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s) {
+    const int mode = (sp.qsrc ? 1 : 0) | (sp.phi ? 2 : 0);
+    switch (mode) {
+        case 1: launch_ds<T, P, MAT, 1>(sp, lds, s); break;
+        case 2: launch_ds<T, P, MAT, 2>(sp, lds, s); break;
+        case 3: launch_ds<T, P, MAT, 3>(sp, lds, s); break;
+        default: throw Error(TSW_E_ARG, "diagonal sweep without a source");
+    }
+}
+
+template <typename T, int P>
+static void launch_ds_p(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s) {
+    if (sp.ainv) launch_ds_mode<T, P, true>(sp, lds, s);
+    else launch_ds_mode<T, P, false>(sp, lds, s);
+}
+
+template <typename T>
+void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s) {
+    const size_t lds = diag_sweep_lds_bytes<T>(sp, p);
+    if (lds == 0) throw Error(TSW_E_UNSUPPORTED, "hip-staggered needs ny <= 448 and its tables within one CU's LDS");
+    switch (p) {
+        case 0: launch_ds_p<T, 1>(sp, lds, s); break;
+        case 1: launch_ds_p<T, 2>(sp, lds, s); break;
+        case 2: launch_ds_p<T, 3>(sp, lds, s); break;
+        case 3: launch_ds_p<T, 4>(sp, lds, s); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree " + std::to_string(p) + " not built (0..3)");
+    }
+}
+
+template size_t diag_sweep_lds_bytes<double>(const DiagSweepParams<double> &, int);
+template size_t diag_sweep_lds_bytes<float>(const DiagSweepParams<float> &, int);
+template void launch_diag_sweep<double>(const DiagSweepParams<double> &, int, hipStream_t);
+template void launch_diag_sweep<float>(const DiagSweepParams<float> &, int, hipStream_t);
+
+// =====================================================================================================
+// Layout conversions.  Grids are (diagonal s, ordinate or quadrant) with one thread per row r, so the
+// diagonal-layout side of every access is contiguous.
+// =====================================================================================================
+namespace {
+inline int rows_block(int ny) { return ny <= 64 ? 64 : (ny <= 128 ? 128 : (ny <= 256 ? 256 : 512)); }
+}
+
+// reference psi [i][j][q][k] (interior) <-> diagonal psi [q][k][pos]; dir 0: ref -> diag, 1: diag -> ref
+template <typename TR, typename TD>
+__global__ void k_psi_convert(TR *ref, TD *diag, DiagMap dm, int nq, int N, int qo1, int qo2, int qo3, int dir) {
+    const int s = blockIdx.x, q = blockIdx.y;
+    const int r = dm.rlo(s) + (int)threadIdx.x;
+    const int r1 = s < dm.ny - 1 ? s : dm.ny - 1;
+    if (r > r1) return;
+    const int Q = q >= qo3 ? 3 : (q >= qo2 ? 2 : (q >= qo1 ? 1 : 0));
+    int i, j;
+    dm.mesh(s - r, r, quad_xneg(Q), quad_yneg(Q), i, j);
+    const size_t ncell = (size_t)dm.cells();
+    TR *pr = ref + (((size_t)i * (dm.ny + 2) + j) * nq + q) * N;
+    TD *pd = diag + (size_t)q * N * ncell + (size_t)dm.pos(s, r);
+    if (dir == 0) {
+        for (int k = 0; k < N; ++k) pd[(size_t)k * ncell] = TD(pr[k]);
+    } else {
+        for (int k = 0; k < N; ++k) pr[k] = TR(pd[(size_t)k * ncell]);
+    }
+}
+
+template <typename TR, typename TD>
+void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s) {
+    const DiagMap dm{g.nx, g.ny};
+    hipLaunchKernelGGL((k_psi_convert<TR, TD>), dim3(g.nx + g.ny - 1, g.nq), dim3(rows_block(g.ny)), 0, s, ref, diag,
+                       dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], dir);
+    TSW_HIP(hipGetLastError());
+}
+template void launch_psi_convert<double, double>(const Geom &, double *, double *, int, hipStream_t);
+template void launch_psi_convert<double, float>(const Geom &, double *, float *, int, hipStream_t);
+
+// per-cell array [i][j][k] (k < K, halo layout) -> per-quadrant diagonal [Q][k][pos]
+template <typename TI, typename TO>
+__global__ void k_cell_to_diag(const TI *cell, TO *out, DiagMap dm, int K) {
+    const int s = blockIdx.x, Q = blockIdx.y;
+    const int r = dm.rlo(s) + (int)threadIdx.x;
+    const int r1 = s < dm.ny - 1 ? s : dm.ny - 1;
+    if (r > r1) return;
+    int i, j;
+    dm.mesh(s - r, r, quad_xneg(Q), quad_yneg(Q), i, j);
+    const size_t ncell = (size_t)dm.cells();
+    const TI *pc = cell + ((size_t)i * (dm.ny + 2) + j) * K;
+    TO *po = out + (size_t)Q * K * ncell + (size_t)dm.pos(s, r);
+    for (int k = 0; k < K; ++k) po[(size_t)k * ncell] = TO(pc[k]);
+}
+
+template <typename TI, typename TO>
+void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s) {
+    const DiagMap dm{g.nx, g.ny};
+    hipLaunchKernelGGL((k_cell_to_diag<TI, TO>), dim3(g.nx + g.ny - 1, 4), dim3(rows_block(g.ny)), 0, s, cell, out, dm,
+                       K);
+    TSW_HIP(hipGetLastError());
+}
+template void launch_cell_to_diag<double, double>(const Geom &, const double *, double *, int, hipStream_t);
+template void launch_cell_to_diag<double, float>(const Geom &, const double *, float *, int, hipStream_t);
+template void launch_cell_to_diag<int32_t, int32_t>(const Geom &, const int32_t *, int32_t *, int, hipStream_t);
+
+// Inflow face traces of the halo cells of a reference-layout psi: tin_x[q][r][P] (x-inflow of row r)
+// and tin_y[q][c][P] (y-inflow of column c), as the sweep consumes them.
+template <typename TO, int P>
+__global__ void k_inflow(const double *ref, TO *tin_x, TO *tin_y, int nx, int ny, int nq, int qo1, int qo2, int qo3) {
+    constexpr int N = P * P;
+    const int q = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Q = q >= qo3 ? 3 : (q >= qo2 ? 2 : (q >= qo1 ? 1 : 0));
+    const bool xneg = quad_xneg(Q), yneg = quad_yneg(Q);
+    if (t < ny) {  // row r = t
+        const int j = yneg ? ny - t : t + 1, i = xneg ? nx + 1 : 0;
+        double tr[P];
+        trace_x<double, P>(ref + (((size_t)i * (ny + 2) + j) * nq + q) * N, xneg, tr);
+#pragma unroll
+        for (int d = 0; d < P; ++d) tin_x[((size_t)q * ny + t) * P + d] = TO(tr[d]);
+    }
+    if (t < nx) {  // column c = t
+        const int i = xneg ? nx - t : t + 1, j = yneg ? ny + 1 : 0;
+        double tr[P];
+        trace_y<double, P>(ref + (((size_t)i * (ny + 2) + j) * nq + q) * N, yneg, tr);
+#pragma unroll
+        for (int d = 0; d < P; ++d) tin_y[((size_t)q * nx + t) * P + d] = TO(tr[d]);
+    }
+}
+
+template <typename TO>
+void launch_inflow(const Geom &g, const double *ref, TO *tin_x, TO *tin_y, hipStream_t s) {
+    const int n = g.nx > g.ny ? g.nx : g.ny;
+    const dim3 grid((n + 127) / 128, g.nq), block(128);
+    const int a = g.quad_off[1], b = g.quad_off[2], c = g.quad_off[3];
+    switch (g.p) {
+        case 0: hipLaunchKernelGGL((k_inflow<TO, 1>), grid, block, 0, s, ref, tin_x, tin_y, g.nx, g.ny, g.nq, a, b, c); break;
+        case 1: hipLaunchKernelGGL((k_inflow<TO, 2>), grid, block, 0, s, ref, tin_x, tin_y, g.nx, g.ny, g.nq, a, b, c); break;
+        case 2: hipLaunchKernelGGL((k_inflow<TO, 3>), grid, block, 0, s, ref, tin_x, tin_y, g.nx, g.ny, g.nq, a, b, c); break;
+        case 3: hipLaunchKernelGGL((k_inflow<TO, 4>), grid, block, 0, s, ref, tin_x, tin_y, g.nx, g.ny, g.nq, a, b, c); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
+    }
+    TSW_HIP(hipGetLastError());
+}
+template void launch_inflow<double>(const Geom &, const double *, double *, double *, hipStream_t);
+template void launch_inflow<float>(const Geom &, const double *, float *, float *, hipStream_t);
+
+// =====================================================================================================
+// Moment contraction on the diagonal layout: per quadrant, phiQ[Q][k][pos] = sum_{q in Q} w_q psi[q][k][pos]
+// (contiguous over pos), then phi[i][j][k] = beta phi + alpha sum_Q phiQ[Q][k][posQ(i,j)]
+// (TransportOperator.cpp:140-158).
+// =====================================================================================================
+template <typename TI>
+__global__ __launch_bounds__(256) void k_pmv_quadrant(const TI *__restrict__ psi, const double *__restrict__ w,
+                                                      double *__restrict__ phiQ, size_t ncell, int N, int qo1,
+                                                      int qo2, int qo3, int nq) {
+    const int Q = blockIdx.y;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * N) return;
+    const int q0 = Q == 0 ? 0 : (Q == 1 ? qo1 : (Q == 2 ? qo2 : qo3));
+    const int q1 = Q == 0 ? qo1 : (Q == 1 ? qo2 : (Q == 2 ? qo3 : nq));
+    double acc = 0.0;
+    for (int q = q0; q < q1; ++q) acc += w[q] * (double)psi[(size_t)q * N * ncell + t];
+    phiQ[(size_t)Q * N * ncell + t] = acc;
+}
+
+__global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *__restrict__ phi, DiagMap dm, int N,
+                                     double alpha, double beta) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncell = (size_t)dm.cells();
+    if (t >= ncell * N) return;
+    const int k = (int)(t % N);
+    const size_t cidx = t / N;
+    const int i = 1 + (int)(cidx / dm.ny), j = 1 + (int)(cidx % dm.ny);
+    double acc = 0.0;
+    for (int Q = 0; Q < 4; ++Q)
+        acc += phiQ[(size_t)Q * N * ncell + (size_t)k * ncell + (size_t)dm.pos_of_cell(i, j, quad_xneg(Q), quad_yneg(Q))];
+    double *y = phi + ((size_t)i * (dm.ny + 2) + j) * N + k;
+    *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * acc;
+}
+
+template <typename TI>
+void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha, double beta,
+                     hipStream_t s) {
+    const DiagMap dm{g.nx, g.ny};
+    const size_t n = (size_t)dm.cells() * g.N();
+    hipLaunchKernelGGL((k_pmv_quadrant<TI>), dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, s, psi, w, phiQ,
+                       (size_t)dm.cells(), g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], g.nq);
+    hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, dm, g.N(),
+                       alpha, beta);
+    TSW_HIP(hipGetLastError());
+}
+template void launch_pmv_diag<double>(const Geom &, const double *, const double *, double *, double *, double, double, hipStream_t);
+template void launch_pmv_diag<float>(const Geom &, const float *, const double *, double *, double *, double, double, hipStream_t);
+
+// =====================================================================================================
+// FP64 angular residual on the diagonal layout, the refinement's r = Q + sigma_s phi/(4 pi) - L psi
+// (Lmv, TransportOperator.cpp:1576-1658, written with face traces), stored in TO.
+// psi's inflow comes from tin_x/tin_y (traces of the halo cells).  Grid (diagonal, ordinate).
+// =====================================================================================================
+template <typename TO, int P>
+__global__ void k_residual_diag(const double *__restrict__ qsrc, const double *__restrict__ phi_d,
+                                const double *__restrict__ sigs_d, const double *__restrict__ sigt_d,
+                                const double *__restrict__ psi, const double *__restrict__ tin_x,
+                                const double *__restrict__ tin_y, const double *__restrict__ xi_,
+                                const double *__restrict__ eta_, TO *__restrict__ res, DiagMap dm, int nq, int qo1,
+                                int qo2, int qo3, double dx, double dy, double inv_dt) {
+    constexpr int N = P * P;
+    const int s = blockIdx.x, q = blockIdx.y;
+    const int r = dm.rlo(s) + (int)threadIdx.x;
+    const int r1 = s < dm.ny - 1 ? s : dm.ny - 1;
+    if (r > r1) return;
+    const int c = s - r;
+    const int Q = q >= qo3 ? 3 : (q >= qo2 ? 2 : (q >= qo1 ? 1 : 0));
+    const size_t ncell = (size_t)dm.cells();
+    const double xi = xi_[q], eta = eta_[q];
+    const bool xneg = xi <= 0.0, yneg = eta <= 0.0;
+    const size_t p = (size_t)dm.pos(s, r);
+    const double *pq = psi + (size_t)q * N * ncell;
+    double x[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = pq[(size_t)k * ncell + p];
+    double tx[P], ty[P];
+    if (c == 0) {
+#pragma unroll
+        for (int d = 0; d < P; ++d) tx[d] = tin_x[((size_t)q * dm.ny + r) * P + d];
+    } else {
+        const size_t pu = (size_t)dm.pos(s - 1, r);
+        double u[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) u[k] = pq[(size_t)k * ncell + pu];
+        trace_x<double, P>(u, xneg, tx);
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int d = 0; d < P; ++d) ty[d] = tin_y[((size_t)q * dm.nx + c) * P + d];
+    } else {
+        const size_t pu = (size_t)dm.pos(s - 1, r - 1);
+        double u[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) u[k] = pq[(size_t)k * ncell + pu];
+        trace_y<double, P>(u, yneg, ty);
+    }
+    // (L + 1/dt) x = (Aq x + (sigma + 1/dt) dx dy x - Kx tx - Ky ty) / (dx dy): the sweep's element
+    // system with the inflow terms moved back to the left
+    double A[N * N];
+    const size_t pQ = (size_t)Q * ncell + p;
+    assemble_matrix<double, P>(A, xi, eta, dx, dy, (sigt_d[pQ] + inv_dt) * dx * dy);
+    double zero[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) zero[k] = 0.0;
+    double inflow[N];
+    rhs_from_traces<double, P>(inflow, zero, tx, ty, xi, eta, dx, dy);
+    const double inv_area = 1.0 / (dx * dy);
+    TO *pr = res + (size_t)q * N * ncell + p;
+#pragma unroll
+    for (int row = 0; row < N; ++row) {
+        double lx = 0.0;
+#pragma unroll
+        for (int col = 0; col < N; ++col) lx += A[row + N * col] * x[col];
+        lx = (lx - inflow[row]) * inv_area;
+        double v = qsrc[(size_t)q * N * ncell + (size_t)row * ncell + p];
+        v += sigs_d[pQ] * phi_d[(size_t)Q * N * ncell + (size_t)row * ncell + p] * kInv4Pi;
+        pr[(size_t)row * ncell] = TO(v - lx);
+    }
+}
+
+template <typename TO>
+void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,
+                          const double *sigt_d, const double *psi, const double *tin_x, const double *tin_y,
+                          const double *xi, const double *eta, double inv_dt, TO *res, hipStream_t s) {
+    const DiagMap dm{g.nx, g.ny};
+    const dim3 grid(g.nx + g.ny - 1, g.nq), block(rows_block(g.ny));
+    const int a = g.quad_off[1], b = g.quad_off[2], c = g.quad_off[3];
+#define TSW_RES(PP)                                                                                                   \
+    hipLaunchKernelGGL((k_residual_diag<TO, PP>), grid, block, 0, s, qsrc, phi_d, sigs_d, sigt_d, psi, tin_x, tin_y, \
+                       xi, eta, res, dm, g.nq, a, b, c, g.dx, g.dy, inv_dt)
+    switch (g.p) {
+        case 0: TSW_RES(1); break;
+        case 1: TSW_RES(2); break;
+        case 2: TSW_RES(3); break;
+        case 3: TSW_RES(4); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
+    }
+#undef TSW_RES
+    TSW_HIP(hipGetLastError());
+}
+template void launch_residual_diag<float>(const Geom &, const double *, const double *, const double *, const double *,
+                                          const double *, const double *, const double *, const double *, const double *,
+                                          double, float *, hipStream_t);
+template void launch_residual_diag<double>(const Geom &, const double *, const double *, const double *, const double *,
+                                           const double *, const double *, const double *, const double *, const double *,
+                                           double, double *, hipStream_t);
+
+}  // namespace tsw

@@ -75,11 +75,48 @@ template <typename T> struct SweepParams {
     int nx, ny, nq;
     int qo1, qo2, qo3;  // quadrant offsets 1..3
     T dx, dy, inv_dt;
-    // optional material table: per-cell material index and element-matrix inverses [M][nq][N][N]
-    const int32_t *mat = nullptr;
-    const T *ainv = nullptr;
-    int M = 0;
 };
+
+// Sweep on the diagonal-major layout (diag_layout.hpp).  Source of a cell: qsrc (if set) plus
+// sig_s * phi / (4 pi) (if phi is set).  Inflow traces tin_x/tin_y (null: vacuum).  Element solve by
+// the material table (mat + ainv) or per-cell elimination with sig_t.
+template <typename T> struct DiagSweepParams {
+    const T *xi, *eta;     // [nq]
+    int nx, ny, nq, qo1, qo2, qo3;
+    T dx, dy, inv_dt;
+    const T *qsrc;         // [nq][N][ncell] or null (may alias psi: in-place sweep)
+    const T *phi;          // [4][N][ncell] or null
+    const T *sig_s;        // [4][ncell] (with phi)
+    const T *tin_x;        // [nq][ny][P] or null
+    const T *tin_y;        // [nq][nx][P] or null
+    const int32_t *mat;    // [4][ncell] (with ainv)
+    const T *ainv;         // [M][nq][N][N] or null
+    int M;
+    const T *sig_t;        // [4][ncell] (without ainv)
+    T *psi;                // out [nq][N][ncell]
+};
+template <typename T>
+void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s);
+template <typename T>
+size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p);
+// dir 0: reference psi -> diagonal psi; dir 1: diagonal -> reference (interior cells only)
+template <typename TR, typename TD>
+void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s);
+// per-cell [i][j][K] -> per-quadrant diagonal [4][K][ncell]
+template <typename TI, typename TO>
+void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s);
+// inflow face traces of the halo cells of reference-layout psi
+template <typename TO>
+void launch_inflow(const Geom &g, const double *ref, TO *tin_x, TO *tin_y, hipStream_t s);
+// phi[i][j][k] = beta phi + alpha sum_q w_q psi (diagonal psi); phiQ: [4][N][ncell] scratch
+template <typename TI>
+void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha,
+                     double beta, hipStream_t s);
+// res = qsrc + sig_s phi/(4 pi) - (L + 1/dt) psi on the diagonal layout (FP64 inputs)
+template <typename TO>
+void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,
+                          const double *sigt_d, const double *psi, const double *tin_x, const double *tin_y,
+                          const double *xi, const double *eta, double inv_dt, TO *res, hipStream_t s);
 
 // Largest dynamic LDS allocation one workgroup may make on the current device (bytes).
 size_t max_lds_per_block();
@@ -87,11 +124,6 @@ size_t max_lds_per_block();
 // ---- launch wrappers (defined in the .hip files) ----
 template <typename T>
 void launch_sweep_wavefront(const SweepParams<T> &sp, int p, hipStream_t s);
-template <typename T>
-void launch_sweep_staggered(const SweepParams<T> &sp, int p, hipStream_t s);
-// bytes of LDS the staggered kernel needs for this problem (0 if it cannot run)
-template <typename T>
-size_t staggered_lds_bytes(const SweepParams<T> &sp, int p);
 template <typename T>
 void launch_material_inverses(const Geom &g, const double *xi, const double *eta, const double *sig_m, int M,
                               double inv_dt, T *out, hipStream_t s);
