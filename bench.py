@@ -4,9 +4,13 @@ versus FP32 sweeps inside FP64 iterative refinement, both to the absolute residu
 
 One step = one full transport solve of the lattice (inputs resident in HBM).  `value` is the
 mixed-precision solve's angular unknowns swept per second (unknowns per sweep x sweeps / wall time).
-The roofline entry is for the FP32 sweep kernel; its algorithmic bytes per sweep are
-U*(4+4) + ncell*4 (read the source, write psi, read sigma) and its duration is measured with HIP
-events recorded by the library on the stream the sweep launches run on.
+
+The roofline entry is for the FP32 sweep kernel under the operator contract (SweepOperator::Apply:
+read the source psi, write psi in place, read sigma): algorithmic bytes per sweep
+U*(4+4) + ncell*4, divided by the sweep kernel's average duration, measured with HIP events the
+library records on the stream it launches the sweep on (tsw_sweep_timing).  `--copy-probe` adds a
+device copy of known size, the calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs.
+
 The CPU baseline is the oracle (a single-thread C restatement of the reference sweep) on a bounded
 sample of sweeps of the same lattice.
 """
@@ -50,13 +54,13 @@ def cpu_baseline(P, sweeps):
                       f"{dt:.1f} s) on {cpu_model()}"}
 
 
-def traffic_from_profiles(kernel_tag):
-    """HBM bytes per sweep from the committed rocprofv3 --pmc summary, if present."""
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             data = json.load(f)
-        return data.get(kernel_tag, {}).get("bytes_per_sweep")
+        return data.get(kernel, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -69,6 +73,9 @@ def main():
     ap.add_argument("--pattern", default="hip-staggered")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-sweeps", type=int, default=3, help="oracle sweeps timed for cpu_baseline (0: skip)")
+    ap.add_argument("--roofline-sweeps", type=int, default=20, help="operator-contract sweeps timed per precision")
+    ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep has no "
@@ -111,6 +118,35 @@ def main():
         return wall, n_sweeps, sweep_ms, its
 
     print(f"bench: lattice {P.nx}x{P.ny} Q{P.p} S12 nq={P.nq} U={P.unknowns}", file=sys.stderr, flush=True)
+    U = P.unknowns
+    ncell = P.nx * P.ny
+
+    def contract_sweeps(precision):
+        """Average kernel time (ms) of in-place operator-contract sweeps of the lattice source."""
+        ctx.sweep(work, d_st, precision=precision)  # warm-up (builds tables)
+        ctx.sweep_timing(reset=True)
+        for _ in range(args.roofline_sweeps):
+            work.copy_(d_src)
+            torch.cuda.synchronize()
+            ctx.sweep(work, d_st, precision=precision)
+        n, ms = ctx.sweep_timing(reset=True)
+        return ms / max(n, 1)
+
+    t_sw32 = contract_sweeps(tsw.FP32)
+    t_sw64 = contract_sweeps(tsw.FP64)
+    print(f"bench: contract sweep fp32 {t_sw32:.3f} ms fp64 {t_sw64:.3f} ms", file=sys.stderr, flush=True)
+    if args.copy_probe:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)                # writes 1 GiB
+        torch.cuda.synchronize()
+        torch.add(a, 1.0, out=b)     # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        print("bench: probes: fill 1 GiB, add 1 GiB -> 1 GiB", file=sys.stderr, flush=True)
+    if args.sweeps_only:
+        ctx.close()
+        return
+
     w64, n64, ms64, its64 = timed(tsw.FP64, phi64)
     print(f"bench: fp64 wall {w64:.3f}s sweeps {n64} sweep-ms {ms64:.1f} its {its64}", file=sys.stderr, flush=True)
     w32, n32, ms32, its32 = timed(tsw.FP32, phi32)
@@ -118,15 +154,12 @@ def main():
 
     inner = (slice(1, -1), slice(1, -1))
     err = float((phi32[inner] - phi64[inner]).norm() / phi64[inner].norm())
-    U = P.unknowns
     value = U * n32 / w32
     value64 = U * n64 / w64
-    ncell = P.nx * P.ny
     bytes32 = U * 8 + ncell * 4
-    t_sweep32 = ms32 / max(n32, 1) * 1e-3
-    achieved = bytes32 / t_sweep32 / 1e9
-    t_sweep64 = ms64 / max(n64, 1) * 1e-3
-    achieved64 = (U * 16 + ncell * 8) / t_sweep64 / 1e9
+    bytes64 = U * 16 + ncell * 8
+    achieved = bytes32 / (t_sw32 * 1e-3) / 1e9
+    achieved64 = bytes64 / (t_sw64 * 1e-3) / 1e9
 
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
@@ -145,14 +178,17 @@ def main():
                                "to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
                    "sweep_pattern": args.pattern, "dsa": False, "unknowns_per_sweep": U},
         "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
-                 "gmres_its": [i for i, _ in its64], "sweep_ms": ms64 / max(n64, 1),
-                 "sweep_hbm_gbs": achieved64},
+                 "gmres_its": [i for i, _ in its64], "solve_sweep_ms": ms64 / max(n64, 1),
+                 "contract_sweep_ms": t_sw64, "contract_sweep_gbs": achieved64},
         "mixed": {"sweeps_per_solve": n32 / args.steps, "gmres_its": [i for i, _ in its32],
-                  "sweep_ms": ms32 / max(n32, 1)},
+                  "solve_sweep_ms": ms32 / max(n32, 1), "contract_sweep_ms": t_sw32},
         "speedup_mixed_vs_fp64": (w64 / w32),
         "mixed_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(f"sweep_fp32_{args.pattern}")},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "kernel": "k_diag_sweep<float,3,true,1,3> (FP32 in-place operator-contract sweep)",
+                     "algorithmic_bytes_per_launch": bytes32,
+                     "traffic": traffic_from_profiles("k_diag_sweep<float, 3, true, 1, 3>")},
     }
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(P, args.cpu_sweeps)
