@@ -45,13 +45,69 @@ def cpu_baseline(P, sweeps):
     from _common import Grid, Ordinates, orc_sweep
     g = Grid(P.nx, P.ny, P.p, P.ax, P.bx, P.ay, P.by)
     o = Ordinates(P.xi, P.eta, P.w, P.quad_off)
+    src = P.source
     t0 = time.perf_counter()
     for _ in range(sweeps):
-        orc_sweep(g, o, P.sigma_t, P.source)
+        orc_sweep(g, o, P.sigma_t, src)
     dt = time.perf_counter() - t0
     return {"value": P.unknowns * sweeps / dt, "unit": "angular unknowns swept/s", "cores": 1, "kind": "port",
-            "sample": f"{sweeps} FP64 sweeps of the 280x280 Q2 S12 lattice by the oracle (1 thread, "
-                      f"{dt:.1f} s) on {cpu_model()}"}
+            "sample": f"{sweeps} FP64 sweeps of the {P.nx}x{P.ny} Q{P.p} nq={P.nq} {P.name} problem by the oracle "
+                      f"(1 thread, {dt:.1f} s) on {cpu_model()}"}
+
+
+def bench_batched(args):
+    """Batched sweep of the seeded 256-problem uneven batch (N 32..256, Q1..Q3, S4..S12, 8x8 block
+    materials): one step = one sweep of every member, inputs resident in HBM."""
+    import torch
+    import tsw
+    from tsw import problems
+    batch = problems.batched(seed=2026)
+    dev = torch.device("cuda:0")
+    srcs = [P.source_device(dev) for P in batch]
+    sig = [torch.from_numpy(P.sigma_t).to(dev) for P in batch]
+    work = [s.clone() for s in srcs]
+    ctx = tsw.Context(0)
+    ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
+    U = sum(P.unknowns for P in batch)
+    ncell = sum(P.nx * P.ny for P in batch)
+    res = {}
+    for name, prec in (("fp32", tsw.FP32), ("fp64", tsw.FP64)):
+        for _ in range(args.warmup):
+            ctx.batch_sweep(work, precision=prec)
+        ctx.sweep_timing(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for w, s in zip(work, srcs):
+                w.copy_(s)
+            torch.cuda.synchronize()
+            ctx.batch_sweep(work, precision=prec)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        n, ms = ctx.sweep_timing(reset=True)
+        res[name] = (wall, ms / max(n, 1))
+        print(f"bench: batched {name} kernel {ms / max(n, 1):.3f} ms per batch sweep, wall {wall:.3f}s",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    t32, t64 = res["fp32"][1] * 1e-3, res["fp64"][1] * 1e-3
+    b32 = U * 8 + ncell * 4
+    line = {
+        "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
+        "value": U / t32, "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded batch generated in-process)",
+        "config": {"workload": "batched sweep: 256 uneven problems, N 32..256, Q1..Q3, S4..S12, 8x8 block "
+                               "materials, seed 2026", "unknowns_per_batch_sweep": U},
+        "fp64": {"value": U / t64, "ms_per_step": t64 * 1e3, "hbm_gbs": (U * 16 + ncell * 8) / t64 / 1e9},
+        "speedup_fp32_vs_fp64": t64 / t32,
+        "roofline": {"bound": "hbm", "achieved": b32 / t32 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b32 / t32 / 1e9 / HBM_PEAK_GBS, "kernel": "k_diag_sweep batched launches (FP32)",
+                     "algorithmic_bytes_per_launch": b32, "traffic": None},
+    }
+    if args.cpu_sweeps > 0:
+        mid = sorted(batch, key=lambda P: P.unknowns)[len(batch) // 2]
+        line["cpu_baseline"] = cpu_baseline(mid, args.cpu_sweeps)
+    print(json.dumps(line), flush=True)
 
 
 def traffic_from_profiles(kernel):
@@ -76,11 +132,15 @@ def main():
     ap.add_argument("--roofline-sweeps", type=int, default=20, help="operator-contract sweeps timed per precision")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "batched"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep has no "
               f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import torch
     import tsw
@@ -88,7 +148,7 @@ def main():
 
     P = problems.lattice()
     dev = torch.device("cuda:0")
-    d_src = torch.from_numpy(P.source).to(dev)
+    d_src = P.source_device(dev)
     d_st = torch.from_numpy(P.sigma_t).to(dev)
     d_ss = torch.from_numpy(P.sigma_s).to(dev)
     work = torch.empty_like(d_src)

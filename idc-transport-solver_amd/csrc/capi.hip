@@ -18,9 +18,10 @@ static bool is_staggered(Pattern p) { return p == Pattern::Staggered || p == Pat
 struct tsw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     std::string err;
     bool have_problem = false;
-    Pattern pattern = Pattern::WavefrontGraph;
+    Pattern pattern = Pattern::Staggered;
     Geom g;
     std::vector<double> h_xi, h_eta, h_w;
 
@@ -64,10 +65,34 @@ struct tsw_ctx {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
     }
+    // Batch of independent problems (tsw_batch_set): one sub-context per member sharing this stream,
+    // launch groups by (DG degree, compute waves, material path), side streams for the groups.
+    std::vector<std::unique_ptr<tsw_ctx>> batch;
+    double batch_dt = 0.0;
+    struct Group {
+        int p = 0, waves = 0, max_ny = 0;
+        bool mat = false;
+        std::vector<int> members;       // batch indices
+        std::vector<int> items;         // pairs {member slot, ordinate}, heaviest first
+        DBuf params64, params32, dev_items;
+        bool built64 = false, built32 = false, mat64 = false, mat32 = false;
+        size_t lds64 = 0, lds32 = 0;
+    };
+    std::vector<Group> groups;
+    static constexpr int kSideStreams = 4;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
+
     ~tsw_ctx() {
         clear_graphs();
         for (auto e : ev_pool) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (auto &sd : side)
+            if (sd) (void)hipStreamDestroy(sd);
+        for (auto &e : join_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        batch.clear();
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -381,6 +406,8 @@ T *dbuf(DBuf &b, size_t n) {
     return static_cast<T *>(b.reserve(n * sizeof(T)));
 }
 
+void diag_load_source(tsw_ctx *c, const double *qref, bool need64, bool need32);
+
 // Builds the per-call inputs: materials or per-cell sigma_t, sigma_s, inflow traces and the source
 // from reference-layout arrays (sigma_s, qref may be null).  need64/need32 select the precisions.
 void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const double *qref, double dt, bool need64,
@@ -398,6 +425,15 @@ void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const 
         if (need64) launch_cell_to_diag<double, double>(g, sigma_s, dbuf<double>(d.sigs64, 4 * ncell), 1, s);
         if (need32) launch_cell_to_diag<double, float>(g, sigma_s, dbuf<float>(d.sigs32, 4 * ncell), 1, s);
     }
+    diag_load_source(c, qref, need64, need32);
+}
+
+// Inflow traces and the source in the diagonal layout, from a reference-layout psi (null: no source).
+void diag_load_source(tsw_ctx *c, const double *qref, bool need64, bool need32) {
+    const Geom &g = c->g;
+    auto &d = c->diag;
+    const size_t ncell = (size_t)g.nx * g.ny, N = g.N();
+    hipStream_t s = c->stream;
     d.inflow = qref != nullptr;
     if (qref) {
         const size_t nt_x = (size_t)g.nq * g.ny * (g.p + 1), nt_y = (size_t)g.nq * g.nx * (g.p + 1);
@@ -428,11 +464,11 @@ const T *diag_inverses(tsw_ctx *c, double dt) {
     return buf.as<T>();
 }
 
-// One staggered sweep: psi_out <- L^{-1}(qsrc + sigma_s phi_d / (4 pi)), inflow from the traces set up
-// by diag_setup when `inflow`, else vacuum.  qsrc or phi_d may be null (not both); qsrc may equal
-// psi_out.
+// Kernel parameters of one staggered sweep: psi_out <- L^{-1}(qsrc + sigma_s phi_d / (4 pi)), inflow from
+// the traces set up by diag_setup when `inflow`, else vacuum.  qsrc or phi_d may be null (not both);
+// qsrc may equal psi_out.
 template <typename T>
-void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow, T *psi_out) {
+DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow, T *psi_out) {
     const Geom &g = c->g;
     auto &d = c->diag;
     const bool f64 = sizeof(T) == 8;
@@ -458,9 +494,16 @@ void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflo
     }
     if (!sp.ainv) sp.sig_t = (const T *)(f64 ? d.sigt64.ptr : d.sigt32.ptr);
     sp.psi = psi_out;
+    return sp;
+}
+
+// One staggered sweep on the context's stream, timed.
+template <typename T>
+void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow, T *psi_out) {
+    const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, psi_out);
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     TSW_HIP(hipEventRecord(e0, c->stream));
-    launch_diag_sweep<T>(sp, g.p, c->stream);
+    launch_diag_sweep<T>(sp, c->g.p, c->stream);
     TSW_HIP(hipEventRecord(e1, c->stream));
     ++c->sweeps_done;
 }
@@ -634,6 +677,50 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
     return converged;
 }
 
+// Validates and installs a mesh + ordinate set (tsw_problem_set, also used per batch member).
+void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
+                 const double *xi, const double *eta, const double *w, const int *quad_off) {
+    require(nx >= 1 && ny >= 1, TSW_E_ARG, "nx and ny must be >= 1");
+    require(bx > ax && by > ay, TSW_E_ARG, "empty domain");
+    require(p >= 0, TSW_E_ARG, "DG degree must be >= 0");
+    require(p <= 3, TSW_E_UNSUPPORTED, "DG degree " + std::to_string(p) + " not built (0..3)");
+    require(nq >= 1 && xi && eta && w && quad_off, TSW_E_ARG, "bad ordinate set");
+    require(quad_off[0] == 0 && quad_off[4] == nq, TSW_E_ARG, "quad_off must span [0, nq]");
+    for (int k = 0; k < 4; ++k) {
+        require(quad_off[k] <= quad_off[k + 1], TSW_E_ARG, "quad_off must be non-decreasing");
+        for (int q = quad_off[k]; q < quad_off[k + 1]; ++q) {
+            const bool xneg = (k >= 2), yneg = (k & 1);
+            require(xi[q] != 0.0 && eta[q] != 0.0 && (xi[q] < 0) == xneg && (eta[q] < 0) == yneg, TSW_E_ARG,
+                    "ordinate " + std::to_string(q) + " is not in quadrant " + std::to_string(k));
+        }
+    }
+    ctx->clear_graphs();
+    Geom &g = ctx->g;
+    g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
+    g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
+    g.dx = (bx - ax) / nx;
+    g.dy = (by - ay) / ny;
+    for (int k = 0; k < 5; ++k) g.quad_off[k] = quad_off[k];
+    ctx->h_xi.assign(xi, xi + nq);
+    ctx->h_eta.assign(eta, eta + nq);
+    ctx->h_w.assign(w, w + nq);
+    std::vector<float> f(nq);
+    auto put64 = [&](DBuf &b, const double *src) {
+        b.reserve(sizeof(double) * nq);
+        TSW_HIP(hipMemcpy(b.ptr, src, sizeof(double) * nq, hipMemcpyHostToDevice));
+    };
+    auto put32 = [&](DBuf &b, const double *src) {
+        for (int q = 0; q < nq; ++q) f[q] = (float)src[q];
+        b.reserve(sizeof(float) * nq);
+        TSW_HIP(hipMemcpy(b.ptr, f.data(), sizeof(float) * nq, hipMemcpyHostToDevice));
+    };
+    put64(ctx->xi64, xi); put64(ctx->eta64, eta); put64(ctx->w64, w);
+    put32(ctx->xi32, xi); put32(ctx->eta32, eta); put32(ctx->w32, w);
+    double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
+    TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
+    ctx->have_problem = true;
+}
+
 }  // namespace
 
 // =====================================================================================================
@@ -668,47 +755,7 @@ const char *tsw_last_error(const tsw_ctx *ctx) { return ctx ? ctx->err.c_str() :
 
 int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
                     const double *xi, const double *eta, const double *w, const int *quad_off) {
-    return guarded(ctx, [&] {
-        require(nx >= 1 && ny >= 1, TSW_E_ARG, "nx and ny must be >= 1");
-        require(bx > ax && by > ay, TSW_E_ARG, "empty domain");
-        require(p >= 0, TSW_E_ARG, "DG degree must be >= 0");
-        require(p <= 3, TSW_E_UNSUPPORTED, "DG degree " + std::to_string(p) + " not built (0..3)");
-        require(nq >= 1 && xi && eta && w && quad_off, TSW_E_ARG, "bad ordinate set");
-        require(quad_off[0] == 0 && quad_off[4] == nq, TSW_E_ARG, "quad_off must span [0, nq]");
-        for (int k = 0; k < 4; ++k) {
-            require(quad_off[k] <= quad_off[k + 1], TSW_E_ARG, "quad_off must be non-decreasing");
-            for (int q = quad_off[k]; q < quad_off[k + 1]; ++q) {
-                const bool xneg = (k >= 2), yneg = (k & 1);
-                require(xi[q] != 0.0 && eta[q] != 0.0 && (xi[q] < 0) == xneg && (eta[q] < 0) == yneg, TSW_E_ARG,
-                        "ordinate " + std::to_string(q) + " is not in quadrant " + std::to_string(k));
-            }
-        }
-        ctx->clear_graphs();
-        Geom &g = ctx->g;
-        g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
-        g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
-        g.dx = (bx - ax) / nx;
-        g.dy = (by - ay) / ny;
-        for (int k = 0; k < 5; ++k) g.quad_off[k] = quad_off[k];
-        ctx->h_xi.assign(xi, xi + nq);
-        ctx->h_eta.assign(eta, eta + nq);
-        ctx->h_w.assign(w, w + nq);
-        std::vector<float> f(nq);
-        auto put64 = [&](DBuf &b, const double *src) {
-            b.reserve(sizeof(double) * nq);
-            TSW_HIP(hipMemcpy(b.ptr, src, sizeof(double) * nq, hipMemcpyHostToDevice));
-        };
-        auto put32 = [&](DBuf &b, const double *src) {
-            for (int q = 0; q < nq; ++q) f[q] = (float)src[q];
-            b.reserve(sizeof(float) * nq);
-            TSW_HIP(hipMemcpy(b.ptr, f.data(), sizeof(float) * nq, hipMemcpyHostToDevice));
-        };
-        put64(ctx->xi64, xi); put64(ctx->eta64, eta); put64(ctx->w64, w);
-        put32(ctx->xi32, xi); put32(ctx->eta32, eta); put32(ctx->w32, w);
-        double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
-        TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
-        ctx->have_problem = true;
-    });
+    return guarded(ctx, [&] { set_problem(ctx, nx, ny, ax, bx, ay, by, p, nq, xi, eta, w, quad_off); });
 }
 
 int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
@@ -962,6 +1009,167 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
         return TSW_E_NOCONV;
     }
     return rc;
+}
+
+int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const double *const *sigma_t, double dt,
+                  int mem) {
+    return guarded(ctx, [&] {
+        require(nprob >= 1 && probs && sigma_t, TSW_E_ARG, "empty batch");
+        ctx->batch.clear();
+        ctx->groups.clear();
+        ctx->batch_dt = dt;
+        for (int b = 0; b < nprob; ++b) {
+            const tsw_problem &pr = probs[b];
+            std::unique_ptr<tsw_ctx> sub(new tsw_ctx());
+            sub->device = ctx->device;
+            sub->stream = ctx->stream;
+            sub->own_stream = false;
+            sub->pattern = Pattern::Staggered;
+            set_problem(sub.get(), pr.nx, pr.ny, pr.ax, pr.bx, pr.ay, pr.by, pr.p, pr.nq, pr.xi, pr.eta, pr.w,
+                        pr.quad_off);
+            require(staggered_ok(sub.get()), TSW_E_UNSUPPORTED,
+                    "batch member " + std::to_string(b) + " does not fit the staggered sweep (ny <= 448)");
+            const Geom &g = sub->g;
+            double *sig = static_cast<double *>(sub->sig_t64.reserve(sizeof(double) * g.ncell_h()));
+            TSW_HIP(hipMemcpyAsync(sig, sigma_t[b], sizeof(double) * g.ncell_h(),
+                                   mem == TSW_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+            diag_setup(sub.get(), sig, nullptr, nullptr, dt, true, true);
+            ctx->batch.push_back(std::move(sub));
+        }
+        // launch groups
+        std::map<std::tuple<int, int, bool>, int> gid;
+        for (int b = 0; b < nprob; ++b) {
+            tsw_ctx *sub = ctx->batch[b].get();
+            const auto key = std::make_tuple(sub->g.p, (sub->g.ny + 63) / 64, sub->mat.valid);
+            auto it = gid.find(key);
+            if (it == gid.end()) {
+                it = gid.emplace(key, (int)ctx->groups.size()).first;
+                ctx->groups.emplace_back();
+                auto &G = ctx->groups.back();
+                G.p = sub->g.p;
+                G.waves = (sub->g.ny + 63) / 64;
+                G.mat = sub->mat.valid;
+            }
+            auto &G = ctx->groups[it->second];
+            G.members.push_back(b);
+            G.max_ny = std::max(G.max_ny, sub->g.ny);
+        }
+        for (auto &G : ctx->groups) {  // work items, heaviest (most diagonal steps) first
+            std::vector<std::tuple<long long, int, int>> w;
+            for (int slot = 0; slot < (int)G.members.size(); ++slot) {
+                const Geom &g = ctx->batch[G.members[slot]]->g;
+                for (int q = 0; q < g.nq; ++q) w.emplace_back(-(long long)(g.nx + g.ny) * g.ny, slot, q);
+            }
+            std::sort(w.begin(), w.end());
+            for (auto &t : w) {
+                G.items.push_back(std::get<1>(t));
+                G.items.push_back(std::get<2>(t));
+            }
+            TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
+                                   sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        for (int k = 0; k < tsw_ctx::kSideStreams; ++k) {
+            if (!ctx->side[k]) TSW_HIP(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+            if (!ctx->join_ev[k]) TSW_HIP(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+        }
+        if (!ctx->fork_ev) TSW_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+        sync(ctx);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+// Builds (once) the device parameter table of a launch group for precision T.
+template <typename T>
+void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G) {
+    bool &built = sizeof(T) == 8 ? G.built64 : G.built32;
+    if (built) return;
+    std::vector<DiagSweepParams<T>> ps;
+    size_t lds = 0;
+    for (int b : G.members) {
+        tsw_ctx *sub = ctx->batch[b].get();
+        T *q = (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
+        DiagSweepParams<T> sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, true, q);
+        lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, sub->g.p));
+        ps.push_back(sp);
+    }
+    // members whose tables do not fit fall back to elimination; a mixed group eliminates everywhere
+    bool all_mat = G.mat;
+    for (auto &sp : ps) all_mat = all_mat && sp.ainv != nullptr;
+    (sizeof(T) == 8 ? G.mat64 : G.mat32) = all_mat;
+    if (!all_mat) {
+        lds = 0;
+        for (size_t k = 0; k < ps.size(); ++k) {
+            tsw_ctx *sub = ctx->batch[G.members[k]].get();
+            ps[k].ainv = nullptr;
+            ps[k].mat = nullptr;
+            ps[k].M = 0;
+            ps[k].sig_t = (const T *)(sizeof(T) == 8 ? sub->diag.sigt64.ptr : sub->diag.sigt32.ptr);
+            lds = std::max(lds, diag_sweep_lds_bytes<T>(ps[k], sub->g.p));
+        }
+    }
+    require(lds != 0, TSW_E_UNSUPPORTED, "batch group does not fit the staggered sweep");
+    DBuf &buf = sizeof(T) == 8 ? G.params64 : G.params32;
+    TSW_HIP(hipMemcpyAsync(buf.reserve(sizeof(DiagSweepParams<T>) * ps.size()), ps.data(),
+                           sizeof(DiagSweepParams<T>) * ps.size(), hipMemcpyHostToDevice, ctx->stream));
+    TSW_HIP(hipStreamSynchronize(ctx->stream));  // ps goes out of scope
+    (sizeof(T) == 8 ? G.lds64 : G.lds32) = lds;
+    built = true;
+}
+
+template <typename T>
+void batch_sweep(tsw_ctx *ctx) {
+    for (auto &G : ctx->groups) batch_group_params<T>(ctx, G);
+    hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
+    TSW_HIP(hipEventRecord(e0, ctx->stream));
+    TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    int k = 0;
+    for (auto &G : ctx->groups) {
+        hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
+        TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
+        const DiagSweepParams<T> *probs = (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : G.params32.ptr);
+        // representative parameters for the host-side dispatch (mode and material path)
+        DiagSweepParams<T> rep{};
+        rep.qsrc = reinterpret_cast<const T *>(1);
+        rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
+        launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny,
+                                   sizeof(T) == 8 ? G.lds64 : G.lds32, G.p, sd);
+        ++k;
+    }
+    for (int j = 0; j < std::min<int>(k, tsw_ctx::kSideStreams); ++j) {
+        TSW_HIP(hipEventRecord(ctx->join_ev[j], ctx->side[j]));
+        TSW_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0));
+    }
+    TSW_HIP(hipEventRecord(e1, ctx->stream));
+    ++ctx->sweeps_done;
+}
+}  // namespace
+
+extern "C" {
+
+int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
+    return guarded(ctx, [&] {
+        require(!ctx->batch.empty(), TSW_E_STATE, "tsw_batch_set has not been called");
+        require(psi != nullptr, TSW_E_ARG, "null psi array list");
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        const bool f64 = precision == TSW_FP64;
+        std::vector<double *> dev(ctx->batch.size());
+        for (size_t b = 0; b < ctx->batch.size(); ++b) {
+            tsw_ctx *sub = ctx->batch[b].get();
+            dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
+            diag_load_source(sub, dev[b], f64, !f64);
+        }
+        if (f64) batch_sweep<double>(ctx);
+        else batch_sweep<float>(ctx);
+        for (size_t b = 0; b < ctx->batch.size(); ++b) {
+            tsw_ctx *sub = ctx->batch[b].get();
+            if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
+            else diag_store<float>(sub, sub->diag.q32.as<float>(), dev[b]);
+            if (mem == TSW_MEM_HOST) stage_out(sub, psi[b], dev[b], sub->g.npsi(), mem);
+        }
+        sync(ctx);
+    });
 }
 
 int tsw_sweep_timing(tsw_ctx *ctx, int reset, long long *n_sweeps, double *total_ms) {

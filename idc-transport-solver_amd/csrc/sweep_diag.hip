@@ -11,6 +11,8 @@
 //
 // The source of a cell is Q (optional, per ordinate) plus sigma_s * phi / (4 pi) (optional, the
 // isotropic scattering source of Smv, TransportOperator.cpp:369-392, fused into the load).
+#include <algorithm>
+
 #include "cell_solve.hpp"
 #include "diag_layout.hpp"
 #include "tsw_internal.hpp"
@@ -36,19 +38,22 @@ struct DiagLds {  // LDS carve-up in units of T
 
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
 template <typename T, int P, bool MAT, int MODE, int D>
-__global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp) {
+__global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
+                                                    const int2 *__restrict__ items) {
     constexpr int N = P * P;
+    // one problem (sp1, workgroup = ordinate) or a batch (items[workgroup] = {problem, ordinate})
+    const DiagSweepParams<T> sp = batch ? batch[items[blockIdx.x].x] : sp1;
+    const int q = batch ? items[blockIdx.x].y : (int)blockIdx.x;
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0;
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int nx = sp.nx, ny = sp.ny;
-    const int cthreads = (int)blockDim.x - 64;
+    const int cthreads = ((ny + 63) / 64) * 64;  // waves beyond this row count idle at the barriers
     const DiagLds<T, P> L(cthreads, nx, ny, MAT ? sp.M : 0);
     T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
     const DiagMap dm{nx, ny};
     const size_t ncell = (size_t)dm.cells();
 
-    const int q = blockIdx.x;
     const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
     const int tid = threadIdx.x;
     const T xi = sp.xi[q], eta = sp.eta[q];
@@ -66,10 +71,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp) {
     const int nsteps = ((steps + D - 1) / D) * D;  // padded steps have no active cell
     T *psi_q = sp.psi + (size_t)q * N * ncell;
 
-    if (tid >= cthreads) {
+    if (tid >= (int)blockDim.x - 64) {
         // storer: barrier k (k >= 1) closes compute step k-1, whose cells are written here while
         // the compute waves fill the other ring slot
-        const int lane = tid - cthreads;
+        const int lane = tid - ((int)blockDim.x - 64);
         lds_barrier();
         for (int st = 0; st < nsteps; ++st) {
             lds_barrier();
@@ -86,8 +91,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp) {
         return;
     }
 
-    const int r = tid;
-    const bool row_ok = r < ny;
+    const int r = tid < cthreads ? tid : cthreads - 1;  // spare waves mirror the last row, stay inactive
+    const bool row_ok = tid < ny;
     T tx[P];
 #pragma unroll
     for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? sp.tin_x[((size_t)q * ny + r) * P + d] : T(0);
@@ -221,13 +226,15 @@ size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
 }
 
 template <typename T, int P, bool MAT, int MODE>
-static void launch_ds(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s) {
+static void launch_ds(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch = nullptr,
+                      const int2 *items = nullptr, int nitems = 0, int max_ny = 0) {
     constexpr int D = sizeof(T) == 8 ? 2 : 3;
-    const int threads = ((sp.ny + 63) / 64) * 64 + 64;
+    const int rows = batch ? max_ny : sp.ny;
+    const int threads = ((rows + 63) / 64) * 64 + 64;
     auto kern = k_diag_sweep<T, P, MAT, MODE, D>;
     if (lds > 64 * 1024)
         TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(sp.nq), dim3(threads), lds, s, sp);
+    hipLaunchKernelGGL(kern, dim3(batch ? nitems : sp.nq), dim3(threads), lds, s, sp, batch, items);
     TSW_HIP(hipGetLastError());
 }
 
@@ -261,6 +268,39 @@ void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s) {
     }
 }
 
+// Batched sweep: one launch over work items {problem, ordinate} that share the DG degree, the mode
+// (which sources are present) and the material/elimination choice; `probs` are device copies.
+template <typename T>
+void launch_diag_sweep_batch(const DiagSweepParams<T> &rep, const DiagSweepParams<T> *probs, const int2 *items,
+                             int nitems, int max_ny, size_t lds, int p, hipStream_t s) {
+    const int mode = (rep.qsrc ? 1 : 0) | (rep.phi ? 2 : 0);
+    const bool mat = rep.ainv != nullptr;
+#define TSW_B(PP)                                                                                           \
+    do {                                                                                                    \
+        if (mat) {                                                                                          \
+            if (mode == 1) launch_ds<T, PP, true, 1>(rep, lds, s, probs, items, nitems, max_ny);            \
+            else if (mode == 2) launch_ds<T, PP, true, 2>(rep, lds, s, probs, items, nitems, max_ny);       \
+            else launch_ds<T, PP, true, 3>(rep, lds, s, probs, items, nitems, max_ny);                      \
+        } else {                                                                                            \
+            if (mode == 1) launch_ds<T, PP, false, 1>(rep, lds, s, probs, items, nitems, max_ny);           \
+            else if (mode == 2) launch_ds<T, PP, false, 2>(rep, lds, s, probs, items, nitems, max_ny);      \
+            else launch_ds<T, PP, false, 3>(rep, lds, s, probs, items, nitems, max_ny);                     \
+        }                                                                                                   \
+    } while (0)
+    switch (p) {
+        case 0: TSW_B(1); break;
+        case 1: TSW_B(2); break;
+        case 2: TSW_B(3); break;
+        case 3: TSW_B(4); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
+    }
+#undef TSW_B
+}
+template void launch_diag_sweep_batch<double>(const DiagSweepParams<double> &, const DiagSweepParams<double> *,
+                                              const int2 *, int, int, size_t, int, hipStream_t);
+template void launch_diag_sweep_batch<float>(const DiagSweepParams<float> &, const DiagSweepParams<float> *,
+                                             const int2 *, int, int, size_t, int, hipStream_t);
+
 template size_t diag_sweep_lds_bytes<double>(const DiagSweepParams<double> &, int);
 template size_t diag_sweep_lds_bytes<float>(const DiagSweepParams<float> &, int);
 template void launch_diag_sweep<double>(const DiagSweepParams<double> &, int, hipStream_t);
@@ -274,31 +314,68 @@ namespace {
 inline int rows_block(int ny) { return ny <= 64 ? 64 : (ny <= 128 ? 128 : (ny <= 256 ? 256 : 512)); }
 }
 
-// reference psi [i][j][q][k] (interior) <-> diagonal psi [q][k][pos]; dir 0: ref -> diag, 1: diag -> ref
+// reference psi [i][j][q][k] (interior) <-> diagonal psi [q][k][pos]; dir 0: ref -> diag, 1: diag -> ref.
+// A block takes R consecutive cells of one diagonal for the ordinates of one quadrant and transposes
+// them through LDS: on the reference side each cell's ordinates of the quadrant are one contiguous
+// run, on the diagonal side each (q, k) plane receives R consecutive positions.
 template <typename TR, typename TD>
-__global__ void k_psi_convert(TR *ref, TD *diag, DiagMap dm, int nq, int N, int qo1, int qo2, int qo3, int dir) {
-    const int s = blockIdx.x, q = blockIdx.y;
-    const int r = dm.rlo(s) + (int)threadIdx.x;
-    const int r1 = s < dm.ny - 1 ? s : dm.ny - 1;
-    if (r > r1) return;
-    const int Q = q >= qo3 ? 3 : (q >= qo2 ? 2 : (q >= qo1 ? 1 : 0));
-    int i, j;
-    dm.mesh(s - r, r, quad_xneg(Q), quad_yneg(Q), i, j);
+__global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap dm, int nq, int N, int qo1, int qo2,
+                                                     int qo3, int R, int dir) {
+    extern __shared__ unsigned char smem_raw[];
+    double *tile = reinterpret_cast<double *>(smem_raw);  // [R][nqQ*N]
+    const int s = blockIdx.x, Q = blockIdx.z;
+    const int r0 = dm.rlo(s) + (int)blockIdx.y * R;
+    const int rhi = s < dm.ny - 1 ? s : dm.ny - 1;
+    if (r0 > rhi) return;
+    const int nr = (rhi - r0 + 1) < R ? (rhi - r0 + 1) : R;
+    const int q0 = Q == 0 ? 0 : (Q == 1 ? qo1 : (Q == 2 ? qo2 : qo3));
+    const int q1 = Q == 0 ? qo1 : (Q == 1 ? qo2 : (Q == 2 ? qo3 : nq));
+    const int run = (q1 - q0) * N;  // values per cell on the reference side
     const size_t ncell = (size_t)dm.cells();
-    TR *pr = ref + (((size_t)i * (dm.ny + 2) + j) * nq + q) * N;
-    TD *pd = diag + (size_t)q * N * ncell + (size_t)dm.pos(s, r);
+    const long long p0 = dm.pos(s, r0);
+    const bool xn = quad_xneg(Q), yn = quad_yneg(Q);
+    auto ref_run = [&](int t) -> TR * {  // start of cell (s, r0 + t)'s run for this quadrant
+        int i, j;
+        dm.mesh(s - (r0 + t), r0 + t, xn, yn, i, j);
+        return ref + (((size_t)i * (dm.ny + 2) + j) * nq + q0) * N;
+    };
     if (dir == 0) {
-        for (int k = 0; k < N; ++k) pd[(size_t)k * ncell] = TD(pr[k]);
+        for (int t = 0; t < nr; ++t) {
+            const TR *src = ref_run(t);
+            for (int e = threadIdx.x; e < run; e += blockDim.x) tile[(size_t)t * run + e] = (double)src[e];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < run * nr; e += blockDim.x) {
+            const int t = e % nr, qk = e / nr;  // consecutive threads: consecutive positions
+            const int q = q0 + qk / N, k = qk % N;
+            diag[(size_t)q * N * ncell + (size_t)k * ncell + (size_t)(p0 + t)] = TD(tile[(size_t)t * run + qk]);
+        }
     } else {
-        for (int k = 0; k < N; ++k) pr[k] = TR(pd[(size_t)k * ncell]);
+        for (int e = threadIdx.x; e < run * nr; e += blockDim.x) {
+            const int t = e % nr, qk = e / nr;
+            const int q = q0 + qk / N, k = qk % N;
+            tile[(size_t)t * run + qk] = (double)diag[(size_t)q * N * ncell + (size_t)k * ncell + (size_t)(p0 + t)];
+        }
+        __syncthreads();
+        for (int t = 0; t < nr; ++t) {
+            TR *dst = ref_run(t);
+            for (int e = threadIdx.x; e < run; e += blockDim.x) dst[e] = TR(tile[(size_t)t * run + e]);
+        }
     }
 }
 
 template <typename TR, typename TD>
 void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s) {
     const DiagMap dm{g.nx, g.ny};
-    hipLaunchKernelGGL((k_psi_convert<TR, TD>), dim3(g.nx + g.ny - 1, g.nq), dim3(rows_block(g.ny)), 0, s, ref, diag,
-                       dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], dir);
+    int max_run = 0;
+    for (int Q = 0; Q < 4; ++Q) max_run = std::max(max_run, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
+    if (max_run == 0) return;
+    int R = 32;
+    while (R > 1 && (size_t)R * max_run * sizeof(double) > 32 * 1024) R /= 2;
+    const int maxlen = g.nx < g.ny ? g.nx : g.ny;
+    const dim3 grid(g.nx + g.ny - 1, (maxlen + R - 1) / R, 4);
+    hipLaunchKernelGGL((k_psi_convert<TR, TD>), grid, dim3(256), (size_t)R * max_run * sizeof(double), s, ref, diag,
+                       dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], R, dir);
     TSW_HIP(hipGetLastError());
 }
 template void launch_psi_convert<double, double>(const Geom &, double *, double *, int, hipStream_t);

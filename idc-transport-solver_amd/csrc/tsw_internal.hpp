@@ -36,6 +36,20 @@ struct DBuf {
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            bytes = o.bytes;
+            o.ptr = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DBuf() { release(); }
     void release() {
         if (ptr) (void)hipFree(ptr);
@@ -99,6 +113,11 @@ template <typename T>
 void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s);
 template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p);
+// Batched sweep over work items {problem, ordinate}; `rep` fixes degree-independent choices (mode,
+// material path), probs/items are device arrays, lds >= diag_sweep_lds_bytes of every problem.
+template <typename T>
+void launch_diag_sweep_batch(const DiagSweepParams<T> &rep, const DiagSweepParams<T> *probs, const int2 *items,
+                             int nitems, int max_ny, size_t lds, int p, hipStream_t s);
 // dir 0: reference psi -> diagonal psi; dir 1: diagonal -> reference (interior cells only)
 template <typename TR, typename TD>
 void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s);

@@ -31,6 +31,14 @@ MEM_HOST = 0
 MEM_DEVICE = 1
 
 
+class TswProblem(ctypes.Structure):
+    """tsw_problem (include/tsw.h)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("ax", ctypes.c_double), ("bx", ctypes.c_double),
+                ("ay", ctypes.c_double), ("by", ctypes.c_double), ("p", ctypes.c_int), ("nq", ctypes.c_int),
+                ("xi", ctypes.POINTER(ctypes.c_double)), ("eta", ctypes.POINTER(ctypes.c_double)),
+                ("w", ctypes.POINTER(ctypes.c_double)), ("quad_off", ctypes.c_int * 5)]
+
+
 class TswError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"tsw error {code}: {msg}")
@@ -74,6 +82,8 @@ def lib():
         L.tsw_gmres_solve.argtypes = [c, _vp, _vp, _vp, d, d, i, i, i, i, _vp, _ip, _dp]
         L.tsw_ordinates_level_symmetric.argtypes = [i, _dp, _dp, _dp, _ip]
         L.tsw_sweep_timing.argtypes = [c, i, ctypes.POINTER(ll), _dp]
+        L.tsw_batch_set.argtypes = [c, i, ctypes.POINTER(TswProblem), ctypes.POINTER(_vp), d, i]
+        L.tsw_batch_sweep.argtypes = [c, ctypes.POINTER(_vp), i, i]
         _lib = L
     return _lib
 
@@ -226,6 +236,40 @@ class Context:
         self._check(self._L.tsw_gmres_solve(self._h, a, b, c, dt, abs_tol, max_its, restart, precision, mem, d,
                                             ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
+
+    def batch_set(self, problems, sigma_t, dt=math.inf):
+        """problems: list of dicts with nx, ny, p, xi, eta, w, quad_off and optional ax, bx, ay, by;
+        sigma_t: list of cell arrays (numpy or device tensors)."""
+        self._batch_keep = []
+        arr = (TswProblem * len(problems))()
+        for k, pr in enumerate(problems):
+            xi = np.ascontiguousarray(pr["xi"], dtype=np.float64)
+            eta = np.ascontiguousarray(pr["eta"], dtype=np.float64)
+            w = np.ascontiguousarray(pr["w"], dtype=np.float64)
+            self._batch_keep += [xi, eta, w]
+            arr[k].nx, arr[k].ny, arr[k].p, arr[k].nq = pr["nx"], pr["ny"], pr["p"], xi.size
+            arr[k].ax, arr[k].bx = pr.get("ax", 0.0), pr.get("bx", 1.0)
+            arr[k].ay, arr[k].by = pr.get("ay", 0.0), pr.get("by", 1.0)
+            arr[k].xi = xi.ctypes.data_as(_dp)
+            arr[k].eta = eta.ctypes.data_as(_dp)
+            arr[k].w = w.ctypes.data_as(_dp)
+            for m in range(5):
+                arr[k].quad_off[m] = int(pr["quad_off"][m])
+        ptrs, mem = _addrs(*sigma_t)
+        parr = (_vp * len(ptrs))(*ptrs)
+        self._check(self._L.tsw_batch_set(self._h, len(problems), arr, parr, dt, mem))
+        self._batch_shapes = [((pr["nx"] + 2) * (pr["ny"] + 2) * np.asarray(pr["xi"]).size * (pr["p"] + 1) ** 2)
+                              for pr in problems]
+
+    def batch_sweep(self, psi_list, precision=FP64):
+        for a, n in zip(psi_list, self._batch_shapes):
+            m = a.size if isinstance(a, np.ndarray) else a.numel()
+            if m != n:
+                raise ValueError(f"batch psi array has {m} elements, its problem needs {n}")
+        ptrs, mem = _addrs(*psi_list)
+        parr = (_vp * len(ptrs))(*ptrs)
+        self._check(self._L.tsw_batch_sweep(self._h, parr, precision, mem))
+        return psi_list
 
     def sweep_timing(self, reset=False):
         n, ms = ctypes.c_longlong(0), ctypes.c_double(0.0)

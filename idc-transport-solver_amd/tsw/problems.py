@@ -32,11 +32,25 @@ class Problem:
     quad_off: np.ndarray
     sigma_t: np.ndarray   # [nx+2][ny+2]
     sigma_s: np.ndarray   # [nx+2][ny+2]
-    source: np.ndarray    # [nx+2][ny+2][nq][p+1][p+1], halos = inflow
+    q_cell: np.ndarray    # [nx+2][ny+2] isotropic source density per cell (halos: isotropic inflow density)
 
     @property
     def nq(self):
         return int(self.xi.size)
+
+    @property
+    def source(self):
+        """Angular source [nx+2][ny+2][nq][p+1][p+1]: q/(4 pi) in every ordinate's (0,0) coefficient."""
+        return _isotropic_source(self.q_cell, self.nq, self.p)
+
+    def source_device(self, device):
+        """The same angular source built directly in device memory (torch)."""
+        import torch
+        src = torch.zeros((self.nx + 2, self.ny + 2, self.nq, self.p + 1, self.p + 1), dtype=torch.float64,
+                          device=device)
+        q = torch.from_numpy(self.q_cell / (4.0 * math.pi)).to(device)
+        src[:, :, :, 0, 0] = q[:, :, None]
+        return src
 
     @property
     def unknowns(self):
@@ -79,8 +93,7 @@ def lattice(cells_per_unit=40, p=2, order=12) -> Problem:
     halo[1:-1, 1:-1] = False
     q[halo] = 0.0  # vacuum inflow
     xi, eta, w, off = level_symmetric(order)
-    return Problem("lattice", n, n, p, 0.0, 7.0, 0.0, 7.0, xi, eta, w, off, sigma_t, sigma_s,
-                   _isotropic_source(q, xi.size, p))
+    return Problem("lattice", n, n, p, 0.0, 7.0, 0.0, 7.0, xi, eta, w, off, sigma_t, sigma_s, q)
 
 
 def central_source(n=64, p=1, order=8, sigma_t=1.0, sigma_s=0.5) -> Problem:
@@ -92,5 +105,40 @@ def central_source(n=64, p=1, order=8, sigma_t=1.0, sigma_s=0.5) -> Problem:
     xi, eta, w, off = level_symmetric(order)
     st = np.full((n + 2, n + 2), sigma_t)
     ss = np.full((n + 2, n + 2), sigma_s)
-    return Problem("central-source", n, n, p, 0.0, 1.0, 0.0, 1.0, xi, eta, w, off, st, ss,
-                   _isotropic_source(q, xi.size, p))
+    return Problem("central-source", n, n, p, 0.0, 1.0, 0.0, 1.0, xi, eta, w, off, st, ss, q)
+
+
+def batched(seed=2026, count=256, n_range=(32, 256), degrees=(1, 2, 3), orders=(4, 6, 8, 10, 12)):
+    """Uneven batch of independent problems on the unit square, generated from `seed` with a
+    counter-based generator (numpy Philox): N ~ U{32..256} cells per side, DG degree ~ U{1,2,3},
+    S_N ~ U{4..12}; an 8x8 block pattern (block edges at floor(k N / 8)) with sigma_t log-uniform in
+    [0.1, 100] and source q ~ U[0,1] per block; one scattering ratio c ~ U[0, 0.999] per problem,
+    sigma_s = c sigma_t; vacuum boundaries."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    sets = {}
+    for b in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        p = int(rng.choice(degrees))
+        order = int(rng.choice(orders))
+        if order not in sets:
+            sets[order] = level_symmetric(order)
+        xi, eta, w, off = sets[order]
+        sig_blk = np.exp(rng.uniform(np.log(0.1), np.log(100.0), size=(8, 8)))
+        q_blk = rng.uniform(0.0, 1.0, size=(8, 8))
+        c = float(rng.uniform(0.0, 0.999))
+        edges = (np.arange(9) * n) // 8
+        blk = np.searchsorted(edges, np.arange(n), side="right") - 1  # block of each cell index
+        st = np.ones((n + 2, n + 2))
+        q = np.zeros((n + 2, n + 2))
+        st[1:-1, 1:-1] = sig_blk[blk[:, None], blk[None, :]]
+        q[1:-1, 1:-1] = q_blk[blk[:, None], blk[None, :]]
+        ss = c * st
+        out.append(Problem(f"batch{b}", n, n, p, 0.0, 1.0, 0.0, 1.0, xi, eta, w, off, st, ss, q))
+    return out
+
+
+def as_batch_member(P: Problem):
+    """The dict tsw.Context.batch_set expects for one problem."""
+    return {"nx": P.nx, "ny": P.ny, "p": P.p, "xi": P.xi, "eta": P.eta, "w": P.w, "quad_off": P.quad_off,
+            "ax": P.ax, "bx": P.bx, "ay": P.ay, "by": P.by}

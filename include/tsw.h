@@ -60,8 +60,14 @@ int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double a
                     const double *xi, const double *eta, const double *w, const int *quad_off);
 
 /* Selects the traversal, as the `sweep_pattern` factory key does (SweepPattern.cpp:314-359):
- *   "hip-wavefront"  one launch per diagonal i+j = const, all quadrants per launch;
- *   "hip-pipelined"  one persistent launch per sweep, row bands pipelined through device flags.
+ *   "hip-wavefront"        one launch per diagonal i+j = const, all quadrants per launch, replayed
+ *                          from a captured HIP graph; reference layout;
+ *   "hip-wavefront-eager"  the same launches issued directly;
+ *   "hip-staggered"        (default) one launch per sweep, one workgroup per ordinate, a diagonal per
+ *                          step, on a diagonal-major device layout; element solves by precomputed
+ *                          inverses for piecewise-constant materials (<= 64 distinct sigma_t);
+ *   "hip-staggered-ge"     the same with per-cell elimination.
+ * Meshes the staggered kernel cannot hold (ny > 448) run the wavefront kernels.
  * Unknown names fail with TSW_E_ARG as the reference's factory throws std::out_of_range. */
 int tsw_set_pattern(tsw_ctx *ctx, const char *name);
 
@@ -108,6 +114,28 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
  * (OrdinateSet.cpp:40-91) and SymmetricReduce (:1171-1190).  Host-only, needs no device.
  * Arrays must hold nq entries, quad_off 5. */
 int tsw_ordinates_level_symmetric(int order, double *xi, double *eta, double *w, int *quad_off);
+
+/* One member of a batch of independent problems (mesh, DG degree, ordinate set; see tsw_problem_set). */
+typedef struct tsw_problem {
+    int nx, ny;
+    double ax, bx, ay, by;
+    int p, nq;
+    const double *xi, *eta, *w; /* host arrays of nq entries */
+    int quad_off[5];
+} tsw_problem;
+
+/* Installs a batch of `nprob` independent problems of uneven size, DG degree and ordinate set, with
+ * their total cross sections (sigma_t[b]: [nx_b+2][ny_b+2], host or device per `mem`) and time step.
+ * Replaces a loop of SweepOperator constructions + Apply calls over many problem instances
+ * (SweepOperator.cpp:25-49,118-135); the batch keeps the cross sections' material tables.
+ * Needs ny_b <= 448 for every member. */
+int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const double *const *sigma_t, double dt,
+                  int mem);
+
+/* Sweeps every member of the batch in place (psi[b] in the reference layout, halos = inflow), in
+ * one set of concurrent launches (one launch per DG degree and row count, workgroups = ordinates of
+ * all members).  SweepOperator::Apply semantics per member.  Timed as one sweep by tsw_sweep_timing. */
+int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem);
 
 /* Timing of the sweeps run on ctx since the last reset, measured with HIP events on the context's
  * stream around each sweep: count, total milliseconds.  Replaces the TMR_AF_Lsv wall timer
