@@ -79,6 +79,11 @@ struct tsw_ctx {
         size_t lds64 = 0, lds32 = 0;
     };
     std::vector<Group> groups;
+    // Adjoint problem: the same mesh with the negated ordinate set in reverse order (its sweeps are the
+    // transposed sweeps of this problem up to the Legendre mass scaling), sharing this stream; built on
+    // the first vector-Jacobian product.  Staging of the vector-Jacobian products' arrays.
+    std::unique_ptr<tsw_ctx> adj;
+    DBuf vjp_psi, vjp_seed, vjp_gsrc, vjp_gst, vjp_gss, vjp_y, vjp_phi;
     static constexpr int kSideStreams = 4;
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
@@ -92,6 +97,7 @@ struct tsw_ctx {
             if (e) (void)hipEventDestroy(e);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         batch.clear();
+        adj.reset();
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -414,7 +420,7 @@ void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const 
                 bool need32) {
     const Geom &g = c->g;
     auto &d = c->diag;
-    const size_t ncell = (size_t)g.nx * g.ny, N = g.N();
+    const size_t ncell = (size_t)g.nx * g.ny;
     hipStream_t s = c->stream;
     prepare_materials(c, sigma_t, dt);
     if (c->mat.valid)
@@ -677,6 +683,128 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
     return converged;
 }
 
+// Peierls GMRES on the wavefront kernels (meshes the staggered kernel cannot hold).  Q: reference-layout
+// source with inflow halos (device), overwritten by psi; pd (may be null): device phi output.
+bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss, double dt, double abs_tol,
+                     int max_its, int restart, int precision, double *pd, int &its_total, double &res_final) {
+    const Geom &g = ctx->g;
+    hipStream_t s = ctx->stream;
+    const size_t n = g.nphi(), npsi = g.npsi();
+    const double *w = ctx->w64.as<double>();
+    bool converged = false;
+    its_total = 0;
+    double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
+    // vectors: b, x, r (phi-shaped, zero halos)
+    DBuf vecs;
+    double *b = static_cast<double *>(vecs.reserve(sizeof(double) * n * 3));
+    double *x = b + n, *r = x + n;
+    TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n * 3, s));
+    Gmres solver(ctx, n, restart);
+
+    // matvec y = (I - P L^{-1} S) x with zero inflow (PETScPeierlsSolver.cpp:548-555)
+    auto apply = [&](const double *xv, double *yv) {
+        launch_smv<double>(g, ss, xv, psi, 1.0, 0.0, 1, s);
+        sweep_prec(ctx, psi, st, dt, precision);
+        TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        launch_pmv<double, double>(g, psi, yv, w, -1.0, 1.0, 0, s);
+    };
+
+    if (precision == TSW_FP64) {
+        // b = P L^{-1} Q (:461-466)
+        TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+        sweep_prec(ctx, psi, st, dt, precision);
+        launch_pmv<double, double>(g, psi, b, w, 1.0, 0.0, 0, s);
+        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
+        // source <- L^{-1}(Q + S phi) (:504-505)
+        launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
+        sweep_prec(ctx, Q, st, dt, precision);
+    } else {
+        // Mixed precision: an FP64 iterate psi64 is refined with corrections computed from FP32
+        // sweeps.  The residual r = Q + S P psi - L psi is formed in FP64 with Lmv; the correction
+        // solves (L - S P) d = r on the Peierls form with FP32 sweeps inside FP64 GMRES.
+        const size_t ncell = g.ncell_h();
+        const int N = g.N();
+        double *psi64 = static_cast<double *>(ctx->ir_state.reserve(sizeof(double) * npsi));
+        double *res64 = static_cast<double *>(ctx->r64.reserve(sizeof(double) * npsi));
+        float *p32 = static_cast<float *>(ctx->psi32.reserve(sizeof(float) * npsi));
+        float *phi32 = static_cast<float *>(ctx->phi32.reserve(sizeof(float) * n));
+        const float *st32 = as_f32(ctx, st, ncell, ctx->sig_t32);
+        const float *ss32 = as_f32(ctx, ss, ncell, ctx->sig_s32);
+        const double *xi64 = ctx->xi64.as<double>(), *eta64 = ctx->eta64.as<double>();
+
+        // y = (I - P L^{-1} S) x with FP32 sweeps
+        auto apply32 = [&](const double *xv, double *yv) {
+            launch_convert_d2f(xv, phi32, n, s);
+            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 0.0f, 1, s);
+            sweep_dev<float>(ctx, p32, st32, dt);
+            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            launch_pmv<float, double>(g, p32, yv, w, -1.0, 1.0, 0, s);
+        };
+
+        // initial iterate: inflow halos from Q, zero interior
+        TSW_HIP(hipMemcpyAsync(psi64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+        launch_fill_region<double>(g, psi64, (size_t)g.nq * N, 0, 0.0, s);
+        const int max_refine = 30;
+        for (int refine = 0;; ++refine) {
+            // r = Q + S P psi - L psi on the interior, zero halos
+            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
+            TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+            launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s);
+            launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s);
+            launch_fill_region<double>(g, res64, (size_t)g.nq * N, 1, 0.0, s);
+            // b = P L^{-1} r: the Peierls residual of the current iterate
+            launch_convert_d2f(res64, p32, npsi, s);
+            sweep_dev<float>(ctx, p32, st32, dt);
+            TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
+            launch_pmv<float, double>(g, p32, b, w, 1.0, 0.0, 0, s);
+            res_final = solver.norm(b);
+            if (res_final <= abs_tol) {
+                converged = true;
+                break;
+            }
+            if (refine >= max_refine || its_total >= max_its) break;
+            // correction: (I - P L^{-1} S) phi_d = b, then d = L^{-1}(r + S phi_d)
+            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
+            int its = 0;
+            double rr = 0.0;
+            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
+            its_total += its;
+            launch_convert_d2f(res64, p32, npsi, s);
+            launch_convert_d2f(x, phi32, n, s);
+            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 1.0f, 1, s);
+            sweep_dev<float>(ctx, p32, st32, dt);
+            launch_accumulate_f2d(p32, psi64, npsi, s);
+        }
+        // outputs: psi and phi = P psi
+        TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+        TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+        launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
+    }
+    if (pd) TSW_HIP(hipMemcpyAsync(pd, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    return converged;
+}
+
+// Peierls GMRES on device arrays with the context's engine.
+bool gmres_device(tsw_ctx *ctx, double *Q, const double *st, const double *ss, double dt, double abs_tol, int max_its,
+                  int restart, int precision, double *pd, int &its_total, double &res_final) {
+    if (staggered_ok(ctx))
+        return gmres_staggered(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
+    return gmres_wavefront(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
+}
+
+// In-place sweep of a reference-layout device array with the context's engine.
+void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int precision) {
+    require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+    if (staggered_ok(ctx)) {
+        if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
+        else apply_staggered<float>(ctx, psi, sig, dt);
+    } else {
+        sweep_prec(ctx, psi, sig, dt, precision);
+    }
+}
+
 // Validates and installs a mesh + ordinate set (tsw_problem_set, also used per batch member).
 void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
                  const double *xi, const double *eta, const double *w, const int *quad_off) {
@@ -695,6 +823,7 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
         }
     }
     ctx->clear_graphs();
+    ctx->adj.reset();
     Geom &g = ctx->g;
     g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
     g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
@@ -719,6 +848,88 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
     TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
     ctx->have_problem = true;
+}
+
+// The adjoint problem of ctx (built on first use): same mesh, ordinates -Omega in reverse order with
+// their weights, so quadrant k of ctx becomes quadrant 3-k and the blocks stay ordered.
+tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
+    if (!ctx->adj) {
+        const Geom &g = ctx->g;
+        const int nq = g.nq;
+        std::vector<double> xi(nq), eta(nq), w(nq);
+        for (int q = 0; q < nq; ++q) {
+            xi[q] = -ctx->h_xi[nq - 1 - q];
+            eta[q] = -ctx->h_eta[nq - 1 - q];
+            w[q] = ctx->h_w[nq - 1 - q];
+        }
+        int off[5];
+        for (int k = 0; k < 5; ++k) off[k] = nq - g.quad_off[4 - k];
+        auto a = std::make_unique<tsw_ctx>();
+        a->device = ctx->device;
+        a->stream = ctx->stream;
+        a->own_stream = false;
+        a->pattern = ctx->pattern;
+        set_problem(a.get(), g.nx, g.ny, g.ax, g.bx, g.ay, g.by, g.p, nq, xi.data(), eta.data(), w.data(), off);
+        ctx->adj = std::move(a);
+    }
+    ctx->adj->pattern = ctx->pattern;
+    return ctx->adj.get();
+}
+
+// Gradients of J = <seed, psi> once the adjoint flux zeta sits in gsrc's interior: cross-section
+// gradients per cell (gst/gss may be null), then the inflow gradient added into gsrc's halos (which
+// hold seed's halos: the forward map passes them through).
+void vjp_finish(tsw_ctx *ctx, const double *psi, const double *phi, double *gsrc, double *gst, double *gss) {
+    const Geom &g = ctx->g;
+    hipStream_t s = ctx->stream;
+    if (gst) TSW_HIP(hipMemsetAsync(gst, 0, sizeof(double) * g.ncell_h(), s));
+    if (gss) TSW_HIP(hipMemsetAsync(gss, 0, sizeof(double) * g.ncell_h(), s));
+    if (gst || gss) launch_cell_grads(g, psi, gsrc, phi, gst, gss, s);
+    launch_inflow_grad(g, gsrc, gsrc, ctx->xi64.as<double>(), ctx->eta64.as<double>(), s);
+}
+
+// Vector-Jacobian product of the in-place sweep (device arrays).  zeta = L^{-T} seed is computed as
+// W^{-1} R sweep_adj(R W seed): R reverses the ordinate order, W scales coefficient (d,e) by (2d+1)(2e+1),
+// sweep_adj is the adjoint problem's forward sweep with vacuum inflow.
+void sweep_vjp_device(tsw_ctx *ctx, const double *psi, const double *st, double dt, const double *seed, double *gsrc,
+                      double *gst, int precision) {
+    const Geom &g = ctx->g;
+    hipStream_t s = ctx->stream;
+    tsw_ctx *a = adjoint_ctx(ctx);
+    double *y = dbuf<double>(ctx->vjp_y, g.npsi());
+    TSW_HIP(hipMemsetAsync(y, 0, sizeof(double) * g.npsi(), s));
+    launch_reverse_ordinates(g, seed, y, 1, nullptr, 0, false, s);
+    sweep_device(a, y, st, dt, precision);
+    if (gsrc != seed) TSW_HIP(hipMemcpyAsync(gsrc, seed, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    launch_reverse_ordinates(g, y, gsrc, 2, nullptr, 0, false, s);
+    vjp_finish(ctx, psi, nullptr, gsrc, gst, nullptr);
+}
+
+// Vector-Jacobian product of the scattering solve psi = (L - K)^{-1} Q, K psi = sigma_s P psi / (4 pi):
+// zeta = L^{-T}(seed + K^T zeta), (K^T zeta)_q = w_q sigma_s sum_q' zeta_q' / (4 pi).  With chi_q = zeta_q / w_q
+// and chi' = R W chi this is the adjoint problem's own scattering solve with source R W (seed_q / w_q), run
+// by the same Peierls GMRES.  Returns convergence.
+bool solve_vjp_device(tsw_ctx *ctx, const double *psi, const double *st, const double *ss, double dt,
+                      const double *seed, double *gsrc, double *gst, double *gss, double abs_tol, int max_its,
+                      int restart, int precision, int &its, double &res) {
+    const Geom &g = ctx->g;
+    hipStream_t s = ctx->stream;
+    tsw_ctx *a = adjoint_ctx(ctx);
+    const double *w = ctx->w64.as<double>();
+    double *y = dbuf<double>(ctx->vjp_y, g.npsi());
+    TSW_HIP(hipMemsetAsync(y, 0, sizeof(double) * g.npsi(), s));
+    launch_reverse_ordinates(g, seed, y, 1, w, 2, false, s);
+    const bool ok = gmres_device(a, y, st, ss, dt, abs_tol, max_its, restart, precision, nullptr, its, res);
+    if (gsrc != seed) TSW_HIP(hipMemcpyAsync(gsrc, seed, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    launch_reverse_ordinates(g, y, gsrc, 2, w, 1, true, s);
+    double *phi = nullptr;
+    if (gss) {
+        phi = dbuf<double>(ctx->vjp_phi, g.nphi());
+        TSW_HIP(hipMemsetAsync(phi, 0, sizeof(double) * g.nphi(), s));
+        launch_pmv<double, double>(g, psi, phi, w, 1.0, 0.0, 0, s);
+    }
+    vjp_finish(ctx, psi, phi, gsrc, gst, gss);
+    return ok;
 }
 
 }  // namespace
@@ -777,12 +988,7 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
         const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        if (staggered_ok(ctx)) {
-            if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
-            else apply_staggered<float>(ctx, psi, sig, dt);
-        } else {
-            sweep_prec(ctx, psi, sig, dt, precision);
-        }
+        sweep_device(ctx, psi, sig, dt, precision);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
         sync(ctx);
     });
@@ -892,113 +1098,11 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
         double *Q = stage_in(ctx, source_inout, npsi, mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
-        const double *w = ctx->w64.as<double>();
-        if (staggered_ok(ctx)) {
-            DBuf phi_dev;
-            double *pd = phi_out ? (mem == TSW_MEM_DEVICE ? phi_out : static_cast<double *>(phi_dev.reserve(sizeof(double) * n))) : nullptr;
-            converged = gmres_staggered(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
-            if (phi_out && mem == TSW_MEM_HOST)
-                TSW_HIP(hipMemcpyAsync(phi_out, pd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            if (mem == TSW_MEM_HOST) stage_out(ctx, source_inout, Q, npsi, mem);
-            sync(ctx);
-            return;
-        }
-        double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
-        // vectors: b, x, r (phi-shaped, zero halos)
-        DBuf vecs;
-        double *b = static_cast<double *>(vecs.reserve(sizeof(double) * n * 3));
-        double *x = b + n, *r = x + n;
-        TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n * 3, s));
-        Gmres solver(ctx, n, restart);
-
-        // matvec y = (I - P L^{-1} S) x with zero inflow (PETScPeierlsSolver.cpp:548-555)
-        auto apply = [&](const double *xv, double *yv) {
-            launch_smv<double>(g, ss, xv, psi, 1.0, 0.0, 1, s);
-            sweep_prec(ctx, psi, st, dt, precision);
-            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            launch_pmv<double, double>(g, psi, yv, w, -1.0, 1.0, 0, s);
-        };
-
-        if (precision == TSW_FP64) {
-            // b = P L^{-1} Q (:461-466)
-            TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-            sweep_prec(ctx, psi, st, dt, precision);
-            launch_pmv<double, double>(g, psi, b, w, 1.0, 0.0, 0, s);
-            converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
-            // source <- L^{-1}(Q + S phi) (:504-505)
-            launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
-            sweep_prec(ctx, Q, st, dt, precision);
-        } else {
-            // Mixed precision: an FP64 iterate psi64 is refined with corrections computed from FP32
-            // sweeps.  The residual r = Q + S P psi - L psi is formed in FP64 with Lmv; the correction
-            // solves (L - S P) d = r on the Peierls form with FP32 sweeps inside FP64 GMRES.
-            const size_t ncell = g.ncell_h();
-            const int N = g.N();
-            double *psi64 = static_cast<double *>(ctx->ir_state.reserve(sizeof(double) * npsi));
-            double *res64 = static_cast<double *>(ctx->r64.reserve(sizeof(double) * npsi));
-            float *p32 = static_cast<float *>(ctx->psi32.reserve(sizeof(float) * npsi));
-            float *phi32 = static_cast<float *>(ctx->phi32.reserve(sizeof(float) * n));
-            const float *st32 = as_f32(ctx, st, ncell, ctx->sig_t32);
-            const float *ss32 = as_f32(ctx, ss, ncell, ctx->sig_s32);
-            const double *xi64 = ctx->xi64.as<double>(), *eta64 = ctx->eta64.as<double>();
-
-            // y = (I - P L^{-1} S) x with FP32 sweeps
-            auto apply32 = [&](const double *xv, double *yv) {
-                launch_convert_d2f(xv, phi32, n, s);
-                launch_smv<float>(g, ss32, phi32, p32, 1.0f, 0.0f, 1, s);
-                sweep_dev<float>(ctx, p32, st32, dt);
-                TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-                launch_pmv<float, double>(g, p32, yv, w, -1.0, 1.0, 0, s);
-            };
-
-            // initial iterate: inflow halos from Q, zero interior
-            TSW_HIP(hipMemcpyAsync(psi64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-            launch_fill_region<double>(g, psi64, (size_t)g.nq * N, 0, 0.0, s);
-            const int max_refine = 30;
-            for (int refine = 0;; ++refine) {
-                // r = Q + S P psi - L psi on the interior, zero halos
-                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-                launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
-                TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-                launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s);
-                launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s);
-                launch_fill_region<double>(g, res64, (size_t)g.nq * N, 1, 0.0, s);
-                // b = P L^{-1} r: the Peierls residual of the current iterate
-                launch_convert_d2f(res64, p32, npsi, s);
-                sweep_dev<float>(ctx, p32, st32, dt);
-                TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
-                launch_pmv<float, double>(g, p32, b, w, 1.0, 0.0, 0, s);
-                res_final = solver.norm(b);
-                if (res_final <= abs_tol) {
-                    converged = true;
-                    break;
-                }
-                if (refine >= max_refine || its_total >= max_its) break;
-                // correction: (I - P L^{-1} S) phi_d = b, then d = L^{-1}(r + S phi_d)
-                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-                const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
-                int its = 0;
-                double rr = 0.0;
-                solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
-                its_total += its;
-                launch_convert_d2f(res64, p32, npsi, s);
-                launch_convert_d2f(x, phi32, n, s);
-                launch_smv<float>(g, ss32, phi32, p32, 1.0f, 1.0f, 1, s);
-                sweep_dev<float>(ctx, p32, st32, dt);
-                launch_accumulate_f2d(p32, psi64, npsi, s);
-            }
-            // outputs: psi and phi = P psi
-            TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-            TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-            launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
-        }
-        if (phi_out) {
-            if (mem == TSW_MEM_HOST) {
-                TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            } else {
-                TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            }
-        }
+        DBuf phi_dev;
+        double *pd = phi_out ? (mem == TSW_MEM_DEVICE ? phi_out : static_cast<double *>(phi_dev.reserve(sizeof(double) * n))) : nullptr;
+        converged = gmres_device(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
+        if (phi_out && mem == TSW_MEM_HOST)
+            TSW_HIP(hipMemcpyAsync(phi_out, pd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         if (mem == TSW_MEM_HOST) stage_out(ctx, source_inout, Q, npsi, mem);
         sync(ctx);
     });
@@ -1006,6 +1110,68 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
     if (resid) *resid = res_final;
     if (rc == TSW_OK && !converged) {
         ctx->err = "GMRES transport solve failed to converge";
+        return TSW_E_NOCONV;
+    }
+    return rc;
+}
+
+int tsw_sweep_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, double dt, const double *seed,
+                  double *grad_src, double *grad_sigma_t, int precision, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        require(seed != nullptr && grad_src != nullptr, TSW_E_ARG, "seed and grad_src are required");
+        require(grad_sigma_t == nullptr || psi != nullptr, TSW_E_ARG, "grad_sigma_t needs the forward psi");
+        const Geom &g = ctx->g;
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *sd = stage_in(ctx, seed, g.npsi(), mem, ctx->vjp_seed);
+        const double *pd = psi ? stage_in(ctx, psi, g.npsi(), mem, ctx->vjp_psi) : nullptr;
+        double *gsrc = mem == TSW_MEM_DEVICE ? grad_src : dbuf<double>(ctx->vjp_gsrc, g.npsi());
+        double *gst = grad_sigma_t ? (mem == TSW_MEM_DEVICE ? grad_sigma_t : dbuf<double>(ctx->vjp_gst, g.ncell_h()))
+                                   : nullptr;
+        sweep_vjp_device(ctx, pd, st, dt, sd, gsrc, gst, precision);
+        if (mem == TSW_MEM_HOST) {
+            stage_out(ctx, grad_src, gsrc, g.npsi(), mem);
+            if (gst) stage_out(ctx, grad_sigma_t, gst, g.ncell_h(), mem);
+        }
+        sync(ctx);
+    });
+}
+
+int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const double *sigma_s, double dt,
+                  const double *seed, double *grad_src, double *grad_sigma_t, double *grad_sigma_s, double abs_tol,
+                  int max_its, int restart, int precision, int mem, int *iters, double *resid) {
+    int its = 0;
+    double res = 0.0;
+    bool converged = false;
+    const int rc = guarded(ctx, [&] {
+        need_problem(ctx);
+        require(seed != nullptr && grad_src != nullptr, TSW_E_ARG, "seed and grad_src are required");
+        require((grad_sigma_t == nullptr && grad_sigma_s == nullptr) || psi != nullptr, TSW_E_ARG,
+                "cross-section gradients need the forward psi");
+        require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
+        const Geom &g = ctx->g;
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *sd = stage_in(ctx, seed, g.npsi(), mem, ctx->vjp_seed);
+        const double *pd = psi ? stage_in(ctx, psi, g.npsi(), mem, ctx->vjp_psi) : nullptr;
+        double *gsrc = mem == TSW_MEM_DEVICE ? grad_src : dbuf<double>(ctx->vjp_gsrc, g.npsi());
+        auto cellbuf = [&](double *user, DBuf &b) {
+            return user ? (mem == TSW_MEM_DEVICE ? user : dbuf<double>(b, g.ncell_h())) : nullptr;
+        };
+        double *gst = cellbuf(grad_sigma_t, ctx->vjp_gst), *gss = cellbuf(grad_sigma_s, ctx->vjp_gss);
+        converged = solve_vjp_device(ctx, pd, st, ss, dt, sd, gsrc, gst, gss, abs_tol, max_its, restart, precision,
+                                     its, res);
+        if (mem == TSW_MEM_HOST) {
+            stage_out(ctx, grad_src, gsrc, g.npsi(), mem);
+            if (gst) stage_out(ctx, grad_sigma_t, gst, g.ncell_h(), mem);
+            if (gss) stage_out(ctx, grad_sigma_s, gss, g.ncell_h(), mem);
+        }
+        sync(ctx);
+    });
+    if (iters) *iters = its;
+    if (resid) *resid = res;
+    if (rc == TSW_OK && !converged) {
+        ctx->err = "adjoint GMRES transport solve failed to converge";
         return TSW_E_NOCONV;
     }
     return rc;

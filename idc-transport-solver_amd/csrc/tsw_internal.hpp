@@ -174,4 +174,12 @@ void launch_sqdist_interior(const Geom &g, const double *a, const double *b, dou
                             int nblocks, hipStream_t s);
 int reduction_blocks();
 
+// adjoint path (adjoint.hip)
+void launch_reverse_ordinates(const Geom &g, const double *in, double *out, int kmode, const double *w, int qmode,
+                              bool w_of_out, hipStream_t s);
+void launch_cell_grads(const Geom &g, const double *psi, const double *zeta, const double *phi, double *gst,
+                       double *gss, hipStream_t s);
+void launch_inflow_grad(const Geom &g, const double *zeta, double *out, const double *xi, const double *eta,
+                        hipStream_t s);
+
 }  // namespace tsw

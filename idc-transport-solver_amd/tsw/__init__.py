@@ -80,6 +80,8 @@ def lib():
         L.tsw_lmv.argtypes = [c, d, d, _vp, _vp, _vp, i]
         L.tsw_si_solve.argtypes = [c, _vp, _vp, _vp, d, d, i, i, i, _ip, _dp]
         L.tsw_gmres_solve.argtypes = [c, _vp, _vp, _vp, d, d, i, i, i, i, _vp, _ip, _dp]
+        L.tsw_sweep_vjp.argtypes = [c, _vp, _vp, d, _vp, _vp, _vp, i, i]
+        L.tsw_solve_vjp.argtypes = [c, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, d, i, i, i, i, _ip, _dp]
         L.tsw_ordinates_level_symmetric.argtypes = [i, _dp, _dp, _dp, _ip]
         L.tsw_sweep_timing.argtypes = [c, i, ctypes.POINTER(ll), _dp]
         L.tsw_batch_set.argtypes = [c, i, ctypes.POINTER(TswProblem), ctypes.POINTER(_vp), d, i]
@@ -128,7 +130,12 @@ def _addrs(*arrays):
         mems.add(m)
     if len(mems) > 1:
         raise TypeError("all arrays of one call must be on the host or all on the device")
-    return out, (mems.pop() if mems else MEM_HOST)
+    mem = mems.pop() if mems else MEM_HOST
+    if mem == MEM_DEVICE:
+        # the library works on its own stream: finish what torch queued on the inputs first
+        import torch
+        torch.cuda.current_stream().synchronize()
+    return out, mem
 
 
 class Context:
@@ -235,6 +242,24 @@ class Context:
         it, res = ctypes.c_int(0), ctypes.c_double(0.0)
         self._check(self._L.tsw_gmres_solve(self._h, a, b, c, dt, abs_tol, max_its, restart, precision, mem, d,
                                             ctypes.byref(it), ctypes.byref(res)))
+        return it.value, res.value
+
+    def sweep_vjp(self, seed, grad_src, sigma_t, psi=None, grad_sigma_t=None, dt=math.inf, precision=FP64):
+        """grad_src <- d<seed, psi>/d(source, inflow); grad_sigma_t <- d/dsigma_t (needs the forward psi)."""
+        self._expect(psi=seed, psi2=grad_src, psi3=psi, cell=sigma_t, cell2=grad_sigma_t)
+        (a, b, c, d, e), mem = _addrs(psi, sigma_t, seed, grad_src, grad_sigma_t)
+        self._check(self._L.tsw_sweep_vjp(self._h, a, b, dt, c, d, e, precision, mem))
+        return grad_src
+
+    def solve_vjp(self, seed, grad_src, sigma_t, sigma_s, psi=None, grad_sigma_t=None, grad_sigma_s=None,
+                  dt=math.inf, abs_tol=1e-10, max_its=3000, restart=30, precision=FP64):
+        """Adjoint of the scattering solve; returns (iterations, residual) of the adjoint GMRES."""
+        self._expect(psi=seed, psi2=grad_src, psi3=psi, cell=sigma_t, cell2=sigma_s, cell3=grad_sigma_t,
+                     cell4=grad_sigma_s)
+        (a, b, c, d, e, f, h), mem = _addrs(psi, sigma_t, sigma_s, seed, grad_src, grad_sigma_t, grad_sigma_s)
+        it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+        self._check(self._L.tsw_solve_vjp(self._h, a, b, c, dt, d, e, f, h, abs_tol, max_its, restart, precision,
+                                          mem, ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
 
     def batch_set(self, problems, sigma_t, dt=math.inf):

@@ -108,6 +108,30 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
                     double abs_tol, int max_its, int restart, int precision, int mem, double *phi_out,
                     int *iters, double *resid);
 
+/* Vector-Jacobian product of tsw_sweep_apply as a map (source with inflow halos, sigma_t) -> psi, for
+ * J = <seed, psi> over every coefficient of the array.  The reference has no adjoint; this is the exact
+ * transpose of the system its sweep solves (SweepLinearSolveSpec.cpp:284-298 matrix, :473-598 right-hand
+ * side, with the upwind couplings transposed).  Outputs:
+ *   grad_src   (psi layout, required): interior = zeta = L^{-T} seed; halo cells = seed's halo (the sweep
+ *              passes halos through) + the gradient with respect to the inflow values;
+ *   grad_sigma_t (cell array, may be NULL): interior = -sum_{q,d,e} zeta psi, halo cells 0; needs `psi`,
+ *              the forward sweep's output.
+ * seed's interior and halo cells are both read; grad_src may alias seed.  precision as tsw_sweep_apply. */
+int tsw_sweep_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, double dt, const double *seed,
+                  double *grad_src, double *grad_sigma_t, int precision, int mem);
+
+/* Vector-Jacobian product of the scattering solve (tsw_gmres_solve / tsw_si_solve) as a map
+ * (Q with inflow halos, sigma_t, sigma_s) -> psi: the adjoint problem
+ * zeta = L^{-T}(seed + K^T zeta), (K^T zeta)_q = w_q sigma_s sum_q' zeta_q' / (4 pi), is solved by the same
+ * Peierls GMRES (restart, abs_tol, max_its, precision as tsw_gmres_solve; iters/resid may be NULL).
+ *   grad_src: interior = zeta = dJ/dQ, halos = seed's halo + dJ/d(inflow);
+ *   grad_sigma_t (may be NULL): -sum zeta psi;  grad_sigma_s (may be NULL): sum_{q,k} zeta_qk phi_k / (4 pi),
+ *   phi = P psi.  Both need `psi`, the converged forward solution.
+ * Returns TSW_E_NOCONV if the adjoint solve does not converge. */
+int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const double *sigma_s, double dt,
+                  const double *seed, double *grad_src, double *grad_sigma_t, double *grad_sigma_s, double abs_tol,
+                  int max_its, int restart, int precision, int mem, int *iters, double *resid);
+
 /* Level-symmetric S_order ordinates reduced to the 2D hemisphere (order in {2,4,6,8,10,12,16}):
  * nq = order*(order+2)/2 directions, weights summing to 4 pi, grouped by quadrant as above.
  * The reference has no level-symmetric set; this plays the role of OrdinateSet's constructors

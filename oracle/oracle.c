@@ -170,6 +170,83 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
 }
 
 /* ---------------------------------------------------------------------------------------------- */
+/* Transposed sweep.  The reference has no adjoint; this is the transpose of the linear system     */
+/* that orc_sweep solves, written out directly.  With row (d,e) of cell c's equation                */
+/*   A_c psi_c - Cx psi_xup - Cy psi_yup = dx dy src_c,                                            */
+/*   Cx[(d,e),(a,e)] = dy xi (2d+1) kx(d,a),  kx = (-1)^(a+1) if xi <= 0 else (-1)^d,             */
+/*   Cy[(d,e),(d,b)] = dx eta (2e+1) ky(e,b), ky = (-1)^(b+1) if eta <= 0 else (-1)^e             */
+/* (CalcB, SweepLinearSolveSpec.cpp:545-598), the transpose is solved cell by cell in reverse      */
+/* sweep order:  A_c^T mu_c = g_c + Cx^T mu_xdown + Cy^T mu_ydown.                                  */
+/* ---------------------------------------------------------------------------------------------- */
+static inline double kx_coef(int neg, int d, int a) { return neg ? sgn_pow(a + 1) : sgn_pow(d); }
+
+void orc_sweep_adjoint(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                       double dx, double dy, const double *sigma_t, double dt, const double *g, double *out)
+{
+    (void)quad_off;
+    const int P = p + 1, N = P * P;
+    const size_t npsi = (size_t)(nx + 2) * (ny + 2) * nq * N;
+    double *Aq = (double *)malloc(sizeof(double) * (size_t)nq * N * N);
+    double *AT = (double *)malloc(sizeof(double) * N * N);
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *mu = (double *)calloc(npsi, sizeof(double));
+    orc_aq(p, nq, xi, eta, dx, dy, Aq);
+    const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+    memset(out, 0, sizeof(double) * npsi);
+    for (int q = 0; q < nq; ++q) {
+        const int xneg = xi[q] <= 0.0, yneg = eta[q] <= 0.0;
+        const double cx = dy * xi[q], cy = dx * eta[q];
+        /* forward order visits i upwind-first; walk it backwards */
+        for (int ii = nx; ii >= 1; --ii)
+            for (int jj = ny; jj >= 1; --jj) {
+                const int i = xneg ? nx - ii + 1 : ii, j = yneg ? ny - jj + 1 : jj;
+                const int idn = xneg ? i - 1 : i + 1, jdn = yneg ? j - 1 : j + 1; /* forward-downwind */
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) {
+                        double v = g[psi_idx(ny, nq, p, q, i, j, d, e)];
+                        if (idn >= 1 && idn <= nx)
+                            for (int dd = 0; dd <= p; ++dd)
+                                v += cx * (2 * dd + 1) * kx_coef(xneg, dd, d) * mu[psi_idx(ny, nq, p, q, idn, j, dd, e)];
+                        if (jdn >= 1 && jdn <= ny)
+                            for (int ee = 0; ee <= p; ++ee)
+                                v += cy * (2 * ee + 1) * kx_coef(yneg, ee, e) * mu[psi_idx(ny, nq, p, q, i, jdn, d, ee)];
+                        B[e + P * d] = v;
+                    }
+                const double coeff = (sigma_t[(size_t)j + (size_t)(ny + 2) * i] + inv_dt) * dx * dy;
+                const double *A = Aq + (size_t)q * N * N;
+                for (int r = 0; r < N; ++r)
+                    for (int c = 0; c < N; ++c) AT[r + N * c] = A[c + N * r] + (r == c ? coeff : 0.0);
+                orc_ge_solve(N, AT, B);
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) {
+                        mu[psi_idx(ny, nq, p, q, i, j, d, e)] = B[e + P * d];
+                        out[psi_idx(ny, nq, p, q, i, j, d, e)] = dx * dy * B[e + P * d];
+                    }
+            }
+        /* gradient with respect to the inflow halo cells: C^T mu over the inflow faces */
+        const int ih = xneg ? nx + 1 : 0, ib = xneg ? nx : 1;
+        for (int j = 1; j <= ny; ++j)
+            for (int a = 0; a <= p; ++a)
+                for (int e = 0; e <= p; ++e) {
+                    double v = 0.0;
+                    for (int d = 0; d <= p; ++d)
+                        v += cx * (2 * d + 1) * kx_coef(xneg, d, a) * mu[psi_idx(ny, nq, p, q, ib, j, d, e)];
+                    out[psi_idx(ny, nq, p, q, ih, j, a, e)] = v;
+                }
+        const int jh = yneg ? ny + 1 : 0, jb = yneg ? ny : 1;
+        for (int i = 1; i <= nx; ++i)
+            for (int d = 0; d <= p; ++d)
+                for (int b = 0; b <= p; ++b) {
+                    double v = 0.0;
+                    for (int e = 0; e <= p; ++e)
+                        v += cy * (2 * e + 1) * kx_coef(yneg, e, b) * mu[psi_idx(ny, nq, p, q, i, jb, d, e)];
+                    out[psi_idx(ny, nq, p, q, i, jh, d, b)] = v;
+                }
+    }
+    free(Aq); free(AT); free(B); free(mu);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
 /* Moment contraction and scattering source.  TransportOperator.cpp:140-158 and :355-392.         */
 /* ---------------------------------------------------------------------------------------------- */
 void orc_pmv(int nx, int ny, int p, int nq, const double *w, double alpha, double beta, const double *psi,

@@ -44,6 +44,14 @@ void orc_ge_solve(int N, double *A, double *b);
 void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
                double dx, double dy, const double *sigma_t, double dt, double *psi);
 
+/* Transposed sweep: out (interior) <- zeta = L^{-T} g for the map src -> psi of orc_sweep with zero
+ * inflow, where L^{-1} src solves A_c psi_c - (upwind couplings) = dx dy src_c.  out's halo cells
+ * receive the gradient of <g, psi> with respect to the inflow halo values (zero on outflow sides and
+ * corners).  g's halo cells are ignored.  The reference has no adjoint; this transposes the system
+ * of CalcA/CalcB (SweepLinearSolveSpec.cpp:284-298,473-598) directly. */
+void orc_sweep_adjoint(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                       double dx, double dy, const double *sigma_t, double dt, const double *g, double *out);
+
 /* phi <- beta*phi + alpha * sum_q w_q psi_q.  all_cells: 0 = interior only, 1 = halos too.
  * TransportOperator.cpp:140-158. */
 void orc_pmv(int nx, int ny, int p, int nq, const double *w, double alpha, double beta,

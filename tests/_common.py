@@ -60,6 +60,7 @@ def oracle():
         lib.orc_l2dist.restype = d
         lib.orc_si_solve.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, P, P, IP, P]
         lib.orc_peierls_gmres.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, i, P, P, IP, P]
+        lib.orc_sweep_adjoint.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, P]
         _oracle = lib
     return _oracle
 
@@ -138,6 +139,56 @@ def orc_sweep(g: Grid, o: Ordinates, sigma_t, psi, dt=math.inf):
     oracle().orc_sweep(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy, ptr(s),
                        dt, ptr(out))
     return out
+
+
+def orc_sweep_adjoint(g: Grid, o: Ordinates, sigma_t, seed, dt=math.inf):
+    """zeta = L^{-T} seed on the interior; halo cells: gradient of <seed, psi> w.r.t. the inflow."""
+    out = np.zeros(g.psi_shape(o.nq))
+    oracle().orc_sweep_adjoint(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy,
+                               ptr(np.ascontiguousarray(sigma_t, dtype=np.float64)), dt,
+                               ptr(np.ascontiguousarray(seed, dtype=np.float64)), ptr(out))
+    return out
+
+
+def interior(a):
+    return a[1:-1, 1:-1]
+
+
+def halo_mask(shape):
+    m = np.ones(shape, dtype=bool)
+    m[1:-1, 1:-1] = False
+    return m
+
+
+def adjoint_sigma_grads(psi, zeta, phi=None):
+    """Per-cell gradients of J = <seed, psi> from the forward flux psi and the adjoint flux zeta
+    (both psi-shaped): dJ/dsigma_t = -sum_{q,k} zeta psi; dJ/dsigma_s = sum_k phi_k sum_q zeta_qk / (4 pi)."""
+    gt = np.zeros(psi.shape[:2])
+    gt[1:-1, 1:-1] = -np.einsum("ijqde,ijqde->ij", interior(zeta), interior(psi))
+    if phi is None:
+        return gt
+    gs = np.zeros(psi.shape[:2])
+    gs[1:-1, 1:-1] = np.einsum("ijde,ijqde->ij", interior(phi), interior(zeta)) / (4 * math.pi)
+    return gt, gs
+
+
+def orc_solve_adjoint(g: Grid, o: Ordinates, sigma_t, sigma_s, seed, dt=math.inf, tol=1e-14, max_its=2000):
+    """Adjoint of the scattering solve psi = L^{-1}(Q + sigma_s P psi / 4pi), by adjoint source iteration
+    zeta = L^{-T}(seed + K^T zeta), (K^T zeta)_q = w_q sigma_s sum_q' zeta_q' / 4pi.  Returns the last
+    transposed sweep's output (interior: zeta = dJ/dQ; halos: dJ/d inflow)."""
+    zeta = np.zeros(g.psi_shape(o.nq))
+    prev = None
+    for _ in range(max_its):
+        rhs = np.zeros_like(zeta)
+        rhs[1:-1, 1:-1] = interior(seed)
+        tot = interior(zeta).sum(axis=2)  # [i][j][d][e]
+        rhs[1:-1, 1:-1] += (o.w[None, None, :, None, None] * interior(sigma_s)[:, :, None, None, None]
+                            * tot[:, :, None] / (4 * math.pi))
+        zeta = orc_sweep_adjoint(g, o, sigma_t, rhs, dt)
+        if prev is not None and np.linalg.norm((zeta - prev).ravel()) <= tol * np.linalg.norm(zeta.ravel()):
+            break
+        prev = zeta
+    return zeta
 
 
 def orc_pmv(g: Grid, o: Ordinates, psi, phi=None, alpha=1.0, beta=0.0, all_cells=0):
