@@ -133,6 +133,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
     ap.add_argument("--workload", default="lattice", choices=["lattice", "batched"])
+    ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
+    ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "
+                                                     "for each and report them (tuning aid)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep has no "
@@ -157,6 +160,7 @@ def main():
     ctx = tsw.Context(0)
     P.set_on(ctx)
     ctx.set_pattern(args.pattern)
+    ctx.set_tuning("band_rows", args.band_rows)
 
     def solve(precision, phi):
         work.copy_(d_src)
@@ -192,6 +196,13 @@ def main():
         n, ms = ctx.sweep_timing(reset=True)
         return ms / max(n, 1)
 
+    if args.band_scan:
+        for br in [int(v) for v in args.band_scan.split(",")]:
+            ctx.set_tuning("band_rows", br)
+            a32, a64 = contract_sweeps(tsw.FP32), contract_sweeps(tsw.FP64)
+            print(f"bench: band_rows {br}: contract sweep fp32 {a32:.4f} ms fp64 {a64:.4f} ms", file=sys.stderr,
+                  flush=True)
+        ctx.set_tuning("band_rows", args.band_rows)
     t_sw32 = contract_sweeps(tsw.FP32)
     t_sw64 = contract_sweeps(tsw.FP64)
     print(f"bench: contract sweep fp32 {t_sw32:.3f} ms fp64 {t_sw64:.3f} ms", file=sys.stderr, flush=True)
@@ -236,7 +247,8 @@ def main():
         "data": "synthetic (lattice cross sections and source generated in-process)",
         "config": {"workload": "checkerboard lattice 280x280 cells, Q2 DG, S12 (84 ordinates), Peierls GMRES(30) "
                                "to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
-                   "sweep_pattern": args.pattern, "dsa": False, "unknowns_per_sweep": U},
+                   "sweep_pattern": args.pattern, "band_rows": args.band_rows, "dsa": False,
+                   "unknowns_per_sweep": U},
         "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
                  "gmres_its": [i for i, _ in its64], "solve_sweep_ms": ms64 / max(n64, 1),
                  "contract_sweep_ms": t_sw64, "contract_sweep_gbs": achieved64},

@@ -84,6 +84,14 @@ struct tsw_ctx {
     // the first vector-Jacobian product.  Staging of the vector-Jacobian products' arrays.
     std::unique_ptr<tsw_ctx> adj;
     DBuf vjp_psi, vjp_seed, vjp_gsrc, vjp_gst, vjp_gss, vjp_y, vjp_phi;
+    // Row bands of the staggered sweep: band_rows_opt 0 = automatic, -1 = one band per ordinate,
+    // > 0 = rows per band.  Exchange records per precision (sentinel-filled when empty), the stall flag
+    // and its pinned host mirror, read back at every synchronisation when bands ran.
+    int band_rows_opt = 0;
+    DBuf xtr64, xtr32, abort_dev;
+    size_t xtr64_n = 0, xtr32_n = 0;
+    int *abort_host = nullptr;
+    bool bands_used = false;
     static constexpr int kSideStreams = 4;
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
@@ -98,6 +106,7 @@ struct tsw_ctx {
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         batch.clear();
         adj.reset();
+        if (abort_host) (void)hipHostFree(abort_host);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -146,7 +155,32 @@ void stage_out(tsw_ctx *c, T *p, const T *d, size_t n, int mem) {
     TSW_HIP(hipMemcpyAsync(p, d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
 }
 
-void sync(tsw_ctx *c) { TSW_HIP(hipStreamSynchronize(c->stream)); }
+void refill_exchange(tsw_ctx *c) {
+    if (c->xtr64_n) launch_exchange_fill<double>(c->xtr64.as<double>(), c->xtr64_n, c->stream);
+    if (c->xtr32_n) launch_exchange_fill<float>(c->xtr32.as<float>(), c->xtr32_n, c->stream);
+}
+
+// Waits for the context's stream; reports (and clears) a row-band stall of this context or of its
+// adjoint problem.
+void sync(tsw_ctx *c) {
+    tsw_ctx *chk[2] = {c->bands_used ? c : nullptr, (c->adj && c->adj->bands_used) ? c->adj.get() : nullptr};
+    for (tsw_ctx *k : chk)
+        if (k) TSW_HIP(hipMemcpyAsync(k->abort_host, k->abort_dev.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TSW_HIP(hipStreamSynchronize(c->stream));
+    bool stalled = false;
+    for (tsw_ctx *k : chk) {
+        if (k && *k->abort_host) {
+            stalled = true;
+            *k->abort_host = 0;
+            TSW_HIP(hipMemsetAsync(k->abort_dev.ptr, 0, sizeof(int), c->stream));
+            refill_exchange(k);
+        }
+    }
+    if (stalled) {
+        TSW_HIP(hipStreamSynchronize(c->stream));
+        throw Error(TSW_E_HIP, "row-band sweep stalled waiting for an upstream band (spin limit reached)");
+    }
+}
 
 // Sum of the recorded (start, stop) event pairs; waits for them.
 double recorded_ms(tsw_ctx *c) {
@@ -470,6 +504,49 @@ const T *diag_inverses(tsw_ctx *c, double dt) {
     return buf.as<T>();
 }
 
+// Rows per band for this context's mesh: 0 when each ordinate is one workgroup.  Automatic choice:
+// one wavefront of rows per band when the mesh has more rows than that, so the ordinates' sweeps
+// spread over several times more compute units than there are ordinates.
+int band_rows_for(const tsw_ctx *c) {
+    const int ny = c->g.ny;
+    int br = c->band_rows_opt;
+    if (br < 0) br = ny > 448 ? 448 : 0;  // a single band cannot hold more than 448 rows
+    else if (br == 0) br = ny > 64 ? 64 : 0;
+    if (br >= ny) br = 0;
+    return br;
+}
+
+// Fills the band fields of sp and makes sure the exchange records exist (filled with the sentinel).
+template <typename T>
+void configure_bands(tsw_ctx *c, DiagSweepParams<T> &sp) {
+    const Geom &g = c->g;
+    const int br = band_rows_for(c);
+    if (br == 0) {
+        sp.nbands = 1;
+        sp.band_rows = g.ny;
+        return;
+    }
+    sp.band_rows = br;
+    sp.nbands = (g.ny + br - 1) / br;
+    const size_t n = (size_t)g.nq * (sp.nbands - 1) * g.nx * (g.p + 1);
+    DBuf &buf = sizeof(T) == 8 ? c->xtr64 : c->xtr32;
+    size_t &have = sizeof(T) == 8 ? c->xtr64_n : c->xtr32_n;
+    if (have < n) {
+        buf.reserve(n * sizeof(T));
+        have = n;
+        launch_exchange_fill<T>(buf.as<T>(), n, c->stream);
+    }
+    if (!c->abort_host) {
+        TSW_HIP(hipHostMalloc((void **)&c->abort_host, sizeof(int), hipHostMallocDefault));
+        *c->abort_host = 0;
+        c->abort_dev.reserve(sizeof(int));
+        TSW_HIP(hipMemsetAsync(c->abort_dev.ptr, 0, sizeof(int), c->stream));
+    }
+    sp.xtr = buf.as<T>();
+    sp.abort_flag = c->abort_dev.as<int>();
+    c->bands_used = true;
+}
+
 // Kernel parameters of one staggered sweep: psi_out <- L^{-1}(qsrc + sigma_s phi_d / (4 pi)), inflow from
 // the traces set up by diag_setup when `inflow`, else vacuum.  qsrc or phi_d may be null (not both);
 // qsrc may equal psi_out.
@@ -492,6 +569,7 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
         sp.tin_x = (const T *)(f64 ? d.tinx64.ptr : d.tinx32.ptr);
         sp.tin_y = (const T *)(f64 ? d.tiny64.ptr : d.tiny32.ptr);
     }
+    configure_bands<T>(c, sp);
     sp.ainv = diag_inverses<T>(c, dt);
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
@@ -545,6 +623,11 @@ bool staggered_ok(tsw_ctx *c) {
     DiagSweepParams<double> probe{};
     probe.nx = c->g.nx;
     probe.ny = c->g.ny;
+    const int br = band_rows_for(c);
+    if (br > 0) {
+        probe.band_rows = br;
+        probe.nbands = (c->g.ny + br - 1) / br;
+    }
     return diag_sweep_lds_bytes<double>(probe, c->g.p) != 0;
 }
 
@@ -869,10 +952,12 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
         a->stream = ctx->stream;
         a->own_stream = false;
         a->pattern = ctx->pattern;
+        a->band_rows_opt = ctx->band_rows_opt;
         set_problem(a.get(), g.nx, g.ny, g.ax, g.bx, g.ay, g.by, g.p, nq, xi.data(), eta.data(), w.data(), off);
         ctx->adj = std::move(a);
     }
     ctx->adj->pattern = ctx->pattern;
+    ctx->adj->band_rows_opt = ctx->band_rows_opt;
     return ctx->adj.get();
 }
 
@@ -978,6 +1063,19 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
         else if (s == "hip-staggered") ctx->pattern = Pattern::Staggered;
         else if (s == "hip-staggered-ge") ctx->pattern = Pattern::StaggeredGE;
         else throw Error(TSW_E_ARG, "unknown sweep_pattern '" + s + "'");
+    });
+}
+
+int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
+    return guarded(ctx, [&] {
+        require(key != nullptr, TSW_E_ARG, "null tuning key");
+        const std::string k(key);
+        if (k == "band_rows") {
+            require(value >= -1, TSW_E_ARG, "band_rows must be -1, 0 or a row count");
+            ctx->band_rows_opt = value;
+        } else {
+            throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
+        }
     });
 }
 
@@ -1191,6 +1289,7 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             sub->stream = ctx->stream;
             sub->own_stream = false;
             sub->pattern = Pattern::Staggered;
+            sub->band_rows_opt = -1;
             set_problem(sub.get(), pr.nx, pr.ny, pr.ax, pr.bx, pr.ay, pr.by, pr.p, pr.nq, pr.xi, pr.eta, pr.w,
                         pr.quad_off);
             require(staggered_ok(sub.get()), TSW_E_UNSUPPORTED,

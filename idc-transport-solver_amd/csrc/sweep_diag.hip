@@ -36,54 +36,144 @@ struct DiagLds {  // LDS carve-up in units of T
     }
 };
 
+// Band exchange: records of P values moved with relaxed device-scope atomics (coherent across the
+// chip's L2 slices); an empty record holds ExchangeSentinel<T>::bits.
+constexpr int kSpinLimit = 1 << 20;
+
+template <typename T>
+__device__ __forceinline__ typename ExchangeSentinel<T>::U to_bits(T v) {
+    return __builtin_bit_cast(typename ExchangeSentinel<T>::U, v);
+}
+template <typename T>
+__device__ __forceinline__ T from_bits(typename ExchangeSentinel<T>::U v) {
+    return __builtin_bit_cast(T, v);
+}
+// Global-address-space view of a pointer: accesses compile to global_* instructions, which count
+// only against vmcnt, so LDS waits do not also wait for outstanding prefetches.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> G(T *p) {
+    return (gptr<T>)p;
+}
+
+template <typename U>
+__device__ __forceinline__ U xload(gptr<const U> p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename U>
+__device__ __forceinline__ void xstore(gptr<U> p, U v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename U, int P>
+struct Record {
+    U v[P];
+};
+
+// Waits until record `rec` (already loaded once into r) is filled; on the spin limit, or when another
+// band has given up (*abort_flag), returns zeros and the caller proceeds.
+template <typename T, int P>
+__device__ __noinline__ Record<typename ExchangeSentinel<T>::U, P> wait_record(
+    gptr<const typename ExchangeSentinel<T>::U> rec, Record<typename ExchangeSentinel<T>::U, P> r, gptr<int> abort_flag) {
+    using U = typename ExchangeSentinel<T>::U;
+    for (int it = 0;; ++it) {
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < P; ++d) ok = ok && r.v[d] != ExchangeSentinel<T>::bits;
+        if (ok) return r;
+        if (it >= kSpinLimit || ((it & 255) == 255 && xload<int>(abort_flag) != 0)) {
+            if (it >= kSpinLimit) xstore<int>(abort_flag, 1);
+#pragma unroll
+            for (int d = 0; d < P; ++d) r.v[d] = 0;
+            return r;
+        }
+        __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+        for (int d = 0; d < P; ++d) r.v[d] = xload<U>(rec + d);
+    }
+}
+
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
+// Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
+// upstream neighbour is dispatched before it and never waits on it).
 template <typename T, int P, bool MAT, int MODE, int D>
 __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
-    // one problem (sp1, workgroup = ordinate) or a batch (items[workgroup] = {problem, ordinate})
-    const DiagSweepParams<T> sp = batch ? batch[items[blockIdx.x].x] : sp1;
-    const int q = batch ? items[blockIdx.x].y : (int)blockIdx.x;
+    using U = typename ExchangeSentinel<T>::U;
+    // one problem (sp1, workgroup = band x ordinate) or a batch (items[workgroup] = {problem, ordinate})
+    int item_p = 0, item_q = 0;
+    if (batch) {
+        const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)blockIdx.x;
+        item_p = it[0];
+        item_q = it[1];
+    }
+    const DiagSweepParams<T> sp = batch ? batch[item_p] : sp1;
+    const int nb = (batch || sp.nbands < 2) ? 1 : sp.nbands;
+    const int q = batch ? item_q : (int)(blockIdx.x % (unsigned)sp.nq);
+    const int band = batch ? 0 : (int)(blockIdx.x / (unsigned)sp.nq);
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0;
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int nx = sp.nx, ny = sp.ny;
-    const int cthreads = ((ny + 63) / 64) * 64;  // waves beyond this row count idle at the barriers
-    const DiagLds<T, P> L(cthreads, nx, ny, MAT ? sp.M : 0);
+    const int BR = nb > 1 ? sp.band_rows : ny;  // rows per band
+    const int R0 = band * BR;                    // first row of this band
+    const int nrows = BR < ny - R0 ? BR : ny - R0;
+    const int cthreads = ((BR + 63) / 64) * 64;  // waves beyond this row count idle at the barriers
+    const DiagLds<T, P> L(cthreads, nx, BR, MAT ? sp.M : 0);
     T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
     const DiagMap dm{nx, ny};
     const size_t ncell = (size_t)dm.cells();
+    // exchange records: read from the band below (xin), written for the band above (xout)
+    const gptr<U> xin = (nb > 1 && band > 0) ? G(reinterpret_cast<U *>(sp.xtr)) + ((size_t)q * (nb - 1) + band - 1) * nx * P : nullptr;
+    const gptr<U> xout = (nb > 1 && band < nb - 1) ? G(reinterpret_cast<U *>(sp.xtr)) + ((size_t)q * (nb - 1) + band) * nx * P : nullptr;
 
     const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
     const int tid = threadIdx.x;
-    const T xi = sp.xi[q], eta = sp.eta[q];
+    const T xi = G(sp.xi)[q], eta = G(sp.eta)[q];
     const bool xneg = xi <= T(0), yneg = eta <= T(0);
 
     if (MAT) {
         for (int k = tid; k < sp.M * N * N; k += blockDim.x) {
             const int m = k / (N * N), e = k - m * N * N;
-            linv[k] = sp.ainv[((size_t)m * sp.nq + q) * N * N + e];
+            linv[k] = G(sp.ainv)[((size_t)m * sp.nq + q) * N * N + e];
         }
     }
-    for (int k = tid; k < nx * P; k += blockDim.x) lin[k] = sp.tin_y ? sp.tin_y[(size_t)q * nx * P + k] : T(0);
+    if (band == 0)
+        for (int k = tid; k < nx * P; k += blockDim.x) lin[k] = sp.tin_y ? G(sp.tin_y)[(size_t)q * nx * P + k] : T(0);
 
-    const int steps = nx + ny - 1;
+    const int steps = nx + nrows - 1;              // local step t is global diagonal R0 + t
     const int nsteps = ((steps + D - 1) / D) * D;  // padded steps have no active cell
-    T *psi_q = sp.psi + (size_t)q * N * ncell;
+    const gptr<T> psi_q = G(sp.psi) + (size_t)q * N * ncell;
+    // rows of this band on diagonal s
+    auto band_lo = [&](int s) { const int a = dm.rlo(s); return a > R0 ? a : R0; };
+    auto band_hi = [&](int s) {
+        const int a = s < ny - 1 ? s : ny - 1, b = R0 + nrows - 1;
+        return a < b ? a : b;
+    };
 
     if (tid >= (int)blockDim.x - 64) {
         // storer: barrier k (k >= 1) closes compute step k-1, whose cells are written here while
-        // the compute waves fill the other ring slot
+        // the compute waves fill the other ring slot; it also hands the band's top trace on and empties
+        // the records this band has consumed
         const int lane = tid - ((int)blockDim.x - 64);
         lds_barrier();
-        for (int st = 0; st < nsteps; ++st) {
+        for (int t = 0; t < nsteps; ++t) {
             lds_barrier();
-            if (st >= steps) continue;
-            const T *slot = lout + (size_t)(st & 1) * ny * N;
-            const int r0 = dm.rlo(st), r1 = st < ny - 1 ? st : ny - 1;
-            const long long p0 = dm.base(st) - r0;
-            for (int r = r0 + lane; r <= r1; r += 64) {
-                const T *src = slot + (size_t)r * N;
+            if (t >= steps) continue;
+            const int s = R0 + t;
+            if (xout) {
+                const int c = t - (nrows - 1);
+                if (c >= 0 && c < nx && lane < P)
+                    xstore(xout + (size_t)c * P + lane, to_bits(ex[((size_t)(t & 1) * cthreads + (nrows - 1)) * P + lane]));
+            }
+            if (xin && t < nx && lane < P) xstore(xin + (size_t)t * P + lane, ExchangeSentinel<T>::bits);
+            const T *slot = lout + (size_t)(t & 1) * BR * N;
+            const int lo = band_lo(s), hi = band_hi(s);
+            const long long p0 = dm.base(s) - dm.rlo(s);
+            for (int r = lo + lane; r <= hi; r += 64) {
+                const T *src = slot + (size_t)(r - R0) * N;
 #pragma unroll
                 for (int k = 0; k < N; ++k) psi_q[(size_t)k * ncell + (size_t)(p0 + r)] = src[k];
             }
@@ -91,31 +181,36 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         return;
     }
 
-    const int r = tid < cthreads ? tid : cthreads - 1;  // spare waves mirror the last row, stay inactive
-    const bool row_ok = tid < ny;
+    const int rl = tid < cthreads ? tid : cthreads - 1;  // spare waves mirror the last row, stay inactive
+    const int r = R0 + rl;                               // mesh row (counted from the inflow edge)
+    const bool row_ok = tid < nrows;
     T tx[P];
 #pragma unroll
-    for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? sp.tin_x[((size_t)q * ny + r) * P + d] : T(0);
+    for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? G(sp.tin_x)[((size_t)q * ny + r) * P + d] : T(0);
 
-    // position of this row's cell at step s; lanes whose row has no cell on diagonal s read the
-    // nearest cell of that diagonal, so a wavefront's loads stay within one contiguous run
+    // position of this row's cell at local step t; lanes whose row has no cell on that diagonal read
+    // the nearest cell of the band on it, so a wavefront's loads stay within one contiguous run
     // (diagonal bounds are wave-uniform scalar arithmetic)
-    auto pos_at = [&](int s) -> size_t {
-        if (s >= steps) return 0;
-        const int lo = dm.rlo(s), hi = s < ny - 1 ? s : ny - 1;
+    auto pos_at = [&](int t) -> size_t {
+        if (t >= steps) return 0;
+        const int s = R0 + t;
+        const int lo = band_lo(s), hi = band_hi(s);
         const int rc = r < lo ? lo : (r > hi ? hi : r);
-        return (size_t)(dm.base(s) + rc - lo);
+        return (size_t)(dm.base(s) + rc - dm.rlo(s));
     };
-    const T *qsrc_q = HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr;
-    const T *phi_Q = HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr;
-    const T *sigs_Q = HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr;
-    const int *mat_Q = MAT ? sp.mat + (size_t)Q * ncell : nullptr;
-    const T *sigt_Q = MAT ? nullptr : sp.sig_t + (size_t)Q * ncell;
+    const gptr<const T> qsrc_q = HASQ ? G(sp.qsrc) + (size_t)q * N * ncell : nullptr;
+    const gptr<const T> phi_Q = HASPHI ? G(sp.phi) + (size_t)Q * N * ncell : nullptr;
+    const gptr<const T> sigs_Q = HASPHI ? G(sp.sig_s) + (size_t)Q * ncell : nullptr;
+    const gptr<const int> mat_Q = MAT ? G(sp.mat) + (size_t)Q * ncell : nullptr;
+    const gptr<const T> sigt_Q = MAT ? nullptr : G(sp.sig_t) + (size_t)Q * ncell;
+    const bool xrow = xin != nullptr && rl == 0;  // this lane takes its y-inflow from the band below
 
-    // prefetch ring: slot u holds the inputs this row meets at a step s with s % D == u
+    // prefetch ring: slot u holds the inputs this row meets at a step t with t % D == u
     T pq[D][HASQ ? N : 1], pf[D][HASPHI ? N : 1], ps[D], pm_sig[D];
     int pm[D];
-    auto fetch = [&](int u, size_t p) {
+    Record<U, P> px[D];
+    auto fetch = [&](int u, int t) {
+        const size_t p = pos_at(t);
         if (HASQ) {
 #pragma unroll
             for (int k = 0; k < N; ++k) pq[u][k] = qsrc_q[(size_t)k * ncell + p];
@@ -127,9 +222,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         }
         if (MAT) pm[u] = mat_Q[p];
         else pm_sig[u] = sigt_Q[p];
+        if (xrow && t < nx) {
+#pragma unroll
+            for (int d = 0; d < P; ++d) px[u].v[d] = xload<U>(xin + (size_t)t * P + d);
+        }
     };
 #pragma unroll
-    for (int u = 0; u < D; ++u) fetch(u, pos_at(u));
+    for (int u = 0; u < D; ++u) fetch(u, u);
     lds_barrier();
 
     // FP32 keeps the current material's inverse in registers and reloads it from LDS only when the
@@ -138,13 +237,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     T Ainv[REGMAT ? N * N : 1];
     int cur_mat = -1;
     const T area = sp.dx * sp.dy;
-    for (int s0 = 0; s0 < nsteps; s0 += D) {
+    for (int t0 = 0; t0 < nsteps; t0 += D) {
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-            const int s = s0 + u;
-            const int c = s - r;
+            const int t = t0 + u;
+            const int c = t - rl;
             const bool active = row_ok && c >= 0 && c < nx;
-            // consume slot u, then refill it with step s + D (kept apart by scheduling barriers so
+            // consume slot u, then refill it with step t + D (kept apart by scheduling barriers so
             // the slot keeps its registers and its wait covers only its own loads)
             T B[N];
             {
@@ -157,16 +256,27 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                     src[k] = v;
                 }
                 T ty[P];
-                const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
-                const T *tsrc = r == 0 ? lin + (size_t)cc * P : ex + ((size_t)((s - 1) & 1) * cthreads + (r - 1)) * P;
+                if (xrow) {
+                    if (active) {
+                        bool ok = true;
 #pragma unroll
-                for (int d = 0; d < P; ++d) ty[d] = tsrc[d];
+                        for (int d = 0; d < P; ++d) ok = ok && px[u].v[d] != ExchangeSentinel<T>::bits;
+                        if (!ok) px[u] = wait_record<T, P>(xin + (size_t)c * P, px[u], G(sp.abort_flag));
+                    }
+#pragma unroll
+                    for (int d = 0; d < P; ++d) ty[d] = active ? from_bits<T>(px[u].v[d]) : T(0);
+                } else {
+                    const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
+                    const T *tsrc = rl == 0 ? lin + (size_t)cc * P : ex + ((size_t)((t - 1) & 1) * cthreads + (rl - 1)) * P;
+#pragma unroll
+                    for (int d = 0; d < P; ++d) ty[d] = tsrc[d];
+                }
                 rhs_from_traces<T, P>(B, src, tx, ty, xi, eta, sp.dx, sp.dy);
             }
             const int mat = pm[u];
             const T sig = pm_sig[u];
             __builtin_amdgcn_sched_barrier(0);
-            fetch(u, pos_at(s + D));
+            fetch(u, t + D);
             __builtin_amdgcn_sched_barrier(0);
             if (active) {
                 T x[N];
@@ -193,13 +303,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
 #pragma unroll
                     for (int k = 0; k < N; ++k) x[k] = B[k];
                 }
-                T *o = lout + ((size_t)(s & 1) * ny + r) * N;
+                T *o = lout + ((size_t)(t & 1) * BR + rl) * N;
 #pragma unroll
                 for (int k = 0; k < N; ++k) o[k] = x[k];
                 trace_x<T, P>(x, xneg, tx);
                 T tyo[P];
                 trace_y<T, P>(x, yneg, tyo);
-                T *dst_l = ex + ((size_t)(s & 1) * cthreads + r) * P;
+                T *dst_l = ex + ((size_t)(t & 1) * cthreads + rl) * P;
 #pragma unroll
                 for (int d = 0; d < P; ++d) dst_l[d] = tyo[d];
             }
@@ -208,17 +318,24 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     }
 }
 
+// rows handled by one workgroup
+template <typename T>
+static int band_rows_of(const DiagSweepParams<T> &sp) {
+    return sp.nbands > 1 ? sp.band_rows : sp.ny;
+}
+
 template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
-    if (sp.ny > 448) return 0;
-    const int cthreads = ((sp.ny + 63) / 64) * 64;
+    const int BR = band_rows_of(sp);
+    if (BR > 448) return 0;
+    const int cthreads = ((BR + 63) / 64) * 64;
     const int M = sp.ainv ? sp.M : 0;
     size_t total;
     switch (p) {
-        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, sp.ny, M).total; break;
-        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, sp.ny, M).total; break;
-        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, sp.ny, M).total; break;
-        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, sp.ny, M).total; break;
+        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, BR, M).total; break;
+        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, BR, M).total; break;
+        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, BR, M).total; break;
+        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, BR, M).total; break;
         default: return 0;
     }
     const size_t bytes = total * sizeof(T);
@@ -229,12 +346,13 @@ template <typename T, int P, bool MAT, int MODE>
 static void launch_ds(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch = nullptr,
                       const int2 *items = nullptr, int nitems = 0, int max_ny = 0) {
     constexpr int D = sizeof(T) == 8 ? 2 : 3;
-    const int rows = batch ? max_ny : sp.ny;
+    const int rows = batch ? max_ny : band_rows_of(sp);
     const int threads = ((rows + 63) / 64) * 64 + 64;
     auto kern = k_diag_sweep<T, P, MAT, MODE, D>;
     if (lds > 64 * 1024)
         TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(batch ? nitems : sp.nq), dim3(threads), lds, s, sp, batch, items);
+    const int blocks = batch ? nitems : sp.nq * (sp.nbands > 1 ? sp.nbands : 1);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, sp, batch, items);
     TSW_HIP(hipGetLastError());
 }
 
@@ -258,7 +376,12 @@ static void launch_ds_p(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s)
 template <typename T>
 void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s) {
     const size_t lds = diag_sweep_lds_bytes<T>(sp, p);
-    if (lds == 0) throw Error(TSW_E_UNSUPPORTED, "hip-staggered needs ny <= 448 and its tables within one CU's LDS");
+    if (lds == 0) throw Error(TSW_E_UNSUPPORTED, "hip-staggered needs <= 448 rows per band and its tables within one CU's LDS");
+    if (sp.nbands > 1) {
+        if (!sp.xtr || !sp.abort_flag || sp.band_rows < 1 || (long long)sp.band_rows * (sp.nbands - 1) >= sp.ny ||
+            (long long)sp.band_rows * sp.nbands < sp.ny)
+            throw Error(TSW_E_ARG, "inconsistent row-band configuration");
+    }
     switch (p) {
         case 0: launch_ds_p<T, 1>(sp, lds, s); break;
         case 1: launch_ds_p<T, 2>(sp, lds, s); break;
@@ -267,6 +390,23 @@ void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s) {
         default: throw Error(TSW_E_UNSUPPORTED, "DG degree " + std::to_string(p) + " not built (0..3)");
     }
 }
+
+template <typename U>
+__global__ void k_fill_bits(U *p, size_t n, U v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename T>
+void launch_exchange_fill(T *xtr, size_t n, hipStream_t s) {
+    using U = typename ExchangeSentinel<T>::U;
+    if (n == 0) return;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_bits<U>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<U *>(xtr), n,
+                       ExchangeSentinel<T>::bits);
+    TSW_HIP(hipGetLastError());
+}
+template void launch_exchange_fill<double>(double *, size_t, hipStream_t);
+template void launch_exchange_fill<float>(float *, size_t, hipStream_t);
 
 // Batched sweep: one launch over work items {problem, ordinate} that share the DG degree, the mode
 // (which sources are present) and the material/elimination choice; `probs` are device copies.

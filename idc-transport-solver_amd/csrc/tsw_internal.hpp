@@ -108,7 +108,26 @@ template <typename T> struct DiagSweepParams {
     int M;
     const T *sig_t;        // [4][ncell] (without ainv)
     T *psi;                // out [nq][N][ncell]
+    // Row bands: nbands > 1 splits each ordinate's rows into bands of band_rows rows, one workgroup
+    // each, pipelined through xtr ([nq][nbands-1][nx][P] y-traces of each band's last row, holding the
+    // sentinel pattern when empty); a band that waits past the spin limit sets *abort_flag.
+    int band_rows, nbands;
+    T *xtr;
+    int *abort_flag;
 };
+
+// Bit pattern of an empty band-exchange record: a signalling NaN, which arithmetic never produces.
+template <typename T> struct ExchangeSentinel;
+template <> struct ExchangeSentinel<float> {
+    using U = uint32_t;
+    static constexpr U bits = 0x7F8ADBADu;
+};
+template <> struct ExchangeSentinel<double> {
+    using U = uint64_t;
+    static constexpr U bits = 0x7FF4DEADBEEFCAFEull;
+};
+template <typename T>
+void launch_exchange_fill(T *xtr, size_t n, hipStream_t s);
 template <typename T>
 void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s);
 template <typename T>

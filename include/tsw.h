@@ -67,9 +67,16 @@ int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double a
  *                          step, on a diagonal-major device layout; element solves by precomputed
  *                          inverses for piecewise-constant materials (<= 64 distinct sigma_t);
  *   "hip-staggered-ge"     the same with per-cell elimination.
- * Meshes the staggered kernel cannot hold (ny > 448) run the wavefront kernels.
+ * The staggered patterns split each ordinate's rows into pipelined bands (tsw_set_tuning).
  * Unknown names fail with TSW_E_ARG as the reference's factory throws std::out_of_range. */
 int tsw_set_pattern(tsw_ctx *ctx, const char *name);
+
+/* Tuning of the staggered sweep.  key "band_rows": rows of an ordinate's mesh handled by one workgroup;
+ * a sweep then runs ceil(ny / band_rows) pipelined workgroups per ordinate (the KBA decomposition of
+ * SweepPatternUniCartKBA2.cpp:212-453 applied to rows within the device).  0 (default) chooses
+ * automatically, -1 keeps one workgroup per ordinate (bands only when ny > 448).  Results do not
+ * depend on the setting.  Unknown keys fail with TSW_E_ARG. */
+int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
  * cells are overwritten.  dt <= 0 or +inf is steady state.  SweepOperator::Apply

@@ -1,0 +1,83 @@
+"""Row-band decomposition of the staggered sweep (tsw_set_tuning "band_rows"): each ordinate's rows are
+split over several pipelined workgroups that hand the y-traces of their top row on through device
+memory.  Results must not depend on the split: every band size is compared with the oracle, FP64 to
+1e-12 and FP32 to 2e-6 (relative l2), including uneven last bands, bands of a few rows, one-column
+meshes, the material-table and elimination paths, scattering sources (source iteration and GMRES) and
+repeated sweeps (the exchange records are reused from sweep to sweep)."""
+import math
+
+import numpy as np
+import pytest
+
+import tsw
+from _common import ls_ordinates, orc_gmres, orc_si, orc_sweep, random_ordinates, random_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(g, o, band_rows, pattern="hip-staggered"):
+    c = tsw.Context(0)
+    c.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
+    c.set_pattern(pattern)
+    c.set_tuning("band_rows", band_rows)
+    return c
+
+
+@pytest.mark.parametrize("band_rows", [-1, 0, 7, 50, 64, 100, 128])
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("pattern", ["hip-staggered", "hip-staggered-ge"])
+def test_banded_sweep_matches_oracle(band_rows, p, pattern):
+    o = ls_ordinates(6)
+    g, st, _, src = random_problem(37, 150, p, o, seed=500 + p, ax=0.0, bx=2.0, by=3.0, materials=5)
+    print(f"bands {band_rows} p={p} {pattern}", flush=True)
+    with _ctx(g, o, band_rows, pattern) as c:
+        for dt in (math.inf, 0.3):
+            ref = orc_sweep(g, o, st, src, dt)
+            for prec, tol in ((tsw.FP64, 1e-12), (tsw.FP32, 2e-6)):
+                for _ in range(2):   # second sweep reuses the emptied exchange records
+                    out = src.copy()
+                    c.sweep(out, st, dt, precision=prec)
+                    assert rel_err(out, ref) <= tol, (dt, prec)
+                    np.testing.assert_array_equal(out[:, 0], src[:, 0])
+
+
+@pytest.mark.parametrize("shape", [(1, 200), (2, 129), (300, 65), (90, 448), (60, 500)])
+def test_banded_sweep_shapes(shape):
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 1000 + ny)
+    o = random_ordinates(2, rng)
+    g, st, _, src = random_problem(nx, ny, 1, o, seed=nx + ny)
+    ref = orc_sweep(g, o, st, src, 0.5)
+    for br in (0, 33, 64):
+        with _ctx(g, o, br) as c:
+            out = src.copy()
+            c.sweep(out, st, 0.5)
+            assert rel_err(out, ref) <= 1e-12, br
+
+
+def test_banded_solves_match_oracle():
+    o = ls_ordinates(4)
+    g, st, ss, src = random_problem(40, 130, 1, o, seed=77, materials=6)
+    ref_si, _, _ = orc_si(g, o, st, ss, src, abs_tol=1e-12, max_its=400)
+    rc, ref_g, _, _, _ = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
+    assert rc == 0
+    for br in (-1, 16, 64):
+        with _ctx(g, o, br) as c:
+            for prec, tol in ((tsw.FP64, 1e-9), (tsw.FP32, 1e-5)):
+                psi = src.copy()
+                c.si_solve(psi, st, ss, abs_tol=1e-12, max_its=400, precision=prec)
+                assert rel_err(psi, ref_si) <= tol, ("si", br, prec)
+            for prec, tol in ((tsw.FP64, 1e-10), (tsw.FP32, 1e-8)):
+                psi = src.copy()
+                c.gmres_solve(psi, st, ss, abs_tol=1e-13 if prec == tsw.FP64 else 1e-11, precision=prec)
+                assert rel_err(psi, ref_g) <= tol, ("gmres", br, prec)
+
+
+def test_band_tuning_errors():
+    o = ls_ordinates(2)
+    g, st, _, src = random_problem(5, 5, 0, o, seed=1)
+    with _ctx(g, o, 0) as c:
+        with pytest.raises(tsw.TswError):
+            c.set_tuning("band_rows", -2)
+        with pytest.raises(tsw.TswError):
+            c.set_tuning("no_such_key", 1)
