@@ -88,6 +88,7 @@ struct tsw_ctx {
     // > 0 = rows per band.  Exchange records per precision (sentinel-filled when empty), the stall flag
     // and its pinned host mirror, read back at every synchronisation when bands ran.
     int band_rows_opt = 0;
+    int prefetch_opt = 0;  // prefetch depth of the staggered sweep (0: default)
     DBuf xtr64, xtr32, abort_dev;
     size_t xtr64_n = 0, xtr32_n = 0;
     int *abort_host = nullptr;
@@ -570,6 +571,7 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
         sp.tin_y = (const T *)(f64 ? d.tiny64.ptr : d.tiny32.ptr);
     }
     configure_bands<T>(c, sp);
+    sp.prefetch = c->prefetch_opt;
     sp.ainv = diag_inverses<T>(c, dt);
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
@@ -953,11 +955,13 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
         a->own_stream = false;
         a->pattern = ctx->pattern;
         a->band_rows_opt = ctx->band_rows_opt;
+        a->prefetch_opt = ctx->prefetch_opt;
         set_problem(a.get(), g.nx, g.ny, g.ax, g.bx, g.ay, g.by, g.p, nq, xi.data(), eta.data(), w.data(), off);
         ctx->adj = std::move(a);
     }
     ctx->adj->pattern = ctx->pattern;
     ctx->adj->band_rows_opt = ctx->band_rows_opt;
+    ctx->adj->prefetch_opt = ctx->prefetch_opt;
     return ctx->adj.get();
 }
 
@@ -1073,6 +1077,10 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         if (k == "band_rows") {
             require(value >= -1, TSW_E_ARG, "band_rows must be -1, 0 or a row count");
             ctx->band_rows_opt = value;
+        } else if (k == "prefetch") {
+            require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
+                    "prefetch must be 0, 2, 3, 4 or 6");
+            ctx->prefetch_opt = value;
         } else {
             throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
         }
@@ -1398,6 +1406,7 @@ void batch_sweep(tsw_ctx *ctx) {
         DiagSweepParams<T> rep{};
         rep.qsrc = reinterpret_cast<const T *>(1);
         rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
+        rep.prefetch = ctx->prefetch_opt;
         launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny,
                                    sizeof(T) == 8 ? G.lds64 : G.lds32, G.p, sd);
         ++k;
@@ -1420,19 +1429,37 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         const bool f64 = precision == TSW_FP64;
         std::vector<double *> dev(ctx->batch.size());
-        for (size_t b = 0; b < ctx->batch.size(); ++b) {
-            tsw_ctx *sub = ctx->batch[b].get();
+        // per-member layout conversions spread over the side streams, joined before the next phase
+        auto on_side_streams = [&](auto &&fn) {
+            TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+            for (auto sd : ctx->side) TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
+            for (size_t b = 0; b < ctx->batch.size(); ++b) {
+                tsw_ctx *sub = ctx->batch[b].get();
+                sub->stream = ctx->side[b % tsw_ctx::kSideStreams];
+                try {
+                    fn(b, sub);
+                } catch (...) {
+                    sub->stream = ctx->stream;
+                    throw;
+                }
+                sub->stream = ctx->stream;
+            }
+            for (int j = 0; j < tsw_ctx::kSideStreams; ++j) {
+                TSW_HIP(hipEventRecord(ctx->join_ev[j], ctx->side[j]));
+                TSW_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0));
+            }
+        };
+        on_side_streams([&](size_t b, tsw_ctx *sub) {
             dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
             diag_load_source(sub, dev[b], f64, !f64);
-        }
+        });
         if (f64) batch_sweep<double>(ctx);
         else batch_sweep<float>(ctx);
-        for (size_t b = 0; b < ctx->batch.size(); ++b) {
-            tsw_ctx *sub = ctx->batch[b].get();
+        on_side_streams([&](size_t b, tsw_ctx *sub) {
             if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
             else diag_store<float>(sub, sub->diag.q32.as<float>(), dev[b]);
             if (mem == TSW_MEM_HOST) stage_out(sub, psi[b], dev[b], sub->g.npsi(), mem);
-        }
+        });
         sync(ctx);
     });
 }

@@ -114,6 +114,7 @@ template <typename T> struct DiagSweepParams {
     int band_rows, nbands;
     T *xtr;
     int *abort_flag;
+    int prefetch;          // prefetch depth in steps (0: default for the precision)
 };
 
 // Bit pattern of an empty band-exchange record: a signalling NaN, which arithmetic never produces.

@@ -74,8 +74,9 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
 /* Tuning of the staggered sweep.  key "band_rows": rows of an ordinate's mesh handled by one workgroup;
  * a sweep then runs ceil(ny / band_rows) pipelined workgroups per ordinate (the KBA decomposition of
  * SweepPatternUniCartKBA2.cpp:212-453 applied to rows within the device).  0 (default) chooses
- * automatically, -1 keeps one workgroup per ordinate (bands only when ny > 448).  Results do not
- * depend on the setting.  Unknown keys fail with TSW_E_ARG. */
+ * automatically, -1 keeps one workgroup per ordinate (bands only when ny > 448).  Key "prefetch": steps
+ * of cell inputs each row keeps in flight (2, 3, 4 or 6; 0 = 2 in FP64, 3 in FP32).  Results do not
+ * depend on either setting.  Unknown keys fail with TSW_E_ARG. */
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior

@@ -81,3 +81,26 @@ def test_band_tuning_errors():
             c.set_tuning("band_rows", -2)
         with pytest.raises(tsw.TswError):
             c.set_tuning("no_such_key", 1)
+        with pytest.raises(tsw.TswError):
+            c.set_tuning("prefetch", 5)
+
+
+@pytest.mark.parametrize("prefetch", [2, 3, 4, 6])
+@pytest.mark.parametrize("band_rows", [-1, 64])
+def test_prefetch_depths_match_oracle(prefetch, band_rows):
+    """Every prefetch depth (steps of inputs in flight per row) gives the same sweep, both precisions,
+    with and without the scattering source of the GMRES matvec."""
+    o = ls_ordinates(4)
+    for p in (1, 2, 3):
+        g, st, ss, src = random_problem(23, 90, p, o, seed=900 + p + prefetch, materials=3)
+        with _ctx(g, o, band_rows) as c:
+            c.set_tuning("prefetch", prefetch)
+            ref = orc_sweep(g, o, st, src, 0.7)
+            for prec, tol in ((tsw.FP64, 1e-12), (tsw.FP32, 2e-6)):
+                out = src.copy()
+                c.sweep(out, st, 0.7, precision=prec)
+                assert rel_err(out, ref) <= tol, (p, prec)
+            rc, ref_g, _, _, _ = orc_gmres(g, o, st, ss, src, abs_tol=1e-13, restart=30)
+            psi = src.copy()
+            c.gmres_solve(psi, st, ss, abs_tol=1e-13)
+            assert rel_err(psi, ref_g) <= 1e-10
