@@ -55,6 +55,51 @@ def cpu_baseline(P, sweeps):
                       f"(1 thread, {dt:.1f} s) on {cpu_model()}"}
 
 
+def bench_si(args):
+    """Source iteration on the CPU-runnable case (64x64, Q1, S8, sigma_t=1, sigma_s=0.5, central source,
+    vacuum) to 1e-10: graph-replayed iterations (device-side loop test) against the eager loop."""
+    import torch
+    import tsw
+    from tsw import problems
+    P = problems.central_source()
+    dev = torch.device("cuda:0")
+    d_src = P.source_device(dev)
+    d_st = torch.from_numpy(P.sigma_t).to(dev)
+    d_ss = torch.from_numpy(P.sigma_s).to(dev)
+    work = torch.empty_like(d_src)
+    out = {}
+    for mode in (1, 0):
+        ctx = tsw.Context(0)
+        P.set_on(ctx)
+        ctx.set_tuning("si_graph", mode)
+        its = []
+        for rep in range(args.warmup + args.steps):
+            if rep == args.warmup:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            ctx.problem_set(P.nx, P.ny, P.p, P.xi, P.eta, P.w, P.quad_off)   # zero solver state
+            work.copy_(d_src)
+            it, err = ctx.si_solve(work, d_st, d_ss, abs_tol=1e-10, max_its=1000)
+            its.append(it)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        out["graph" if mode else "eager"] = (wall, its[-1])
+        print(f"bench: si {'graph' if mode else 'eager'}: {wall * 1e3:.3f} ms per solve, {its[-1]} iterations",
+              file=sys.stderr, flush=True)
+        ctx.close()
+    U = P.unknowns
+    wg, ig = out["graph"]
+    we, ie = out["eager"]
+    print(json.dumps({"metric": "angular unknowns swept/s (source iteration)", "value": U * ig / wg,
+                      "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": wg * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "central source 64x64, Q1, S8, sigma_t=1, sigma_s=0.5, SI to 1e-10",
+                                 "iterations": ig},
+                      "eager": {"value": U * ie / we, "ms_per_step": we * 1e3, "iterations": ie},
+                      "speedup_graph_vs_eager": we / wg}), flush=True)
+
+
 def bench_batched(args):
     """Batched sweep of the seeded 256-problem uneven batch (N 32..256, Q1..Q3, S4..S12, 8x8 block
     materials): one step = one sweep of every member, inputs resident in HBM."""
@@ -135,9 +180,10 @@ def main():
     ap.add_argument("--roofline-sweeps", type=int, default=20, help="operator-contract sweeps timed per precision")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
-    ap.add_argument("--workload", default="lattice", choices=["lattice", "batched"])
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "batched", "si"])
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
+    ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
     ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "
                                                      "for each and report them (tuning aid)")
     ap.add_argument("--prefetch-scan", default="", help="comma-separated prefetch depths crossed with --band-scan")
@@ -148,6 +194,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "si":
+        bench_si(args)
         return
 
     import torch
@@ -215,7 +264,10 @@ def main():
         ctx.set_tuning("band_rows", args.band_rows)
         ctx.set_tuning("prefetch", args.prefetch)
     t_sw32 = contract_sweeps(tsw.FP32)
+    tuned32 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
     t_sw64 = contract_sweeps(tsw.FP64)
+    tuned64 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
+    print(f"bench: sweep tuning (band_rows, prefetch): fp32 {tuned32} fp64 {tuned64}", file=sys.stderr, flush=True)
     print(f"bench: contract sweep fp32 {t_sw32:.3f} ms fp64 {t_sw64:.3f} ms", file=sys.stderr, flush=True)
     if args.copy_probe:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
@@ -229,6 +281,10 @@ def main():
         ctx.close()
         return
 
+    ctx.set_tuning("use_dsa", 0)
+    w64n, n64n, ms64n, its64n = timed(tsw.FP64, phi64)
+    print(f"bench: fp64 without DSA: wall {w64n:.3f}s sweeps {n64n} its {its64n}", file=sys.stderr, flush=True)
+    ctx.set_tuning("use_dsa", args.dsa)
     w64, n64, ms64, its64 = timed(tsw.FP64, phi64)
     print(f"bench: fp64 wall {w64:.3f}s sweeps {n64} sweep-ms {ms64:.1f} its {its64}", file=sys.stderr, flush=True)
     w32, n32, ms32, its32 = timed(tsw.FP32, phi32)
@@ -257,13 +313,15 @@ def main():
         "dtype": "f32 sweep / f64 refinement",
         "data": "synthetic (lattice cross sections and source generated in-process)",
         "config": {"workload": "checkerboard lattice 280x280 cells, Q2 DG, S12 (84 ordinates), Peierls GMRES(30) "
-                               "to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
+                               "with DSA to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
                    "sweep_pattern": args.pattern, "band_rows": args.band_rows, "prefetch": args.prefetch,
-                   "dsa": False,
+                   "tuned_band_rows_prefetch": {"fp32": tuned32, "fp64": tuned64}, "dsa": bool(args.dsa),
                    "unknowns_per_sweep": U},
         "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
                  "gmres_its": [i for i, _ in its64], "solve_sweep_ms": ms64 / max(n64, 1),
                  "contract_sweep_ms": t_sw64, "contract_sweep_gbs": achieved64},
+        "fp64_no_dsa": {"value": U * n64n / w64n, "ms_per_step": w64n / args.steps * 1e3,
+                        "sweeps_per_solve": n64n / args.steps, "gmres_its": [i for i, _ in its64n]},
         "mixed": {"sweeps_per_solve": n32 / args.steps, "gmres_its": [i for i, _ in its32],
                   "solve_sweep_ms": ms32 / max(n32, 1), "contract_sweep_ms": t_sw32},
         "speedup_mixed_vs_fp64": (w64 / w32),

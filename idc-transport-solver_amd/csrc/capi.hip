@@ -1,6 +1,8 @@
 // C-ABI of libtsw.so: context, problem set-up, sweep dispatch and the solver drivers that run on the
 // device (source iteration, Peierls GMRES with FP64 or mixed-precision sweeps).
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -89,6 +91,37 @@ struct tsw_ctx {
     // and its pinned host mirror, read back at every synchronisation when bands ran.
     int band_rows_opt = 0;
     int prefetch_opt = 0;  // prefetch depth of the staggered sweep (0: default)
+    // Autotuner: with band_rows_opt and prefetch_opt both automatic, the first staggered sweep of each
+    // kind (mesh, degree, ordinates, precision, source mode, material path) times the candidate
+    // (band rows, prefetch) pairs and keeps the fastest; tuned_br/tuned_pf hold the choice in force
+    // (INT_MIN: none), last_br/last_pf what the last sweep used.
+    bool autotune = true;
+    std::map<std::array<int, 7>, std::pair<int, int>> tuned;
+    int tuned_br = INT_MIN, tuned_pf = INT_MIN;
+    int last_br = 0, last_pf = 0;
+    DBuf tune_out;
+    // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
+    // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
+    bool si_graph = true;
+    static constexpr int kSiChunk = 8;
+    DBuf si_ctl;
+    SiControl *si_ctl_host = nullptr;
+    const int *skip_flag = nullptr;
+    // Diffusion synthetic acceleration of the Peierls GMRES (use_dsa, PETScPeierlsSolver.cpp:151):
+    // multigrid hierarchy of the diffusion operator for the current cross sections, and the apply
+    // y = x + C^{-1} sigma_s x captured as a graph over staging vectors.
+    bool use_dsa = false;
+    int dsa_cycles = 2;
+    struct Dsa {
+        std::vector<MgLevel> lv;
+        std::vector<DBuf> store;
+        DBuf in, out;
+        const double *sigma_s = nullptr;
+        hipGraphExec_t graph = nullptr;
+        ~Dsa() {
+            if (graph) (void)hipGraphExecDestroy(graph);
+        }
+    } dsa;
     DBuf xtr64, xtr32, abort_dev;
     size_t xtr64_n = 0, xtr32_n = 0;
     int *abort_host = nullptr;
@@ -108,6 +141,7 @@ struct tsw_ctx {
         batch.clear();
         adj.reset();
         if (abort_host) (void)hipHostFree(abort_host);
+        if (si_ctl_host) (void)hipHostFree(si_ctl_host);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -337,6 +371,161 @@ double device_norm2_interior_diff(tsw_ctx *c, const double *a, const double *b) 
 // Restarted GMRES on phi-shaped FP64 vectors (zero halos), CGS2 orthogonalisation with batched dot
 // products so each Arnoldi step costs two host round trips.
 // ---------------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------------
+// DSA preconditioner (dsa.hip): hierarchy construction on the host from the cross sections, V-cycles,
+// and the captured apply.
+// ---------------------------------------------------------------------------------------------------
+void dsa_vcycle(tsw_ctx *c, size_t l, bool zero_init) {
+    auto &lv = c->dsa.lv;
+    const MgLevel &L = lv[l];
+    hipStream_t s = c->stream;
+    if (zero_init) launch_mg_zero(L.x, L.nx * L.ny, s);
+    if (l + 1 == lv.size()) {  // coarsest: a few cells, Gauss-Seidel to convergence
+        for (int k = 0; k < 32; ++k) {
+            launch_mg_rbgs(L, L.x, L.b, 0, s);
+            launch_mg_rbgs(L, L.x, L.b, 1, s);
+        }
+        return;
+    }
+    for (int k = 0; k < 2; ++k) {
+        launch_mg_rbgs(L, L.x, L.b, 0, s);
+        launch_mg_rbgs(L, L.x, L.b, 1, s);
+    }
+    launch_mg_residual(L, L.x, L.b, L.r, s);
+    launch_mg_restrict(L, lv[l + 1], L.r, lv[l + 1].b, s);
+    dsa_vcycle(c, l + 1, true);
+    launch_mg_prolong_add(L, lv[l + 1], lv[l + 1].x, L.x, s);
+    for (int k = 0; k < 2; ++k) {
+        launch_mg_rbgs(L, L.x, L.b, 1, s);
+        launch_mg_rbgs(L, L.x, L.b, 0, s);
+    }
+}
+
+// Builds the diffusion hierarchy for device cross sections (cell arrays) and time step dt, and captures
+// the apply graph out = in + C^{-1} sigma_s in (cell means).
+void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double dt) {
+    const Geom &g = c->g;
+    auto &D = c->dsa;
+    if (D.graph) {
+        (void)hipGraphExecDestroy(D.graph);
+        D.graph = nullptr;
+    }
+    const size_t nh = g.ncell_h();
+    std::vector<double> st(nh), ss(nh);
+    TSW_HIP(hipMemcpyAsync(st.data(), sigma_t, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
+    TSW_HIP(hipMemcpyAsync(ss.data(), sigma_s, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    const double idt = inverse_dt(dt);
+    struct HostLevel {
+        int nx, ny;
+        std::vector<double> cx, cy, sv;
+    };
+    std::vector<HostLevel> hl(1);
+    {
+        HostLevel &L = hl[0];
+        L.nx = g.nx;
+        L.ny = g.ny;
+        L.cx.assign((size_t)(g.nx + 1) * g.ny, 0.0);
+        L.cy.assign((size_t)g.nx * (g.ny + 1), 0.0);
+        L.sv.assign((size_t)g.nx * g.ny, 0.0);
+        auto Dc = [&](int i, int j) {  // diffusion coefficient of interior cell (i,j), 0-based
+            const double t = st[(size_t)(i + 1) * (g.ny + 2) + j + 1] + idt;
+            return 1.0 / (3.0 * std::max(t, 1e-12));
+        };
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i <= g.nx; ++i) {
+                double rsum;
+                if (i == 0) rsum = g.dx / (2 * Dc(0, j)) + 2.0;
+                else if (i == g.nx) rsum = g.dx / (2 * Dc(g.nx - 1, j)) + 2.0;
+                else rsum = g.dx / (2 * Dc(i - 1, j)) + g.dx / (2 * Dc(i, j));
+                L.cx[(size_t)i * g.ny + j] = g.dy / rsum;
+            }
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j <= g.ny; ++j) {
+                double rsum;
+                if (j == 0) rsum = g.dy / (2 * Dc(i, 0)) + 2.0;
+                else if (j == g.ny) rsum = g.dy / (2 * Dc(i, g.ny - 1)) + 2.0;
+                else rsum = g.dy / (2 * Dc(i, j - 1)) + g.dy / (2 * Dc(i, j));
+                L.cy[(size_t)i * (g.ny + 1) + j] = g.dx / rsum;
+            }
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j) {
+                const size_t h = (size_t)(i + 1) * (g.ny + 2) + j + 1;
+                L.sv[(size_t)i * g.ny + j] = std::max(st[h] + idt - ss[h], 0.0) * g.dx * g.dy;
+            }
+    }
+    // 2x2 aggregation: coarse conductances sum the fine faces they contain, absorption sums the cells
+    while ((hl.back().nx > 3 || hl.back().ny > 3) && hl.size() < 16) {
+        const HostLevel &f = hl.back();
+        HostLevel L;
+        L.nx = (f.nx + 1) / 2;
+        L.ny = (f.ny + 1) / 2;
+        L.cx.assign((size_t)(L.nx + 1) * L.ny, 0.0);
+        L.cy.assign((size_t)L.nx * (L.ny + 1), 0.0);
+        L.sv.assign((size_t)L.nx * L.ny, 0.0);
+        for (int I = 0; I <= L.nx; ++I) {
+            const int fi = I == L.nx ? f.nx : 2 * I;
+            for (int j = 0; j < f.ny; ++j) L.cx[(size_t)I * L.ny + j / 2] += f.cx[(size_t)fi * f.ny + j];
+        }
+        for (int J = 0; J <= L.ny; ++J) {
+            const int fj = J == L.ny ? f.ny : 2 * J;
+            for (int i = 0; i < f.nx; ++i) L.cy[(size_t)(i / 2) * (L.ny + 1) + J] += f.cy[(size_t)i * (f.ny + 1) + fj];
+        }
+        for (int i = 0; i < f.nx; ++i)
+            for (int j = 0; j < f.ny; ++j) L.sv[(size_t)(i / 2) * L.ny + j / 2] += f.sv[(size_t)i * f.ny + j];
+        hl.push_back(std::move(L));
+    }
+    D.lv.assign(hl.size(), MgLevel{});
+    D.store.clear();
+    D.store.resize(hl.size() * 6);
+    for (size_t l = 0; l < hl.size(); ++l) {
+        const HostLevel &h = hl[l];
+        MgLevel &L = D.lv[l];
+        L.nx = h.nx;
+        L.ny = h.ny;
+        auto up = [&](DBuf &b, const std::vector<double> &v) {
+            double *d = static_cast<double *>(b.reserve(sizeof(double) * v.size()));
+            TSW_HIP(hipMemcpyAsync(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, c->stream));
+            return d;
+        };
+        L.cx = up(D.store[6 * l + 0], h.cx);
+        L.cy = up(D.store[6 * l + 1], h.cy);
+        L.sv = up(D.store[6 * l + 2], h.sv);
+        const size_t n = (size_t)h.nx * h.ny;
+        L.x = static_cast<double *>(D.store[6 * l + 3].reserve(sizeof(double) * n));
+        L.b = static_cast<double *>(D.store[6 * l + 4].reserve(sizeof(double) * n));
+        L.r = static_cast<double *>(D.store[6 * l + 5].reserve(sizeof(double) * n));
+    }
+    sync(c);  // host vectors go out of scope
+    D.sigma_s = sigma_s;
+    double *in = static_cast<double *>(D.in.reserve(sizeof(double) * g.nphi()));
+    double *out = static_cast<double *>(D.out.reserve(sizeof(double) * g.nphi()));
+    hipGraph_t graph;
+    TSW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        TSW_HIP(hipMemcpyAsync(out, in, sizeof(double) * g.nphi(), hipMemcpyDeviceToDevice, c->stream));
+        launch_dsa_rhs(g, in, sigma_s, D.lv[0].b, c->stream);
+        for (int k = 0; k < c->dsa_cycles; ++k) dsa_vcycle(c, 0, k == 0);
+        launch_dsa_add(g, D.lv[0].x, out, c->stream);
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &graph);
+        throw;
+    }
+    TSW_HIP(hipStreamEndCapture(c->stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&D.graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    TSW_HIP(ie);
+}
+
+// y = x + C^{-1} sigma_s x on phi-layout vectors (y may equal x)
+void dsa_apply(tsw_ctx *c, const double *x, double *y) {
+    auto &D = c->dsa;
+    const size_t n = c->g.nphi();
+    TSW_HIP(hipMemcpyAsync(D.in.ptr, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    TSW_HIP(hipGraphLaunch(D.graph, c->stream));
+    TSW_HIP(hipMemcpyAsync(y, D.out.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+}
+
 struct Gmres {
     tsw_ctx *c;
     size_t n;
@@ -510,7 +699,7 @@ const T *diag_inverses(tsw_ctx *c, double dt) {
 // spread over several times more compute units than there are ordinates.
 int band_rows_for(const tsw_ctx *c) {
     const int ny = c->g.ny;
-    int br = c->band_rows_opt;
+    int br = c->tuned_br != INT_MIN ? c->tuned_br : c->band_rows_opt;
     if (br < 0) br = ny > 448 ? 448 : 0;  // a single band cannot hold more than 448 rows
     else if (br == 0) br = ny > 64 ? 64 : 0;
     if (br >= ny) br = 0;
@@ -571,7 +760,9 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
         sp.tin_y = (const T *)(f64 ? d.tiny64.ptr : d.tiny32.ptr);
     }
     configure_bands<T>(c, sp);
-    sp.prefetch = c->prefetch_opt;
+    sp.prefetch = c->tuned_pf != INT_MIN ? c->tuned_pf : c->prefetch_opt;
+    c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
+    c->last_pf = sp.prefetch > 0 ? sp.prefetch : (sizeof(T) == 8 ? 2 : 3);
     sp.ainv = diag_inverses<T>(c, dt);
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
@@ -583,10 +774,73 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
     return sp;
 }
 
+// Autotuning of the staggered sweep: the candidates are one band per ordinate and bands of 64..256 rows
+// (below ny), crossed with prefetch depths 2, 3, 4, 6; each runs twice into a scratch output (inputs
+// untouched) and the second run is timed with events on the context's stream.  Every candidate gives
+// the same result, so the choice only affects speed.
+template <typename T>
+void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow) {
+    c->tuned_br = c->tuned_pf = INT_MIN;
+    if (!c->autotune || c->band_rows_opt != 0 || c->prefetch_opt != 0) return;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    TSW_HIP(hipStreamIsCapturing(c->stream, &cap));
+    const Geom &g = c->g;
+    const int mode = (qsrc ? 1 : 0) | (phi_d ? 2 : 0);
+    const std::array<int, 7> key{g.nx, g.ny, g.p, g.nq, (int)sizeof(T), mode, c->mat.valid ? c->mat.M : 0};
+    auto it = c->tuned.find(key);
+    if (it == c->tuned.end()) {
+        if (cap != hipStreamCaptureStatusNone) return;  // no timing inside a capture: defaults
+        std::vector<int> brs{-1};
+        for (int br = 64; br <= 256 && br < g.ny; br += 64) brs.push_back(br);
+        const int pfs[] = {2, 3, 4, 6};
+        T *out = dbuf<T>(c->tune_out, (size_t)g.nq * g.N() * g.nx * g.ny);
+        hipEvent_t e0, e1;
+        TSW_HIP(hipEventCreate(&e0));
+        TSW_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        std::pair<int, int> choice{0, 0};
+        try {
+            for (int br : brs)
+                for (int pf : pfs) {
+                    c->tuned_br = br;
+                    c->tuned_pf = pf;
+                    const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, out);
+                    launch_diag_sweep<T>(sp, g.p, c->stream);
+                    TSW_HIP(hipEventRecord(e0, c->stream));
+                    launch_diag_sweep<T>(sp, g.p, c->stream);
+                    TSW_HIP(hipEventRecord(e1, c->stream));
+                    TSW_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    TSW_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        choice = {br, pf};
+                    }
+                }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            c->tuned_br = c->tuned_pf = INT_MIN;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        it = c->tuned.emplace(key, choice).first;
+    }
+    c->tuned_br = it->second.first;
+    c->tuned_pf = it->second.second;
+}
+
 // One staggered sweep on the context's stream, timed.
 template <typename T>
 void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow, T *psi_out) {
-    const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, psi_out);
+    select_tuning<T>(c, dt, qsrc, phi_d, inflow);
+    DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, psi_out);
+    sp.skip = c->skip_flag;
+    if (c->skip_flag) {  // inside a graph capture: the replay is timed as a whole
+        launch_diag_sweep<T>(sp, c->g.p, c->stream);
+        return;
+    }
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     TSW_HIP(hipEventRecord(e0, c->stream));
     launch_diag_sweep<T>(sp, c->g.p, c->stream);
@@ -601,7 +855,7 @@ T *diag_phi(tsw_ctx *c, const double *phi) {
     auto &d = c->diag;
     const size_t n = 4 * (size_t)g.nx * g.ny * g.N();
     T *out = dbuf<T>(sizeof(T) == 8 ? d.phid64 : d.phid32, n);
-    launch_cell_to_diag<double, T>(g, phi, out, g.N(), c->stream);
+    launch_cell_to_diag<double, T>(g, phi, out, g.N(), c->stream, c->skip_flag);
     return out;
 }
 
@@ -610,7 +864,7 @@ template <typename T>
 void diag_pmv(tsw_ctx *c, const T *psi_d, double *phi, double alpha, double beta) {
     const Geom &g = c->g;
     double *phiQ = dbuf<double>(c->diag.phiQ, 4 * (size_t)g.nx * g.ny * g.N());
-    launch_pmv_diag<T>(g, psi_d, c->w64.as<double>(), phiQ, phi, alpha, beta, c->stream);
+    launch_pmv_diag<T>(g, psi_d, c->w64.as<double>(), phiQ, phi, alpha, beta, c->stream, c->skip_flag);
 }
 
 // out (reference layout, interior) <- psi_d; halos of out untouched
@@ -684,6 +938,97 @@ void si_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, dou
     err = e;
 }
 
+// Source iteration replayed from a graph: kSiChunk iterations (an even number, so the scalar-flux buffers
+// alternate the same way in every replay) are captured once per call, each iteration being the
+// conversion of phi, the fused sweep L^{-1}(Q + S phi), the moment contraction and the device-side loop
+// test; replays continue until the test stops the loop, and iterations after that point do nothing, so
+// the iterates, the count and the last change equal the eager loop's.  Timing: each replay is one
+// interval of tsw_sweep_timing, counted as the iterations it ran.
+template <typename T>
+void si_staggered_graph(tsw_ctx *c, double *Q, const double *st, const double *ss, double dt, double abs_tol,
+                        int max_its, int &iters, double &err) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    const bool f64 = sizeof(T) == 8;
+    diag_setup(c, st, ss, Q, dt, f64, !f64);
+    double *psi_state = c->si_state.as<double>();
+    double *bufA = dbuf<double>(c->phi64a, g.nphi());
+    double *bufB = dbuf<double>(c->phi64b, g.nphi());
+    TSW_HIP(hipMemsetAsync(bufA, 0, sizeof(double) * g.nphi(), s));
+    TSW_HIP(hipMemsetAsync(bufB, 0, sizeof(double) * g.nphi(), s));
+    launch_pmv<double, double>(g, psi_state, bufA, c->w64.as<double>(), 1.0, 0.0, 0, s);  // :214
+    const size_t npsi_d = (size_t)g.nq * g.N() * g.nx * g.ny;
+    T *psi_d = dbuf<T>(f64 ? c->diag.psi64 : c->diag.psi32, npsi_d);
+    const T *q_d = (const T *)(f64 ? c->diag.q64.ptr : c->diag.q32.ptr);
+    const int nb = reduction_blocks();
+    double *part = dbuf<double>(c->red_part, (size_t)nb * 64);
+    // everything the captured kernels use exists before the capture: buffers, tables, tuning
+    dbuf<T>(f64 ? c->diag.phid64 : c->diag.phid32, 4 * (size_t)g.nx * g.ny * g.N());
+    dbuf<double>(c->diag.phiQ, 4 * (size_t)g.nx * g.ny * g.N());
+    select_tuning<T>(c, dt, q_d, diag_phi<T>(c, bufA), true);
+    (void)make_diag_params<T>(c, dt, q_d, (const T *)(f64 ? c->diag.phid64.ptr : c->diag.phid32.ptr), true, psi_d);
+    SiControl *ctl = static_cast<SiControl *>(c->si_ctl.reserve(sizeof(SiControl)));
+    if (!c->si_ctl_host) TSW_HIP(hipHostMalloc((void **)&c->si_ctl_host, sizeof(SiControl), hipHostMallocDefault));
+    SiControl init{};
+    init.done = max_its <= 0 ? 1 : 0;
+    init.iters = 0;
+    init.max_its = max_its;
+    init.tol = abs_tol;
+    init.err = 1.79769313486231570e+308;
+    TSW_HIP(hipMemcpyAsync(ctl, &init, sizeof(SiControl), hipMemcpyHostToDevice, s));
+    TSW_HIP(hipStreamSynchronize(s));
+
+    // iteration k (1-based) writes phi into bufB when k is odd, bufA when even (the eager loop's swap)
+    hipGraph_t graph;
+    c->skip_flag = &ctl->done;
+    TSW_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int j = 0; j < tsw_ctx::kSiChunk; ++j) {
+            double *cur = (j & 1) ? bufA : bufB, *last = (j & 1) ? bufB : bufA;
+            const T *phi_d = diag_phi<T>(c, last);
+            diag_sweep<T>(c, dt, q_d, phi_d, true, psi_d);  // L^{-1}(Q + S phi), :226-232
+            diag_pmv<T>(c, psi_d, cur, 1.0, 0.0);          // :236
+            launch_si_check(g, cur, last, part, nb, ctl, s); // :239 and the loop test
+        }
+    } catch (...) {
+        c->skip_flag = nullptr;
+        (void)hipStreamEndCapture(s, &graph);
+        throw;
+    }
+    c->skip_flag = nullptr;
+    TSW_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    TSW_HIP(ie);
+    try {
+        SiControl &h = *c->si_ctl_host;
+        h = init;
+        while (!h.done) {
+            const int before = h.iters;
+            hipEvent_t e0 = next_event(c), e1 = next_event(c);
+            TSW_HIP(hipEventRecord(e0, s));
+            TSW_HIP(hipGraphLaunch(exec, s));
+            TSW_HIP(hipEventRecord(e1, s));
+            TSW_HIP(hipMemcpyAsync(&h, ctl, sizeof(SiControl), hipMemcpyDeviceToHost, s));
+            sync(c);
+            c->sweeps_done += h.iters - before;
+        }
+    } catch (...) {
+        (void)hipGraphExecDestroy(exec);
+        throw;
+    }
+    (void)hipGraphExecDestroy(exec);
+    const SiControl &h = *c->si_ctl_host;
+    if (h.iters > 0) {
+        TSW_HIP(hipMemcpyAsync(psi_state, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+        diag_store<T>(c, psi_d, psi_state);
+    }
+    TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    iters = h.iters;
+    err = h.err;
+}
+
 // Peierls GMRES on the staggered engine.  FP64: sweeps in double.  FP32: an FP64 iterate psi64 (diagonal
 // layout) refined with corrections whose sweeps run in float; the residual r = Q + S P psi - (L + 1/dt) psi
 // is formed in FP64.  Q: reference layout with inflow halos, overwritten by psi; phi (phi layout) receives
@@ -701,6 +1046,10 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
     Gmres solver(c, n, restart);
     bool converged = false;
     its_total = 0;
+    // left DSA preconditioning, as PETSc applies the reference's shell preconditioner (the stop test is
+    // then on the preconditioned residual, PETSc's default norm)
+    const bool dsa = c->use_dsa;
+    if (dsa) dsa_build(c, st, ss, dt);
     if (precision == TSW_FP64) {
         diag_setup(c, st, ss, Q, dt, true, false);
         double *psi_d = dbuf<double>(c->diag.psi64, npsi_d);
@@ -710,9 +1059,11 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
             diag_sweep<double>(c, dt, nullptr, phi_d, false, psi_d);
             TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             diag_pmv<double>(c, psi_d, yv, -1.0, 1.0);
+            if (dsa) dsa_apply(c, yv, yv);
         };
         diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);  // b = P L^{-1} Q (:461-466)
         diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
+        if (dsa) dsa_apply(c, b, b);
         converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
         const double *phi_d = diag_phi<double>(c, x);            // psi = L^{-1}(Q + S phi) (:504-505)
         diag_sweep<double>(c, dt, q_d, phi_d, true, psi_d);
@@ -729,6 +1080,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
             diag_sweep<float>(c, dt, nullptr, phi_d, false, psi32);
             TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             diag_pmv<float>(c, psi32, yv, -1.0, 1.0);
+            if (dsa) dsa_apply(c, yv, yv);
         };
         const int max_refine = 30;
         for (int refine = 0;; ++refine) {
@@ -743,6 +1095,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
             diag_sweep<float>(c, dt, res32, nullptr, false, psi32);
             TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
             diag_pmv<float>(c, psi32, b, 1.0, 0.0);
+            if (dsa) dsa_apply(c, b, b);
             res_final = solver.norm(b);
             if (res_final <= abs_tol) {
                 converged = true;
@@ -778,6 +1131,8 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
     const double *w = ctx->w64.as<double>();
     bool converged = false;
     its_total = 0;
+    const bool dsa = ctx->use_dsa;
+    if (dsa) dsa_build(ctx, st, ss, dt);
     double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
     // vectors: b, x, r (phi-shaped, zero halos)
     DBuf vecs;
@@ -792,6 +1147,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         sweep_prec(ctx, psi, st, dt, precision);
         TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         launch_pmv<double, double>(g, psi, yv, w, -1.0, 1.0, 0, s);
+        if (dsa) dsa_apply(ctx, yv, yv);
     };
 
     if (precision == TSW_FP64) {
@@ -799,6 +1155,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
         sweep_prec(ctx, psi, st, dt, precision);
         launch_pmv<double, double>(g, psi, b, w, 1.0, 0.0, 0, s);
+        if (dsa) dsa_apply(ctx, b, b);
         converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
         // source <- L^{-1}(Q + S phi) (:504-505)
         launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
@@ -824,6 +1181,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
             sweep_dev<float>(ctx, p32, st32, dt);
             TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             launch_pmv<float, double>(g, p32, yv, w, -1.0, 1.0, 0, s);
+            if (dsa) dsa_apply(ctx, yv, yv);
         };
 
         // initial iterate: inflow halos from Q, zero interior
@@ -843,6 +1201,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
             sweep_dev<float>(ctx, p32, st32, dt);
             TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
             launch_pmv<float, double>(g, p32, b, w, 1.0, 0.0, 0, s);
+            if (dsa) dsa_apply(ctx, b, b);
             res_final = solver.norm(b);
             if (res_final <= abs_tol) {
                 converged = true;
@@ -909,6 +1268,7 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     }
     ctx->clear_graphs();
     ctx->adj.reset();
+    ctx->tuned.clear();
     Geom &g = ctx->g;
     g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
     g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
@@ -956,12 +1316,18 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
         a->pattern = ctx->pattern;
         a->band_rows_opt = ctx->band_rows_opt;
         a->prefetch_opt = ctx->prefetch_opt;
+        a->autotune = ctx->autotune;
+        a->use_dsa = ctx->use_dsa;
+        a->dsa_cycles = ctx->dsa_cycles;
         set_problem(a.get(), g.nx, g.ny, g.ax, g.bx, g.ay, g.by, g.p, nq, xi.data(), eta.data(), w.data(), off);
         ctx->adj = std::move(a);
     }
     ctx->adj->pattern = ctx->pattern;
     ctx->adj->band_rows_opt = ctx->band_rows_opt;
     ctx->adj->prefetch_opt = ctx->prefetch_opt;
+    ctx->adj->autotune = ctx->autotune;
+    ctx->adj->use_dsa = ctx->use_dsa;
+    ctx->adj->dsa_cycles = ctx->dsa_cycles;
     return ctx->adj.get();
 }
 
@@ -1077,6 +1443,18 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         if (k == "band_rows") {
             require(value >= -1, TSW_E_ARG, "band_rows must be -1, 0 or a row count");
             ctx->band_rows_opt = value;
+        } else if (k == "use_dsa") {
+            require(value == 0 || value == 1, TSW_E_ARG, "use_dsa must be 0 or 1");
+            ctx->use_dsa = value != 0;
+        } else if (k == "dsa_cycles") {
+            require(value >= 1 && value <= 16, TSW_E_ARG, "dsa_cycles must be 1..16");
+            ctx->dsa_cycles = value;
+        } else if (k == "si_graph") {
+            require(value == 0 || value == 1, TSW_E_ARG, "si_graph must be 0 or 1");
+            ctx->si_graph = value != 0;
+        } else if (k == "autotune") {
+            require(value == 0 || value == 1, TSW_E_ARG, "autotune must be 0 or 1");
+            ctx->autotune = value != 0;
         } else if (k == "prefetch") {
             require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
                     "prefetch must be 0, 2, 3, 4 or 6");
@@ -1084,6 +1462,20 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else {
             throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
         }
+    });
+}
+
+int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
+    return guarded(ctx, [&] {
+        require(key != nullptr && value != nullptr, TSW_E_ARG, "null argument");
+        const std::string k(key);
+        if (k == "band_rows") *value = ctx->last_br;
+        else if (k == "prefetch") *value = ctx->last_pf;
+        else if (k == "autotune") *value = ctx->autotune ? 1 : 0;
+        else if (k == "si_graph") *value = ctx->si_graph ? 1 : 0;
+        else if (k == "use_dsa") *value = ctx->use_dsa ? 1 : 0;
+        else if (k == "dsa_cycles") *value = ctx->dsa_cycles;
+        else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }
 
@@ -1153,8 +1545,13 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         if (staggered_ok(ctx)) {
             int it = 0;
             double e = 0.0;
-            if (precision == TSW_FP64) si_staggered<double>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
-            else si_staggered<float>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+            if (ctx->si_graph) {
+                if (precision == TSW_FP64) si_staggered_graph<double>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+                else si_staggered_graph<float>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+            } else {
+                if (precision == TSW_FP64) si_staggered<double>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+                else si_staggered<float>(ctx, Q, st, ss, dt, abs_tol, max_its, it, e);
+            }
             if (mem == TSW_MEM_HOST || Q != source_inout) stage_out(ctx, source_inout, Q, g.npsi(), mem);
             sync(ctx);
             if (iters) *iters = it;

@@ -329,6 +329,48 @@ __global__ __launch_bounds__(kBlock) void k_sqdist_partial(const double *__restr
     acc = block_sum(acc);
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
+// Source-iteration control for replayed graphs (SISolver.cpp:219-241 loop test, on the device): the
+// partial sums of |cur - last|^2 over interior coefficients, then one block that finishes the norm,
+// counts the iteration and sets ctl->done when the reference loop would stop (err <= tol or
+// iters >= max_its).  Both do nothing once ctl->done is set, as does every kernel of the iteration.
+__global__ __launch_bounds__(kBlock) void k_si_dist_partial(const double *__restrict__ a, const double *__restrict__ b,
+                                                            int nx, int ny, int N, double *__restrict__ partials,
+                                                            const int *__restrict__ skip) {
+    if (*skip) return;
+    const size_t n = (size_t)nx * ny * N;
+    double acc = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t / N;
+        const int k = (int)(t - c * N);
+        const size_t cell = (size_t)(1 + c / ny) * (ny + 2) + 1 + c % ny;
+        const double d = a[cell * N + k] - b[cell * N + k];
+        acc += d * d;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_si_control(const double *__restrict__ partials, int nblocks, SiControl *ctl) {
+    if (ctl->done) return;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partials[b];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) {
+        const double err = sqrt(acc);
+        ctl->err = err;
+        ctl->iters += 1;
+        if (err <= ctl->tol || ctl->iters >= ctl->max_its) ctl->done = 1;
+    }
+}
+
+void launch_si_check(const Geom &g, const double *cur, const double *last, double *partials, int nblocks,
+                     SiControl *ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_si_dist_partial, dim3(nblocks), dim3(kBlock), 0, s, cur, last, g.nx, g.ny, g.N(), partials,
+                       &ctl->done);
+    hipLaunchKernelGGL(k_si_control, dim3(1), dim3(kBlock), 0, s, partials, nblocks, ctl);
+    TSW_HIP(hipGetLastError());
+}
+
 void launch_sqdist_interior(const Geom &g, const double *a, const double *b, double *out, double *partials,
                             int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_sqdist_partial, dim3(nblocks), dim3(kBlock), 0, s, a, b, g.nx, g.ny, g.N(), partials);

@@ -170,7 +170,8 @@ template void launch_psi_convert<double, float>(const Geom &, double *, float *,
 
 // per-cell array [i][j][k] (k < K, halo layout) -> per-quadrant diagonal [Q][k][pos]
 template <typename TI, typename TO>
-__global__ void k_cell_to_diag(const TI *cell, TO *out, DiagMap dm, int K) {
+__global__ void k_cell_to_diag(const TI *cell, TO *out, DiagMap dm, int K, const int *skip) {
+    if (skip && *skip) return;
     const int s = blockIdx.x, Q = blockIdx.y;
     const int r = dm.rlo(s) + (int)threadIdx.x;
     const int r1 = s < dm.ny - 1 ? s : dm.ny - 1;
@@ -184,15 +185,16 @@ __global__ void k_cell_to_diag(const TI *cell, TO *out, DiagMap dm, int K) {
 }
 
 template <typename TI, typename TO>
-void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s) {
+void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s, const int *skip) {
     const DiagMap dm{g.nx, g.ny};
     hipLaunchKernelGGL((k_cell_to_diag<TI, TO>), dim3(g.nx + g.ny - 1, 4), dim3(rows_block(g.ny)), 0, s, cell, out, dm,
-                       K);
+                       K, skip);
     TSW_HIP(hipGetLastError());
 }
-template void launch_cell_to_diag<double, double>(const Geom &, const double *, double *, int, hipStream_t);
-template void launch_cell_to_diag<double, float>(const Geom &, const double *, float *, int, hipStream_t);
-template void launch_cell_to_diag<int32_t, int32_t>(const Geom &, const int32_t *, int32_t *, int, hipStream_t);
+template void launch_cell_to_diag<double, double>(const Geom &, const double *, double *, int, hipStream_t, const int *);
+template void launch_cell_to_diag<double, float>(const Geom &, const double *, float *, int, hipStream_t, const int *);
+template void launch_cell_to_diag<int32_t, int32_t>(const Geom &, const int32_t *, int32_t *, int, hipStream_t,
+                                                    const int *);
 
 // Inflow face traces of the halo cells of a reference-layout psi: tin_x[q][r][P] (x-inflow of row r)
 // and tin_y[q][c][P] (y-inflow of column c), as the sweep consumes them.
@@ -244,7 +246,8 @@ template void launch_inflow<float>(const Geom &, const double *, float *, float 
 template <typename TI>
 __global__ __launch_bounds__(256) void k_pmv_quadrant(const TI *__restrict__ psi, const double *__restrict__ w,
                                                       double *__restrict__ phiQ, size_t ncell, int N, int qo1,
-                                                      int qo2, int qo3, int nq) {
+                                                      int qo2, int qo3, int nq, const int *skip) {
+    if (skip && *skip) return;
     const int Q = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncell * N) return;
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(256) void k_pmv_quadrant(const TI *__restrict__ psi
 }
 
 __global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *__restrict__ phi, DiagMap dm, int N,
-                                     double alpha, double beta) {
+                                     double alpha, double beta, const int *skip) {
+    if (skip && *skip) return;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t ncell = (size_t)dm.cells();
     if (t >= ncell * N) return;
@@ -272,17 +276,19 @@ __global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *__
 
 template <typename TI>
 void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha, double beta,
-                     hipStream_t s) {
+                     hipStream_t s, const int *skip) {
     const DiagMap dm{g.nx, g.ny};
     const size_t n = (size_t)dm.cells() * g.N();
     hipLaunchKernelGGL((k_pmv_quadrant<TI>), dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, s, psi, w, phiQ,
-                       (size_t)dm.cells(), g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], g.nq);
+                       (size_t)dm.cells(), g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], g.nq, skip);
     hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, dm, g.N(),
-                       alpha, beta);
+                       alpha, beta, skip);
     TSW_HIP(hipGetLastError());
 }
-template void launch_pmv_diag<double>(const Geom &, const double *, const double *, double *, double *, double, double, hipStream_t);
-template void launch_pmv_diag<float>(const Geom &, const float *, const double *, double *, double *, double, double, hipStream_t);
+template void launch_pmv_diag<double>(const Geom &, const double *, const double *, double *, double *, double, double,
+                                      hipStream_t, const int *);
+template void launch_pmv_diag<float>(const Geom &, const float *, const double *, double *, double *, double, double,
+                                     hipStream_t, const int *);
 
 // =====================================================================================================
 // FP64 angular residual on the diagonal layout, the refinement's r = Q + sigma_s phi/(4 pi) - L psi

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         item_q = it[1];
     }
     const DiagSweepParams<T> sp = batch ? batch[item_p] : sp1;
+    if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
     const int nb = (batch || sp.nbands < 2) ? 1 : sp.nbands;
     const int q = batch ? item_q : (int)(blockIdx.x % (unsigned)sp.nq);
     const int band = batch ? 0 : (int)(blockIdx.x / (unsigned)sp.nq);

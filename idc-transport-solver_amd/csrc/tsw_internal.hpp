@@ -115,7 +115,34 @@ template <typename T> struct DiagSweepParams {
     T *xtr;
     int *abort_flag;
     int prefetch;          // prefetch depth in steps (0: default for the precision)
+    const int *skip;       // device flag: the launch does nothing while *skip != 0 (null: always runs)
 };
+
+// Device-side state of a source iteration replayed from a graph (SISolver.cpp:219-241).
+struct SiControl {
+    int done;     // set when the loop test fails; every kernel of an iteration then does nothing
+    int iters;    // iterations run
+    int max_its;
+    int pad;
+    double tol;   // absolute l2 change of the scalar-flux coefficients
+    double err;   // last change
+};
+void launch_si_check(const Geom &g, const double *cur, const double *last, double *partials, int nblocks,
+                     SiControl *ctl, hipStream_t s);
+
+// One multigrid level of the DSA diffusion operator (dsa.hip): device arrays.
+struct MgLevel {
+    int nx, ny;
+    double *cx, *cy, *sv;  // face conductances ((nx+1)*ny, nx*(ny+1)), absorption x volume (nx*ny)
+    double *x, *b, *r;     // iterate, right-hand side, residual (nx*ny each)
+};
+void launch_dsa_rhs(const Geom &g, const double *x, const double *sigma_s, double *b, hipStream_t s);
+void launch_dsa_add(const Geom &g, const double *d, double *y, hipStream_t s);
+void launch_mg_zero(double *x, int n, hipStream_t s);
+void launch_mg_rbgs(const MgLevel &L, double *x, const double *b, int colour, hipStream_t s);
+void launch_mg_residual(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s);
+void launch_mg_restrict(const MgLevel &f, const MgLevel &c, const double *rf, double *bc, hipStream_t s);
+void launch_mg_prolong_add(const MgLevel &f, const MgLevel &c, const double *xc, double *xf, hipStream_t s);
 
 // Bit pattern of an empty band-exchange record: a signalling NaN, which arithmetic never produces.
 template <typename T> struct ExchangeSentinel;
@@ -143,14 +170,14 @@ template <typename TR, typename TD>
 void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s);
 // per-cell [i][j][K] -> per-quadrant diagonal [4][K][ncell]
 template <typename TI, typename TO>
-void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s);
+void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s, const int *skip = nullptr);
 // inflow face traces of the halo cells of reference-layout psi
 template <typename TO>
 void launch_inflow(const Geom &g, const double *ref, TO *tin_x, TO *tin_y, hipStream_t s);
 // phi[i][j][k] = beta phi + alpha sum_q w_q psi (diagonal psi); phiQ: [4][N][ncell] scratch
 template <typename TI>
 void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha,
-                     double beta, hipStream_t s);
+                     double beta, hipStream_t s, const int *skip = nullptr);
 // res = qsrc + sig_s phi/(4 pi) - (L + 1/dt) psi on the diagonal layout (FP64 inputs)
 template <typename TO>
 void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,

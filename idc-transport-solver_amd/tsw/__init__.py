@@ -75,6 +75,7 @@ def lib():
         L.tsw_problem_set.argtypes = [c, i, i, d, d, d, d, i, i, _dp, _dp, _dp, _ip]
         L.tsw_set_pattern.argtypes = [c, ctypes.c_char_p]
         L.tsw_set_tuning.argtypes = [c, ctypes.c_char_p, i]
+        L.tsw_get_tuning.argtypes = [c, ctypes.c_char_p, _ip]
         L.tsw_sweep_apply.argtypes = [c, _vp, _vp, d, i, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
         L.tsw_smv.argtypes = [c, d, d, _vp, _vp, _vp, i, i]
@@ -206,6 +207,11 @@ class Context:
 
     def set_tuning(self, key, value):
         self._check(self._L.tsw_set_tuning(self._h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int(0)
+        self._check(self._L.tsw_get_tuning(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
 
     def sweep(self, psi, sigma_t, dt=math.inf, precision=FP64):
         self._expect(psi=psi, cell=sigma_t)

@@ -75,9 +75,24 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * a sweep then runs ceil(ny / band_rows) pipelined workgroups per ordinate (the KBA decomposition of
  * SweepPatternUniCartKBA2.cpp:212-453 applied to rows within the device).  0 (default) chooses
  * automatically, -1 keeps one workgroup per ordinate (bands only when ny > 448).  Key "prefetch": steps
- * of cell inputs each row keeps in flight (2, 3, 4 or 6; 0 = 2 in FP64, 3 in FP32).  Results do not
- * depend on either setting.  Unknown keys fail with TSW_E_ARG. */
+ * of cell inputs each row keeps in flight (2, 3, 4 or 6; 0 = 2 in FP64, 3 in FP32).  Key "autotune"
+ * (1 default, 0 off): while band_rows and prefetch are both 0, the first sweep of each kind (mesh,
+ * degree, ordinates, precision, source terms, material path) times every candidate pair into scratch
+ * memory and keeps the fastest for later sweeps; inside a HIP graph capture the defaults apply.  Results
+ * do not depend on any of these settings.
+ * Solver keys: "use_dsa" (0 default, 1): diffusion synthetic acceleration of tsw_gmres_solve and
+ * tsw_solve_vjp, the reference's `use_dsa` parameter (PETScPeierlsSolver.cpp:151, shell preconditioner
+ * :180-182, apply :588-640), applied on the left as PETSc does, so the residual test is on
+ * y = r + C^{-1} sigma_s r; C is the cell-mean finite-volume diffusion operator (D = 1/(3(sigma_t+1/dt)),
+ * vacuum Marshak faces) inverted approximately by "dsa_cycles" (1..16, default 2) multigrid V-cycles.
+ * "si_graph" (1 default, 0): staggered source iteration replays captured graphs of 8 iterations with
+ * the loop test on the device (same iterates and counts as the eager loop).
+ * Unknown keys fail with TSW_E_ARG. */
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
+
+/* Reads back "band_rows" (rows per band of the last staggered sweep, 0 = one band per ordinate),
+ * "prefetch" (its prefetch depth), or the setting of "autotune", "use_dsa", "dsa_cycles", "si_graph". */
+int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
  * cells are overwritten.  dt <= 0 or +inf is steady state.  SweepOperator::Apply

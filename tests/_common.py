@@ -373,3 +373,37 @@ def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# ------------------------------------------------------------------------------------------------
+# IDC restatement (oracle/idc.py) over the oracle's operators
+# ------------------------------------------------------------------------------------------------
+class OracleIdcOps:
+    """Transport operators of one problem for oracle/idc.py: GMRES solves to `tol`, Lmv, S P."""
+
+    def __init__(self, g: Grid, o: Ordinates, sigma_t, sigma_s, tol=1e-13):
+        self.g, self.o, self.st, self.ss, self.tol = g, o, sigma_t, sigma_s, tol
+
+    def solve(self, rhs, dt_sub):
+        rc, psi, _, _, _ = orc_gmres(self.g, self.o, self.st, self.ss, rhs, dt=dt_sub, abs_tol=self.tol,
+                                     restart=40)
+        assert rc == 0
+        return psi
+
+    def lmv(self, x):
+        y = orc_lmv(self.g, self.o, self.st, x)
+        self.zero_halo(y)
+        return y
+
+    def k(self, x):
+        phi = orc_pmv(self.g, self.o, x)
+        y = orc_smv(self.g, self.o, self.ss, phi, np.zeros_like(x), alpha=1.0, beta=0.0)
+        self.zero_halo(y)
+        return y
+
+    @staticmethod
+    def zero_halo(x):
+        x[0] = 0.0
+        x[-1] = 0.0
+        x[:, 0] = 0.0
+        x[:, -1] = 0.0

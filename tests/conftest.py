@@ -4,6 +4,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "idc-transport-solver_amd"))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
 
 
 def pytest_configure(config):

@@ -104,3 +104,34 @@ def test_prefetch_depths_match_oracle(prefetch, band_rows):
             psi = src.copy()
             c.gmres_solve(psi, st, ss, abs_tol=1e-13)
             assert rel_err(psi, ref_g) <= 1e-10
+
+
+def test_autotuner_choice_is_valid_and_result_identical():
+    """With band_rows and prefetch automatic the first sweep times the candidates; the sweep result is
+    bitwise the same as with fixed settings, and the reported choice is one of the candidates."""
+    o = ls_ordinates(6)
+    g, st, ss, src = random_problem(60, 200, 2, o, seed=4242, materials=4)
+    outs = []
+    for autotune in (1, 0):
+        with _ctx(g, o, 0) as c:
+            c.set_tuning("autotune", autotune)
+            for prec in (tsw.FP64, tsw.FP32):
+                for _ in range(2):
+                    out = src.copy()
+                    c.sweep(out, st, precision=prec)
+                    outs.append(out)
+                br, pf = c.get_tuning("band_rows"), c.get_tuning("prefetch")
+                assert br in (0, 64, 128, 192) and pf in (2, 3, 4, 6), (br, pf)
+                if not autotune:
+                    assert br == 64 and pf == (2 if prec == tsw.FP64 else 3)
+            psi = src.copy()
+            c.gmres_solve(psi, st, ss, abs_tol=1e-12)
+            outs.append(psi)
+    n = len(outs) // 2
+    for a, b in zip(outs[:n], outs[n:]):
+        np.testing.assert_array_equal(a, b)
+    ref = orc_sweep(g, o, st, src)
+    assert rel_err(outs[0], ref) <= 1e-12
+    with _ctx(g, o, 0) as c:
+        with pytest.raises(tsw.TswError):
+            c.set_tuning("autotune", 2)
