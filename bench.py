@@ -22,6 +22,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "idc-transport-solver_amd"))
 
@@ -98,6 +100,115 @@ def bench_si(args):
                                  "iterations": ig},
                       "eager": {"value": U * ie / we, "ms_per_step": we * 1e3, "iterations": ie},
                       "speedup_graph_vs_eager": we / wg}), flush=True)
+
+
+def _line_source(n=200, p=2, order=16, width=0.03):
+    """Line-source pulse: [-1.5,1.5]^2, n x n cells, Q_p, level-symmetric S_order, sigma_s = 1, sigma_a = 0,
+    vacuum; isotropic Gaussian initial condition of unit mass, exp(-r^2/(2 s^2)) / (2 pi s^2) / (4 pi) per
+    ordinate, projected exactly per cell (separable 1D Gauss-Legendre quadrature)."""
+    from numpy.polynomial import legendre as L
+    from tsw import level_symmetric
+    xi, eta, w, off = level_symmetric(order)
+    a, b = -1.5, 1.5
+    h = (b - a) / n
+    xg, wg = L.leggauss(12)
+    P = p + 1
+    leg = np.array([L.legval(xg, np.eye(P)[d]) for d in range(P)])  # [d][node]
+    centres = a + (np.arange(n) + 0.5) * h
+    X = centres[:, None] + 0.5 * h * xg[None, :]                      # [cell][node]
+    gx = np.exp(-X ** 2 / (2 * width ** 2)) / math.sqrt(2 * math.pi * width ** 2)
+    # 1D Legendre coefficients of gx on each cell: (2d+1)/2 int P_d g dxhat
+    cx = np.einsum("cn,dn,n->cd", gx, leg, wg) * (2 * np.arange(P) + 1)[None, :] / 2.0
+    coef = np.einsum("id,je->ijde", cx, cx) / (4 * math.pi)
+    nq = xi.size
+    psi0 = np.zeros((n + 2, n + 2, nq, P, P))
+    psi0[1:-1, 1:-1] = coef[:, :, None]
+    st = np.ones((n + 2, n + 2))
+    ss = np.ones((n + 2, n + 2))
+    return dict(n=n, p=p, a=a, b=b, h=h, xi=xi, eta=eta, w=w, off=off, psi0=psi0, st=st, ss=ss)
+
+
+def _ring_seed(L_, r0=0.8, r1=1.0, sub=8):
+    """Seed of J = integral of phi over the ring r0 <= r <= r1: w_q * int_{ring & cell} P_d P_e dx dy, by
+    sub x sub midpoint subcells per cell (the ring's cut cells are resolved to that quadrature)."""
+    from numpy.polynomial import legendre as Lg
+    n, p, a, h = L_["n"], L_["p"], L_["a"], L_["h"]
+    P = p + 1
+    t = (np.arange(sub) + 0.5) / sub * 2 - 1
+    leg = np.array([Lg.legval(t, np.eye(P)[d]) for d in range(P)])
+    centres = a + (np.arange(n) + 0.5) * h
+    X = centres[:, None, None, None] + 0.5 * h * t[None, None, :, None]
+    Y = centres[None, :, None, None] + 0.5 * h * t[None, None, None, :]
+    R = np.sqrt(X ** 2 + Y ** 2)
+    ind = ((R >= r0) & (R <= r1)).astype(float) * (h / sub) ** 2
+    M = np.einsum("ijuv,du,ev->ijde", ind, leg, leg)
+    nq = L_["xi"].size
+    seed = np.zeros((n + 2, n + 2, nq, P, P))
+    seed[1:-1, 1:-1] = L_["w"][None, None, :, None, None] * M[:, :, None]
+    return seed
+
+
+def bench_idc(args):
+    """IMEX/IDC line-source pulse: 200x200, Q2, S16, 4 Gauss-Lobatto nodes, 3 corrections, dt = 0.01, FP32
+    sweeps with FP64 correction sums; `--idc-steps` of the 100 steps to t = 1 are timed (inputs resident),
+    then the adjoint of the ring response (0.8 <= r <= 1.0) w.r.t. the initial condition over the same
+    steps, with sigma_s gradients and a checkpoint every 10 steps."""
+    import torch
+    import tsw
+    dev = torch.device("cuda:0")
+    L_ = _line_source()
+    n, p = L_["n"], L_["p"]
+    ctx = tsw.Context(0)
+    ctx.problem_set(n, n, p, L_["xi"], L_["eta"], L_["w"], L_["off"], L_["a"], L_["b"], L_["a"], L_["b"])
+    ctx.idc_set(4, 3, tsw.IDC_LOBATTO, False)
+    psi = torch.from_numpy(L_["psi0"]).to(dev)
+    psi0 = psi.clone()
+    src = torch.zeros_like(psi)
+    st = torch.from_numpy(L_["st"]).to(dev)
+    ss = torch.from_numpy(L_["ss"]).to(dev)
+    dt, steps = 0.01, args.idc_steps
+    ctx.idc_step(psi, src, st, ss, dt, abs_tol=1e-10, precision=tsw.FP32)  # warm-up (tables, tuning)
+    psi.copy_(psi0)
+    ctx.sweep_timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.idc_run(psi, src, st, ss, dt, steps, abs_tol=1e-10, precision=tsw.FP32)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nsw, swms = ctx.sweep_timing(reset=True)
+    U = L_["xi"].size * n * n * (p + 1) ** 2
+    phi_mass = float(psi[1:-1, 1:-1, :, 0, 0].sum(dim=2).sum()) * 4 * math.pi * L_["h"] ** 2 / (4 * math.pi)
+    print(f"bench: idc {steps} steps {wall:.3f}s, {nsw} sweeps ({swms:.1f} ms in sweeps)", file=sys.stderr, flush=True)
+    seed = torch.from_numpy(_ring_seed(L_)).to(dev)
+    J = float((seed * psi).sum())
+    grad = torch.zeros_like(psi)
+    gss = torch.zeros_like(ss)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.idc_adjoint(seed, grad, src, st, ss, dt, steps, psi0=psi0, checkpoint_every=10, grad_sigma_s=gss,
+                    abs_tol=1e-10, precision=tsw.FP32)
+    torch.cuda.synchronize()
+    wadj = time.perf_counter() - t1
+    # first-order check of the initial-condition gradient: J is affine in psi0
+    pert = torch.zeros_like(psi0)
+    pert[1:-1, 1:-1] = 1e-3 * torch.rand(pert[1:-1, 1:-1].shape, device=dev, dtype=torch.float64)
+    x = psi0 + pert
+    ctx.idc_run(x, src, st, ss, dt, steps, abs_tol=1e-10, precision=tsw.FP32)
+    dJ = float((seed * x).sum()) - J
+    pred = float((grad * pert).sum())
+    print(f"bench: idc adjoint {wadj:.3f}s; J = {J:.6e}; dJ {dJ:.6e} vs grad.dpsi0 {pred:.6e}", file=sys.stderr,
+          flush=True)
+    ctx.close()
+    print(json.dumps({
+        "metric": "angular unknowns swept/s (IDC line source)", "value": U * nsw / wall,
+        "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": wall / max(steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 sweep / f64 correction sums", "data": "synthetic (Gaussian pulse)",
+        "config": {"workload": "line-source pulse 200x200, Q2, S16 (144), sigma_s=1, sigma_a=0, IDC 4 Lobatto "
+                               "nodes / 3 corrections, dt=0.01", "timed_steps": steps, "sweeps": nsw,
+                   "mass_phi_t": phi_mass},
+        "adjoint": {"seconds": wadj, "checkpoint_every": 10, "J_ring": J, "dJ_fd": dJ, "dJ_grad": pred,
+                    "rel_diff": abs(dJ - pred) / max(abs(dJ), 1e-300)}}), flush=True)
 
 
 def bench_batched(args):
@@ -180,7 +291,8 @@ def main():
     ap.add_argument("--roofline-sweeps", type=int, default=20, help="operator-contract sweeps timed per precision")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
-    ap.add_argument("--workload", default="lattice", choices=["lattice", "batched", "si"])
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "batched", "si", "idc"])
+    ap.add_argument("--idc-steps", type=int, default=10, help="IDC steps timed by --workload idc (100 reach t=1)")
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
@@ -197,6 +309,9 @@ def main():
         return
     if args.workload == "si":
         bench_si(args)
+        return
+    if args.workload == "idc":
+        bench_idc(args)
         return
 
     import torch

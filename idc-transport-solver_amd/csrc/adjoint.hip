@@ -49,13 +49,14 @@ void launch_reverse_ordinates(const Geom &g, const double *in, double *out, int 
     TSW_HIP(hipGetLastError());
 }
 
-// Per interior cell c (one wavefront per cell):
-//   gst[c] = -sum_{q,k} zeta psi                       (d/dsigma_t of <seed, psi>)
-//   gss[c] =  sum_{q,k} zeta_qk phi_k / (4 pi)          (d/dsigma_s, when phi is given)
+// Per interior cell c (one wavefront per cell), written or (accumulate) added:
+//   gst[c] (+)= -scale sum_{q,k} zeta psi               (d/dsigma_t of <seed, psi>)
+//   gss[c] (+)=  scale sum_{q,k} zeta_qk phi_k / (4 pi)  (d/dsigma_s, when phi is given)
 // psi, zeta: reference layout; phi: phi layout; gst/gss: cell arrays (halo entries not written).
 __global__ __launch_bounds__(256) void k_cell_grads(const double *__restrict__ psi, const double *__restrict__ zeta,
                                                     const double *__restrict__ phi, double *__restrict__ gst,
-                                                    double *__restrict__ gss, int nx, int ny, int nq, int N) {
+                                                    double *__restrict__ gss, int nx, int ny, int nq, int N,
+                                                    double scale, int accumulate) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long cell = (long long)blockIdx.x * 4 + wave;
     if (cell >= (long long)nx * ny) return;  // whole wavefront leaves; no block barrier follows
@@ -74,16 +75,41 @@ __global__ __launch_bounds__(256) void k_cell_grads(const double *__restrict__ p
         ss += __shfl_down(ss, off);
     }
     if (lane == 0) {
-        if (gst) gst[c] = -st;
-        if (gss) gss[c] = ss * (1.0 / (4.0 * 3.14159265358979323846));
+        if (gst) gst[c] = (accumulate ? gst[c] : 0.0) - scale * st;
+        if (gss) gss[c] = (accumulate ? gss[c] : 0.0) + scale * ss * (1.0 / (4.0 * 3.14159265358979323846));
     }
 }
 
 void launch_cell_grads(const Geom &g, const double *psi, const double *zeta, const double *phi, double *gst,
-                       double *gss, hipStream_t s) {
+                       double *gss, hipStream_t s, double scale, bool accumulate) {
     const long long cells = (long long)g.nx * g.ny;
     const int blocks = (int)((cells + 3) / 4);
-    hipLaunchKernelGGL(k_cell_grads, dim3(blocks), dim3(256), 0, s, psi, zeta, phi, gst, gss, g.nx, g.ny, g.nq, g.N());
+    hipLaunchKernelGGL(k_cell_grads, dim3(blocks), dim3(256), 0, s, psi, zeta, phi, gst, gss, g.nx, g.ny, g.nq, g.N(),
+                       scale, (int)accumulate);
+    TSW_HIP(hipGetLastError());
+}
+
+// Transpose of the scattering source S P on the interior: out_q += alpha w_q sigma_s sum_q' y_q' / (4 pi).
+// One thread per (interior cell, coefficient).
+__global__ void k_kt_apply(const double *__restrict__ y, double *__restrict__ out, const double *__restrict__ ss,
+                           const double *__restrict__ w, double alpha, int nx, int ny, int nq, int N) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nx * ny * N) return;
+    const size_t c = t / N;
+    const int k = (int)(t - c * N);
+    const size_t cell = (size_t)(1 + c / ny) * (ny + 2) + 1 + c % ny;
+    const size_t base = cell * nq * N + k;
+    double sum = 0.0;
+    for (int q = 0; q < nq; ++q) sum += y[base + (size_t)q * N];
+    const double f = alpha * ss[cell] * sum * (1.0 / (4.0 * 3.14159265358979323846));
+    for (int q = 0; q < nq; ++q) out[base + (size_t)q * N] += f * w[q];
+}
+
+void launch_kt_apply(const Geom &g, const double *y, double *out, const double *ss, const double *w, double alpha,
+                     hipStream_t s) {
+    const size_t n = (size_t)g.nx * g.ny * g.N();
+    hipLaunchKernelGGL(k_kt_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, out, ss, w, alpha, g.nx, g.ny,
+                       g.nq, g.N());
     TSW_HIP(hipGetLastError());
 }
 

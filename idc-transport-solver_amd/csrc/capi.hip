@@ -112,6 +112,18 @@ struct tsw_ctx {
     // y = x + C^{-1} sigma_s x captured as a graph over staging vectors.
     bool use_dsa = false;
     int dsa_cycles = 2;
+    // IDC time integrator (tsw_idc_set / tsw_idc_step): nodes on [0,1] (stage 0 = left end), the
+    // integration matrix W (row 0 over [0,1], row n over [t_{n-1}, t_n]) and the stage, residual and
+    // correction vectors in the reference layout, FP64.
+    struct Idc {
+        int num_nodes = 4, num_corr = 3, kind = TSW_IDC_LOBATTO;
+        bool collocation = false, left = true;
+        std::vector<double> nodes, W;  // S and S*S
+        DBuf stages, res, err, phi;
+        // adjoint: saved forward intermediates, stage adjoints, residual adjoints, work vectors,
+        // checkpoints and the states of one checkpoint segment
+        DBuf saved, lam, lamR, t1, t2, t3, ckpt, seg;
+    } idc;
     struct Dsa {
         std::vector<MgLevel> lv;
         std::vector<DBuf> store;
@@ -1249,6 +1261,189 @@ void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int p
     }
 }
 
+// ---------------------------------------------------------------------------------------------------
+// IDC time step (IDCIntegrator::Step_Nonhybrid, idc.cpp:846-940) on device arrays.
+// ---------------------------------------------------------------------------------------------------
+// Legendre polynomial P_n and its derivative at x
+void legendre_pd(int n, double x, double &p, double &dp) {
+    double p0 = 1.0, p1 = x;
+    if (n == 0) {
+        p = 1.0;
+        dp = 0.0;
+        return;
+    }
+    for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    p = p1;
+    dp = std::fabs(1.0 - x * x) < 1e-300 ? 0.5 * n * (n + 1) * (x > 0 ? 1.0 : (n % 2 ? 1.0 : -1.0))
+                                         : n * (p0 - x * p1) / (1.0 - x * x);
+}
+
+// Gauss-Legendre nodes (ascending) and weights on [-1, 1], Newton from Chebyshev guesses
+void gauss_legendre(int m, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(m, 0.0);
+    w.assign(m, 0.0);
+    for (int k = 0; k < m; ++k) {
+        double t = -std::cos(M_PI * (k + 0.75) / (m + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre_pd(m, t, p, dp);
+            const double dt = p / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-16) break;
+        }
+        double p, dp;
+        legendre_pd(m, t, p, dp);
+        x[k] = t;
+        w[k] = 2.0 / ((1.0 - t * t) * dp * dp);
+    }
+}
+
+// Gauss-Lobatto nodes on [-1, 1]: the endpoints and the roots of P'_{m-1} (Newton with P'' from the
+// Legendre equation)
+std::vector<double> gauss_lobatto(int m) {
+    std::vector<double> x(m);
+    x[0] = -1.0;
+    x[m - 1] = 1.0;
+    const int n = m - 1;
+    for (int k = 1; k < m - 1; ++k) {
+        double t = -std::cos(M_PI * k / n);
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre_pd(n, t, p, dp);
+            const double d2p = (2.0 * t * dp - n * (n + 1.0) * p) / (1.0 - t * t);
+            const double dt = dp / d2p;
+            t -= dt;
+            if (std::fabs(dt) < 1e-16) break;
+        }
+        x[k] = t;
+    }
+    return x;
+}
+
+// integral over [a, b] of the Lagrange basis polynomials of `nodes`
+std::vector<double> lagrange_integrals(const std::vector<double> &nodes, double a, double b) {
+    const int m = (int)nodes.size();
+    std::vector<double> xg, wg, out(m, 0.0);
+    gauss_legendre(m + 2, xg, wg);
+    for (size_t k = 0; k < xg.size(); ++k) {
+        const double t = 0.5 * (b - a) * (xg[k] + 1.0) + a;
+        for (int l = 0; l < m; ++l) {
+            double v = 1.0;
+            for (int j = 0; j < m; ++j)
+                if (j != l) v *= (t - nodes[j]) / (nodes[l] - nodes[j]);
+            out[l] += 0.5 * (b - a) * wg[k] * v;
+        }
+    }
+    return out;
+}
+
+// Nodes and integration matrix (SetNodes, idc.cpp:2305-2376)
+void idc_tables(tsw_ctx::Idc &I) {
+    std::vector<double> nodes;
+    if (I.kind == TSW_IDC_LOBATTO) {
+        nodes = gauss_lobatto(I.num_nodes);
+        I.left = true;
+    } else {
+        std::vector<double> w;
+        gauss_legendre(I.num_nodes, nodes, w);
+        I.left = false;
+    }
+    for (double &t : nodes) t = 0.5 * (t + 1.0);
+    I.nodes.clear();
+    if (!I.left) I.nodes.push_back(0.0);
+    I.nodes.insert(I.nodes.end(), nodes.begin(), nodes.end());
+    const int S = (int)I.nodes.size(), off = I.left ? 0 : 1;
+    I.W.assign((size_t)S * S, 0.0);
+    std::vector<double> row = lagrange_integrals(nodes, 0.0, 1.0);
+    for (size_t l = 0; l < nodes.size(); ++l) I.W[off + l] = row[l];
+    for (int n = 1; n < S; ++n) {
+        row = lagrange_integrals(nodes, I.nodes[n - 1], I.nodes[n]);
+        for (size_t l = 0; l < nodes.size(); ++l) I.W[(size_t)n * S + off + l] = row[l];
+    }
+}
+
+// R = L x - S P x on the interior (ComputeResiduals, idc.cpp:1658-1667); R's halos stay zero
+void idc_residual(tsw_ctx *c, const double *x, double *R, const double *st, const double *ss, double *phi) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    launch_lmv<double>(g, c->xi64.as<double>(), c->eta64.as<double>(), st, x, R, 1.0, 0.0, s);
+    launch_pmv<double, double>(g, x, phi, c->w64.as<double>(), 1.0, 0.0, 0, s);
+    launch_smv<double>(g, ss, phi, R, -1.0, 1.0, 0, s);
+}
+
+// implicit transport solve (1/dt_sub + L - S P) psi = rhs in place (rhs halos = inflow)
+bool idc_solve(tsw_ctx *c, double *rhs, const double *st, const double *ss, double dt_sub, double tol, int max_its,
+               int precision, int &its_total) {
+    int its = 0;
+    double res = 0.0;
+    const bool ok = gmres_device(c, rhs, st, ss, dt_sub, tol, max_its, 30, precision, nullptr, its, res);
+    its_total += its;
+    return ok;
+}
+
+// One IDC step in place on psi (device, zero halos on entry and exit); src: Q with inflow halos.
+bool idc_step_device(tsw_ctx *c, double *psi, const double *src, const double *st, const double *ss, double dt,
+                     double tol, int max_its, int precision, int &its_total) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    auto &I = c->idc;
+    const int S = (int)I.nodes.size(), l0 = I.left ? 0 : 1;
+    const size_t n = g.npsi(), K = (size_t)g.nq * g.N();
+    double *stg = dbuf<double>(I.stages, (size_t)S * n);
+    double *R = dbuf<double>(I.res, (size_t)S * n);
+    double *e = dbuf<double>(I.err, n);
+    double *phi = dbuf<double>(I.phi, g.nphi());
+    TSW_HIP(hipMemsetAsync(R, 0, sizeof(double) * S * n, s));  // residual halos stay zero
+    TSW_HIP(hipMemsetAsync(phi, 0, sizeof(double) * g.nphi(), s));
+    auto stage = [&](int k) { return stg + (size_t)k * n; };
+    bool ok = true;
+    its_total = 0;
+    // stage 0: the initial condition, ghost cells zero (idc.cpp:833-835, 850)
+    TSW_HIP(hipMemcpyAsync(stage(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    launch_fill_region<double>(g, stage(0), K, 1, 0.0, s);
+    // prediction: implicit Euler substeps (:863-876)
+    for (int k = 1; k < S; ++k) {
+        const double h = I.nodes[k] - I.nodes[k - 1];
+        launch_axpby(n, 1.0 / (h * dt), stage(k - 1), 0.0, stage(k), s);
+        launch_fill_region<double>(g, stage(k), K, 1, 0.0, s);
+        launch_axpby(n, 1.0, src, 1.0, stage(k), s);
+        ok = idc_solve(c, stage(k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+    }
+    // corrections in error form (:882-930, ComputeSource :1686-1740)
+    for (int p = 0; p < I.num_corr; ++p) {
+        for (int l = l0; l < S; ++l) idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
+        for (int k = 1; k < S; ++k) {
+            const double h = I.nodes[k] - I.nodes[k - 1];
+            launch_axpby(n, -1.0 / (h * dt), stage(k), 0.0, e, s);
+            for (int l = l0; l < S; ++l) launch_axpby(n, -I.W[(size_t)k * S + l] / h, R + (size_t)l * n, 1.0, e, s);
+            launch_axpby(n, 1.0 / (h * dt), stage(k - 1), 1.0, e, s);
+            launch_axpby(n, 1.0, src, 1.0, e, s);
+            launch_fill_region<double>(g, e, K, 1, 0.0, s);
+            ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+            launch_axpby(n, 1.0, e, 1.0, stage(k), s);
+        }
+    }
+    // result: the last node, or the collocation formula (:933-939, ComputeCollocation_Nonhybrid :1541-1566)
+    if (!I.left || I.collocation) {
+        // e = psi0 + dt Q + dt sum_l W(0,l) (S P - L) psi_l
+        TSW_HIP(hipMemcpyAsync(e, stage(0), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        launch_axpby(n, dt, src, 1.0, e, s);
+        for (int l = l0; l < S; ++l) {
+            idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
+            launch_axpby(n, -dt * I.W[l], R + (size_t)l * n, 1.0, e, s);
+        }
+        TSW_HIP(hipMemcpyAsync(psi, e, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+        TSW_HIP(hipMemcpyAsync(psi, stage(S - 1), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    launch_fill_region<double>(g, psi, K, 1, 0.0, s);
+    return ok;
+}
+
 // Validates and installs a mesh + ordinate set (tsw_problem_set, also used per batch member).
 void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, double by, int p, int nq,
                  const double *xi, const double *eta, const double *w, const int *quad_off) {
@@ -1384,6 +1579,153 @@ bool solve_vjp_device(tsw_ctx *ctx, const double *psi, const double *st, const d
         launch_pmv<double, double>(g, psi, phi, w, 1.0, 0.0, 0, s);
     }
     vjp_finish(ctx, psi, phi, gsrc, gst, gss);
+    return ok;
+}
+
+
+// zeta (interior) = A^{-T} seed (interior), A = 1/dt + L - S P, by the adjoint problem's Peierls GMRES
+// (the transposed solve of solve_vjp_device); zeta's halos are not written.
+bool adjoint_solve(tsw_ctx *c, const double *seed, double *zeta, const double *st, const double *ss, double dt,
+                   double tol, int max_its, int precision, int &its_total) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    tsw_ctx *a = adjoint_ctx(c);
+    const double *w = c->w64.as<double>();
+    double *y = dbuf<double>(c->vjp_y, g.npsi());
+    TSW_HIP(hipMemsetAsync(y, 0, sizeof(double) * g.npsi(), s));
+    launch_reverse_ordinates(g, seed, y, 1, w, 2, false, s);
+    int its = 0;
+    double res = 0.0;
+    const bool ok = gmres_device(a, y, st, ss, dt, tol, max_its, 30, precision, nullptr, its, res);
+    its_total += its;
+    launch_reverse_ordinates(g, y, zeta, 2, w, 1, true, s);
+    return ok;
+}
+
+// out (interior) += alpha B^T y, B = L - S P (y's halos ignored): L^T = W^{-1} L(-Omega) W with vacuum
+// inflow, evaluated with the adjoint problem's ordinates; (S P)^T by launch_kt_apply.
+void bt_apply(tsw_ctx *c, const double *y, double *out, const double *st, const double *ss, double alpha) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    tsw_ctx *a = adjoint_ctx(c);
+    auto &I = c->idc;
+    const size_t n = g.npsi();
+    double *t1 = dbuf<double>(I.t1, n), *t2 = dbuf<double>(I.t2, n), *t3 = dbuf<double>(I.t3, n);
+    TSW_HIP(hipMemsetAsync(t1, 0, sizeof(double) * n, s));
+    TSW_HIP(hipMemsetAsync(t3, 0, sizeof(double) * n, s));
+    launch_reverse_ordinates(g, y, t1, 1, nullptr, 0, false, s);
+    launch_lmv<double>(g, a->xi64.as<double>(), a->eta64.as<double>(), st, t1, t2, 1.0, 0.0, s);
+    launch_reverse_ordinates(g, t2, t3, 2, nullptr, 0, false, s);
+    launch_axpby(n, alpha, t3, 1.0, out, s);
+    launch_kt_apply(g, y, out, ss, c->w64.as<double>(), -alpha, s);
+}
+
+// scalar flux of x (interior, zero halos) into c->idc.phi
+const double *idc_phi(tsw_ctx *c, const double *x) {
+    const Geom &g = c->g;
+    double *phi = dbuf<double>(c->idc.phi, g.nphi());
+    TSW_HIP(hipMemsetAsync(phi, 0, sizeof(double) * g.nphi(), c->stream));
+    launch_pmv<double, double>(g, x, phi, c->w64.as<double>(), 1.0, 0.0, 0, c->stream);
+    return phi;
+}
+
+// Vector-Jacobian product of one IDC step (linear in psi^n for fixed cross sections): lam holds
+// d J / d psi^{n+1} (interior) on entry and d J / d psi^n (interior, zero halos) on exit.  With gst / gss
+// (cell arrays, accumulated; may be null) the step is first re-run from psi_n keeping its intermediates,
+// and every implicit solve y = A^{-1} r adds <z, dA/dsigma y> terms (z its adjoint) and every residual
+// R = B x adds <lam_R, dB/dsigma x> terms.  The reverse pass mirrors idc_step_device operation by operation.
+bool idc_step_vjp_device(tsw_ctx *c, const double *psi_n, const double *src, const double *st, const double *ss,
+                         double dt, double tol, int max_its, int precision, double *lam, double *gst, double *gss,
+                         int &its_total) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    auto &I = c->idc;
+    const int S = (int)I.nodes.size(), P = I.num_corr, l0 = I.left ? 0 : 1;
+    const size_t n = g.npsi(), K = (size_t)g.nq * g.N();
+    const bool grads = gst || gss;
+    const bool colloc = !I.left || I.collocation;
+    bool ok = true;
+    its_total = 0;
+    // saved forward values: levels 0..P of all stages, then the corrections e[p][k], p = 1..P
+    const size_t nlev = (size_t)(P + 1) * S, nsave = grads ? nlev + (size_t)P * S : 0;
+    double *saved = grads ? dbuf<double>(I.saved, nsave * n) : nullptr;
+    auto lev = [&](int p, int k) { return saved + ((size_t)p * S + k) * n; };
+    auto corr = [&](int p, int k) { return saved + (nlev + (size_t)(p - 1) * S + k) * n; };
+    if (grads) {
+        // forward re-run (idc_step_device with every intermediate kept)
+        double *R = dbuf<double>(I.res, (size_t)S * n);
+        double *phi = dbuf<double>(I.phi, g.nphi());
+        TSW_HIP(hipMemsetAsync(R, 0, sizeof(double) * S * n, s));
+        TSW_HIP(hipMemsetAsync(phi, 0, sizeof(double) * g.nphi(), s));
+        TSW_HIP(hipMemcpyAsync(lev(0, 0), psi_n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        launch_fill_region<double>(g, lev(0, 0), K, 1, 0.0, s);
+        for (int k = 1; k < S; ++k) {
+            const double h = I.nodes[k] - I.nodes[k - 1];
+            launch_axpby(n, 1.0 / (h * dt), lev(0, k - 1), 0.0, lev(0, k), s);
+            launch_fill_region<double>(g, lev(0, k), K, 1, 0.0, s);
+            launch_axpby(n, 1.0, src, 1.0, lev(0, k), s);
+            ok = idc_solve(c, lev(0, k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+        }
+        for (int p = 1; p <= P; ++p) {
+            for (int l = l0; l < S; ++l) idc_residual(c, lev(p - 1, l), R + (size_t)l * n, st, ss, phi);
+            TSW_HIP(hipMemcpyAsync(lev(p, 0), lev(p - 1, 0), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            for (int k = 1; k < S; ++k) {
+                const double h = I.nodes[k] - I.nodes[k - 1];
+                double *e = corr(p, k);
+                launch_axpby(n, -1.0 / (h * dt), lev(p - 1, k), 0.0, e, s);
+                for (int l = l0; l < S; ++l) launch_axpby(n, -I.W[(size_t)k * S + l] / h, R + (size_t)l * n, 1.0, e, s);
+                launch_axpby(n, 1.0 / (h * dt), lev(p, k - 1), 1.0, e, s);
+                launch_axpby(n, 1.0, src, 1.0, e, s);
+                launch_fill_region<double>(g, e, K, 1, 0.0, s);
+                ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+                TSW_HIP(hipMemcpyAsync(lev(p, k), lev(p - 1, k), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                launch_axpby(n, 1.0, e, 1.0, lev(p, k), s);
+            }
+        }
+    }
+    // reverse pass
+    double *ls = dbuf<double>(I.lam, (size_t)S * n);     // stage adjoints (interior; halos zero)
+    double *lR = dbuf<double>(I.lamR, (size_t)S * n);    // residual adjoints
+    double *z = dbuf<double>(I.err, n);
+    TSW_HIP(hipMemsetAsync(ls, 0, sizeof(double) * S * n, s));
+    TSW_HIP(hipMemsetAsync(z, 0, sizeof(double) * n, s));
+    auto lam_s = [&](int k) { return ls + (size_t)k * n; };
+    launch_fill_region<double>(g, lam, K, 1, 0.0, s);
+    if (colloc) {
+        // out = psi0 + dt Q + dt sum_l W(0,l) (S P - L) psi_l^P
+        launch_axpby(n, 1.0, lam, 1.0, lam_s(0), s);
+        for (int l = l0; l < S; ++l) {
+            bt_apply(c, lam, lam_s(l), st, ss, -dt * I.W[l]);
+            if (grads) launch_cell_grads(g, lev(P, l), lam, idc_phi(c, lev(P, l)), gst, gss, s, dt * I.W[l], true);
+        }
+    } else {
+        TSW_HIP(hipMemcpyAsync(lam_s(S - 1), lam, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    for (int p = P; p >= 1; --p) {
+        TSW_HIP(hipMemsetAsync(lR, 0, sizeof(double) * S * n, s));
+        for (int k = S - 1; k >= 1; --k) {
+            const double h = I.nodes[k] - I.nodes[k - 1];
+            ok = adjoint_solve(c, lam_s(k), z, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+            if (grads) launch_cell_grads(g, corr(p, k), z, idc_phi(c, corr(p, k)), gst, gss, s, 1.0, true);
+            launch_axpby(n, 1.0 / (h * dt), z, 1.0, lam_s(k - 1), s);
+            launch_axpby(n, -1.0 / (h * dt), z, 1.0, lam_s(k), s);
+            for (int l = l0; l < S; ++l) launch_axpby(n, -I.W[(size_t)k * S + l] / h, z, 1.0, lR + (size_t)l * n, s);
+        }
+        for (int l = l0; l < S; ++l) {
+            bt_apply(c, lR + (size_t)l * n, lam_s(l), st, ss, 1.0);
+            if (grads)
+                launch_cell_grads(g, lev(p - 1, l), lR + (size_t)l * n, idc_phi(c, lev(p - 1, l)), gst, gss, s, -1.0,
+                                  true);
+        }
+    }
+    for (int k = S - 1; k >= 1; --k) {
+        const double h = I.nodes[k] - I.nodes[k - 1];
+        ok = adjoint_solve(c, lam_s(k), z, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+        if (grads) launch_cell_grads(g, lev(0, k), z, idc_phi(c, lev(0, k)), gst, gss, s, 1.0, true);
+        launch_axpby(n, 1.0 / (h * dt), z, 1.0, lam_s(k - 1), s);
+    }
+    TSW_HIP(hipMemcpyAsync(lam, lam_s(0), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    launch_fill_region<double>(g, lam, K, 1, 0.0, s);
     return ok;
 }
 
@@ -1675,6 +2017,152 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
     if (resid) *resid = res;
     if (rc == TSW_OK && !converged) {
         ctx->err = "adjoint GMRES transport solve failed to converge";
+        return TSW_E_NOCONV;
+    }
+    return rc;
+}
+
+int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type, int use_collocation) {
+    return guarded(ctx, [&] {
+        require(nodes_type == TSW_IDC_LOBATTO || nodes_type == TSW_IDC_LEGENDRE, TSW_E_ARG, "unknown IDC node type");
+        require(num_nodes >= (nodes_type == TSW_IDC_LOBATTO ? 2 : 1) && num_nodes <= 12, TSW_E_ARG,
+                "IDC node count out of range");
+        require(num_corrections >= 0 && num_corrections <= 32, TSW_E_ARG, "IDC correction count out of range");
+        auto &I = ctx->idc;
+        I.num_nodes = num_nodes;
+        I.num_corr = num_corrections;
+        I.kind = nodes_type;
+        I.collocation = use_collocation != 0;
+        idc_tables(I);
+    });
+}
+
+int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const double *sigma_t,
+                 const double *sigma_s, double dt, double abs_tol, int max_its, int precision, int mem) {
+    bool ok = true;
+    const int rc = guarded(ctx, [&] {
+        need_problem(ctx);
+        require(dt > 0.0 && std::isfinite(dt), TSW_E_ARG, "IDC needs a finite time step");
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
+        const Geom &g = ctx->g;
+        double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->q64);
+        const double *src = stage_in(ctx, source, g.npsi(), mem, ctx->vjp_seed);
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        int its = 0;
+        ok = idc_step_device(ctx, psi, src, st, ss, dt, abs_tol, max_its, precision, its);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+    if (rc == TSW_OK && !ok) {
+        ctx->err = "an implicit solve of the IDC step failed to converge";
+        return TSW_E_NOCONV;
+    }
+    return rc;
+}
+
+int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const double *sigma_t, const double *sigma_s,
+                double dt, int nsteps, double abs_tol, int max_its, int precision, int mem) {
+    bool ok = true;
+    const int rc = guarded(ctx, [&] {
+        need_problem(ctx);
+        require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
+        const Geom &g = ctx->g;
+        double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->q64);
+        const double *src = stage_in(ctx, source, g.npsi(), mem, ctx->vjp_seed);
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        for (int k = 0; k < nsteps && ok; ++k) {
+            int its = 0;
+            ok = idc_step_device(ctx, psi, src, st, ss, dt, abs_tol, max_its, precision, its);
+        }
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+    if (rc == TSW_OK && !ok) {
+        ctx->err = "an implicit solve of an IDC step failed to converge";
+        return TSW_E_NOCONV;
+    }
+    return rc;
+}
+
+int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, const double *sigma_t,
+                    const double *sigma_s, double dt, int nsteps, int checkpoint_every, const double *seed,
+                    double *grad_psi0, double *grad_sigma_t, double *grad_sigma_s, double abs_tol, int max_its,
+                    int precision, int mem) {
+    bool ok = true;
+    const int rc = guarded(ctx, [&] {
+        need_problem(ctx);
+        require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
+        require(checkpoint_every >= 1, TSW_E_ARG, "checkpoint_every must be >= 1");
+        require(seed && grad_psi0, TSW_E_ARG, "seed and grad_psi0 are required");
+        const bool grads = grad_sigma_t || grad_sigma_s;
+        require(!grads || psi0, TSW_E_ARG, "cross-section gradients need psi0");
+        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
+        const Geom &g = ctx->g;
+        hipStream_t s = ctx->stream;
+        const size_t n = g.npsi();
+        const double *src = stage_in(ctx, source, n, mem, ctx->vjp_seed);
+        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *p0 = psi0 ? stage_in(ctx, psi0, n, mem, ctx->vjp_psi) : nullptr;
+        double *lam = mem == TSW_MEM_DEVICE ? grad_psi0 : dbuf<double>(ctx->vjp_gsrc, n);
+        if (lam != seed) {
+            const double *sd = stage_in(ctx, seed, n, mem, ctx->q64);
+            TSW_HIP(hipMemcpyAsync(lam, sd, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        }
+        auto cellbuf = [&](double *user, DBuf &b) {
+            return user ? (mem == TSW_MEM_DEVICE ? user : dbuf<double>(b, g.ncell_h())) : nullptr;
+        };
+        double *gst = cellbuf(grad_sigma_t, ctx->vjp_gst), *gss = cellbuf(grad_sigma_s, ctx->vjp_gss);
+        if (gst) TSW_HIP(hipMemsetAsync(gst, 0, sizeof(double) * g.ncell_h(), s));
+        if (gss) TSW_HIP(hipMemsetAsync(gss, 0, sizeof(double) * g.ncell_h(), s));
+        auto &I = ctx->idc;
+        int its = 0;
+        if (grads) {
+            // forward pass keeping a checkpoint every `checkpoint_every` steps
+            const int C = checkpoint_every, nck = (nsteps + C - 1) / C;
+            double *ck = dbuf<double>(I.ckpt, (size_t)std::max(nck, 1) * n);
+            double *seg = dbuf<double>(I.seg, (size_t)C * n);
+            double *cur = dbuf<double>(ctx->vjp_y, n);  // idc steps use their own buffers, not vjp_y
+            TSW_HIP(hipMemcpyAsync(cur, p0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            for (int k = 0; k < nsteps && ok; ++k) {
+                if (k % C == 0) TSW_HIP(hipMemcpyAsync(ck + (size_t)(k / C) * n, cur, sizeof(double) * n,
+                                                       hipMemcpyDeviceToDevice, s));
+                ok = idc_step_device(ctx, cur, src, st, ss, dt, abs_tol, max_its, precision, its);
+            }
+            // backward pass: per segment, recompute its states from the checkpoint, then step back
+            for (int sg = nck - 1; sg >= 0 && ok; --sg) {
+                const int k0 = sg * C, k1 = std::min(k0 + C, nsteps);
+                TSW_HIP(hipMemcpyAsync(seg, ck + (size_t)sg * n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                for (int k = k0 + 1; k < k1 && ok; ++k) {
+                    double *dst = seg + (size_t)(k - k0) * n;
+                    TSW_HIP(hipMemcpyAsync(dst, dst - n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                    ok = idc_step_device(ctx, dst, src, st, ss, dt, abs_tol, max_its, precision, its);
+                }
+                for (int k = k1 - 1; k >= k0 && ok; --k)
+                    ok = idc_step_vjp_device(ctx, seg + (size_t)(k - k0) * n, src, st, ss, dt, abs_tol, max_its,
+                                             precision, lam, gst, gss, its);
+            }
+        } else {
+            // the step is linear in psi: its transpose needs no forward states
+            for (int k = nsteps - 1; k >= 0 && ok; --k)
+                ok = idc_step_vjp_device(ctx, nullptr, src, st, ss, dt, abs_tol, max_its, precision, lam, nullptr,
+                                         nullptr, its);
+        }
+        if (mem == TSW_MEM_HOST) {
+            stage_out(ctx, grad_psi0, lam, n, mem);
+            if (gst) stage_out(ctx, grad_sigma_t, gst, g.ncell_h(), mem);
+            if (gss) stage_out(ctx, grad_sigma_s, gss, g.ncell_h(), mem);
+        }
+        sync(ctx);
+    });
+    if (rc == TSW_OK && !ok) {
+        ctx->err = "an implicit solve of the IDC adjoint failed to converge";
         return TSW_E_NOCONV;
     }
     return rc;

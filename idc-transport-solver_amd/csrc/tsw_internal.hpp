@@ -225,7 +225,9 @@ int reduction_blocks();
 void launch_reverse_ordinates(const Geom &g, const double *in, double *out, int kmode, const double *w, int qmode,
                               bool w_of_out, hipStream_t s);
 void launch_cell_grads(const Geom &g, const double *psi, const double *zeta, const double *phi, double *gst,
-                       double *gss, hipStream_t s);
+                       double *gss, hipStream_t s, double scale = 1.0, bool accumulate = false);
+void launch_kt_apply(const Geom &g, const double *y, double *out, const double *ss, const double *w, double alpha,
+                     hipStream_t s);
 void launch_inflow_grad(const Geom &g, const double *zeta, double *out, const double *xi, const double *eta,
                         hipStream_t s);
 

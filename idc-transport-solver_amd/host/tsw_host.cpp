@@ -106,6 +106,13 @@ int precision_from(const ParameterList &p) {
     throw std::invalid_argument("sweep_precision must be fp64 or fp32, not '" + v + "'");
 }
 int all_cells(OpDomain d) { return ((int)d & (int)OpDomain::Boundary) ? 1 : 0; }
+bool flag_from(const ParameterList &p, const std::string &key, bool dflt) {
+    if (!p.Has(key)) return dflt;
+    const std::string v = p.GetString(key);
+    if (v == "true" || v == "1" || v == "yes") return true;
+    if (v == "false" || v == "0" || v == "no") return false;
+    throw std::invalid_argument(key + " must be true or false, not '" + v + "'");
+}
 }  // namespace
 
 SweepOperator::SweepOperator(std::shared_ptr<Device> dev, const ParameterList &params, double)
@@ -172,6 +179,9 @@ PeierlsGMRESSolver::PeierlsGMRESSolver(std::shared_ptr<Device> dev, const Parame
     if (p.Has("abs_tol")) abs_tol_ = p.GetDouble("abs_tol");
     if (p.Has("max_its")) max_its_ = p.GetInt("max_its");
     if (p.Has("gmres_restart")) restart_ = p.GetInt("gmres_restart");
+    dev_->Check(tsw_set_tuning(dev_->ctx(), "use_dsa", flag_from(p, "use_dsa", false) ? 1 : 0), "use_dsa");
+    if (p.Has("dsa_cycles")) dev_->Check(tsw_set_tuning(dev_->ctx(), "dsa_cycles", (int)p.GetInt("dsa_cycles")),
+                                         "dsa_cycles");
 }
 
 void PeierlsGMRESSolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t, const CrossSection &sigma_s,
@@ -184,6 +194,32 @@ void PeierlsGMRESSolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t
     final_error_ = res;
     if (rc == TSW_E_NOCONV) throw std::runtime_error("GMRES transport solve failed to converge");
     dev_->Check(rc, "PeierlsGMRESSolver::Solve");
+}
+
+IDCIntegrator::IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &p)
+    : dev_(std::move(dev)), precision_(precision_from(p)) {
+    if (p.Has("abs_tol")) abs_tol_ = p.GetDouble("abs_tol");
+    if (p.Has("max_its")) max_its_ = p.GetInt("max_its");
+    const int nodes = p.Has("idc_num_nodes") ? (int)p.GetInt("idc_num_nodes") : 4;
+    const int corr = p.Has("idc_num_corr") ? (int)p.GetInt("idc_num_corr") : 3;
+    int type = TSW_IDC_LOBATTO;
+    if (p.Has("idc_nodes_type")) {
+        const std::string t = p.GetString("idc_nodes_type");
+        if (t == "gauss-lobatto") type = TSW_IDC_LOBATTO;
+        else if (t == "gauss-legendre") type = TSW_IDC_LEGENDRE;
+        else throw std::out_of_range("idc_nodes_type '" + t + "' is not built");
+    }
+    const int rc = tsw_idc_set(dev_->ctx(), nodes, corr, type, flag_from(p, "idc_use_collocation", false) ? 1 : 0);
+    if (rc == TSW_E_ARG) throw std::invalid_argument(tsw_last_error(dev_->ctx()));
+    dev_->Check(rc, "tsw_idc_set");
+}
+
+void IDCIntegrator::Step(OrdinateFlux &initial_condition, const OrdinateFlux &source, const CrossSection &sigma_t,
+                         const CrossSection &sigma_s, double dt) {
+    const int rc = tsw_idc_step(dev_->ctx(), initial_condition.data(), source.data(), sigma_t.data(), sigma_s.data(),
+                                dt, abs_tol_, (int)max_its_, precision_, TSW_MEM_HOST);
+    if (rc == TSW_E_NOCONV) throw std::runtime_error("an implicit solve of the IDC step failed to converge");
+    dev_->Check(rc, "IDCIntegrator::Step");
 }
 
 }  // namespace tsw_host

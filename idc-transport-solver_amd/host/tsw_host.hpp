@@ -194,7 +194,7 @@ class SISolver : public ImplicitSolver {
 
 // Sweep-preconditioned GMRES on the Peierls form (PETScPeierlsSolver.cpp:423-572); parameters
 // abs_tol (1e-8), max_its (3000), gmres_restart (30), sweep_precision ("fp32" selects FP32 sweeps
-// with FP64 refinement).
+// with FP64 refinement), use_dsa (false; PETScPeierlsSolver.cpp:151) and dsa_cycles (2).
 class PeierlsGMRESSolver : public ImplicitSolver {
   public:
     PeierlsGMRESSolver(std::shared_ptr<Device> dev, const ParameterList &params);
@@ -207,6 +207,25 @@ class PeierlsGMRESSolver : public ImplicitSolver {
     double abs_tol_ = 1e-8;
     int64_t max_its_ = 3000;
     int64_t restart_ = 30;
+    int precision_ = TSW_FP64;
+};
+
+// IDC time integrator, the non-hybrid error-form IDCIntegrator (time_integrators/RKDG/idc.cpp:846-940)
+// over the same device.  Parameters: idc_num_nodes (4), idc_num_corr (3), idc_nodes_type
+// ("gauss-lobatto" default, "gauss-legendre"), idc_use_collocation (false), abs_tol (1e-10) and max_its
+// (3000) of the implicit solves, sweep_precision.  Step advances initial_condition by dt in place
+// (halos zero), with the material source and boundary inflow in `source`, constant over the step.
+class IDCIntegrator {
+  public:
+    IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &params);
+    void Step(OrdinateFlux &initial_condition, const OrdinateFlux &source, const CrossSection &sigma_t,
+              const CrossSection &sigma_s, double dt);
+    std::string GetDescriptor() const { return "idc"; }
+
+  private:
+    std::shared_ptr<Device> dev_;
+    double abs_tol_ = 1e-10;
+    int64_t max_its_ = 3000;
     int precision_ = TSW_FP64;
 };
 

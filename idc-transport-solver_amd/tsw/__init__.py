@@ -29,6 +29,8 @@ FP64 = 0
 FP32 = 1
 MEM_HOST = 0
 MEM_DEVICE = 1
+IDC_LOBATTO = 0
+IDC_LEGENDRE = 1
 
 
 class TswProblem(ctypes.Structure):
@@ -84,6 +86,10 @@ def lib():
         L.tsw_gmres_solve.argtypes = [c, _vp, _vp, _vp, d, d, i, i, i, i, _vp, _ip, _dp]
         L.tsw_sweep_vjp.argtypes = [c, _vp, _vp, d, _vp, _vp, _vp, i, i]
         L.tsw_solve_vjp.argtypes = [c, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, d, i, i, i, i, _ip, _dp]
+        L.tsw_idc_set.argtypes = [c, i, i, i, i]
+        L.tsw_idc_step.argtypes = [c, _vp, _vp, _vp, _vp, d, d, i, i, i]
+        L.tsw_idc_run.argtypes = [c, _vp, _vp, _vp, _vp, d, i, d, i, i, i]
+        L.tsw_idc_adjoint.argtypes = [c, _vp, _vp, _vp, _vp, d, i, i, _vp, _vp, _vp, _vp, d, i, i, i]
         L.tsw_ordinates_level_symmetric.argtypes = [i, _dp, _dp, _dp, _ip]
         L.tsw_sweep_timing.argtypes = [c, i, ctypes.POINTER(ll), _dp]
         L.tsw_batch_set.argtypes = [c, i, ctypes.POINTER(TswProblem), ctypes.POINTER(_vp), d, i]
@@ -271,6 +277,33 @@ class Context:
         self._check(self._L.tsw_solve_vjp(self._h, a, b, c, dt, d, e, f, h, abs_tol, max_its, restart, precision,
                                           mem, ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
+
+    def idc_set(self, num_nodes=4, num_corrections=3, nodes_type=IDC_LOBATTO, use_collocation=False):
+        self._check(self._L.tsw_idc_set(self._h, num_nodes, num_corrections, nodes_type, int(use_collocation)))
+
+    def idc_step(self, psi, source, sigma_t, sigma_s, dt, abs_tol=1e-12, max_its=3000, precision=FP64):
+        """psi^n -> psi^{n+1} in place (zero halos); source holds Q and the inflow."""
+        self._expect(psi=psi, psi2=source, cell=sigma_t, cell2=sigma_s)
+        (a, b, c, d), mem = _addrs(psi, source, sigma_t, sigma_s)
+        self._check(self._L.tsw_idc_step(self._h, a, b, c, d, dt, abs_tol, max_its, precision, mem))
+        return psi
+
+    def idc_run(self, psi, source, sigma_t, sigma_s, dt, nsteps, abs_tol=1e-12, max_its=3000, precision=FP64):
+        self._expect(psi=psi, psi2=source, cell=sigma_t, cell2=sigma_s)
+        (a, b, c, d), mem = _addrs(psi, source, sigma_t, sigma_s)
+        self._check(self._L.tsw_idc_run(self._h, a, b, c, d, dt, nsteps, abs_tol, max_its, precision, mem))
+        return psi
+
+    def idc_adjoint(self, seed, grad_psi0, source, sigma_t, sigma_s, dt, nsteps, psi0=None, checkpoint_every=10,
+                    grad_sigma_t=None, grad_sigma_s=None, abs_tol=1e-12, max_its=3000, precision=FP64):
+        """grad_psi0 <- dJ/dpsi^0 of J = <seed, psi^nsteps>; optional cross-section gradients (need psi0)."""
+        self._expect(psi=seed, psi2=grad_psi0, psi3=source, psi4=psi0, cell=sigma_t, cell2=sigma_s,
+                     cell3=grad_sigma_t, cell4=grad_sigma_s)
+        (a, b, c, d, e, f, h, k), mem = _addrs(psi0, source, sigma_t, sigma_s, seed, grad_psi0, grad_sigma_t,
+                                               grad_sigma_s)
+        self._check(self._L.tsw_idc_adjoint(self._h, a, b, c, d, dt, nsteps, checkpoint_every, e, f, h, k, abs_tol,
+                                            max_its, precision, mem))
+        return grad_psi0
 
     def batch_set(self, problems, sigma_t, dt=math.inf):
         """problems: list of dicts with nx, ny, p, xi, eta, w, quad_off and optional ax, bx, ay, by;

@@ -155,6 +155,42 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
                   const double *seed, double *grad_src, double *grad_sigma_t, double *grad_sigma_s, double abs_tol,
                   int max_its, int restart, int precision, int mem, int *iters, double *resid);
 
+#define TSW_IDC_LOBATTO 0  /* Gauss-Lobatto nodes (both endpoints) */
+#define TSW_IDC_LEGENDRE 1 /* Gauss-Legendre nodes (interior), output by collocation */
+
+/* IDC integrator configuration, the reference's idc_num_nodes, idc_num_corr, idc_nodes_type and
+ * idc_use_collocation parameters (idc.cpp:223, SetNodes :2305-2376): num_nodes quadrature nodes on each
+ * step (2..12 Lobatto, 1..12 Legendre), num_corrections correction sweeps (0..32).  Defaults: 4 Lobatto
+ * nodes, 3 corrections, no collocation.  Only the non-hybrid error-form integrator is built. */
+int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type, int use_collocation);
+
+/* One IDC step psi^n -> psi^{n+1} in place, IDCIntegrator::Step_Nonhybrid (idc.cpp:846-940) with
+ * IDCType::Error: implicit-Euler prediction over the nodes, corrections whose sources integrate the
+ * residuals (L - S P) psi with the nodes' integration matrix (ComputeSource :1686-1740), result at the
+ * last node or by collocation.  psi's halos are taken as zero (as the reference assumes) and returned
+ * zero; source holds Q with the inflow in its halos, constant over the step.  Each implicit solve is the
+ * Peierls GMRES of tsw_gmres_solve (restart 30, abs_tol, max_its, precision: FP32 sweeps inside FP64
+ * refinement for TSW_FP32); residuals and correction sums are FP64.  TSW_E_NOCONV if a solve fails. */
+int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const double *sigma_t,
+                 const double *sigma_s, double dt, double abs_tol, int max_its, int precision, int mem);
+
+/* nsteps IDC steps of size dt in place (TimeIntegrator stepping loop); arguments as tsw_idc_step. */
+int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const double *sigma_t, const double *sigma_s,
+                double dt, int nsteps, double abs_tol, int max_its, int precision, int mem);
+
+/* Adjoint of tsw_idc_run for J = <seed, psi^{nsteps}> (interior coefficients): grad_psi0 (psi layout,
+ * halos 0) receives dJ/dpsi^0; grad_sigma_t / grad_sigma_s (cell arrays, may be NULL) receive dJ/dsigma
+ * summed over every implicit solve and residual of every step.  Each step's transpose reverses its
+ * operations: transposed solves by the adjoint problem's GMRES, (L - S P)^T for the residuals.  For the
+ * cross-section gradients the forward trajectory is run once keeping psi every `checkpoint_every`
+ * steps, and each checkpoint segment is recomputed before it is stepped back through; without them no
+ * forward state is needed (the step is linear in psi) and psi0 may be NULL.  The reference has no
+ * adjoint time integrator. */
+int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, const double *sigma_t,
+                    const double *sigma_s, double dt, int nsteps, int checkpoint_every, const double *seed,
+                    double *grad_psi0, double *grad_sigma_t, double *grad_sigma_s, double abs_tol, int max_its,
+                    int precision, int mem);
+
 /* Level-symmetric S_order ordinates reduced to the 2D hemisphere (order in {2,4,6,8,10,12,16}):
  * nq = order*(order+2)/2 directions, weights summing to 4 pi, grouped by quadrant as above.
  * The reference has no level-symmetric set; this plays the role of OrdinateSet's constructors

@@ -99,6 +99,41 @@ int main(int argc, char **argv) {
     }
     check(std::sqrt(num / den) < 1e-10, "GMRES and SI reach the same scalar flux");
 
+    // DSA-preconditioned GMRES (use_dsa) reaches the same solution
+    params.Set("use_dsa", "true");
+    auto gd = ImplicitSolver::Create(dev, params);
+    OrdinateFlux psi_gd = q;
+    gd->Solve(psi_gd, st, ss);
+    check(gd->final_error() <= 1e-13, "Peierls GMRES with DSA converged");
+    dump(out + "/gmres_dsa.bin", psi_gd.data(), psi_gd.size());
+    params.Set("use_dsa", "false");
+
+    // two IDC steps (4 Gauss-Lobatto nodes, 3 corrections) from the interior of q, vacuum inflow
+    ParameterList ip = ParameterList::FromText("idc_num_nodes 4\nidc_num_corr 3\nidc_nodes_type gauss-lobatto\n"
+                                               "abs_tol 1e-13\n");
+    IDCIntegrator idc(dev, ip);
+    OrdinateFlux psi_t = q, src0 = q;
+    for (int64_t i = 0; i < dd.nx + 2; ++i)
+        for (int64_t j = 0; j < dd.ny + 2; ++j)
+            if (i == 0 || j == 0 || i == dd.nx + 1 || j == dd.ny + 1)
+                for (int64_t k = 0; k < os.nq() * (p + 1) * (p + 1); ++k) {
+                    const size_t at = ((size_t)i * (dd.ny + 2) + j) * os.nq() * (p + 1) * (p + 1) + k;
+                    psi_t.data()[at] = 0.0;
+                    src0.data()[at] = 0.0;
+                }
+    idc.Step(psi_t, src0, st, ss, 0.05);
+    idc.Step(psi_t, src0, st, ss, 0.05);
+    dump(out + "/idc.bin", psi_t.data(), psi_t.size());
+    bool threw_idc = false;
+    try {
+        ParameterList badi;
+        badi.Set("idc_nodes_type", "gauss-radau");
+        IDCIntegrator bad_idc(dev, badi);
+    } catch (const std::out_of_range &) {
+        threw_idc = true;
+    }
+    check(threw_idc, "unbuilt idc_nodes_type throws std::out_of_range");
+
     bool threw = false;
     try {
         params.Set("solve_type", "no-such-solver");

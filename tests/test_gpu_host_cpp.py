@@ -30,3 +30,14 @@ def test_host_cpp_driver_matches_oracle(tmp_path):
     ref_si, _, _ = orc_si(g, o, st, ss, src, abs_tol=1e-13, max_its=1000)
     assert rel_err(load("si.bin", src.shape), ref_si) <= 1e-12
     assert rel_err(load("gmres.bin", src.shape), ref_si) <= 1e-10
+    assert rel_err(load("gmres_dsa.bin", src.shape), ref_si) <= 1e-10
+    # two IDC steps of the C++ IDCIntegrator against the oracle's restatement
+    import idc
+    from _common import OracleIdcOps
+    psi0 = src.copy()
+    src0 = src.copy()
+    OracleIdcOps.zero_halo(psi0)
+    OracleIdcOps.zero_halo(src0)
+    ops = OracleIdcOps(g, o, st, ss, tol=1e-14)
+    ref = idc.idc_step(ops, idc.idc_step(ops, psi0, src0, 0.05), src0, 0.05)
+    assert rel_err(load("idc.bin", src.shape), ref) <= 1e-10
