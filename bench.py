@@ -42,19 +42,23 @@ def cpu_model():
 
 
 def cpu_baseline(P, sweeps):
-    """Times the oracle's FP64 sweep (one thread) on `sweeps` sweeps of the same lattice."""
+    """Times the oracle's FP64 wavefront sweep on `sweeps` sweeps of the same problem, with each diagonal's
+    cells x ordinates shared among the host threads (OMP_NUM_THREADS, else the CPU count, at most 16:
+    one GPU's share of the host)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from _common import Grid, Ordinates, orc_sweep
+    from _common import Grid, Ordinates, orc_sweep_omp
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     g = Grid(P.nx, P.ny, P.p, P.ax, P.bx, P.ay, P.by)
     o = Ordinates(P.xi, P.eta, P.w, P.quad_off)
     src = P.source
+    orc_sweep_omp(g, o, P.sigma_t, src, threads=threads)   # warm-up (threads, pages)
     t0 = time.perf_counter()
     for _ in range(sweeps):
-        orc_sweep(g, o, P.sigma_t, src)
+        orc_sweep_omp(g, o, P.sigma_t, src, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": P.unknowns * sweeps / dt, "unit": "angular unknowns swept/s", "cores": 1, "kind": "port",
-            "sample": f"{sweeps} FP64 sweeps of the {P.nx}x{P.ny} Q{P.p} nq={P.nq} {P.name} problem by the oracle "
-                      f"(1 thread, {dt:.1f} s) on {cpu_model()}"}
+    return {"value": P.unknowns * sweeps / dt, "unit": "angular unknowns swept/s", "cores": threads, "kind": "port",
+            "sample": f"{sweeps} FP64 sweeps of the {P.nx}x{P.ny} Q{P.p} nq={P.nq} {P.name} problem by the oracle's "
+                      f"wavefront sweep on {threads} threads ({dt:.2f} s) on {cpu_model()}"}
 
 
 def bench_si(args):

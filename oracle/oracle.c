@@ -170,6 +170,46 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
 }
 
 /* ---------------------------------------------------------------------------------------------- */
+/* The same wavefront with the cells x ordinates of each diagonal shared among OpenMP threads, one   */
+/* barrier per diagonal, as SweepPatternUniCartWavefront::Sweep distributes them with its `omp for`  */
+/* (SweepPatternUniCartWavefront.cpp:146,253-296).  Per-cell arithmetic is that of orc_sweep, so the */
+/* result is bitwise identical; this is the CPU baseline's multi-core form.                          */
+/* ---------------------------------------------------------------------------------------------- */
+void orc_sweep_omp(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                   double dx, double dy, const double *sigma_t, double dt, double *psi, int nthreads)
+{
+    const int P = p + 1, N = P * P;
+    double *Aq = (double *)malloc(sizeof(double) * (size_t)nq * N * N);
+    orc_aq(p, nq, xi, eta, dx, dy, Aq);
+    const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+    /* quadrant of each ordinate */
+    int *quad = (int *)malloc(sizeof(int) * (size_t)nq);
+    for (int k = 0; k < 4; ++k)
+        for (int q = quad_off[k]; q < quad_off[k + 1]; ++q) quad[q] = k;
+    const int num_diags = nx + ny - 1;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        double A[256], B[16], W0[16], W1[16];
+        for (int diag = 1; diag <= num_diags; ++diag) {
+            const int i_hi = diag < nx ? diag : nx;
+            const int i_lo = diag - ny + 1 > 1 ? diag - ny + 1 : 1;
+            const int diag_length = i_hi - i_lo + 1;
+#pragma omp for schedule(static)
+            for (long long t = 0; t < (long long)diag_length * nq; ++t) {
+                const int cell = (int)(t / nq) + 1, q = (int)(t % nq);
+                int i = i_hi - cell + 1;
+                int j = diag - i + 1;
+                if (quad[q] >= 2) i = nx - i + 1;
+                if (quad[q] & 1) j = ny - j + 1;
+                cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, inv_dt, q, i, j, psi, A, B, W0, W1);
+            }   /* implicit barrier: the diagonal is complete */
+        }
+    }
+    free(Aq);
+    free(quad);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
 /* Transposed sweep.  The reference has no adjoint; this is the transpose of the linear system     */
 /* that orc_sweep solves, written out directly.  With row (d,e) of cell c's equation                */
 /*   A_c psi_c - Cx psi_xup - Cy psi_yup = dx dy src_c,                                            */

@@ -44,6 +44,11 @@ void orc_ge_solve(int N, double *A, double *b);
 void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
                double dx, double dy, const double *sigma_t, double dt, double *psi);
 
+/* orc_sweep with each diagonal's cells x ordinates shared among `nthreads` OpenMP threads (one barrier
+ * per diagonal, SweepPatternUniCartWavefront.cpp:146,253-296); bitwise equal to orc_sweep. */
+void orc_sweep_omp(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                   double dx, double dy, const double *sigma_t, double dt, double *psi, int nthreads);
+
 /* Transposed sweep: out (interior) <- zeta = L^{-T} g for the map src -> psi of orc_sweep with zero
  * inflow, where L^{-1} src solves A_c psi_c - (upwind couplings) = dx dy src_c.  out's halo cells
  * receive the gradient of <g, psi> with respect to the inflow halo values (zero on outflow sides and

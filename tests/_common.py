@@ -61,6 +61,7 @@ def oracle():
         lib.orc_si_solve.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, P, P, IP, P]
         lib.orc_peierls_gmres.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, i, P, P, IP, P]
         lib.orc_sweep_adjoint.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, P]
+        lib.orc_sweep_omp.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, i]
         _oracle = lib
     return _oracle
 
@@ -138,6 +139,14 @@ def orc_sweep(g: Grid, o: Ordinates, sigma_t, psi, dt=math.inf):
     s = np.ascontiguousarray(sigma_t, dtype=np.float64)
     oracle().orc_sweep(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy, ptr(s),
                        dt, ptr(out))
+    return out
+
+
+def orc_sweep_omp(g: Grid, o: Ordinates, sigma_t, psi, dt=math.inf, threads=4):
+    out = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    s = np.ascontiguousarray(sigma_t, dtype=np.float64)
+    oracle().orc_sweep_omp(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy, ptr(s), dt,
+                           ptr(out), threads)
     return out
 
 

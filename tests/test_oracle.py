@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from _common import (Grid, dg_weak_form_solve, oracle, orc_gmres, orc_lmv, orc_pmv, orc_si, orc_smv,
-                     orc_sweep, project_field, ptr, random_ordinates)
+                     orc_sweep, orc_sweep_omp, project_field, ptr, random_ordinates)
 
 
 @pytest.mark.parametrize("N", [1, 4, 9, 16])
@@ -199,3 +199,15 @@ def test_si_balance_and_gmres_fixed_point():
     assert rc == 0 and res <= 1e-12
     np.testing.assert_allclose(phi_g[1:-1, 1:-1], phi[1:-1, 1:-1], rtol=1e-9, atol=1e-11)
     np.testing.assert_allclose(psi_g[1:-1, 1:-1], psi[1:-1, 1:-1], rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_sweep_bitwise_equal(threads):
+    """The multi-core baseline sweep distributes each diagonal's cells and ordinates over threads with a
+    barrier per diagonal; every cell's arithmetic is unchanged, so it equals the serial sweep bitwise."""
+    rng = np.random.default_rng(17)
+    g = Grid(13, 9, 2, 0.0, 1.3, 0.0, 0.9)
+    o = random_ordinates(3, rng)
+    st = rng.uniform(0.4, 3.0, size=(g.nx + 2, g.ny + 2))
+    src = rng.normal(size=g.psi_shape(o.nq))
+    np.testing.assert_array_equal(orc_sweep_omp(g, o, st, src, 0.2, threads), orc_sweep(g, o, st, src, 0.2))
