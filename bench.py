@@ -181,7 +181,8 @@ def bench_idc(args):
     wall = time.perf_counter() - t0
     nsw, swms = ctx.sweep_timing(reset=True)
     U = L_["xi"].size * n * n * (p + 1) ** 2
-    phi_mass = float(psi[1:-1, 1:-1, :, 0, 0].sum(dim=2).sum()) * 4 * math.pi * L_["h"] ** 2 / (4 * math.pi)
+    w_d = torch.from_numpy(L_["w"]).to(dev)
+    phi_mass = float((psi[1:-1, 1:-1, :, 0, 0] * w_d).sum()) * L_["h"] ** 2   # integral of phi (1 at t=0)
     print(f"bench: idc {steps} steps {wall:.3f}s, {nsw} sweeps ({swms:.1f} ms in sweeps)", file=sys.stderr, flush=True)
     seed = torch.from_numpy(_ring_seed(L_)).to(dev)
     J = float((seed * psi).sum())
@@ -273,13 +274,17 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
+def traffic_from_profiles(kernel, band_rows=None):
+    """HBM bytes per launch of `kernel` (run with `band_rows`) from the committed rocprofv3 --pmc summary,
+    if one was measured for that configuration."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             data = json.load(f)
-        return data.get(kernel, {}).get("hbm_bytes_per_launch")
+        entry = data.get(kernel, {})
+        if band_rows is not None and entry.get("band_rows", band_rows) != band_rows:
+            return None
+        return entry.get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -395,7 +400,9 @@ def main():
         torch.cuda.synchronize()
         torch.add(a, 1.0, out=b)     # reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
-        print("bench: probes: fill 1 GiB, add 1 GiB -> 1 GiB", file=sys.stderr, flush=True)
+        s_ = float(b.sum())          # reads 1 GiB
+        torch.cuda.synchronize()
+        print(f"bench: probes: fill 1 GiB, add 1 GiB -> 1 GiB, sum 1 GiB ({s_:.3e})", file=sys.stderr, flush=True)
     if args.sweeps_only:
         ctx.close()
         return
@@ -447,9 +454,11 @@ def main():
         "mixed_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": "k_diag_sweep<float,3,true,1,3> (FP32 in-place operator-contract sweep)",
+                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}> (FP32 in-place operator-contract "
+                               f"sweep, band_rows {tuned32[0]})",
                      "algorithmic_bytes_per_launch": bytes32,
-                     "traffic": traffic_from_profiles("k_diag_sweep<float, 3, true, 1, 3>")},
+                     "traffic": traffic_from_profiles(f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}>",
+                                                      tuned32[0])},
     }
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(P, args.cpu_sweeps)

@@ -1201,12 +1201,13 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         launch_fill_region<double>(g, psi64, (size_t)g.nq * N, 0, 0.0, s);
         const int max_refine = 30;
         for (int refine = 0;; ++refine) {
-            // r = Q + S P psi - L psi on the interior, zero halos
+            // r = Q + S P psi - (L + 1/dt) psi on the interior, zero halos
             TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
             launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
             TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
             launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s);
             launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s);
+            if (inverse_dt(dt) != 0.0) launch_axpby(npsi, -inverse_dt(dt), psi64, 1.0, res64, s);
             launch_fill_region<double>(g, res64, (size_t)g.nq * N, 1, 0.0, s);
             // b = P L^{-1} r: the Peierls residual of the current iterate
             launch_convert_d2f(res64, p32, npsi, s);
