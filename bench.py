@@ -305,6 +305,7 @@ def main():
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
+    ap.add_argument("--dsa-cycles", type=int, default=1, help="multigrid V-cycles per DSA application")
     ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "
                                                      "for each and report them (tuning aid)")
     ap.add_argument("--prefetch-scan", default="", help="comma-separated prefetch depths crossed with --band-scan")
@@ -411,6 +412,7 @@ def main():
     w64n, n64n, ms64n, its64n = timed(tsw.FP64, phi64)
     print(f"bench: fp64 without DSA: wall {w64n:.3f}s sweeps {n64n} its {its64n}", file=sys.stderr, flush=True)
     ctx.set_tuning("use_dsa", args.dsa)
+    ctx.set_tuning("dsa_cycles", args.dsa_cycles)
     w64, n64, ms64, its64 = timed(tsw.FP64, phi64)
     print(f"bench: fp64 wall {w64:.3f}s sweeps {n64} sweep-ms {ms64:.1f} its {its64}", file=sys.stderr, flush=True)
     w32, n32, ms32, its32 = timed(tsw.FP32, phi32)
@@ -442,6 +444,7 @@ def main():
                                "with DSA to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
                    "sweep_pattern": args.pattern, "band_rows": args.band_rows, "prefetch": args.prefetch,
                    "tuned_band_rows_prefetch": {"fp32": tuned32, "fp64": tuned64}, "dsa": bool(args.dsa),
+                   "dsa_cycles": args.dsa_cycles,
                    "unknowns_per_sweep": U},
         "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
                  "gmres_its": [i for i, _ in its64], "solve_sweep_ms": ms64 / max(n64, 1),

@@ -111,7 +111,7 @@ struct tsw_ctx {
     // multigrid hierarchy of the diffusion operator for the current cross sections, and the apply
     // y = x + C^{-1} sigma_s x captured as a graph over staging vectors.
     bool use_dsa = false;
-    int dsa_cycles = 2;
+    int dsa_cycles = 1;
     // IDC time integrator (tsw_idc_set / tsw_idc_step): nodes on [0,1] (stage 0 = left end), the
     // integration matrix W (row 0 over [0,1], row n over [t_{n-1}, t_n]) and the stage, residual and
     // correction vectors in the reference layout, FP64.
@@ -127,9 +127,14 @@ struct tsw_ctx {
     struct Dsa {
         std::vector<MgLevel> lv;
         std::vector<DBuf> store;
-        DBuf in, out;
+        DBuf in, out, lv_dev;
+        int fused_from = 0;  // first level handled by the single-workgroup kernel
         const double *sigma_s = nullptr;
         hipGraphExec_t graph = nullptr;
+        // what the hierarchy was built for (rebuilt when any of it changes)
+        std::vector<double> key_st, key_ss;
+        double key_idt = -1.0;
+        int key_cycles = 0;
         ~Dsa() {
             if (graph) (void)hipGraphExecDestroy(graph);
         }
@@ -391,9 +396,13 @@ void dsa_vcycle(tsw_ctx *c, size_t l, bool zero_init) {
     auto &lv = c->dsa.lv;
     const MgLevel &L = lv[l];
     hipStream_t s = c->stream;
+    if (zero_init && (int)l >= c->dsa.fused_from) {  // the small levels in one launch
+        launch_mg_fused(c->dsa.lv_dev.as<MgLevel>(), (int)l, (int)lv.size(), s);
+        return;
+    }
     if (zero_init) launch_mg_zero(L.x, L.nx * L.ny, s);
     if (l + 1 == lv.size()) {  // coarsest: a few cells, Gauss-Seidel to convergence
-        for (int k = 0; k < 32; ++k) {
+        for (int k = 0; k < 16; ++k) {
             launch_mg_rbgs(L, L.x, L.b, 0, s);
             launch_mg_rbgs(L, L.x, L.b, 1, s);
         }
@@ -418,16 +427,20 @@ void dsa_vcycle(tsw_ctx *c, size_t l, bool zero_init) {
 void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double dt) {
     const Geom &g = c->g;
     auto &D = c->dsa;
-    if (D.graph) {
-        (void)hipGraphExecDestroy(D.graph);
-        D.graph = nullptr;
-    }
     const size_t nh = g.ncell_h();
     std::vector<double> st(nh), ss(nh);
     TSW_HIP(hipMemcpyAsync(st.data(), sigma_t, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
     TSW_HIP(hipMemcpyAsync(ss.data(), sigma_s, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     const double idt = inverse_dt(dt);
+    // the captured apply reads sigma_s from its pointer: reuse it only for the same pointer and data
+    if (D.graph && D.sigma_s == sigma_s && D.key_idt == idt && D.key_cycles == c->dsa_cycles && D.key_st == st &&
+        D.key_ss == ss)
+        return;
+    if (D.graph) {
+        (void)hipGraphExecDestroy(D.graph);
+        D.graph = nullptr;
+    }
     struct HostLevel {
         int nx, ny;
         std::vector<double> cx, cy, sv;
@@ -508,8 +521,17 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
         L.b = static_cast<double *>(D.store[6 * l + 4].reserve(sizeof(double) * n));
         L.r = static_cast<double *>(D.store[6 * l + 5].reserve(sizeof(double) * n));
     }
+    // levels of at most 4096 cells run inside one workgroup
+    D.fused_from = (int)hl.size() - 1;
+    while (D.fused_from > 0 && (size_t)hl[D.fused_from - 1].nx * hl[D.fused_from - 1].ny <= 4096) --D.fused_from;
+    TSW_HIP(hipMemcpyAsync(D.lv_dev.reserve(sizeof(MgLevel) * D.lv.size()), D.lv.data(), sizeof(MgLevel) * D.lv.size(),
+                           hipMemcpyHostToDevice, c->stream));
     sync(c);  // host vectors go out of scope
     D.sigma_s = sigma_s;
+    D.key_st = std::move(st);
+    D.key_ss = std::move(ss);
+    D.key_idt = idt;
+    D.key_cycles = c->dsa_cycles;
     double *in = static_cast<double *>(D.in.reserve(sizeof(double) * g.nphi()));
     double *out = static_cast<double *>(D.out.reserve(sizeof(double) * g.nphi()));
     hipGraph_t graph;
@@ -1465,6 +1487,8 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     ctx->clear_graphs();
     ctx->adj.reset();
     ctx->tuned.clear();
+    if (ctx->dsa.graph) (void)hipGraphExecDestroy(ctx->dsa.graph);
+    ctx->dsa.graph = nullptr;
     Geom &g = ctx->g;
     g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
     g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;

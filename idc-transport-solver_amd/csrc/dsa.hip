@@ -91,8 +91,87 @@ __global__ void k_mg_prolong_add(const double *__restrict__ xc, double *__restri
     xf[c] += xc[(i / 2) * nyc + j / 2];
 }
 
+// The V-cycle below level l0 in one workgroup, for levels small enough that launches would dominate:
+// descend (zero x, two red-black sweeps, residual, restrict), coarsest level (16 sweeps), ascend (prolong,
+// two sweeps in reverse colour order).  Phases are separated by block barriers; lv is a device array.
+__device__ void mg_rbgs_block(const MgLevel &L, int colour) {
+    const int n = L.nx * L.ny;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int i = c / L.ny, j = c % L.ny;
+        if (((i + j) & 1) != colour) continue;
+        const double gw = L.cx[i * L.ny + j], ge = L.cx[(i + 1) * L.ny + j];
+        const double gs = L.cy[i * (L.ny + 1) + j], gn = L.cy[i * (L.ny + 1) + j + 1];
+        double acc = L.b[c];
+        if (i > 0) acc += gw * L.x[c - L.ny];
+        if (i < L.nx - 1) acc += ge * L.x[c + L.ny];
+        if (j > 0) acc += gs * L.x[c - 1];
+        if (j < L.ny - 1) acc += gn * L.x[c + 1];
+        L.x[c] = acc / (L.sv[c] + gw + ge + gs + gn);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_mg_fused(const MgLevel *__restrict__ lv, int l0, int nlev) {
+    const int last = nlev - 1;
+    for (int l = l0; l <= last; ++l) {
+        const MgLevel L = lv[l];
+        const int n = L.nx * L.ny;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) L.x[c] = 0.0;
+        __syncthreads();
+        const int sweeps = l == last ? 16 : 2;
+        for (int k = 0; k < sweeps; ++k) {
+            mg_rbgs_block(L, 0);
+            mg_rbgs_block(L, 1);
+        }
+        if (l == last) break;
+        const MgLevel C = lv[l + 1];
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int i = c / L.ny, j = c % L.ny;
+            const double gw = L.cx[i * L.ny + j], ge = L.cx[(i + 1) * L.ny + j];
+            const double gs = L.cy[i * (L.ny + 1) + j], gn = L.cy[i * (L.ny + 1) + j + 1];
+            double ax = (L.sv[c] + gw + ge + gs + gn) * L.x[c];
+            if (i > 0) ax -= gw * L.x[c - L.ny];
+            if (i < L.nx - 1) ax -= ge * L.x[c + L.ny];
+            if (j > 0) ax -= gs * L.x[c - 1];
+            if (j < L.ny - 1) ax -= gn * L.x[c + 1];
+            L.r[c] = L.b[c] - ax;
+        }
+        __syncthreads();
+        const int nc = C.nx * C.ny;
+        for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+            const int I = c / C.ny, J = c % C.ny;
+            double acc = 0.0;
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int i = 2 * I + di, j = 2 * J + dj;
+                    if (i < L.nx && j < L.ny) acc += L.r[i * L.ny + j];
+                }
+            C.b[c] = acc;
+        }
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= l0; --l) {
+        const MgLevel L = lv[l], C = lv[l + 1];
+        const int n = L.nx * L.ny;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int i = c / L.ny, j = c % L.ny;
+            L.x[c] += C.x[(i / 2) * C.ny + j / 2];
+        }
+        __syncthreads();
+        for (int k = 0; k < 2; ++k) {
+            mg_rbgs_block(L, 1);
+            mg_rbgs_block(L, 0);
+        }
+    }
+}
+
 namespace {
 inline dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
+}
+
+void launch_mg_fused(const MgLevel *lv_dev, int l0, int nlev, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_fused, dim3(1), dim3(1024), 0, s, lv_dev, l0, nlev);
+    TSW_HIP(hipGetLastError());
 }
 
 void launch_dsa_rhs(const Geom &g, const double *x, const double *sigma_s, double *b, hipStream_t s) {

@@ -143,6 +143,8 @@ void launch_mg_rbgs(const MgLevel &L, double *x, const double *b, int colour, hi
 void launch_mg_residual(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s);
 void launch_mg_restrict(const MgLevel &f, const MgLevel &c, const double *rf, double *bc, hipStream_t s);
 void launch_mg_prolong_add(const MgLevel &f, const MgLevel &c, const double *xc, double *xf, hipStream_t s);
+// the V-cycle from level l0 (zero initial guess) to the coarsest in one workgroup; lv_dev: device array
+void launch_mg_fused(const MgLevel *lv_dev, int l0, int nlev, hipStream_t s);
 
 // Bit pattern of an empty band-exchange record: a signalling NaN, which arithmetic never produces.
 template <typename T> struct ExchangeSentinel;

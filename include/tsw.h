@@ -84,7 +84,7 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * tsw_solve_vjp, the reference's `use_dsa` parameter (PETScPeierlsSolver.cpp:151, shell preconditioner
  * :180-182, apply :588-640), applied on the left as PETSc does, so the residual test is on
  * y = r + C^{-1} sigma_s r; C is the cell-mean finite-volume diffusion operator (D = 1/(3(sigma_t+1/dt)),
- * vacuum Marshak faces) inverted approximately by "dsa_cycles" (1..16, default 2) multigrid V-cycles.
+ * vacuum Marshak faces) inverted approximately by "dsa_cycles" (1..16, default 1) multigrid V-cycles.
  * "si_graph" (1 default, 0): staggered source iteration replays captured graphs of 8 iterations with
  * the loop test on the device (same iterates and counts as the eager loop).
  * Unknown keys fail with TSW_E_ARG. */
