@@ -106,6 +106,8 @@ struct tsw_ctx {
     static constexpr int kSiChunk = 8;
     DBuf si_ctl;
     SiControl *si_ctl_host = nullptr;
+    double *gm_host = nullptr;  // pinned: one Arnoldi step's Hessenberg column on its way to the host
+    size_t gm_host_n = 0;
     const int *skip_flag = nullptr;
     // Diffusion synthetic acceleration of the Peierls GMRES (use_dsa, PETScPeierlsSolver.cpp:151):
     // multigrid hierarchy of the diffusion operator for the current cross sections, and the apply
@@ -127,8 +129,9 @@ struct tsw_ctx {
     struct Dsa {
         std::vector<MgLevel> lv;
         std::vector<DBuf> store;
-        DBuf in, out, lv_dev;
-        int fused_from = 0;  // first level handled by the single-workgroup kernel
+        DBuf lv_dev;
+        int fused_from = 0;    // first level handled by the single-workgroup kernel
+        size_t fused_lds = 0;  // its LDS bytes
         const double *sigma_s = nullptr;
         hipGraphExec_t graph = nullptr;
         // what the hierarchy was built for (rebuilt when any of it changes)
@@ -159,6 +162,7 @@ struct tsw_ctx {
         adj.reset();
         if (abort_host) (void)hipHostFree(abort_host);
         if (si_ctl_host) (void)hipHostFree(si_ctl_host);
+        if (gm_host) (void)hipHostFree(gm_host);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -392,34 +396,22 @@ double device_norm2_interior_diff(tsw_ctx *c, const double *a, const double *b) 
 // DSA preconditioner (dsa.hip): hierarchy construction on the host from the cross sections, V-cycles,
 // and the captured apply.
 // ---------------------------------------------------------------------------------------------------
+// One V-cycle from level l: right-hand side lv[l].b, start zero (zero_init) or lv[l].y, result in
+// lv[l].y.  Fine levels: tile smoothing, the residual restricted, the coarse cycle, the prolongated
+// correction smoothed; from fused_from down, one single-workgroup launch.
 void dsa_vcycle(tsw_ctx *c, size_t l, bool zero_init) {
-    auto &lv = c->dsa.lv;
+    auto &D = c->dsa;
+    auto &lv = D.lv;
     const MgLevel &L = lv[l];
     hipStream_t s = c->stream;
-    if (zero_init && (int)l >= c->dsa.fused_from) {  // the small levels in one launch
-        launch_mg_fused(c->dsa.lv_dev.as<MgLevel>(), (int)l, (int)lv.size(), s);
+    if ((int)l >= D.fused_from) {
+        launch_mg_fused(D.lv_dev.as<MgLevel>(), (int)l, (int)lv.size(), D.fused_lds, !zero_init, s);
         return;
     }
-    if (zero_init) launch_mg_zero(L.x, L.nx * L.ny, s);
-    if (l + 1 == lv.size()) {  // coarsest: a few cells, Gauss-Seidel to convergence
-        for (int k = 0; k < 16; ++k) {
-            launch_mg_rbgs(L, L.x, L.b, 0, s);
-            launch_mg_rbgs(L, L.x, L.b, 1, s);
-        }
-        return;
-    }
-    for (int k = 0; k < 2; ++k) {
-        launch_mg_rbgs(L, L.x, L.b, 0, s);
-        launch_mg_rbgs(L, L.x, L.b, 1, s);
-    }
-    launch_mg_residual(L, L.x, L.b, L.r, s);
-    launch_mg_restrict(L, lv[l + 1], L.r, lv[l + 1].b, s);
+    launch_mg_smooth(L, zero_init ? nullptr : L.y, L.x, nullptr, 0, s);  // pre-smoothing
+    launch_mg_restrict_residual(L, lv[l + 1], s);
     dsa_vcycle(c, l + 1, true);
-    launch_mg_prolong_add(L, lv[l + 1], lv[l + 1].x, L.x, s);
-    for (int k = 0; k < 2; ++k) {
-        launch_mg_rbgs(L, L.x, L.b, 1, s);
-        launch_mg_rbgs(L, L.x, L.b, 0, s);
-    }
+    launch_mg_smooth(L, L.x, L.y, &lv[l + 1], 1, s);  // prolongated correction, post-smoothing
 }
 
 // Builds the diffusion hierarchy for device cross sections (cell arrays) and time step dt, and captures
@@ -519,11 +511,19 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
         const size_t n = (size_t)h.nx * h.ny;
         L.x = static_cast<double *>(D.store[6 * l + 3].reserve(sizeof(double) * n));
         L.b = static_cast<double *>(D.store[6 * l + 4].reserve(sizeof(double) * n));
-        L.r = static_cast<double *>(D.store[6 * l + 5].reserve(sizeof(double) * n));
+        L.y = static_cast<double *>(D.store[6 * l + 5].reserve(sizeof(double) * n));
     }
-    // levels of at most 4096 cells run inside one workgroup
-    D.fused_from = (int)hl.size() - 1;
-    while (D.fused_from > 0 && (size_t)hl[D.fused_from - 1].nx * hl[D.fused_from - 1].ny <= 4096) --D.fused_from;
+    // the small levels (at most 4096 cells each, all of their arrays within one workgroup's LDS) run
+    // inside one workgroup; the coarsest level always does
+    const int nl = (int)hl.size();
+    require(nl <= kMgMaxLevels, TSW_E_UNSUPPORTED, "diffusion hierarchy too deep");
+    const size_t lds_cap = max_lds_per_block() - 4096;
+    D.fused_from = nl - 1;
+    while (D.fused_from > 0 && (size_t)hl[D.fused_from - 1].nx * hl[D.fused_from - 1].ny <= 4096 &&
+           mg_fused_lds_doubles(D.lv.data(), D.fused_from - 1, nl) * sizeof(double) <= lds_cap)
+        --D.fused_from;
+    D.fused_lds = mg_fused_lds_doubles(D.lv.data(), D.fused_from, nl) * sizeof(double);
+    require(D.fused_lds <= lds_cap, TSW_E_UNSUPPORTED, "coarsest diffusion level does not fit in LDS");
     TSW_HIP(hipMemcpyAsync(D.lv_dev.reserve(sizeof(MgLevel) * D.lv.size()), D.lv.data(), sizeof(MgLevel) * D.lv.size(),
                            hipMemcpyHostToDevice, c->stream));
     sync(c);  // host vectors go out of scope
@@ -532,15 +532,12 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
     D.key_ss = std::move(ss);
     D.key_idt = idt;
     D.key_cycles = c->dsa_cycles;
-    double *in = static_cast<double *>(D.in.reserve(sizeof(double) * g.nphi()));
-    double *out = static_cast<double *>(D.out.reserve(sizeof(double) * g.nphi()));
+    // the graph holds the V-cycles only (level-0 right-hand side in, level-0 solution out); the
+    // right-hand side and the correction are launched per apply on the caller's vectors
     hipGraph_t graph;
     TSW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     try {
-        TSW_HIP(hipMemcpyAsync(out, in, sizeof(double) * g.nphi(), hipMemcpyDeviceToDevice, c->stream));
-        launch_dsa_rhs(g, in, sigma_s, D.lv[0].b, c->stream);
         for (int k = 0; k < c->dsa_cycles; ++k) dsa_vcycle(c, 0, k == 0);
-        launch_dsa_add(g, D.lv[0].x, out, c->stream);
     } catch (...) {
         (void)hipStreamEndCapture(c->stream, &graph);
         throw;
@@ -554,10 +551,11 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
 // y = x + C^{-1} sigma_s x on phi-layout vectors (y may equal x)
 void dsa_apply(tsw_ctx *c, const double *x, double *y) {
     auto &D = c->dsa;
-    const size_t n = c->g.nphi();
-    TSW_HIP(hipMemcpyAsync(D.in.ptr, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    const Geom &g = c->g;
+    launch_dsa_rhs(g, x, D.sigma_s, D.lv[0].b, c->stream);
     TSW_HIP(hipGraphLaunch(D.graph, c->stream));
-    TSW_HIP(hipMemcpyAsync(y, D.out.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    if (y != x) TSW_HIP(hipMemcpyAsync(y, x, sizeof(double) * g.nphi(), hipMemcpyDeviceToDevice, c->stream));
+    launch_dsa_add(g, D.lv[0].y, y, c->stream);
 }
 
 struct Gmres {
@@ -565,38 +563,46 @@ struct Gmres {
     size_t n;
     int m;
     DBuf vbuf, hbuf, pbuf;
-    double *V;        // (m+1) x n
-    double *hdev;     // m+2 doubles
+    double *V;     // (m+1) x n, halos kept zero
+    double *hdev;  // [2][m+2]: the two Gram-Schmidt passes' coefficients; slot j+1 of each holds
+                   // ||w|| and ||w||^2 of step j
+    double *hh;    // pinned host copy of hdev
     double *part;
     int nb;
 
     Gmres(tsw_ctx *ctx, size_t n_, int m_) : c(ctx), n(n_), m(m_) {
         V = static_cast<double *>(vbuf.reserve(sizeof(double) * n * (m + 1)));
-        hdev = static_cast<double *>(hbuf.reserve(sizeof(double) * (m + 2)));
+        TSW_HIP(hipMemsetAsync(V, 0, sizeof(double) * n * (m + 1), c->stream));
+        hdev = static_cast<double *>(hbuf.reserve(sizeof(double) * 2 * (m + 2)));
         nb = reduction_blocks();
         part = static_cast<double *>(pbuf.reserve(sizeof(double) * nb * (m + 2)));
-    }
-    std::vector<double> dots(int k, const double *y) {  // <V_i, y>, i < k
-        std::vector<double> h(k);
-        launch_multi_dot(n, k, V, n, y, hdev, part, nb, c->stream);
-        TSW_HIP(hipMemcpyAsync(h.data(), hdev, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
-        sync(c);
-        return h;
+        const size_t need = 2 * (size_t)(m + 2);
+        if (c->gm_host_n < need) {
+            if (c->gm_host) TSW_HIP(hipHostFree(c->gm_host));
+            c->gm_host = nullptr;
+            c->gm_host_n = 0;
+            TSW_HIP(hipHostMalloc((void **)&c->gm_host, sizeof(double) * need, hipHostMallocDefault));
+            c->gm_host_n = need;
+        }
+        hh = c->gm_host;
     }
     double norm(const double *y) {
-        double h = 0.0;
         launch_multi_dot(n, 1, y, n, y, hdev, part, nb, c->stream);
-        TSW_HIP(hipMemcpyAsync(&h, hdev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        TSW_HIP(hipMemcpyAsync(hh, hdev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         sync(c);
-        return std::sqrt(h);
+        return std::sqrt(hh[0]);
     }
 
-    // Solves A x = b (x holds the initial guess, updated in place). apply(x_dev, y_dev) computes y = A x.
-    // Stops when the residual estimate <= abs_tol.  Returns true when converged.
+    // Solves A x = b (x holds the initial guess, updated in place). apply(x_dev, y_dev) computes y = A x
+    // and leaves y's halo entries as they are (zero).  Arnoldi with CGS2: the projections, the
+    // re-orthogonalisation, the norm and the normalisation stay on the device, and the Hessenberg column
+    // comes back in one copy, so a step costs one host round trip.  Stops when the residual estimate
+    // <= abs_tol.  Returns true when converged.
     template <typename Apply>
     bool solve(Apply &&apply, const double *b, double *x, double *r, double abs_tol, int max_its, int &its,
                double &res) {
         std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+        double *h1 = hdev, *h2 = hdev + (m + 2);
         its = 0;
         res = 0.0;
         while (true) {
@@ -615,25 +621,26 @@ struct Gmres {
                 double *vn = V + (size_t)(j + 1) * n;
                 apply(V + (size_t)j * n, vn);
                 ++its;
-                std::vector<double> h1 = dots(j + 1, vn);
-                TSW_HIP(hipMemcpyAsync(hdev, h1.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
-                launch_multi_axpy(n, j + 1, V, n, hdev, vn, c->stream);
-                std::vector<double> h2 = dots(j + 1, vn);
-                TSW_HIP(hipMemcpyAsync(hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
-                launch_multi_axpy(n, j + 1, V, n, hdev, vn, c->stream);
-                for (int i = 0; i <= j; ++i) H[i + (m + 1) * j] = h1[i] + h2[i];
-                const double hn = norm(vn);
-                H[j + 1 + (m + 1) * j] = hn;
-                if (hn > 0.0) launch_axpby(n, 1.0 / hn, vn, 0.0, vn, c->stream);
+                launch_multi_dot(n, j + 1, V, n, vn, h1, part, nb, c->stream);
+                launch_multi_axpy(n, j + 1, V, n, h1, vn, c->stream);
+                launch_multi_dot(n, j + 1, V, n, vn, h2, part, nb, c->stream);
+                launch_multi_axpy(n, j + 1, V, n, h2, vn, c->stream);
+                launch_multi_dot(n, 1, vn, n, vn, h2 + j + 1, part, nb, c->stream);
+                launch_normalize(n, h2 + j + 1, vn, h1 + j + 1, c->stream);
+                TSW_HIP(hipMemcpyAsync(hh, hdev, sizeof(double) * 2 * (m + 2), hipMemcpyDeviceToHost, c->stream));
+                sync(c);
+                const double *hh1 = hh, *hh2 = hh + (m + 2);
+                for (int i = 0; i <= j; ++i) H[i + (m + 1) * j] = hh1[i] + hh2[i];
+                H[j + 1 + (m + 1) * j] = hh1[j + 1];
                 for (int i = 0; i < j; ++i) {
-                    const double h0 = H[i + (m + 1) * j], hh = H[i + 1 + (m + 1) * j];
-                    H[i + (m + 1) * j] = cs[i] * h0 + sn[i] * hh;
-                    H[i + 1 + (m + 1) * j] = -sn[i] * h0 + cs[i] * hh;
+                    const double h0 = H[i + (m + 1) * j], hx = H[i + 1 + (m + 1) * j];
+                    H[i + (m + 1) * j] = cs[i] * h0 + sn[i] * hx;
+                    H[i + 1 + (m + 1) * j] = -sn[i] * h0 + cs[i] * hx;
                 }
-                const double h0 = H[j + (m + 1) * j], hh = H[j + 1 + (m + 1) * j];
-                const double den = std::sqrt(h0 * h0 + hh * hh);
+                const double h0 = H[j + (m + 1) * j], hx = H[j + 1 + (m + 1) * j];
+                const double den = std::sqrt(h0 * h0 + hx * hx);
                 cs[j] = h0 / den;
-                sn[j] = hh / den;
+                sn[j] = hx / den;
                 H[j + (m + 1) * j] = den;
                 H[j + 1 + (m + 1) * j] = 0.0;
                 g[j + 1] = -sn[j] * g[j];
@@ -673,9 +680,10 @@ T *dbuf(DBuf &b, size_t n) {
 void diag_load_source(tsw_ctx *c, const double *qref, bool need64, bool need32);
 
 // Builds the per-call inputs: materials or per-cell sigma_t, sigma_s, inflow traces and the source
-// from reference-layout arrays (sigma_s, qref may be null).  need64/need32 select the precisions.
+// from reference-layout arrays (sigma_s, qref may be null).  need64/need32 select the precisions; src32
+// false leaves out the FP32 source and inflow traces (callers whose FP32 sweeps only see residuals).
 void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const double *qref, double dt, bool need64,
-                bool need32) {
+                bool need32, bool src32 = true) {
     const Geom &g = c->g;
     auto &d = c->diag;
     const size_t ncell = (size_t)g.nx * g.ny;
@@ -689,7 +697,7 @@ void diag_setup(tsw_ctx *c, const double *sigma_t, const double *sigma_s, const 
         if (need64) launch_cell_to_diag<double, double>(g, sigma_s, dbuf<double>(d.sigs64, 4 * ncell), 1, s);
         if (need32) launch_cell_to_diag<double, float>(g, sigma_s, dbuf<float>(d.sigs32, 4 * ncell), 1, s);
     }
-    diag_load_source(c, qref, need64, need32);
+    diag_load_source(c, qref, need64, need32 && src32);
 }
 
 // Inflow traces and the source in the diagonal layout, from a reference-layout psi (null: no source).
@@ -893,12 +901,12 @@ T *diag_phi(tsw_ctx *c, const double *phi) {
     return out;
 }
 
-// phi = beta phi + alpha P psi_d
+// phi = beta xb + alpha P psi_d on the interior (xb null: xb = phi)
 template <typename T>
-void diag_pmv(tsw_ctx *c, const T *psi_d, double *phi, double alpha, double beta) {
+void diag_pmv(tsw_ctx *c, const T *psi_d, double *phi, double alpha, double beta, const double *xb = nullptr) {
     const Geom &g = c->g;
     double *phiQ = dbuf<double>(c->diag.phiQ, 4 * (size_t)g.nx * g.ny * g.N());
-    launch_pmv_diag<T>(g, psi_d, c->w64.as<double>(), phiQ, phi, alpha, beta, c->stream, c->skip_flag);
+    launch_pmv_diag<T>(g, psi_d, c->w64.as<double>(), phiQ, phi, alpha, beta, c->stream, c->skip_flag, xb);
 }
 
 // out (reference layout, interior) <- psi_d; halos of out untouched
@@ -1091,8 +1099,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         auto apply = [&](const double *xv, double *yv) {  // (I - P L^{-1} S) x, zero inflow (:548-555)
             const double *phi_d = diag_phi<double>(c, xv);
             diag_sweep<double>(c, dt, nullptr, phi_d, false, psi_d);
-            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            diag_pmv<double>(c, psi_d, yv, -1.0, 1.0);
+            diag_pmv<double>(c, psi_d, yv, -1.0, 1.0, xv);  // y = x - P psi (y's halos stay zero)
             if (dsa) dsa_apply(c, yv, yv);
         };
         diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);  // b = P L^{-1} Q (:461-466)
@@ -1103,7 +1110,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_sweep<double>(c, dt, q_d, phi_d, true, psi_d);
         diag_store<double>(c, psi_d, Q);
     } else {
-        diag_setup(c, st, ss, Q, dt, true, true);
+        diag_setup(c, st, ss, Q, dt, true, true, false);  // FP32 sweeps of residuals: no FP32 source or inflow
         double *psi64 = dbuf<double>(c->diag.ir64, npsi_d);
         float *res32 = dbuf<float>(c->diag.res32, npsi_d);
         float *psi32 = dbuf<float>(c->diag.psi32, npsi_d);
@@ -1112,8 +1119,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         auto apply32 = [&](const double *xv, double *yv) {
             const float *phi_d = diag_phi<float>(c, xv);
             diag_sweep<float>(c, dt, nullptr, phi_d, false, psi32);
-            TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            diag_pmv<float>(c, psi32, yv, -1.0, 1.0);
+            diag_pmv<float>(c, psi32, yv, -1.0, 1.0, xv);
             if (dsa) dsa_apply(c, yv, yv);
         };
         const int max_refine = 30;

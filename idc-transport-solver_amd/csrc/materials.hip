@@ -4,7 +4,8 @@
 // one of M values per ordinate (SweepLinearSolveSpec.cpp:284-298), M = number of distinct sigma_t
 // cell values.  When M is small, the sweep multiplies by a precomputed A^{-1} instead of eliminating
 // per cell.  The inverses are formed here in FP64 by the same elimination applied to the identity
-// columns and stored row-major as [M][nq][N][N] in the sweep's precision.
+// columns, scaled by dx dy (the sweep then applies them to B / (dx dy), whose source term is the
+// unscaled source), and stored row-major as [M][nq][N][N] in the sweep's precision, rounded once.
 #include "cell_solve.hpp"
 #include "tsw_internal.hpp"
 
@@ -27,7 +28,7 @@ __global__ __launch_bounds__(64) void k_material_inverses(const double *__restri
         for (int k = 0; k < N; ++k) e[k] = (k == c) ? 1.0 : 0.0;
         ge_solve<double, N>(A, e);
 #pragma unroll
-        for (int r = 0; r < N; ++r) dst[r * N + c] = T(e[r]);
+        for (int r = 0; r < N; ++r) dst[r * N + c] = T(e[r] * (dx * dy));
     }
 }
 

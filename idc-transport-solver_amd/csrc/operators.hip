@@ -315,6 +315,18 @@ void launch_multi_axpy(size_t n, int m, const double *X, size_t ld, const double
     TSW_HIP(hipGetLastError());
 }
 
+__global__ __launch_bounds__(kBlock) void k_normalize(size_t n, const double *__restrict__ sq, double *__restrict__ v,
+                                                      double *__restrict__ hn) {
+    const double h = sqrt(*sq);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *hn = h;
+    if (t < n && h > 0.0) v[t] = (1.0 / h) * v[t];
+}
+void launch_normalize(size_t n, const double *sq, double *v, double *hn, hipStream_t s) {
+    hipLaunchKernelGGL(k_normalize, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sq, v, hn);
+    TSW_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(kBlock) void k_sqdist_partial(const double *__restrict__ a, const double *__restrict__ b,
                                                            int nx, int ny, int N, double *__restrict__ partials) {
     const size_t n = (size_t)nx * ny * N;

@@ -18,14 +18,16 @@ template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     const int BR = band_rows_of(sp);
     if (BR > 448) return 0;
+    // the kernel addresses an ordinate's coefficient planes with 32-bit byte offsets
+    if ((long long)sp.nx * sp.ny * (p + 1) * (p + 1) * (long long)sizeof(T) >= (1ll << 32)) return 0;
     const int cthreads = ((BR + 63) / 64) * 64;
     const int M = sp.ainv ? sp.M : 0;
     size_t total;
     switch (p) {
-        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, BR, M).total; break;
-        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, BR, M).total; break;
-        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, BR, M).total; break;
-        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, BR, M).total; break;
+        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, M).total; break;
+        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, M).total; break;
+        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, M).total; break;
+        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, M).total; break;
         default: return 0;
     }
     const size_t bytes = total * sizeof(T);
@@ -258,7 +260,8 @@ __global__ __launch_bounds__(256) void k_pmv_quadrant(const TI *__restrict__ psi
     phiQ[(size_t)Q * N * ncell + t] = acc;
 }
 
-__global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *__restrict__ phi, DiagMap dm, int N,
+// phi = beta xb + alpha sum_Q phiQ on the interior (xb null: xb = phi); halos of phi are not written
+__global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *phi, const double *xb, DiagMap dm, int N,
                                      double alpha, double beta, const int *skip) {
     if (skip && *skip) return;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,25 +273,25 @@ __global__ void k_phi_from_quadrants(const double *__restrict__ phiQ, double *__
     double acc = 0.0;
     for (int Q = 0; Q < 4; ++Q)
         acc += phiQ[(size_t)Q * N * ncell + (size_t)k * ncell + (size_t)dm.pos_of_cell(i, j, quad_xneg(Q), quad_yneg(Q))];
-    double *y = phi + ((size_t)i * (dm.ny + 2) + j) * N + k;
-    *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * acc;
+    const size_t at = ((size_t)i * (dm.ny + 2) + j) * N + k;
+    phi[at] = (beta == 0.0 ? 0.0 : beta * (xb ? xb[at] : phi[at])) + alpha * acc;
 }
 
 template <typename TI>
 void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha, double beta,
-                     hipStream_t s, const int *skip) {
+                     hipStream_t s, const int *skip, const double *xb) {
     const DiagMap dm{g.nx, g.ny};
     const size_t n = (size_t)dm.cells() * g.N();
     hipLaunchKernelGGL((k_pmv_quadrant<TI>), dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, s, psi, w, phiQ,
                        (size_t)dm.cells(), g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], g.nq, skip);
-    hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, dm, g.N(),
-                       alpha, beta, skip);
+    hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, xb, dm,
+                       g.N(), alpha, beta, skip);
     TSW_HIP(hipGetLastError());
 }
 template void launch_pmv_diag<double>(const Geom &, const double *, const double *, double *, double *, double, double,
-                                      hipStream_t, const int *);
+                                      hipStream_t, const int *, const double *);
 template void launch_pmv_diag<float>(const Geom &, const float *, const double *, double *, double *, double, double,
-                                     hipStream_t, const int *);
+                                     hipStream_t, const int *, const double *);
 
 // =====================================================================================================
 // FP64 angular residual on the diagonal layout, the refinement's r = Q + sigma_s phi/(4 pi) - L psi

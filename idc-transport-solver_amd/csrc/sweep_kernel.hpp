@@ -25,16 +25,78 @@ namespace tsw {
 
 constexpr double kInv4Pi = 0.079577471545947667884;
 
+// 16-byte LDS vectors.  Every LDS row below starts on a 16-byte boundary and is padded to whole vectors,
+// so a row moves with ds_read_b128 / ds_write_b128 instead of one access per value.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double V __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
+};
+template <typename T>
+__host__ __device__ constexpr int vround(int n) {
+    return (n + Vec16<T>::W - 1) / Vec16<T>::W * Vec16<T>::W;
+}
+
+// v[0..LEN) <- p[0..LEN), p 16-byte aligned with vround(LEN) readable entries
+template <typename T, int LEN>
+__device__ __forceinline__ void lds_get(const T *p, T (&v)[LEN]) {
+    using V = typename Vec16<T>::V;
+    constexpr int W = Vec16<T>::W;
+    const V *pv = reinterpret_cast<const V *>(p);
+#pragma unroll
+    for (int i = 0; i < (LEN + W - 1) / W; ++i) {
+        const V x = pv[i];
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            if (i * W + j < LEN) v[i * W + j] = x[j];
+    }
+}
+// p[0..LEN) <- v: whole 16-byte vectors, the remainder with one narrower access per element pair or
+// element (entries past LEN are left as they are)
+template <typename T, int LEN>
+__device__ __forceinline__ void lds_put(T *p, const T (&v)[LEN]) {
+    using V = typename Vec16<T>::V;
+    constexpr int W = Vec16<T>::W;
+    V *pv = reinterpret_cast<V *>(p);
+#pragma unroll
+    for (int i = 0; i < LEN / W; ++i) {
+        V x;
+#pragma unroll
+        for (int j = 0; j < W; ++j) x[j] = v[i * W + j];
+        pv[i] = x;
+    }
+    int k = LEN / W * W;
+    if constexpr (sizeof(T) == 4) {
+        typedef float V2 __attribute__((ext_vector_type(2)));
+        if (LEN - k >= 2) {
+            V2 y;
+            y[0] = v[k];
+            y[1] = v[k + 1];
+            *reinterpret_cast<V2 *>(p + k) = y;
+            k += 2;
+        }
+    }
+    if (k < LEN) p[k] = v[k];
+}
+
 template <typename T, int P>
-struct DiagLds {  // LDS carve-up in units of T
+struct DiagLds {  // LDS carve-up in units of T; every array and row is 16-byte aligned
     static constexpr int N = P * P;
+    static constexpr int PS = vround<T>(P), NS = vround<T>(N), MS = vround<T>(N * N);
     size_t ex, in, out, inv, total;
-    __host__ __device__ DiagLds(int cthreads, int nx, int ny, int M) {
-        ex = 0;                              // [2][cthreads][P] y-traces between rows
-        in = ex + (size_t)2 * cthreads * P;  // [nx][P] y-inflow traces of row 0
-        out = in + (size_t)nx * P;           // [2][ny][N] solved cells awaiting the store
-        inv = out + (size_t)2 * ny * N;      // [M][N][N] element inverses (material path)
-        total = inv + (size_t)M * N * N;
+    __host__ __device__ DiagLds(int cthreads, int nx, int M) {
+        ex = 0;                                 // [2][cthreads][PS] y-traces between rows
+        in = ex + (size_t)2 * cthreads * PS;    // [nx][PS] y-inflow traces of row 0
+        out = in + (size_t)nx * PS;             // [2][cthreads][NS] solved cells awaiting the store
+        inv = out + (size_t)2 * cthreads * NS;  // [M][MS] element inverses (material path)
+        total = inv + (size_t)M * MS;
     }
 };
 
@@ -57,6 +119,28 @@ using gptr = __attribute__((address_space(1))) T *;
 template <typename T>
 __device__ __forceinline__ gptr<T> G(T *p) {
     return (gptr<T>)p;
+}
+// Raw buffer access: a resource over one array, a per-lane 32-bit byte offset and a wave-uniform
+// 32-bit byte offset (the coefficient plane), so a cell's N coefficients share one address register.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned long long bytes) {
+    const int n = bytes >= 0xFFFFFFFFull ? -1 : (int)(unsigned)bytes;  // num_records in bytes
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, n, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T v) {
+    if constexpr (sizeof(T) == 8) {
+        typedef unsigned V2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, (int)voff, (int)soff, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+    }
 }
 
 template <typename U>
@@ -96,6 +180,18 @@ __device__ __noinline__ Record<typename ExchangeSentinel<T>::U, P> wait_record(
     }
 }
 
+// Position bookkeeping along the diagonals of a band: with off(s) = base(s) - rlo(s), cell (s, r) sits at
+// off(s) + r, and off(s+1) = off(s) + min(s, ny-1) + 1 - rlo(s+1) (diagonal s holds min(s, ny-1) - rlo(s) + 1
+// cells).  Wave-uniform scalar arithmetic, advanced one diagonal per step.
+struct DiagWalk {
+    int s, off;
+    __device__ void next(int nx, int ny) {
+        const int rlo1 = s + 2 - nx > 0 ? s + 2 - nx : 0;
+        off += (s < ny - 1 ? s : ny - 1) + 1 - rlo1;
+        ++s;
+    }
+};
+
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
 // Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
 // upstream neighbour is dispatched before it and never waits on it).
@@ -103,6 +199,8 @@ template <typename T, int P, bool MAT, int MODE, int D>
 __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
+    using Lds = DiagLds<T, P>;
+    constexpr int PS = Lds::PS, NS = Lds::NS, MS = Lds::MS;
     using U = typename ExchangeSentinel<T>::U;
     // one problem (sp1, workgroup = band x ordinate) or a batch (items[workgroup] = {problem, ordinate})
     int item_p = 0, item_q = 0;
@@ -117,14 +215,16 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     const int q = batch ? item_q : (int)(blockIdx.x % (unsigned)sp.nq);
     const int band = batch ? 0 : (int)(blockIdx.x / (unsigned)sp.nq);
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int nx = sp.nx, ny = sp.ny;
     const int BR = nb > 1 ? sp.band_rows : ny;  // rows per band
     const int R0 = band * BR;                    // first row of this band
     const int nrows = BR < ny - R0 ? BR : ny - R0;
-    const int cthreads = ((BR + 63) / 64) * 64;  // waves beyond this row count idle at the barriers
-    const DiagLds<T, P> L(cthreads, nx, BR, MAT ? sp.M : 0);
+    // every compute thread owns one LDS row slot; threads past the band's rows compute on clamped
+    // inputs and their results are never read
+    const int cthreads = (int)blockDim.x - 64;
+    const Lds L(cthreads, nx, MAT ? sp.M : 0);
     T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
     const DiagMap dm{nx, ny};
     const size_t ncell = (size_t)dm.cells();
@@ -137,101 +237,112 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     const T xi = G(sp.xi)[q], eta = G(sp.eta)[q];
     const bool xneg = xi <= T(0), yneg = eta <= T(0);
 
-    if (MAT) {
+    if (MAT) {  // the table holds dx dy A^{-1}: the material path solves with the source unscaled (see B below)
         for (int k = tid; k < sp.M * N * N; k += blockDim.x) {
             const int m = k / (N * N), e = k - m * N * N;
-            linv[k] = G(sp.ainv)[((size_t)m * sp.nq + q) * N * N + e];
+            linv[(size_t)m * MS + e] = G(sp.ainv)[((size_t)m * sp.nq + q) * N * N + e];
         }
     }
     if (band == 0)
-        for (int k = tid; k < nx * P; k += blockDim.x) lin[k] = sp.tin_y ? G(sp.tin_y)[(size_t)q * nx * P + k] : T(0);
+        for (int k = tid; k < nx * P; k += blockDim.x) {
+            const int c = k / P, d = k - c * P;
+            lin[(size_t)c * PS + d] = sp.tin_y ? G(sp.tin_y)[(size_t)q * nx * P + k] : T(0);
+        }
 
     const int steps = nx + nrows - 1;              // local step t is global diagonal R0 + t
     const int nsteps = ((steps + D - 1) / D) * D;  // padded steps have no active cell
-    const gptr<T> psi_q = G(sp.psi) + (size_t)q * N * ncell;
-    // rows of this band on diagonal s
-    auto band_lo = [&](int s) { const int a = dm.rlo(s); return a > R0 ? a : R0; };
-    auto band_hi = [&](int s) {
-        const int a = s < ny - 1 ? s : ny - 1, b = R0 + nrows - 1;
-        return a < b ? a : b;
-    };
+    const unsigned plane = (unsigned)(ncell * sizeof(T));  // bytes of one coefficient plane (host-checked)
+    const DiagWalk walk0{R0, (int)(dm.base(R0) - dm.rlo(R0))};
 
-    if (tid >= (int)blockDim.x - 64) {
+    if (tid >= cthreads) {
         // storer: barrier k (k >= 1) closes compute step k-1, whose cells are written here while
         // the compute waves fill the other ring slot; it also hands the band's top trace on and empties
         // the records this band has consumed
-        const int lane = tid - ((int)blockDim.x - 64);
+        const int lane = tid - cthreads;
+        const __amdgpu_buffer_rsrc_t rpsi = buf_rsrc(sp.psi + (size_t)q * N * ncell, (unsigned long long)N * plane);
+        DiagWalk w = walk0;
         lds_barrier();
         for (int t = 0; t < nsteps; ++t) {
             lds_barrier();
             if (t >= steps) continue;
-            const int s = R0 + t;
             if (xout) {
                 const int c = t - (nrows - 1);
                 if (c >= 0 && c < nx && lane < P)
-                    xstore(xout + (size_t)c * P + lane, to_bits(ex[((size_t)(t & 1) * cthreads + (nrows - 1)) * P + lane]));
+                    xstore(xout + (size_t)c * P + lane, to_bits(ex[((size_t)(t & 1) * cthreads + (nrows - 1)) * PS + lane]));
             }
             if (xin && t < nx && lane < P) xstore(xin + (size_t)t * P + lane, ExchangeSentinel<T>::bits);
-            const T *slot = lout + (size_t)(t & 1) * BR * N;
-            const int lo = band_lo(s), hi = band_hi(s);
-            const long long p0 = dm.base(s) - dm.rlo(s);
+            const T *slot = lout + (size_t)(t & 1) * cthreads * NS;
+            const int s = w.s;
+            const int lo = s - nx + 1 > R0 ? s - nx + 1 : R0;
+            const int hi = s < R0 + nrows - 1 ? s : R0 + nrows - 1;
             for (int r = lo + lane; r <= hi; r += 64) {
-                const T *src = slot + (size_t)(r - R0) * N;
+                T v[N];
+                lds_get<T, N>(slot + (size_t)(r - R0) * NS, v);
+                const unsigned boff = (unsigned)(w.off + r) * (unsigned)sizeof(T);
 #pragma unroll
-                for (int k = 0; k < N; ++k) psi_q[(size_t)k * ncell + (size_t)(p0 + r)] = src[k];
+                for (int k = 0; k < N; ++k) buf_st<T>(rpsi, boff, k * plane, v[k]);
             }
+            w.next(nx, ny);
         }
         return;
     }
 
-    const int rl = tid < cthreads ? tid : cthreads - 1;  // spare waves mirror the last row, stay inactive
-    const int r = R0 + rl;                               // mesh row (counted from the inflow edge)
-    const bool row_ok = tid < nrows;
+    const int rl = tid;     // row slot
+    const int r = R0 + rl;  // mesh row (counted from the inflow edge)
+    const bool row_ok = rl < nrows;
     T tx[P];
 #pragma unroll
     for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? G(sp.tin_x)[((size_t)q * ny + r) * P + d] : T(0);
 
-    // position of this row's cell at local step t; lanes whose row has no cell on that diagonal read
-    // the nearest cell of the band on it, so a wavefront's loads stay within one contiguous run
-    // (diagonal bounds are wave-uniform scalar arithmetic)
-    auto pos_at = [&](int t) -> size_t {
-        if (t >= steps) return 0;
-        const int s = R0 + t;
-        const int lo = band_lo(s), hi = band_hi(s);
-        const int rc = r < lo ? lo : (r > hi ? hi : r);
-        return (size_t)(dm.base(s) + rc - dm.rlo(s));
-    };
-    const gptr<const T> qsrc_q = HASQ ? G(sp.qsrc) + (size_t)q * N * ncell : nullptr;
-    const gptr<const T> phi_Q = HASPHI ? G(sp.phi) + (size_t)Q * N * ncell : nullptr;
-    const gptr<const T> sigs_Q = HASPHI ? G(sp.sig_s) + (size_t)Q * ncell : nullptr;
-    const gptr<const int> mat_Q = MAT ? G(sp.mat) + (size_t)Q * ncell : nullptr;
-    const gptr<const T> sigt_Q = MAT ? nullptr : G(sp.sig_t) + (size_t)Q * ncell;
-    const bool xrow = xin != nullptr && rl == 0;  // this lane takes its y-inflow from the band below
+    const unsigned long long nplanes = (unsigned long long)N * plane;
+    const __amdgpu_buffer_rsrc_t rq = buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr, HASPHI ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rsigs = buf_rsrc(HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr, HASPHI ? plane : 0);
+    const __amdgpu_buffer_rsrc_t rmat = buf_rsrc(MAT ? sp.mat + (size_t)Q * ncell : nullptr, MAT ? ncell * 4 : 0);
+    const __amdgpu_buffer_rsrc_t rsigt = buf_rsrc(MAT ? nullptr : sp.sig_t + (size_t)Q * ncell, MAT ? 0 : plane);
+    // the wave holding the band's first row takes that row's y-inflow from the band below (xwave is
+    // wave-uniform, so the exchange code is a scalar branch the other waves skip)
+    const int w0 = __builtin_amdgcn_readfirstlane(tid & ~63);  // first row slot of this wave
+    // the wave's rows w0 .. w0 + live_n - nx have cells on the diagonals of local steps [w0, w0 + live_n)
+    const int live_n = w0 < nrows ? (nrows - 1 - w0 < 63 ? nrows - 1 - w0 : 63) + nx : 0;
+    const bool xwave = xin != nullptr && w0 == 0;
+    const bool xrow = xwave && rl == 0;
 
-    // prefetch ring: slot u holds the inputs this row meets at a step t with t % D == u
+    // prefetch ring: slot u holds the inputs this row meets at a step t with t % D == u.  Lanes whose row
+    // has no cell on the fetched diagonal read the nearest cell of the band on it (so a wavefront's loads
+    // stay within one contiguous run); padded steps read position 0.
     T pq[D][HASQ ? N : 1], pf[D][HASPHI ? N : 1], ps[D], pm_sig[D];
     int pm[D];
     Record<U, P> px[D];
-    auto fetch = [&](int u, int t) {
-        const size_t p = pos_at(t);
+    DiagWalk fw = walk0;  // diagonal of the next fetch
+    int ft = 0;           // its local step
+    auto fetch = [&](int u) {
+        const bool live = ft < steps;
+        const int lo = live ? (fw.s - nx + 1 > R0 ? fw.s - nx + 1 : R0) : 0;
+        const int hi = live ? (fw.s < R0 + nrows - 1 ? fw.s : R0 + nrows - 1) : 0;
+        const int rc = r < lo ? lo : (r > hi ? hi : r);
+        const unsigned p = (unsigned)((live ? fw.off : 0) + rc);
+        const unsigned bt = p * (unsigned)sizeof(T), bi = p * 4u;
         if (HASQ) {
 #pragma unroll
-            for (int k = 0; k < N; ++k) pq[u][k] = qsrc_q[(size_t)k * ncell + p];
+            for (int k = 0; k < N; ++k) pq[u][k] = buf_ld<T>(rq, bt, k * plane);
         }
         if (HASPHI) {
 #pragma unroll
-            for (int k = 0; k < N; ++k) pf[u][k] = phi_Q[(size_t)k * ncell + p];
-            ps[u] = sigs_Q[p];
+            for (int k = 0; k < N; ++k) pf[u][k] = buf_ld<T>(rphi, bt, k * plane);
+            ps[u] = buf_ld<T>(rsigs, bt, 0);
         }
-        if (MAT) pm[u] = mat_Q[p];
-        else pm_sig[u] = sigt_Q[p];
-        if (xrow && t < nx) {
+        if (MAT) pm[u] = buf_ld<int>(rmat, bi, 0);
+        else pm_sig[u] = buf_ld<T>(rsigt, bt, 0);
+        if (xwave && ft < nx) {  // every lane of the wave reads the record; row 0's lane uses it
 #pragma unroll
-            for (int d = 0; d < P; ++d) px[u].v[d] = xload<U>(xin + (size_t)t * P + d);
+            for (int d = 0; d < P; ++d) px[u].v[d] = xload<U>(xin + (size_t)ft * P + d);
         }
+        fw.next(nx, ny);
+        ++ft;
     };
 #pragma unroll
-    for (int u = 0; u < D; ++u) fetch(u, u);
+    for (int u = 0; u < D; ++u) fetch(u);
     lds_barrier();
 
     // FP32 keeps the current material's inverse in registers and reloads it from LDS only when the
@@ -240,82 +351,116 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     T Ainv[REGMAT ? N * N : 1];
     int cur_mat = -1;
     const T area = sp.dx * sp.dy;
+    // B(d,e) = area src(d,e) + kx(d) Tx(e) + ky(e) Ty(d), the face-trace form of the reference's CalcB
+    // (cell_solve.hpp: rhs_from_traces); kx, ky are wave-uniform.  The material path solves
+    // (dx dy A^{-1}) (B / (dx dy)), so its source term needs no scaling.
+    const T bsc = MAT ? T(1) : area;
+    T kxd[P], kye[P];
+#pragma unroll
+    for (int d = 0; d < P; ++d) {
+        const T kd = xneg ? T(-(2 * d + 1)) : T(ipow_m1(d) * (2 * d + 1));
+        const T ke = yneg ? T(-(2 * d + 1)) : T(ipow_m1(d) * (2 * d + 1));
+        if (MAT) {  // (dy xi kx) / (dx dy) = xi kx / dx, rounded once
+            kxd[d] = T((double)xi * (double)kd / (double)sp.dx);
+            kye[d] = T((double)eta * (double)ke / (double)sp.dy);
+        } else {
+            kxd[d] = (sp.dy * xi) * kd;
+            kye[d] = (sp.dx * eta) * ke;
+        }
+    }
+    // P_a(-1) = (-1)^a on an outflow face at the cell's low side; kept opaque so the traces below are
+    // one multiply-add per term rather than a sign select and an add
+    T sx = xneg ? T(-1) : T(1), sy = yneg ? T(-1) : T(1);
+    asm volatile("" : "+v"(sx), "+v"(sy));
     for (int t0 = 0; t0 < nsteps; t0 += D) {
 #pragma unroll
         for (int u = 0; u < D; ++u) {
             const int t = t0 + u;
             const int c = t - rl;
-            const bool active = row_ok && c >= 0 && c < nx;
-            // consume slot u, then refill it with step t + D (kept apart by scheduling barriers so
-            // the slot keeps its registers and its wait covers only its own loads)
+            const bool active = row_ok && (unsigned)c < (unsigned)nx;
+            // a wave none of whose rows has a cell on this diagonal skips the step's arithmetic (its
+            // results would never be read), so the other waves of the CU get its issue slots
+            const bool live = (unsigned)(t - w0) < (unsigned)live_n;
             T B[N];
-            {
-                T src[N];
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    T v = T(0);
-                    if (HASQ) v = pq[u][k];
-                    if (HASPHI) v += ps[u] * pf[u][k] * T(kInv4Pi);
-                    src[k] = v;
-                }
+            if (live) {
                 T ty[P];
-                if (xrow) {
-                    if (active) {
+                const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
+                const T *tsrc = rl == 0 ? lin + (size_t)cc * PS : ex + ((size_t)((t - 1) & 1) * cthreads + (rl - 1)) * PS;
+                lds_get<T, P>(tsrc, ty);
+                if (xwave) {
+                    if (xrow && active) {
                         bool ok = true;
 #pragma unroll
                         for (int d = 0; d < P; ++d) ok = ok && px[u].v[d] != ExchangeSentinel<T>::bits;
                         if (!ok) px[u] = wait_record<T, P>(xin + (size_t)c * P, px[u], G(sp.abort_flag));
                     }
 #pragma unroll
-                    for (int d = 0; d < P; ++d) ty[d] = active ? from_bits<T>(px[u].v[d]) : T(0);
-                } else {
-                    const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
-                    const T *tsrc = rl == 0 ? lin + (size_t)cc * P : ex + ((size_t)((t - 1) & 1) * cthreads + (rl - 1)) * P;
-#pragma unroll
-                    for (int d = 0; d < P; ++d) ty[d] = tsrc[d];
+                    for (int d = 0; d < P; ++d) ty[d] = xrow ? (active ? from_bits<T>(px[u].v[d]) : T(0)) : ty[d];
                 }
-                rhs_from_traces<T, P>(B, src, tx, ty, xi, eta, sp.dx, sp.dy);
+                T a[N];  // bsc * source
+                if (HASPHI) {
+                    const T wsc = bsc * ps[u] * T(kInv4Pi);
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+                        a[k] = HASQ ? (MAT ? pq[u][k] : bsc * pq[u][k]) + wsc * pf[u][k] : wsc * pf[u][k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; ++k) a[k] = MAT ? pq[u][k] : bsc * pq[u][k];
+                }
+#pragma unroll
+                for (int d = 0; d < P; ++d)
+#pragma unroll
+                    for (int e = 0; e < P; ++e) B[e + P * d] = a[e + P * d] + kxd[d] * tx[e] + kye[e] * ty[d];
             }
+            if (live) {
             const int mat = pm[u];
             const T sig = pm_sig[u];
-            __builtin_amdgcn_sched_barrier(0);
-            fetch(u, t + D);
-            __builtin_amdgcn_sched_barrier(0);
-            if (active) {
-                T x[N];
-                if (REGMAT) {
-                    if (mat != cur_mat) {
-                        cur_mat = mat;
-                        const T *Mx = linv + (size_t)mat * N * N;
-#pragma unroll
-                        for (int k = 0; k < N * N; ++k) Ainv[k] = Mx[k];
-                    }
-#pragma unroll
-                    for (int row = 0; row < N; ++row) {
-                        T acc = T(0);
-#pragma unroll
-                        for (int col = 0; col < N; ++col) acc += Ainv[row * N + col] * B[col];
-                        x[row] = acc;
-                    }
-                } else if (MAT) {
-                    matvec_lds<T, N>(linv + (size_t)mat * N * N, B, x);
-                } else {
-                    T A[N * N];
-                    assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, (sig + sp.inv_dt) * area);
-                    ge_solve_rcp<T, N>(A, B);
-#pragma unroll
-                    for (int k = 0; k < N; ++k) x[k] = B[k];
+            T x[N];
+            if constexpr (REGMAT) {
+                if (mat != cur_mat) {
+                    cur_mat = mat;
+                    lds_get<T, N * N>(linv + (size_t)mat * MS, Ainv);
                 }
-                T *o = lout + ((size_t)(t & 1) * BR + rl) * N;
 #pragma unroll
-                for (int k = 0; k < N; ++k) o[k] = x[k];
-                trace_x<T, P>(x, xneg, tx);
-                T tyo[P];
-                trace_y<T, P>(x, yneg, tyo);
-                T *dst_l = ex + ((size_t)(t & 1) * cthreads + rl) * P;
+                for (int row = 0; row < N; ++row) {
+                    T acc = Ainv[row * N] * B[0];
 #pragma unroll
-                for (int d = 0; d < P; ++d) dst_l[d] = tyo[d];
+                    for (int col = 1; col < N; ++col) acc += Ainv[row * N + col] * B[col];
+                    x[row] = acc;
+                }
+            } else if constexpr (MAT) {
+                matvec_lds<T, N>(linv + (size_t)mat * MS, B, x);
+            } else {
+                T A[N * N];
+                assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, (sig + sp.inv_dt) * area);
+                ge_solve_rcp<T, N>(A, B);
+#pragma unroll
+                for (int k = 0; k < N; ++k) x[k] = B[k];
             }
+            lds_put<T, N>(lout + ((size_t)(t & 1) * cthreads + rl) * NS, x);
+            T txo[P], tyo[P];  // outflow face traces: sum_a s^a psi(a, .), s = +-1
+#pragma unroll
+            for (int e = 0; e < P; ++e) {
+                T v = x[e];
+#pragma unroll
+                for (int a = 1; a < P; ++a) v = (a & 1) ? v + sx * x[e + P * a] : v + x[e + P * a];
+                txo[e] = v;
+            }
+#pragma unroll
+            for (int d = 0; d < P; ++d) {
+                T v = x[P * d];
+#pragma unroll
+                for (int b = 1; b < P; ++b) v = (b & 1) ? v + sy * x[b + P * d] : v + x[b + P * d];
+                tyo[d] = v;
+            }
+#pragma unroll
+            for (int d = 0; d < P; ++d) tx[d] = active ? txo[d] : tx[d];
+            lds_put<T, P>(ex + ((size_t)(t & 1) * cthreads + rl) * PS, tyo);
+            }
+            // refill slot u with step t + D (scheduling barriers keep the slot in its registers)
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(u);
+            __builtin_amdgcn_sched_barrier(0);
             lds_barrier();
         }
     }

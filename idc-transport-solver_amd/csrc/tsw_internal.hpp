@@ -134,17 +134,18 @@ void launch_si_check(const Geom &g, const double *cur, const double *last, doubl
 struct MgLevel {
     int nx, ny;
     double *cx, *cy, *sv;  // face conductances ((nx+1)*ny, nx*(ny+1)), absorption x volume (nx*ny)
-    double *x, *b, *r;     // iterate, right-hand side, residual (nx*ny each)
+    double *x, *b, *y;     // pre-smoothed iterate, right-hand side, post-smoothed result (nx*ny each)
 };
+constexpr int kMgMaxLevels = 16;
 void launch_dsa_rhs(const Geom &g, const double *x, const double *sigma_s, double *b, hipStream_t s);
 void launch_dsa_add(const Geom &g, const double *d, double *y, hipStream_t s);
-void launch_mg_zero(double *x, int n, hipStream_t s);
-void launch_mg_rbgs(const MgLevel &L, double *x, const double *b, int colour, hipStream_t s);
-void launch_mg_residual(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s);
-void launch_mg_restrict(const MgLevel &f, const MgLevel &c, const double *rf, double *bc, hipStream_t s);
-void launch_mg_prolong_add(const MgLevel &f, const MgLevel &c, const double *xc, double *xf, hipStream_t s);
-// the V-cycle from level l0 (zero initial guess) to the coarsest in one workgroup; lv_dev: device array
-void launch_mg_fused(const MgLevel *lv_dev, int l0, int nlev, hipStream_t s);
+// two tile-wise red-black sweeps on level L from in (null: zero; coarse non-null: in plus the
+// prolongated coarse result coarse->y) into out, colour c0 first; the fused residual-and-restriction
+void launch_mg_smooth(const MgLevel &L, const double *in, double *out, const MgLevel *coarse, int c0, hipStream_t s);
+void launch_mg_restrict_residual(const MgLevel &f, const MgLevel &c, hipStream_t s);
+// LDS doubles the single-workgroup V-cycle needs for levels l0..nlev-1 (host arrays of levels)
+size_t mg_fused_lds_doubles(const MgLevel *lv, int l0, int nlev);
+void launch_mg_fused(const MgLevel *lv_dev, int l0, int nlev, size_t lds_bytes, bool from_y, hipStream_t s);
 
 // Bit pattern of an empty band-exchange record: a signalling NaN, which arithmetic never produces.
 template <typename T> struct ExchangeSentinel;
@@ -176,10 +177,10 @@ void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStrea
 // inflow face traces of the halo cells of reference-layout psi
 template <typename TO>
 void launch_inflow(const Geom &g, const double *ref, TO *tin_x, TO *tin_y, hipStream_t s);
-// phi[i][j][k] = beta phi + alpha sum_q w_q psi (diagonal psi); phiQ: [4][N][ncell] scratch
+// phi[i][j][k] = beta xb + alpha sum_q w_q psi (diagonal psi; xb null: xb = phi); phiQ: [4][N][ncell] scratch
 template <typename TI>
 void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha,
-                     double beta, hipStream_t s, const int *skip = nullptr);
+                     double beta, hipStream_t s, const int *skip = nullptr, const double *xb = nullptr);
 // res = qsrc + sig_s phi/(4 pi) - (L + 1/dt) psi on the diagonal layout (FP64 inputs)
 template <typename TO>
 void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,
@@ -218,6 +219,8 @@ void launch_multi_dot(size_t n, int m, const double *X, size_t ld, const double 
                       int nblocks, hipStream_t s);
 // y[k] -= sum_i h[i] X_i[k]
 void launch_multi_axpy(size_t n, int m, const double *X, size_t ld, const double *h, double *y, hipStream_t s);
+// v /= sqrt(*sq) when that is positive; *hn = sqrt(*sq) (sq, hn: device scalars)
+void launch_normalize(size_t n, const double *sq, double *v, double *hn, hipStream_t s);
 // out = sum over interior coefficients of (a-b)^2
 void launch_sqdist_interior(const Geom &g, const double *a, const double *b, double *out, double *partials,
                             int nblocks, hipStream_t s);
