@@ -218,7 +218,9 @@ def bench_idc(args):
 
 def bench_batched(args):
     """Batched sweep of the seeded 256-problem uneven batch (N 32..256, Q1..Q3, S4..S12, 8x8 block
-    materials): one step = one sweep of every member, inputs resident in HBM."""
+    materials): one step = one sweep of every member, inputs resident in HBM.  Timed in FP64, in FP32 and
+    mixed (FP32 sweeps with one FP64 refinement step, TSW_MIXED); value = unknowns/s of the mixed sweep,
+    whose error against the FP64 sweep is reported beside it."""
     import torch
     import tsw
     from tsw import problems
@@ -226,14 +228,14 @@ def bench_batched(args):
     dev = torch.device("cuda:0")
     srcs = [P.source_device(dev) for P in batch]
     sig = [torch.from_numpy(P.sigma_t).to(dev) for P in batch]
-    work = [s.clone() for s in srcs]
     ctx = tsw.Context(0)
     ctx.set_tuning("prefetch", args.prefetch)
     ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
     U = sum(P.unknowns for P in batch)
     ncell = sum(P.nx * P.ny for P in batch)
-    res = {}
-    for name, prec in (("fp32", tsw.FP32), ("fp64", tsw.FP64)):
+    res, outs = {}, {}
+    for name, prec in (("fp32", tsw.FP32), ("mixed", tsw.MIXED), ("fp64", tsw.FP64)):
+        work = [s.clone() for s in srcs]
         for _ in range(args.warmup):
             ctx.batch_sweep(work, precision=prec)
         ctx.sweep_timing(reset=True)
@@ -246,24 +248,35 @@ def bench_batched(args):
             ctx.batch_sweep(work, precision=prec)   # returns when the device work is done
             wall += time.perf_counter() - t0
         n, ms = ctx.sweep_timing(reset=True)
-        res[name] = (wall / args.steps, ms / max(n, 1))
-        print(f"bench: batched {name}: call {wall / args.steps * 1e3:.3f} ms, sweep kernels {ms / max(n, 1):.3f} ms "
-              f"per batch sweep", file=sys.stderr, flush=True)
+        res[name] = (wall / args.steps, ms / max(n, 1), n // max(args.steps, 1))
+        outs[name] = work
+        print(f"bench: batched {name}: call {wall / args.steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
+              f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
     ctx.close()
-    c32, c64 = res["fp32"][0], res["fp64"][0]           # whole tsw_batch_sweep call (s)
-    t32, t64 = res["fp32"][1] * 1e-3, res["fp64"][1] * 1e-3  # sweep kernels alone (s)
+
+    def rel(a_list, b_list):
+        num = sum(float((a - b).norm() ** 2) for a, b in zip(a_list, b_list))
+        return math.sqrt(num / sum(float(b.norm() ** 2) for b in b_list))
+
+    err_mixed, err32 = rel(outs["mixed"], outs["fp64"]), rel(outs["fp32"], outs["fp64"])
+    print(f"bench: batched rel err vs FP64: mixed {err_mixed:.3e}, fp32 {err32:.3e}", file=sys.stderr, flush=True)
+    cm, c32, c64 = res["mixed"][0], res["fp32"][0], res["fp64"][0]   # whole tsw_batch_sweep call (s)
+    t32, t64 = res["fp32"][1] * 1e-3, res["fp64"][1] * 1e-3          # one batched sweep's kernels (s)
     b32 = U * 8 + ncell * 4
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
-        "value": U / c32, "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": c32 * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded batch generated in-process)",
+        "value": U / cm, "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": cm * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 sweep / f64 refinement",
+        "data": "synthetic (seeded batch generated in-process)",
         "config": {"workload": "batched sweep: 256 uneven problems, N 32..256, Q1..Q3, S4..S12, 8x8 block "
-                               "materials, seed 2026", "unknowns_per_batch_sweep": U},
-        "sweep_kernels_ms": t32 * 1e3,
+                               "materials, seed 2026; mixed = FP32 sweep + one FP64 refinement step",
+                   "unknowns_per_batch_sweep": U},
+        "mixed_rel_err_vs_fp64": err_mixed,
+        "fp32": {"value": U / c32, "ms_per_step": c32 * 1e3, "sweep_kernels_ms": t32 * 1e3, "rel_err_vs_fp64": err32},
         "fp64": {"value": U / c64, "ms_per_step": c64 * 1e3, "sweep_kernels_ms": t64 * 1e3,
                  "hbm_gbs": (U * 16 + ncell * 8) / t64 / 1e9},
-        "speedup_fp32_vs_fp64": c64 / c32,
+        "speedup_mixed_vs_fp64": c64 / cm,
         "roofline": {"bound": "hbm", "achieved": b32 / t32 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": b32 / t32 / 1e9 / HBM_PEAK_GBS, "kernel": "k_diag_sweep batched launches (FP32)",
                      "algorithmic_bytes_per_launch": b32, "traffic": None},

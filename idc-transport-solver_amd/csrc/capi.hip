@@ -76,9 +76,9 @@ struct tsw_ctx {
         bool mat = false;
         std::vector<int> members;       // batch indices
         std::vector<int> items;         // pairs {member slot, ordinate}, heaviest first
-        DBuf params64, params32, dev_items;
-        bool built64 = false, built32 = false, mat64 = false, mat32 = false;
-        size_t lds64 = 0, lds32 = 0;
+        DBuf params64, params32, params32c, dev_items;  // params32c: FP32 correction sweeps of TSW_MIXED
+        bool built64 = false, built32 = false, built32c = false, mat64 = false, mat32 = false;
+        size_t lds64 = 0, lds32 = 0, lds32c = 0;
     };
     std::vector<Group> groups;
     // Adjoint problem: the same mesh with the negated ordinate set in reverse order (its sweeps are the
@@ -103,6 +103,7 @@ struct tsw_ctx {
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
     // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
     bool si_graph = true;
+    int refine_steps = 1;  // FP64 refinement steps of a TSW_MIXED sweep
     static constexpr int kSiChunk = 8;
     DBuf si_ctl;
     SiControl *si_ctl_host = nullptr;
@@ -940,6 +941,44 @@ void apply_staggered(tsw_ctx *c, double *psi_ref, const double *sigma_t, double 
 }
 
 
+// TSW_MIXED sweep on the staggered engine: psi_0 = L32^{-1} Q (FP32, with the inflow), then refine_steps
+// times psi += L32^{-1}(Q - (L + 1/dt) psi), the residual formed in FP64 (with the FP64 inflow) and swept in
+// FP32 with zero inflow; psi_ref (reference layout) receives the FP64 result.  With one step the FP64
+// iterate is never stored: the residual reads psi_0 itself and the result is written as psi_0 + d.
+void mixed_refine_store(tsw_ctx *c, double dt, const float *psi0, double *psi_ref) {
+    const Geom &g = c->g;
+    auto &d = c->diag;
+    hipStream_t s = c->stream;
+    const size_t nd = (size_t)g.nq * g.N() * g.nx * g.ny;
+    float *res32 = dbuf<float>(d.res32, nd);
+    auto residual = [&](const auto *psi) {
+        launch_residual_diag<float>(g, d.q64.as<double>(), nullptr, nullptr, d.sigt64.as<double>(), psi,
+                                    d.tinx64.as<double>(), d.tiny64.as<double>(), c->xi64.as<double>(),
+                                    c->eta64.as<double>(), inverse_dt(dt), res32, s);
+    };
+    if (c->refine_steps == 1) {
+        residual(psi0);
+        diag_sweep<float>(c, dt, res32, nullptr, false, res32);
+        launch_psi_store_sum(g, psi_ref, psi0, res32, s);
+        return;
+    }
+    double *psi64 = dbuf<double>(d.ir64, nd);
+    launch_convert_f2d(psi0, psi64, nd, s);
+    for (int k = 0; k < c->refine_steps; ++k) {
+        residual(static_cast<const double *>(psi64));
+        diag_sweep<float>(c, dt, res32, nullptr, false, res32);
+        launch_accumulate_f2d(res32, psi64, nd, s);
+    }
+    diag_store<double>(c, psi64, psi_ref);
+}
+
+void apply_staggered_mixed(tsw_ctx *c, double *psi_ref, const double *sigma_t, double dt) {
+    diag_setup(c, sigma_t, nullptr, psi_ref, dt, true, true);
+    float *q32 = c->diag.q32.as<float>();
+    diag_sweep<float>(c, dt, q32, nullptr, true, q32);
+    mixed_refine_store(c, dt, q32, psi_ref);
+}
+
 // Source iteration on the staggered engine (SISolver.cpp:206-249).  Q: reference-layout source with
 // inflow halos (device).  The solver's angular flux state (reference layout) seeds phi and receives
 // the last iterate.  Returns iterations and the last change.
@@ -1280,11 +1319,39 @@ bool gmres_device(tsw_ctx *ctx, double *Q, const double *st, const double *ss, d
 }
 
 // In-place sweep of a reference-layout device array with the context's engine.
+// TSW_MIXED on the wavefront engine: from the iterate with zero interior and Q's inflow halos,
+// 1 + refine_steps times psi += L32^{-1}(Q - (L + 1/dt) psi) (FP64 residual, zero-inflow FP32 sweep).
+void sweep_wavefront_mixed(tsw_ctx *ctx, double *Q, const double *st, double dt) {
+    const Geom &g = ctx->g;
+    hipStream_t s = ctx->stream;
+    const size_t npsi = g.npsi();
+    const size_t per = (size_t)g.nq * g.N();
+    double *psi64 = static_cast<double *>(ctx->ir_state.reserve(sizeof(double) * npsi));
+    double *res64 = static_cast<double *>(ctx->r64.reserve(sizeof(double) * npsi));
+    float *p32 = static_cast<float *>(ctx->psi32.reserve(sizeof(float) * npsi));
+    const float *st32 = as_f32(ctx, st, g.ncell_h(), ctx->sig_t32);
+    TSW_HIP(hipMemcpyAsync(psi64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+    launch_fill_region<double>(g, psi64, per, 0, 0.0, s);
+    for (int k = 0; k <= ctx->refine_steps; ++k) {
+        TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), st, psi64, res64, -1.0, 1.0, s);
+        if (inverse_dt(dt) != 0.0) launch_axpby(npsi, -inverse_dt(dt), psi64, 1.0, res64, s);
+        launch_fill_region<double>(g, res64, per, 1, 0.0, s);
+        launch_convert_d2f(res64, p32, npsi, s);
+        sweep_dev<float>(ctx, p32, st32, dt);
+        launch_accumulate_f2d(p32, psi64, npsi, s);
+    }
+    TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+}
+
 void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int precision) {
-    require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+    require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG, "bad precision");
     if (staggered_ok(ctx)) {
         if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
-        else apply_staggered<float>(ctx, psi, sig, dt);
+        else if (precision == TSW_FP32) apply_staggered<float>(ctx, psi, sig, dt);
+        else apply_staggered_mixed(ctx, psi, sig, dt);
+    } else if (precision == TSW_MIXED) {
+        sweep_wavefront_mixed(ctx, psi, sig, dt);
     } else {
         sweep_prec(ctx, psi, sig, dt, precision);
     }
@@ -1825,6 +1892,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "si_graph") {
             require(value == 0 || value == 1, TSW_E_ARG, "si_graph must be 0 or 1");
             ctx->si_graph = value != 0;
+        } else if (k == "refine_steps") {
+            require(value >= 1 && value <= 8, TSW_E_ARG, "refine_steps must be 1..8");
+            ctx->refine_steps = value;
         } else if (k == "autotune") {
             require(value == 0 || value == 1, TSW_E_ARG, "autotune must be 0 or 1");
             ctx->autotune = value != 0;
@@ -1848,6 +1918,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "si_graph") *value = ctx->si_graph ? 1 : 0;
         else if (k == "use_dsa") *value = ctx->use_dsa ? 1 : 0;
         else if (k == "dsa_cycles") *value = ctx->dsa_cycles;
+        else if (k == "refine_steps") *value = ctx->refine_steps;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }
@@ -1856,7 +1927,8 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
     return guarded(ctx, [&] {
         need_problem(ctx);
         const Geom &g = ctx->g;
-        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
+        require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
+                "bad precision");
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
         const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
         sweep_device(ctx, psi, sig, dt, precision);
@@ -1909,6 +1981,7 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
                  double abs_tol, int max_its, int precision, int mem, int *iters, double *err) {
     return guarded(ctx, [&] {
         need_problem(ctx);
+        if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         const Geom &g = ctx->g;
         hipStream_t s = ctx->stream;
@@ -1967,6 +2040,7 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
     const int rc = guarded(ctx, [&] {
         need_problem(ctx);
         require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
+        if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         const Geom &g = ctx->g;
         hipStream_t s = ctx->stream;
@@ -2074,6 +2148,7 @@ int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const do
     const int rc = guarded(ctx, [&] {
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt), TSW_E_ARG, "IDC needs a finite time step");
+        if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
         const Geom &g = ctx->g;
@@ -2099,6 +2174,7 @@ int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const dou
     const int rc = guarded(ctx, [&] {
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
+        if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
         const Geom &g = ctx->g;
@@ -2132,6 +2208,7 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
         require(seed && grad_psi0, TSW_E_ARG, "seed and grad_psi0 are required");
         const bool grads = grad_sigma_t || grad_sigma_s;
         require(!grads || psi0, TSW_E_ARG, "cross-section gradients need psi0");
+        if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
         if (ctx->idc.nodes.empty()) idc_tables(ctx->idc);
         const Geom &g = ctx->g;
@@ -2269,17 +2346,20 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
 }  // extern "C"
 
 namespace {
-// Builds (once) the device parameter table of a launch group for precision T.
+// Builds (once) the device parameter table of a launch group for precision T: the sweep of each member's
+// source in place with its inflow, or (corr, FP32 only) the zero-inflow sweep of its residual buffer.
 template <typename T>
-void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G) {
-    bool &built = sizeof(T) == 8 ? G.built64 : G.built32;
+void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
+    bool &built = sizeof(T) == 8 ? G.built64 : (corr ? G.built32c : G.built32);
     if (built) return;
     std::vector<DiagSweepParams<T>> ps;
     size_t lds = 0;
     for (int b : G.members) {
         tsw_ctx *sub = ctx->batch[b].get();
-        T *q = (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
-        DiagSweepParams<T> sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, true, q);
+        const size_t nd = (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny;
+        T *q = corr ? (T *)dbuf<float>(sub->diag.res32, nd)
+                    : (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
+        DiagSweepParams<T> sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, !corr, q);
         lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, sub->g.p));
         ps.push_back(sp);
     }
@@ -2299,17 +2379,17 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G) {
         }
     }
     require(lds != 0, TSW_E_UNSUPPORTED, "batch group does not fit the staggered sweep");
-    DBuf &buf = sizeof(T) == 8 ? G.params64 : G.params32;
+    DBuf &buf = sizeof(T) == 8 ? G.params64 : (corr ? G.params32c : G.params32);
     TSW_HIP(hipMemcpyAsync(buf.reserve(sizeof(DiagSweepParams<T>) * ps.size()), ps.data(),
                            sizeof(DiagSweepParams<T>) * ps.size(), hipMemcpyHostToDevice, ctx->stream));
     TSW_HIP(hipStreamSynchronize(ctx->stream));  // ps goes out of scope
-    (sizeof(T) == 8 ? G.lds64 : G.lds32) = lds;
+    (sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32)) = lds;
     built = true;
 }
 
 template <typename T>
-void batch_sweep(tsw_ctx *ctx) {
-    for (auto &G : ctx->groups) batch_group_params<T>(ctx, G);
+void batch_sweep(tsw_ctx *ctx, bool corr = false) {
+    for (auto &G : ctx->groups) batch_group_params<T>(ctx, G, corr);
     hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
     TSW_HIP(hipEventRecord(e0, ctx->stream));
     TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
@@ -2317,14 +2397,15 @@ void batch_sweep(tsw_ctx *ctx) {
     for (auto &G : ctx->groups) {
         hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
         TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
-        const DiagSweepParams<T> *probs = (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : G.params32.ptr);
+        const DiagSweepParams<T> *probs =
+            (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : (corr ? G.params32c.ptr : G.params32.ptr));
         // representative parameters for the host-side dispatch (mode and material path)
         DiagSweepParams<T> rep{};
         rep.qsrc = reinterpret_cast<const T *>(1);
         rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
         rep.prefetch = ctx->prefetch_opt;
         launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny,
-                                   sizeof(T) == 8 ? G.lds64 : G.lds32, G.p, sd);
+                                   sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32), G.p, sd);
         ++k;
     }
     for (int j = 0; j < std::min<int>(k, tsw_ctx::kSideStreams); ++j) {
@@ -2342,8 +2423,9 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
     return guarded(ctx, [&] {
         require(!ctx->batch.empty(), TSW_E_STATE, "tsw_batch_set has not been called");
         require(psi != nullptr, TSW_E_ARG, "null psi array list");
-        require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
-        const bool f64 = precision == TSW_FP64;
+        require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
+                "bad precision");
+        const bool f64 = precision == TSW_FP64, mixed = precision == TSW_MIXED;
         std::vector<double *> dev(ctx->batch.size());
         // per-member layout conversions spread over the side streams, joined before the next phase
         auto on_side_streams = [&](auto &&fn) {
@@ -2367,12 +2449,44 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
         };
         on_side_streams([&](size_t b, tsw_ctx *sub) {
             dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
-            diag_load_source(sub, dev[b], f64, !f64);
+            diag_load_source(sub, dev[b], f64 || mixed, !f64);
         });
         if (f64) batch_sweep<double>(ctx);
         else batch_sweep<float>(ctx);
+        auto nd_of = [](const tsw_ctx *sub) { return (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny; };
+        // the refinement of mixed_refine_store, its FP32 sweeps batched across members
+        const bool fused_store = mixed && ctx->refine_steps == 1;
+        auto residual = [&](tsw_ctx *sub, const auto *psi) {
+            auto &d = sub->diag;
+            launch_residual_diag<float>(sub->g, d.q64.as<double>(), nullptr, nullptr, d.sigt64.as<double>(), psi,
+                                        d.tinx64.as<double>(), d.tiny64.as<double>(), sub->xi64.as<double>(),
+                                        sub->eta64.as<double>(), inverse_dt(ctx->batch_dt),
+                                        dbuf<float>(d.res32, nd_of(sub)), sub->stream);
+        };
+        if (fused_store) {
+            on_side_streams([&](size_t, tsw_ctx *sub) { residual(sub, sub->diag.q32.as<float>()); });
+            batch_sweep<float>(ctx, true);
+        } else if (mixed) {
+            on_side_streams([&](size_t, tsw_ctx *sub) {
+                launch_convert_f2d(sub->diag.q32.as<float>(), dbuf<double>(sub->diag.ir64, nd_of(sub)), nd_of(sub),
+                                   sub->stream);
+            });
+            for (int k = 0; k < ctx->refine_steps; ++k) {
+                on_side_streams([&](size_t, tsw_ctx *sub) {
+                    residual(sub, static_cast<const double *>(sub->diag.ir64.as<double>()));
+                });
+                batch_sweep<float>(ctx, true);
+                on_side_streams([&](size_t, tsw_ctx *sub) {
+                    launch_accumulate_f2d(sub->diag.res32.as<float>(), sub->diag.ir64.as<double>(), nd_of(sub),
+                                          sub->stream);
+                });
+            }
+        }
         on_side_streams([&](size_t b, tsw_ctx *sub) {
             if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
+            else if (fused_store)
+                launch_psi_store_sum(sub->g, dev[b], sub->diag.q32.as<float>(), sub->diag.res32.as<float>(), sub->stream);
+            else if (mixed) diag_store<double>(sub, sub->diag.ir64.as<double>(), dev[b]);
             else diag_store<float>(sub, sub->diag.q32.as<float>(), dev[b]);
             if (mem == TSW_MEM_HOST) stage_out(sub, psi[b], dev[b], sub->g.npsi(), mem);
         });

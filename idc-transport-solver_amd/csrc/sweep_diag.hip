@@ -1,6 +1,8 @@
 // Staggered-sweep dispatch and the companion operators on the diagonal-major layout (diag_layout.hpp):
 // layout conversions, inflow traces, the moment contraction and the FP64 residual.  The sweep kernel
 // itself is in sweep_kernel.hpp.
+#include <type_traits>
+
 #include "sweep_kernel.hpp"
 
 namespace tsw {
@@ -105,11 +107,13 @@ inline int rows_block(int ny) { return ny <= 64 ? 64 : (ny <= 128 ? 128 : (ny <=
 // them through LDS (held in the diagonal precision): on the reference side each cell's ordinates of
 // the quadrant are one contiguous run, on the diagonal side each (q, k) plane receives R consecutive
 // positions.  Both sides are walked with all threads over the flattened tile.
-template <typename TR, typename TD>
+// SUM (dir 1 only): ref = (TR) diag + (TR) diag2, the tile held in TR.
+template <typename TR, typename TD, bool SUM = false>
 __global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap dm, int nq, int N, int qo1, int qo2,
-                                                     int qo3, int R, int dir) {
+                                                     int qo3, int R, int dir, const TD *diag2 = nullptr) {
+    using TT = std::conditional_t<SUM, TR, TD>;
     extern __shared__ unsigned char smem_raw[];
-    TD *tile = reinterpret_cast<TD *>(smem_raw);  // [R][nqQ*N]
+    TT *tile = reinterpret_cast<TT *>(smem_raw);  // [R][nqQ*N]
     const int s = blockIdx.x, Q = blockIdx.z;
     const int r0 = dm.rlo(s) + (int)blockIdx.y * R;
     const int rhi = s < dm.ny - 1 ? s : dm.ny - 1;
@@ -130,20 +134,22 @@ __global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap 
     const gptr<TR> base = (gptr<TR>)(ref + (((size_t)i0 * (dm.ny + 2) + j0) * nq + q0) * N);
     const gptr<TD> dbase = (gptr<TD>)(diag + (size_t)q0 * N * ncell + (size_t)p0);
     const int total = run * nr;
-    if (dir == 0) {
+    if (!SUM && dir == 0) {
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e / run, v = e - t * run;
-            tile[e] = TD(base[t * cstride + v]);
+            tile[e] = TT(base[t * cstride + v]);
         }
         __syncthreads();
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e % nr, qk = e / nr;  // consecutive threads: consecutive positions
-            dbase[(size_t)qk * ncell + t] = tile[t * run + qk];
+            dbase[(size_t)qk * ncell + t] = TD(tile[t * run + qk]);
         }
     } else {
+        const gptr<const TD> dbase2 = SUM ? (gptr<const TD>)(diag2 + (size_t)q0 * N * ncell + (size_t)p0) : nullptr;
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e % nr, qk = e / nr;
-            tile[t * run + qk] = dbase[(size_t)qk * ncell + t];
+            if (SUM) tile[t * run + qk] = TT(dbase[(size_t)qk * ncell + t]) + TT(dbase2[(size_t)qk * ncell + t]);
+            else tile[t * run + qk] = TT(dbase[(size_t)qk * ncell + t]);
         }
         __syncthreads();
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -169,6 +175,21 @@ void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s
 }
 template void launch_psi_convert<double, double>(const Geom &, double *, double *, int, hipStream_t);
 template void launch_psi_convert<double, float>(const Geom &, double *, float *, int, hipStream_t);
+
+void launch_psi_store_sum(const Geom &g, double *ref, const float *a, const float *b, hipStream_t s) {
+    const DiagMap dm{g.nx, g.ny};
+    int max_run = 0;
+    for (int Q = 0; Q < 4; ++Q) max_run = std::max(max_run, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
+    if (max_run == 0) return;
+    int R = 64;
+    while (R > 1 && (size_t)R * max_run * sizeof(double) > 48 * 1024) R /= 2;
+    const int maxlen = g.nx < g.ny ? g.nx : g.ny;
+    const dim3 grid(g.nx + g.ny - 1, (maxlen + R - 1) / R, 4);
+    hipLaunchKernelGGL((k_psi_convert<double, float, true>), grid, dim3(256), (size_t)R * max_run * sizeof(double), s,
+                       ref, const_cast<float *>(a), dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], R, 1,
+                       b);
+    TSW_HIP(hipGetLastError());
+}
 
 // per-cell array [i][j][k] (k < K, halo layout) -> per-quadrant diagonal [Q][k][pos]
 template <typename TI, typename TO>
@@ -295,13 +316,14 @@ template void launch_pmv_diag<float>(const Geom &, const float *, const double *
 
 // =====================================================================================================
 // FP64 angular residual on the diagonal layout, the refinement's r = Q + sigma_s phi/(4 pi) - L psi
-// (Lmv, TransportOperator.cpp:1576-1658, written with face traces), stored in TO.
+// (Lmv, TransportOperator.cpp:1576-1658, written with face traces), stored in TO; phi_d null: no
+// scattering term.
 // psi's inflow comes from tin_x/tin_y (traces of the halo cells).  Grid (diagonal, ordinate).
 // =====================================================================================================
-template <typename TO, int P>
+template <typename TO, typename TP, int P>
 __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *__restrict__ phi_d,
                                 const double *__restrict__ sigs_d, const double *__restrict__ sigt_d,
-                                const double *__restrict__ psi, const double *__restrict__ tin_x,
+                                const TP *__restrict__ psi, const double *__restrict__ tin_x,
                                 const double *__restrict__ tin_y, const double *__restrict__ xi_,
                                 const double *__restrict__ eta_, TO *__restrict__ res, DiagMap dm, int nq, int qo1,
                                 int qo2, int qo3, double dx, double dy, double inv_dt) {
@@ -316,10 +338,10 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
     const double xi = xi_[q], eta = eta_[q];
     const bool xneg = xi <= 0.0, yneg = eta <= 0.0;
     const size_t p = (size_t)dm.pos(s, r);
-    const double *pq = psi + (size_t)q * N * ncell;
+    const TP *pq = psi + (size_t)q * N * ncell;
     double x[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) x[k] = pq[(size_t)k * ncell + p];
+    for (int k = 0; k < N; ++k) x[k] = (double)pq[(size_t)k * ncell + p];
     double tx[P], ty[P];
     if (c == 0) {
 #pragma unroll
@@ -328,7 +350,7 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
         const size_t pu = (size_t)dm.pos(s - 1, r);
         double u[N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) u[k] = pq[(size_t)k * ncell + pu];
+        for (int k = 0; k < N; ++k) u[k] = (double)pq[(size_t)k * ncell + pu];
         trace_x<double, P>(u, xneg, tx);
     }
     if (r == 0) {
@@ -338,7 +360,7 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
         const size_t pu = (size_t)dm.pos(s - 1, r - 1);
         double u[N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) u[k] = pq[(size_t)k * ncell + pu];
+        for (int k = 0; k < N; ++k) u[k] = (double)pq[(size_t)k * ncell + pu];
         trace_y<double, P>(u, yneg, ty);
     }
     // (L + 1/dt) x = (Aq x + (sigma + 1/dt) dx dy x - Kx tx - Ky ty) / (dx dy): the sweep's element
@@ -360,20 +382,20 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
         for (int col = 0; col < N; ++col) lx += A[row + N * col] * x[col];
         lx = (lx - inflow[row]) * inv_area;
         double v = qsrc[(size_t)q * N * ncell + (size_t)row * ncell + p];
-        v += sigs_d[pQ] * phi_d[(size_t)Q * N * ncell + (size_t)row * ncell + p] * kInv4Pi;
+        if (phi_d) v += sigs_d[pQ] * phi_d[(size_t)Q * N * ncell + (size_t)row * ncell + p] * kInv4Pi;
         pr[(size_t)row * ncell] = TO(v - lx);
     }
 }
 
-template <typename TO>
+template <typename TO, typename TP>
 void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,
-                          const double *sigt_d, const double *psi, const double *tin_x, const double *tin_y,
+                          const double *sigt_d, const TP *psi, const double *tin_x, const double *tin_y,
                           const double *xi, const double *eta, double inv_dt, TO *res, hipStream_t s) {
     const DiagMap dm{g.nx, g.ny};
     const dim3 grid(g.nx + g.ny - 1, g.nq), block(rows_block(g.ny));
     const int a = g.quad_off[1], b = g.quad_off[2], c = g.quad_off[3];
 #define TSW_RES(PP)                                                                                                   \
-    hipLaunchKernelGGL((k_residual_diag<TO, PP>), grid, block, 0, s, qsrc, phi_d, sigs_d, sigt_d, psi, tin_x, tin_y, \
+    hipLaunchKernelGGL((k_residual_diag<TO, TP, PP>), grid, block, 0, s, qsrc, phi_d, sigs_d, sigt_d, psi, tin_x, tin_y, \
                        xi, eta, res, dm, g.nq, a, b, c, g.dx, g.dy, inv_dt)
     switch (g.p) {
         case 0: TSW_RES(1); break;
@@ -385,11 +407,14 @@ void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d
 #undef TSW_RES
     TSW_HIP(hipGetLastError());
 }
-template void launch_residual_diag<float>(const Geom &, const double *, const double *, const double *, const double *,
-                                          const double *, const double *, const double *, const double *, const double *,
-                                          double, float *, hipStream_t);
-template void launch_residual_diag<double>(const Geom &, const double *, const double *, const double *, const double *,
-                                           const double *, const double *, const double *, const double *, const double *,
-                                           double, double *, hipStream_t);
+template void launch_residual_diag<float, double>(const Geom &, const double *, const double *, const double *,
+                                                  const double *, const double *, const double *, const double *,
+                                                  const double *, const double *, double, float *, hipStream_t);
+template void launch_residual_diag<double, double>(const Geom &, const double *, const double *, const double *,
+                                                   const double *, const double *, const double *, const double *,
+                                                   const double *, const double *, double, double *, hipStream_t);
+template void launch_residual_diag<float, float>(const Geom &, const double *, const double *, const double *,
+                                                 const double *, const float *, const double *, const double *,
+                                                 const double *, const double *, double, float *, hipStream_t);
 
 }  // namespace tsw

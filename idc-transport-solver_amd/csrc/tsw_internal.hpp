@@ -182,10 +182,12 @@ template <typename TI>
 void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ, double *phi, double alpha,
                      double beta, hipStream_t s, const int *skip = nullptr, const double *xb = nullptr);
 // res = qsrc + sig_s phi/(4 pi) - (L + 1/dt) psi on the diagonal layout (FP64 inputs)
-template <typename TO>
+template <typename TO, typename TP = double>
 void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d, const double *sigs_d,
-                          const double *sigt_d, const double *psi, const double *tin_x, const double *tin_y,
+                          const double *sigt_d, const TP *psi, const double *tin_x, const double *tin_y,
                           const double *xi, const double *eta, double inv_dt, TO *res, hipStream_t s);
+// reference psi (interior) <- (double) a + (double) b, a and b diagonal-layout FP32 arrays
+void launch_psi_store_sum(const Geom &g, double *ref, const float *a, const float *b, hipStream_t s);
 
 // Largest dynamic LDS allocation one workgroup may make on the current device (bytes).
 size_t max_lds_per_block();

@@ -103,7 +103,8 @@ int precision_from(const ParameterList &p) {
     const std::string v = p.GetString("sweep_precision");
     if (v == "fp64") return TSW_FP64;
     if (v == "fp32") return TSW_FP32;
-    throw std::invalid_argument("sweep_precision must be fp64 or fp32, not '" + v + "'");
+    if (v == "mixed") return TSW_MIXED;  // FP32 sweeps refined to FP64 accuracy
+    throw std::invalid_argument("sweep_precision must be fp64, fp32 or mixed, not '" + v + "'");
 }
 int all_cells(OpDomain d) { return ((int)d & (int)OpDomain::Boundary) ? 1 : 0; }
 bool flag_from(const ParameterList &p, const std::string &key, bool dflt) {

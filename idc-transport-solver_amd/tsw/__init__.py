@@ -27,6 +27,7 @@ TSW_E_NOCONV = -4
 TSW_E_UNSUPPORTED = -5
 FP64 = 0
 FP32 = 1
+MIXED = 2  # sweeps: FP32 with FP64 iterative refinement; solvers: as FP32
 MEM_HOST = 0
 MEM_DEVICE = 1
 IDC_LOBATTO = 0

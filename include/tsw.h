@@ -34,6 +34,9 @@ extern "C" {
 
 #define TSW_FP64 0 /* sweep and element solves in double precision */
 #define TSW_FP32 1 /* sweep and element solves in single precision (outer loops stay FP64) */
+#define TSW_MIXED 2 /* sweeps (tsw_sweep_apply, tsw_batch_sweep): FP32 sweeps with FP64 iterative refinement,
+                     * psi_0 = L32^{-1} src, then refine_steps times psi += L32^{-1}(src - L psi) with the
+                     * residual in FP64; solvers treat it as TSW_FP32 */
 
 #define TSW_MEM_HOST 0
 #define TSW_MEM_DEVICE 1
@@ -86,18 +89,20 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * y = r + C^{-1} sigma_s r; C is the cell-mean finite-volume diffusion operator (D = 1/(3(sigma_t+1/dt)),
  * vacuum Marshak faces) inverted approximately by "dsa_cycles" (1..16, default 1) multigrid V-cycles.
  * "si_graph" (1 default, 0): staggered source iteration replays captured graphs of 8 iterations with
- * the loop test on the device (same iterates and counts as the eager loop).
+ * the loop test on the device (same iterates and counts as the eager loop).  "refine_steps" (1..8,
+ * default 1): FP64 refinement steps of a TSW_MIXED sweep.
  * Unknown keys fail with TSW_E_ARG. */
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* Reads back "band_rows" (rows per band of the last staggered sweep, 0 = one band per ordinate),
- * "prefetch" (its prefetch depth), or the setting of "autotune", "use_dsa", "dsa_cycles", "si_graph". */
+ * "prefetch" (its prefetch depth), or the setting of "autotune", "use_dsa", "dsa_cycles", "si_graph",
+ * "refine_steps". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
  * cells are overwritten.  dt <= 0 or +inf is steady state.  SweepOperator::Apply
  * (SweepOperator.hpp:126-131, SweepOperator.cpp:118-135).  precision TSW_FP32 converts psi and sigma
- * to single precision for the sweep and back. */
+ * to single precision for the sweep and back; TSW_MIXED refines that FP32 sweep to FP64 accuracy. */
 int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem);
 
 /* phi <- beta*phi + alpha*sum_q w_q psi_q.  TransportOperator::Pmv (TransportOperator.hpp:25-43,
@@ -217,7 +222,8 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
 
 /* Sweeps every member of the batch in place (psi[b] in the reference layout, halos = inflow), in
  * one set of concurrent launches (one launch per DG degree and row count, workgroups = ordinates of
- * all members).  SweepOperator::Apply semantics per member.  Timed as one sweep by tsw_sweep_timing. */
+ * all members).  SweepOperator::Apply semantics per member.  Timed as one sweep by tsw_sweep_timing
+ * (TSW_MIXED: each of its FP32 batched sweeps is timed as one). */
 int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem);
 
 /* Timing of the sweeps run on ctx since the last reset, measured with HIP events on the context's

@@ -18,7 +18,7 @@ def oracle_sweep(P, src):
     return orc_sweep(g, o, P.sigma_t, src)
 
 
-@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
 def test_small_uneven_batch_matches_oracle(precision):
     batch = problems.batched(seed=7, count=10, n_range=(3, 40), degrees=(0, 1, 2, 3), orders=(2, 4, 6, 8))
     rng = np.random.default_rng(3)
@@ -33,9 +33,12 @@ def test_small_uneven_batch_matches_oracle(precision):
         ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
         out = [s.copy() for s in srcs]
         ctx.batch_sweep(out, precision=precision)
-    tol = 1e-12 if precision == tsw.FP64 else 2e-6
+    # FP32 sweeps with one FP64 refinement step agree with FP64 to the square of the FP32 error
+    tol = {tsw.FP64: 1e-12, tsw.FP32: 2e-6, tsw.MIXED: 1e-10}[precision]
     for P, s, o in zip(batch, srcs, out):
-        assert rel_err(o, oracle_sweep(P, s)) <= tol, (P.name, P.nx, P.p, P.nq)
+        e = rel_err(o, oracle_sweep(P, s))
+        print(P.name, P.nx, P.p, P.nq, "rel err", e, flush=True)
+        assert e <= tol, (P.name, P.nx, P.p, P.nq)
         np.testing.assert_array_equal(o[0], s[0])  # halos untouched
 
 
@@ -51,12 +54,20 @@ def test_full_batch_fp32_vs_fp64_and_oracle_members():
         ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
         o64 = [s.clone() for s in srcs]
         o32 = [s.clone() for s in srcs]
+        omx = [s.clone() for s in srcs]
         torch.cuda.synchronize()
         ctx.batch_sweep(o64, precision=tsw.FP64)
         ctx.batch_sweep(o32, precision=tsw.FP32)
-    num = sum(float((a - b).norm() ** 2) for a, b in zip(o32, o64))
-    den = sum(float(b.norm() ** 2) for b in o64)
-    assert math.sqrt(num / den) <= 2e-5
+        ctx.batch_sweep(omx, precision=tsw.MIXED)
+
+    def err(xs):
+        num = sum(float((a - b).norm() ** 2) for a, b in zip(xs, o64))
+        return math.sqrt(num / sum(float(b.norm() ** 2) for b in o64))
+
+    e32, emx = err(o32), err(omx)
+    print("batch rel err vs FP64: fp32", e32, "mixed", emx, flush=True)
+    assert e32 <= 2e-5
+    assert emx <= 1e-10  # one FP64 refinement step
     order = np.argsort([P.unknowns for P in batch])[:3]
     for k in order:
         ref = oracle_sweep(batch[k], batch[k].source)
