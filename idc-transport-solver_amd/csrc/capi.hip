@@ -78,6 +78,7 @@ struct tsw_ctx {
         std::vector<int> items;         // pairs {member slot, ordinate}, heaviest first
         DBuf params64, params32, params32c, dev_items;  // params32c: FP32 correction sweeps of TSW_MIXED
         bool built64 = false, built32 = false, built32c = false, mat64 = false, mat32 = false;
+        bool direct64 = false, direct32 = false;  // direct-store layout (set with the parameter table)
         size_t lds64 = 0, lds32 = 0, lds32c = 0;
     };
     std::vector<Group> groups;
@@ -807,10 +808,17 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
     c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
     c->last_pf = sp.prefetch > 0 ? sp.prefetch : (sizeof(T) == 8 ? 2 : 3);
     sp.ainv = diag_inverses<T>(c, dt);
+    sp.direct = 0;
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
         sp.M = c->mat.M;
-        if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) sp.ainv = nullptr;  // table too large: eliminate per cell
+        if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // without the store staging buffer (one band)
+            sp.direct = 1;
+            if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // table too large: eliminate per cell
+                sp.direct = 0;
+                sp.ainv = nullptr;
+            }
+        }
     }
     if (!sp.ainv) sp.sig_t = (const T *)(f64 ? d.sigt64.ptr : d.sigt32.ptr);
     sp.psi = psi_out;
@@ -2363,9 +2371,24 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
         lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, sub->g.p));
         ps.push_back(sp);
     }
-    // members whose tables do not fit fall back to elimination; a mixed group eliminates everywhere
+    // one launch serves the group: if any member needs the direct-store layout every member uses it;
+    // members whose tables do not fit fall back to elimination, and a mixed group eliminates everywhere
+    bool any_direct = false;
+    for (auto &sp : ps) any_direct = any_direct || (sp.ainv && sp.direct);
+    if (any_direct) {
+        lds = 0;
+        for (size_t k = 0; k < ps.size(); ++k) {
+            tsw_ctx *sub = ctx->batch[G.members[k]].get();
+            if (!ps[k].ainv) continue;
+            ps[k].direct = 1;
+            const size_t b = diag_sweep_lds_bytes<T>(ps[k], sub->g.p);
+            if (b == 0) ps[k].ainv = nullptr;
+            lds = std::max(lds, b);
+        }
+    }
     bool all_mat = G.mat;
     for (auto &sp : ps) all_mat = all_mat && sp.ainv != nullptr;
+    (sizeof(T) == 8 ? G.direct64 : G.direct32) = all_mat && any_direct;
     (sizeof(T) == 8 ? G.mat64 : G.mat32) = all_mat;
     if (!all_mat) {
         lds = 0;
@@ -2374,6 +2397,7 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
             ps[k].ainv = nullptr;
             ps[k].mat = nullptr;
             ps[k].M = 0;
+            ps[k].direct = 0;
             ps[k].sig_t = (const T *)(sizeof(T) == 8 ? sub->diag.sigt64.ptr : sub->diag.sigt32.ptr);
             lds = std::max(lds, diag_sweep_lds_bytes<T>(ps[k], sub->g.p));
         }
@@ -2403,6 +2427,7 @@ void batch_sweep(tsw_ctx *ctx, bool corr = false) {
         DiagSweepParams<T> rep{};
         rep.qsrc = reinterpret_cast<const T *>(1);
         rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
+        rep.direct = (sizeof(T) == 8 ? G.direct64 : G.direct32) ? 1 : 0;
         rep.prefetch = ctx->prefetch_opt;
         launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny,
                                    sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32), G.p, sd);

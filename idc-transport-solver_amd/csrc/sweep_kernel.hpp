@@ -91,11 +91,11 @@ struct DiagLds {  // LDS carve-up in units of T; every array and row is 16-byte 
     static constexpr int N = P * P;
     static constexpr int PS = vround<T>(P), NS = vround<T>(N), MS = vround<T>(N * N);
     size_t ex, in, out, inv, total;
-    __host__ __device__ DiagLds(int cthreads, int nx, int M) {
+    __host__ __device__ DiagLds(int cthreads, int nx, int M, bool direct) {
         ex = 0;                                 // [2][cthreads][PS] y-traces between rows
         in = ex + (size_t)2 * cthreads * PS;    // [nx][PS] y-inflow traces of row 0
         out = in + (size_t)nx * PS;             // [2][cthreads][NS] solved cells awaiting the store
-        inv = out + (size_t)2 * cthreads * NS;  // [M][MS] element inverses (material path)
+        inv = out + (direct ? 0 : (size_t)2 * cthreads * NS);  // [M][MS] element inverses (material path)
         total = inv + (size_t)M * MS;
     }
 };
@@ -194,8 +194,9 @@ struct DiagWalk {
 
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
 // Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
-// upstream neighbour is dispatched before it and never waits on it).
-template <typename T, int P, bool MAT, int MODE, int D>
+// upstream neighbour is dispatched before it and never waits on it).  DIRECT (one band only): no storer
+// wave; each compute lane stores its own cell, which frees the LDS staging buffer for larger tables.
+template <typename T, int P, bool MAT, int MODE, int D, bool DIRECT>
 __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
@@ -223,8 +224,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     const int nrows = BR < ny - R0 ? BR : ny - R0;
     // every compute thread owns one LDS row slot; threads past the band's rows compute on clamped
     // inputs and their results are never read
-    const int cthreads = (int)blockDim.x - 64;
-    const Lds L(cthreads, nx, MAT ? sp.M : 0);
+    const int cthreads = DIRECT ? (int)blockDim.x : (int)blockDim.x - 64;
+    const Lds L(cthreads, nx, MAT ? sp.M : 0, DIRECT);
     T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
     const DiagMap dm{nx, ny};
     const size_t ncell = (size_t)dm.cells();
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     const unsigned plane = (unsigned)(ncell * sizeof(T));  // bytes of one coefficient plane (host-checked)
     const DiagWalk walk0{R0, (int)(dm.base(R0) - dm.rlo(R0))};
 
-    if (tid >= cthreads) {
+    if (!DIRECT && tid >= cthreads) {
         // storer: barrier k (k >= 1) closes compute step k-1, whose cells are written here while
         // the compute waves fill the other ring slot; it also hands the band's top trace on and empties
         // the records this band has consumed
@@ -296,6 +297,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
 
     const unsigned long long nplanes = (unsigned long long)N * plane;
     const __amdgpu_buffer_rsrc_t rq = buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rout = buf_rsrc(DIRECT ? sp.psi + (size_t)q * N * ncell : nullptr, DIRECT ? nplanes : 0);
+    DiagWalk cw = walk0;  // diagonal of the current step (DIRECT stores)
     const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr, HASPHI ? nplanes : 0);
     const __amdgpu_buffer_rsrc_t rsigs = buf_rsrc(HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr, HASPHI ? plane : 0);
     const __amdgpu_buffer_rsrc_t rmat = buf_rsrc(MAT ? sp.mat + (size_t)Q * ncell : nullptr, MAT ? ncell * 4 : 0);
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
 
     // FP32 keeps the current material's inverse in registers and reloads it from LDS only when the
     // row enters another material (the LDS reads then move data only for the lanes that changed)
-    constexpr bool REGMAT = MAT && sizeof(T) == 4;
+    constexpr bool REGMAT = MAT && sizeof(T) == 4 && P <= 3;  // a Q3 inverse (256 values) would spill
     T Ainv[REGMAT ? N * N : 1];
     int cur_mat = -1;
     const T area = sp.dx * sp.dy;
@@ -437,7 +440,15 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
 #pragma unroll
                 for (int k = 0; k < N; ++k) x[k] = B[k];
             }
-            lds_put<T, N>(lout + ((size_t)(t & 1) * cthreads + rl) * NS, x);
+            if constexpr (DIRECT) {
+                if (active) {
+                    const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) buf_st<T>(rout, boff, k * plane, x[k]);
+                }
+            } else {
+                lds_put<T, N>(lout + ((size_t)(t & 1) * cthreads + rl) * NS, x);
+            }
             T txo[P], tyo[P];  // outflow face traces: sum_a s^a psi(a, .), s = +-1
 #pragma unroll
             for (int e = 0; e < P; ++e) {
@@ -457,6 +468,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             for (int d = 0; d < P; ++d) tx[d] = active ? txo[d] : tx[d];
             lds_put<T, P>(ex + ((size_t)(t & 1) * cthreads + rl) * PS, tyo);
             }
+            if (DIRECT) cw.next(nx, ny);
             // refill slot u with step t + D (scheduling barriers keep the slot in its registers)
             __builtin_amdgcn_sched_barrier(0);
             fetch(u);
@@ -476,8 +488,10 @@ template <typename T, int P, bool MAT, int MODE, int D>
 void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch,
                         const int2 *items, int nitems, int max_ny) {
     const int rows = batch ? max_ny : band_rows_of(sp);
-    const int threads = ((rows + 63) / 64) * 64 + 64;
-    auto kern = k_diag_sweep<T, P, MAT, MODE, D>;
+    const bool direct = MAT && sp.direct;
+    if (direct && !batch && sp.nbands > 1) throw Error(TSW_E_ARG, "direct-store sweep with row bands");
+    const int threads = ((rows + 63) / 64) * 64 + (direct ? 0 : 64);
+    auto kern = direct ? k_diag_sweep<T, P, MAT, MODE, D, MAT> : k_diag_sweep<T, P, MAT, MODE, D, false>;
     if (lds > 64 * 1024)
         TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int blocks = batch ? nitems : sp.nq * (sp.nbands > 1 ? sp.nbands : 1);

@@ -115,6 +115,8 @@ template <typename T> struct DiagSweepParams {
     T *xtr;
     int *abort_flag;
     int prefetch;          // prefetch depth in steps (0: default for the precision)
+    int direct;            // 1: no storer wave, the compute waves store psi (one band only; used when
+                           // only that LDS layout, without the store staging buffer, fits)
     const int *skip;       // device flag: the launch does nothing while *skip != 0 (null: always runs)
 };
 
