@@ -105,6 +105,7 @@ struct tsw_ctx {
     // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
     bool si_graph = true;
     int refine_steps = 1;  // FP64 refinement steps of a TSW_MIXED sweep
+    bool gmres_refine = false;  // mixed GMRES by iterative refinement instead of the relaxed strategy
     static constexpr int kSiChunk = 8;
     DBuf si_ctl;
     SiControl *si_ctl_host = nullptr;
@@ -560,6 +561,10 @@ void dsa_apply(tsw_ctx *c, const double *x, double *y) {
     launch_dsa_add(g, D.lv[0].y, y, c->stream);
 }
 
+// Relative error bound assumed for an FP32 sweep product in the relaxed mixed-precision GMRES (the
+// measured FP32 sweep error is 1e-6 relative or below on the lattice and on random problems).
+constexpr double kFp32SweepErr = 1e-5;
+
 struct Gmres {
     tsw_ctx *c;
     size_t n;
@@ -603,12 +608,23 @@ struct Gmres {
     template <typename Apply>
     bool solve(Apply &&apply, const double *b, double *x, double *r, double abs_tol, int max_its, int &its,
                double &res) {
+        return solve(apply, apply, -1.0, b, x, r, abs_tol, max_its, its, res);
+    }
+
+    // Inexact (relaxed) form: apply_lo, a cheaper and less accurate product, replaces apply once the
+    // residual estimate is <= lo_below.  A product error below ||b|| abs_tol / ||r_k|| at step k keeps the
+    // attainable residual at abs_tol (the relaxation strategy of inexact Krylov methods: Bouras and
+    // Fraysse 2005, Simoncini and Szyld 2003), so a fixed relative error d of apply_lo is admissible from
+    // ||r_k|| <= abs_tol / d on.
+    template <typename Apply, typename ApplyLo>
+    bool solve(Apply &&apply, ApplyLo &&apply_lo, double lo_below, const double *b, double *x, double *r,
+               double abs_tol, int max_its, int &its, double &res) {
         std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
         double *h1 = hdev, *h2 = hdev + (m + 2);
         its = 0;
         res = 0.0;
         while (true) {
-            apply(x, r);
+            apply(x, r);  // restart residuals with the accurate product
             launch_axpby(n, 1.0, b, -1.0, r, c->stream);  // r = b - A x
             const double beta = norm(r);
             res = beta;
@@ -621,7 +637,8 @@ struct Gmres {
             bool conv = false;
             for (int j = 0; j < m && its < max_its; ++j) {
                 double *vn = V + (size_t)(j + 1) * n;
-                apply(V + (size_t)j * n, vn);
+                if (res <= lo_below) apply_lo(V + (size_t)j * n, vn);
+                else apply(V + (size_t)j * n, vn);
                 ++its;
                 launch_multi_dot(n, j + 1, V, n, vn, h1, part, nb, c->stream);
                 launch_multi_axpy(n, j + 1, V, n, h1, vn, c->stream);
@@ -1156,6 +1173,31 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         const double *phi_d = diag_phi<double>(c, x);            // psi = L^{-1}(Q + S phi) (:504-505)
         diag_sweep<double>(c, dt, q_d, phi_d, true, psi_d);
         diag_store<double>(c, psi_d, Q);
+    } else if (!c->gmres_refine) {
+        // Mixed precision, relaxed: FP64 sweeps for b = P L^{-1} Q, the Krylov steps while the residual
+        // estimate exceeds abs_tol / kFp32SweepErr and the final psi; FP32 sweeps for the other steps.  The
+        // Krylov vectors, the orthogonalisation and the DSA correction stay FP64.
+        diag_setup(c, st, ss, Q, dt, true, true, false);
+        double *psi_d = dbuf<double>(c->diag.psi64, npsi_d);
+        float *psi32 = dbuf<float>(c->diag.psi32, npsi_d);
+        const double *q_d = c->diag.q64.as<double>();
+        auto apply64 = [&](const double *xv, double *yv) {
+            diag_sweep<double>(c, dt, nullptr, diag_phi<double>(c, xv), false, psi_d);
+            diag_pmv<double>(c, psi_d, yv, -1.0, 1.0, xv);
+            if (dsa) dsa_apply(c, yv, yv);
+        };
+        auto apply32 = [&](const double *xv, double *yv) {
+            diag_sweep<float>(c, dt, nullptr, diag_phi<float>(c, xv), false, psi32);
+            diag_pmv<float>(c, psi32, yv, -1.0, 1.0, xv);
+            if (dsa) dsa_apply(c, yv, yv);
+        };
+        diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);
+        diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
+        if (dsa) dsa_apply(c, b, b);
+        converged = solver.solve(apply64, apply32, abs_tol / kFp32SweepErr, b, x, r, abs_tol, max_its, its_total,
+                                 res_final);
+        diag_sweep<double>(c, dt, q_d, diag_phi<double>(c, x), true, psi_d);
+        diag_store<double>(c, psi_d, Q);
     } else {
         diag_setup(c, st, ss, Q, dt, true, true, false);  // FP32 sweeps of residuals: no FP32 source or inflow
         double *psi64 = dbuf<double>(c->diag.ir64, npsi_d);
@@ -1629,6 +1671,8 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
     ctx->adj->autotune = ctx->autotune;
     ctx->adj->use_dsa = ctx->use_dsa;
     ctx->adj->dsa_cycles = ctx->dsa_cycles;
+    ctx->adj->gmres_refine = ctx->gmres_refine;
+    ctx->adj->refine_steps = ctx->refine_steps;
     return ctx->adj.get();
 }
 
@@ -1900,6 +1944,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "si_graph") {
             require(value == 0 || value == 1, TSW_E_ARG, "si_graph must be 0 or 1");
             ctx->si_graph = value != 0;
+        } else if (k == "gmres_refine") {
+            require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
+            ctx->gmres_refine = value != 0;
         } else if (k == "refine_steps") {
             require(value >= 1 && value <= 8, TSW_E_ARG, "refine_steps must be 1..8");
             ctx->refine_steps = value;
@@ -1927,6 +1974,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "use_dsa") *value = ctx->use_dsa ? 1 : 0;
         else if (k == "dsa_cycles") *value = ctx->dsa_cycles;
         else if (k == "refine_steps") *value = ctx->refine_steps;
+        else if (k == "gmres_refine") *value = ctx->gmres_refine ? 1 : 0;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }

@@ -90,13 +90,14 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * vacuum Marshak faces) inverted approximately by "dsa_cycles" (1..16, default 1) multigrid V-cycles.
  * "si_graph" (1 default, 0): staggered source iteration replays captured graphs of 8 iterations with
  * the loop test on the device (same iterates and counts as the eager loop).  "refine_steps" (1..8,
- * default 1): FP64 refinement steps of a TSW_MIXED sweep.
+ * default 1): FP64 refinement steps of a TSW_MIXED sweep.  "gmres_refine" (0 default, 1): mixed-precision
+ * GMRES by iterative refinement instead of the relaxed strategy.
  * Unknown keys fail with TSW_E_ARG. */
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* Reads back "band_rows" (rows per band of the last staggered sweep, 0 = one band per ordinate),
  * "prefetch" (its prefetch depth), or the setting of "autotune", "use_dsa", "dsa_cycles", "si_graph",
- * "refine_steps". */
+ * "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
@@ -129,8 +130,11 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
  * source <- L^{-1}(Q + S phi): ImplicitSolver::Solve under solve_type "petsc-peierls-gmres"
  * (PETScPeierlsSolver.cpp:423-514, matvec IPLS :529-572), restarted GMRES(restart) with the
  * absolute residual test ||r||_2 <= abs_tol (:139-143).  phi_out (may be NULL): converged scalar
- * flux, phi layout.  precision TSW_FP64: FP64 sweeps.  TSW_FP32: FP32 sweeps inside an FP64
- * iterative refinement on the angular residual r = Q + S P psi - L psi (FP64 Lmv), repeated until
+ * flux, phi layout.  precision TSW_FP64: FP64 sweeps.  TSW_FP32 (staggered patterns): relaxed inexact
+ * GMRES, FP64 Krylov vectors and DSA with FP32 sweeps in the Arnoldi steps taken once the residual
+ * estimate is <= abs_tol / 1e-5 (FP64 sweeps for the right-hand side, earlier steps, restarts and the
+ * final psi); with tuning key "gmres_refine" = 1, and on the wavefront patterns, FP32 sweeps inside an
+ * FP64 iterative refinement on the angular residual r = Q + S P psi - L psi (FP64 Lmv), repeated until
  * ||P r||_2 <= abs_tol.  Returns TSW_E_NOCONV if max_its is reached. */
 int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
                     double abs_tol, int max_its, int restart, int precision, int mem, double *phi_out,

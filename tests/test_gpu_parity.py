@@ -193,16 +193,23 @@ def test_gmres_fp64_matches_oracle(ctx, p, pattern, materials):
 @pytest.mark.parametrize("dt", [math.inf, 0.1])
 @pytest.mark.parametrize("materials", [None, 3])
 @pytest.mark.parametrize("pattern", ["hip-wavefront", "hip-staggered"])
-def test_gmres_mixed_precision_refinement(ctx, pattern, materials, dt):
-    """FP32 sweeps inside FP64 refinement; with finite dt the refinement residual must carry 1/dt."""
+@pytest.mark.parametrize("refine", [0, 1])
+def test_gmres_mixed_precision_refinement(ctx, pattern, materials, dt, refine):
+    """Mixed-precision GMRES: the relaxed strategy (FP32 sweeps in the late Arnoldi steps) and FP32 sweeps
+    inside FP64 refinement (gmres_refine; with finite dt its residual must carry 1/dt)."""
     o = ls_ordinates(6)
     g, st, ss, src = random_problem(12, 12, 2, o, seed=50, materials=materials)
     setup(ctx, g, o)
     ctx.set_pattern(pattern)
+    ctx.set_tuning("gmres_refine", refine)
+    assert ctx.get_tuning("gmres_refine") == refine
     rc, ref_psi, ref_phi, _, _ = orc_gmres(g, o, st, ss, src, dt=dt, abs_tol=1e-13, restart=30)
     out = src.copy()
     phi = np.zeros(g.phi_shape())
     its, res = ctx.gmres_solve(out, st, ss, phi_out=phi, dt=dt, abs_tol=1e-10, restart=30, precision=tsw.FP32)
+    ctx.set_tuning("gmres_refine", 0)
+    print(f"mixed gmres {pattern} refine={refine}: its {its} res {res:.3e} phi err {rel_err(phi, ref_phi):.3e}",
+          flush=True)
     assert res <= 1e-10
     assert rel_err(phi, ref_phi) <= 1e-8
     assert rel_err(out, ref_psi) <= 1e-8
