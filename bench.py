@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
-"""Benchmark: sweep-preconditioned GMRES(30) on the 280x280 Q2 S12 checkerboard lattice, FP64 sweeps
-versus FP32 sweeps inside FP64 iterative refinement, both to the absolute residual 1e-8.
+"""Benchmark: sweep-preconditioned GMRES(30) with DSA on the 280x280 Q2 S12 checkerboard lattice, FP64
+sweeps versus mixed precision (FP64 Krylov vectors, DSA and residual estimates; FP32 sweeps in the Arnoldi
+steps once the residual estimate admits their error), both to the absolute residual 1e-8.
 
 One step = one full transport solve of the lattice (inputs resident in HBM).  `value` is the
 mixed-precision solve's angular unknowns swept per second (unknowns per sweep x sweeps / wall time).
@@ -287,19 +288,19 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
-def traffic_from_profiles(kernel, band_rows=None):
-    """HBM bytes per launch of `kernel` (run with `band_rows`) from the committed rocprofv3 --pmc summary,
-    if one was measured for that configuration."""
+def traffic_from_profiles(family):
+    """HBM bytes per launch of the kernel family `family` (the sweep template without its prefetch depth
+    and store variant, which move the same data) from the committed rocprofv3 --pmc summary, with the
+    configuration it was measured in; None when no summary exists."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            data = json.load(f)
-        entry = data.get(kernel, {})
-        if band_rows is not None and entry.get("band_rows", band_rows) != band_rows:
-            return None
-        return entry.get("hbm_bytes_per_launch")
+            entry = json.load(f).get(family)
     except (OSError, ValueError):
-        return None
+        return None, None
+    if not entry:
+        return None, None
+    return entry.get("hbm_bytes_per_launch"), entry.get("measured_as")
 
 
 def main():
@@ -440,6 +441,7 @@ def main():
     achieved = bytes32 / (t_sw32 * 1e-3) / 1e9
     achieved64 = bytes64 / (t_sw64 * 1e-3) / 1e9
 
+    traffic32, traffic32_as = traffic_from_profiles("k_diag_sweep<float, 3, true, 1>")
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
         "value": value,
@@ -451,10 +453,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweep / f64 refinement",
+        "dtype": "f32 sweep / f64 Krylov",
         "data": "synthetic (lattice cross sections and source generated in-process)",
         "config": {"workload": "checkerboard lattice 280x280 cells, Q2 DG, S12 (84 ordinates), Peierls GMRES(30) "
-                               "with DSA to abs residual 1e-8, FP32 sweeps + FP64 iterative refinement",
+                               "with DSA to abs residual 1e-8; mixed = FP32 sweeps in the Arnoldi steps once the FP64 residual "
+                               "estimate is <= 1e-3, FP64 sweeps otherwise, FP64 Krylov vectors and DSA",
                    "sweep_pattern": args.pattern, "band_rows": args.band_rows, "prefetch": args.prefetch,
                    "tuned_band_rows_prefetch": {"fp32": tuned32, "fp64": tuned64}, "dsa": bool(args.dsa),
                    "dsa_cycles": args.dsa_cycles,
@@ -470,11 +473,11 @@ def main():
         "mixed_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}> (FP32 in-place operator-contract "
-                               f"sweep, band_rows {tuned32[0]})",
+                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, false> (FP32 in-place "
+                               f"operator-contract sweep, band_rows {tuned32[0]})",
                      "algorithmic_bytes_per_launch": bytes32,
-                     "traffic": traffic_from_profiles(f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}>",
-                                                      tuned32[0])},
+                     "traffic": traffic32,
+                     "traffic_measured_as": traffic32_as},
     }
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(P, args.cpu_sweeps)
