@@ -97,9 +97,10 @@ struct tsw_ctx {
     // (band rows, prefetch) pairs and keeps the fastest; tuned_br/tuned_pf hold the choice in force
     // (INT_MIN: none), last_br/last_pf what the last sweep used.
     bool autotune = true;
-    std::map<std::array<int, 7>, std::pair<int, int>> tuned;
-    int tuned_br = INT_MIN, tuned_pf = INT_MIN;
-    int last_br = 0, last_pf = 0;
+    std::map<std::array<int, 7>, std::array<int, 3>> tuned;  // (band rows, prefetch, one wave)
+    int tuned_br = INT_MIN, tuned_pf = INT_MIN, tuned_w1 = INT_MIN;
+    int last_br = 0, last_pf = 0, last_w1 = 0;
+    int one_wave_opt = 0;  // 1: one wave per band of 64 rows, lane-to-lane traces (tuning key one_wave)
     DBuf tune_out;
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
     // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
@@ -758,8 +759,11 @@ const T *diag_inverses(tsw_ctx *c, double dt) {
 // Rows per band for this context's mesh: 0 when each ordinate is one workgroup.  Automatic choice:
 // one wavefront of rows per band when the mesh has more rows than that, so the ordinates' sweeps
 // spread over several times more compute units than there are ordinates.
+bool one_wave_for(const tsw_ctx *c) { return (c->tuned_w1 != INT_MIN ? c->tuned_w1 : c->one_wave_opt) == 1; }
+
 int band_rows_for(const tsw_ctx *c) {
     const int ny = c->g.ny;
+    if (one_wave_for(c)) return ny > 64 ? 64 : 0;  // the one-wave kernel takes bands of one wavefront
     int br = c->tuned_br != INT_MIN ? c->tuned_br : c->band_rows_opt;
     if (br < 0) br = ny > 448 ? 448 : 0;  // a single band cannot hold more than 448 rows
     else if (br == 0) br = ny > 64 ? 64 : 0;
@@ -822,6 +826,8 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
     }
     configure_bands<T>(c, sp);
     sp.prefetch = c->tuned_pf != INT_MIN ? c->tuned_pf : c->prefetch_opt;
+    sp.wave1 = one_wave_for(c) && (sp.nbands > 1 ? sp.band_rows : sp.ny) <= 64 ? 1 : 0;
+    c->last_w1 = sp.wave1;
     c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
     c->last_pf = sp.prefetch > 0 ? sp.prefetch : (sizeof(T) == 8 ? 2 : 3);
     sp.ainv = diag_inverses<T>(c, dt);
@@ -829,7 +835,7 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
         sp.M = c->mat.M;
-        if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // without the store staging buffer (one band)
+        if (!sp.wave1 && diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // without the store staging buffer (one band)
             sp.direct = 1;
             if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // table too large: eliminate per cell
                 sp.direct = 0;
@@ -848,8 +854,8 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
 // the same result, so the choice only affects speed.
 template <typename T>
 void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow) {
-    c->tuned_br = c->tuned_pf = INT_MIN;
-    if (!c->autotune || c->band_rows_opt != 0 || c->prefetch_opt != 0) return;
+    c->tuned_br = c->tuned_pf = c->tuned_w1 = INT_MIN;
+    if (!c->autotune || c->band_rows_opt != 0 || c->prefetch_opt != 0 || c->one_wave_opt != 0) return;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     TSW_HIP(hipStreamIsCapturing(c->stream, &cap));
     const Geom &g = c->g;
@@ -866,37 +872,40 @@ void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool in
         TSW_HIP(hipEventCreate(&e0));
         TSW_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        std::pair<int, int> choice{0, 0};
+        std::array<int, 3> choice{0, 0, 0};
         try {
             for (int br : brs)
-                for (int pf : pfs) {
-                    c->tuned_br = br;
-                    c->tuned_pf = pf;
-                    const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, out);
-                    launch_diag_sweep<T>(sp, g.p, c->stream);
-                    TSW_HIP(hipEventRecord(e0, c->stream));
-                    launch_diag_sweep<T>(sp, g.p, c->stream);
-                    TSW_HIP(hipEventRecord(e1, c->stream));
-                    TSW_HIP(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    TSW_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (ms < best) {
-                        best = ms;
-                        choice = {br, pf};
+                for (int pf : pfs)
+                    for (int w1 = 0; w1 <= ((br == 64 || (br == -1 && g.ny <= 64)) ? 1 : 0); ++w1) {
+                        c->tuned_br = br;
+                        c->tuned_pf = pf;
+                        c->tuned_w1 = w1;
+                        const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, out);
+                        launch_diag_sweep<T>(sp, g.p, c->stream);
+                        TSW_HIP(hipEventRecord(e0, c->stream));
+                        launch_diag_sweep<T>(sp, g.p, c->stream);
+                        TSW_HIP(hipEventRecord(e1, c->stream));
+                        TSW_HIP(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        TSW_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (ms < best) {
+                            best = ms;
+                            choice = {br, pf, w1};
+                        }
                     }
-                }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            c->tuned_br = c->tuned_pf = INT_MIN;
+            c->tuned_br = c->tuned_pf = c->tuned_w1 = INT_MIN;
             throw;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         it = c->tuned.emplace(key, choice).first;
     }
-    c->tuned_br = it->second.first;
-    c->tuned_pf = it->second.second;
+    c->tuned_br = it->second[0];
+    c->tuned_pf = it->second[1];
+    c->tuned_w1 = it->second[2];
 }
 
 // One staggered sweep on the context's stream, timed.
@@ -1672,6 +1681,7 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
     ctx->adj->use_dsa = ctx->use_dsa;
     ctx->adj->dsa_cycles = ctx->dsa_cycles;
     ctx->adj->gmres_refine = ctx->gmres_refine;
+    ctx->adj->one_wave_opt = ctx->one_wave_opt;
     ctx->adj->refine_steps = ctx->refine_steps;
     return ctx->adj.get();
 }
@@ -1944,6 +1954,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "si_graph") {
             require(value == 0 || value == 1, TSW_E_ARG, "si_graph must be 0 or 1");
             ctx->si_graph = value != 0;
+        } else if (k == "one_wave") {
+            require(value == 0 || value == 1, TSW_E_ARG, "one_wave must be 0 or 1");
+            ctx->one_wave_opt = value;
         } else if (k == "gmres_refine") {
             require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
             ctx->gmres_refine = value != 0;
@@ -1975,6 +1988,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "dsa_cycles") *value = ctx->dsa_cycles;
         else if (k == "refine_steps") *value = ctx->refine_steps;
         else if (k == "gmres_refine") *value = ctx->gmres_refine ? 1 : 0;
+        else if (k == "one_wave") *value = ctx->last_w1;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }

@@ -22,10 +22,11 @@ size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     if (BR > 448) return 0;
     // the kernel addresses an ordinate's coefficient planes with 32-bit byte offsets
     if ((long long)sp.nx * sp.ny * (p + 1) * (p + 1) * (long long)sizeof(T) >= (1ll << 32)) return 0;
+    if (sp.wave1 && BR > 64) return 0;
     const int cthreads = ((BR + 63) / 64) * 64;
     const int M = sp.ainv ? sp.M : 0;
-    const bool direct = sp.ainv && sp.direct;
-    if (direct && sp.nbands > 1) return 0;
+    const bool direct = (sp.ainv && sp.direct) || sp.wave1;
+    if (sp.ainv && sp.direct && sp.nbands > 1) return 0;
     size_t total;
     switch (p) {
         case 0: total = DiagLds<T, 1>(cthreads, sp.nx, M, direct).total; break;

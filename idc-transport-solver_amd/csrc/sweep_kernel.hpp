@@ -194,12 +194,28 @@ struct DiagWalk {
 
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
 // Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
-// upstream neighbour is dispatched before it and never waits on it).  DIRECT (one band only): no storer
-// wave; each compute lane stores its own cell, which frees the LDS staging buffer for larger tables.
-template <typename T, int P, bool MAT, int MODE, int D, bool DIRECT>
+// upstream neighbour is dispatched before it and never waits on it).  Store variant ST: 0, a storer wave
+// writes the cells staged in LDS; 1 (DIRECT, one band only), no storer wave, each compute lane stores its
+// own cell, which frees the LDS staging buffer for larger tables; 2 (one wave per band of at most 64
+// rows), the compute lanes store their cells, a row takes the y-trace of the row below from the
+// neighbouring lane (a wave shift) instead of LDS, and no workgroup barrier is needed.
+template <typename T>
+__device__ __forceinline__ T lane_up(T v) {  // lane i receives lane i-1's v, lane 0 receives 0
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x138, 0xf, 0xf, false);
+        const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
+template <typename T, int P, bool MAT, int MODE, int D, int ST>
 __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
+    constexpr bool DIRECT = ST != 0, W1 = ST == 2;
     using Lds = DiagLds<T, P>;
     constexpr int PS = Lds::PS, NS = Lds::NS, MS = Lds::MS;
     using U = typename ExchangeSentinel<T>::U;
@@ -353,6 +369,9 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     constexpr bool REGMAT = MAT && sizeof(T) == 4 && P <= 3;  // a Q3 inverse (256 values) would spill
     T Ainv[REGMAT ? N * N : 1];
     int cur_mat = -1;
+    T typrev[W1 ? P : 1];  // W1: this row's outflow y-trace of the previous step
+#pragma unroll
+    for (int d = 0; d < (W1 ? P : 1); ++d) typrev[d] = T(0);
     const T area = sp.dx * sp.dy;
     // B(d,e) = area src(d,e) + kx(d) Tx(e) + ky(e) Ty(d), the face-trace form of the reference's CalcB
     // (cell_solve.hpp: rhs_from_traces); kx, ky are wave-uniform.  The material path solves
@@ -388,8 +407,24 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             if (live) {
                 T ty[P];
                 const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
-                const T *tsrc = rl == 0 ? lin + (size_t)cc * PS : ex + ((size_t)((t - 1) & 1) * cthreads + (rl - 1)) * PS;
-                lds_get<T, P>(tsrc, ty);
+                if constexpr (W1) {  // the row below produced it in the previous step, one lane down
+                    T tl[P];
+                    lds_get<T, P>(lin + (size_t)cc * PS, tl);
+#pragma unroll
+                    for (int d = 0; d < P; ++d) {
+                        ty[d] = lane_up(typrev[d]);
+                        ty[d] = rl == 0 ? tl[d] : ty[d];
+                    }
+                } else {
+                    const T *tsrc = rl == 0 ? lin + (size_t)cc * PS : ex + ((size_t)((t - 1) & 1) * cthreads + (rl - 1)) * PS;
+                    lds_get<T, P>(tsrc, ty);
+                }
+                if constexpr (REGMAT) {  // a material change reloads the inverse now, behind the work on B
+                    if (pm[u] != cur_mat) {
+                        cur_mat = pm[u];
+                        lds_get<T, REGMAT ? N * N : 1>(linv + (size_t)cur_mat * MS, Ainv);
+                    }
+                }
                 if (xwave) {
                     if (xrow && active) {
                         bool ok = true;
@@ -420,10 +455,6 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             const T sig = pm_sig[u];
             T x[N];
             if constexpr (REGMAT) {
-                if (mat != cur_mat) {
-                    cur_mat = mat;
-                    lds_get<T, N * N>(linv + (size_t)mat * MS, Ainv);
-                }
 #pragma unroll
                 for (int row = 0; row < N; ++row) {
                     T acc = Ainv[row * N] * B[0];
@@ -466,14 +497,29 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             }
 #pragma unroll
             for (int d = 0; d < P; ++d) tx[d] = active ? txo[d] : tx[d];
-            lds_put<T, P>(ex + ((size_t)(t & 1) * cthreads + rl) * PS, tyo);
+            if constexpr (W1) {
+#pragma unroll
+                for (int d = 0; d < P; ++d) typrev[d] = tyo[d];
+                // the band's top row hands its trace to the band above; the first row empties the record
+                // it consumed (the records hold the sentinel again for the next sweep)
+                if (xout && rl == nrows - 1 && active) {
+#pragma unroll
+                    for (int d = 0; d < P; ++d) xstore(xout + (size_t)c * P + d, to_bits(tyo[d]));
+                }
+                if (xrow && active) {
+#pragma unroll
+                    for (int d = 0; d < P; ++d) xstore(xin + (size_t)c * P + d, ExchangeSentinel<T>::bits);
+                }
+            } else {
+                lds_put<T, P>(ex + ((size_t)(t & 1) * cthreads + rl) * PS, tyo);
+            }
             }
             if (DIRECT) cw.next(nx, ny);
             // refill slot u with step t + D (scheduling barriers keep the slot in its registers)
             __builtin_amdgcn_sched_barrier(0);
             fetch(u);
             __builtin_amdgcn_sched_barrier(0);
-            lds_barrier();
+            if constexpr (!W1) lds_barrier();
         }
     }
 }
@@ -488,10 +534,12 @@ template <typename T, int P, bool MAT, int MODE, int D>
 void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch,
                         const int2 *items, int nitems, int max_ny) {
     const int rows = batch ? max_ny : band_rows_of(sp);
-    const bool direct = MAT && sp.direct;
+    const bool direct = MAT && sp.direct, wave1 = sp.wave1 != 0;
     if (direct && !batch && sp.nbands > 1) throw Error(TSW_E_ARG, "direct-store sweep with row bands");
-    const int threads = ((rows + 63) / 64) * 64 + (direct ? 0 : 64);
-    auto kern = direct ? k_diag_sweep<T, P, MAT, MODE, D, MAT> : k_diag_sweep<T, P, MAT, MODE, D, false>;
+    if (wave1 && (batch || rows > 64)) throw Error(TSW_E_ARG, "one-wave sweep needs bands of at most 64 rows");
+    const int threads = wave1 ? 64 : ((rows + 63) / 64) * 64 + (direct ? 0 : 64);
+    auto kern = wave1 ? k_diag_sweep<T, P, MAT, MODE, D, 2>
+                      : (direct ? k_diag_sweep<T, P, MAT, MODE, D, MAT ? 1 : 0> : k_diag_sweep<T, P, MAT, MODE, D, 0>);
     if (lds > 64 * 1024)
         TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int blocks = batch ? nitems : sp.nq * (sp.nbands > 1 ? sp.nbands : 1);

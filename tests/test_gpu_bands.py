@@ -41,6 +41,30 @@ def test_banded_sweep_matches_oracle(band_rows, p, pattern):
                     np.testing.assert_array_equal(out[:, 0], src[:, 0])
 
 
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [(37, 150), (1, 9), (12, 64), (5, 65), (70, 3), (1, 200)])
+@pytest.mark.parametrize("materials", [5, None])
+def test_one_wave_sweep_matches_oracle(shape, p, materials):
+    """One wavefront per band of 64 rows ("one_wave"): rows pass y-traces lane to lane and bands through
+    device memory; FP64, FP32 and repeated sweeps (the emptied records are reused), steady and dt."""
+    o = ls_ordinates(6)
+    g, st, _, src = random_problem(shape[0], shape[1], p, o, seed=700 + p, ax=0.0, bx=2.0, by=3.0,
+                                   materials=materials)
+    print(f"one wave {shape} p={p} materials={materials}", flush=True)
+    with _ctx(g, o, 0) as c:
+        c.set_tuning("one_wave", 1)
+        for dt in (math.inf, 0.3):
+            ref = orc_sweep(g, o, st, src, dt)
+            for prec, tol in ((tsw.FP64, 1e-12), (tsw.FP32, 2e-6)):
+                for _ in range(2):
+                    out = src.copy()
+                    c.sweep(out, st, dt, precision=prec)
+                    assert c.get_tuning("one_wave") == 1
+                    assert c.get_tuning("band_rows") == (64 if g.ny > 64 else 0)
+                    assert rel_err(out, ref) <= tol, (dt, prec)
+                    np.testing.assert_array_equal(out[0], src[0])
+
+
 @pytest.mark.parametrize("shape", [(1, 200), (2, 129), (300, 65), (90, 448), (60, 500)])
 def test_banded_sweep_shapes(shape):
     nx, ny = shape

@@ -64,6 +64,27 @@ def test_sweep_material_table_matches_oracle(ctx, p, materials, precision):
         assert rel_err(out, ref) <= (1e-12 if precision == tsw.FP64 else 2e-6)
 
 
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
+def test_sweep_direct_store_layout(ctx, precision):
+    """Q3 with 64 materials and 200 rows in one band per ordinate: the FP64 inverse table (128 KB) fits in
+    LDS only without the storer's staging buffer, so the direct-store kernel runs; against the oracle."""
+    o = ls_ordinates(4)
+    g, st, ss, src = random_problem(40, 200, 3, o, seed=70, materials=64)
+    setup(ctx, g, o)
+    ctx.set_pattern("hip-staggered")
+    ctx.set_tuning("band_rows", -1)
+    try:
+        ref = orc_sweep(g, o, st, src, 0.5)
+        out = src.copy()
+        ctx.sweep(out, st, 0.5, precision=precision)
+        assert ctx.get_tuning("band_rows") == 0
+    finally:
+        ctx.set_tuning("band_rows", 0)
+    e = rel_err(out, ref)
+    print(f"direct-store layout precision {precision}: rel err {e:.3e}", flush=True)
+    assert e <= {tsw.FP64: 1e-12, tsw.FP32: 2e-6, tsw.MIXED: 1e-10}[precision]
+
+
 @pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (12, 12), (3, 70), (70, 3)])
 def test_sweep_degenerate_shapes(ctx, shape, pattern):
