@@ -18,6 +18,7 @@
 #include <cmath>
 #include <map>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "tsw.h"
@@ -155,6 +156,247 @@ extern "C" int tsw_ordinates_level_symmetric(int order, double *xi, double *eta,
             }
         }
         quad_off[4] = q;
+        return TSW_OK;
+    } catch (...) {
+        return TSW_E_ARG;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Gauss-Legendre rules and the reference's named ordinate sets.
+namespace {
+
+// Legendre P_n and P_n' at x by the three-term recurrence, in quadruple precision (__float128,
+// software arithmetic on the host).
+using quad_t = __float128;
+void legendre_q(int n, quad_t x, quad_t &p, quad_t &dp) {
+    quad_t p0 = 1, p1 = x;
+    if (n == 0) {
+        p = 1;
+        dp = 0;
+        return;
+    }
+    for (int k = 2; k <= n; ++k) {
+        const quad_t pk = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = pk;
+    }
+    p = p1;
+    dp = n * (x * p1 - p0) / (x * x - 1);
+}
+
+// Nodes ascending, exactly antisymmetric (x[n-1-i] = -x[i], middle node 0), and weights; Newton in
+// quadruple precision from the Tricomi guess, rounded once to double.  The reference tabulates these
+// rules to 25 digits (Tabulated.cpp:29-1092); rounding the quadruple-precision root gives the same
+// doubles, the correctly rounded roots.
+void gauss_legendre(int n, double *x, double *w) {
+    for (int i = 0; i < n / 2; ++i) {
+        quad_t r = std::cos(M_PI * (i + 0.75) / (n + 0.5)), p, dp;
+        for (int it = 0; it < 100; ++it) {
+            legendre_q(n, r, p, dp);
+            const quad_t dx = p / dp;
+            r -= dx;
+            if (std::fabs((double)dx) < 1e-30) break;
+        }
+        legendre_q(n, r, p, dp);
+        const double xr = (double)r, wr = (double)(2 / ((1 - r * r) * dp * dp));
+        x[n - 1 - i] = xr;
+        x[i] = -xr;
+        w[n - 1 - i] = w[i] = wr;
+    }
+    if (n % 2) {
+        quad_t p, dp;
+        legendre_q(n, 0, p, dp);
+        x[n / 2] = 0.0;
+        w[n / 2] = (double)(2 / (dp * dp));
+    }
+}
+
+struct Dir {
+    double xi, eta, mu, w;
+};
+
+// Octant code (xi > 0, eta > 0, mu > 0) as bits 2..0, the reference's labelling of octants
+// (OrdinateSet.cpp:878-939) for directions with no zero component.
+unsigned octant_of(const Dir &d) { return (d.xi > 0) << 2 | (d.eta > 0) << 1 | (d.mu > 0); }
+
+// Chebyshev-Legendre product set (OrdinateSet.cpp:441-488): GL nodes in mu, 2N equally spaced azimuths
+// phi = (k + 1/2) pi / N, weight pi/N * w_GL; ordered by descending (octant, xi, eta, mu), which is what
+// the reference's heap sort with a min-heap produces (:1065-1090).
+std::vector<Dir> chebyshev_legendre(int n) {
+    std::vector<double> gx(n), gw(n);
+    gauss_legendre(n, gx.data(), gw.data());
+    std::vector<Dir> d;
+    for (int a = 0; a < n; ++a)
+        for (int k = 0; k < 2 * n; ++k) {
+            const double phi = (k + 0.5) * M_PI / n, mu = gx[a], s = std::sqrt(1.0 - mu * mu);
+            d.push_back({s * std::cos(phi), s * std::sin(phi), mu, M_PI / n * gw[a]});
+        }
+    std::sort(d.begin(), d.end(), [](const Dir &a, const Dir &b) {
+        const unsigned oa = octant_of(a), ob = octant_of(b);
+        if (oa != ob) return oa > ob;
+        if (a.xi != b.xi) return a.xi > b.xi;
+        if (a.eta != b.eta) return a.eta > b.eta;
+        return a.mu > b.mu;
+    });
+    return d;
+}
+
+// T_N spherical-triangle set (SphericalTriangle.cpp:268-351): the octant face x + y + z = 1, written in
+// coordinates (u, v) with x = 1/2 - u - v/2, y = 1/2 + u - v/2, z = v, is cut into N^2 triangles; each
+// gives the direction of its centroid and the solid angle of the spherical triangle on its vertices.
+struct Tri {
+    double u[3], v[3];
+};
+
+void xyz_of(double u, double v, double out[3]) {
+    const double x = 0.5 - u - 0.5 * v, y = 0.5 + u - 0.5 * v, z = v;
+    const double r = std::sqrt(x * x + y * y + z * z);
+    out[0] = x / r;
+    out[1] = y / r;
+    out[2] = z / r;
+}
+
+// Solid angle as the spherical excess A + B + C - pi, the angles from the half-angle formulas of
+// spherical trigonometry (tan(A/2) = tan(r) / sin(s - a), tan(r) the tangent of the inradius).
+double solid_angle(const Tri &t) {
+    double p[3][3];
+    for (int k = 0; k < 3; ++k) xyz_of(t.u[k], t.v[k], p[k]);
+    auto arc = [&](int i, int j) { return std::acos(p[i][0] * p[j][0] + p[i][1] * p[j][1] + p[i][2] * p[j][2]); };
+    const double a = arc(1, 2), b = arc(0, 2), c = arc(0, 1), s = 0.5 * (a + b + c);
+    const double tr = std::sqrt(std::sin(s - a) * std::sin(s - b) * std::sin(s - c) / std::sin(s));
+    return 2.0 * std::atan(tr / std::sin(s - a)) + 2.0 * std::atan(tr / std::sin(s - b)) +
+           2.0 * std::atan(tr / std::sin(s - c)) - M_PI;
+}
+
+// Power-of-two N: the base triangle split into four by its edge midpoints, recursively, children in
+// the order (v1, m12, m13), (m12, v2, m23), (m23, m12, m13), (m13, m23, v3) (:180-230).
+void subdivide(const Tri &t, int level, std::vector<Tri> &out) {
+    if (level == 0) {
+        out.push_back(t);
+        return;
+    }
+    const double m12u = 0.5 * (t.u[0] + t.u[1]), m12v = 0.5 * (t.v[0] + t.v[1]);
+    const double m13u = 0.5 * (t.u[0] + t.u[2]), m13v = 0.5 * (t.v[0] + t.v[2]);
+    const double m23u = 0.5 * (t.u[1] + t.u[2]), m23v = 0.5 * (t.v[1] + t.v[2]);
+    subdivide({{t.u[0], m12u, m13u}, {t.v[0], m12v, m13v}}, level >> 1, out);
+    subdivide({{m12u, t.u[1], m23u}, {m12v, t.v[1], m23v}}, level >> 1, out);
+    subdivide({{m23u, m12u, m13u}, {m23v, m12v, m13v}}, level >> 1, out);
+    subdivide({{m13u, m23u, t.u[2]}, {m13v, m23v, t.v[2]}}, level >> 1, out);
+}
+
+std::vector<Dir> spherical_triangle(int n) {
+    std::vector<Tri> tris;
+    if ((n & (n - 1)) == 0) {
+        subdivide({{0.0, -0.5, 0.5}, {1.0, 0.0, 0.0}}, n >> 1, tris);
+    } else {
+        // rows r = 0..N-1 from the apex, 2r + 1 triangles each, alternately pointing up and down
+        const double h = 1.0 / n;
+        for (int r = 0; r < n; ++r)
+            for (int k = 0; k < 2 * r + 1; ++k) {
+                const double top = 1.0 - r * h, bot = 1.0 - (r + 1) * h;
+                if (k % 2)
+                    tris.push_back({{(double)(k - r + 1) / (2 * n), (double)(k - r - 1) / (2 * n),
+                                     (double)(k - r) / (2 * n)},
+                                    {top, top, bot}});
+                else
+                    tris.push_back({{(double)(k - r) / (2 * n), (double)(k - r - 1) / (2 * n),
+                                     (double)(k - r + 1) / (2 * n)},
+                                    {top, bot, bot}});
+            }
+    }
+    const size_t m = tris.size();
+    std::vector<Dir> d(8 * m);
+    for (size_t q = 0; q < m; ++q) {
+        const Tri &t = tris[q];
+        double c[3];
+        xyz_of((t.u[0] + t.u[1] + t.u[2]) / 3.0, (t.v[0] + t.v[1] + t.v[2]) / 3.0, c);
+        d[q] = {c[0], c[1], c[2], solid_angle(t)};
+    }
+    // the other octants by reflection: mu, then eta, then xi (FillByOctahedralReflection, :790-815)
+    for (size_t q = 0; q < m; ++q) d[m + q] = {d[q].xi, d[q].eta, -d[q].mu, d[q].w};
+    for (size_t q = 0; q < 2 * m; ++q) d[2 * m + q] = {d[q].xi, -d[q].eta, d[q].mu, d[q].w};
+    for (size_t q = 0; q < 4 * m; ++q) d[4 * m + q] = {-d[q].xi, d[q].eta, d[q].mu, d[q].w};
+    return d;
+}
+
+// 2D reduction (SymmetricReduce, OrdinateSet.cpp:1171-1190): each direction absorbs the weight of the
+// first later direction with the same (xi, eta) and opposite mu, which is then removed.
+void symmetric_reduce(std::vector<Dir> &d) {
+    for (size_t i = 0; i < d.size(); ++i)
+        for (size_t j = i + 1; j < d.size(); ++j)
+            if (d[i].xi == d[j].xi && d[i].eta == d[j].eta && d[i].mu == -d[j].mu) {
+                d[i].w += d[j].w;
+                d.erase(d.begin() + (ptrdiff_t)j);
+                break;
+            }
+}
+
+}  // namespace
+
+extern "C" int tsw_gauss_legendre(int n, double *nodes, double *weights) {
+    if (n < 1 || n > 4096 || !nodes || !weights) return TSW_E_ARG;
+    gauss_legendre(n, nodes, weights);
+    return TSW_OK;
+}
+
+extern "C" int tsw_ordinates(const char *type, int order, int symmetric_reduce, int capacity, int *nq, double *xi,
+                             double *eta, double *w, int *quad_off) {
+    if (!type || !nq || order < 1 || order > 256) return TSW_E_ARG;
+    try {
+        const std::string t(type);
+        std::vector<Dir> d;
+        if (t == "level-symmetric") {
+            const int n = order * (order + 2) / 2;
+            std::vector<double> a(n), b(n), c(n);
+            int off[5];
+            if (order % 2 || tsw_ordinates_level_symmetric(order, a.data(), b.data(), c.data(), off) != TSW_OK)
+                return TSW_E_ARG;
+            *nq = n;
+            if (xi) {
+                if (!eta || !w || !quad_off || capacity < n) return TSW_E_ARG;
+                std::copy(a.begin(), a.end(), xi);
+                std::copy(b.begin(), b.end(), eta);
+                std::copy(c.begin(), c.end(), w);
+                std::copy(off, off + 5, quad_off);
+            }
+            return TSW_OK;
+        } else if (t == "chebyshev-legendre") {
+            // odd N has a node at mu = 0, whose octant the reference takes from an azimuth shifted by pi
+            // (OrdinateSet.cpp:466,893-905), so only even N is accepted
+            if (order % 2) return TSW_E_ARG;
+            d = chebyshev_legendre(order);
+        } else if (t == "spherical-triangle") {
+            d = spherical_triangle(order);
+        } else {
+            return TSW_E_ARG;
+        }
+        if (symmetric_reduce) ::symmetric_reduce(d);
+        // octants must come in the reference's order 111, 110, ..., 000; quadrant k of the 2D sweep is the
+        // pair of octants with the same (xi, eta) signs
+        int off[5] = {0, 0, 0, 0, 0};
+        unsigned prev = 8;
+        for (size_t q = 0; q < d.size(); ++q) {
+            const unsigned o = octant_of(d[q]);
+            if (o > prev) return TSW_E_ARG;
+            prev = o;
+        }
+        for (int quad = 0; quad < 4; ++quad) {
+            int q = 0;
+            while (q < (int)d.size() && (int)(7 - octant_of(d[q])) / 2 < quad) ++q;
+            off[quad] = q;
+        }
+        off[4] = (int)d.size();
+        *nq = (int)d.size();
+        if (xi) {
+            if (!eta || !w || !quad_off || capacity < *nq) return TSW_E_ARG;
+            for (size_t q = 0; q < d.size(); ++q) {
+                xi[q] = d[q].xi;
+                eta[q] = d[q].eta;
+                w[q] = d[q].w;
+            }
+            std::copy(off, off + 5, quad_off);
+        }
         return TSW_OK;
     } catch (...) {
         return TSW_E_ARG;

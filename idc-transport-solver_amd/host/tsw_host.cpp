@@ -37,19 +37,32 @@ OrdinateSet OrdinateSet::LevelSymmetric(int order) {
     std::array<int, 5> off{};
     if (tsw_ordinates_level_symmetric(order, xi.data(), eta.data(), w.data(), off.data()) != TSW_OK)
         throw std::invalid_argument("no level-symmetric set of order " + std::to_string(order));
-    return OrdinateSet(xi, eta, w, off);
+    return OrdinateSet(xi, eta, w, off, "level-symmetric", order, true);
+}
+
+OrdinateSet OrdinateSet::Named(const std::string &type, int order, bool symmetric_reduce) {
+    int nq = 0;
+    if (tsw_ordinates(type.c_str(), order, symmetric_reduce, 0, &nq, nullptr, nullptr, nullptr, nullptr) != TSW_OK)
+        throw std::invalid_argument("no ordinate set '" + type + "' of order " + std::to_string(order));
+    std::vector<double> xi(nq), eta(nq), w(nq);
+    std::array<int, 5> off{};
+    if (tsw_ordinates(type.c_str(), order, symmetric_reduce, nq, &nq, xi.data(), eta.data(), w.data(), off.data()) !=
+        TSW_OK)
+        throw std::invalid_argument("no ordinate set '" + type + "' of order " + std::to_string(order));
+    return OrdinateSet(xi, eta, w, off, type, order, type == "level-symmetric" ? true : symmetric_reduce);
 }
 
 OrdinateSet::OrdinateSet(std::vector<double> xi, std::vector<double> eta, std::vector<double> w,
-                         std::array<int, 5> quadrants)
-    : xi_(std::move(xi)), eta_(std::move(eta)), w_(std::move(w)), quad_(quadrants) {
+                         std::array<int, 5> quadrants, std::string type, int64_t ang_order, bool symmetric_reduce)
+    : xi_(std::move(xi)), eta_(std::move(eta)), w_(std::move(w)), quad_(quadrants), type_(std::move(type)),
+      order_(ang_order), reduced_(symmetric_reduce) {
     if (xi_.size() != eta_.size() || xi_.size() != w_.size() || quad_[0] != 0 || quad_[4] != (int)xi_.size())
         throw std::invalid_argument("inconsistent ordinate set");
 }
 
 // ---------------------------------------------------------------------------------------------------
 OrdinateFlux::OrdinateFlux(const DomainDecomposition &dd, const OrdinateSet &os, int degree)
-    : DG_degree(degree), nx_(dd.nx), ny_(dd.ny), nq_(os.nq()),
+    : DG_degree(degree), dd_(dd), os_(os), nx_(dd.nx), ny_(dd.ny), nq_(os.nq()),
       density_((size_t)(dd.nx + 2) * (dd.ny + 2) * os.nq() * (degree + 1) * (degree + 1), 0.0) {}
 
 double &OrdinateFlux::operator()(int64_t q, int64_t i, int64_t j, int64_t d, int64_t e) {
@@ -61,7 +74,7 @@ double OrdinateFlux::operator()(int64_t q, int64_t i, int64_t j, int64_t d, int6
 }
 
 ScalarFlux::ScalarFlux(const DomainDecomposition &dd, int degree)
-    : DG_degree(degree), ny_(dd.ny), density_((size_t)(dd.nx + 2) * (dd.ny + 2) * (degree + 1) * (degree + 1), 0.0) {}
+    : DG_degree(degree), dd_(dd), ny_(dd.ny), density_((size_t)(dd.nx + 2) * (dd.ny + 2) * (degree + 1) * (degree + 1), 0.0) {}
 
 double &ScalarFlux::operator()(int64_t i, int64_t j, int64_t d, int64_t e) {
     const int64_t P = DG_degree + 1;
@@ -114,12 +127,20 @@ bool flag_from(const ParameterList &p, const std::string &key, bool dflt) {
     if (v == "false" || v == "0" || v == "no") return false;
     throw std::invalid_argument(key + " must be true or false, not '" + v + "'");
 }
+// The reference's sweep_pattern names (SweepPattern*.cpp: "sequential", "wavefront", "wavefront-mpi",
+// "kba", "kba2") choose a traversal order of the same upwind solves; on the device they all run the
+// staggered wavefront kernel, whose results equal theirs up to rounding.  Device names pass through.
+std::string device_pattern(const std::string &name) {
+    if (name == "sequential" || name == "wavefront" || name == "wavefront-mpi" || name == "kba" || name == "kba2")
+        return "hip-staggered";
+    return name;
+}
 }  // namespace
 
 SweepOperator::SweepOperator(std::shared_ptr<Device> dev, const ParameterList &params, double)
     : dev_(std::move(dev)), precision_(precision_from(params)) {
     const std::string pattern = params.Has("sweep_pattern") ? params.GetString("sweep_pattern") : "hip-staggered";
-    const int rc = tsw_set_pattern(dev_->ctx(), pattern.c_str());
+    const int rc = tsw_set_pattern(dev_->ctx(), device_pattern(pattern).c_str());
     if (rc == TSW_E_ARG) throw std::out_of_range("no sweep_pattern '" + pattern + "'");
     dev_->Check(rc, "tsw_set_pattern");
 }
@@ -148,7 +169,7 @@ std::unique_ptr<ImplicitSolver> ImplicitSolver::Create(std::shared_ptr<Device> d
     const std::string key = params.GetString("solve_type");
     if (params.Has("sweep_pattern")) {
         const std::string pattern = params.GetString("sweep_pattern");
-        const int rc = tsw_set_pattern(dev->ctx(), pattern.c_str());
+        const int rc = tsw_set_pattern(dev->ctx(), device_pattern(pattern).c_str());
         if (rc == TSW_E_ARG) throw std::out_of_range("no sweep_pattern '" + pattern + "'");
         dev->Check(rc, "tsw_set_pattern");
     }

@@ -8,6 +8,11 @@
 //   TransportOperator::Pmv / Smv / Lmv                src/operators/RKDG/TransportOperator.hpp:25-43
 //   ImplicitSolver + factory key solve_type           src/linear_solvers/Abstract/ImplicitSolver.hpp:117-123,
 //                                                     ImplicitSolver.cpp:182-229
+//   ParameterList::ReadFromFile / WriteToFile         src/utils/ParameterList.cpp:202-285 (decks)
+//   DensityFunction / OrdinateFlux Read/WriteToDisk   src/objects/RKDG/DensityFunction.cpp:1880-2291,
+//                                                     OrdinateFlux.cpp:2720-2885, SpatialDomain.cpp:247-407
+//   DensityFunction L1Norm / L2Norm / LinfNorm        src/objects/RKDG/DensityFunction.cpp:1082-1400
+//   Problem setup from a deck ("problem" key)          src/utils/init.cpp, src/problems/*Problem.cpp
 // Arrays live on the host in the reference's layouts; every call hands them to the GPU library.
 // Errors follow the reference: std::invalid_argument for bad arguments, std::out_of_range for unknown
 // factory keys, std::runtime_error for failed solves and device errors.
@@ -33,8 +38,17 @@ class ParameterList {
     std::string GetString(const std::string &key) const;
     double GetDouble(const std::string &key) const { return std::stod(GetString(key)); }
     int64_t GetInt(const std::string &key) const { return std::stoll(GetString(key)); }
+    // "true" / "false" as std::boolalpha reads them (ParameterList.cpp:292-335); other text throws
+    // std::runtime_error, a missing key std::out_of_range.
+    bool GetBool(const std::string &key) const;
     // Reads "key value" lines; '#' starts a comment.
     static ParameterList FromText(const std::string &text);
+    // A deck by ReadFromFile's rules (ParameterList.cpp:202-245): text from '#' on is dropped, the line
+    // is lower-cased and its first two words are key and value; keys already present are kept.
+    ParameterList &ReadFromFile(const std::string &filename);
+    // "key value" lines in key order (ParameterList.cpp:254-285).
+    const ParameterList &WriteToFile(const std::string &filename) const;
+    const std::map<std::string, std::string> &items() const { return values_; }
 
   private:
     std::map<std::string, std::string> values_;
@@ -51,8 +65,17 @@ struct DomainDecomposition {
 class OrdinateSet {
   public:
     static OrdinateSet LevelSymmetric(int order);
+    // From the deck keys ordinate_type and ang_order (OrdinateSet constructor, OrdinateSet.cpp:240-280);
+    // "level-symmetric" is built, other types throw std::invalid_argument.
+    static OrdinateSet FromParameters(const ParameterList &params);
+    // The named sets of tsw_ordinates: "spherical-triangle", "chebyshev-legendre", "level-symmetric".
+    static OrdinateSet Named(const std::string &type, int order, bool symmetric_reduce = true);
     OrdinateSet(std::vector<double> xi, std::vector<double> eta, std::vector<double> w,
-                std::array<int, 5> quadrants);
+                std::array<int, 5> quadrants, std::string type = "none", int64_t ang_order = 0,
+                bool symmetric_reduce = true);
+    const std::string &type() const { return type_; }
+    int64_t ang_order() const { return order_; }
+    bool symmetric_reduce() const { return reduced_; }
     int64_t nq() const { return (int64_t)xi_.size(); }
     double xi(int64_t q) const { return xi_[q]; }
     double eta(int64_t q) const { return eta_[q]; }
@@ -66,6 +89,9 @@ class OrdinateSet {
   private:
     std::vector<double> xi_, eta_, w_;
     std::array<int, 5> quad_;
+    std::string type_;
+    int64_t order_ = 0;
+    bool reduced_ = true;
 };
 
 // Angular flux, density[((i*(ny+2)+j)*nq+q)*(p+1)^2 + d*(p+1) + e] (OrdinateFlux.hpp:639-700).
@@ -78,9 +104,19 @@ class OrdinateFlux {
     const double *data() const { return density_.data(); }
     size_t size() const { return density_.size(); }
     int64_t nq() const { return nq_; }
+    const DomainDecomposition &domain() const { return dd_; }
+    const OrdinateSet &ordinates() const { return os_; }
+    // Binary file of RKDG::OrdinateFlux::WriteToDisk (OrdinateFlux.cpp:2720-2885, one rank): the mesh
+    // header, the ordinate-set name with "#R" (the 2D sets are symmetry-reduced) as a null-terminated
+    // string, the angular order, then DG degree, byte count, value count and every value with halos.
+    void WriteToDisk(const std::string &filename) const;
+    // Reads such a file; the ordinate set is rebuilt from its name and order (level-symmetric only).
+    static OrdinateFlux ReadFromDisk(const std::string &filename);
     const int DG_degree;
 
   private:
+    DomainDecomposition dd_;
+    OrdinateSet os_;
     int64_t nx_, ny_, nq_;
     std::vector<double> density_;
 };
@@ -94,9 +130,29 @@ class ScalarFlux {
     double *data() { return density_.data(); }
     const double *data() const { return density_.data(); }
     size_t size() const { return density_.size(); }
+    const DomainDecomposition &domain() const { return dd_; }
+    // Binary file of RKDG::DensityFunction::WriteToDisk (DensityFunction.cpp:2226-2291, one rank): per
+    // dimension int64 nx, double ax, double bx (SpatialDomain.cpp:385-407), then int64 DG degree, int64
+    // byte count, int64 value count and the values, halo cells included.
+    void WriteToDisk(const std::string &filename) const;
+    static ScalarFlux ReadFromDisk(const std::string &filename);  // throws std::runtime_error
+    // Norms over the interior, as DensityFunction::L1Norm (Gauss-Legendre with DG degree + 1 points per
+    // axis), L2Norm (Legendre orthogonality) and LinfNorm (per cell, uniform samples doubled until the
+    // maximum changes by at most tol relative) compute them (DensityFunction.cpp:1082-1400).
+    double L1Norm() const;
+    double L2Norm() const;
+    double LinfNorm(double tol = 1e-3) const;
+    double EvaluateAtPoint(double x, double y) const;  // the cell's polynomial at (x, y)
+    // first - second, as DensityFunction::ComputeDifference (DensityFunction.cpp:525-720) forms it: on
+    // the common mesh the coefficients are subtracted at the higher of the two DG degrees; otherwise both
+    // are projected exactly onto the mesh with lcm(nx) x lcm(ny) cells (each old cell split into equal
+    // subcells) and subtracted there.  The domains must agree to 1e-14 (std::invalid_argument).  The
+    // reference returns the result through an output argument that it reconfigures; here it is returned.
+    static ScalarFlux ComputeDifference(const ScalarFlux &first, const ScalarFlux &second);
     const int DG_degree;
 
   private:
+    DomainDecomposition dd_;
     int64_t ny_;
     std::vector<double> density_;
 };
@@ -228,5 +284,61 @@ class IDCIntegrator {
     int64_t max_its_ = 3000;
     int precision_ = TSW_FP64;
 };
+
+// ---------------------------------------------------------------------------------------------------
+// Problem definitions selected by the deck key "problem" (Factory<Problem>, src/utils/Problems/*):
+// functions of position for the initial condition, isotropic source and cross sections, and the inflow
+// written into the source's halo cells.  Built: "lattice" (LatticeProblem.cpp:100-324), "hohlraum"
+// (HohlraumProblem.cpp:60-272) and "hohlraum2" (Hohlraum2Problem.cpp:14-148); other names throw
+// std::out_of_range.  The deck key moll_radius is required, as Problem's constructor requires it
+// (Problem.cpp:20-35); mollified evaluation (moll_radius >= 1e-14) is not built and throws
+// std::invalid_argument.
+class Problem {
+  public:
+    virtual ~Problem() = default;
+    virtual std::string GetDescriptor() const = 0;
+    virtual double InitialCondition(double, double) const { return 0.0; }
+    virtual double Source(double, double) const { return 0.0; }
+    virtual double TotalCross(double, double) const { return 0.0; }
+    virtual double ScatterCross(double, double) const { return 0.0; }
+    // Writes the boundary inflow density into halo cells of the projected source (Problem.cpp:105-116).
+    virtual void SetInflowBoundaries(ScalarFlux &) const {}
+    // Replaces deck values the problem fixes (the domain), where the deck names them.
+    virtual void OverrideOptions(ParameterList &) const {}
+    static std::unique_ptr<Problem> Create(const ParameterList &params);
+};
+using ProblemEvalFcn = double (Problem::*)(double, double) const;
+
+// L2 projection of a problem function onto the DG space of dest, ComputeProjection of init.cpp:50-240:
+// proj_order x proj_order Gauss-Legendre points per interior cell, then, if apply_limiter and the cell
+// mean is not negative, the moments of degree > 0 are scaled by the largest theta in [0, 1] (bisection
+// to 1e-14) that keeps the polynomial non-negative on a 21 x 21 grid of the cell.
+void ComputeProjection(const Problem &prob, ScalarFlux &dest, ProblemEvalFcn f, bool apply_limiter = true,
+                       int proj_order = 14);
+
+// Everything Init (init.cpp:422-946) builds from a deck, for a solve on the GPU.
+struct ProblemInputs {
+    ProblemInputs(const DomainDecomposition &dd, const OrdinateSet &os, int DG_degree);
+    DomainDecomposition dd;
+    OrdinateSet os;
+    int DG_degree;
+    ScalarFlux sigma_t_density, sigma_s_density;  // the DG degree 0 projections
+    ScalarFlux initial_density, source_density;   // projected initial condition and source (+ inflow)
+    CrossSection sigma_t, sigma_s;                // the same cross sections, one value per cell
+    OrdinateFlux psi;                             // initial_density / (4 pi) in every ordinate
+    OrdinateFlux source;                          // source_density / (4 pi), halos included
+    std::string problem;
+    double final_time;     // "tfinal", NaN when absent
+    double max_step_size;  // "cfl" * dx (TimeIntegrator.cpp:129-130), NaN when absent
+};
+
+// Builds the inputs of a deck as Init does: the problem (key "problem") overrides its domain; the mesh
+// from nx, ny, ax, bx, ay, by; the DG degree from dg_degree (or dg_degree_x, global.cpp:198-215); the
+// ordinate set from ordinate_type (default "chebyshev-legendre"), ang_order and ordinate_sym_reduce
+// (default false).  The sweeps take one cross-section value per cell, so piecewise_cross must be true
+// (std::invalid_argument otherwise, also for periodic or reflecting boundaries).  With output_init true
+// each projection is read from, or else computed and written to, the file the reference names
+// (sigma_t_dg0_nx0000224.cs2, sigma_s_dg0_..., initial_dg<p>_..., source_dg<p>_....sd2) in `directory`.
+ProblemInputs Init(ParameterList &params, const std::string &directory = ".");
 
 }  // namespace tsw_host

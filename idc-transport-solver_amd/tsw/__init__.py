@@ -92,6 +92,8 @@ def lib():
         L.tsw_idc_run.argtypes = [c, _vp, _vp, _vp, _vp, d, i, d, i, i, i]
         L.tsw_idc_adjoint.argtypes = [c, _vp, _vp, _vp, _vp, d, i, i, _vp, _vp, _vp, _vp, d, i, i, i]
         L.tsw_ordinates_level_symmetric.argtypes = [i, _dp, _dp, _dp, _ip]
+        L.tsw_ordinates.argtypes = [ctypes.c_char_p, i, i, i, _ip, _dp, _dp, _dp, _ip]
+        L.tsw_gauss_legendre.argtypes = [i, _dp, _dp]
         L.tsw_sweep_timing.argtypes = [c, i, ctypes.POINTER(ll), _dp]
         L.tsw_batch_set.argtypes = [c, i, ctypes.POINTER(TswProblem), ctypes.POINTER(_vp), d, i]
         L.tsw_batch_sweep.argtypes = [c, ctypes.POINTER(_vp), i, i]
@@ -115,6 +117,33 @@ def level_symmetric(order):
     if rc != TSW_OK:
         raise TswError(rc, f"no level-symmetric set of order {order}")
     return xi, eta, w, off
+
+
+def ordinates(ordinate_type, order, symmetric_reduce=True):
+    """(xi, eta, w, quad_off) of a named 2D ordinate set: "spherical-triangle", "chebyshev-legendre"
+    (the reference's sets, in its order) or "level-symmetric"."""
+    n = ctypes.c_int(0)
+    name = ordinate_type.encode()
+    rc = lib().tsw_ordinates(name, order, int(bool(symmetric_reduce)), 0, ctypes.byref(n), None, None, None, None)
+    if rc != TSW_OK:
+        raise TswError(rc, f"no ordinate set '{ordinate_type}' of order {order}")
+    xi, eta, w = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+    off = np.zeros(5, dtype=np.int32)
+    rc = lib().tsw_ordinates(name, order, int(bool(symmetric_reduce)), n.value, ctypes.byref(n),
+                             xi.ctypes.data_as(_dp), eta.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                             off.ctypes.data_as(_ip))
+    if rc != TSW_OK:
+        raise TswError(rc, f"no ordinate set '{ordinate_type}' of order {order}")
+    return xi, eta, w, off
+
+
+def gauss_legendre(n):
+    """n-point Gauss-Legendre nodes (ascending) and weights on [-1, 1]."""
+    x, w = np.zeros(n), np.zeros(n)
+    rc = lib().tsw_gauss_legendre(n, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    if rc != TSW_OK:
+        raise TswError(rc, f"no Gauss-Legendre rule with {n} points")
+    return x, w
 
 
 def _addr(a):

@@ -1,0 +1,222 @@
+"""Reference-style input decks: "key value" files read by ParameterList::ReadFromFile's rules, and the
+problem inputs Init builds from them (src/utils/init.cpp:422-946): cross sections, source and initial
+condition projected into the DG space, inflow halos, and the named ordinate set.
+
+Problems: "lattice" (LatticeProblem.cpp:100-324), "hohlraum" (HohlraumProblem.cpp:60-272),
+"hohlraum2" (Hohlraum2Problem.cpp:14-148).  The GPU sweeps take one cross-section value per cell, so
+piecewise_cross must be true; periodic and reflecting boundaries and mollified problem functions are
+not available (ValueError)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import gauss_legendre, ordinates
+from .fields import Mesh, legendre_table
+from .problems import Problem
+
+
+def read_deck(path, params=None):
+    """Key/value pairs of a deck (ParameterList.cpp:202-245): text from '#' on is dropped, the line is
+    lower-cased, its first two words are key and value.  As in the reference, key and value carry over
+    from the previous line when a line lacks them (so a key alone takes the previous value and a blank
+    line repeats the previous pair), and keys already present are not replaced."""
+    out = {} if params is None else params
+    key, value = "", ""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    if lines[-1] == "":  # std::getline yields no empty line after the final newline
+        lines.pop()
+    for line in lines:
+        words = line.split("#", 1)[0].lower().split()
+        if words:
+            key = words[0]
+            if len(words) > 1:
+                value = words[1]
+        out.setdefault(key, value)
+    return out
+
+
+def write_deck(path, params):
+    """"key value" lines in key order (ParameterList.cpp:254-285)."""
+    with open(path, "w") as f:
+        for k in sorted(params):
+            f.write(f"{k} {params[k]}\n")
+
+
+def _bool(params, key, default):
+    """std::boolalpha reading: "true" / "false"; other text raises ValueError."""
+    if key not in params:
+        return default
+    v = params[key]
+    if v not in ("true", "false"):
+        raise ValueError(f"Error converting value '{v}' for key '{key}' to type 'bool'")
+    return v == "true"
+
+
+# ---------------------------------------------------------------------------------------------------
+def _in(v, a):
+    return (v > a) & (v <= a + 1.0)
+
+
+class Lattice:
+    domain = (0.0, 7.0, 0.0, 7.0)
+
+    @staticmethod
+    def source(x, y):
+        return np.where((x > 3.0) & (x < 4.0) & (y > 3.0) & (y < 4.0), 1.0, 0.0)
+
+    @staticmethod
+    def absorber(x, y):
+        a = (_in(x, 1.0) | _in(x, 5.0)) & (_in(y, 1.0) | _in(y, 3.0) | _in(y, 5.0))
+        a |= (_in(x, 2.0) | _in(x, 4.0)) & (_in(y, 2.0) | _in(y, 4.0))
+        return a | (_in(x, 3.0) & _in(y, 5.0))
+
+    @classmethod
+    def total(cls, x, y):
+        return np.where(cls.absorber(x, y), 10.0, 1.0)
+
+    @classmethod
+    def scatter(cls, x, y):
+        return np.where(cls.absorber(x, y), 0.0, 1.0)
+
+    @staticmethod
+    def inflow(q):
+        pass
+
+
+class Hohlraum:
+    domain = (0.0, 1.3, 0.0, 1.3)
+
+    @staticmethod
+    def source(x, y):
+        return np.zeros(np.broadcast(x, y).shape)
+
+    @staticmethod
+    def wall(x, y):
+        w = (x > 1.25) | (y <= 0.05) | (y > 1.25)
+        return w | ((y > 0.25) & (y <= 1.05) & ((x <= 0.05) | ((x > 0.45) & (x <= 0.85))))
+
+    @classmethod
+    def total(cls, x, y):
+        return np.where(cls.wall(x, y), 100.0, 0.1)
+
+    @classmethod
+    def scatter(cls, x, y):
+        return np.where(cls.wall(x, y), 1.0, 0.1)
+
+    @staticmethod
+    def inflow(q):
+        q[0, :, 0, 0] = 1.0
+
+
+class Hohlraum2(Hohlraum):
+    @staticmethod
+    def scatter(x, y):
+        x, y = np.broadcast_arrays(x, y)
+        s = np.full(x.shape, 0.1)
+        conds = [  # first matching rule wins, so apply in reverse order
+            ((x > 1.25) | (y > 1.25) | (y < 0.05), 50.0),
+            ((x > 0.50) & (x <= 0.85) & (y > 0.30) & (y <= 1.00), 0.0),
+            ((x > 0.45) & (x <= 0.50) & (y > 0.25) & (y <= 1.05), 90.0),
+            ((x > 0.50) & (x <= 0.85) & (y > 1.00) & (y <= 1.05), 90.0),
+            ((x > 0.50) & (x <= 0.85) & (y > 0.25) & (y <= 0.30), 90.0),
+            ((x <= 0.05) & (y > 0.25) & (y <= 1.05), 95.0),
+        ]
+        for c, v in reversed(conds):
+            s = np.where(c, v, s)
+        return s
+
+
+PROBLEMS = {"lattice": Lattice, "hohlraum": Hohlraum, "hohlraum2": Hohlraum2}
+
+
+def project(fn, mesh: Mesh, p, apply_limiter=True, proj_order=14):
+    """ComputeProjection (init.cpp:50-240): [nx+2][ny+2][p+1][p+1] coefficients from a proj_order^2
+    Gauss rule per interior cell; with apply_limiter, a cell with non-negative mean whose polynomial is
+    negative somewhere on the 21 x 21 sample grid has its moments of degree > 0 scaled by the largest
+    theta (bisection to 1e-14) that removes the negative samples."""
+    g, w = gauss_legendre(proj_order)
+    X = mesh.ax + np.arange(mesh.nx)[:, None] * mesh.dx + (g[None, :] + 1.0) * mesh.dx / 2.0
+    Y = mesh.ay + np.arange(mesh.ny)[:, None] * mesh.dy + (g[None, :] + 1.0) * mesh.dy / 2.0
+    F = fn(X[:, None, :, None], Y[None, :, None, :])  # [nx][ny][l][m]
+    L = legendre_table(p, g) * ((2 * np.arange(p + 1) + 1) / 2.0)[None, :]
+    c = np.einsum("ijlm,l,m,ld,me->ijde", F, w, w, L, L)
+    if apply_limiter and p > 0:
+        S = legendre_table(p, -1.0 + 0.1 * np.arange(21))
+        hi = np.ones((p + 1, p + 1))
+        hi[0, 0] = 0.0
+
+        def negative(cc, theta):
+            scaled = cc * (theta * hi + (1.0 - hi))
+            return np.any(np.einsum("...de,ld,me->...lm", scaled, S, S) < 0.0, axis=(-1, -2))
+
+        flag = (c[..., 0, 0] >= 0.0) & negative(c, 1.0)
+        for i, j in zip(*np.nonzero(flag)):
+            lo, up = 0.0, 1.0
+            while up - lo > 1e-14:
+                mid = 0.5 * (lo + up)
+                if negative(c[i, j], mid):
+                    up = mid
+                else:
+                    lo = mid
+            c[i, j] = c[i, j] * (lo * hi + (1.0 - hi))
+    out = np.zeros((mesh.nx + 2, mesh.ny + 2, p + 1, p + 1))
+    out[1:-1, 1:-1] = c
+    return out
+
+
+def init(params):
+    """(Problem, extras) for a deck given as a dict (see read_deck).  Problem carries the mesh, DG
+    degree, ordinate set, per-cell cross sections and the source density with inflow halos
+    (q_dg); extras holds the projections by the reference's file names, the initial angular flux, and
+    final_time / max_step_size (tfinal, cfl * dx)."""
+    params = dict(params)
+    name = params["problem"]
+    if name not in PROBLEMS:
+        raise KeyError(f"problem '{name}' is not available")
+    if float(params["moll_radius"]) >= 1e-14:
+        raise ValueError("mollified problem functions (moll_radius > 0) are not available")
+    prob = PROBLEMS[name]
+    for k, v in zip(("ax", "bx", "ay", "by"), prob.domain):  # OverrideOptions: only keys the deck names
+        if k in params and float(params[k]) != v:
+            params[k] = repr(v)
+    if name.startswith("hohlraum"):
+        for k in ("periodic", "reflect_x_min"):
+            if _bool(params, k, False):
+                params[k] = "false"
+    if _bool(params, "periodic", False):
+        raise ValueError("periodic boundaries are not available (the sweeps take inflow halos)")
+    for k in ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"):
+        if _bool(params, k, False):
+            raise ValueError(f"reflecting boundaries are not available: {k}")
+    if not _bool(params, "piecewise_cross", False):
+        raise ValueError("piecewise_cross must be true: the sweeps take one cross-section value per cell")
+    mesh = Mesh(int(params["nx"]), int(params["ny"]), float(params["ax"]), float(params["bx"]),
+                float(params["ay"]), float(params["by"]))
+    p = int(params.get("dg_degree_x", params.get("dg_degree", -1)))
+    if p < 0:
+        raise ValueError("Failed to determine degree of DG spatial discretization")
+    otype = params.get("ordinate_type", "chebyshev-legendre")
+    order = int(params["ang_order"])
+    reduce = _bool(params, "ordinate_sym_reduce", False)
+    xi, eta, w, off = ordinates(otype, order, reduce)
+
+    st = project(prob.total, mesh, 0)
+    ss = project(prob.scatter, mesh, 0)
+    q0 = project(prob.source, mesh, p)
+    ic = project(lambda x, y: np.zeros(np.broadcast(x, y).shape), mesh, p)
+    q = q0.copy()
+    prob.inflow(q)
+    P = Problem(name, mesh.nx, mesh.ny, p, mesh.ax, mesh.bx, mesh.ay, mesh.by, xi, eta, w, off,
+                st[:, :, 0, 0].copy(), ss[:, :, 0, 0].copy(), q[:, :, 0, 0].copy(), q_dg=q)
+    nxs = f"{mesh.nx:07d}"
+    extras = {
+        f"sigma_t_dg0_nx{nxs}.cs2": st, f"sigma_s_dg0_nx{nxs}.cs2": ss,
+        f"initial_dg{p}_nx{nxs}.sd2": ic, f"source_dg{p}_nx{nxs}.sd2": q0,
+        "ordinate_info": (otype, order, reduce), "mesh": mesh,
+        "final_time": float(params["tfinal"]) if "tfinal" in params else math.nan,
+        "max_step_size": float(params["cfl"]) * mesh.dx if "cfl" in params else math.nan,
+    }
+    return P, extras

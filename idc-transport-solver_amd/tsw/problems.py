@@ -33,6 +33,7 @@ class Problem:
     sigma_t: np.ndarray   # [nx+2][ny+2]
     sigma_s: np.ndarray   # [nx+2][ny+2]
     q_cell: np.ndarray    # [nx+2][ny+2] isotropic source density per cell (halos: isotropic inflow density)
+    q_dg: np.ndarray = None  # optional [nx+2][ny+2][p+1][p+1] DG source density; replaces q_cell
 
     @property
     def nq(self):
@@ -40,7 +41,11 @@ class Problem:
 
     @property
     def source(self):
-        """Angular source [nx+2][ny+2][nq][p+1][p+1]: q/(4 pi) in every ordinate's (0,0) coefficient."""
+        """Angular source [nx+2][ny+2][nq][p+1][p+1]: the source density / (4 pi) in every ordinate,
+        all DG coefficients of q_dg when given, else q_cell in the (0,0) coefficient."""
+        if self.q_dg is not None:
+            return np.ascontiguousarray(np.broadcast_to(self.q_dg[:, :, None] / (4.0 * math.pi),
+                                                        (self.nx + 2, self.ny + 2, self.nq, self.p + 1, self.p + 1)))
         return _isotropic_source(self.q_cell, self.nq, self.p)
 
     def source_device(self, device):
@@ -48,6 +53,9 @@ class Problem:
         import torch
         src = torch.zeros((self.nx + 2, self.ny + 2, self.nq, self.p + 1, self.p + 1), dtype=torch.float64,
                           device=device)
+        if self.q_dg is not None:
+            src[:] = torch.from_numpy(self.q_dg / (4.0 * math.pi)).to(device)[:, :, None]
+            return src
         q = torch.from_numpy(self.q_cell / (4.0 * math.pi)).to(device)
         src[:, :, :, 0, 0] = q[:, :, None]
         return src

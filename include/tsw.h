@@ -209,6 +209,27 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
  * Arrays must hold nq entries, quad_off 5. */
 int tsw_ordinates_level_symmetric(int order, double *xi, double *eta, double *w, int *quad_off);
 
+/* The ordinate set a deck names by ordinate_type / ang_order / ordinate_sym_reduce, replacing the
+ * OrdinateSet constructor (OrdinateSet.cpp:126-287) for SPACE_DIMS == 2:
+ *   "spherical-triangle": T_N, 8 N^2 directions on the sphere (SphericalTriangle.cpp:268-351,
+ *     OrdinateSet.cpp:490-520), in the reference's order: the first octant's triangles in generation
+ *     order, then the octants made by reflecting mu, eta and xi;
+ *   "chebyshev-legendre": 2 N^2 directions (OrdinateSet.cpp:441-488), N even, sorted as the reference
+ *     sorts them (descending octant, xi, eta, mu);
+ *   "level-symmetric": as tsw_ordinates_level_symmetric (always reduced; no reference counterpart).
+ * With symmetric_reduce the pairs (xi, eta, +-mu) are merged as SymmetricReduce does (:1171-1190),
+ * halving the count.  Weights sum to 4 pi.  quad_off[k] starts quadrant k (0: xi>0, eta>0; 1: xi>0,
+ * eta<0; 2: xi<0, eta>0; 3: xi<0, eta<0), quad_off[4] = nq.  Writes *nq; the arrays are filled when xi
+ * is non-NULL and must then hold `capacity` >= nq entries.  Unknown types (the reference's "lebedev"
+ * tables are not shipped with it) and orders fail with TSW_E_ARG.  Host-only, needs no device. */
+int tsw_ordinates(const char *type, int order, int symmetric_reduce, int capacity, int *nq, double *xi,
+                  double *eta, double *w, int *quad_off);
+
+/* n-point Gauss-Legendre rule on [-1, 1]: nodes ascending, exactly antisymmetric, weights summing to 2
+ * (the rules Quadrule::ComputeQuadrature returns for NodesType::GaussLegendre, Quadrule.cpp:1004-1035).
+ * Host-only. */
+int tsw_gauss_legendre(int n, double *nodes, double *weights);
+
 /* One member of a batch of independent problems (mesh, DG degree, ordinate set; see tsw_problem_set). */
 typedef struct tsw_problem {
     int nx, ny;

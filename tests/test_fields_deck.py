@@ -1,0 +1,294 @@
+"""Host-side checks of the deck and field-file path (no GPU): the named ordinate sets and Gauss-Legendre
+rules against the reference's tables and the geometry that defines them, the C++ field/deck test
+program, and the Python readers, norms and deck projection against the C++ tools."""
+import json
+import math
+import os
+import subprocess
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import tsw
+from tsw import deck, fields
+from _common import REPO
+
+PKG = os.path.join(REPO, "idc-transport-solver_amd")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def _tool(name):
+    path = os.path.join(PKG, name)
+    assert os.path.exists(path), "build with __graft_entry__.build()"
+    return path
+
+
+# ------------------------------------------------------------------------------------------------
+# Gauss-Legendre rules
+def test_gauss_legendre_equals_reference_tables():
+    """Bitwise equal to the doubles of the reference's 25-digit tables (Tabulated.cpp GLTable)."""
+    rules = json.load(open(os.path.join(GOLDEN, "gl_table.json")))["rules"]
+    assert len(rules) >= 14
+    for n, r in rules.items():
+        x, w = tsw.gauss_legendre(int(n))
+        np.testing.assert_array_equal(x, [float(v) for v in r["nodes"]], err_msg=f"nodes n={n}")
+        np.testing.assert_array_equal(w, [float(v) for v in r["weights"]], err_msg=f"weights n={n}")
+
+
+def _legendre_exact(n, x):
+    p0, p1 = Fraction(1), x
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    return p1 if n else Fraction(1)
+
+
+@pytest.mark.parametrize("n", [11, 13, 20, 24])
+def test_gauss_legendre_nodes_are_correctly_rounded_roots(n):
+    """For sizes the tables lack: P_n changes sign within half an ulp of every positive node, in exact
+    rational arithmetic, so each node is the double nearest a root; weights integrate x^(2n-2) exactly."""
+    x, w = tsw.gauss_legendre(n)
+    for xi in x[x > 0]:
+        h = Fraction(np.spacing(xi)) / 2
+        a, b = _legendre_exact(n, Fraction(xi) - h), _legendre_exact(n, Fraction(xi) + h)
+        assert a * b < 0, xi
+    np.testing.assert_allclose(np.sum(w * x ** (2 * n - 2)), 2.0 / (2 * n - 1), rtol=1e-13)
+    assert np.all(x[::-1] == -x)
+
+
+# ------------------------------------------------------------------------------------------------
+# Named ordinate sets
+def _tri_solid_angle(a, b, c):
+    """Van Oosterom-Strackee: tan(omega/2) = |a.(b x c)| / (1 + a.b + b.c + c.a) for unit vectors."""
+    num = abs(np.dot(a, np.cross(b, c)))
+    den = 1 + np.dot(a, b) + np.dot(b, c) + np.dot(c, a)
+    return 2 * math.atan2(num, den)
+
+
+def _unit(p):
+    p = np.asarray(p, dtype=float)
+    return p / np.linalg.norm(p)
+
+
+def _tessellation(n):
+    """Triangles of the regular n-subdivision of the octant face x + y + z = 1, as barycentric
+    lattice points (i, j, k)/n with i + j + k = n."""
+    tris = []
+    for i in range(n):
+        for j in range(n - i):
+            k = n - i - j
+            tris.append(((i, j, k), (i + 1, j, k - 1), (i, j + 1, k - 1)))  # upward
+            if k >= 2:
+                tris.append(((i + 1, j, k - 1), (i + 1, j + 1, k - 2), (i, j + 1, k - 1)))  # downward
+    return [[np.array(v, dtype=float) / n for v in t] for t in tris]
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 8])
+def test_spherical_triangle_geometry(order):
+    """T_N first octant: the centroids of the N^2 triangles of the subdivided octant face projected to
+    the sphere, each with its solid angle; compared as sets against an independent tessellation and
+    an independent solid-angle formula."""
+    xi, eta, w, off = tsw.ordinates("spherical-triangle", order, symmetric_reduce=False)
+    m = order * order
+    assert xi.size == 8 * m and list(off) == [0, 2 * m, 4 * m, 6 * m, 8 * m]
+    mu = np.sqrt(np.maximum(0.0, 1 - xi ** 2 - eta ** 2))
+    got = sorted(zip(np.round(xi[:m], 12), np.round(eta[:m], 12), w[:m]))
+    ref = []
+    for t in _tessellation(order):
+        c = _unit(sum(t) / 3)
+        ref.append((round(c[0], 12), round(c[1], 12), _tri_solid_angle(*[_unit(v) for v in t])))
+    ref.sort()
+    assert len(ref) == m
+    for (a, b, wa), (c, d, wc) in zip(got, ref):
+        assert abs(a - c) < 1e-11 and abs(b - d) < 1e-11
+        assert abs(wa - wc) < 1e-13 * max(1.0, wc / 1e-3)
+    np.testing.assert_allclose(w.sum(), 4 * math.pi, rtol=1e-13)
+    # octants 111, 110 (mu negated), 101, 100 (eta negated), then xi negated
+    np.testing.assert_array_equal(xi[m:2 * m], xi[:m])
+    np.testing.assert_array_equal(eta[2 * m:4 * m], -eta[:2 * m])
+    np.testing.assert_array_equal(xi[4 * m:], -xi[:4 * m])
+    assert np.all(mu > 0)
+
+
+@pytest.mark.parametrize("otype,order", [("spherical-triangle", 4), ("spherical-triangle", 8),
+                                         ("spherical-triangle", 3), ("chebyshev-legendre", 4),
+                                         ("chebyshev-legendre", 8)])
+def test_symmetric_reduction_merges_mu_pairs(otype, order):
+    xi, eta, w, off = tsw.ordinates(otype, order, symmetric_reduce=False)
+    rx, re_, rw, roff = tsw.ordinates(otype, order, symmetric_reduce=True)
+    assert rx.size * 2 == xi.size and roff[4] == rx.size
+    assert np.all(np.diff(roff) == rx.size // 4)
+    np.testing.assert_allclose(rw.sum(), 4 * math.pi, rtol=1e-13)
+    # every reduced direction is an unreduced one, with the weight of the +mu and -mu pair
+    full = {(a, b): 0.0 for a, b in zip(xi, eta)}
+    for a, b, c in zip(xi, eta, w):
+        full[(a, b)] += c
+    for a, b, c in zip(rx, re_, rw):
+        assert full[(a, b)] == pytest.approx(c, rel=1e-15)
+    signs = [(1, 1), (1, -1), (-1, 1), (-1, -1)]
+    for k, (sx, sy) in enumerate(signs):
+        sl = slice(roff[k], roff[k + 1])
+        assert np.all(np.sign(rx[sl]) == sx) and np.all(np.sign(re_[sl]) == sy)
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8, 16])
+def test_chebyshev_legendre_product_and_order(order):
+    """2 N^2 directions: GL nodes in mu times 2N azimuths (k + 1/2) pi / N, weights pi/N w_GL, sorted by
+    descending (octant, xi, eta, mu) as the reference's heap sort leaves them."""
+    xi, eta, w, off = tsw.ordinates("chebyshev-legendre", order, symmetric_reduce=False)
+    g, gw = tsw.gauss_legendre(order)
+    assert xi.size == 2 * order * order
+    mu2 = 1 - xi ** 2 - eta ** 2
+    ref = sorted((round(float(m * m), 12), round(math.atan2(math.sin(p), math.cos(p)) % (2 * math.pi), 10))
+                 for m in g for p in (np.arange(2 * order) + 0.5) * math.pi / order)
+    # each node appears once with +mu and once with -mu; the sorted set keeps mu^2 and azimuth
+    got = sorted((round(float(v), 12), round(math.atan2(e, x) % (2 * math.pi), 10)) for v, x, e in zip(mu2, xi, eta))
+    assert got == ref
+    np.testing.assert_allclose(w.sum(), 4 * math.pi, rtol=1e-14)
+    assert set(np.round(w / (math.pi / order), 14)) <= set(np.round(gw, 14))
+    # within each octant pair, xi descending (mu sign alternates by octant block)
+    half = xi.size // 8
+    for b in range(8):
+        blk = xi[b * half:(b + 1) * half]
+        assert np.all(np.diff(blk) <= 0)
+
+
+def test_unknown_sets_and_orders_are_refused():
+    with pytest.raises(tsw.TswError):
+        tsw.ordinates("lebedev", 5)
+    with pytest.raises(tsw.TswError):
+        tsw.ordinates("chebyshev-legendre", 3)  # odd: node at mu = 0
+    with pytest.raises(tsw.TswError):
+        tsw.ordinates("level-symmetric", 14)
+
+
+# ------------------------------------------------------------------------------------------------
+# C++ host program and tools
+def test_cpp_fields_and_decks(tmp_path):
+    r = subprocess.run([os.path.join(REPO, "tests", "cpp", "fields_test"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "FAIL" not in r.stdout
+
+
+def _small_lattice_deck(path, extra=""):
+    path.write_text("ax 0.0\nbx 7.0\nay 0.0\nby 7.0\nnx 21\nny 14\ndg_degree 2\npiecewise_cross true\n"
+                    "problem lattice\nmoll_radius 0.0\nordinate_type spherical-triangle\nang_order 4\n"
+                    "ordinate_sym_reduce true\nsolve_type petsc-peierls-gmres\nsweep_pattern wavefront\n" + extra)
+
+
+def test_python_deck_init_matches_cpp_tool(tmp_path):
+    """The numpy projection and the C++ Init write the same inputs (lattice with cells straddling the
+    block edges in y, so the limiter acts) and hohlraum (inflow halo)."""
+    d = tmp_path / "lat.deck"
+    _small_lattice_deck(d)
+    r = subprocess.run([_tool("tsw_init"), str(d), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    P, ex = deck.init(deck.read_deck(str(d)))
+    for name in ("sigma_t_dg0_nx0000021.cs2", "sigma_s_dg0_nx0000021.cs2", "source_dg2_nx0000021.sd2",
+                 "initial_dg2_nx0000021.sd2"):
+        v, m = fields.read_density(str(tmp_path / name))
+        assert m == ex["mesh"]
+        np.testing.assert_allclose(v, ex[name], rtol=0, atol=1e-15, err_msg=name)
+    af, m, info = fields.read_ordinate_flux(str(tmp_path / "source_af_dg2_nx0000021.af2"))
+    assert info == fields.OrdinateInfo("spherical-triangle", 4, True) and af.shape[2] == P.nq == 64
+    np.testing.assert_allclose(af, P.source, rtol=0, atol=1e-16)
+    # ny = 14 on [0,7]: cells of height 1/2 align; nx = 21: cells of width 1/3 align too; the limiter is
+    # exercised on a mesh where cells straddle the source edges
+    d2 = tmp_path / "lat2.deck"
+    d2.write_text(d.read_text().replace("nx 21", "nx 10").replace("ny 14", "ny 9"))
+    r = subprocess.run([_tool("tsw_init"), str(d2), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    P2, ex2 = deck.init(deck.read_deck(str(d2)))
+    v, _ = fields.read_density(str(tmp_path / "source_dg2_nx0000010.sd2"))
+    np.testing.assert_allclose(v, ex2["source_dg2_nx0000010.sd2"], rtol=0, atol=1e-15)
+    raw = deck.project(deck.Lattice.source, ex2["mesh"], 2, apply_limiter=False)
+    assert np.any(np.abs(raw - v) > 1e-3), "no cell was limited"
+    np.testing.assert_allclose(raw[..., 0, 0], v[..., 0, 0], atol=1e-15)  # limiting keeps cell means
+    h = tmp_path / "h.deck"
+    h.write_text("nx 13\nny 13\nax 0\nbx 1.3\nay 0\nby 1.3\ndg_degree 1\npiecewise_cross true\n"
+                 "problem hohlraum2\nmoll_radius 0\nang_order 2\nordinate_type chebyshev-legendre\n")
+    r = subprocess.run([_tool("tsw_init"), str(h), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    Ph, exh = deck.init(deck.read_deck(str(h)))
+    for name in ("sigma_t_dg0_nx0000013.cs2", "sigma_s_dg0_nx0000013.cs2"):
+        v, _ = fields.read_density(str(tmp_path / name))
+        np.testing.assert_allclose(v, exh[name], rtol=1e-15, atol=0, err_msg=name)
+    af, _, _ = fields.read_ordinate_flux(str(tmp_path / "source_af_dg1_nx0000013.af2"))
+    np.testing.assert_array_equal(af[0, :, :, 0, 0], 1 / (4 * math.pi))
+    np.testing.assert_allclose(af, Ph.source, rtol=0, atol=1e-16)
+
+
+def test_deck_reader_rules(tmp_path):
+    d = tmp_path / "q.deck"
+    d.write_text("# comment\nNX  12   # trailing\nProblem Lattice\n\nlonely\nnx 99\n")
+    p = deck.read_deck(str(d), {"keep": "1"})
+    assert p == {"": "", "nx": "12", "problem": "lattice", "lonely": "lattice", "keep": "1"}
+    deck.write_deck(str(tmp_path / "w.deck"), p)
+    assert deck.read_deck(str(tmp_path / "w.deck")) == p
+    assert deck.read_deck(str(tmp_path / "w.deck")) == p
+    e = tmp_path / "e.deck"
+    e.write_text("")
+    assert deck.read_deck(str(e)) == {}
+
+
+def test_python_norms_and_difference_match_cpp_comparator(tmp_path):
+    rng = np.random.default_rng(5)
+    m1 = fields.Mesh(6, 4, 0.0, 3.0, -1.0, 1.0)
+    m2 = fields.Mesh(4, 6, 0.0, 3.0, -1.0, 1.0)
+    a = rng.standard_normal((8, 6, 3, 3))
+    b = rng.standard_normal((6, 8, 2, 2))
+    fields.write_density(str(tmp_path / "a.sd2"), a, m1)
+    fields.write_density(str(tmp_path / "b.sd2"), b, m2)
+    r = subprocess.run([_tool("tsw_compare"), str(tmp_path / "a.sd2"), str(tmp_path / "b.sd2"),
+                        str(tmp_path / "a.sd2")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split() for line in r.stdout.splitlines()[1:]]
+    diff, m = fields.compute_difference(a, m1, b, m2)
+    assert m.nx == 12 and m.ny == 12
+    l1, l2, li = fields.l1_norm(diff, m), fields.l2_norm(diff, m), fields.linf_norm(diff, m)
+    np.testing.assert_allclose([float(v) for v in rows[0][1:]], [l1, l2, li], rtol=2e-6)  # printed to 7 digits
+    assert [float(v) for v in rows[1][1:]] == [0.0, 0.0, 0.0]
+    # exact projection: the L2 norm of the difference equals the one from a dense quadrature of
+    # f_a - f_b evaluated cell by cell on the fine mesh
+    g, gw = tsw.gauss_legendre(8)
+
+    def value(c, mesh, x, y):
+        i = np.minimum(((x - mesh.ax) / mesh.dx).astype(int), mesh.nx - 1)
+        j = np.minimum(((y - mesh.ay) / mesh.dy).astype(int), mesh.ny - 1)
+        xb = 2 * (x - mesh.ax) / mesh.dx - (2 * i + 1)
+        yb = 2 * (y - mesh.ay) / mesh.dy - (2 * j + 1)
+        p = c.shape[2] - 1
+        Lx, Ly = fields.legendre_table(p, xb), fields.legendre_table(p, yb)
+        return np.einsum("...de,...d,...e->...", c[i + 1, j + 1], Lx, Ly)
+
+    tot = 0.0
+    for I in range(12):
+        for J in range(12):
+            x = m.ax + (I + (g + 1) / 2) * m.dx
+            y = m.ay + (J + (g + 1) / 2) * m.dy
+            X, Y = np.meshgrid(x, y, indexing="ij")
+            f = value(a, m1, X, Y) - value(b, m2, X, Y)
+            tot += np.sum(np.outer(gw, gw) * f * f) * m.dx * m.dy / 4
+    assert l2 == pytest.approx(math.sqrt(tot), rel=1e-12)
+
+
+def test_python_field_files_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    m = fields.Mesh(3, 5, 0.5, 1.0, 2.0, 4.0)
+    v = rng.standard_normal((5, 7, 2, 2))
+    fields.write_density(str(tmp_path / "v.sd2"), v, m)
+    w, m2 = fields.read_density(str(tmp_path / "v.sd2"))
+    assert m2 == m and np.array_equal(w, v)
+    raw = (tmp_path / "v.sd2").read_bytes()
+    assert len(raw) == 48 + 24 + v.size * 8
+    with pytest.raises(RuntimeError):
+        (tmp_path / "t.sd2").write_bytes(raw[:-8])
+        fields.read_density(str(tmp_path / "t.sd2"))
+    xi, _, _, _ = tsw.ordinates("chebyshev-legendre", 2, True)
+    psi = rng.standard_normal((5, 7, xi.size, 2, 2))
+    info = fields.OrdinateInfo("chebyshev-legendre", 2, True)
+    fields.write_ordinate_flux(str(tmp_path / "p.af2"), psi, m, info)
+    back, m3, info3 = fields.read_ordinate_flux(str(tmp_path / "p.af2"))
+    assert info3 == info and m3 == m and np.array_equal(back, psi)
