@@ -231,6 +231,7 @@ def bench_batched(args):
     sig = [torch.from_numpy(P.sigma_t).to(dev) for P in batch]
     ctx = tsw.Context(0)
     ctx.set_tuning("prefetch", args.prefetch)
+    ctx.set_tuning("batch_direct", args.batch_direct)
     ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
     U = sum(P.unknowns for P in batch)
     ncell = sum(P.nx * P.ny for P in batch)
@@ -318,6 +319,8 @@ def main():
     ap.add_argument("--idc-steps", type=int, default=10, help="IDC steps timed by --workload idc (100 reach t=1)")
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
+    ap.add_argument("--batch-direct", type=int, default=0,
+                    help="batch groups: -1 storer wave, 0 or 1 direct stores where the tables allow")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
     ap.add_argument("--dsa-cycles", type=int, default=1, help="multigrid V-cycles per DSA application")
     ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "

@@ -101,6 +101,9 @@ struct tsw_ctx {
     int tuned_br = INT_MIN, tuned_pf = INT_MIN, tuned_w1 = INT_MIN;
     int last_br = 0, last_pf = 0, last_w1 = 0;
     int one_wave_opt = 0;  // 1: one wave per band of 64 rows, lane-to-lane traces (tuning key one_wave)
+    // batch launch groups: -1 storer wave unless a table only fits without it, else (0, 1) direct stores
+    // whenever every member has element-inverse tables (tuning key batch_direct)
+    int batch_direct_opt = 0;
     DBuf tune_out;
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
     // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
@@ -1957,6 +1960,10 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "one_wave") {
             require(value == 0 || value == 1, TSW_E_ARG, "one_wave must be 0 or 1");
             ctx->one_wave_opt = value;
+        } else if (k == "batch_direct") {
+            require(value >= -1 && value <= 1, TSW_E_ARG, "batch_direct must be -1, 0 or 1");
+            ctx->batch_direct_opt = value;
+            for (auto &G : ctx->groups) G.built32 = G.built32c = G.built64 = false;
         } else if (k == "gmres_refine") {
             require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
             ctx->gmres_refine = value != 0;
@@ -1989,6 +1996,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "refine_steps") *value = ctx->refine_steps;
         else if (k == "gmres_refine") *value = ctx->gmres_refine ? 1 : 0;
         else if (k == "one_wave") *value = ctx->last_w1;
+        else if (k == "batch_direct") *value = ctx->batch_direct_opt;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }
@@ -2437,6 +2445,14 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
     // members whose tables do not fit fall back to elimination, and a mixed group eliminates everywhere
     bool any_direct = false;
     for (auto &sp : ps) any_direct = any_direct || (sp.ainv && sp.direct);
+    if (!any_direct && ctx->batch_direct_opt != -1) {
+        // Without the storer wave and its staging rows a workgroup is one wave and 2*rows*NS values
+        // smaller, so more of them share a CU: the batched sweep is limited by how many item sweeps run
+        // at once (on the 256-problem batch the FP32 kernels take 6.5 ms direct, 8.6 ms with the storer).
+        bool all_tables = true;
+        for (auto &sp : ps) all_tables = all_tables && sp.ainv != nullptr;
+        any_direct = all_tables;
+    }
     if (any_direct) {
         lds = 0;
         for (size_t k = 0; k < ps.size(); ++k) {

@@ -18,8 +18,9 @@ def oracle_sweep(P, src):
     return orc_sweep(g, o, P.sigma_t, src)
 
 
+@pytest.mark.parametrize("batch_direct", [-1, 0, 1])
 @pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
-def test_small_uneven_batch_matches_oracle(precision):
+def test_small_uneven_batch_matches_oracle(precision, batch_direct):
     batch = problems.batched(seed=7, count=10, n_range=(3, 40), degrees=(0, 1, 2, 3), orders=(2, 4, 6, 8))
     rng = np.random.default_rng(3)
     srcs = []
@@ -30,6 +31,7 @@ def test_small_uneven_batch_matches_oracle(precision):
         srcs.append(s)
     print("batch of", len(batch), [(P.nx, P.p, P.nq) for P in batch], flush=True)
     with tsw.Context(0) as ctx:
+        ctx.set_tuning("batch_direct", batch_direct)
         ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
         out = [s.copy() for s in srcs]
         ctx.batch_sweep(out, precision=precision)
