@@ -48,7 +48,10 @@ struct tsw_ctx {
         double inv_dt = -1.0;
         DBuf idx, sig, inv64, inv32;
         bool have64 = false, have32 = false;
+        DBuf key;            // device copy of the sigma_t the table was built from
+        bool keyed = false;  // key holds that copy
     } mat;
+    DBuf same_flag;  // device flag of the input change checks
     static constexpr int kMaxMaterials = 64;
 
     // Inputs and work arrays of the staggered (diagonal-layout) sweep, per precision where needed:
@@ -137,12 +140,15 @@ struct tsw_ctx {
         std::vector<MgLevel> lv;
         std::vector<DBuf> store;
         DBuf lv_dev;
+        DBuf coarse_inv;       // dense inverse of the coarsest level's operator
         int fused_from = 0;    // first level handled by the single-workgroup kernel
         size_t fused_lds = 0;  // its LDS bytes
         const double *sigma_s = nullptr;
         hipGraphExec_t graph = nullptr;
-        // what the hierarchy was built for (rebuilt when any of it changes)
-        std::vector<double> key_st, key_ss;
+        // what the hierarchy was built for (rebuilt when any of it changes): device copies of the cross
+        // sections, compared on the device
+        DBuf key_st, key_ss;
+        bool keyed = false;
         double key_idt = -1.0;
         int key_cycles = 0;
         ~Dsa() {
@@ -288,15 +294,45 @@ SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt)
     return sp;
 }
 
+// True when device array a (n doubles) equals the copy held in key, checked on the device; otherwise
+// key receives a copy of a and the result is false.
+bool same_as_key(tsw_ctx *c, const double *a, size_t n, DBuf &key, bool keyed) {
+    if (keyed && key.bytes >= n * sizeof(double)) {
+        int *flag = static_cast<int *>(c->same_flag.reserve(sizeof(int)));
+        launch_differs(n, a, key.as<double>(), flag, c->stream);
+        int h = 1;
+        TSW_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TSW_HIP(hipStreamSynchronize(c->stream));
+        if (h == 0) return true;
+    }
+    TSW_HIP(hipMemcpyAsync(key.reserve(n * sizeof(double)), a, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return false;
+}
+
 // Builds the material table from device sigma_t (double cell array) for time step dt: reads sigma
 // back, numbers its distinct interior values and uploads the per-cell index.  Leaves the table
-// invalid (so sweeps eliminate per cell) when there are more than kMaxMaterials values.
+// invalid (so sweeps eliminate per cell) when there are more than kMaxMaterials values.  When sigma
+// equals the array of the last build (compared on the device) the table is kept, and the inverses too
+// unless dt changed.
 void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
     auto &m = c->mat;
+    if (c->pattern != Pattern::Staggered) {
+        m.valid = false;
+        m.keyed = false;
+        return;
+    }
+    const Geom &g = c->g;
+    const bool same = same_as_key(c, sigma, g.ncell_h(), m.key, m.keyed);
+    m.keyed = true;
+    if (same) {
+        if (m.inv_dt != inverse_dt(dt)) {
+            m.inv_dt = inverse_dt(dt);
+            m.have64 = m.have32 = false;
+        }
+        return;
+    }
     m.valid = false;
     m.have64 = m.have32 = false;
-    if (c->pattern != Pattern::Staggered) return;
-    const Geom &g = c->g;
     std::vector<double> h(g.ncell_h());
     TSW_HIP(hipMemcpyAsync(h.data(), sigma, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     TSW_HIP(hipStreamSynchronize(c->stream));
@@ -427,15 +463,17 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
     const Geom &g = c->g;
     auto &D = c->dsa;
     const size_t nh = g.ncell_h();
+    const double idt = inverse_dt(dt);
+    const bool same_st = same_as_key(c, sigma_t, nh, D.key_st, D.keyed);
+    const bool same_ss = same_as_key(c, sigma_s, nh, D.key_ss, D.keyed);
+    D.keyed = true;
+    // the apply reads sigma_s from its pointer: reuse the hierarchy only for the same pointer and data
+    if (D.graph && D.sigma_s == sigma_s && D.key_idt == idt && D.key_cycles == c->dsa_cycles && same_st && same_ss)
+        return;
     std::vector<double> st(nh), ss(nh);
     TSW_HIP(hipMemcpyAsync(st.data(), sigma_t, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
     TSW_HIP(hipMemcpyAsync(ss.data(), sigma_s, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
     sync(c);
-    const double idt = inverse_dt(dt);
-    // the captured apply reads sigma_s from its pointer: reuse it only for the same pointer and data
-    if (D.graph && D.sigma_s == sigma_s && D.key_idt == idt && D.key_cycles == c->dsa_cycles && D.key_st == st &&
-        D.key_ss == ss)
-        return;
     if (D.graph) {
         (void)hipGraphExecDestroy(D.graph);
         D.graph = nullptr;
@@ -499,6 +537,58 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
             for (int j = 0; j < f.ny; ++j) L.sv[(size_t)(i / 2) * L.ny + j / 2] += f.sv[(size_t)i * f.ny + j];
         hl.push_back(std::move(L));
     }
+    // the coarsest level (at most 3 x 3 cells) is solved exactly with its dense inverse, the same
+    // operator the smoother relaxes: diagonal sv + the four face conductances, -conductance to each
+    // interior neighbour
+    std::vector<double> cinv;
+    {
+        const HostLevel &h = hl.back();
+        const int n = h.nx * h.ny;
+        std::vector<double> A((size_t)n * n, 0.0);
+        for (int i = 0; i < h.nx; ++i)
+            for (int j = 0; j < h.ny; ++j) {
+                const int c = i * h.ny + j;
+                const double gw = h.cx[(size_t)i * h.ny + j], ge = h.cx[(size_t)(i + 1) * h.ny + j];
+                const double gs = h.cy[(size_t)i * (h.ny + 1) + j], gn = h.cy[(size_t)i * (h.ny + 1) + j + 1];
+                A[(size_t)c * n + c] = h.sv[c] + gw + ge + gs + gn;
+                if (i > 0) A[(size_t)c * n + c - h.ny] = -gw;
+                if (i < h.nx - 1) A[(size_t)c * n + c + h.ny] = -ge;
+                if (j > 0) A[(size_t)c * n + c - 1] = -gs;
+                if (j < h.ny - 1) A[(size_t)c * n + c + 1] = -gn;
+            }
+        // Gauss-Jordan with partial pivoting on [A | I]
+        std::vector<double> I((size_t)n * n, 0.0);
+        for (int k = 0; k < n; ++k) I[(size_t)k * n + k] = 1.0;
+        bool ok = true;
+        for (int k = 0; k < n && ok; ++k) {
+            int piv = k;
+            for (int r = k + 1; r < n; ++r)
+                if (std::fabs(A[(size_t)r * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = r;
+            if (!(std::fabs(A[(size_t)piv * n + k]) > 0.0)) {
+                ok = false;
+                break;
+            }
+            for (int col = 0; col < n; ++col) {
+                std::swap(A[(size_t)k * n + col], A[(size_t)piv * n + col]);
+                std::swap(I[(size_t)k * n + col], I[(size_t)piv * n + col]);
+            }
+            const double d = A[(size_t)k * n + k];
+            for (int col = 0; col < n; ++col) {
+                A[(size_t)k * n + col] /= d;
+                I[(size_t)k * n + col] /= d;
+            }
+            for (int r = 0; r < n; ++r) {
+                if (r == k) continue;
+                const double f = A[(size_t)r * n + k];
+                if (f == 0.0) continue;
+                for (int col = 0; col < n; ++col) {
+                    A[(size_t)r * n + col] -= f * A[(size_t)k * n + col];
+                    I[(size_t)r * n + col] -= f * I[(size_t)k * n + col];
+                }
+            }
+        }
+        if (ok) cinv = std::move(I);
+    }
     D.lv.assign(hl.size(), MgLevel{});
     D.store.clear();
     D.store.resize(hl.size() * 6);
@@ -519,6 +609,8 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
         L.x = static_cast<double *>(D.store[6 * l + 3].reserve(sizeof(double) * n));
         L.b = static_cast<double *>(D.store[6 * l + 4].reserve(sizeof(double) * n));
         L.y = static_cast<double *>(D.store[6 * l + 5].reserve(sizeof(double) * n));
+        L.inv = nullptr;
+        if (l + 1 == hl.size() && !cinv.empty()) L.inv = up(D.coarse_inv, cinv);
     }
     // the small levels (at most 4096 cells each, all of their arrays within one workgroup's LDS) run
     // inside one workgroup; the coarsest level always does
@@ -535,8 +627,6 @@ void dsa_build(tsw_ctx *c, const double *sigma_t, const double *sigma_s, double 
                            hipMemcpyHostToDevice, c->stream));
     sync(c);  // host vectors go out of scope
     D.sigma_s = sigma_s;
-    D.key_st = std::move(st);
-    D.key_ss = std::move(ss);
     D.key_idt = idt;
     D.key_cycles = c->dsa_cycles;
     // the graph holds the V-cycles only (level-0 right-hand side in, level-0 solution out); the
@@ -732,14 +822,14 @@ void diag_load_source(tsw_ctx *c, const double *qref, bool need64, bool need32) 
     d.inflow = qref != nullptr;
     if (qref) {
         const size_t nt_x = (size_t)g.nq * g.ny * (g.p + 1), nt_y = (size_t)g.nq * g.nx * (g.p + 1);
-        if (need64) {
-            launch_inflow<double>(g, qref, dbuf<double>(d.tinx64, nt_x), dbuf<double>(d.tiny64, nt_y), s);
+        if (need64) launch_inflow<double>(g, qref, dbuf<double>(d.tinx64, nt_x), dbuf<double>(d.tiny64, nt_y), s);
+        if (need32) launch_inflow<float>(g, qref, dbuf<float>(d.tinx32, nt_x), dbuf<float>(d.tiny32, nt_y), s);
+        if (need64 && need32)  // both precisions from one read of the source
+            launch_psi_load_dual(g, qref, dbuf<double>(d.q64, g.nq * N * ncell), dbuf<float>(d.q32, g.nq * N * ncell), s);
+        else if (need64)
             launch_psi_convert<double, double>(g, const_cast<double *>(qref), dbuf<double>(d.q64, g.nq * N * ncell), 0, s);
-        }
-        if (need32) {
-            launch_inflow<float>(g, qref, dbuf<float>(d.tinx32, nt_x), dbuf<float>(d.tiny32, nt_y), s);
+        else if (need32)
             launch_psi_convert<double, float>(g, const_cast<double *>(qref), dbuf<float>(d.q32, g.nq * N * ncell), 0, s);
-        }
     }
 }
 
@@ -1624,6 +1714,8 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     ctx->tuned.clear();
     if (ctx->dsa.graph) (void)hipGraphExecDestroy(ctx->dsa.graph);
     ctx->dsa.graph = nullptr;
+    ctx->dsa.keyed = false;
+    ctx->mat.valid = ctx->mat.keyed = ctx->mat.have64 = ctx->mat.have32 = false;
     Geom &g = ctx->g;
     g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
     g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
@@ -2550,13 +2642,14 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
                 TSW_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0));
             }
         };
+        auto nd_of = [](const tsw_ctx *sub) { return (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny; };
+        // inputs per member on the side streams: inflow traces and the source in the diagonal layout
         on_side_streams([&](size_t b, tsw_ctx *sub) {
             dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
             diag_load_source(sub, dev[b], f64 || mixed, !f64);
         });
         if (f64) batch_sweep<double>(ctx);
         else batch_sweep<float>(ctx);
-        auto nd_of = [](const tsw_ctx *sub) { return (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny; };
         // the refinement of mixed_refine_store, its FP32 sweeps batched across members
         const bool fused_store = mixed && ctx->refine_steps == 1;
         auto residual = [&](tsw_ctx *sub, const auto *psi) {

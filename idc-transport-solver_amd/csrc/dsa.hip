@@ -185,10 +185,20 @@ __global__ __launch_bounds__(1024) void k_mg_fused(const MgLevel *__restrict__ l
             for (int c = threadIdx.x; c < n; c += blockDim.x) L.x[c] = 0.0;
             __syncthreads();
         }
-        const int sweeps = k == last ? 16 : 2;
-        for (int it = 0; it < sweeps; ++it) {
-            mg_rbgs_lds(L, 0);
-            mg_rbgs_lds(L, 1);
+        const double *inv = k == last ? lv[l0 + k].inv : nullptr;
+        if (inv) {  // coarsest level solved exactly: x = A^{-1} b, one row per thread
+            for (int c = threadIdx.x; c < n; c += blockDim.x) {
+                double acc = 0.0;
+                for (int m = 0; m < n; ++m) acc += inv[(size_t)c * n + m] * L.b[m];
+                L.x[c] = acc;
+            }
+            __syncthreads();
+        } else {
+            const int sweeps = k == last ? 16 : 2;
+            for (int it = 0; it < sweeps; ++it) {
+                mg_rbgs_lds(L, 0);
+                mg_rbgs_lds(L, 1);
+            }
         }
         if (k == last) break;
         const MgLds C = Ls[k + 1];

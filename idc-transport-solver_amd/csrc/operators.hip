@@ -1,6 +1,7 @@
 // Transport operators around the sweep (Pmv, Smv, Lmv) and the FP64 vector kernels used by the
 // Krylov and source-iteration drivers.  All are HBM-bound streaming kernels over the reference
 // layouts; every thread owns whole coefficient vectors so loads of a wavefront are contiguous.
+#include <algorithm>
 #include <cmath>
 
 #include "cell_solve.hpp"
@@ -231,6 +232,26 @@ __global__ void k_acc_f2d(const float *__restrict__ in, double *__restrict__ out
 }
 void launch_accumulate_f2d(const float *in, double *out, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_acc_f2d, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- input change detection: flag[0] = 1 when a and b differ bitwise in any of n doubles ----
+__global__ void k_clear_flag(int *flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) flag[0] = 0;
+}
+__global__ void k_differs(size_t n, const unsigned long long *__restrict__ a, const unsigned long long *__restrict__ b,
+                          int *flag) {
+    bool diff = false;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        diff |= a[t] != b[t];
+    if (diff) flag[0] = 1;  // every writer stores the same value
+}
+void launch_differs(size_t n, const double *a, const double *b, int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, flag);
+    const size_t blocks = std::min<size_t>(grid_for(n), 1024);
+    hipLaunchKernelGGL(k_differs, dim3((unsigned)blocks), dim3(kBlock), 0, s, n,
+                       reinterpret_cast<const unsigned long long *>(a), reinterpret_cast<const unsigned long long *>(b),
+                       flag);
     TSW_HIP(hipGetLastError());
 }
 

@@ -107,23 +107,31 @@ inline int rows_block(int ny) { return ny <= 64 ? 64 : (ny <= 128 ? 128 : (ny <=
 
 // reference psi [i][j][q][k] (interior) <-> diagonal psi [q][k][pos]; dir 0: ref -> diag, 1: diag -> ref.
 // A block takes R consecutive cells of one diagonal for the ordinates of one quadrant and transposes
-// them through LDS (held in the diagonal precision): on the reference side each cell's ordinates of
-// the quadrant are one contiguous run, on the diagonal side each (q, k) plane receives R consecutive
-// positions.  Both sides are walked with all threads over the flattened tile.
-// SUM (dir 1 only): ref = (TR) diag + (TR) diag2, the tile held in TR.
-template <typename TR, typename TD, bool SUM = false>
-__global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap dm, int nq, int N, int qo1, int qo2,
-                                                     int qo3, int R, int dir, const TD *diag2 = nullptr) {
-    using TT = std::conditional_t<SUM, TR, TD>;
-    extern __shared__ unsigned char smem_raw[];
-    TT *tile = reinterpret_cast<TT *>(smem_raw);  // [R][nqQ*N]
-    const int s = blockIdx.x, Q = blockIdx.z;
-    const int r0 = dm.rlo(s) + (int)blockIdx.y * R;
+// them through LDS: on the reference side each cell's ordinates of the quadrant are one contiguous run,
+// on the diagonal side each (q, k) plane receives R consecutive positions.  Both sides are walked with
+// all threads over the flattened tile, four independent accesses in flight per thread.
+// dir 0: diag = (TD) ref, and d32 (if not null) = (float) ref as well.  dir 1: ref = (TR) diag, or with
+// diag2 ref = (TR) diag + (TR) diag2; the tile is held in TT (double when two outputs or a sum need it).
+struct ConvJob {
+    double *ref;
+    void *diag;         // diagonal array, TD
+    const void *diag2;  // second summand of a store, TD, or null
+    float *d32;         // second FP32 output of a load, or null
+    int nx, ny, nq, N, qo1, qo2, qo3;
+    int R;  // cells per block
+};
+
+template <typename TR, typename TD, typename TT>
+__device__ __forceinline__ void convert_tile(const ConvJob &J, int s, int chunk, int Q, int dir, TT *tile) {
+    const DiagMap dm{J.nx, J.ny};
+    const int R = J.R;
+    const int r0 = dm.rlo(s) + chunk * R;
     const int rhi = s < dm.ny - 1 ? s : dm.ny - 1;
     if (r0 > rhi) return;
     const int nr = (rhi - r0 + 1) < R ? (rhi - r0 + 1) : R;
-    const int q0 = Q == 0 ? 0 : (Q == 1 ? qo1 : (Q == 2 ? qo2 : qo3));
-    const int q1 = Q == 0 ? qo1 : (Q == 1 ? qo2 : (Q == 2 ? qo3 : nq));
+    const int q0 = Q == 0 ? 0 : (Q == 1 ? J.qo1 : (Q == 2 ? J.qo2 : J.qo3));
+    const int q1 = Q == 0 ? J.qo1 : (Q == 1 ? J.qo2 : (Q == 2 ? J.qo3 : J.nq));
+    const int N = J.N, nq = J.nq;
     const int run = (q1 - q0) * N;  // values per cell on the reference side
     if (run == 0) return;
     const size_t ncell = (size_t)dm.cells();
@@ -134,27 +142,36 @@ __global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap 
     dm.mesh(s - r0, r0, xn, yn, i0, j0);
     dm.mesh(s - r0 - 1, r0 + 1, xn, yn, i1, j1);
     const long long cstride = ((long long)(i1 - i0) * (dm.ny + 2) + (j1 - j0)) * nq * N;
-    const gptr<TR> base = (gptr<TR>)(ref + (((size_t)i0 * (dm.ny + 2) + j0) * nq + q0) * N);
-    const gptr<TD> dbase = (gptr<TD>)(diag + (size_t)q0 * N * ncell + (size_t)p0);
+    const gptr<TR> base = (gptr<TR>)(J.ref + (((size_t)i0 * (dm.ny + 2) + j0) * nq + q0) * N);
+    const size_t doff = (size_t)q0 * N * ncell + (size_t)p0;
+    const gptr<TD> dbase = (gptr<TD>)((TD *)J.diag + doff);
     const int total = run * nr;
-    if (!SUM && dir == 0) {
+    if (dir == 0) {
+#pragma unroll 4
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e / run, v = e - t * run;
             tile[e] = TT(base[t * cstride + v]);
         }
         __syncthreads();
+        const gptr<float> d32 = J.d32 ? (gptr<float>)(J.d32 + doff) : nullptr;
+#pragma unroll 4
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e % nr, qk = e / nr;  // consecutive threads: consecutive positions
-            dbase[(size_t)qk * ncell + t] = TD(tile[t * run + qk]);
+            const TT v = tile[t * run + qk];
+            dbase[(size_t)qk * ncell + t] = TD(v);
+            if (d32) d32[(size_t)qk * ncell + t] = float(v);
         }
     } else {
-        const gptr<const TD> dbase2 = SUM ? (gptr<const TD>)(diag2 + (size_t)q0 * N * ncell + (size_t)p0) : nullptr;
+        const gptr<const TD> dbase2 = J.diag2 ? (gptr<const TD>)((const TD *)J.diag2 + doff) : nullptr;
+#pragma unroll 4
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e % nr, qk = e / nr;
-            if (SUM) tile[t * run + qk] = TT(dbase[(size_t)qk * ncell + t]) + TT(dbase2[(size_t)qk * ncell + t]);
-            else tile[t * run + qk] = TT(dbase[(size_t)qk * ncell + t]);
+            TT v = TT(dbase[(size_t)qk * ncell + t]);
+            if (dbase2) v += TT(dbase2[(size_t)qk * ncell + t]);
+            tile[t * run + qk] = v;
         }
         __syncthreads();
+#pragma unroll 4
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int t = e / run, v = e - t * run;
             base[t * cstride + v] = TR(tile[e]);
@@ -162,36 +179,163 @@ __global__ __launch_bounds__(256) void k_psi_convert(TR *ref, TD *diag, DiagMap 
     }
 }
 
-template <typename TR, typename TD>
-void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s) {
-    const DiagMap dm{g.nx, g.ny};
-    int max_run = 0;
-    for (int Q = 0; Q < 4; ++Q) max_run = std::max(max_run, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
-    if (max_run == 0) return;
+template <typename TR, typename TD, typename TT>
+__global__ __launch_bounds__(256) void k_psi_convert(ConvJob J, int dir) {
+    extern __shared__ unsigned char smem_raw[];
+    convert_tile<TR, TD, TT>(J, blockIdx.x, blockIdx.y, blockIdx.z, dir, reinterpret_cast<TT *>(smem_raw));
+}
+
+namespace {
+// values per cell of the largest quadrant
+int max_run_of(const Geom &g) {
+    int m = 0;
+    for (int Q = 0; Q < 4; ++Q) m = std::max(m, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
+    return m;
+}
+// cells per block: the largest power of two (<= 64) whose tile fits in 24 KB of LDS, so that several
+// blocks share a CU (on the 280x280 S12 Q2 lattice this takes the FP64 load from 340 to 284 us)
+int conv_rows(int run, size_t tbytes) {
     int R = 64;
-    while (R > 1 && (size_t)R * max_run * sizeof(TD) > 48 * 1024) R /= 2;
+    while (R > 1 && (size_t)R * run * tbytes > 24 * 1024) R /= 2;
+    return R;
+}
+ConvJob conv_job(const Geom &g, double *ref, void *diag, const void *diag2, float *d32, int R) {
+    ConvJob J{};
+    J.ref = ref;
+    J.diag = diag;
+    J.diag2 = diag2;
+    J.d32 = d32;
+    J.nx = g.nx;
+    J.ny = g.ny;
+    J.nq = g.nq;
+    J.N = g.N();
+    J.qo1 = g.quad_off[1];
+    J.qo2 = g.quad_off[2];
+    J.qo3 = g.quad_off[3];
+    J.R = R;
+    return J;
+}
+template <typename TR, typename TD, typename TT>
+void launch_convert_one(const Geom &g, double *ref, void *diag, const void *diag2, float *d32, int dir, hipStream_t s) {
+    const int max_run = max_run_of(g);
+    if (max_run == 0) return;
+    const int R = conv_rows(max_run, sizeof(TT));
+    const ConvJob J = conv_job(g, ref, diag, diag2, d32, R);
     const int maxlen = g.nx < g.ny ? g.nx : g.ny;
     const dim3 grid(g.nx + g.ny - 1, (maxlen + R - 1) / R, 4);
-    hipLaunchKernelGGL((k_psi_convert<TR, TD>), grid, dim3(256), (size_t)R * max_run * sizeof(TD), s, ref, diag,
-                       dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], R, dir);
+    hipLaunchKernelGGL((k_psi_convert<TR, TD, TT>), grid, dim3(256), (size_t)R * max_run * sizeof(TT), s, J, dir);
     TSW_HIP(hipGetLastError());
+}
+}  // namespace
+
+template <typename TR, typename TD>
+void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s) {
+    launch_convert_one<TR, TD, TD>(g, ref, diag, nullptr, nullptr, dir, s);
 }
 template void launch_psi_convert<double, double>(const Geom &, double *, double *, int, hipStream_t);
 template void launch_psi_convert<double, float>(const Geom &, double *, float *, int, hipStream_t);
 
+void launch_psi_load_dual(const Geom &g, const double *ref, double *d64, float *d32, hipStream_t s) {
+    launch_convert_one<double, double, double>(g, const_cast<double *>(ref), d64, nullptr, d32, 0, s);
+}
+
 void launch_psi_store_sum(const Geom &g, double *ref, const float *a, const float *b, hipStream_t s) {
-    const DiagMap dm{g.nx, g.ny};
-    int max_run = 0;
-    for (int Q = 0; Q < 4; ++Q) max_run = std::max(max_run, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
-    if (max_run == 0) return;
-    int R = 64;
-    while (R > 1 && (size_t)R * max_run * sizeof(double) > 48 * 1024) R /= 2;
-    const int maxlen = g.nx < g.ny ? g.nx : g.ny;
-    const dim3 grid(g.nx + g.ny - 1, (maxlen + R - 1) / R, 4);
-    hipLaunchKernelGGL((k_psi_convert<double, float, true>), grid, dim3(256), (size_t)R * max_run * sizeof(double), s,
-                       ref, const_cast<float *>(a), dm, g.nq, g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], R, 1,
-                       b);
-    TSW_HIP(hipGetLastError());
+    launch_convert_one<double, float, double>(g, ref, const_cast<float *>(a), b, nullptr, 1, s);
+}
+
+
+// Square tiles of kT x kT cells for the transfers between the cell layout [i][j][k] and the four
+// quadrant diagonal layouts [Q][k][pos].  In each quadrant's sweep coordinates (c, r) a tile is again a
+// kT x kT square, whose cells lie on 2 kT - 1 diagonals as runs of consecutive positions; element e of
+// the tile is placed so that consecutive e walk those runs (tile_cell), and the cell-layout side is read
+// or written one tile row (kT cells x K values, contiguous) at a time.
+constexpr int kT = 16;
+
+__device__ __forceinline__ void tile_cell(int e, int &a, int &b) {
+    constexpr int H = kT * (kT + 1) / 2;  // cells on the tile diagonals 0 .. kT-1
+    const bool hi = e >= H;
+    const int f = hi ? kT * kT - 1 - e : e;
+    int d = (int)((sqrtf(8.0f * (float)f + 1.0f) - 1.0f) * 0.5f);
+    if ((d + 1) * (d + 2) / 2 <= f) ++d;
+    if (d * (d + 1) / 2 > f) --d;
+    const int k = f - d * (d + 1) / 2;
+    a = d - k;  // offset in c
+    b = k;      // offset in r
+    if (hi) {   // the upper diagonals, as the mirror image of the lower ones
+        a = kT - 1 - a;
+        b = kT - 1 - b;
+    }
+}
+
+// phi = beta xb + alpha sum_Q phiQ on the interior, one tile per block (xb null: xb = phi)
+__global__ __launch_bounds__(kT *kT) void k_phi_from_quadrants_tiled(const double *__restrict__ phiQ, double *phi,
+                                                                     const double *xb, DiagMap dm, int N,
+                                                                     double alpha, double beta, const int *skip) {
+    if (skip && *skip) return;
+    extern __shared__ double acc[];  // [kT][kT][N], tile rows in i
+    const int i0 = 1 + (int)blockIdx.x * kT, j0 = 1 + (int)blockIdx.y * kT;
+    const int e = threadIdx.x;
+    for (int v = e; v < kT * kT * N; v += kT * kT) acc[v] = 0.0;
+    __syncthreads();
+    const size_t ncell = (size_t)dm.cells();
+    int a, b;
+    tile_cell(e, a, b);
+    for (int Q = 0; Q < 4; ++Q) {
+        const bool xn = quad_xneg(Q), yn = quad_yneg(Q);
+        int c0, r0;  // sweep coordinates of the tile's lowest corner in this quadrant
+        dm.local(xn ? i0 + kT - 1 : i0, yn ? j0 + kT - 1 : j0, xn, yn, c0, r0);
+        const int c = c0 + a, r = r0 + b;
+        int i, j;
+        dm.mesh(c, r, xn, yn, i, j);
+        if (c >= 0 && r >= 0 && c < dm.nx && r < dm.ny && i >= i0 && i < i0 + kT && j >= j0 && j < j0 + kT) {
+            const double *src = phiQ + (size_t)Q * N * ncell + (size_t)dm.pos(c + r, r);
+            double *dst = acc + ((size_t)(i - i0) * kT + (j - j0)) * N;
+            for (int k = 0; k < N; ++k) dst[k] += src[(size_t)k * ncell];
+        }
+        __syncthreads();
+    }
+    const int ilim = dm.nx - i0 + 1 < kT ? dm.nx - i0 + 1 : kT, jlim = dm.ny - j0 + 1 < kT ? dm.ny - j0 + 1 : kT;
+    const int row = jlim * N;  // values of one tile row, contiguous in phi
+    for (int v = e; v < ilim * row; v += kT * kT) {
+        const int ti = v / row, w = v - ti * row;
+        const size_t at = ((size_t)(i0 + ti) * (dm.ny + 2) + j0) * N + w;
+        const double s = acc[(size_t)ti * kT * N + w];
+        phi[at] = (beta == 0.0 ? 0.0 : beta * (xb ? xb[at] : phi[at])) + alpha * s;
+    }
+}
+
+// per-cell array [i][j][k] (halo layout) -> per-quadrant diagonal [Q][k][pos], one tile per block
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kT *kT) void k_cell_to_diag_tiled(const TI *__restrict__ cell, TO *__restrict__ out,
+                                                               DiagMap dm, int K, const int *skip) {
+    if (skip && *skip) return;
+    extern __shared__ double buf[];  // [kT][kT][K] as TI
+    TI *tile = reinterpret_cast<TI *>(buf);
+    const int i0 = 1 + (int)blockIdx.x * kT, j0 = 1 + (int)blockIdx.y * kT;
+    const int e = threadIdx.x;
+    const int ilim = dm.nx - i0 + 1 < kT ? dm.nx - i0 + 1 : kT, jlim = dm.ny - j0 + 1 < kT ? dm.ny - j0 + 1 : kT;
+    const int row = jlim * K;
+    for (int v = e; v < ilim * row; v += kT * kT) {
+        const int ti = v / row, w = v - ti * row;
+        tile[(size_t)ti * kT * K + w] = cell[((size_t)(i0 + ti) * (dm.ny + 2) + j0) * K + w];
+    }
+    __syncthreads();
+    const size_t ncell = (size_t)dm.cells();
+    int a, b;
+    tile_cell(e, a, b);
+    for (int Q = 0; Q < 4; ++Q) {
+        const bool xn = quad_xneg(Q), yn = quad_yneg(Q);
+        int c0, r0;
+        dm.local(xn ? i0 + kT - 1 : i0, yn ? j0 + kT - 1 : j0, xn, yn, c0, r0);
+        const int c = c0 + a, r = r0 + b;
+        int i, j;
+        dm.mesh(c, r, xn, yn, i, j);
+        if (c >= 0 && r >= 0 && c < dm.nx && r < dm.ny && i >= i0 && i < i0 + kT && j >= j0 && j < j0 + kT) {
+            TO *dst = out + (size_t)Q * K * ncell + (size_t)dm.pos(c + r, r);
+            const TI *src = tile + ((size_t)(i - i0) * kT + (j - j0)) * K;
+            for (int k = 0; k < K; ++k) dst[(size_t)k * ncell] = TO(src[k]);
+        }
+    }
 }
 
 // per-cell array [i][j][k] (k < K, halo layout) -> per-quadrant diagonal [Q][k][pos]
@@ -213,8 +357,14 @@ __global__ void k_cell_to_diag(const TI *cell, TO *out, DiagMap dm, int K, const
 template <typename TI, typename TO>
 void launch_cell_to_diag(const Geom &g, const TI *cell, TO *out, int K, hipStream_t s, const int *skip) {
     const DiagMap dm{g.nx, g.ny};
-    hipLaunchKernelGGL((k_cell_to_diag<TI, TO>), dim3(g.nx + g.ny - 1, 4), dim3(rows_block(g.ny)), 0, s, cell, out, dm,
-                       K, skip);
+    const size_t lds = (size_t)kT * kT * K * sizeof(TI);
+    if (lds <= 32 * 1024) {
+        const dim3 grid((unsigned)((g.nx + kT - 1) / kT), (unsigned)((g.ny + kT - 1) / kT));
+        hipLaunchKernelGGL((k_cell_to_diag_tiled<TI, TO>), grid, dim3(kT * kT), lds, s, cell, out, dm, K, skip);
+    } else {
+        hipLaunchKernelGGL((k_cell_to_diag<TI, TO>), dim3(g.nx + g.ny - 1, 4), dim3(rows_block(g.ny)), 0, s, cell, out,
+                           dm, K, skip);
+    }
     TSW_HIP(hipGetLastError());
 }
 template void launch_cell_to_diag<double, double>(const Geom &, const double *, double *, int, hipStream_t, const int *);
@@ -308,8 +458,15 @@ void launch_pmv_diag(const Geom &g, const TI *psi, const double *w, double *phiQ
     const size_t n = (size_t)dm.cells() * g.N();
     hipLaunchKernelGGL((k_pmv_quadrant<TI>), dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, s, psi, w, phiQ,
                        (size_t)dm.cells(), g.N(), g.quad_off[1], g.quad_off[2], g.quad_off[3], g.nq, skip);
-    hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, xb, dm,
-                       g.N(), alpha, beta, skip);
+    const size_t lds = (size_t)kT * kT * g.N() * sizeof(double);
+    if (lds <= 32 * 1024) {
+        const dim3 grid((unsigned)((g.nx + kT - 1) / kT), (unsigned)((g.ny + kT - 1) / kT));
+        hipLaunchKernelGGL(k_phi_from_quadrants_tiled, grid, dim3(kT * kT), lds, s, phiQ, phi, xb, dm, g.N(), alpha,
+                           beta, skip);
+    } else {
+        hipLaunchKernelGGL(k_phi_from_quadrants, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phiQ, phi, xb, dm,
+                           g.N(), alpha, beta, skip);
+    }
     TSW_HIP(hipGetLastError());
 }
 template void launch_pmv_diag<double>(const Geom &, const double *, const double *, double *, double *, double, double,

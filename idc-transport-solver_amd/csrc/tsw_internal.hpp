@@ -138,6 +138,7 @@ struct MgLevel {
     int nx, ny;
     double *cx, *cy, *sv;  // face conductances ((nx+1)*ny, nx*(ny+1)), absorption x volume (nx*ny)
     double *x, *b, *y;     // pre-smoothed iterate, right-hand side, post-smoothed result (nx*ny each)
+    const double *inv;     // coarsest level: dense inverse of its operator (row-major n x n), else null
 };
 constexpr int kMgMaxLevels = 16;
 void launch_dsa_rhs(const Geom &g, const double *x, const double *sigma_s, double *b, hipStream_t s);
@@ -191,6 +192,8 @@ void launch_residual_diag(const Geom &g, const double *qsrc, const double *phi_d
                           const double *xi, const double *eta, double inv_dt, TO *res, hipStream_t s);
 // reference psi (interior) <- (double) a + (double) b, a and b diagonal-layout FP32 arrays
 void launch_psi_store_sum(const Geom &g, double *ref, const float *a, const float *b, hipStream_t s);
+// reference psi -> diagonal psi in FP64 and FP32 from one read
+void launch_psi_load_dual(const Geom &g, const double *ref, double *d64, float *d32, hipStream_t s);
 
 // Largest dynamic LDS allocation one workgroup may make on the current device (bytes).
 size_t max_lds_per_block();
@@ -219,6 +222,8 @@ void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
 
 // y[k] = a*x[k] + b*y[k]
 void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipStream_t s);
+// flag[0] <- 1 if a and b differ bitwise anywhere in n doubles, else 0
+void launch_differs(size_t n, const double *a, const double *b, int *flag, hipStream_t s);
 // out[i] = sum_k X_i[k] * y[k], i < m, X_i = X + i*ld; uses `partials` scratch; result on device
 void launch_multi_dot(size_t n, int m, const double *X, size_t ld, const double *y, double *out, double *partials,
                       int nblocks, hipStream_t s);
