@@ -81,6 +81,38 @@ __device__ __forceinline__ void assemble_matrix(T (&A)[P * P * P * P], T xi, T e
     for (int r = 0; r < N; ++r) A[r + N * r] += diag_add;
 }
 
+// y = A x for the matrix assemble_matrix builds, without forming it: A is Kx (x) I + I (x) Ky plus
+// diag_add on the diagonal, Kx(d,a) = cx (2d+1) (s(d,a) - 2 [a < d, d - a odd]) with s = 1, or
+// (-1)^(d+a+1) for xi < 0 (and Ky likewise in e, b with cy, eta), so the product costs 2 P^3 + P^2
+// multiply-adds and P^2 coefficients instead of P^4 of each.
+template <typename T, int P>
+__device__ __forceinline__ void apply_element(const T (&x)[P * P], T xi, T eta, T dx, T dy, T diag_add,
+                                              T (&y)[P * P]) {
+    const bool xneg = xi < T(0), yneg = eta < T(0);
+    const T cx = dy * xi, cy = dx * eta;
+    T kx[P][P], ky[P][P];
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            const T sx = xneg ? T(ipow_m1(d + a + 1)) : T(1), sy = yneg ? T(ipow_m1(d + a + 1)) : T(1);
+            const T low = (a < d && ((d - a) & 1)) ? T(2) : T(0);
+            kx[d][a] = cx * T(2 * d + 1) * (sx - low);
+            ky[d][a] = cy * T(2 * d + 1) * (sy - low);
+        }
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int e = 0; e < P; ++e) {
+            T acc = diag_add * x[e + P * d];
+#pragma unroll
+            for (int a = 0; a < P; ++a) acc += kx[d][a] * x[e + P * a];
+#pragma unroll
+            for (int b = 0; b < P; ++b) acc += ky[e][b] * x[b + P * d];
+            y[e + P * d] = acc;
+        }
+}
+
 // Unpivoted Gaussian elimination, same operation order as the reference's SIMD kernel.
 template <typename T, int N>
 __device__ __forceinline__ void ge_solve(T (&A)[N * N], T (&b)[N]) {

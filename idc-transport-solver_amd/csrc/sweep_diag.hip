@@ -525,9 +525,9 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
     }
     // (L + 1/dt) x = (Aq x + (sigma + 1/dt) dx dy x - Kx tx - Ky ty) / (dx dy): the sweep's element
     // system with the inflow terms moved back to the left
-    double A[N * N];
     const size_t pQ = (size_t)Q * ncell + p;
-    assemble_matrix<double, P>(A, xi, eta, dx, dy, (sigt_d[pQ] + inv_dt) * dx * dy);
+    double ax[N];
+    apply_element<double, P>(x, xi, eta, dx, dy, (sigt_d[pQ] + inv_dt) * dx * dy, ax);
     double zero[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) zero[k] = 0.0;
@@ -537,10 +537,7 @@ __global__ void k_residual_diag(const double *__restrict__ qsrc, const double *_
     TO *pr = res + (size_t)q * N * ncell + p;
 #pragma unroll
     for (int row = 0; row < N; ++row) {
-        double lx = 0.0;
-#pragma unroll
-        for (int col = 0; col < N; ++col) lx += A[row + N * col] * x[col];
-        lx = (lx - inflow[row]) * inv_area;
+        const double lx = (ax[row] - inflow[row]) * inv_area;
         double v = qsrc[(size_t)q * N * ncell + (size_t)row * ncell + p];
         if (phi_d) v += sigs_d[pQ] * phi_d[(size_t)Q * N * ncell + (size_t)row * ncell + p] * kInv4Pi;
         pr[(size_t)row * ncell] = TO(v - lx);
