@@ -180,7 +180,7 @@ __device__ __forceinline__ void convert_tile(const ConvJob &J, int s, int chunk,
 }
 
 template <typename TR, typename TD, typename TT>
-__global__ __launch_bounds__(256) void k_psi_convert(ConvJob J, int dir) {
+__global__ __launch_bounds__(1024) void k_psi_convert(ConvJob J, int dir) {
     extern __shared__ unsigned char smem_raw[];
     convert_tile<TR, TD, TT>(J, blockIdx.x, blockIdx.y, blockIdx.z, dir, reinterpret_cast<TT *>(smem_raw));
 }
@@ -192,11 +192,14 @@ int max_run_of(const Geom &g) {
     for (int Q = 0; Q < 4; ++Q) m = std::max(m, (g.quad_off[Q + 1] - g.quad_off[Q]) * g.N());
     return m;
 }
-// cells per block: the largest power of two (<= 64) whose tile fits in 24 KB of LDS, so that several
-// blocks share a CU (on the 280x280 S12 Q2 lattice this takes the FP64 load from 340 to 284 us)
-int conv_rows(int run, size_t tbytes) {
+// Cells per block: the largest power of two (<= 64) whose tile fits the direction's LDS budget.  Long
+// runs of consecutive positions on the diagonal side pay off most when loading (measured on the
+// 280x280 S12 Q2 lattice, FP64: 280 us with 64-cell tiles and 1024 threads against 353 us with 16-cell
+// tiles; the store runs at 244 us with 32-cell tiles and 512 threads).
+int conv_rows(int run, size_t tbytes, int dir) {
+    const size_t budget = (dir == 0 ? 96 : 48) * 1024;
     int R = 64;
-    while (R > 1 && (size_t)R * run * tbytes > 24 * 1024) R /= 2;
+    while (R > 1 && (size_t)R * run * tbytes > budget) R /= 2;
     return R;
 }
 ConvJob conv_job(const Geom &g, double *ref, void *diag, const void *diag2, float *d32, int R) {
@@ -219,11 +222,15 @@ template <typename TR, typename TD, typename TT>
 void launch_convert_one(const Geom &g, double *ref, void *diag, const void *diag2, float *d32, int dir, hipStream_t s) {
     const int max_run = max_run_of(g);
     if (max_run == 0) return;
-    const int R = conv_rows(max_run, sizeof(TT));
+    const int R = conv_rows(max_run, sizeof(TT), dir);
     const ConvJob J = conv_job(g, ref, diag, diag2, d32, R);
     const int maxlen = g.nx < g.ny ? g.nx : g.ny;
     const dim3 grid(g.nx + g.ny - 1, (maxlen + R - 1) / R, 4);
-    hipLaunchKernelGGL((k_psi_convert<TR, TD, TT>), grid, dim3(256), (size_t)R * max_run * sizeof(TT), s, J, dir);
+    const size_t lds = (size_t)R * max_run * sizeof(TT);
+    auto kern = k_psi_convert<TR, TD, TT>;
+    if (lds > 64 * 1024)
+        TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, grid, dim3(dir == 0 ? 1024 : 512), lds, s, J, dir);
     TSW_HIP(hipGetLastError());
 }
 }  // namespace
