@@ -407,6 +407,7 @@ def main():
         ctx.set_tuning("prefetch", args.prefetch)
     t_sw32 = contract_sweeps(tsw.FP32)
     tuned32 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
+    store32 = 2 if ctx.get_tuning("one_wave") else 0  # kernel template argument: storer wave or one wave
     t_sw64 = contract_sweeps(tsw.FP64)
     tuned64 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
     print(f"bench: sweep tuning (band_rows, prefetch): fp32 {tuned32} fp64 {tuned64}", file=sys.stderr, flush=True)
@@ -476,7 +477,7 @@ def main():
         "mixed_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, false> (FP32 in-place "
+                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, {store32}> (FP32 in-place "
                                f"operator-contract sweep, band_rows {tuned32[0]})",
                      "algorithmic_bytes_per_launch": bytes32,
                      "traffic": traffic32,
