@@ -701,8 +701,8 @@ struct Gmres {
     // <= abs_tol.  Returns true when converged.
     template <typename Apply>
     bool solve(Apply &&apply, const double *b, double *x, double *r, double abs_tol, int max_its, int &its,
-               double &res) {
-        return solve(apply, apply, -1.0, b, x, r, abs_tol, max_its, its, res);
+               double &res, bool x_zero = false) {
+        return solve(apply, apply, -1.0, b, x, r, abs_tol, max_its, its, res, x_zero);
     }
 
     // Inexact (relaxed) form: apply_lo, a cheaper and less accurate product, replaces apply once the
@@ -710,16 +710,23 @@ struct Gmres {
     // attainable residual at abs_tol (the relaxation strategy of inexact Krylov methods: Bouras and
     // Fraysse 2005, Simoncini and Szyld 2003), so a fixed relative error d of apply_lo is admissible from
     // ||r_k|| <= abs_tol / d on.
+    // x_zero: x is zero on entry, so the first residual is b itself and needs no product.
     template <typename Apply, typename ApplyLo>
     bool solve(Apply &&apply, ApplyLo &&apply_lo, double lo_below, const double *b, double *x, double *r,
-               double abs_tol, int max_its, int &its, double &res) {
+               double abs_tol, int max_its, int &its, double &res, bool x_zero = false) {
         std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
         double *h1 = hdev, *h2 = hdev + (m + 2);
         its = 0;
         res = 0.0;
+        bool zero = x_zero;
         while (true) {
-            apply(x, r);  // restart residuals with the accurate product
-            launch_axpby(n, 1.0, b, -1.0, r, c->stream);  // r = b - A x
+            if (zero) {
+                launch_axpby(n, 1.0, b, 0.0, r, c->stream);  // r = b
+                zero = false;
+            } else {
+                apply(x, r);  // restart residuals with the accurate product
+                launch_axpby(n, 1.0, b, -1.0, r, c->stream);  // r = b - A x
+            }
             const double beta = norm(r);
             res = beta;
             if (beta <= abs_tol) return true;
@@ -1271,7 +1278,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);  // b = P L^{-1} Q (:461-466)
         diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
         if (dsa) dsa_apply(c, b, b);
-        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
+        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final, true);
         const double *phi_d = diag_phi<double>(c, x);            // psi = L^{-1}(Q + S phi) (:504-505)
         diag_sweep<double>(c, dt, q_d, phi_d, true, psi_d);
         diag_store<double>(c, psi_d, Q);
@@ -1297,7 +1304,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
         if (dsa) dsa_apply(c, b, b);
         converged = solver.solve(apply64, apply32, abs_tol / kFp32SweepErr, b, x, r, abs_tol, max_its, its_total,
-                                 res_final);
+                                 res_final, true);
         diag_sweep<double>(c, dt, q_d, diag_phi<double>(c, x), true, psi_d);
         diag_store<double>(c, psi_d, Q);
     } else {
@@ -1337,7 +1344,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
             const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
             int its = 0;
             double rr = 0.0;
-            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
+            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr, true);
             its_total += its;
             // d = L^{-1}(r + S phi_d); psi += d
             const float *phi_d = diag_phi<float>(c, x);
@@ -1387,7 +1394,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         sweep_prec(ctx, psi, st, dt, precision);
         launch_pmv<double, double>(g, psi, b, w, 1.0, 0.0, 0, s);
         if (dsa) dsa_apply(ctx, b, b);
-        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final);
+        converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final, true);
         // source <- L^{-1}(Q + S phi) (:504-505)
         launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
         sweep_prec(ctx, Q, st, dt, precision);
@@ -1445,7 +1452,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
             const double inner_tol = std::max(0.5 * abs_tol, 1e-5 * res_final);
             int its = 0;
             double rr = 0.0;
-            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr);
+            solver.solve(apply32, b, x, r, inner_tol, max_its - its_total, its, rr, true);
             its_total += its;
             launch_convert_d2f(res64, p32, npsi, s);
             launch_convert_d2f(x, phi32, n, s);
