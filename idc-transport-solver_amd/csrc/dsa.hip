@@ -112,7 +112,8 @@ __global__ void k_mg_restrict_residual(const double *__restrict__ x, const doubl
 
 // The V-cycle from level l0 down, in one workgroup with every level's arrays in LDS (for levels small
 // enough that launches would dominate): descend (two red-black sweeps, residual restricted to the next
-// level), coarsest level (16 sweeps), ascend (prolongation added, two sweeps in reverse colour order).
+// level), coarsest level (x = A^{-1} b with its dense inverse when the level carries one, else 16
+// sweeps), ascend (prolongation added, two sweeps in reverse colour order).
 // Gauss-Seidel in place, phases separated by block barriers.  Level l0 starts from zero, or from its y
 // when from_y; its result goes to lv[l0].y.  LDS per level: cx, cy, sv, b, x (mg_fused_lds_doubles).
 struct MgLds {
