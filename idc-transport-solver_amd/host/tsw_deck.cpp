@@ -140,6 +140,74 @@ class Hohlraum2Problem : public HohlraumProblem {
     }
 };
 
+// Isotropic initial bump exp(1 / (r^2/R^2 - 1)) for r < R, normalised to unit integral, in a medium of
+// uniform sigma_t = sigma_s = sigma_val with no source (BumpProblem.cpp, Mollifier.cpp:200-300).  The
+// integral over the disc is pi R^2 int_0^1 exp(-1/t) dt (substituting t = 1 - r^2/R^2); the reference
+// evaluates it by adaptive Gauss-Kronrod to 1e-15, here by Gauss-Legendre panels to the same accuracy.
+class BumpProblem : public Problem {
+  public:
+    explicit BumpProblem(const ParameterList &params)
+        : sigma_(params.GetDouble("sigma_val")), R_(params.GetDouble("bump_radius")) {
+        if (!(R_ > 0.0)) throw std::invalid_argument("bump_radius must be positive");
+        double g[20], w[20];
+        tsw_gauss_legendre(20, g, w);
+        double I = 0.0;
+        const int panels = 64;
+        for (int k = 0; k < panels; ++k)
+            for (int l = 0; l < 20; ++l) {
+                const double t = (k + 0.5 * (g[l] + 1.0)) / panels;
+                I += 0.5 / panels * w[l] * std::exp(-1.0 / t);
+            }
+        norm_ = 1.0 / (M_PI * R_ * R_ * I);
+    }
+    std::string GetDescriptor() const override { return "bump"; }
+    double InitialCondition(double x, double y) const override {
+        const double u = (x * x + y * y) / (R_ * R_);
+        return u < 1.0 ? std::exp(1.0 / (u - 1.0)) * norm_ : 0.0;
+    }
+    double TotalCross(double, double) const override { return sigma_; }
+    double ScatterCross(double, double) const override { return sigma_; }
+
+  private:
+    double sigma_, R_, norm_;
+};
+
+// Sphere (disc in 2D) of a pure scatterer with smooth transitions between an inner, an intermediate and
+// an outer cross section, and a unit source smoothed over the inner transition
+// (InhomogeneousSphereProblem.cpp:24-275).  Each transition of width w around radius R is
+// 1 / (1 + exp(2 rho / (R + rho - r) - 2 rho / (r - R + rho))), rho = w / 2.
+class InhomogeneousSphereProblem : public Problem {
+  public:
+    explicit InhomogeneousSphereProblem(const ParameterList &params)
+        : ri_(params.GetDouble("sphere_inner_radius")), ro_(params.GetDouble("sphere_outer_radius")),
+          wi_(params.GetDouble("sphere_inner_trans_radius")), wo_(params.GetDouble("sphere_outer_trans_radius")),
+          si_(params.GetDouble("sphere_inner_sigma_val")), sm_(params.GetDouble("sphere_interm_sigma_val")),
+          so_(params.GetDouble("sphere_outer_sigma_val")) {}
+    std::string GetDescriptor() const override { return "inhomogeneous-sphere"; }
+    double Source(double x, double y) const override {
+        const double r = std::sqrt(x * x + y * y);
+        if (r <= ri_ - 0.5 * wi_) return 1.0;
+        if (r < ri_ + 0.5 * wi_) return step(r, ri_, wi_);
+        return 0.0;
+    }
+    double TotalCross(double x, double y) const override {
+        const double r = std::sqrt(x * x + y * y);
+        if (r <= ri_ - 0.5 * wi_) return si_;
+        if (r < ri_ + 0.5 * wi_) return sm_ + (si_ - sm_) * step(r, ri_, wi_);
+        if (r <= ro_ - 0.5 * wo_) return sm_;
+        if (r < ro_ + 0.5 * wo_) return so_ + (sm_ - so_) * step(r, ro_, wo_);
+        return so_;
+    }
+    double ScatterCross(double x, double y) const override { return TotalCross(x, y); }
+
+  private:
+    static double step(double r, double R, double w) {
+        const double rho = 0.5 * w;
+        return 1.0 / (1.0 + std::exp(2 * rho / ((R + rho) - r) - 2 * rho / (r - (R - rho))));
+    }
+    double ri_, ro_, wi_, wo_, si_, sm_, so_;
+};
+
 double legendre(int n, double x) {
     double p0 = 1.0, p1 = x;
     if (n == 0) return 1.0;
@@ -163,6 +231,10 @@ std::unique_ptr<Problem> Problem::Create(const ParameterList &params) {
         p.reset(new HohlraumProblem);
     else if (name == "hohlraum2")
         p.reset(new Hohlraum2Problem);
+    else if (name == "bump")
+        p.reset(new BumpProblem(params));
+    else if (name == "inhomogeneous-sphere")
+        p.reset(new InhomogeneousSphereProblem(params));
     else
         throw std::out_of_range("problem '" + name + "' is not available");
     if (moll >= 1e-14) throw std::invalid_argument("mollified problem functions (moll_radius > 0) are not available");

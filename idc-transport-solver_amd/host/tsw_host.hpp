@@ -289,8 +289,10 @@ class IDCIntegrator {
 // Problem definitions selected by the deck key "problem" (Factory<Problem>, src/utils/Problems/*):
 // functions of position for the initial condition, isotropic source and cross sections, and the inflow
 // written into the source's halo cells.  Built: "lattice" (LatticeProblem.cpp:100-324), "hohlraum"
-// (HohlraumProblem.cpp:60-272) and "hohlraum2" (Hohlraum2Problem.cpp:14-148); other names throw
-// std::out_of_range.  The deck key moll_radius is required, as Problem's constructor requires it
+// (HohlraumProblem.cpp:60-272), "hohlraum2" (Hohlraum2Problem.cpp:14-148), "bump" (BumpProblem.cpp,
+// keys sigma_val, bump_radius) and "inhomogeneous-sphere" (InhomogeneousSphereProblem.cpp, keys
+// sphere_inner/outer_radius, sphere_inner/outer_trans_radius, sphere_inner/interm/outer_sigma_val);
+// other names (Reed's problem is 1D) throw std::out_of_range.  The deck key moll_radius is required, as Problem's constructor requires it
 // (Problem.cpp:20-35); mollified evaluation (moll_radius >= 1e-14) is not built and throws
 // std::invalid_argument.
 class Problem {

@@ -3,7 +3,8 @@ problem inputs Init builds from them (src/utils/init.cpp:422-946): cross section
 condition projected into the DG space, inflow halos, and the named ordinate set.
 
 Problems: "lattice" (LatticeProblem.cpp:100-324), "hohlraum" (HohlraumProblem.cpp:60-272),
-"hohlraum2" (Hohlraum2Problem.cpp:14-148).  The GPU sweeps take one cross-section value per cell, so
+"hohlraum2" (Hohlraum2Problem.cpp:14-148), "bump" (BumpProblem.cpp) and "inhomogeneous-sphere"
+(InhomogeneousSphereProblem.cpp).  The GPU sweeps take one cross-section value per cell, so
 piecewise_cross must be true; periodic and reflecting boundaries and mollified problem functions are
 not available (ValueError)."""
 from __future__ import annotations
@@ -60,11 +61,28 @@ def _in(v, a):
     return (v > a) & (v <= a + 1.0)
 
 
-class Lattice:
+def _zeros(x, y):
+    return np.zeros(np.broadcast(x, y).shape)
+
+
+class ProblemFunctions:
+    """Problem functions of position (vectorised); defaults: no source, no initial condition, vacuum."""
+    domain = None  # (ax, bx, ay, by) the problem imposes, or None
+
+    def source(self, x, y):
+        return _zeros(x, y)
+
+    def initial(self, x, y):
+        return _zeros(x, y)
+
+    def inflow(self, q):
+        pass
+
+
+class Lattice(ProblemFunctions):
     domain = (0.0, 7.0, 0.0, 7.0)
 
-    @staticmethod
-    def source(x, y):
+    def source(self, x, y):
         return np.where((x > 3.0) & (x < 4.0) & (y > 3.0) & (y < 4.0), 1.0, 0.0)
 
     @staticmethod
@@ -73,47 +91,33 @@ class Lattice:
         a |= (_in(x, 2.0) | _in(x, 4.0)) & (_in(y, 2.0) | _in(y, 4.0))
         return a | (_in(x, 3.0) & _in(y, 5.0))
 
-    @classmethod
-    def total(cls, x, y):
-        return np.where(cls.absorber(x, y), 10.0, 1.0)
+    def total(self, x, y):
+        return np.where(self.absorber(x, y), 10.0, 1.0)
 
-    @classmethod
-    def scatter(cls, x, y):
-        return np.where(cls.absorber(x, y), 0.0, 1.0)
-
-    @staticmethod
-    def inflow(q):
-        pass
+    def scatter(self, x, y):
+        return np.where(self.absorber(x, y), 0.0, 1.0)
 
 
-class Hohlraum:
+class Hohlraum(ProblemFunctions):
     domain = (0.0, 1.3, 0.0, 1.3)
-
-    @staticmethod
-    def source(x, y):
-        return np.zeros(np.broadcast(x, y).shape)
 
     @staticmethod
     def wall(x, y):
         w = (x > 1.25) | (y <= 0.05) | (y > 1.25)
         return w | ((y > 0.25) & (y <= 1.05) & ((x <= 0.05) | ((x > 0.45) & (x <= 0.85))))
 
-    @classmethod
-    def total(cls, x, y):
-        return np.where(cls.wall(x, y), 100.0, 0.1)
+    def total(self, x, y):
+        return np.where(self.wall(x, y), 100.0, 0.1)
 
-    @classmethod
-    def scatter(cls, x, y):
-        return np.where(cls.wall(x, y), 1.0, 0.1)
+    def scatter(self, x, y):
+        return np.where(self.wall(x, y), 1.0, 0.1)
 
-    @staticmethod
-    def inflow(q):
+    def inflow(self, q):
         q[0, :, 0, 0] = 1.0
 
 
 class Hohlraum2(Hohlraum):
-    @staticmethod
-    def scatter(x, y):
+    def scatter(self, x, y):
         x, y = np.broadcast_arrays(x, y)
         s = np.full(x.shape, 0.1)
         conds = [  # first matching rule wins, so apply in reverse order
@@ -129,7 +133,67 @@ class Hohlraum2(Hohlraum):
         return s
 
 
-PROBLEMS = {"lattice": Lattice, "hohlraum": Hohlraum, "hohlraum2": Hohlraum2}
+class Bump(ProblemFunctions):
+    """Normalised bump exp(1/(r^2/R^2 - 1)), r < R, in uniform sigma_t = sigma_s (BumpProblem.cpp); the
+    disc integral is pi R^2 int_0^1 exp(-1/t) dt."""
+
+    def __init__(self, params):
+        self.sigma = float(params["sigma_val"])
+        self.R = float(params["bump_radius"])
+        if not self.R > 0:
+            raise ValueError("bump_radius must be positive")
+        g, w = gauss_legendre(20)
+        panels = 64
+        t = (np.arange(panels)[:, None] + 0.5 * (g[None, :] + 1.0)) / panels
+        integral = float(np.sum(0.5 / panels * w[None, :] * np.exp(-1.0 / t)))
+        self.norm = 1.0 / (math.pi * self.R * self.R * integral)
+
+    def initial(self, x, y):
+        u = (x * x + y * y) / (self.R * self.R)
+        with np.errstate(divide="ignore", over="ignore"):
+            return np.where(u < 1.0, np.exp(1.0 / np.where(u < 1.0, u - 1.0, -1.0)) * self.norm, 0.0)
+
+    def total(self, x, y):
+        return np.full(np.broadcast(x, y).shape, self.sigma)
+
+    scatter = total
+
+
+class InhomogeneousSphere(ProblemFunctions):
+    """Disc of pure scatterer with smooth transitions (InhomogeneousSphereProblem.cpp)."""
+
+    def __init__(self, params):
+        g = lambda k: float(params["sphere_" + k])
+        self.ri, self.ro = g("inner_radius"), g("outer_radius")
+        self.wi, self.wo = g("inner_trans_radius"), g("outer_trans_radius")
+        self.si, self.sm, self.so = g("inner_sigma_val"), g("interm_sigma_val"), g("outer_sigma_val")
+
+    @staticmethod
+    def step(r, R, w):
+        rho = 0.5 * w
+        inside = (r > R - rho) & (r < R + rho)
+        rr = np.where(inside, r, R)  # keep the exponent finite outside the transition
+        with np.errstate(over="ignore"):
+            return 1.0 / (1.0 + np.exp(2 * rho / ((R + rho) - rr) - 2 * rho / (rr - (R - rho))))
+
+    def source(self, x, y):
+        r = np.sqrt(x * x + y * y)
+        return np.where(r <= self.ri - 0.5 * self.wi, 1.0,
+                        np.where(r < self.ri + 0.5 * self.wi, self.step(r, self.ri, self.wi), 0.0))
+
+    def total(self, x, y):
+        r = np.sqrt(x * x + y * y)
+        out = np.full(r.shape, self.so)
+        out = np.where(r < self.ro + 0.5 * self.wo, self.so + (self.sm - self.so) * self.step(r, self.ro, self.wo), out)
+        out = np.where(r <= self.ro - 0.5 * self.wo, self.sm, out)
+        out = np.where(r < self.ri + 0.5 * self.wi, self.sm + (self.si - self.sm) * self.step(r, self.ri, self.wi), out)
+        return np.where(r <= self.ri - 0.5 * self.wi, self.si, out)
+
+    scatter = total
+
+
+PROBLEMS = {"lattice": lambda p: Lattice(), "hohlraum": lambda p: Hohlraum(), "hohlraum2": lambda p: Hohlraum2(),
+            "bump": Bump, "inhomogeneous-sphere": InhomogeneousSphere}
 
 
 def project(fn, mesh: Mesh, p, apply_limiter=True, proj_order=14):
@@ -170,16 +234,16 @@ def project(fn, mesh: Mesh, p, apply_limiter=True, proj_order=14):
 def init(params):
     """(Problem, extras) for a deck given as a dict (see read_deck).  Problem carries the mesh, DG
     degree, ordinate set, per-cell cross sections and the source density with inflow halos
-    (q_dg); extras holds the projections by the reference's file names, the initial angular flux, and
-    final_time / max_step_size (tfinal, cfl * dx)."""
+    (q_dg); extras holds the projections by the reference's file names, the initial angular flux psi0
+    (zero halos), and final_time / max_step_size (tfinal, cfl * dx)."""
     params = dict(params)
     name = params["problem"]
     if name not in PROBLEMS:
         raise KeyError(f"problem '{name}' is not available")
     if float(params["moll_radius"]) >= 1e-14:
         raise ValueError("mollified problem functions (moll_radius > 0) are not available")
-    prob = PROBLEMS[name]
-    for k, v in zip(("ax", "bx", "ay", "by"), prob.domain):  # OverrideOptions: only keys the deck names
+    prob = PROBLEMS[name](params)
+    for k, v in zip(("ax", "bx", "ay", "by"), prob.domain or ()):  # OverrideOptions: only keys the deck names
         if k in params and float(params[k]) != v:
             params[k] = repr(v)
     if name.startswith("hohlraum"):
@@ -206,13 +270,16 @@ def init(params):
     st = project(prob.total, mesh, 0)
     ss = project(prob.scatter, mesh, 0)
     q0 = project(prob.source, mesh, p)
-    ic = project(lambda x, y: np.zeros(np.broadcast(x, y).shape), mesh, p)
+    ic = project(prob.initial, mesh, p)
     q = q0.copy()
     prob.inflow(q)
     P = Problem(name, mesh.nx, mesh.ny, p, mesh.ax, mesh.bx, mesh.ay, mesh.by, xi, eta, w, off,
                 st[:, :, 0, 0].copy(), ss[:, :, 0, 0].copy(), q[:, :, 0, 0].copy(), q_dg=q)
+    psi0 = np.zeros((mesh.nx + 2, mesh.ny + 2, xi.size, p + 1, p + 1))
+    psi0[1:-1, 1:-1] = ic[1:-1, 1:-1, None] / (4.0 * math.pi)
     nxs = f"{mesh.nx:07d}"
     extras = {
+        "psi0": psi0,
         f"sigma_t_dg0_nx{nxs}.cs2": st, f"sigma_s_dg0_nx{nxs}.cs2": ss,
         f"initial_dg{p}_nx{nxs}.sd2": ic, f"source_dg{p}_nx{nxs}.sd2": q0,
         "ordinate_info": (otype, order, reduce), "mesh": mesh,

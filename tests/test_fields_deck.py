@@ -203,7 +203,7 @@ def test_python_deck_init_matches_cpp_tool(tmp_path):
     P2, ex2 = deck.init(deck.read_deck(str(d2)))
     v, _ = fields.read_density(str(tmp_path / "source_dg2_nx0000010.sd2"))
     np.testing.assert_allclose(v, ex2["source_dg2_nx0000010.sd2"], rtol=0, atol=1e-15)
-    raw = deck.project(deck.Lattice.source, ex2["mesh"], 2, apply_limiter=False)
+    raw = deck.project(deck.Lattice().source, ex2["mesh"], 2, apply_limiter=False)
     assert np.any(np.abs(raw - v) > 1e-3), "no cell was limited"
     np.testing.assert_allclose(raw[..., 0, 0], v[..., 0, 0], atol=1e-15)  # limiting keeps cell means
     h = tmp_path / "h.deck"
@@ -218,6 +218,44 @@ def test_python_deck_init_matches_cpp_tool(tmp_path):
     af, _, _ = fields.read_ordinate_flux(str(tmp_path / "source_af_dg1_nx0000013.af2"))
     np.testing.assert_array_equal(af[0, :, :, 0, 0], 1 / (4 * math.pi))
     np.testing.assert_allclose(af, Ph.source, rtol=0, atol=1e-16)
+
+
+def test_bump_and_sphere_decks_match_cpp_tool_and_closed_forms(tmp_path):
+    """Bump: unit mass (the projected initial condition integrates to 1) in a uniform medium.
+    Sphere: inner / intermediate / outer cross sections and the smoothed unit source.  Both projected
+    identically by the numpy path and tsw_init."""
+    bump = ("ax -1\nbx 1\nay -1\nby 1\nnx 40\nny 40\ndg_degree 2\npiecewise_cross true\nproblem bump\n"
+            "moll_radius 0\nsigma_val 0.7\nbump_radius 0.6\nang_order 2\nordinate_type spherical-triangle\n")
+    sphere = ("ax -2\nbx 2\nay -2\nby 2\nnx 32\nny 32\ndg_degree 1\npiecewise_cross true\n"
+              "problem inhomogeneous-sphere\nmoll_radius 0\nsphere_inner_radius 0.5\nsphere_outer_radius 1.5\n"
+              "sphere_inner_trans_radius 0.2\nsphere_outer_trans_radius 0.3\nsphere_inner_sigma_val 5\n"
+              "sphere_interm_sigma_val 1\nsphere_outer_sigma_val 0.1\nang_order 2\n")
+    for name, text in (("bump", bump), ("sphere", sphere)):
+        d = tmp_path / f"{name}.deck"
+        d.write_text(text)
+        r = subprocess.run([_tool("tsw_init"), str(d), str(tmp_path)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        P, ex = deck.init(deck.read_deck(str(d)))
+        m = ex["mesh"]
+        for key in ex:
+            if key.endswith(".cs2") or key.endswith(".sd2"):
+                v, _ = fields.read_density(str(tmp_path / key))
+                np.testing.assert_allclose(v, ex[key], rtol=1e-13, atol=1e-15, err_msg=f"{name} {key}")
+        if name == "bump":
+            ic = ex["initial_dg2_nx0000040.sd2"]
+            assert fields.l1_norm(ic, m) == pytest.approx(1.0, abs=1e-9)
+            assert ic[..., 0, 0].min() >= 0.0  # non-negative cell means
+            np.testing.assert_allclose(P.sigma_t[1:-1, 1:-1], 0.7) and np.testing.assert_allclose(P.sigma_s, P.sigma_t)
+            assert np.all(ex["source_dg2_nx0000040.sd2"] == 0.0)
+        else:
+            st = P.sigma_t
+            assert st[16, 16] == pytest.approx(5.0) and st[1, 1] == pytest.approx(0.1)  # centre and corner
+            # a cell in the intermediate shell, r about 1: x, y = 1.0625 .. 1.1875 -> r = 0.75..1.68; use (8,16)
+            c = -2 + (8 - 0.5) * 4 / 32, -2 + (16 - 0.5) * 4 / 32
+            assert 0.8 < math.hypot(*c) < 1.2 and st[8, 16] == pytest.approx(1.0)
+            np.testing.assert_array_equal(P.sigma_s, P.sigma_t)
+            q = ex["source_dg1_nx0000032.sd2"]
+            assert q[16, 16, 0, 0] == pytest.approx(1.0) and q[2, 2, 0, 0] == 0.0
 
 
 def test_deck_reader_rules(tmp_path):
