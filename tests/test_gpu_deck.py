@@ -126,11 +126,13 @@ def test_tsw_solve_idc_bump_deck_matches_oracle(tmp_path):
     e = rel_err(got[1:-1, 1:-1], psi[1:-1, 1:-1])
     print("bump IDC rel err vs oracle", e, flush=True)
     assert e <= 1e-9
-    # particle balance: a pure scatterer with the bump far from the vacuum boundary keeps its mass
-    # (cell means times cell area) through the implicit steps, to the solver tolerance
+    # particle balance: in a pure scatterer the mass (cell means times cell area) changes only by the
+    # outflow through the vacuum boundary, which sits 0.5 beyond the bump; the implicit steps' exponential
+    # tails leak about 2e-7 of it here (the oracle's own figure), so a lost or doubled scattering term
+    # shows far above the 1e-5 allowed
     phi, m = fields.read_density(str(tmp_path / "bump.sd2"))
     ic = ex["initial_dg1_nx0000016.sd2"]
     mass0 = float(ic[1:-1, 1:-1, 0, 0].sum()) * m.dx * m.dy
     mass = float(phi[1:-1, 1:-1, 0, 0].sum()) * m.dx * m.dy
     print("bump mass", mass0, mass, flush=True)
-    assert mass0 == pytest.approx(1.0, abs=1e-6) and mass == pytest.approx(mass0, abs=1e-9)
+    assert mass0 == pytest.approx(1.0, abs=1e-6) and mass == pytest.approx(mass0, abs=1e-5)
