@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--idc-steps", type=int, default=10, help="IDC steps timed by --workload idc (100 reach t=1)")
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
+    ap.add_argument("--lane-split", type=int, default=0,
+                    help="1: lane-split sweep kernel; 0: the autotuner chooses (with automatic bands and prefetch)")
     ap.add_argument("--batch-direct", type=int, default=0,
                     help="batch groups: -1 storer wave, 0 or 1 direct stores where the tables allow")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
@@ -358,6 +360,7 @@ def main():
     ctx.set_pattern(args.pattern)
     ctx.set_tuning("band_rows", args.band_rows)
     ctx.set_tuning("prefetch", args.prefetch)
+    ctx.set_tuning("lane_split", args.lane_split)
 
     def solve(precision, phi):
         work.copy_(d_src)
@@ -408,9 +411,12 @@ def main():
     t_sw32 = contract_sweeps(tsw.FP32)
     tuned32 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
     store32 = 2 if ctx.get_tuning("one_wave") else 0  # kernel template argument: storer wave or one wave
+    split32 = ctx.get_tuning("lane_split")  # 1: the lane-split kernel
     t_sw64 = contract_sweeps(tsw.FP64)
     tuned64 = (ctx.get_tuning("band_rows"), ctx.get_tuning("prefetch"))
-    print(f"bench: sweep tuning (band_rows, prefetch): fp32 {tuned32} fp64 {tuned64}", file=sys.stderr, flush=True)
+    split64 = ctx.get_tuning("lane_split")
+    print(f"bench: sweep tuning (band_rows, prefetch): fp32 {tuned32} fp64 {tuned64}; lane split fp32 {split32} "
+          f"fp64 {split64}", file=sys.stderr, flush=True)
     print(f"bench: contract sweep fp32 {t_sw32:.3f} ms fp64 {t_sw64:.3f} ms", file=sys.stderr, flush=True)
     if args.copy_probe:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
@@ -445,7 +451,10 @@ def main():
     achieved = bytes32 / (t_sw32 * 1e-3) / 1e9
     achieved64 = bytes64 / (t_sw64 * 1e-3) / 1e9
 
-    traffic32, traffic32_as = traffic_from_profiles("k_diag_sweep<float, 3, true, 1>")
+    family32 = "k_diag_sweep_split<float, 3, 1>" if split32 else "k_diag_sweep<float, 3, true, 1>"
+    kernel32 = (f"k_diag_sweep_split<float, 3, 1, {tuned32[1]}>" if split32
+                else f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, {store32}>")
+    traffic32, traffic32_as = traffic_from_profiles(family32)
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
         "value": value,
@@ -463,7 +472,8 @@ def main():
                                "with DSA to abs residual 1e-8; mixed = FP32 sweeps in the Arnoldi steps once the FP64 residual "
                                "estimate is <= 1e-3, FP64 sweeps otherwise, FP64 Krylov vectors and DSA",
                    "sweep_pattern": args.pattern, "band_rows": args.band_rows, "prefetch": args.prefetch,
-                   "tuned_band_rows_prefetch": {"fp32": tuned32, "fp64": tuned64}, "dsa": bool(args.dsa),
+                   "tuned_band_rows_prefetch": {"fp32": tuned32, "fp64": tuned64},
+                   "lane_split": {"fp32": split32, "fp64": split64}, "dsa": bool(args.dsa),
                    "dsa_cycles": args.dsa_cycles,
                    "unknowns_per_sweep": U},
         "fp64": {"value": value64, "ms_per_step": w64 / args.steps * 1e3, "sweeps_per_solve": n64 / args.steps,
@@ -477,8 +487,7 @@ def main():
         "mixed_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, {store32}> (FP32 in-place "
-                               f"operator-contract sweep, band_rows {tuned32[0]})",
+                     "kernel": f"{kernel32} (FP32 in-place operator-contract sweep, band_rows {tuned32[0]})",
                      "algorithmic_bytes_per_launch": bytes32,
                      "traffic": traffic32,
                      "traffic_measured_as": traffic32_as},

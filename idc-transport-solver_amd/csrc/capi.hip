@@ -100,10 +100,11 @@ struct tsw_ctx {
     // (band rows, prefetch) pairs and keeps the fastest; tuned_br/tuned_pf hold the choice in force
     // (INT_MIN: none), last_br/last_pf what the last sweep used.
     bool autotune = true;
-    std::map<std::array<int, 7>, std::array<int, 3>> tuned;  // (band rows, prefetch, one wave)
-    int tuned_br = INT_MIN, tuned_pf = INT_MIN, tuned_w1 = INT_MIN;
-    int last_br = 0, last_pf = 0, last_w1 = 0;
+    std::map<std::array<int, 7>, std::array<int, 4>> tuned;  // (band rows, prefetch, one wave, lane split)
+    int tuned_br = INT_MIN, tuned_pf = INT_MIN, tuned_w1 = INT_MIN, tuned_sl = INT_MIN;
+    int last_br = 0, last_pf = 0, last_w1 = 0, last_sl = 0;
     int one_wave_opt = 0;  // 1: one wave per band of 64 rows, lane-to-lane traces (tuning key one_wave)
+    int split_opt = 0;     // 1: 2 or 4 lanes per row on the material path (tuning key lane_split)
     // batch launch groups: -1 storer wave unless a table only fits without it, else (0, 1) direct stores
     // whenever every member has element-inverse tables (tuning key batch_direct)
     int batch_direct_opt = 0;
@@ -860,6 +861,7 @@ const T *diag_inverses(tsw_ctx *c, double dt) {
 // one wavefront of rows per band when the mesh has more rows than that, so the ordinates' sweeps
 // spread over several times more compute units than there are ordinates.
 bool one_wave_for(const tsw_ctx *c) { return (c->tuned_w1 != INT_MIN ? c->tuned_w1 : c->one_wave_opt) == 1; }
+bool lane_split_for(const tsw_ctx *c) { return (c->tuned_sl != INT_MIN ? c->tuned_sl : c->split_opt) == 1; }
 
 int band_rows_for(const tsw_ctx *c) {
     const int ny = c->g.ny;
@@ -927,15 +929,20 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
     configure_bands<T>(c, sp);
     sp.prefetch = c->tuned_pf != INT_MIN ? c->tuned_pf : c->prefetch_opt;
     sp.wave1 = one_wave_for(c) && (sp.nbands > 1 ? sp.band_rows : sp.ny) <= 64 ? 1 : 0;
-    c->last_w1 = sp.wave1;
     c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
     c->last_pf = sp.prefetch > 0 ? sp.prefetch : (sizeof(T) == 8 ? 2 : 3);
     sp.ainv = diag_inverses<T>(c, dt);
     sp.direct = 0;
+    sp.split = 0;
     if (sp.ainv) {
         sp.mat = d.mat.as<int32_t>();
         sp.M = c->mat.M;
-        if (!sp.wave1 && diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // without the store staging buffer (one band)
+        if (lane_split_for(c)) {  // kept where the lane-split kernel is built and its tables fit
+            sp.split = 1;
+            sp.wave1 = 0;
+            if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) sp.split = 0;
+        }
+        if (!sp.split && !sp.wave1 && diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // without the store staging buffer (one band)
             sp.direct = 1;
             if (diag_sweep_lds_bytes<T>(sp, g.p) == 0) {  // table too large: eliminate per cell
                 sp.direct = 0;
@@ -944,6 +951,8 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
         }
     }
     if (!sp.ainv) sp.sig_t = (const T *)(f64 ? d.sigt64.ptr : d.sigt32.ptr);
+    c->last_w1 = sp.wave1;
+    c->last_sl = sp.split;
     sp.psi = psi_out;
     return sp;
 }
@@ -954,8 +963,9 @@ DiagSweepParams<T> make_diag_params(tsw_ctx *c, double dt, const T *qsrc, const 
 // the same result, so the choice only affects speed.
 template <typename T>
 void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflow) {
-    c->tuned_br = c->tuned_pf = c->tuned_w1 = INT_MIN;
-    if (!c->autotune || c->band_rows_opt != 0 || c->prefetch_opt != 0 || c->one_wave_opt != 0) return;
+    c->tuned_br = c->tuned_pf = c->tuned_w1 = c->tuned_sl = INT_MIN;
+    if (!c->autotune || c->band_rows_opt != 0 || c->prefetch_opt != 0 || c->one_wave_opt != 0 || c->split_opt != 0)
+        return;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     TSW_HIP(hipStreamIsCapturing(c->stream, &cap));
     const Geom &g = c->g;
@@ -964,39 +974,51 @@ void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool in
     auto it = c->tuned.find(key);
     if (it == c->tuned.end()) {
         if (cap != hipStreamCaptureStatusNone) return;  // no timing inside a capture: defaults
-        std::vector<int> brs{-1};
-        for (int br = 64; br <= 256 && br < g.ny; br += 64) brs.push_back(br);
+        // (band rows, one wave, lane split): one-lane kernels over one band and bands of 64..256 rows,
+        // the one-wave kernel over 64-row bands, the lane-split kernel over one band (meshes of at most
+        // 128 rows) and bands of 32..128 rows
+        std::vector<std::array<int, 3>> shapes{{-1, 0, 0}};
+        for (int br = 64; br <= 256 && br < g.ny; br += 64) shapes.push_back({br, 0, 0});
+        if (g.ny > 64) shapes.push_back({64, 1, 0});
+        else shapes.push_back({-1, 1, 0});
+        if (c->mat.valid && g.p >= 1) {
+            if (g.ny <= 128) shapes.push_back({-1, 0, 1});
+            for (int br : {32, 64, 96, 128})
+                if (br < g.ny) shapes.push_back({br, 0, 1});
+        }
         const int pfs[] = {2, 3, 4, 6};
         T *out = dbuf<T>(c->tune_out, (size_t)g.nq * g.N() * g.nx * g.ny);
         hipEvent_t e0, e1;
         TSW_HIP(hipEventCreate(&e0));
         TSW_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        std::array<int, 3> choice{0, 0, 0};
+        std::array<int, 4> choice{0, 0, 0, 0};
         try {
-            for (int br : brs)
-                for (int pf : pfs)
-                    for (int w1 = 0; w1 <= ((br == 64 || (br == -1 && g.ny <= 64)) ? 1 : 0); ++w1) {
-                        c->tuned_br = br;
-                        c->tuned_pf = pf;
-                        c->tuned_w1 = w1;
-                        const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, out);
-                        launch_diag_sweep<T>(sp, g.p, c->stream);
-                        TSW_HIP(hipEventRecord(e0, c->stream));
-                        launch_diag_sweep<T>(sp, g.p, c->stream);
-                        TSW_HIP(hipEventRecord(e1, c->stream));
-                        TSW_HIP(hipEventSynchronize(e1));
-                        float ms = 0.f;
-                        TSW_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        if (ms < best) {
-                            best = ms;
-                            choice = {br, pf, w1};
-                        }
+            for (const auto &sh : shapes)
+                for (int pf : pfs) {
+                    const int br = sh[0], w1 = sh[1], sl = sh[2];
+                    c->tuned_br = br;
+                    c->tuned_pf = pf;
+                    c->tuned_w1 = w1;
+                    c->tuned_sl = sl;
+                    const DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, out);
+                    if (sp.split != sl) continue;  // not built for this problem: the one-lane shapes cover it
+                    launch_diag_sweep<T>(sp, g.p, c->stream);
+                    TSW_HIP(hipEventRecord(e0, c->stream));
+                    launch_diag_sweep<T>(sp, g.p, c->stream);
+                    TSW_HIP(hipEventRecord(e1, c->stream));
+                    TSW_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    TSW_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        choice = {br, pf, w1, sl};
                     }
+                }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            c->tuned_br = c->tuned_pf = c->tuned_w1 = INT_MIN;
+            c->tuned_br = c->tuned_pf = c->tuned_w1 = c->tuned_sl = INT_MIN;
             throw;
         }
         (void)hipEventDestroy(e0);
@@ -1006,6 +1028,7 @@ void select_tuning(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool in
     c->tuned_br = it->second[0];
     c->tuned_pf = it->second[1];
     c->tuned_w1 = it->second[2];
+    c->tuned_sl = it->second[3];
 }
 
 // One staggered sweep on the context's stream, timed.
@@ -1784,6 +1807,7 @@ tsw_ctx *adjoint_ctx(tsw_ctx *ctx) {
     ctx->adj->dsa_cycles = ctx->dsa_cycles;
     ctx->adj->gmres_refine = ctx->gmres_refine;
     ctx->adj->one_wave_opt = ctx->one_wave_opt;
+    ctx->adj->split_opt = ctx->split_opt;
     ctx->adj->refine_steps = ctx->refine_steps;
     return ctx->adj.get();
 }
@@ -2059,6 +2083,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "one_wave") {
             require(value == 0 || value == 1, TSW_E_ARG, "one_wave must be 0 or 1");
             ctx->one_wave_opt = value;
+        } else if (k == "lane_split") {
+            require(value == 0 || value == 1, TSW_E_ARG, "lane_split must be 0 or 1");
+            ctx->split_opt = value;
         } else if (k == "batch_direct") {
             require(value >= -1 && value <= 1, TSW_E_ARG, "batch_direct must be -1, 0 or 1");
             ctx->batch_direct_opt = value;
@@ -2095,6 +2122,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "refine_steps") *value = ctx->refine_steps;
         else if (k == "gmres_refine") *value = ctx->gmres_refine ? 1 : 0;
         else if (k == "one_wave") *value = ctx->last_w1;
+        else if (k == "lane_split") *value = ctx->last_sl;
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });

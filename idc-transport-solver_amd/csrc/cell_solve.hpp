@@ -18,6 +18,14 @@ namespace tsw {
 
 __host__ __device__ constexpr int ipow_m1(int k) { return (k & 1) ? -1 : 1; }
 
+// c + a * b with one rounding.  Sums of two products (a0 b0 + a1 b1) may be contracted with either product
+// rounded first, so the element matvecs accumulate through this to give every kernel variant the same bits.
+template <typename T>
+__device__ __forceinline__ T fmadd(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
 // Builds B (right-hand side) for one lane from the source and the two upwind neighbours.
 template <typename T, int P>
 __device__ __forceinline__ void assemble_rhs(T (&B)[P * P], const T *__restrict__ src, const T *__restrict__ upx,
@@ -234,9 +242,9 @@ template <typename T, int N>
 __device__ __forceinline__ void matvec_lds(const T *__restrict__ M, const T (&b)[N], T (&psi)[N]) {
 #pragma unroll
     for (int r = 0; r < N; ++r) {
-        T acc = T(0);
+        T acc = M[r * N] * b[0];
 #pragma unroll
-        for (int c = 0; c < N; ++c) acc += M[r * N + c] * b[c];
+        for (int c = 1; c < N; ++c) acc = fmadd(M[r * N + c], b[c], acc);
         psi[r] = acc;
     }
 }

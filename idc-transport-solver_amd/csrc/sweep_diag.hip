@@ -16,6 +16,19 @@ TSW_SWEEP_EXTERN(float, 2)
 TSW_SWEEP_EXTERN(float, 3)
 TSW_SWEEP_EXTERN(float, 4)
 
+// LDS of the lane-split sweep for bands of BR rows (0: not built for (T, P) or too many rows)
+template <typename T, int P>
+size_t split_lds_total(int BR, int nx, int M) {
+    if constexpr (!split_supported<T, P>()) {
+        return 0;
+    } else {
+        constexpr int LG = split_lanes<P>(), rpw = 64 / LG;
+        const int slots = (BR + rpw - 1) / rpw * rpw;
+        if (slots * LG > 512) return 0;
+        return SplitLds<T, P>(slots, nx, M).total;
+    }
+}
+
 template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     const int BR = band_rows_of(sp);
@@ -23,6 +36,18 @@ size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     // the kernel addresses an ordinate's coefficient planes with 32-bit byte offsets
     if ((long long)sp.nx * sp.ny * (p + 1) * (p + 1) * (long long)sizeof(T) >= (1ll << 32)) return 0;
     if (sp.wave1 && BR > 64) return 0;
+    if (sp.split) {
+        if (!sp.ainv || sp.wave1) return 0;
+        size_t total = 0;
+        switch (p) {
+            case 1: total = split_lds_total<T, 2>(BR, sp.nx, sp.M); break;
+            case 2: total = split_lds_total<T, 3>(BR, sp.nx, sp.M); break;
+            case 3: total = split_lds_total<T, 4>(BR, sp.nx, sp.M); break;
+            default: break;
+        }
+        const size_t bytes = total * sizeof(T);
+        return bytes <= max_lds_per_block() ? bytes : 0;
+    }
     const int cthreads = ((BR + 63) / 64) * 64;
     const int M = sp.ainv ? sp.M : 0;
     const bool direct = (sp.ainv && sp.direct) || sp.wave1;

@@ -440,7 +440,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                     const T wsc = bsc * ps[u] * T(kInv4Pi);
 #pragma unroll
                     for (int k = 0; k < N; ++k)
-                        a[k] = HASQ ? (MAT ? pq[u][k] : bsc * pq[u][k]) + wsc * pf[u][k] : wsc * pf[u][k];
+                        a[k] = HASQ ? fmadd(wsc, pf[u][k], MAT ? pq[u][k] : bsc * pq[u][k]) : wsc * pf[u][k];
                 } else {
 #pragma unroll
                     for (int k = 0; k < N; ++k) a[k] = MAT ? pq[u][k] : bsc * pq[u][k];
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
 #pragma unroll
                 for (int d = 0; d < P; ++d)
 #pragma unroll
-                    for (int e = 0; e < P; ++e) B[e + P * d] = a[e + P * d] + kxd[d] * tx[e] + kye[e] * ty[d];
+                    for (int e = 0; e < P; ++e) B[e + P * d] = fmadd(kye[e], ty[d], fmadd(kxd[d], tx[e], a[e + P * d]));
             }
             if (live) {
             const int mat = pm[u];
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                 for (int row = 0; row < N; ++row) {
                     T acc = Ainv[row * N] * B[0];
 #pragma unroll
-                    for (int col = 1; col < N; ++col) acc += Ainv[row * N + col] * B[col];
+                    for (int col = 1; col < N; ++col) acc = fmadd(Ainv[row * N + col], B[col], acc);
                     x[row] = acc;
                 }
             } else if constexpr (MAT) {
@@ -524,6 +524,12 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     }
 }
 
+}  // namespace tsw
+
+#include "sweep_split.hpp"
+
+namespace tsw {
+
 // rows handled by one workgroup
 template <typename T>
 inline int band_rows_of(const DiagSweepParams<T> &sp) {
@@ -534,6 +540,21 @@ template <typename T, int P, bool MAT, int MODE, int D>
 void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch,
                         const int2 *items, int nitems, int max_ny) {
     const int rows = batch ? max_ny : band_rows_of(sp);
+    if (sp.split) {
+        if constexpr (MAT && split_supported<T, P>()) {
+            constexpr int LG = split_lanes<P>(), RPW = 64 / LG;  // lanes per row, rows per wavefront
+            const int threads = LG * ((rows + RPW - 1) / RPW * RPW);  // LG lanes for each row slot
+            if (batch || threads > 512) throw Error(TSW_E_ARG, "lane-split sweep needs one problem and bands of at most 512 / lanes rows");
+            auto kern = k_diag_sweep_split<T, P, MODE, D>;
+            if (lds > 64 * 1024)
+                TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kern, dim3(sp.nq * (sp.nbands > 1 ? sp.nbands : 1)), dim3(threads), lds, s, sp);
+            TSW_HIP(hipGetLastError());
+            return;
+        } else {
+            throw Error(TSW_E_ARG, "lane-split sweep not built for this precision, degree or solve path");
+        }
+    }
     const bool direct = MAT && sp.direct, wave1 = sp.wave1 != 0;
     if (direct && !batch && sp.nbands > 1) throw Error(TSW_E_ARG, "direct-store sweep with row bands");
     if (wave1 && (batch || rows > 64)) throw Error(TSW_E_ARG, "one-wave sweep needs bands of at most 64 rows");

@@ -118,6 +118,7 @@ template <typename T> struct DiagSweepParams {
     int direct;            // 1: no storer wave, the compute waves store psi (one band only; used when
                            // only that LDS layout, without the store staging buffer, fits)
     int wave1;             // 1: one wave per band of at most 64 rows, traces passed between lanes
+    int split;             // 1: lane-split kernel (2 or 4 lanes per row, material path, direct stores)
     const int *skip;       // device flag: the launch does nothing while *skip != 0 (null: always runs)
 };
 

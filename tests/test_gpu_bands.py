@@ -131,8 +131,9 @@ def test_prefetch_depths_match_oracle(prefetch, band_rows):
 
 
 def test_autotuner_choice_is_valid_and_result_identical():
-    """With band_rows and prefetch automatic the first sweep times the candidates; the sweep result is
-    bitwise the same as with fixed settings, and the reported choice is one of the candidates."""
+    """With band_rows and prefetch automatic the first sweep times the candidates (one-lane and lane-split
+    kernels); the sweep result is bitwise the same as with fixed settings, and the reported choice is one of
+    the candidates."""
     o = ls_ordinates(6)
     g, st, ss, src = random_problem(60, 200, 2, o, seed=4242, materials=4)
     outs = []
@@ -145,9 +146,12 @@ def test_autotuner_choice_is_valid_and_result_identical():
                     c.sweep(out, st, precision=prec)
                     outs.append(out)
                 br, pf = c.get_tuning("band_rows"), c.get_tuning("prefetch")
-                assert br in (0, 64, 128, 192) and pf in (2, 3, 4, 6), (br, pf)
+                split = c.get_tuning("lane_split")
+                # one-lane shapes: one band or 64..192-row bands; lane-split shapes: 32..128-row bands
+                shapes = (32, 64, 96, 128) if split else (0, 64, 128, 192)
+                assert br in shapes and pf in (2, 3, 4, 6), (br, pf, split)
                 if not autotune:
-                    assert br == 64 and pf == (2 if prec == tsw.FP64 else 3)
+                    assert br == 64 and pf == (2 if prec == tsw.FP64 else 3) and split == 0
             psi = src.copy()
             c.gmres_solve(psi, st, ss, abs_tol=1e-12)
             outs.append(psi)
