@@ -80,13 +80,15 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * automatically, -1 keeps one workgroup per ordinate (bands only when ny > 448).  Key "prefetch": steps
  * of cell inputs each row keeps in flight (2, 3, 4 or 6; 0 = 2 in FP64, 3 in FP32).  Key "one_wave"
  * (0 default, 1): bands of at most 64 rows, each swept by a single wavefront that passes the y-traces
- * lane to lane instead of through LDS.  Key "batch_direct" (0 default or 1, -1): launch groups of
+ * lane to lane instead of through LDS.  Key "lane_split" (0 default, 1): on the material-table path,
+ * 2 (DG degree 1) or 4 (degrees 2, 3) lanes per row, each solving its share of the cell's coefficients
+ * (FP32 degrees 1-3, FP64 degrees 1-2; elsewhere the one-lane kernel runs).  Key "batch_direct" (0 default or 1, -1): launch groups of
  * tsw_batch_sweep store from the compute lanes instead of a storer wave (smaller workgroups, more of
  * them per CU), or keep the storer wave where the tables fit with it.  Key "autotune" (1 default, 0
- * off): while band_rows, prefetch and one_wave are all 0, the first sweep of each kind (mesh, degree,
- * ordinates, precision, source terms, material path) times every candidate (band rows, prefetch, one
- * wave) into scratch memory and keeps the fastest for later sweeps; inside a HIP graph capture the
- * defaults apply.  Results do not depend on any of these settings.
+ * off): while band_rows, prefetch, one_wave and lane_split are all 0, the first sweep of each kind
+ * (mesh, degree, ordinates, precision, source terms, material path) times every candidate (kernel, band
+ * rows, prefetch) into scratch memory and keeps the fastest for later sweeps; inside a HIP graph capture
+ * the defaults apply.  Results do not depend on any of these settings (bit for bit).
  * Solver keys: "use_dsa" (0 default, 1): diffusion synthetic acceleration of tsw_gmres_solve and
  * tsw_solve_vjp, the reference's `use_dsa` parameter (PETScPeierlsSolver.cpp:151, shell preconditioner
  * :180-182, apply :588-640), applied on the left as PETSc does, so the residual test is on
@@ -100,7 +102,8 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* Reads back "band_rows" (rows per band of the last staggered sweep, 0 = one band per ordinate),
- * "prefetch" (its prefetch depth), "one_wave" (1 if it ran the one-wavefront kernel), or the setting of
+ * "prefetch" (its prefetch depth), "one_wave" (1 if it ran the one-wavefront kernel), "lane_split" (1 if
+ * it ran the lane-split kernel), or the setting of
  * "batch_direct", "autotune", "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
