@@ -289,10 +289,11 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
-def traffic_from_profiles(family):
+def traffic_from_profiles(family, band_rows=None):
     """HBM bytes per launch of the kernel family `family` (the sweep template without its prefetch depth
     and store variant, which move the same data) from the committed rocprofv3 --pmc summary, with the
-    configuration it was measured in; None when no summary exists."""
+    configuration it was measured in; the figure for `band_rows` where the summary has one per band
+    height; None when no summary exists."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -301,6 +302,7 @@ def traffic_from_profiles(family):
         return None, None
     if not entry:
         return None, None
+    entry = entry.get("by_band_rows", {}).get(str(band_rows), entry)
     return entry.get("hbm_bytes_per_launch"), entry.get("measured_as")
 
 
@@ -454,7 +456,7 @@ def main():
     family32 = "k_diag_sweep_split<float, 3, 1>" if split32 else "k_diag_sweep<float, 3, true, 1>"
     kernel32 = (f"k_diag_sweep_split<float, 3, 1, {tuned32[1]}>" if split32
                 else f"k_diag_sweep<float, 3, true, 1, {tuned32[1]}, {store32}>")
-    traffic32, traffic32_as = traffic_from_profiles(family32)
+    traffic32, traffic32_as = traffic_from_profiles(family32, tuned32[0])
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
         "value": value,
