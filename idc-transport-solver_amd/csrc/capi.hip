@@ -1684,11 +1684,13 @@ bool idc_step_device(tsw_ctx *c, double *psi, const double *src, const double *s
     TSW_HIP(hipMemcpyAsync(stage(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     launch_fill_region<double>(g, stage(0), K, 1, 0.0, s);
     // prediction: implicit Euler substeps (:863-876)
+    // (the sources below are each one pass over the arrays they combine, launch_lincomb)
     for (int k = 1; k < S; ++k) {
         const double h = I.nodes[k] - I.nodes[k - 1];
-        launch_axpby(n, 1.0 / (h * dt), stage(k - 1), 0.0, stage(k), s);
-        launch_fill_region<double>(g, stage(k), K, 1, 0.0, s);
-        launch_axpby(n, 1.0, src, 1.0, stage(k), s);
+        LinComb L;  // stage k = stage(k-1) / (h dt) on the interior + Q (with its inflow halos)
+        L.add(1.0 / (h * dt), stage(k - 1));
+        L.add(1.0, src, true);
+        launch_lincomb(g, K, L, stage(k), s);
         ok = idc_solve(c, stage(k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
     }
     // corrections in error form (:882-930, ComputeSource :1686-1740)
@@ -1696,25 +1698,25 @@ bool idc_step_device(tsw_ctx *c, double *psi, const double *src, const double *s
         for (int l = l0; l < S; ++l) idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
         for (int k = 1; k < S; ++k) {
             const double h = I.nodes[k] - I.nodes[k - 1];
-            launch_axpby(n, -1.0 / (h * dt), stage(k), 0.0, e, s);
-            for (int l = l0; l < S; ++l) launch_axpby(n, -I.W[(size_t)k * S + l] / h, R + (size_t)l * n, 1.0, e, s);
-            launch_axpby(n, 1.0 / (h * dt), stage(k - 1), 1.0, e, s);
-            launch_axpby(n, 1.0, src, 1.0, e, s);
-            launch_fill_region<double>(g, e, K, 1, 0.0, s);
+            LinComb L;  // e = -stage(k)/(h dt) - sum_l W(k,l) R_l / h + stage(k-1)/(h dt) + Q, zero halos
+            L.add(-1.0 / (h * dt), stage(k));
+            for (int l = l0; l < S; ++l) L.add(-I.W[(size_t)k * S + l] / h, R + (size_t)l * n);
+            L.add(1.0 / (h * dt), stage(k - 1));
+            L.add(1.0, src);
+            launch_lincomb(g, K, L, e, s);
             ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
             launch_axpby(n, 1.0, e, 1.0, stage(k), s);
         }
     }
     // result: the last node, or the collocation formula (:933-939, ComputeCollocation_Nonhybrid :1541-1566)
     if (!I.left || I.collocation) {
-        // e = psi0 + dt Q + dt sum_l W(0,l) (S P - L) psi_l
-        TSW_HIP(hipMemcpyAsync(e, stage(0), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        launch_axpby(n, dt, src, 1.0, e, s);
-        for (int l = l0; l < S; ++l) {
-            idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
-            launch_axpby(n, -dt * I.W[l], R + (size_t)l * n, 1.0, e, s);
-        }
-        TSW_HIP(hipMemcpyAsync(psi, e, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        // psi = psi0 + dt Q + dt sum_l W(0,l) (S P - L) psi_l (its halos are zeroed below)
+        for (int l = l0; l < S; ++l) idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
+        LinComb L;
+        L.add(1.0, stage(0));
+        L.add(dt, src);
+        for (int l = l0; l < S; ++l) L.add(-dt * I.W[l], R + (size_t)l * n);
+        launch_lincomb(g, K, L, psi, s);
     } else {
         TSW_HIP(hipMemcpyAsync(psi, stage(S - 1), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     }
@@ -1947,9 +1949,10 @@ bool idc_step_vjp_device(tsw_ctx *c, const double *psi_n, const double *src, con
         launch_fill_region<double>(g, lev(0, 0), K, 1, 0.0, s);
         for (int k = 1; k < S; ++k) {
             const double h = I.nodes[k] - I.nodes[k - 1];
-            launch_axpby(n, 1.0 / (h * dt), lev(0, k - 1), 0.0, lev(0, k), s);
-            launch_fill_region<double>(g, lev(0, k), K, 1, 0.0, s);
-            launch_axpby(n, 1.0, src, 1.0, lev(0, k), s);
+            LinComb L;  // as idc_step_device's prediction
+            L.add(1.0 / (h * dt), lev(0, k - 1));
+            L.add(1.0, src, true);
+            launch_lincomb(g, K, L, lev(0, k), s);
             ok = idc_solve(c, lev(0, k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
         }
         for (int p = 1; p <= P; ++p) {
@@ -1958,14 +1961,17 @@ bool idc_step_vjp_device(tsw_ctx *c, const double *psi_n, const double *src, con
             for (int k = 1; k < S; ++k) {
                 const double h = I.nodes[k] - I.nodes[k - 1];
                 double *e = corr(p, k);
-                launch_axpby(n, -1.0 / (h * dt), lev(p - 1, k), 0.0, e, s);
-                for (int l = l0; l < S; ++l) launch_axpby(n, -I.W[(size_t)k * S + l] / h, R + (size_t)l * n, 1.0, e, s);
-                launch_axpby(n, 1.0 / (h * dt), lev(p, k - 1), 1.0, e, s);
-                launch_axpby(n, 1.0, src, 1.0, e, s);
-                launch_fill_region<double>(g, e, K, 1, 0.0, s);
+                LinComb L;  // as idc_step_device's correction source
+                L.add(-1.0 / (h * dt), lev(p - 1, k));
+                for (int l = l0; l < S; ++l) L.add(-I.W[(size_t)k * S + l] / h, R + (size_t)l * n);
+                L.add(1.0 / (h * dt), lev(p, k - 1));
+                L.add(1.0, src);
+                launch_lincomb(g, K, L, e, s);
                 ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
-                TSW_HIP(hipMemcpyAsync(lev(p, k), lev(p - 1, k), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-                launch_axpby(n, 1.0, e, 1.0, lev(p, k), s);
+                LinComb U;  // lev(p, k) = lev(p-1, k) + e, halos included
+                U.add(1.0, lev(p - 1, k), true);
+                U.add(1.0, e, true);
+                launch_lincomb(g, K, U, lev(p, k), s);
             }
         }
     }

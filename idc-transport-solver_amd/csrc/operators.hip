@@ -265,6 +265,40 @@ void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipS
     TSW_HIP(hipGetLastError());
 }
 
+// y = sum_j a_j x_j over psi-layout arrays (per_cell values per cell of the (nx+2) x (ny+2) mesh) in one
+// pass, accumulated in term order with fused multiply-adds; on halo cells only the terms with bit j of
+// halo_terms set contribute (none: the halo is zero).  y may be one of the x_j.
+template <int M>
+__global__ void k_lincomb(size_t n, LinComb L, double *y, int nx, int ny, size_t per_cell) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t cell = t / per_cell;
+    const int i = (int)(cell / (size_t)(ny + 2)), j = (int)(cell % (size_t)(ny + 2));
+    const bool halo = i == 0 || j == 0 || i == nx + 1 || j == ny + 1;
+    const unsigned use = halo ? L.halo_terms : ~0u;
+    double v[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) v[k] = ((use >> k) & 1u) ? L.x[k][t] : 0.0;
+    double acc = L.a[0] * v[0];
+#pragma unroll
+    for (int k = 1; k < M; ++k) acc = __builtin_fma(L.a[k], v[k], acc);
+    y[t] = use ? acc : 0.0;
+}
+
+void launch_lincomb(const Geom &g, size_t per_cell, const LinComb &L, double *y, hipStream_t s) {
+    const size_t n = g.ncell_h() * per_cell;
+    switch (L.m) {
+#define TSW_LINCOMB_CASE(M) \
+    case M: hipLaunchKernelGGL(k_lincomb<M>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, L, y, g.nx, g.ny, per_cell); break;
+        TSW_LINCOMB_CASE(1) TSW_LINCOMB_CASE(2) TSW_LINCOMB_CASE(3) TSW_LINCOMB_CASE(4) TSW_LINCOMB_CASE(5)
+        TSW_LINCOMB_CASE(6) TSW_LINCOMB_CASE(7) TSW_LINCOMB_CASE(8) TSW_LINCOMB_CASE(9) TSW_LINCOMB_CASE(10)
+        TSW_LINCOMB_CASE(11) TSW_LINCOMB_CASE(12)
+#undef TSW_LINCOMB_CASE
+        default: throw Error(TSW_E_ARG, "linear combination of 1..12 arrays");
+    }
+    TSW_HIP(hipGetLastError());
+}
+
 __device__ __forceinline__ double block_sum(double v) {
     __shared__ double red[kBlock / 64];
 #pragma unroll

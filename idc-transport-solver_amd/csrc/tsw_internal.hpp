@@ -223,6 +223,21 @@ void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
 
 // y[k] = a*x[k] + b*y[k]
 void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipStream_t s);
+// y = sum_j a[j] x[j] (j < m <= 12) over psi-layout arrays in one pass; halo cells take only the terms
+// whose bit is set in halo_terms
+struct LinComb {
+    int m = 0;
+    unsigned halo_terms = 0;
+    double a[12] = {};
+    const double *x[12] = {};
+    void add(double coef, const double *arr, bool on_halo = false) {
+        if (m >= 12) throw Error(TSW_E_ARG, "linear combination of more than 12 arrays");
+        if (on_halo) halo_terms |= 1u << m;
+        a[m] = coef;
+        x[m++] = arr;
+    }
+};
+void launch_lincomb(const Geom &g, size_t per_cell, const LinComb &L, double *y, hipStream_t s);
 // flag[0] <- 1 if a and b differ bitwise anywhere in n doubles, else 0
 void launch_differs(size_t n, const double *a, const double *b, int *flag, hipStream_t s);
 // out[i] = sum_k X_i[k] * y[k], i < m, X_i = X + i*ld; uses `partials` scratch; result on device
