@@ -107,52 +107,6 @@ def bench_si(args):
                       "speedup_graph_vs_eager": we / wg}), flush=True)
 
 
-def _line_source(n=200, p=2, order=16, width=0.03):
-    """Line-source pulse: [-1.5,1.5]^2, n x n cells, Q_p, level-symmetric S_order, sigma_s = 1, sigma_a = 0,
-    vacuum; isotropic Gaussian initial condition of unit mass, exp(-r^2/(2 s^2)) / (2 pi s^2) / (4 pi) per
-    ordinate, projected exactly per cell (separable 1D Gauss-Legendre quadrature)."""
-    from numpy.polynomial import legendre as L
-    from tsw import level_symmetric
-    xi, eta, w, off = level_symmetric(order)
-    a, b = -1.5, 1.5
-    h = (b - a) / n
-    xg, wg = L.leggauss(12)
-    P = p + 1
-    leg = np.array([L.legval(xg, np.eye(P)[d]) for d in range(P)])  # [d][node]
-    centres = a + (np.arange(n) + 0.5) * h
-    X = centres[:, None] + 0.5 * h * xg[None, :]                      # [cell][node]
-    gx = np.exp(-X ** 2 / (2 * width ** 2)) / math.sqrt(2 * math.pi * width ** 2)
-    # 1D Legendre coefficients of gx on each cell: (2d+1)/2 int P_d g dxhat
-    cx = np.einsum("cn,dn,n->cd", gx, leg, wg) * (2 * np.arange(P) + 1)[None, :] / 2.0
-    coef = np.einsum("id,je->ijde", cx, cx) / (4 * math.pi)
-    nq = xi.size
-    psi0 = np.zeros((n + 2, n + 2, nq, P, P))
-    psi0[1:-1, 1:-1] = coef[:, :, None]
-    st = np.ones((n + 2, n + 2))
-    ss = np.ones((n + 2, n + 2))
-    return dict(n=n, p=p, a=a, b=b, h=h, xi=xi, eta=eta, w=w, off=off, psi0=psi0, st=st, ss=ss)
-
-
-def _ring_seed(L_, r0=0.8, r1=1.0, sub=8):
-    """Seed of J = integral of phi over the ring r0 <= r <= r1: w_q * int_{ring & cell} P_d P_e dx dy, by
-    sub x sub midpoint subcells per cell (the ring's cut cells are resolved to that quadrature)."""
-    from numpy.polynomial import legendre as Lg
-    n, p, a, h = L_["n"], L_["p"], L_["a"], L_["h"]
-    P = p + 1
-    t = (np.arange(sub) + 0.5) / sub * 2 - 1
-    leg = np.array([Lg.legval(t, np.eye(P)[d]) for d in range(P)])
-    centres = a + (np.arange(n) + 0.5) * h
-    X = centres[:, None, None, None] + 0.5 * h * t[None, None, :, None]
-    Y = centres[None, :, None, None] + 0.5 * h * t[None, None, None, :]
-    R = np.sqrt(X ** 2 + Y ** 2)
-    ind = ((R >= r0) & (R <= r1)).astype(float) * (h / sub) ** 2
-    M = np.einsum("ijuv,du,ev->ijde", ind, leg, leg)
-    nq = L_["xi"].size
-    seed = np.zeros((n + 2, n + 2, nq, P, P))
-    seed[1:-1, 1:-1] = L_["w"][None, None, :, None, None] * M[:, :, None]
-    return seed
-
-
 def bench_idc(args):
     """IMEX/IDC line-source pulse: 200x200, Q2, S16, 4 Gauss-Lobatto nodes, 3 corrections, dt = 0.01, FP32
     sweeps with FP64 correction sums; `--idc-steps` of the 100 steps to t = 1 are timed (inputs resident),
@@ -160,17 +114,19 @@ def bench_idc(args):
     steps, with sigma_s gradients and a checkpoint every 10 steps."""
     import torch
     import tsw
+    from tsw import problems
     dev = torch.device("cuda:0")
-    L_ = _line_source()
-    n, p = L_["n"], L_["p"]
+    LP, psi0_h = problems.line_source()
+    n, p = LP.nx, LP.p
+    h = (LP.bx - LP.ax) / n
     ctx = tsw.Context(0)
-    ctx.problem_set(n, n, p, L_["xi"], L_["eta"], L_["w"], L_["off"], L_["a"], L_["b"], L_["a"], L_["b"])
+    LP.set_on(ctx)
     ctx.idc_set(4, 3, tsw.IDC_LOBATTO, False)
-    psi = torch.from_numpy(L_["psi0"]).to(dev)
+    psi = torch.from_numpy(psi0_h).to(dev)
     psi0 = psi.clone()
     src = torch.zeros_like(psi)
-    st = torch.from_numpy(L_["st"]).to(dev)
-    ss = torch.from_numpy(L_["ss"]).to(dev)
+    st = torch.from_numpy(LP.sigma_t).to(dev)
+    ss = torch.from_numpy(LP.sigma_s).to(dev)
     dt, steps = 0.01, args.idc_steps
     ctx.idc_step(psi, src, st, ss, dt, abs_tol=1e-10, precision=tsw.FP32)  # warm-up (tables, tuning)
     psi.copy_(psi0)
@@ -181,11 +137,11 @@ def bench_idc(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     nsw, swms = ctx.sweep_timing(reset=True)
-    U = L_["xi"].size * n * n * (p + 1) ** 2
-    w_d = torch.from_numpy(L_["w"]).to(dev)
-    phi_mass = float((psi[1:-1, 1:-1, :, 0, 0] * w_d).sum()) * L_["h"] ** 2   # integral of phi (1 at t=0)
+    U = LP.unknowns
+    w_d = torch.from_numpy(LP.w).to(dev)
+    phi_mass = float((psi[1:-1, 1:-1, :, 0, 0] * w_d).sum()) * h ** 2   # integral of phi (1 at t=0)
     print(f"bench: idc {steps} steps {wall:.3f}s, {nsw} sweeps ({swms:.1f} ms in sweeps)", file=sys.stderr, flush=True)
-    seed = torch.from_numpy(_ring_seed(L_)).to(dev)
+    seed = torch.from_numpy(problems.ring_seed(LP)).to(dev)
     J = float((seed * psi).sum())
     grad = torch.zeros_like(psi)
     gss = torch.zeros_like(ss)

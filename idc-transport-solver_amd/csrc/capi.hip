@@ -50,8 +50,27 @@ struct tsw_ctx {
         bool have64 = false, have32 = false;
         DBuf key;            // device copy of the sigma_t the table was built from
         bool keyed = false;  // key holds that copy
+        unsigned long long build = 0;  // counts builds and cross-section changes
     } mat;
     DBuf same_flag;  // device flag of the input change checks
+    // Cross-section degree (tsw_set_xs_degree): 0, cell means; p, Legendre coefficients per cell, used
+    // through their triple-product tensors, built once per call and per coefficient array (two slots,
+    // keyed by device pointer, cleared at every entry point), plus their cell means for DSA.
+    int xs_deg = 0;
+    struct XsCache {
+        const double *key[2] = {nullptr, nullptr};
+        DBuf t64[2], t32[2], mean[2];
+        bool have32[2] = {false, false}, have_mean[2] = {false, false};
+        int next = 0;
+    } xs;
+    // relative error of an FP32-sweep GMRES product, measured for the material table build `mat_build`,
+    // 1/dt and the DSA setting (value < 0: not measured)
+    struct Fp32Err {
+        double value = -1.0;
+        unsigned long long mat_build = 0;
+        double inv_dt = -1.0;
+        bool dsa = false;
+    } fp32_err;
     static constexpr int kMaxMaterials = 64;
 
     // Inputs and work arrays of the staggered (diagonal-layout) sweep, per precision where needed:
@@ -160,6 +179,9 @@ struct tsw_ctx {
     size_t xtr64_n = 0, xtr32_n = 0;
     int *abort_host = nullptr;
     bool bands_used = false;
+    // test hook (tuning key stall_test): the next banded sweep starts with the stall flag raised, so every
+    // bounded wait of its bands gives up within one poll interval and the call reports the stall
+    bool stall_test = false;
     static constexpr int kSideStreams = 4;
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
@@ -188,6 +210,7 @@ int guarded(tsw_ctx *ctx, F &&f) {
     if (!ctx) return TSW_E_ARG;
     try {
         ctx->err.clear();
+        ctx->xs.key[0] = ctx->xs.key[1] = nullptr;  // cross sections may have changed since the last call
         TSW_HIP(hipSetDevice(ctx->device));
         f();
         return TSW_OK;
@@ -204,7 +227,61 @@ void require(bool cond, int code, const std::string &msg) {
     if (!cond) throw Error(code, msg);
 }
 
+double *dbufd(DBuf &b, size_t n) { return static_cast<double *>(b.reserve(n * sizeof(double))); }
+
 void need_problem(tsw_ctx *c) { require(c->have_problem, TSW_E_STATE, "tsw_problem_set has not been called"); }
+
+// Doubles in one cross-section array: cell means, or (p+1)^2 Legendre coefficients per cell.
+size_t xs_n(const tsw_ctx *c) { return c->g.ncell_h() * (c->xs_deg > 0 ? (size_t)c->g.N() : 1); }
+
+// Legendre triple products with max(2p, 2) Gauss points, entries below 5e-16 set to zero
+// (ComputeTPI, global.cpp:67-107).
+Tpi tpi_table(int p);
+
+// Slot of the tensor cache holding coefficient array sig (device), built on first use in this call.
+int xs_slot(tsw_ctx *c, const double *sig) {
+    auto &X = c->xs;
+    for (int k = 0; k < 2; ++k)
+        if (X.key[k] == sig) return k;
+    const int k = X.next;
+    X.next ^= 1;
+    const Geom &g = c->g;
+    launch_xs_tensor(g, sig, tpi_table(g.p), dbufd(X.t64[k], g.ncell_h() * g.N() * g.N()), c->stream);
+    X.key[k] = sig;
+    X.have32[k] = X.have_mean[k] = false;
+    return k;
+}
+
+// Tensor of degree-p cross section sig in precision T, or null for cell means.
+template <typename T>
+const T *xs_tensor(tsw_ctx *c, const double *sig) {
+    if (c->xs_deg == 0 || !sig) return nullptr;
+    const int k = xs_slot(c, sig);
+    auto &X = c->xs;
+    if constexpr (sizeof(T) == 8) {
+        return X.t64[k].as<double>();
+    } else {
+        const size_t n = c->g.ncell_h() * c->g.N() * c->g.N();
+        if (!X.have32[k]) {
+            launch_convert_d2f(X.t64[k].as<double>(), static_cast<float *>(X.t32[k].reserve(n * sizeof(float))), n,
+                               c->stream);
+            X.have32[k] = true;
+        }
+        return X.t32[k].as<float>();
+    }
+}
+
+// Cell means of a cross-section array (the array itself for cell means).
+const double *xs_means(tsw_ctx *c, const double *sig) {
+    if (c->xs_deg == 0 || !sig) return sig;
+    const int k = xs_slot(c, sig);
+    auto &X = c->xs;
+    if (!X.have_mean[k]) {
+        launch_xs_means(c->g, sig, dbufd(X.mean[k], c->g.ncell_h()), c->stream);
+        X.have_mean[k] = true;
+    }
+    return X.mean[k].as<double>();
+}
 
 // Device view of a caller array: the pointer itself for TSW_MEM_DEVICE, else a staged copy.
 template <typename T>
@@ -334,6 +411,7 @@ void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
     }
     m.valid = false;
     m.have64 = m.have32 = false;
+    ++m.build;
     std::vector<double> h(g.ncell_h());
     TSW_HIP(hipMemcpyAsync(h.data(), sigma, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     TSW_HIP(hipStreamSynchronize(c->stream));
@@ -361,10 +439,12 @@ void prepare_materials(tsw_ctx *c, const double *sigma, double dt) {
     m.valid = true;
 }
 
-// One in-place sweep of device array psi with device cell cross sections sigma.
+// One in-place sweep of device array psi with device cell cross sections sigma (sig64: the caller's FP64
+// array, whose tensor replaces sigma for degree-p cross sections).
 template <typename T>
-void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt) {
-    const SweepParams<T> sp = params<T>(c, psi, sigma, dt);
+void sweep_dev(tsw_ctx *c, T *psi, const T *sigma, double dt, const double *sig64) {
+    SweepParams<T> sp = params<T>(c, psi, sigma, dt);
+    sp.xs = xs_tensor<T>(c, sig64);
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     TSW_HIP(hipEventRecord(e0, c->stream));
     switch (c->pattern) {
@@ -409,7 +489,7 @@ const float *as_f32(tsw_ctx *c, const double *d, size_t n, DBuf &buf) {
 // Sweep a double device array in the requested precision.
 void sweep_prec(tsw_ctx *c, double *psi, const double *sigma, double dt, int precision) {
     if (precision == TSW_FP64) {
-        sweep_dev<double>(c, psi, sigma, dt);
+        sweep_dev<double>(c, psi, sigma, dt, sigma);
         return;
     }
     require(precision == TSW_FP32, TSW_E_ARG, "precision must be TSW_FP64 or TSW_FP32");
@@ -417,7 +497,7 @@ void sweep_prec(tsw_ctx *c, double *psi, const double *sigma, double dt, int pre
     float *p32 = static_cast<float *>(c->psi32.reserve(n * sizeof(float)));
     launch_convert_d2f(psi, p32, n, c->stream);
     const float *s32 = as_f32(c, sigma, c->g.ncell_h(), c->sig_t32);
-    sweep_dev<float>(c, p32, s32, dt);
+    sweep_dev<float>(c, p32, s32, dt, sigma);
     launch_convert_f2d(p32, psi, n, c->stream);
 }
 
@@ -656,9 +736,15 @@ void dsa_apply(tsw_ctx *c, const double *x, double *y) {
     launch_dsa_add(g, D.lv[0].y, y, c->stream);
 }
 
-// Relative error bound assumed for an FP32 sweep product in the relaxed mixed-precision GMRES (the
-// measured FP32 sweep error is 1e-6 relative or below on the lattice and on random problems).
-constexpr double kFp32SweepErr = 1e-5;
+// Relaxed mixed-precision GMRES: the relative error d of an FP32-sweep product is measured once per
+// (material table, dt, DSA setting) as ||A32 b - A64 b|| / ||b||, and FP32 products are admitted once the
+// residual estimate is <= abs_tol / (kFp32ErrSafety d).  d is kept within [kFp32ErrFloor, kFp32ErrCeil]: the
+// floor keeps an exactly representable problem (d = 0) from admitting FP32 products at the first step,
+// the ceiling keeps a badly conditioned one from never admitting them (the final FP64 residual check
+// guards the result either way).
+constexpr double kFp32ErrSafety = 10.0;
+constexpr double kFp32ErrFloor = 1e-7;
+constexpr double kFp32ErrCeil = 1e-4;
 
 struct Gmres {
     tsw_ctx *c;
@@ -712,6 +798,8 @@ struct Gmres {
     // Fraysse 2005, Simoncini and Szyld 2003), so a fixed relative error d of apply_lo is admissible from
     // ||r_k|| <= abs_tol / d on.
     // x_zero: x is zero on entry, so the first residual is b itself and needs no product.
+    // used_lo: set when a call took any apply_lo product (its residual estimate then rests on them).
+    bool used_lo = false;
     template <typename Apply, typename ApplyLo>
     bool solve(Apply &&apply, ApplyLo &&apply_lo, double lo_below, const double *b, double *x, double *r,
                double abs_tol, int max_its, int &its, double &res, bool x_zero = false) {
@@ -719,6 +807,7 @@ struct Gmres {
         double *h1 = hdev, *h2 = hdev + (m + 2);
         its = 0;
         res = 0.0;
+        used_lo = false;
         bool zero = x_zero;
         while (true) {
             if (zero) {
@@ -739,8 +828,12 @@ struct Gmres {
             bool conv = false;
             for (int j = 0; j < m && its < max_its; ++j) {
                 double *vn = V + (size_t)(j + 1) * n;
-                if (res <= lo_below) apply_lo(V + (size_t)j * n, vn);
-                else apply(V + (size_t)j * n, vn);
+                if (res <= lo_below) {
+                    apply_lo(V + (size_t)j * n, vn);
+                    used_lo = true;
+                } else {
+                    apply(V + (size_t)j * n, vn);
+                }
                 ++its;
                 launch_multi_dot(n, j + 1, V, n, vn, h1, part, nb, c->stream);
                 launch_multi_axpy(n, j + 1, V, n, h1, vn, c->stream);
@@ -1037,6 +1130,11 @@ void diag_sweep(tsw_ctx *c, double dt, const T *qsrc, const T *phi_d, bool inflo
     select_tuning<T>(c, dt, qsrc, phi_d, inflow);
     DiagSweepParams<T> sp = make_diag_params<T>(c, dt, qsrc, phi_d, inflow, psi_out);
     sp.skip = c->skip_flag;
+    if (c->stall_test && sp.nbands > 1 && !c->skip_flag) {
+        c->stall_test = false;
+        *c->abort_host = 1;
+        TSW_HIP(hipMemcpyAsync(c->abort_dev.ptr, c->abort_host, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
     if (c->skip_flag) {  // inside a graph capture: the replay is timed as a whole
         launch_diag_sweep<T>(sp, c->g.p, c->stream);
         return;
@@ -1075,7 +1173,7 @@ void diag_store(tsw_ctx *c, const T *psi_d, double *out_ref) {
 
 // True when the pattern is a staggered one and the mesh fits its kernel (else the wavefront kernels run).
 bool staggered_ok(tsw_ctx *c) {
-    if (!is_staggered(c->pattern)) return false;
+    if (!is_staggered(c->pattern) || c->xs_deg > 0) return false;  // degree-p cross sections: wavefront kernels
     DiagSweepParams<double> probe{};
     probe.nx = c->g.nx;
     probe.ny = c->g.ny;
@@ -1307,8 +1405,8 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_store<double>(c, psi_d, Q);
     } else if (!c->gmres_refine) {
         // Mixed precision, relaxed: FP64 sweeps for b = P L^{-1} Q, the Krylov steps while the residual
-        // estimate exceeds abs_tol / kFp32SweepErr and the final psi; FP32 sweeps for the other steps.  The
-        // Krylov vectors, the orthogonalisation and the DSA correction stay FP64.
+        // estimate exceeds abs_tol / (kFp32ErrSafety d) and the final psi; FP32 sweeps for the other steps.
+        // The Krylov vectors, the orthogonalisation and the DSA correction stay FP64.
         diag_setup(c, st, ss, Q, dt, true, true, false);
         double *psi_d = dbuf<double>(c->diag.psi64, npsi_d);
         float *psi32 = dbuf<float>(c->diag.psi32, npsi_d);
@@ -1326,9 +1424,38 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_sweep<double>(c, dt, q_d, nullptr, true, psi_d);
         diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
         if (dsa) dsa_apply(c, b, b);
-        converged = solver.solve(apply64, apply32, abs_tol / kFp32SweepErr, b, x, r, abs_tol, max_its, its_total,
-                                 res_final, true);
-        diag_sweep<double>(c, dt, q_d, diag_phi<double>(c, x), true, psi_d);
+        // measured FP32 product error, once per material table, dt and DSA setting
+        auto &fe = c->fp32_err;
+        if (!(fe.value >= 0.0 && fe.mat_build == c->mat.build && fe.inv_dt == inverse_dt(dt) && fe.dsa == dsa)) {
+            const double nb = solver.norm(b);
+            double d = kFp32ErrFloor;
+            if (nb > 0.0) {
+                apply64(b, x);
+                apply32(b, r);
+                launch_axpby(n, -1.0, x, 1.0, r, s);  // r = A32 b - A64 b
+                d = solver.norm(r) / nb;
+                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            }
+            fe = {std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d)), c->mat.build, inverse_dt(dt), dsa};
+        }
+        const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
+        converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
+        // psi = L^{-1}(Q + S phi) in FP64; with FP32 products in the solve, the true residual
+        // b - A x = M (P psi - x) comes from the same sweep, and the solve resumes from x while it exceeds
+        // abs_tol (FP64 restart residual, relaxed steps as before)
+        for (int check = 0;; ++check) {
+            diag_sweep<double>(c, dt, q_d, diag_phi<double>(c, x), true, psi_d);
+            if (!solver.used_lo) break;
+            diag_pmv<double>(c, psi_d, r, 1.0, -1.0, x);
+            if (dsa) dsa_apply(c, r, r);
+            res_final = solver.norm(r);
+            converged = res_final <= abs_tol;
+            if (converged || its_total >= max_its || check >= 3) break;
+            int its = 0;
+            double rr = 0.0;
+            converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its - its_total, its, rr, false);
+            its_total += its;
+        }
         diag_store<double>(c, psi_d, Q);
     } else {
         diag_setup(c, st, ss, Q, dt, true, true, false);  // FP32 sweeps of residuals: no FP32 source or inflow
@@ -1393,7 +1520,8 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
     bool converged = false;
     its_total = 0;
     const bool dsa = ctx->use_dsa;
-    if (dsa) dsa_build(ctx, st, ss, dt);
+    if (dsa) dsa_build(ctx, xs_means(ctx, st), xs_means(ctx, ss), dt);
+    const double *xt64 = xs_tensor<double>(ctx, st), *xss64 = xs_tensor<double>(ctx, ss);
     double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
     // vectors: b, x, r (phi-shaped, zero halos)
     DBuf vecs;
@@ -1404,7 +1532,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
 
     // matvec y = (I - P L^{-1} S) x with zero inflow (PETScPeierlsSolver.cpp:548-555)
     auto apply = [&](const double *xv, double *yv) {
-        launch_smv<double>(g, ss, xv, psi, 1.0, 0.0, 1, s);
+        launch_smv<double>(g, ss, xv, psi, 1.0, 0.0, 1, s, xss64);
         sweep_prec(ctx, psi, st, dt, precision);
         TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         launch_pmv<double, double>(g, psi, yv, w, -1.0, 1.0, 0, s);
@@ -1419,7 +1547,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         if (dsa) dsa_apply(ctx, b, b);
         converged = solver.solve(apply, b, x, r, abs_tol, max_its, its_total, res_final, true);
         // source <- L^{-1}(Q + S phi) (:504-505)
-        launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s);
+        launch_smv<double>(g, ss, x, Q, 1.0, 1.0, 1, s, xss64);
         sweep_prec(ctx, Q, st, dt, precision);
     } else {
         // Mixed precision: an FP64 iterate psi64 is refined with corrections computed from FP32
@@ -1433,13 +1561,14 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
         float *phi32 = static_cast<float *>(ctx->phi32.reserve(sizeof(float) * n));
         const float *st32 = as_f32(ctx, st, ncell, ctx->sig_t32);
         const float *ss32 = as_f32(ctx, ss, ncell, ctx->sig_s32);
+        const float *xss32 = xs_tensor<float>(ctx, ss);
         const double *xi64 = ctx->xi64.as<double>(), *eta64 = ctx->eta64.as<double>();
 
         // y = (I - P L^{-1} S) x with FP32 sweeps
         auto apply32 = [&](const double *xv, double *yv) {
             launch_convert_d2f(xv, phi32, n, s);
-            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 0.0f, 1, s);
-            sweep_dev<float>(ctx, p32, st32, dt);
+            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 0.0f, 1, s, xss32);
+            sweep_dev<float>(ctx, p32, st32, dt, st);
             TSW_HIP(hipMemcpyAsync(yv, xv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             launch_pmv<float, double>(g, p32, yv, w, -1.0, 1.0, 0, s);
             if (dsa) dsa_apply(ctx, yv, yv);
@@ -1454,13 +1583,13 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
             TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
             launch_pmv<double, double>(g, psi64, x, w, 1.0, 0.0, 0, s);
             TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-            launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s);
-            launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s);
+            launch_smv<double>(g, ss, x, res64, 1.0, 1.0, 0, s, xss64);
+            launch_lmv<double>(g, xi64, eta64, st, psi64, res64, -1.0, 1.0, s, xt64);
             if (inverse_dt(dt) != 0.0) launch_axpby(npsi, -inverse_dt(dt), psi64, 1.0, res64, s);
             launch_fill_region<double>(g, res64, (size_t)g.nq * N, 1, 0.0, s);
             // b = P L^{-1} r: the Peierls residual of the current iterate
             launch_convert_d2f(res64, p32, npsi, s);
-            sweep_dev<float>(ctx, p32, st32, dt);
+            sweep_dev<float>(ctx, p32, st32, dt, st);
             TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n, s));
             launch_pmv<float, double>(g, p32, b, w, 1.0, 0.0, 0, s);
             if (dsa) dsa_apply(ctx, b, b);
@@ -1479,8 +1608,8 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
             its_total += its;
             launch_convert_d2f(res64, p32, npsi, s);
             launch_convert_d2f(x, phi32, n, s);
-            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 1.0f, 1, s);
-            sweep_dev<float>(ctx, p32, st32, dt);
+            launch_smv<float>(g, ss32, phi32, p32, 1.0f, 1.0f, 1, s, xss32);
+            sweep_dev<float>(ctx, p32, st32, dt, st);
             launch_accumulate_f2d(p32, psi64, npsi, s);
         }
         // outputs: psi and phi = P psi
@@ -1516,11 +1645,12 @@ void sweep_wavefront_mixed(tsw_ctx *ctx, double *Q, const double *st, double dt)
     launch_fill_region<double>(g, psi64, per, 0, 0.0, s);
     for (int k = 0; k <= ctx->refine_steps; ++k) {
         TSW_HIP(hipMemcpyAsync(res64, Q, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
-        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), st, psi64, res64, -1.0, 1.0, s);
+        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), st, psi64, res64, -1.0, 1.0, s,
+                           xs_tensor<double>(ctx, st));
         if (inverse_dt(dt) != 0.0) launch_axpby(npsi, -inverse_dt(dt), psi64, 1.0, res64, s);
         launch_fill_region<double>(g, res64, per, 1, 0.0, s);
         launch_convert_d2f(res64, p32, npsi, s);
-        sweep_dev<float>(ctx, p32, st32, dt);
+        sweep_dev<float>(ctx, p32, st32, dt, st);
         launch_accumulate_f2d(p32, psi64, npsi, s);
     }
     TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
@@ -1578,6 +1708,27 @@ void gauss_legendre(int m, std::vector<double> &x, std::vector<double> &w) {
         x[k] = t;
         w[k] = 2.0 / ((1.0 - t * t) * dp * dp);
     }
+}
+
+Tpi tpi_table(int p) {
+    const int P = p + 1, m = std::max(2 * p, 2);
+    std::vector<double> x, w;
+    gauss_legendre(m, x, w);
+    Tpi t{};
+    for (int a = 0; a < P; ++a)
+        for (int b = 0; b < P; ++b)
+            for (int c = 0; c < P; ++c) {
+                double v = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    double pa, pb, pc, dp;
+                    legendre_pd(a, x[i], pa, dp);
+                    legendre_pd(b, x[i], pb, dp);
+                    legendre_pd(c, x[i], pc, dp);
+                    v += w[i] * pa * pb * pc;
+                }
+                t.v[a + P * (b + P * c)] = std::fabs(v) < 5e-16 ? 0.0 : v;
+            }
+    return t;
 }
 
 // Gauss-Lobatto nodes on [-1, 1]: the endpoints and the roots of P'_{m-1} (Newton with P'' from the
@@ -1648,9 +1799,9 @@ void idc_tables(tsw_ctx::Idc &I) {
 void idc_residual(tsw_ctx *c, const double *x, double *R, const double *st, const double *ss, double *phi) {
     const Geom &g = c->g;
     hipStream_t s = c->stream;
-    launch_lmv<double>(g, c->xi64.as<double>(), c->eta64.as<double>(), st, x, R, 1.0, 0.0, s);
+    launch_lmv<double>(g, c->xi64.as<double>(), c->eta64.as<double>(), st, x, R, 1.0, 0.0, s, xs_tensor<double>(c, st));
     launch_pmv<double, double>(g, x, phi, c->w64.as<double>(), 1.0, 0.0, 0, s);
-    launch_smv<double>(g, ss, phi, R, -1.0, 1.0, 0, s);
+    launch_smv<double>(g, ss, phi, R, -1.0, 1.0, 0, s, xs_tensor<double>(c, ss));
 }
 
 // implicit transport solve (1/dt_sub + L - S P) psi = rhs in place (rhs halos = inflow)
@@ -1748,6 +1899,9 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     ctx->dsa.graph = nullptr;
     ctx->dsa.keyed = false;
     ctx->mat.valid = ctx->mat.keyed = ctx->mat.have64 = ctx->mat.have32 = false;
+    ++ctx->mat.build;
+    ctx->xs_deg = 0;
+    ctx->xs.key[0] = ctx->xs.key[1] = nullptr;
     Geom &g = ctx->g;
     g.nx = nx; g.ny = ny; g.p = p; g.nq = nq;
     g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
@@ -2070,6 +2224,15 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name) {
     });
 }
 
+int tsw_set_xs_degree(tsw_ctx *ctx, int degree) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        require(degree == 0 || degree == ctx->g.p, TSW_E_ARG,
+                "cross-section degree must be 0 or the problem's DG degree " + std::to_string(ctx->g.p));
+        ctx->xs_deg = degree;
+    });
+}
+
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
     return guarded(ctx, [&] {
         require(key != nullptr, TSW_E_ARG, "null tuning key");
@@ -2109,6 +2272,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
             require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
                     "prefetch must be 0, 2, 3, 4 or 6");
             ctx->prefetch_opt = value;
+        } else if (k == "stall_test") {
+            require(value == 0 || value == 1, TSW_E_ARG, "stall_test must be 0 or 1");
+            ctx->stall_test = value != 0;
         } else {
             throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
         }
@@ -2141,7 +2307,7 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
         require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
                 "bad precision");
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
-        const double *sig = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *sig = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         sweep_device(ctx, psi, sig, dt, precision);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
         sync(ctx);
@@ -2165,10 +2331,10 @@ int tsw_smv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_s, cons
     return guarded(ctx, [&] {
         need_problem(ctx);
         const Geom &g = ctx->g;
-        const double *s = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *s = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         const double *f = stage_in(ctx, phi, g.nphi(), mem, ctx->phi64a);
         double *y = stage_in(ctx, (const double *)psi, g.npsi(), mem, ctx->psi64);
-        launch_smv<double>(g, s, f, y, alpha, beta, all_cells ? 1 : 0, ctx->stream);
+        launch_smv<double>(g, s, f, y, alpha, beta, all_cells ? 1 : 0, ctx->stream, xs_tensor<double>(ctx, s));
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi, y, g.npsi(), mem);
         sync(ctx);
     });
@@ -2177,12 +2343,21 @@ int tsw_smv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_s, cons
 int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, const double *x, double *y, int mem) {
     return guarded(ctx, [&] {
         need_problem(ctx);
-        require(x != y, TSW_E_ARG, "tsw_lmv needs x != y");
+        require(x != nullptr && y != nullptr, TSW_E_ARG, "null array argument");
         const Geom &g = ctx->g;
-        const double *s = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *s = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         const double *xd = stage_in(ctx, x, g.npsi(), mem, ctx->psi64);
         double *yd = stage_in(ctx, (const double *)y, g.npsi(), mem, ctx->r64);
-        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), s, xd, yd, alpha, beta, ctx->stream);
+        if (xd == yd) {
+            // y = beta y + alpha L y (Lmv_InPlace, TransportOperator.cpp:602-1200, dispatched at :1224-1228):
+            // the product reads a copy of y, so every cell sees its upwind neighbours' values from before
+            // the call, as the reference's in-place wavefront does
+            double *xc = dbufd(ctx->psi64, g.npsi());
+            TSW_HIP(hipMemcpyAsync(xc, yd, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, ctx->stream));
+            xd = xc;
+        }
+        launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), s, xd, yd, alpha, beta, ctx->stream,
+                           xs_tensor<double>(ctx, s));
         if (mem == TSW_MEM_HOST) stage_out(ctx, y, yd, g.npsi(), mem);
         sync(ctx);
     });
@@ -2197,8 +2372,8 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         const Geom &g = ctx->g;
         hipStream_t s = ctx->stream;
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         if (staggered_ok(ctx)) {
             int it = 0;
             double e = 0.0;
@@ -2228,7 +2403,7 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
             ++it;
             std::swap(cur, last);
             TSW_HIP(hipMemcpyAsync(psi, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));  // :226
-            launch_smv<double>(g, ss, last, psi, 1.0, 1.0, 0, s);                                     // :229
+            launch_smv<double>(g, ss, last, psi, 1.0, 1.0, 0, s, xs_tensor<double>(ctx, ss));         // :229
             sweep_prec(ctx, psi, st, dt, precision);                                                  // :232
             launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);                                      // :236
             e = device_norm2_interior_diff(ctx, cur, last);                                           // :239
@@ -2257,8 +2432,8 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
         hipStream_t s = ctx->stream;
         const size_t n = g.nphi(), npsi = g.npsi();
         double *Q = stage_in(ctx, source_inout, npsi, mem, ctx->q64);
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         DBuf phi_dev;
         double *pd = phi_out ? (mem == TSW_MEM_DEVICE ? phi_out : static_cast<double *>(phi_dev.reserve(sizeof(double) * n))) : nullptr;
         converged = gmres_device(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
@@ -2280,10 +2455,12 @@ int tsw_sweep_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, double
                   double *grad_src, double *grad_sigma_t, int precision, int mem) {
     return guarded(ctx, [&] {
         need_problem(ctx);
+        require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
+                                                     "(tsw_set_xs_degree 0)");
         require(seed != nullptr && grad_src != nullptr, TSW_E_ARG, "seed and grad_src are required");
         require(grad_sigma_t == nullptr || psi != nullptr, TSW_E_ARG, "grad_sigma_t needs the forward psi");
         const Geom &g = ctx->g;
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         const double *sd = stage_in(ctx, seed, g.npsi(), mem, ctx->vjp_seed);
         const double *pd = psi ? stage_in(ctx, psi, g.npsi(), mem, ctx->vjp_psi) : nullptr;
         double *gsrc = mem == TSW_MEM_DEVICE ? grad_src : dbuf<double>(ctx->vjp_gsrc, g.npsi());
@@ -2306,13 +2483,15 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
     bool converged = false;
     const int rc = guarded(ctx, [&] {
         need_problem(ctx);
+        require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
+                                                     "(tsw_set_xs_degree 0)");
         require(seed != nullptr && grad_src != nullptr, TSW_E_ARG, "seed and grad_src are required");
         require((grad_sigma_t == nullptr && grad_sigma_s == nullptr) || psi != nullptr, TSW_E_ARG,
                 "cross-section gradients need the forward psi");
         require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
         const Geom &g = ctx->g;
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         const double *sd = stage_in(ctx, seed, g.npsi(), mem, ctx->vjp_seed);
         const double *pd = psi ? stage_in(ctx, psi, g.npsi(), mem, ctx->vjp_psi) : nullptr;
         double *gsrc = mem == TSW_MEM_DEVICE ? grad_src : dbuf<double>(ctx->vjp_gsrc, g.npsi());
@@ -2365,8 +2544,8 @@ int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const do
         const Geom &g = ctx->g;
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->q64);
         const double *src = stage_in(ctx, source, g.npsi(), mem, ctx->vjp_seed);
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         int its = 0;
         ok = idc_step_device(ctx, psi, src, st, ss, dt, abs_tol, max_its, precision, its);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
@@ -2391,8 +2570,8 @@ int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const dou
         const Geom &g = ctx->g;
         double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->q64);
         const double *src = stage_in(ctx, source, g.npsi(), mem, ctx->vjp_seed);
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         for (int k = 0; k < nsteps && ok; ++k) {
             int its = 0;
             ok = idc_step_device(ctx, psi, src, st, ss, dt, abs_tol, max_its, precision, its);
@@ -2414,6 +2593,8 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
     bool ok = true;
     const int rc = guarded(ctx, [&] {
         need_problem(ctx);
+        require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
+                                                     "(tsw_set_xs_degree 0)");
         require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
         require(checkpoint_every >= 1, TSW_E_ARG, "checkpoint_every must be >= 1");
         require(seed && grad_psi0, TSW_E_ARG, "seed and grad_psi0 are required");
@@ -2426,8 +2607,8 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
         hipStream_t s = ctx->stream;
         const size_t n = g.npsi();
         const double *src = stage_in(ctx, source, n, mem, ctx->vjp_seed);
-        const double *st = stage_in(ctx, sigma_t, g.ncell_h(), mem, ctx->sig_t64);
-        const double *ss = stage_in(ctx, sigma_s, g.ncell_h(), mem, ctx->sig_s64);
+        const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
         const double *p0 = psi0 ? stage_in(ctx, psi0, n, mem, ctx->vjp_psi) : nullptr;
         double *lam = mem == TSW_MEM_DEVICE ? grad_psi0 : dbuf<double>(ctx->vjp_gsrc, n);
         if (lam != seed) {

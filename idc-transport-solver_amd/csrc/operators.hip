@@ -71,13 +71,83 @@ __global__ __launch_bounds__(kBlock) void k_smv(const T *__restrict__ sigma_s, c
     *y = beta == T(0) ? val : beta * *y + val;
 }
 
+// Smv with a degree-p sigma_s (TransportOperator.cpp:375-381): the projection of sigma_s phi / (4 pi),
+// val(d,e) = alpha (2d+1)(2e+1)/(16 pi) sum_{u,v} sigma(i,j,d,e,u,v) phi(u,v).
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_smv_xs(const T *__restrict__ xs, const T *__restrict__ phi,
+                                                   T *__restrict__ psi, T alpha, T beta, Domain dom, int ny,
+                                                   int nq) {
+    constexpr int N = P * P;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_cell = (size_t)nq * N;
+    if (t >= dom.count() * per_cell) return;
+    const size_t c = t / per_cell;
+    const int r = (int)(t - c * per_cell);
+    const int k = r % N, d = k / P, e = k - P * d;
+    const int i = dom.i0 + (int)(c / dom.nj), j = dom.j0 + (int)(c % dom.nj);
+    const size_t cell = (size_t)i * (ny + 2) + j;
+    const T *tc = xs + cell * N * N + k;
+    const T *f = phi + cell * N;
+    T v = T(0);
+#pragma unroll
+    for (int uv = 0; uv < N; ++uv) v += tc[(size_t)uv * N] * f[uv];
+    v *= T((2 * d + 1) * (2 * e + 1)) / T(16.0 * M_PI);
+    const T val = v * alpha;
+    T *y = psi + cell * per_cell + r;
+    *y = beta == T(0) ? val : beta * *y + val;
+}
+
 template <typename T>
 void launch_smv(const Geom &g, const T *sigma_s, const T *phi, T *psi, T alpha, T beta, int all_cells,
-                hipStream_t s) {
+                hipStream_t s, const T *xs) {
     const Domain dom = domain(g, all_cells);
     const size_t n = dom.count() * g.nq * g.N();
-    hipLaunchKernelGGL(k_smv<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, sigma_s, phi, psi, alpha, beta, dom, g.ny,
-                       g.nq, g.N());
+    if (xs) {
+        switch (g.p) {
+#define TSW_SMV_XS(PP) \
+    case PP - 1: hipLaunchKernelGGL((k_smv_xs<T, PP>), dim3(grid_for(n)), dim3(kBlock), 0, s, xs, phi, psi, alpha, beta, dom, g.ny, g.nq); break;
+            TSW_SMV_XS(1) TSW_SMV_XS(2) TSW_SMV_XS(3) TSW_SMV_XS(4)
+#undef TSW_SMV_XS
+            default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built (0..3)");
+        }
+    } else {
+        hipLaunchKernelGGL(k_smv<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, sigma_s, phi, psi, alpha, beta, dom,
+                           g.ny, g.nq, g.N());
+    }
+    TSW_HIP(hipGetLastError());
+}
+
+// ---- tensor of a degree-p cross section (CrossSection::ComputeTensor, CrossSection.cpp:165-189) ----
+__global__ __launch_bounds__(kBlock) void k_xs_tensor(const double *__restrict__ sig, double *__restrict__ t,
+                                                      size_t ncell, int P, Tpi tpi) {
+    const int N = P * P;
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ncell * N * N) return;
+    const size_t cell = x / ((size_t)N * N);
+    const int r = (int)(x - cell * N * N), de = r / N, ab = r - de * N;
+    const int d = de / P, e = de - d * P, a = ab / P, b = ab - a * P;
+    const double *sc = sig + cell * N;
+    double acc = 0.0;
+    for (int u = 0; u < P; ++u)
+        for (int v = 0; v < P; ++v)
+            acc += sc[u * P + v] * tpi.v[a + P * (d + P * u)] * tpi.v[b + P * (e + P * v)];
+    t[x] = acc;
+}
+
+void launch_xs_tensor(const Geom &g, const double *sig, const Tpi &tpi, double *t, hipStream_t s) {
+    const size_t n = g.ncell_h() * g.N() * g.N();
+    hipLaunchKernelGGL(k_xs_tensor, dim3(grid_for(n)), dim3(kBlock), 0, s, sig, t, g.ncell_h(), g.p + 1, tpi);
+    TSW_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_xs_means(const double *__restrict__ sig, double *__restrict__ out,
+                                                     size_t ncell, int N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell) out[c] = sig[c * N];
+}
+
+void launch_xs_means(const Geom &g, const double *sig, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_xs_means, dim3(grid_for(g.ncell_h())), dim3(kBlock), 0, s, sig, out, g.ncell_h(), g.N());
     TSW_HIP(hipGetLastError());
 }
 
@@ -86,7 +156,7 @@ template <typename T, int P>
 __global__ __launch_bounds__(kBlock) void k_lmv(const T *__restrict__ xi_, const T *__restrict__ eta_,
                                                 const T *__restrict__ sigma, const T *__restrict__ x,
                                                 T *__restrict__ y, T alpha, T beta, int nx, int ny, int nq, T dx,
-                                                T dy) {
+                                                T dy, const T *__restrict__ xs) {
     constexpr int N = P * P;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nx * ny * nq) return;
@@ -97,10 +167,22 @@ __global__ __launch_bounds__(kBlock) void k_lmv(const T *__restrict__ xi_, const
     const size_t base = (size_t)i * si + (size_t)j * sj + (size_t)q * N;
     const T *xc = x + base;
     const T xi = xi_[q], eta = eta_[q];
-    const T sig = sigma[(size_t)i * (ny + 2) + j];
     T out[N];
+    if (!xs) {
+        const T sig = sigma[(size_t)i * (ny + 2) + j];
 #pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = (beta == T(0) ? T(0) : y[base + k] * beta) + alpha * sig * xc[k];
+        for (int k = 0; k < N; ++k) out[k] = (beta == T(0) ? T(0) : y[base + k] * beta) + alpha * sig * xc[k];
+    } else {  // degree-p sigma (:1587-1596): alpha (2d+1)(2e+1) sigma(i,j,d,e,u,v) x(u,v) / 4
+        const T *tc = xs + ((size_t)i * (ny + 2) + j) * N * N;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int d = k / P, e = k - P * d;
+            T acc = beta == T(0) ? T(0) : y[base + k] * beta;
+#pragma unroll
+            for (int uv = 0; uv < N; ++uv) acc += alpha * T((2 * d + 1) * (2 * e + 1)) * tc[uv * N + k] * xc[uv] / T(4);
+            out[k] = acc;
+        }
+    }
     T coeff = alpha * xi / dx;
     if (xi < T(0)) {
         const T *xn = xc + si;
@@ -163,15 +245,15 @@ __global__ __launch_bounds__(kBlock) void k_lmv(const T *__restrict__ xi_, const
 
 template <typename T>
 void launch_lmv(const Geom &g, const T *xi, const T *eta, const T *sigma_t, const T *x, T *y, T alpha, T beta,
-                hipStream_t s) {
+                hipStream_t s, const T *xs) {
     const size_t n = (size_t)g.nx * g.ny * g.nq;
     const dim3 grid(grid_for(n)), block(kBlock);
     const T dx = T(g.dx), dy = T(g.dy);
     switch (g.p) {
-        case 0: hipLaunchKernelGGL((k_lmv<T, 1>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
-        case 1: hipLaunchKernelGGL((k_lmv<T, 2>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
-        case 2: hipLaunchKernelGGL((k_lmv<T, 3>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
-        case 3: hipLaunchKernelGGL((k_lmv<T, 4>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy); break;
+        case 0: hipLaunchKernelGGL((k_lmv<T, 1>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy, xs); break;
+        case 1: hipLaunchKernelGGL((k_lmv<T, 2>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy, xs); break;
+        case 2: hipLaunchKernelGGL((k_lmv<T, 3>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy, xs); break;
+        case 3: hipLaunchKernelGGL((k_lmv<T, 4>), grid, block, 0, s, xi, eta, sigma_t, x, y, alpha, beta, g.nx, g.ny, g.nq, dx, dy, xs); break;
         default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
     }
     TSW_HIP(hipGetLastError());
@@ -180,10 +262,14 @@ void launch_lmv(const Geom &g, const T *xi, const T *eta, const T *sigma_t, cons
 template void launch_pmv<double, double>(const Geom &, const double *, double *, const double *, double, double, int, hipStream_t);
 template void launch_pmv<float, float>(const Geom &, const float *, float *, const float *, float, float, int, hipStream_t);
 template void launch_pmv<float, double>(const Geom &, const float *, double *, const double *, double, double, int, hipStream_t);
-template void launch_smv<double>(const Geom &, const double *, const double *, double *, double, double, int, hipStream_t);
-template void launch_smv<float>(const Geom &, const float *, const float *, float *, float, float, int, hipStream_t);
-template void launch_lmv<double>(const Geom &, const double *, const double *, const double *, const double *, double *, double, double, hipStream_t);
-template void launch_lmv<float>(const Geom &, const float *, const float *, const float *, const float *, float *, float, float, hipStream_t);
+template void launch_smv<double>(const Geom &, const double *, const double *, double *, double, double, int, hipStream_t,
+                                 const double *);
+template void launch_smv<float>(const Geom &, const float *, const float *, float *, float, float, int, hipStream_t,
+                                const float *);
+template void launch_lmv<double>(const Geom &, const double *, const double *, const double *, const double *, double *, double, double,
+                                 hipStream_t, const double *);
+template void launch_lmv<float>(const Geom &, const float *, const float *, const float *, const float *, float *, float, float,
+                                hipStream_t, const float *);
 
 // ---- precision conversion ----
 __global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, size_t n) {
@@ -292,9 +378,10 @@ void launch_lincomb(const Geom &g, size_t per_cell, const LinComb &L, double *y,
     case M: hipLaunchKernelGGL(k_lincomb<M>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, L, y, g.nx, g.ny, per_cell); break;
         TSW_LINCOMB_CASE(1) TSW_LINCOMB_CASE(2) TSW_LINCOMB_CASE(3) TSW_LINCOMB_CASE(4) TSW_LINCOMB_CASE(5)
         TSW_LINCOMB_CASE(6) TSW_LINCOMB_CASE(7) TSW_LINCOMB_CASE(8) TSW_LINCOMB_CASE(9) TSW_LINCOMB_CASE(10)
-        TSW_LINCOMB_CASE(11) TSW_LINCOMB_CASE(12)
+        TSW_LINCOMB_CASE(11) TSW_LINCOMB_CASE(12) TSW_LINCOMB_CASE(13) TSW_LINCOMB_CASE(14) TSW_LINCOMB_CASE(15)
+        TSW_LINCOMB_CASE(16)
 #undef TSW_LINCOMB_CASE
-        default: throw Error(TSW_E_ARG, "linear combination of 1..12 arrays");
+        default: throw Error(TSW_E_ARG, "linear combination of 1..16 arrays");
     }
     TSW_HIP(hipGetLastError());
 }

@@ -36,8 +36,23 @@ __global__ __launch_bounds__(256) void k_sweep_diag(SweepParams<T> sp, int diag,
     T B[N];
     assemble_rhs<T, P>(B, self, upx, upy, xi, eta, sp.dx, sp.dy);
     T A[N * N];
-    const T sig = sp.sigma[(size_t)i * (sp.ny + 2) + j];
-    assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, (sig + sp.inv_dt) * sp.dx * sp.dy);
+    if (!sp.xs) {
+        const T sig = sp.sigma[(size_t)i * (sp.ny + 2) + j];
+        assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, (sig + sp.inv_dt) * sp.dx * sp.dy);
+    } else {
+        // degree-p sigma_t (SweepLinearSolveSpec.cpp:300-321): dx dy / dt on the diagonal, then
+        // dx dy / 4 (2d+1)(2e+1) sigma(i,j,d,e,u,v) in row (d,e), column (u,v)
+        assemble_matrix<T, P>(A, xi, eta, sp.dx, sp.dy, sp.dx * sp.dy * sp.inv_dt);
+        const T *tc = sp.xs + ((size_t)i * (sp.ny + 2) + j) * N * N;
+        const T quarter = sp.dx * sp.dy / T(4);
+#pragma unroll
+        for (int uv = 0; uv < N; ++uv)
+#pragma unroll
+            for (int de = 0; de < N; ++de) {
+                const int d = de / P, e = de - P * d;
+                A[de + N * uv] += quarter * T((2 * d + 1) * (2 * e + 1)) * tc[uv * N + de];
+            }
+    }
     ge_solve<T, N>(A, B);
 #pragma unroll
     for (int k = 0; k < N; ++k) self[k] = B[k];

@@ -89,7 +89,21 @@ template <typename T> struct SweepParams {
     int nx, ny, nq;
     int qo1, qo2, qo3;  // quadrant offsets 1..3
     T dx, dy, inv_dt;
+    const T *xs;      // degree-p sigma_t: its triple-product tensor (launch_xs_tensor), replaces sigma; or null
 };
+
+// Triple products of Legendre polynomials on [-1, 1], tpi[a + P(b + P c)] = int P_a P_b P_c, P <= 4
+// (ComputeTPI, global.cpp:67-107).
+struct Tpi {
+    double v[64];
+};
+// Tensor of a degree-p cross section from its Legendre coefficients sig[cell][u][v] (cells incl. halo):
+// t[cell][d*P+e][a*P+b] = sum_{u,v} sig[cell][u][v] tpi(a,d,u) tpi(b,e,v) (CrossSection::ComputeTensor,
+// CrossSection.cpp:165-189; TensorIndex CrossSection.hpp:397-478), so the coefficient sigma(i,j,d,e,u,v)
+// of CalcA, Smv and Lmv sits at t[cell][u*P+v][d*P+e].
+void launch_xs_tensor(const Geom &g, const double *sig, const Tpi &tpi, double *t, hipStream_t s);
+// out[cell] = sig[cell][0][0] (cell means of a degree-p cross section)
+void launch_xs_means(const Geom &g, const double *sig, double *out, hipStream_t s);
 
 // Sweep on the diagonal-major layout (diag_layout.hpp).  Source of a cell: qsrc (if set) plus
 // sig_s * phi / (4 pi) (if phi is set).  Inflow traces tin_x/tin_y (null: vacuum).  Element solve by
@@ -212,26 +226,29 @@ void launch_pmv(const Geom &g, const TI *psi, TO *phi, const TO *w, TO alpha, TO
 template <typename T>
 void launch_fill_region(const Geom &g, T *a, size_t per_cell, int halo, T value, hipStream_t s);
 void launch_accumulate_f2d(const float *in, double *out, size_t n, hipStream_t s);
+// xs (optional): tensor of a degree-p sigma (launch_xs_tensor), used instead of the cell means
 template <typename T>
 void launch_smv(const Geom &g, const T *sigma_s, const T *phi, T *psi, T alpha, T beta, int all_cells,
-                hipStream_t s);
+                hipStream_t s, const T *xs = nullptr);
 template <typename T>
 void launch_lmv(const Geom &g, const T *xi, const T *eta, const T *sigma_t, const T *x, T *y, T alpha, T beta,
-                hipStream_t s);
+                hipStream_t s, const T *xs = nullptr);
 void launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
 void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
 
 // y[k] = a*x[k] + b*y[k]
 void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipStream_t s);
-// y = sum_j a[j] x[j] (j < m <= 12) over psi-layout arrays in one pass; halo cells take only the terms
-// whose bit is set in halo_terms
+// y = sum_j a[j] x[j] (j < m <= kMax) over psi-layout arrays in one pass; halo cells take only the terms
+// whose bit is set in halo_terms.  kMax covers the largest IDC source: 3 + 12 residual terms (12 nodes,
+// plus the left end point for Gauss-Legendre nodes).
 struct LinComb {
+    static constexpr int kMax = 16;
     int m = 0;
     unsigned halo_terms = 0;
-    double a[12] = {};
-    const double *x[12] = {};
+    double a[kMax] = {};
+    const double *x[kMax] = {};
     void add(double coef, const double *arr, bool on_halo = false) {
-        if (m >= 12) throw Error(TSW_E_ARG, "linear combination of more than 12 arrays");
+        if (m >= kMax) throw Error(TSW_E_ARG, "linear combination of more than 16 arrays");
         if (on_halo) halo_terms |= 1u << m;
         a[m] = coef;
         x[m++] = arr;

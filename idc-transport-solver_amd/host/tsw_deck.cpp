@@ -299,9 +299,10 @@ void ComputeProjection(const Problem &prob, ScalarFlux &dest, ProblemEvalFcn f, 
 }
 
 // ---------------------------------------------------------------------------------------------------
-ProblemInputs::ProblemInputs(const DomainDecomposition &dd_, const OrdinateSet &os_, int degree)
-    : dd(dd_), os(os_), DG_degree(degree), sigma_t_density(dd_, 0), sigma_s_density(dd_, 0),
-      initial_density(dd_, degree), source_density(dd_, degree), sigma_t(dd_), sigma_s(dd_), psi(dd_, os_, degree),
+ProblemInputs::ProblemInputs(const DomainDecomposition &dd_, const OrdinateSet &os_, int degree, int sdeg)
+    : dd(dd_), os(os_), DG_degree(degree), sigma_t_density(dd_, sdeg), sigma_s_density(dd_, sdeg),
+      initial_density(dd_, degree), source_density(dd_, degree), sigma_t(dd_, sdeg, 0.0), sigma_s(dd_, sdeg, 0.0),
+      psi(dd_, os_, degree),
       source(dd_, os_, degree), final_time(std::numeric_limits<double>::quiet_NaN()),
       max_step_size(std::numeric_limits<double>::quiet_NaN()) {}
 
@@ -344,8 +345,7 @@ ProblemInputs Init(ParameterList &params, const std::string &directory) {
     for (const char *k : {"reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"})
         if (optional_bool(params, k, false))
             throw std::invalid_argument(std::string("reflecting boundaries are not available: ") + k);
-    if (!optional_bool(params, "piecewise_cross", false))
-        throw std::invalid_argument("piecewise_cross must be true: the sweeps take one cross-section value per cell");
+    const bool pwc_cross_sections = optional_bool(params, "piecewise_cross", false);
 
     DomainDecomposition dd;
     dd.nx = params.GetInt("nx");
@@ -367,7 +367,8 @@ ProblemInputs Init(ParameterList &params, const std::string &directory) {
     if (degree < 0) throw std::runtime_error("Failed to determine degree of DG spatial discretization.");
     const OrdinateSet os = OrdinateSet::FromParameters(params);
 
-    ProblemInputs in(dd, os, (int)degree);
+    const int sigma_degree = pwc_cross_sections ? 0 : (int)degree;  // init.cpp:741
+    ProblemInputs in(dd, os, (int)degree, sigma_degree);
     in.problem = prob->GetDescriptor();
     if (params.Has("tfinal")) in.final_time = params.GetDouble("tfinal");
     if (params.Has("cfl")) in.max_step_size = params.GetDouble("cfl") * dd.dx(0);
@@ -377,20 +378,23 @@ ProblemInputs Init(ParameterList &params, const std::string &directory) {
     std::snprintf(nxs, sizeof nxs, "%07" PRId64, dd.nx);
     const std::string dir = directory.empty() ? std::string(".") : directory;
     const std::string ps = std::to_string(degree);
+    const std::string ss = std::to_string(sigma_degree);
     projection_or_file(*prob, in.sigma_t_density, &Problem::TotalCross, files,
-                       dir + "/sigma_t_dg0_nx" + nxs + ".cs2");
+                       dir + "/sigma_t_dg" + ss + "_nx" + nxs + ".cs2");
     projection_or_file(*prob, in.sigma_s_density, &Problem::ScatterCross, files,
-                       dir + "/sigma_s_dg0_nx" + nxs + ".cs2");
+                       dir + "/sigma_s_dg" + ss + "_nx" + nxs + ".cs2");
     projection_or_file(*prob, in.initial_density, &Problem::InitialCondition, files,
                        dir + "/initial_dg" + ps + "_nx" + nxs + ".sd2");
     projection_or_file(*prob, in.source_density, &Problem::Source, files, dir + "/source_dg" + ps + "_nx" + nxs + ".sd2");
     prob->SetInflowBoundaries(in.source_density);
 
     for (int64_t i = 0; i <= dd.nx + 1; ++i)
-        for (int64_t j = 0; j <= dd.ny + 1; ++j) {
-            in.sigma_t(i, j) = in.sigma_t_density(i, j, 0, 0);
-            in.sigma_s(i, j) = in.sigma_s_density(i, j, 0, 0);
-        }
+        for (int64_t j = 0; j <= dd.ny + 1; ++j)
+            for (int u = 0; u <= sigma_degree; ++u)
+                for (int v = 0; v <= sigma_degree; ++v) {
+                    in.sigma_t(i, j, u, v) = in.sigma_t_density(i, j, u, v);
+                    in.sigma_s(i, j, u, v) = in.sigma_s_density(i, j, u, v);
+                }
     const double fourpi = 4.0 * M_PI;
     for (int64_t i = 0; i <= dd.nx + 1; ++i)
         for (int64_t j = 0; j <= dd.ny + 1; ++j)

@@ -85,7 +85,14 @@ double ScalarFlux::operator()(int64_t i, int64_t j, int64_t d, int64_t e) const 
 }
 
 CrossSection::CrossSection(const DomainDecomposition &dd, double value)
-    : ny_(dd.ny), v_((size_t)(dd.nx + 2) * (dd.ny + 2), value) {}
+    : DG_degree(0), ny_(dd.ny), v_((size_t)(dd.nx + 2) * (dd.ny + 2), value) {}
+
+CrossSection::CrossSection(const DomainDecomposition &dd, int degree, double mean)
+    : DG_degree(degree), ny_(dd.ny), v_((size_t)(dd.nx + 2) * (dd.ny + 2) * (degree + 1) * (degree + 1), 0.0) {
+    if (degree < 0) throw std::invalid_argument("negative cross-section DG degree");
+    for (int64_t i = 0; i <= dd.nx + 1; ++i)
+        for (int64_t j = 0; j <= dd.ny + 1; ++j) (*this)(i, j) = mean;
+}
 
 // ---------------------------------------------------------------------------------------------------
 Device::Device(const DomainDecomposition &dd, const OrdinateSet &os, int degree, int device) {
@@ -120,6 +127,12 @@ int precision_from(const ParameterList &p) {
     throw std::invalid_argument("sweep_precision must be fp64, fp32 or mixed, not '" + v + "'");
 }
 int all_cells(OpDomain d) { return ((int)d & (int)OpDomain::Boundary) ? 1 : 0; }
+// The device takes cross sections of the degree they carry (0 or the flux degree); both of a solve share it.
+void set_xs(Device &dev, const CrossSection &a, const CrossSection *b = nullptr) {
+    if (b && b->DG_degree != a.DG_degree)
+        throw std::invalid_argument("sigma_t and sigma_s must have the same DG degree");
+    dev.Check(tsw_set_xs_degree(dev.ctx(), a.DG_degree), "cross-section DG degree");
+}
 bool flag_from(const ParameterList &p, const std::string &key, bool dflt) {
     if (!p.Has(key)) return dflt;
     const std::string v = p.GetString(key);
@@ -146,6 +159,7 @@ SweepOperator::SweepOperator(std::shared_ptr<Device> dev, const ParameterList &p
 }
 
 void SweepOperator::Apply(OrdinateFlux &source, const CrossSection &sigma, double dt, const OrdinateFlux *) const {
+    set_xs(*dev_, sigma);
     dev_->Check(tsw_sweep_apply(dev_->ctx(), source.data(), sigma.data(), dt, precision_, TSW_MEM_HOST),
                 "SweepOperator::Apply");
 }
@@ -156,9 +170,11 @@ void Pmv(Device &dev, double alpha, double beta, const OrdinateFlux &x, ScalarFl
 }
 void Smv(Device &dev, double alpha, double beta, const CrossSection &sigma, const ScalarFlux &x, OrdinateFlux &y,
          OpDomain d) {
+    set_xs(dev, sigma);
     dev.Check(tsw_smv(dev.ctx(), alpha, beta, sigma.data(), x.data(), y.data(), all_cells(d), TSW_MEM_HOST), "Smv");
 }
 void Lmv(Device &dev, double alpha, double beta, const CrossSection &sigma, const OrdinateFlux &x, OrdinateFlux &y) {
+    set_xs(dev, sigma);
     dev.Check(tsw_lmv(dev.ctx(), alpha, beta, sigma.data(), x.data(), y.data(), TSW_MEM_HOST), "Lmv");
 }
 }  // namespace TransportOperator
@@ -189,6 +205,7 @@ void SISolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t, const Cr
                      const OrdinateFlux *) {
     int it = 0;
     double err = 0.0;
+    set_xs(*dev_, sigma_t, &sigma_s);
     dev_->Check(tsw_si_solve(dev_->ctx(), source.data(), sigma_t.data(), sigma_s.data(), dt, abs_tol_, (int)max_its_,
                              precision_, TSW_MEM_HOST, &it, &err),
                 "SISolver::Solve");
@@ -210,6 +227,7 @@ void PeierlsGMRESSolver::Solve(OrdinateFlux &source, const CrossSection &sigma_t
                                double dt, const OrdinateFlux *) {
     int it = 0;
     double res = 0.0;
+    set_xs(*dev_, sigma_t, &sigma_s);
     const int rc = tsw_gmres_solve(dev_->ctx(), source.data(), sigma_t.data(), sigma_s.data(), dt, abs_tol_,
                                    (int)max_its_, (int)restart_, precision_, TSW_MEM_HOST, nullptr, &it, &res);
     iterations_ = it;
@@ -238,6 +256,7 @@ IDCIntegrator::IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &p
 
 void IDCIntegrator::Step(OrdinateFlux &initial_condition, const OrdinateFlux &source, const CrossSection &sigma_t,
                          const CrossSection &sigma_s, double dt) {
+    set_xs(*dev_, sigma_t, &sigma_s);
     const int rc = tsw_idc_step(dev_->ctx(), initial_condition.data(), source.data(), sigma_t.data(), sigma_s.data(),
                                 dt, abs_tol_, (int)max_its_, precision_, TSW_MEM_HOST);
     if (rc == TSW_E_NOCONV) throw std::runtime_error("an implicit solve of the IDC step failed to converge");

@@ -157,16 +157,28 @@ class ScalarFlux {
     std::vector<double> density_;
 };
 
-// Cross section with DG degree 0: one value per cell, [i][j] with halo (CrossSection.hpp:397-478).
+// Cross section, [i][j][u][v] with halo (CrossSection.hpp:397-478): DG degree 0, one value per cell, or
+// the flux's DG degree p, (p+1)^2 Legendre coefficients per cell (init.cpp:741 with piecewise_cross
+// false).  The operators and solvers pass the degree to the device with tsw_set_xs_degree.
 class CrossSection {
   public:
     explicit CrossSection(const DomainDecomposition &dd, double value = 0.0);
-    double &operator()(int64_t i, int64_t j) { return v_[(size_t)i * (ny_ + 2) + j]; }
-    double operator()(int64_t i, int64_t j) const { return v_[(size_t)i * (ny_ + 2) + j]; }
+    // degree DG_degree, cell means `mean`, higher coefficients zero
+    CrossSection(const DomainDecomposition &dd, int DG_degree, double mean);
+    double &operator()(int64_t i, int64_t j) { return v_[idx(i, j, 0, 0)]; }  // the cell mean
+    double operator()(int64_t i, int64_t j) const { return v_[idx(i, j, 0, 0)]; }
+    double &operator()(int64_t i, int64_t j, int64_t u, int64_t v) { return v_[idx(i, j, u, v)]; }
+    double operator()(int64_t i, int64_t j, int64_t u, int64_t v) const { return v_[idx(i, j, u, v)]; }
     const double *data() const { return v_.data(); }
-    const int DG_degree = 0;
+    double *data() { return v_.data(); }
+    size_t size() const { return v_.size(); }
+    const int DG_degree;
 
   private:
+    size_t idx(int64_t i, int64_t j, int64_t u, int64_t v) const {
+        const int64_t P = DG_degree + 1;
+        return (size_t)(v + P * (u + P * (j + (ny_ + 2) * i)));
+    }
     int64_t ny_;
     std::vector<double> v_;
 };
@@ -320,13 +332,13 @@ void ComputeProjection(const Problem &prob, ScalarFlux &dest, ProblemEvalFcn f, 
 
 // Everything Init (init.cpp:422-946) builds from a deck, for a solve on the GPU.
 struct ProblemInputs {
-    ProblemInputs(const DomainDecomposition &dd, const OrdinateSet &os, int DG_degree);
+    ProblemInputs(const DomainDecomposition &dd, const OrdinateSet &os, int DG_degree, int sigma_degree = 0);
     DomainDecomposition dd;
     OrdinateSet os;
     int DG_degree;
-    ScalarFlux sigma_t_density, sigma_s_density;  // the DG degree 0 projections
+    ScalarFlux sigma_t_density, sigma_s_density;  // the cross-section projections (degree 0 or DG_degree)
     ScalarFlux initial_density, source_density;   // projected initial condition and source (+ inflow)
-    CrossSection sigma_t, sigma_s;                // the same cross sections, one value per cell
+    CrossSection sigma_t, sigma_s;                // the same cross sections
     OrdinateFlux psi;                             // initial_density / (4 pi) in every ordinate
     OrdinateFlux source;                          // source_density / (4 pi), halos included
     std::string problem;
@@ -337,10 +349,11 @@ struct ProblemInputs {
 // Builds the inputs of a deck as Init does: the problem (key "problem") overrides its domain; the mesh
 // from nx, ny, ax, bx, ay, by; the DG degree from dg_degree (or dg_degree_x, global.cpp:198-215); the
 // ordinate set from ordinate_type (default "chebyshev-legendre"), ang_order and ordinate_sym_reduce
-// (default false).  The sweeps take one cross-section value per cell, so piecewise_cross must be true
-// (std::invalid_argument otherwise, also for periodic or reflecting boundaries).  With output_init true
-// each projection is read from, or else computed and written to, the file the reference names
-// (sigma_t_dg0_nx0000224.cs2, sigma_s_dg0_..., initial_dg<p>_..., source_dg<p>_....sd2) in `directory`.
+// (default false).  The cross sections are projected at DG degree 0 when piecewise_cross is true and at
+// the flux's degree otherwise (init.cpp:741).  Periodic or reflecting boundaries throw
+// std::invalid_argument.  With output_init true each projection is read from, or else computed and
+// written to, the file the reference names (sigma_t_dg<0 or p>_nx0000224.cs2, sigma_s_dg..., 
+// initial_dg<p>_..., source_dg<p>_....sd2) in `directory`.
 ProblemInputs Init(ParameterList &params, const std::string &directory = ".");
 
 }  // namespace tsw_host

@@ -78,6 +78,7 @@ def lib():
         L.tsw_problem_set.argtypes = [c, i, i, d, d, d, d, i, i, _dp, _dp, _dp, _ip]
         L.tsw_set_pattern.argtypes = [c, ctypes.c_char_p]
         L.tsw_set_tuning.argtypes = [c, ctypes.c_char_p, i]
+        L.tsw_set_xs_degree.argtypes = [c, i]
         L.tsw_get_tuning.argtypes = [c, ctypes.c_char_p, _ip]
         L.tsw_sweep_apply.argtypes = [c, _vp, _vp, d, i, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
@@ -187,6 +188,7 @@ class Context:
             raise TswError(rc, f"tsw_create({device}) failed (no HIP device?)")
         self._h = h
         self.nx = self.ny = self.p = self.nq = None
+        self.xs_degree = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -212,7 +214,8 @@ class Context:
             return None
         cells = (self.nx + 2) * (self.ny + 2)
         N = (self.p + 1) ** 2
-        return {"psi": cells * self.nq * N, "phi": cells * N, "cell": cells}
+        return {"psi": cells * self.nq * N, "phi": cells * N, "cell": cells,
+                "xs": cells * (N if self.xs_degree else 1)}
 
     def _expect(self, **arrays):
         """Refuses arrays whose element count does not match the problem (device arrays are passed by
@@ -237,6 +240,12 @@ class Context:
                                             eta.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
                                             off.ctypes.data_as(_ip)))
         self.nx, self.ny, self.p, self.nq = nx, ny, p, xi.size
+        self.xs_degree = 0
+
+    def set_xs_degree(self, degree):
+        """0: cross sections are cell means [i][j]; p: Legendre coefficients [i][j][u][v] of degree p."""
+        self._check(self._L.tsw_set_xs_degree(self._h, int(degree)))
+        self.xs_degree = int(degree)
 
     def set_pattern(self, name):
         self._check(self._L.tsw_set_pattern(self._h, name.encode()))
@@ -250,7 +259,7 @@ class Context:
         return v.value
 
     def sweep(self, psi, sigma_t, dt=math.inf, precision=FP64):
-        self._expect(psi=psi, cell=sigma_t)
+        self._expect(psi=psi, xs=sigma_t)
         (pp, sp), mem = _addrs(psi, sigma_t)
         self._check(self._L.tsw_sweep_apply(self._h, pp, sp, dt, precision, mem))
         return psi
@@ -262,19 +271,19 @@ class Context:
         return phi
 
     def smv(self, sigma_s, phi, psi, alpha=1.0, beta=1.0, all_cells=False):
-        self._expect(cell=sigma_s, phi=phi, psi=psi)
+        self._expect(xs=sigma_s, phi=phi, psi=psi)
         (a, b, c), mem = _addrs(sigma_s, phi, psi)
         self._check(self._L.tsw_smv(self._h, alpha, beta, a, b, c, int(all_cells), mem))
         return psi
 
     def lmv(self, sigma_t, x, y, alpha=1.0, beta=0.0):
-        self._expect(cell=sigma_t, psi=x, psi2=y)
+        self._expect(xs=sigma_t, psi=x, psi2=y)
         (a, b, c), mem = _addrs(sigma_t, x, y)
         self._check(self._L.tsw_lmv(self._h, alpha, beta, a, b, c, mem))
         return y
 
     def si_solve(self, source, sigma_t, sigma_s, dt=math.inf, abs_tol=1e-8, max_its=500, precision=FP64):
-        self._expect(psi=source, cell=sigma_t, cell2=sigma_s)
+        self._expect(psi=source, xs=sigma_t, xs2=sigma_s)
         (a, b, c), mem = _addrs(source, sigma_t, sigma_s)
         it, err = ctypes.c_int(0), ctypes.c_double(0.0)
         self._check(self._L.tsw_si_solve(self._h, a, b, c, dt, abs_tol, max_its, precision, mem, ctypes.byref(it),
@@ -283,7 +292,7 @@ class Context:
 
     def gmres_solve(self, source, sigma_t, sigma_s, phi_out=None, dt=math.inf, abs_tol=1e-8, max_its=3000,
                     restart=30, precision=FP64):
-        self._expect(psi=source, cell=sigma_t, cell2=sigma_s, phi=phi_out)
+        self._expect(psi=source, xs=sigma_t, xs2=sigma_s, phi=phi_out)
         (a, b, c, d), mem = _addrs(source, sigma_t, sigma_s, phi_out)
         it, res = ctypes.c_int(0), ctypes.c_double(0.0)
         self._check(self._L.tsw_gmres_solve(self._h, a, b, c, dt, abs_tol, max_its, restart, precision, mem, d,
@@ -292,7 +301,7 @@ class Context:
 
     def sweep_vjp(self, seed, grad_src, sigma_t, psi=None, grad_sigma_t=None, dt=math.inf, precision=FP64):
         """grad_src <- d<seed, psi>/d(source, inflow); grad_sigma_t <- d/dsigma_t (needs the forward psi)."""
-        self._expect(psi=seed, psi2=grad_src, psi3=psi, cell=sigma_t, cell2=grad_sigma_t)
+        self._expect(psi=seed, psi2=grad_src, psi3=psi, xs=sigma_t, cell=grad_sigma_t)
         (a, b, c, d, e), mem = _addrs(psi, sigma_t, seed, grad_src, grad_sigma_t)
         self._check(self._L.tsw_sweep_vjp(self._h, a, b, dt, c, d, e, precision, mem))
         return grad_src
@@ -300,7 +309,7 @@ class Context:
     def solve_vjp(self, seed, grad_src, sigma_t, sigma_s, psi=None, grad_sigma_t=None, grad_sigma_s=None,
                   dt=math.inf, abs_tol=1e-10, max_its=3000, restart=30, precision=FP64):
         """Adjoint of the scattering solve; returns (iterations, residual) of the adjoint GMRES."""
-        self._expect(psi=seed, psi2=grad_src, psi3=psi, cell=sigma_t, cell2=sigma_s, cell3=grad_sigma_t,
+        self._expect(psi=seed, psi2=grad_src, psi3=psi, xs=sigma_t, xs2=sigma_s, cell3=grad_sigma_t,
                      cell4=grad_sigma_s)
         (a, b, c, d, e, f, h), mem = _addrs(psi, sigma_t, sigma_s, seed, grad_src, grad_sigma_t, grad_sigma_s)
         it, res = ctypes.c_int(0), ctypes.c_double(0.0)
@@ -313,13 +322,13 @@ class Context:
 
     def idc_step(self, psi, source, sigma_t, sigma_s, dt, abs_tol=1e-12, max_its=3000, precision=FP64):
         """psi^n -> psi^{n+1} in place (zero halos); source holds Q and the inflow."""
-        self._expect(psi=psi, psi2=source, cell=sigma_t, cell2=sigma_s)
+        self._expect(psi=psi, psi2=source, xs=sigma_t, xs2=sigma_s)
         (a, b, c, d), mem = _addrs(psi, source, sigma_t, sigma_s)
         self._check(self._L.tsw_idc_step(self._h, a, b, c, d, dt, abs_tol, max_its, precision, mem))
         return psi
 
     def idc_run(self, psi, source, sigma_t, sigma_s, dt, nsteps, abs_tol=1e-12, max_its=3000, precision=FP64):
-        self._expect(psi=psi, psi2=source, cell=sigma_t, cell2=sigma_s)
+        self._expect(psi=psi, psi2=source, xs=sigma_t, xs2=sigma_s)
         (a, b, c, d), mem = _addrs(psi, source, sigma_t, sigma_s)
         self._check(self._L.tsw_idc_run(self._h, a, b, c, d, dt, nsteps, abs_tol, max_its, precision, mem))
         return psi
@@ -327,7 +336,7 @@ class Context:
     def idc_adjoint(self, seed, grad_psi0, source, sigma_t, sigma_s, dt, nsteps, psi0=None, checkpoint_every=10,
                     grad_sigma_t=None, grad_sigma_s=None, abs_tol=1e-12, max_its=3000, precision=FP64):
         """grad_psi0 <- dJ/dpsi^0 of J = <seed, psi^nsteps>; optional cross-section gradients (need psi0)."""
-        self._expect(psi=seed, psi2=grad_psi0, psi3=source, psi4=psi0, cell=sigma_t, cell2=sigma_s,
+        self._expect(psi=seed, psi2=grad_psi0, psi3=source, psi4=psi0, xs=sigma_t, xs2=sigma_s,
                      cell3=grad_sigma_t, cell4=grad_sigma_s)
         (a, b, c, d, e, f, h, k), mem = _addrs(psi0, source, sigma_t, sigma_s, seed, grad_psi0, grad_sigma_t,
                                                grad_sigma_s)

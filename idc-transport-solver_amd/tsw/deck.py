@@ -4,9 +4,9 @@ condition projected into the DG space, inflow halos, and the named ordinate set.
 
 Problems: "lattice" (LatticeProblem.cpp:100-324), "hohlraum" (HohlraumProblem.cpp:60-272),
 "hohlraum2" (Hohlraum2Problem.cpp:14-148), "bump" (BumpProblem.cpp) and "inhomogeneous-sphere"
-(InhomogeneousSphereProblem.cpp).  The GPU sweeps take one cross-section value per cell, so
-piecewise_cross must be true; periodic and reflecting boundaries and mollified problem functions are
-not available (ValueError)."""
+(InhomogeneousSphereProblem.cpp).  Cross sections are projected at DG degree 0 when piecewise_cross is
+true and at the flux's degree otherwise (init.cpp:741); periodic and reflecting boundaries and mollified
+problem functions are not available (ValueError)."""
 from __future__ import annotations
 
 import math
@@ -255,8 +255,7 @@ def init(params):
     for k in ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"):
         if _bool(params, k, False):
             raise ValueError(f"reflecting boundaries are not available: {k}")
-    if not _bool(params, "piecewise_cross", False):
-        raise ValueError("piecewise_cross must be true: the sweeps take one cross-section value per cell")
+    sdeg_pwc = _bool(params, "piecewise_cross", False)
     mesh = Mesh(int(params["nx"]), int(params["ny"]), float(params["ax"]), float(params["bx"]),
                 float(params["ay"]), float(params["by"]))
     p = int(params.get("dg_degree_x", params.get("dg_degree", -1)))
@@ -267,20 +266,22 @@ def init(params):
     reduce = _bool(params, "ordinate_sym_reduce", False)
     xi, eta, w, off = ordinates(otype, order, reduce)
 
-    st = project(prob.total, mesh, 0)
-    ss = project(prob.scatter, mesh, 0)
+    sdeg = 0 if sdeg_pwc else p   # init.cpp:741
+    st = project(prob.total, mesh, sdeg)
+    ss = project(prob.scatter, mesh, sdeg)
     q0 = project(prob.source, mesh, p)
     ic = project(prob.initial, mesh, p)
     q = q0.copy()
     prob.inflow(q)
     P = Problem(name, mesh.nx, mesh.ny, p, mesh.ax, mesh.bx, mesh.ay, mesh.by, xi, eta, w, off,
-                st[:, :, 0, 0].copy(), ss[:, :, 0, 0].copy(), q[:, :, 0, 0].copy(), q_dg=q)
+                st[:, :, 0, 0].copy(), ss[:, :, 0, 0].copy(), q[:, :, 0, 0].copy(), q_dg=q,
+                sigma_t_dg=st if sdeg else None, sigma_s_dg=ss if sdeg else None)
     psi0 = np.zeros((mesh.nx + 2, mesh.ny + 2, xi.size, p + 1, p + 1))
     psi0[1:-1, 1:-1] = ic[1:-1, 1:-1, None] / (4.0 * math.pi)
     nxs = f"{mesh.nx:07d}"
     extras = {
         "psi0": psi0,
-        f"sigma_t_dg0_nx{nxs}.cs2": st, f"sigma_s_dg0_nx{nxs}.cs2": ss,
+        f"sigma_t_dg{sdeg}_nx{nxs}.cs2": st, f"sigma_s_dg{sdeg}_nx{nxs}.cs2": ss,
         f"initial_dg{p}_nx{nxs}.sd2": ic, f"source_dg{p}_nx{nxs}.sd2": q0,
         "ordinate_info": (otype, order, reduce), "mesh": mesh,
         "final_time": float(params["tfinal"]) if "tfinal" in params else math.nan,

@@ -34,6 +34,20 @@ class Problem:
     sigma_s: np.ndarray   # [nx+2][ny+2]
     q_cell: np.ndarray    # [nx+2][ny+2] isotropic source density per cell (halos: isotropic inflow density)
     q_dg: np.ndarray = None  # optional [nx+2][ny+2][p+1][p+1] DG source density; replaces q_cell
+    # optional degree-p cross sections [nx+2][ny+2][p+1][p+1] (sigma_t / sigma_s hold their cell means)
+    sigma_t_dg: np.ndarray = None
+    sigma_s_dg: np.ndarray = None
+
+    @property
+    def xs_degree(self):
+        """DG degree of the cross sections the solver is given: p with sigma_t_dg, else 0."""
+        return self.p if self.sigma_t_dg is not None else 0
+
+    def cross_sections(self):
+        """(sigma_t, sigma_s) in the form tsw takes at xs_degree: coefficient arrays or cell means."""
+        if self.sigma_t_dg is not None:
+            return self.sigma_t_dg, self.sigma_s_dg
+        return self.sigma_t, self.sigma_s
 
     @property
     def nq(self):
@@ -68,6 +82,8 @@ class Problem:
     def set_on(self, ctx):
         ctx.problem_set(self.nx, self.ny, self.p, self.xi, self.eta, self.w, self.quad_off, self.ax, self.bx,
                         self.ay, self.by)
+        if self.xs_degree:
+            ctx.set_xs_degree(self.xs_degree)
 
 
 def _centres(n, a, b):
@@ -150,3 +166,52 @@ def as_batch_member(P: Problem):
     """The dict tsw.Context.batch_set expects for one problem."""
     return {"nx": P.nx, "ny": P.ny, "p": P.p, "xi": P.xi, "eta": P.eta, "w": P.w, "quad_off": P.quad_off,
             "ax": P.ax, "bx": P.bx, "ay": P.ay, "by": P.by}
+
+
+def line_source(n=200, p=2, order=16, width=0.03):
+    """Line-source pulse on [-1.5, 1.5]^2 with n x n cells, Q_p, level-symmetric S_order, sigma_s = 1,
+    sigma_a = 0, no source, vacuum boundaries (the reference has no line-source problem; its nearest is
+    the bump, BumpProblem.cpp:107-128).  Returns (problem, psi0): the initial condition is an isotropic
+    Gaussian exp(-r^2 / (2 width^2)) / (2 pi width^2) of unit mass, divided by 4 pi per ordinate and
+    projected exactly per cell (separable 12-point Gauss-Legendre quadrature of each 1D factor)."""
+    from numpy.polynomial import legendre as L
+    xi, eta, w, off = level_symmetric(order)
+    a, b = -1.5, 1.5
+    h = (b - a) / n
+    xg, wg = L.leggauss(12)
+    P = p + 1
+    leg = np.array([L.legval(xg, np.eye(P)[d]) for d in range(P)])  # [d][node]
+    centres = a + (np.arange(n) + 0.5) * h
+    X = centres[:, None] + 0.5 * h * xg[None, :]                    # [cell][node]
+    gx = np.exp(-X ** 2 / (2 * width ** 2)) / math.sqrt(2 * math.pi * width ** 2)
+    cx = np.einsum("cn,dn,n->cd", gx, leg, wg) * (2 * np.arange(P) + 1)[None, :] / 2.0
+    coef = np.einsum("id,je->ijde", cx, cx) / (4 * math.pi)
+    nq = xi.size
+    psi0 = np.zeros((n + 2, n + 2, nq, P, P))
+    psi0[1:-1, 1:-1] = coef[:, :, None]
+    st = np.ones((n + 2, n + 2))
+    ss = np.ones((n + 2, n + 2))
+    q = np.zeros((n + 2, n + 2))
+    prob = Problem("line-source", n, n, p, a, b, a, b, xi, eta, w, off, st, ss, q)
+    return prob, psi0
+
+
+def ring_seed(P: Problem, r0=0.8, r1=1.0, sub=8):
+    """Seed of J = integral of phi over the ring r0 <= r <= r1 about the origin, in the psi layout:
+    w_q * int_{ring and cell} P_d P_e dx dy by sub x sub midpoint sub-cells per cell (cut cells resolved
+    to that quadrature), so that J = <seed, psi> over the interior coefficients."""
+    from numpy.polynomial import legendre as Lg
+    n, p, a = P.nx, P.p, P.ax
+    h = (P.bx - P.ax) / n
+    Pd = p + 1
+    t = (np.arange(sub) + 0.5) / sub * 2 - 1
+    leg = np.array([Lg.legval(t, np.eye(Pd)[d]) for d in range(Pd)])
+    centres = a + (np.arange(n) + 0.5) * h
+    X = centres[:, None, None, None] + 0.5 * h * t[None, None, :, None]
+    Y = centres[None, :, None, None] + 0.5 * h * t[None, None, None, :]
+    R = np.sqrt(X ** 2 + Y ** 2)
+    ind = ((R >= r0) & (R <= r1)).astype(float) * (h / sub) ** 2
+    M = np.einsum("ijuv,du,ev->ijde", ind, leg, leg)
+    seed = np.zeros((n + 2, n + 2, P.nq, Pd, Pd))
+    seed[1:-1, 1:-1] = P.w[None, None, :, None, None] * M[:, :, None]
+    return seed

@@ -9,7 +9,8 @@
  *   - arrays use the reference host layouts (1-cell halo on each side, 1-based interior):
  *       psi   [i][j][q][d][e]  (nx+2)*(ny+2)*nq*(p+1)^2 doubles  (OrdinateFlux.hpp:639-700)
  *       phi   [i][j][d][e]     (nx+2)*(ny+2)*(p+1)^2 doubles     (DensityFunction.hpp:475-535)
- *       sigma [i][j]           (nx+2)*(ny+2) doubles, cell means   (CrossSection.hpp:397-478, DG degree 0)
+ *       sigma [i][j]           (nx+2)*(ny+2) doubles, cell means   (CrossSection.hpp:397-478, DG degree 0),
+ *             or [i][j][u][v] Legendre coefficients of DG degree p after tsw_set_xs_degree(ctx, p)
  *   - `mem` says where the caller's arrays live: TSW_MEM_HOST (pageable or pinned host memory)
  *     or TSW_MEM_DEVICE (device pointers on the context's device, same layouts);
  *   - all buffers are caller-owned; the context owns only its scratch and solver state;
@@ -74,6 +75,18 @@ int tsw_problem_set(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double a
  * Unknown names fail with TSW_E_ARG as the reference's factory throws std::out_of_range. */
 int tsw_set_pattern(tsw_ctx *ctx, const char *name);
 
+/* Degree of the cross sections that the following calls on ctx take (sigma_t, sigma_s): 0 (default, and
+ * after every tsw_problem_set) cell means [i][j]; the problem's DG degree p, Legendre coefficients
+ * [i][j][u][v], (p+1)^2 doubles per cell, the reference's CrossSection of degree p (init.cpp:741 with
+ * piecewise_cross false).  They enter through the tensor of Legendre triple products (ComputeTPI
+ * global.cpp:67-107, ComputeTensor CrossSection.cpp:165-189): the element matrices gain
+ * dx dy/4 (2d+1)(2e+1) sigma(i,j,d,e,u,v) (CalcA, SweepLinearSolveSpec.cpp:300-321), Smv and Lmv project
+ * sigma times the flux (TransportOperator.cpp:375-381, :1587-1596), and DSA takes the cell means.  Sweeps
+ * and solves then run on the wavefront kernels with per-cell elimination (any pattern).  Not built:
+ * tsw_sweep_vjp, tsw_solve_vjp and tsw_idc_adjoint fail with TSW_E_UNSUPPORTED; tsw_batch_* always take
+ * cell means.  Other degrees fail with TSW_E_ARG. */
+int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
+
 /* Tuning of the staggered sweep.  key "band_rows": rows of an ordinate's mesh handled by one workgroup;
  * a sweep then runs ceil(ny / band_rows) pipelined workgroups per ordinate (the KBA decomposition of
  * SweepPatternUniCartKBA2.cpp:212-453 applied to rows within the device).  0 (default) chooses
@@ -98,6 +111,9 @@ int tsw_set_pattern(tsw_ctx *ctx, const char *name);
  * the loop test on the device (same iterates and counts as the eager loop).  "refine_steps" (1..8,
  * default 1): FP64 refinement steps of a TSW_MIXED sweep.  "gmres_refine" (0 default, 1): mixed-precision
  * GMRES by iterative refinement instead of the relaxed strategy.
+ * Test key "stall_test" (1): the next banded staggered sweep starts with the row-band stall flag raised,
+ * so its bounded waits give up at once and the call fails with TSW_E_HIP as a real stall would (the
+ * exchange records are refilled and later sweeps are unaffected).
  * Unknown keys fail with TSW_E_ARG. */
 int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
@@ -122,7 +138,9 @@ int tsw_smv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_s, cons
             int all_cells, int mem);
 
 /* y <- beta*y + alpha*(Omega.grad + sigma_t) x on interior cells; x's halos give the inflow.
- * TransportOperator::Lmv (TransportOperator.cpp:1215-1683), x != y. */
+ * TransportOperator::Lmv (TransportOperator.cpp:1215-1683); x == y is the in-place product
+ * (Lmv_InPlace, :602-1200, dispatched at :1224-1228): every cell uses the values of y from before the
+ * call. */
 int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, const double *x, double *y, int mem);
 
 /* Source iteration, ImplicitSolver::Solve under solve_type "source-iteration" (SISolver.cpp:190-254):
@@ -139,10 +157,12 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
  * absolute residual test ||r||_2 <= abs_tol (:139-143).  phi_out (may be NULL): converged scalar
  * flux, phi layout.  precision TSW_FP64: FP64 sweeps.  TSW_FP32 (staggered patterns): relaxed inexact
  * GMRES, FP64 Krylov vectors and DSA with FP32 sweeps in the Arnoldi steps taken once the residual
- * estimate is <= abs_tol / 1e-5 (FP64 sweeps for the right-hand side, earlier steps, restarts and the
- * final psi); with tuning key "gmres_refine" = 1, and on the wavefront patterns, FP32 sweeps inside an
- * FP64 iterative refinement on the angular residual r = Q + S P psi - L psi (FP64 Lmv), repeated until
- * ||P r||_2 <= abs_tol.  Returns TSW_E_NOCONV if max_its is reached. */
+ * estimate is <= abs_tol / (10 d), d the relative error of an FP32-sweep product measured once per cross
+ * sections, dt and DSA setting (FP64 sweeps for the right-hand side, earlier steps, restarts and the
+ * final psi); the final FP64 sweep also gives the true residual, which *resid reports, and the solve
+ * resumes while it exceeds abs_tol.  With tuning key "gmres_refine" = 1, and on the wavefront patterns,
+ * FP32 sweeps inside an FP64 iterative refinement on the angular residual r = Q + S P psi - L psi (FP64
+ * Lmv), repeated until ||P r||_2 <= abs_tol.  Returns TSW_E_NOCONV if max_its is reached. */
 int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
                     double abs_tol, int max_its, int restart, int precision, int mem, double *phi_out,
                     int *iters, double *resid);
@@ -185,8 +205,8 @@ int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type
  * residuals (L - S P) psi with the nodes' integration matrix (ComputeSource :1686-1740), result at the
  * last node or by collocation.  psi's halos are taken as zero (as the reference assumes) and returned
  * zero; source holds Q with the inflow in its halos, constant over the step.  Each implicit solve is the
- * Peierls GMRES of tsw_gmres_solve (restart 30, abs_tol, max_its, precision: FP32 sweeps inside FP64
- * refinement for TSW_FP32); residuals and correction sums are FP64.  TSW_E_NOCONV if a solve fails. */
+ * Peierls GMRES of tsw_gmres_solve (restart 30, abs_tol, max_its, precision: FP32 sweeps with FP64 Krylov
+ * vectors for TSW_FP32); residuals and correction sums are FP64.  TSW_E_NOCONV if a solve fails. */
 int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const double *sigma_t,
                  const double *sigma_s, double dt, double abs_tol, int max_its, int precision, int mem);
 

@@ -34,6 +34,104 @@ static inline size_t phi_idx(int ny, int p, int i, int j, int d, int e)
 }
 
 /* ---------------------------------------------------------------------------------------------- */
+/* Cross sections of DG degree p (init.cpp:741 with piecewise_cross false).  While the degree set   */
+/* by orc_set_xs_degree is > 0, every sigma argument below is a [i][j][u][v] coefficient array and    */
+/* enters through the tensor of Legendre triple products (CrossSection::ComputeTensor,              */
+/* CrossSection.cpp:165-189).                                                                        */
+/* ---------------------------------------------------------------------------------------------- */
+static int g_xs_deg = 0;
+
+void orc_set_xs_degree(int degree) { g_xs_deg = degree; }
+
+/* Legendre P_n(x) by the three-term recurrence, and Gauss-Legendre nodes/weights by Newton from the
+ * Chebyshev-like guess (own quadrature; the reference reads tabulated rules, Tabulated.cpp:29). */
+static double legendre(int n, double x)
+{
+    double p0 = 1.0, p1 = x;
+    if (n == 0) return 1.0;
+    for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+static void gauss_legendre(int m, double *x, double *w)
+{
+    for (int k = 0; k < m; ++k) {
+        double t = -cos(ORC_PI * (k + 0.75) / (m + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            const double pm = legendre(m, t), pm1 = legendre(m - 1, t);
+            dp = m * (pm1 - t * pm) / (1.0 - t * t);
+            const double dt = pm / dp;
+            t -= dt;
+            if (fabs(dt) < 1e-16) break;
+        }
+        const double pm = legendre(m, t), pm1 = legendre(m - 1, t);
+        dp = m * (pm1 - t * pm) / (1.0 - t * t);
+        x[k] = t;
+        w[k] = 2.0 / ((1.0 - t * t) * dp * dp);
+    }
+}
+
+/* TPI[a + P(b + P c)] = int_{-1}^{1} P_a P_b P_c, by max(2p, 2) Gauss points, entries below 5e-16 set
+ * to zero (ComputeTPI, global.cpp:67-107; ITPI global.hpp:644). */
+static void tpi_table(int p, double *tpi)
+{
+    const int P = p + 1, m = 2 * p > 2 ? 2 * p : 2;
+    double xg[16], wg[16];
+    gauss_legendre(m, xg, wg);
+    for (int a = 0; a <= p; ++a)
+        for (int b = 0; b <= p; ++b)
+            for (int c = 0; c <= p; ++c) {
+                double v = 0.0;
+                for (int i = 0; i < m; ++i) v += wg[i] * legendre(a, xg[i]) * legendre(b, xg[i]) * legendre(c, xg[i]);
+                if (fabs(v) < 5e-16) v = 0.0;
+                tpi[a + P * (b + P * c)] = v;
+            }
+}
+
+/* Tensor of a degree-p cross section, t[cell*N*N + (d*P+e)*N + a*P+b] = sigma(i,j,a,b,d,e) =
+ * sum_{u,v} sigma(i,j,u,v) TPI(a,d,u) TPI(b,e,v) (CrossSection.cpp:179-188, index map TensorIndex
+ * CrossSection.hpp:397-478).  Caller frees. */
+static double *xs_tensor(int nx, int ny, int p, const double *sig)
+{
+    const int P = p + 1, N = P * P;
+    const size_t ncell = (size_t)(nx + 2) * (ny + 2);
+    double tpi[64];
+    tpi_table(p, tpi);
+    double *t = (double *)calloc(ncell * N * N, sizeof(double));
+    for (size_t c = 0; c < ncell; ++c)
+        for (int a = 0; a <= p; ++a)
+            for (int b = 0; b <= p; ++b)
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) {
+                        double *out = &t[c * N * N + (size_t)(d * P + e) * N + a * P + b];
+                        for (int u = 0; u <= p; ++u)
+                            for (int v = 0; v <= p; ++v)
+                                *out += sig[c * N + u * P + v] * tpi[a + P * (d + P * u)] * tpi[b + P * (e + P * v)];
+                    }
+    return t;
+}
+
+/* sigma(i,j,d,e,u,v) of the tensor above (CrossSection::operator(), CrossSection.hpp:481-535) */
+static inline double xs_at(const double *t, int ny, int p, int i, int j, int d, int e, int u, int v)
+{
+    const int P = p + 1, N = P * P;
+    const size_t c = (size_t)j + (size_t)(ny + 2) * i;
+    return t[c * N * N + (size_t)(u * P + v) * N + d * P + e];
+}
+
+void orc_xs_tensor(int nx, int ny, int p, const double *sigma, double *tensor)
+{
+    const int N = (p + 1) * (p + 1);
+    double *t = xs_tensor(nx, ny, p, sigma);
+    memcpy(tensor, t, sizeof(double) * (size_t)(nx + 2) * (ny + 2) * N * N);
+    free(t);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
 /* Angular matrices.  SweepLinearSolveSpec.cpp:98-158 (loop over quadrants, lanes, d, e).          */
 /* ---------------------------------------------------------------------------------------------- */
 void orc_aq(int p, int nq, const double *xi, const double *eta, double dx, double dy, double *Aq)
@@ -91,8 +189,8 @@ void orc_ge_solve(int N, double *A, double *b)
 /* One cell, one ordinate: CalcB (SweepLinearSolveSpec.cpp:473-598), CalcA (:284-298),
  * SolveSystem, StoreResult (SweepCellSolve.cpp:86-99). */
 static void cell_solve(int nx, int ny, int p, int nq, const double *xi, const double *eta, double dx,
-                       double dy, const double *Aq, const double *sigma_t, double inv_dt, int q, int i,
-                       int j, double *psi, double *A, double *B, double *W0, double *W1)
+                       double dy, const double *Aq, const double *sigma_t, const double *tensor, double inv_dt,
+                       int q, int i, int j, double *psi, double *A, double *B, double *W0, double *W1)
 {
     (void)nx;
     const int P = p + 1, N = P * P;
@@ -121,9 +219,21 @@ static void cell_solve(int nx, int ny, int p, int nq, const double *xi, const do
             }
 
     memcpy(A, Aq + (size_t)q * N * N, sizeof(double) * N * N);
-    const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
-    const double coeff = (sig + inv_dt) * dx * dy;
-    for (int r = 0; r < N; ++r) A[r + N * r] += coeff;
+    if (!tensor) {
+        const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
+        const double coeff = (sig + inv_dt) * dx * dy;
+        for (int r = 0; r < N; ++r) A[r + N * r] += coeff;
+    } else { /* general sigma, :300-321 (1/dt = 0 for steady state) */
+        const double dt_coeff = dx * dy * inv_dt, dx_coeff = dx * dy / 4.0;
+        for (int d = 0; d <= p; ++d)
+            for (int e = 0; e <= p; ++e) {
+                A[(e + P * d) * (N + 1)] += dt_coeff;
+                for (int u = 0; u <= p; ++u)
+                    for (int v = 0; v <= p; ++v)
+                        A[(e + P * d) + N * (v + P * u)] += dx_coeff * (2 * d + 1) * (2 * e + 1) *
+                                                            xs_at(tensor, ny, p, i, j, d, e, u, v);
+            }
+    }
 
     orc_ge_solve(N, A, B);
 
@@ -146,6 +256,7 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
     double *W1 = (double *)malloc(sizeof(double) * N);
     orc_aq(p, nq, xi, eta, dx, dy, Aq);
     const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+    double *tensor = g_xs_deg > 0 ? xs_tensor(nx, ny, p, sigma_t) : NULL;
 
     const int num_diags = nx + ny - 1;
     for (int diag = 1; diag <= num_diags; ++diag) {
@@ -160,13 +271,13 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
                     int j = diag - i + 1;
                     if (quad >= 2) i = nx - i + 1;      /* xi < 0 */
                     if (quad & 1) j = ny - j + 1;       /* eta < 0 */
-                    cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, inv_dt, q, i, j, psi, A, B, W0,
-                               W1);
+                    cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, tensor, inv_dt, q, i, j, psi, A, B,
+                               W0, W1);
                 }
             }
         }
     }
-    free(Aq); free(A); free(B); free(W0); free(W1);
+    free(Aq); free(A); free(B); free(W0); free(W1); free(tensor);
 }
 
 /* ---------------------------------------------------------------------------------------------- */
@@ -182,6 +293,7 @@ void orc_sweep_omp(int nx, int ny, int p, int nq, const double *xi, const double
     double *Aq = (double *)malloc(sizeof(double) * (size_t)nq * N * N);
     orc_aq(p, nq, xi, eta, dx, dy, Aq);
     const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+    double *tensor = g_xs_deg > 0 ? xs_tensor(nx, ny, p, sigma_t) : NULL;
     /* quadrant of each ordinate */
     int *quad = (int *)malloc(sizeof(int) * (size_t)nq);
     for (int k = 0; k < 4; ++k)
@@ -201,12 +313,14 @@ void orc_sweep_omp(int nx, int ny, int p, int nq, const double *xi, const double
                 int j = diag - i + 1;
                 if (quad[q] >= 2) i = nx - i + 1;
                 if (quad[q] & 1) j = ny - j + 1;
-                cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, inv_dt, q, i, j, psi, A, B, W0, W1);
+                cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, tensor, inv_dt, q, i, j, psi, A, B, W0,
+                           W1);
             }   /* implicit barrier: the diagonal is complete */
         }
     }
     free(Aq);
     free(quad);
+    free(tensor);
 }
 
 /* ---------------------------------------------------------------------------------------------- */
@@ -310,12 +424,25 @@ void orc_smv(int nx, int ny, int p, int nq, double alpha, double beta, const dou
     const int o = all_cells ? 1 : 0;
     const int P = p + 1;
     double val[16];
+    double *tensor = g_xs_deg > 0 ? xs_tensor(nx, ny, p, sigma_s) : NULL;
     for (int i = 1 - o; i <= nx + o; ++i)
         for (int j = 1 - o; j <= ny + o; ++j) {
-            const double s = sigma_s[(size_t)j + (size_t)(ny + 2) * i];
-            for (int d = 0; d <= p; ++d)
-                for (int e = 0; e <= p; ++e)
-                    val[e + P * d] = alpha * (s * phi[phi_idx(ny, p, i, j, d, e)] / (4.0 * ORC_PI));
+            if (!tensor) {
+                const double s = sigma_s[(size_t)j + (size_t)(ny + 2) * i];
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e)
+                        val[e + P * d] = alpha * (s * phi[phi_idx(ny, p, i, j, d, e)] / (4.0 * ORC_PI));
+            } else { /* general sigma, :375-381 */
+                for (int d = 0; d <= p; ++d)
+                    for (int e = 0; e <= p; ++e) {
+                        double v = 0.0;
+                        for (int u = 0; u <= p; ++u)
+                            for (int w = 0; w <= p; ++w)
+                                v += xs_at(tensor, ny, p, i, j, d, e, u, w) * phi[phi_idx(ny, p, i, j, u, w)];
+                        v *= (2 * d + 1) * (2 * e + 1) / (16.0 * ORC_PI);
+                        val[e + P * d] = v * alpha;
+                    }
+            }
             for (int q = 0; q < nq; ++q)
                 for (int d = 0; d <= p; ++d)
                     for (int e = 0; e <= p; ++e) {
@@ -323,6 +450,7 @@ void orc_smv(int nx, int ny, int p, int nq, double alpha, double beta, const dou
                         *y = beta * *y + val[e + P * d];
                     }
         }
+    free(tensor);
 }
 
 /* ---------------------------------------------------------------------------------------------- */
@@ -333,14 +461,24 @@ void orc_lmv(int nx, int ny, int p, int nq, const double *xi, const double *eta,
 {
 #define X(q, i, j, d, e) x[psi_idx(ny, nq, p, q, i, j, d, e)]
 #define Y(q, i, j, d, e) y[psi_idx(ny, nq, p, q, i, j, d, e)]
+    double *tensor = g_xs_deg > 0 ? xs_tensor(nx, ny, p, sigma_t) : NULL;
     for (int i = 1; i <= nx; ++i)
         for (int j = 1; j <= ny; ++j)
             for (int q = 0; q < nq; ++q) {
                 for (int d = 0; d <= p; ++d)
                     for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) *= beta;
-                const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
-                for (int d = 0; d <= p; ++d)
-                    for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) += alpha * sig * X(q, i, j, d, e);
+                if (!tensor) {
+                    const double sig = sigma_t[(size_t)j + (size_t)(ny + 2) * i];
+                    for (int d = 0; d <= p; ++d)
+                        for (int e = 0; e <= p; ++e) Y(q, i, j, d, e) += alpha * sig * X(q, i, j, d, e);
+                } else { /* general sigma, :1587-1596 */
+                    for (int d = 0; d <= p; ++d)
+                        for (int e = 0; e <= p; ++e)
+                            for (int u = 0; u <= p; ++u)
+                                for (int v = 0; v <= p; ++v)
+                                    Y(q, i, j, d, e) += alpha * (2 * d + 1) * (2 * e + 1) *
+                                                        xs_at(tensor, ny, p, i, j, d, e, u, v) * X(q, i, j, u, v) / 4.0;
+                }
 
                 double coeff = alpha * xi[q] / dx;
                 if (xi[q] < 0) {
@@ -378,6 +516,7 @@ void orc_lmv(int nx, int ny, int p, int nq, const double *xi, const double *eta,
                     for (int e = 0; e <= p; ++e)
                         for (int b = e - 1; b >= 0; b -= 2) Y(q, i, j, d, e) += coeff * (2 * e + 1) * X(q, i, j, d, b);
             }
+    free(tensor);
 #undef X
 #undef Y
 }

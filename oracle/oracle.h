@@ -27,6 +27,19 @@
 extern "C" {
 #endif
 
+/* Cross-section degree of the following calls (0 default: sigma arrays are cell means).  Degree p (the
+ * flux degree): every sigma argument of orc_sweep, orc_sweep_omp, orc_smv, orc_lmv, orc_si_solve and
+ * orc_peierls_gmres is a [i][j][u][v] array of Legendre coefficients, entering through the tensor of
+ * Legendre triple products (init.cpp:741, CrossSection::ComputeTensor CrossSection.cpp:165-189; CalcA
+ * SweepLinearSolveSpec.cpp:300-321, Smv TransportOperator.cpp:375-381, Lmv :1587-1596).  Process-wide
+ * setting; orc_sweep_adjoint supports degree 0 only. */
+void orc_set_xs_degree(int degree);
+
+/* tensor[((i*(ny+2)+j)*N + d*P+e)*N + a*P+b] = sigma(i,j,a,b,d,e) = sum_{u,v} sigma[i][j][u][v] TPI(a,d,u)
+ * TPI(b,e,v), TPI(a,b,c) = int P_a P_b P_c on [-1,1] (ComputeTPI global.cpp:67-107, ComputeTensor
+ * CrossSection.cpp:179-188, TensorIndex CrossSection.hpp:397-478). */
+void orc_xs_tensor(int nx, int ny, int p, const double *sigma, double *tensor);
+
 /* Angular part of the element matrices, one N x N column-major matrix per ordinate,
  * Aq[q*N*N + row + N*col], row/col = e + (p+1)*d.
  * SweepLinearSolveSpec.cpp:98-158 (2D), index map SweepCellSolve.hpp:310-378. */

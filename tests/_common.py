@@ -62,6 +62,10 @@ def oracle():
         lib.orc_peierls_gmres.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, i, P, P, IP, P]
         lib.orc_sweep_adjoint.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, P]
         lib.orc_sweep_omp.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, i]
+        lib.orc_set_xs_degree.argtypes = [i]
+        lib.orc_set_xs_degree.restype = None
+        lib.orc_xs_tensor.argtypes = [i, i, i, P, P]
+        lib.orc_xs_tensor.restype = None
         _oracle = lib
     return _oracle
 
@@ -200,6 +204,40 @@ def orc_solve_adjoint(g: Grid, o: Ordinates, sigma_t, sigma_s, seed, dt=math.inf
     return zeta
 
 
+class xs_degree:
+    """Context manager: the oracle's sigma arguments are degree-`degree` Legendre coefficient arrays
+    ([nx+2][ny+2][p+1][p+1]) inside the block (orc_set_xs_degree), cell means again after it."""
+
+    def __init__(self, degree):
+        self.degree = degree
+
+    def __enter__(self):
+        oracle().orc_set_xs_degree(self.degree)
+        return self
+
+    def __exit__(self, *exc):
+        oracle().orc_set_xs_degree(0)
+
+
+def orc_xs_tensor(g: Grid, sigma):
+    t = np.zeros((g.nx + 2, g.ny + 2, g.N, g.N))
+    oracle().orc_xs_tensor(g.nx, g.ny, g.p, ptr(np.ascontiguousarray(sigma, dtype=np.float64)), ptr(t))
+    return t
+
+
+def random_poly_sigma(g: Grid, rng, lo=0.5, hi=3.0, rel=0.15):
+    """Degree-p cross-section coefficients [nx+2][ny+2][p+1][p+1] that stay positive on every cell: mean
+    uniform in [lo, hi], every higher Legendre coefficient at most rel * mean / (p+1)^2 in magnitude
+    (|P_k| <= 1 on the cell, so the polynomial is >= mean (1 - rel) > 0)."""
+    P = g.p + 1
+    c = np.zeros((g.nx + 2, g.ny + 2, P, P))
+    mean = rng.uniform(lo, hi, size=(g.nx + 2, g.ny + 2))
+    c[..., 0, 0] = mean
+    hi_terms = rng.uniform(-1.0, 1.0, size=c.shape) * (rel * mean / (P * P))[..., None, None]
+    hi_terms[..., 0, 0] = 0.0
+    return c + hi_terms
+
+
 def orc_pmv(g: Grid, o: Ordinates, psi, phi=None, alpha=1.0, beta=0.0, all_cells=0):
     out = np.zeros(g.phi_shape()) if phi is None else np.ascontiguousarray(phi, dtype=np.float64).copy()
     oracle().orc_pmv(g.nx, g.ny, g.p, o.nq, ptr(o.w), alpha, beta, ptr(np.ascontiguousarray(psi)), ptr(out),
@@ -290,15 +328,18 @@ def project_field(f, g: Grid, halo=True):
     return out
 
 
-def dg_weak_form_solve(g: Grid, xi, eta, sigma_t, src_coef, inflow_coef, inv_dt=0.0):
+def dg_weak_form_solve(g: Grid, xi, eta, sigma_t, src_coef, inflow_coef, inv_dt=0.0, sigma_coef=None):
     """Solve the upwind-DG transport equation for ONE ordinate by dense global assembly of the weak form
         -int psi Omega.grad v + sum_faces int (Omega.n) psi_upwind v + (sigma + 1/dt) int psi v = int Q v
     using Gauss quadrature of Legendre basis products (no closed-form element coefficients).
     src_coef, inflow_coef: [nx+2][ny+2][p+1][p+1] Legendre coefficients; returns psi on the same shape
-    with the inflow coefficients copied to the halo cells."""
+    with the inflow coefficients copied to the halo cells.  sigma_coef ([nx+2][ny+2][p+1][p+1], optional):
+    sigma_t as a polynomial per cell, int sigma psi v then evaluated by quadrature of sigma's expansion
+    (sigma_t is ignored)."""
     p = g.p
     P = p + 1
     xg, wg = gauss(p + 3)
+    Lg = np.array([leg(a, xg) for a in range(P)])  # [degree][node]
     M1 = np.array([[np.sum(wg * leg(d, xg) * leg(a, xg)) for a in range(P)] for d in range(P)])
     D1 = np.array([[np.sum(wg * leg(a, xg) * dleg(d, xg)) for a in range(P)] for d in range(P)])  # int P_a P_d'
     Pp = np.array([leg(a, 1.0) for a in range(P)])
@@ -321,7 +362,10 @@ def dg_weak_form_solve(g: Grid, xi, eta, sigma_t, src_coef, inflow_coef, inv_dt=
 
     for i in range(1, nx + 1):
         for j in range(1, ny + 1):
-            sig = sigma_t[i, j] + inv_dt
+            sig = sigma_t[i, j] + inv_dt if sigma_coef is None else inv_dt
+            if sigma_coef is not None:
+                sv = np.einsum("uv,ug,vh->gh", sigma_coef[i, j], Lg, Lg)   # sigma at the Gauss points
+                Ms = np.einsum("g,h,gh,dg,ag,eh,bh->deab", wg, wg, sv, Lg, Lg, Lg, Lg)
             for d in range(P):
                 for e in range(P):
                     r = gid(i, j, d, e)
@@ -330,6 +374,8 @@ def dg_weak_form_solve(g: Grid, xi, eta, sigma_t, src_coef, inflow_coef, inv_dt=
                         for b in range(P):
                             v = -xi * (dy / 2.0) * D1[d, a] * M1[e, b] - eta * (dx / 2.0) * M1[d, a] * D1[e, b]
                             v += sig * dx * dy / 4.0 * M1[d, a] * M1[e, b]
+                            if sigma_coef is not None:
+                                v += dx * dy / 4.0 * Ms[d, e, a, b]
                             K[r, gid(i, j, a, b)] += v
                             # x faces
                             if xi > 0:

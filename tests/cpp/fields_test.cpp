@@ -329,7 +329,21 @@ static void test_decks(const std::string &dir) {
         check(t, what);
     };
     const std::string base = "nx 4\nny 4\nax 0\nbx 7\nay 0\nby 7\ndg_degree 1\nproblem lattice\nmoll_radius 0\nang_order 2\n";
-    throws(base, "piecewise_cross absent (false) is refused");
+    {
+        // piecewise_cross absent (false): cross sections at the flux degree (init.cpp:741); the 4 x 4 mesh of
+        // the 7 x 7 lattice cuts the blocks, so the projections carry x and y moments
+        write_text(dir + "/ho.deck", base);
+        ParameterList x;
+        x.ReadFromFile(dir + "/ho.deck");
+        const ProblemInputs ho = Init(x);
+        double hi = 0.0;
+        for (int64_t i = 1; i <= 4; ++i)
+            for (int64_t j = 1; j <= 4; ++j) hi = std::max(hi, std::fabs(ho.sigma_t(i, j, 1, 0)));
+        check(ho.sigma_t.DG_degree == 1 && ho.sigma_s.DG_degree == 1 && hi > 0.1 &&
+                  ho.sigma_t(2, 2) == ho.sigma_t_density(2, 2, 0, 0) &&
+                  ho.sigma_t(2, 2, 1, 1) == ho.sigma_t_density(2, 2, 1, 1),
+              "piecewise_cross false: degree-1 cross sections with higher moments");
+    }
     throws(base + "piecewise_cross true\nperiodic true\n", "periodic boundaries are refused");
     throws(base + "piecewise_cross true\nreflect_y_min true\n", "reflecting boundaries are refused");
     throws("nx 4\nny 4\nax 0\nbx 7\nay 0\nby 7\ndg_degree 1\nproblem reeds\nmoll_radius 0\nang_order 2\npiecewise_cross true\n",

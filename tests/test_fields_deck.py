@@ -330,3 +330,27 @@ def test_python_field_files_round_trip(tmp_path):
     fields.write_ordinate_flux(str(tmp_path / "p.af2"), psi, m, info)
     back, m3, info3 = fields.read_ordinate_flux(str(tmp_path / "p.af2"))
     assert info3 == info and m3 == m and np.array_equal(back, psi)
+
+
+def test_high_order_cross_sections_python_matches_cpp_tool(tmp_path):
+    """piecewise_cross false (init.cpp:741): both Init paths project the cross sections at the flux degree
+    and write them under the reference's names (sigma_t_dg2_...); the projections agree, carry higher
+    moments where cells straddle the lattice blocks, and keep the cell means of the degree-0 projection."""
+    d = tmp_path / "ho.deck"
+    _small_lattice_deck(d)
+    d.write_text(d.read_text().replace("piecewise_cross true", "piecewise_cross false").replace("nx 21", "nx 10")
+                 .replace("ny 14", "ny 9"))
+    r = subprocess.run([_tool("tsw_init"), str(d), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    P, ex = deck.init(deck.read_deck(str(d)))
+    assert P.xs_degree == 2
+    for name in ("sigma_t_dg2_nx0000010.cs2", "sigma_s_dg2_nx0000010.cs2"):
+        v, m = fields.read_density(str(tmp_path / name))
+        np.testing.assert_allclose(v, ex[name], rtol=0, atol=1e-14, err_msg=name)
+    st, ss = P.cross_sections()
+    assert np.abs(st[1:-1, 1:-1, 1:, :]).max() > 0.1
+    d0 = tmp_path / "pw.deck"
+    d0.write_text(d.read_text().replace("piecewise_cross false", "piecewise_cross true"))
+    P0, _ = deck.init(deck.read_deck(str(d0)))
+    np.testing.assert_allclose(st[..., 0, 0], P0.sigma_t, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(P.sigma_t, P0.sigma_t, rtol=0, atol=1e-14)

@@ -163,3 +163,29 @@ def test_autotuner_choice_is_valid_and_result_identical():
     with _ctx(g, o, 0) as c:
         with pytest.raises(tsw.TswError):
             c.set_tuning("autotune", 2)
+
+
+@pytest.mark.parametrize("lane_split", [0, 1])
+def test_band_stall_is_reported_and_recovered(lane_split):
+    """The stall path of the banded sweep, forced from the host (tuning key stall_test raises the stall
+    flag before the sweep, so every bounded wait gives up within one poll interval and every workgroup
+    still exits): the call fails with TSW_E_HIP, and the next sweep, after the exchange records are
+    refilled, matches the oracle again."""
+    o = ls_ordinates(4)
+    g, st, _, src = random_problem(30, 140, 2, o, seed=811, materials=3)
+    ref = orc_sweep(g, o, st, src)
+    with _ctx(g, o, 32) as c:
+        c.set_tuning("lane_split", lane_split)
+        for prec, tol in ((tsw.FP64, 1e-12), (tsw.FP32, 2e-6)):
+            out = src.copy()
+            c.sweep(out, st, precision=prec)
+            assert rel_err(out, ref) <= tol
+            c.set_tuning("stall_test", 1)
+            out = src.copy()
+            with pytest.raises(tsw.TswError) as ei:
+                c.sweep(out, st, precision=prec)
+            assert ei.value.code == tsw.TSW_E_HIP and "stalled" in str(ei.value)
+            for _ in range(2):
+                out = src.copy()
+                c.sweep(out, st, precision=prec)
+                assert rel_err(out, ref) <= tol, prec

@@ -136,3 +136,36 @@ def test_tsw_solve_idc_bump_deck_matches_oracle(tmp_path):
     mass = float(phi[1:-1, 1:-1, 0, 0].sum()) * m.dx * m.dy
     print("bump mass", mass0, mass, flush=True)
     assert mass0 == pytest.approx(1.0, abs=1e-6) and mass == pytest.approx(mass0, abs=1e-5)
+
+
+def test_deck_high_order_cross_sections_match_oracle(tmp_path):
+    """piecewise_cross false: the cross sections are projected at the flux's DG degree (init.cpp:741).  On
+    a 30 x 30 mesh of the 7 x 7 lattice the cells straddle the absorber blocks' edges, so the projections
+    carry genuine higher moments.  The Python deck path and the C++ driver tsw_solve both solve it on the
+    GPU; both are compared with the oracle's GMRES on the same coefficient arrays."""
+    from _common import xs_degree
+    text = LATTICE.replace("piecewise_cross true", "piecewise_cross false").replace("nx 28", "nx 30").replace(
+        "ny 28", "ny 30") + "output_af true\n"
+    d = tmp_path / "ho.deck"
+    d.write_text(text)
+    P, ex = deck.init(deck.read_deck(str(d)))
+    assert P.xs_degree == P.p == 2 and "sigma_t_dg2_nx0000030.cs2" in ex
+    st, ss = P.cross_sections()
+    assert np.abs(st[1:-1, 1:-1, 1:, :]).max() > 0.1   # cells cut by block edges
+    with xs_degree(P.p):
+        rc, ref, _, its, _ = orc_gmres(_grid(P), _ords(P), st, ss, P.source, abs_tol=1e-13, restart=40)
+    assert rc == 0
+    with tsw.Context(0) as ctx:
+        P.set_on(ctx)
+        out = P.source.copy()
+        ctx.gmres_solve(out, st, ss, abs_tol=1e-13, restart=40)
+    e = rel_err(out, ref)
+    print("high-order cross sections: rel err vs oracle", e, "oracle its", its, flush=True)
+    assert e <= 1e-9
+    print("running tsw_solve on a piecewise_cross false deck", flush=True)
+    r = subprocess.run([os.path.join(PKG, "tsw_solve"), str(d)], capture_output=True, text=True, timeout=300,
+                       cwd=str(tmp_path))
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    psi, _, _ = fields.read_ordinate_flux(str(tmp_path / "ho.af2"))
+    assert rel_err(psi, ref) <= 1e-9

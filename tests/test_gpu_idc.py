@@ -1,7 +1,11 @@
 """IDC time steps on the GPU (tsw_idc_step) against the oracle's restatement of the reference's
 non-hybrid error-form IDC (oracle/idc.py, pinned by order-of-accuracy tests): several steps with
-Lobatto nodes (FP64 to 1e-10 relative, FP32 sweeps with FP64 refinement and correction sums to 1e-8),
-Legendre nodes with collocation, and the line-source configuration's shape at a reduced mesh."""
+Lobatto nodes (FP64 to 1e-10 relative, FP32 sweeps with FP64 Krylov vectors and correction sums to
+1e-8), Legendre nodes with collocation, sources with and without inflow halos, and the largest node
+counts (12 Lobatto or Legendre nodes, whose correction sources combine 15 arrays).  The adjoint of
+several steps is pinned by the dot-product identity and by central differences of the cross-section
+gradients, with and without inflow halos.  The line-source configuration itself is in
+test_gpu_line_source.py."""
 import math
 
 import numpy as np
@@ -14,9 +18,9 @@ from _common import Grid, OracleIdcOps, ls_ordinates, random_problem, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _vacuum_problem(nx, ny, p, order, seed):
+def _vacuum_problem(nx, ny, p, order, seed, halo_inflow=False):
     o = ls_ordinates(order)
-    g, st, ss, src = random_problem(nx, ny, p, o, seed=seed, halo_inflow=False, materials=4)
+    g, st, ss, src = random_problem(nx, ny, p, o, seed=seed, halo_inflow=halo_inflow, materials=4)
     rng = np.random.default_rng(seed + 1)
     psi0 = np.zeros(g.psi_shape(o.nq))
     psi0[1:-1, 1:-1] = rng.uniform(0.0, 1.0, size=psi0[1:-1, 1:-1].shape)
@@ -27,8 +31,9 @@ def _vacuum_problem(nx, ny, p, order, seed):
                                                              (tsw.IDC_LOBATTO, 3, 1, True),
                                                              (tsw.IDC_LEGENDRE, 3, 2, True)])
 @pytest.mark.parametrize("pattern", ["hip-staggered", "hip-wavefront"])
-def test_idc_steps_match_oracle(kind, num_nodes, num_corr, colloc, pattern):
-    g, o, st, ss, src, psi0 = _vacuum_problem(9, 12, 1, 4, seed=71)
+@pytest.mark.parametrize("halo_inflow", [False, True])
+def test_idc_steps_match_oracle(kind, num_nodes, num_corr, colloc, pattern, halo_inflow):
+    g, o, st, ss, src, psi0 = _vacuum_problem(9, 12, 1, 4, seed=71, halo_inflow=halo_inflow)
     ops = OracleIdcOps(g, o, st, ss, tol=1e-14)
     ref = psi0.copy()
     name = "lobatto" if kind == tsw.IDC_LOBATTO else "legendre"
@@ -46,6 +51,23 @@ def test_idc_steps_match_oracle(kind, num_nodes, num_corr, colloc, pattern):
                 c.idc_step(psi, src, st, ss, dt, abs_tol=1e-13 if prec == tsw.FP64 else 1e-11, precision=prec)
             assert rel_err(psi, ref) <= tol, prec
             assert not psi[0].any() and not psi[:, -1].any()
+
+
+@pytest.mark.parametrize("kind,colloc", [(tsw.IDC_LOBATTO, False), (tsw.IDC_LOBATTO, True),
+                                         (tsw.IDC_LEGENDRE, True)])
+def test_idc_twelve_nodes_match_oracle(kind, colloc):
+    """The largest node count tsw_idc_set accepts: the correction source combines 3 + 12 arrays."""
+    g, o, st, ss, src, psi0 = _vacuum_problem(4, 5, 1, 2, seed=17, halo_inflow=True)
+    ops = OracleIdcOps(g, o, st, ss, tol=1e-14)
+    name = "lobatto" if kind == tsw.IDC_LOBATTO else "legendre"
+    dt = 0.05
+    ref = idc.idc_step(ops, psi0.copy(), src, dt, num_nodes=12, num_corr=2, kind=name, collocation=colloc)
+    with tsw.Context(0) as c:
+        c.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
+        c.idc_set(12, 2, kind, colloc)
+        psi = psi0.copy()
+        c.idc_step(psi, src, st, ss, dt, abs_tol=1e-13)
+        assert rel_err(psi, ref) <= 1e-10
 
 
 def test_idc_device_arrays_and_errors():
@@ -72,14 +94,18 @@ def _ctx_for(g, o, num_nodes=3, num_corr=2):
     return c
 
 
-def test_idc_adjoint_dot_product_over_steps():
+@pytest.mark.parametrize("halo_inflow", [False, True])
+@pytest.mark.parametrize("kind,colloc", [(tsw.IDC_LOBATTO, False), (tsw.IDC_LEGENDRE, True)])
+def test_idc_adjoint_dot_product_over_steps(halo_inflow, kind, colloc):
     """The IDC map is affine in psi^0: <g, run(x) - run(0)> = <dJ/dpsi^0, x> pins the transposed step
-    (adjoint solves, (L - S P)^T residual terms, integration-matrix weights) over several steps."""
-    g, o, st, ss, src, psi0 = _vacuum_problem(8, 7, 1, 4, seed=90)
+    (adjoint solves, (L - S P)^T residual terms, integration-matrix weights) over several steps; with
+    inflow halos the affine part changes and the linear part must not."""
+    g, o, st, ss, src, psi0 = _vacuum_problem(8, 7, 1, 4, seed=90, halo_inflow=halo_inflow)
     rng = np.random.default_rng(91)
     seed = np.zeros(g.psi_shape(o.nq))
     seed[1:-1, 1:-1] = rng.normal(size=seed[1:-1, 1:-1].shape)
     with _ctx_for(g, o) as c:
+        c.idc_set(3, 2, kind, colloc)
         dt, nsteps = 0.07, 4
         a = psi0.copy()
         c.idc_run(a, src, st, ss, dt, nsteps, abs_tol=1e-14)
@@ -93,8 +119,9 @@ def test_idc_adjoint_dot_product_over_steps():
     assert not grad[0].any() and not grad[:, -1].any()
 
 
-def test_idc_adjoint_cross_section_gradients_fd_and_checkpoints():
-    g, o, st, ss, src, psi0 = _vacuum_problem(7, 6, 1, 2, seed=33)
+@pytest.mark.parametrize("halo_inflow", [False, True])
+def test_idc_adjoint_cross_section_gradients_fd_and_checkpoints(halo_inflow):
+    g, o, st, ss, src, psi0 = _vacuum_problem(7, 6, 1, 2, seed=33, halo_inflow=halo_inflow)
     rng = np.random.default_rng(34)
     seed = np.zeros(g.psi_shape(o.nq))
     seed[1:-1, 1:-1] = rng.normal(size=seed[1:-1, 1:-1].shape)
