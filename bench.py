@@ -16,6 +16,7 @@ The CPU baseline is the oracle (a single-thread C restatement of the reference s
 sample of sweeps of the same lattice.
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -60,6 +61,52 @@ def cpu_baseline(P, sweeps):
     return {"value": P.unknowns * sweeps / dt, "unit": "angular unknowns swept/s", "cores": threads, "kind": "port",
             "sample": f"{sweeps} FP64 sweeps of the {P.nx}x{P.ny} Q{P.p} nq={P.nq} {P.name} problem by the oracle's "
                       f"wavefront sweep on {threads} threads ({dt:.2f} s) on {cpu_model()}"}
+
+
+REF_LATTICE_DECK = ("ax 0.0\nbx 7.0\nay 0.0\nby 7.0\nnx {n}\nny {n}\ndg_degree 2\npiecewise_cross true\nproblem lattice\n"
+                    "moll_radius 0.0\nordinate_type spherical-triangle\nang_order 5\nordinate_sym_reduce true\n"
+                    "solve_type source-iteration\nsweep_pattern wavefront\nsweep_solve gaussian-elimination\n"
+                    "abs_tol 1e-8\nmax_its 500\ntfinal 1.0\noutput_log false\n")
+
+
+def cpu_baseline_reference(n, sweeps):
+    """The reference's own sweep, compiled unmodified from its sources with its optimisation flags
+    (oracle/_ref/libref_fast.so: -O3, OpenMP, SIMD blocking with SIMD_LEN 16 as in
+    make/Makefile.workstation.inc:30), on the n x n checkerboard lattice at Q2.  The reference has no
+    level-symmetric set, so its T_5 set reduced to the hemisphere (100 ordinates) stands in for S12 (84);
+    the unknowns/s figure counts the ordinates actually swept.  Threads: OMP_NUM_THREADS (the host share
+    this process runs with).  None when the library is not there."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import tempfile
+    import _ref
+    if not _ref.available(fast=True):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "lattice.deck")
+        with open(path, "w") as f:
+            f.write(REF_LATTICE_DECK.format(n=n))
+        old = os.dup(1)  # the reference's Init prints its parameter summary on stdout
+        os.dup2(os.open(os.devnull, os.O_WRONLY), 1)
+        try:
+            _ref.lib(fast=True)
+            d = _ref.Deck(path)
+            src = d.get_flux(1)
+            d.sweep(src)  # warm-up (threads, pages, cell-solve set-up)
+            t0 = time.perf_counter()
+            for _ in range(sweeps):
+                d.sweep(src)
+            dt = time.perf_counter() - t0
+        finally:
+            ctypes.CDLL(None).fflush(None)  # the C library's buffered output goes to /dev/null too
+            os.dup2(old, 1)
+            os.close(old)
+    U = d.nq * d.nx * d.ny * (d.p + 1) ** 2
+    return {"value": U * sweeps / dt, "unit": "angular unknowns swept/s", "cores": threads, "kind": "reference",
+            "sample": f"{sweeps} FP64 sweeps by the reference's own wavefront sweep (its sources compiled with -O3, "
+                      f"OpenMP, SIMD_LEN 16) of the {d.nx}x{d.ny} Q2 checkerboard lattice with T_5 reduced "
+                      f"({d.nq} ordinates; the reference has no S12), {dt:.2f} s on {threads} threads of "
+                      f"{cpu_model()}"}
 
 
 def bench_si(args):
@@ -451,7 +498,8 @@ def main():
                      "traffic_measured_as": traffic32_as},
     }
     if args.cpu_sweeps > 0:
-        line["cpu_baseline"] = cpu_baseline(P, args.cpu_sweeps)
+        ref = cpu_baseline_reference(P.nx, args.cpu_sweeps)
+        line["cpu_baseline"] = ref if ref else cpu_baseline(P, args.cpu_sweeps)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -498,7 +498,7 @@ void sweep_prec(tsw_ctx *c, double *psi, const double *sigma, double dt, int pre
     launch_convert_d2f(psi, p32, n, c->stream);
     const float *s32 = as_f32(c, sigma, c->g.ncell_h(), c->sig_t32);
     sweep_dev<float>(c, p32, s32, dt, sigma);
-    launch_convert_f2d(p32, psi, n, c->stream);
+    launch_convert_f2d_interior(c->g, (size_t)c->g.nq * c->g.N(), p32, psi, c->stream);  // inflow halos kept
 }
 
 double device_norm2_interior_diff(tsw_ctx *c, const double *a, const double *b) {

@@ -291,6 +291,21 @@ void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s) {
     TSW_HIP(hipGetLastError());
 }
 
+// out = (double) in on the interior cells of a per-cell array (halo cells of out untouched)
+__global__ void k_f2d_interior(const float *__restrict__ in, double *__restrict__ out, int nx, int ny,
+                               size_t per_cell) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)(nx + 2) * (ny + 2) * per_cell) return;
+    const size_t cell = t / per_cell;
+    const int i = (int)(cell / (ny + 2)), j = (int)(cell % (ny + 2));
+    if (i >= 1 && j >= 1 && i <= nx && j <= ny) out[t] = (double)in[t];
+}
+void launch_convert_f2d_interior(const Geom &g, size_t per_cell, const float *in, double *out, hipStream_t s) {
+    const size_t n = g.ncell_h() * per_cell;
+    hipLaunchKernelGGL(k_f2d_interior, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, g.nx, g.ny, per_cell);
+    TSW_HIP(hipGetLastError());
+}
+
 // ---- region fill: halo cells (halo=1) or interior cells (halo=0) of a per-cell array get `value` ----
 template <typename T>
 __global__ void k_fill_region(T *__restrict__ a, int nx, int ny, size_t per_cell, int halo, T value) {

@@ -235,6 +235,8 @@ void launch_lmv(const Geom &g, const T *xi, const T *eta, const T *sigma_t, cons
                 hipStream_t s, const T *xs = nullptr);
 void launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
 void launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
+// out = (double) in on the interior cells of a per-cell array; halo cells of out are left as they are
+void launch_convert_f2d_interior(const Geom &g, size_t per_cell, const float *in, double *out, hipStream_t s);
 
 // y[k] = a*x[k] + b*y[k]
 void launch_axpby(size_t n, double a, const double *x, double b, double *y, hipStream_t s);

@@ -1,0 +1,92 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/ref_fixtures.npz from the reference's own code (oracle/_ref/libref.so, built by
+oracle/build_ref.sh from the unmodified reference sources; run in the container that holds the reference):
+
+* ordinate sets: OrdinateSet(order, reduce, type) for spherical-triangle T_1..T_8 and chebyshev-legendre
+  orders 2..8 (even), full and symmetry-reduced: xi, eta, w and the quadrant starts, in the reference's
+  order (SphericalTriangle.cpp:180-351, OrdinateSet.cpp:441-520,1065-1190);
+* sweeps: SweepOperator::Apply (sweep_pattern wavefront, sweep_solve gaussian-elimination) on seeded random
+  sources with inflow halos, for decks whose cross sections Init projects at degree 0 and at the flux
+  degree (piecewise_cross false), steady and with dt;
+* source iteration: SISolver::Solve on two lattice decks (steady, abs_tol 1e-10 and 1e-12): the angular
+  flux, the iteration count and the last change (printed with four decimals).
+Each case also stores the inputs the reference's Init built (cross-section coefficients, ordinates, mesh),
+so the tests need only the fixture.  Usage: python tests/golden/make_ref_fixtures.py
+"""
+import math
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import _ref  # noqa: E402
+
+LATTICE = ("ax 0.0\nbx 7.0\nay 0.0\nby 7.0\nnx {n}\nny {m}\ndg_degree {p}\npiecewise_cross {pwc}\nproblem lattice\n"
+           "moll_radius 0.0\nordinate_type {otype}\nang_order {order}\nordinate_sym_reduce true\n"
+           "solve_type source-iteration\nsweep_pattern wavefront\nsweep_solve gaussian-elimination\n"
+           "abs_tol {tol}\nmax_its 5000\ntfinal 1.0\noutput_log false\n")
+
+SWEEPS = {  # name: deck parameters, seed
+    "sweep_pwc": (dict(n=9, m=8, p=1, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"), 11),
+    "sweep_ho": (dict(n=8, m=9, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-10"), 12),
+    "sweep_ho3": (dict(n=6, m=5, p=3, pwc="false", otype="spherical-triangle", order=2, tol="1e-10"), 13),
+}
+SOLVES = {
+    "si_pwc": dict(n=14, m=14, p=1, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"),
+    "si_ho": dict(n=10, m=10, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-12"),
+}
+
+
+def main():
+    if not _ref.available():
+        sys.exit("oracle/_ref/libref.so is missing: run oracle/build_ref.sh")
+    out = {}
+    for kind, orders in (("spherical-triangle", range(1, 9)), ("chebyshev-legendre", (2, 4, 6, 8))):
+        for order in orders:
+            for red in (0, 1):
+                xi, eta, w, quad = _ref.ordinate_set(kind, order, red)
+                key = f"ords/{kind}/{order}/{red}"
+                out[key + "/xi"], out[key + "/eta"], out[key + "/w"], out[key + "/quad"] = xi, eta, w, quad
+    tmp = tempfile.mkdtemp()
+    for name, (params, seed) in SWEEPS.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(LATTICE.format(**params))
+        d = _ref.Deck(path)
+        rng = np.random.default_rng(seed)
+        src = rng.uniform(-0.5, 1.0, size=d.psi_shape())
+        _store_inputs(out, name, d)
+        out[name + "/src"] = src
+        out[name + "/steady"] = d.sweep(src)
+        out[name + "/dt03"] = d.sweep(src, dt=0.3)
+    for name, params in SOLVES.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(LATTICE.format(**params))
+        d = _ref.Deck(path)
+        _store_inputs(out, name, d)
+        src = d.get_flux(1)
+        out[name + "/src"] = src
+        out[name + "/tol"] = np.array(float(params["tol"]))
+        psi, its, err = d.solve(src, os.path.join(tmp, name + ".log"))
+        out[name + "/psi"] = psi
+        out[name + "/its"] = np.array(its)
+        out[name + "/err"] = np.array(err)
+        print(name, "iterations", its, "last change", err)
+    np.savez_compressed(os.path.join(HERE, "ref_fixtures.npz"), **out)
+    print("wrote", os.path.join(HERE, "ref_fixtures.npz"), len(out), "arrays")
+
+
+def _store_inputs(out, name, d):
+    out[name + "/dims"] = np.array([d.nx, d.ny, d.p, d.nq, d.xs_degree])
+    out[name + "/box"] = np.array(d.box)
+    out[name + "/xi"], out[name + "/eta"], out[name + "/w"], out[name + "/quad"] = d.xi, d.eta, d.w, d.quad
+    out[name + "/sigma_t"] = d.get_xs(0)
+    out[name + "/sigma_s"] = d.get_xs(1)
+
+
+if __name__ == "__main__":
+    main()
