@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -202,6 +203,29 @@ struct tsw_ctx {
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// ---------------------------------------------------------------------------------------------------
+// Scratch of the allocation-free sweep (tsw_sweep_apply_noalloc): one file-scope device array, of which
+// every GPU holds its own copy for the life of the process.  Host code never uses the array's name as an
+// address: its per-device address is looked up with hipGetSymbolAddress when the first context is
+// created on that device, and that pointer is what the kernels receive.
+// ---------------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(128) << 20;  // 128 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Looks up the address of `scratch` on `device` (the current device) once; TSW_E_HIP on failure.
+static int scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return TSW_OK;  // such devices run no allocation-free sweep
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return TSW_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TSW_E_HIP;
+    g_scratch_ptr[device] = p;
+    return TSW_OK;
+}
 
 namespace {
 
@@ -2190,7 +2214,8 @@ int tsw_create(int device, tsw_ctx **out) {
     tsw_ctx *c = new (std::nothrow) tsw_ctx();
     if (!c) return TSW_E_HIP;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        scratch_init(device) != TSW_OK) {
         delete c;
         return TSW_E_HIP;
     }
@@ -2311,6 +2336,61 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
         sweep_device(ctx, psi, sig, dt, precision);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
         sync(ctx);
+    });
+}
+
+int tsw_sweep_apply_noalloc(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
+                "bad precision");
+        require(psi_inout != nullptr && sigma_t != nullptr, TSW_E_ARG, "null array argument");
+        require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "the allocation-free sweep takes cell-mean cross sections");
+        require(ctx->device < kMaxScratchDevices && g_scratch_ptr[ctx->device] != nullptr, TSW_E_STATE,
+                "no scratch address for this device");
+        const Geom &g = ctx->g;
+        hipStream_t s = ctx->stream;
+        const size_t npsi = g.npsi(), nc = g.ncell_h(), per = (size_t)g.nq * g.N();
+        auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
+        // scratch carve-up: FP32 copies of psi and sigma_t; TSW_MIXED adds the FP64 iterate and residual
+        const size_t b32 = r256(npsi * sizeof(float)), bs32 = r256(nc * sizeof(float)), b64 = r256(npsi * sizeof(double));
+        const size_t need = precision == TSW_FP64 ? 0 : b32 + bs32 + (precision == TSW_MIXED ? 2 * b64 : 0);
+        require(need <= kScratchBytes, TSW_E_ARG,
+                "the allocation-free sweep needs " + std::to_string(need) + " bytes of scratch for this problem and "
+                "precision; it holds " + std::to_string(kScratchBytes));
+        unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[ctx->device]);
+        float *p32 = reinterpret_cast<float *>(base);
+        float *s32 = reinterpret_cast<float *>(base + b32);
+        double *psi64 = reinterpret_cast<double *>(base + b32 + bs32);
+        double *res64 = reinterpret_cast<double *>(base + b32 + bs32 + b64);
+        if (precision == TSW_FP64) {
+            launch_sweep_wavefront<double>(params<double>(ctx, psi_inout, sigma_t, dt), g.p, s);
+        } else {
+            launch_convert_d2f(sigma_t, s32, nc, s);
+            const SweepParams<float> sp = params<float>(ctx, p32, s32, dt);
+            if (precision == TSW_FP32) {
+                launch_convert_d2f(psi_inout, p32, npsi, s);
+                launch_sweep_wavefront<float>(sp, g.p, s);
+                launch_convert_f2d_interior(g, per, p32, psi_inout, s);
+            } else {
+                // as sweep_wavefront_mixed: from the iterate with zero interior and the source's inflow halos,
+                // 1 + refine_steps times psi += L32^{-1}(Q - (L + 1/dt) psi), FP64 residual, FP32 zero-inflow sweep
+                TSW_HIP(hipMemcpyAsync(psi64, psi_inout, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+                launch_fill_region<double>(g, psi64, per, 0, 0.0, s);
+                for (int k = 0; k <= ctx->refine_steps; ++k) {
+                    TSW_HIP(hipMemcpyAsync(res64, psi_inout, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+                    launch_lmv<double>(g, ctx->xi64.as<double>(), ctx->eta64.as<double>(), sigma_t, psi64, res64, -1.0,
+                                       1.0, s);
+                    if (inverse_dt(dt) != 0.0) launch_axpby(npsi, -inverse_dt(dt), psi64, 1.0, res64, s);
+                    launch_fill_region<double>(g, res64, per, 1, 0.0, s);
+                    launch_convert_d2f(res64, p32, npsi, s);
+                    launch_sweep_wavefront<float>(sp, g.p, s);
+                    launch_accumulate_f2d(p32, psi64, npsi, s);
+                }
+                TSW_HIP(hipMemcpyAsync(psi_inout, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
+            }
+        }
+        TSW_HIP(hipStreamSynchronize(s));
     });
 }
 

@@ -81,6 +81,7 @@ def lib():
         L.tsw_set_xs_degree.argtypes = [c, i]
         L.tsw_get_tuning.argtypes = [c, ctypes.c_char_p, _ip]
         L.tsw_sweep_apply.argtypes = [c, _vp, _vp, d, i, i]
+        L.tsw_sweep_apply_noalloc.argtypes = [c, _vp, _vp, d, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
         L.tsw_smv.argtypes = [c, d, d, _vp, _vp, _vp, i, i]
         L.tsw_lmv.argtypes = [c, d, d, _vp, _vp, _vp, i]
@@ -262,6 +263,15 @@ class Context:
         self._expect(psi=psi, xs=sigma_t)
         (pp, sp), mem = _addrs(psi, sigma_t)
         self._check(self._L.tsw_sweep_apply(self._h, pp, sp, dt, precision, mem))
+        return psi
+
+    def sweep_noalloc(self, psi, sigma_t, dt=math.inf, precision=FP64):
+        """tsw_sweep_apply_noalloc: device tensors only; no device allocation inside the call."""
+        self._expect(psi=psi, xs=sigma_t)
+        (pp, sp), mem = _addrs(psi, sigma_t)
+        if mem != MEM_DEVICE:
+            raise TypeError("the allocation-free sweep takes device tensors")
+        self._check(self._L.tsw_sweep_apply_noalloc(self._h, pp, sp, dt, precision))
         return psi
 
     def pmv(self, psi, phi, alpha=1.0, beta=0.0, all_cells=False):

@@ -129,6 +129,18 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
  * to single precision for the sweep and back; TSW_MIXED refines that FP32 sweep to FP64 accuracy. */
 int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem);
 
+/* tsw_sweep_apply without any device allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  psi_inout and sigma_t are device arrays (TSW_MEM_DEVICE layouts, cell-mean cross sections);
+ * the sweep runs the wavefront kernels of "hip-wavefront-eager" whatever the pattern, so its results are
+ * bitwise those of tsw_sweep_apply on that pattern.  TSW_FP32 and TSW_MIXED stage their FP32 copies and
+ * FP64 iterate in a per-GPU scratch array of 128 MiB held by the library for the life of the process
+ * (4 bytes per psi value + 4 per cell for TSW_FP32, 20 + 4 for TSW_MIXED, none for TSW_FP64); a problem
+ * that needs more fails with TSW_E_ARG before any work.  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (several contexts or threads) race.  The context must have been created
+ * and given its problem beforehand (those calls allocate); the call itself makes no hipMalloc/hipFree. */
+int tsw_sweep_apply_noalloc(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision);
+
 /* phi <- beta*phi + alpha*sum_q w_q psi_q.  TransportOperator::Pmv (TransportOperator.hpp:25-43,
  * TransportOperator.cpp:37-186).  all_cells: 0 = OpDomain::Interior, 1 = OpDomain::All. */
 int tsw_pmv(tsw_ctx *ctx, double alpha, double beta, const double *psi, double *phi, int all_cells, int mem);
