@@ -12,8 +12,12 @@ U*(4+4) + ncell*4, divided by the sweep kernel's average duration, measured with
 library records on the stream it launches the sweep on (tsw_sweep_timing).  `--copy-probe` adds a
 device copy of known size, the calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs.
 
-The CPU baseline is the oracle (a single-thread C restatement of the reference sweep) on a bounded
-sample of sweeps of the same lattice.
+The CPU baseline is the reference's own sweep (its sources compiled unmodified, OpenMP, SIMD_LEN 16) on a
+bounded sample of sweeps of the same lattice, or the oracle's OpenMP sweep where that build is absent.
+
+The line carries a `batched` sub-object: the 256-problem uneven batch (the configuration the
+mixed-precision targets are stated for) swept in FP32, mixed (FP32 + one FP64 refinement step) and FP64,
+with whole-call and kernel times, errors against FP64 and the FP32 kernels' roofline.
 """
 import argparse
 import ctypes
@@ -220,11 +224,14 @@ def bench_idc(args):
                     "rel_diff": abs(dJ - pred) / max(abs(dJ), 1e-300)}}), flush=True)
 
 
-def bench_batched(args):
+def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
     """Batched sweep of the seeded 256-problem uneven batch (N 32..256, Q1..Q3, S4..S12, 8x8 block
     materials): one step = one sweep of every member, inputs resident in HBM.  Timed in FP64, in FP32 and
     mixed (FP32 sweeps with one FP64 refinement step, TSW_MIXED); value = unknowns/s of the mixed sweep,
-    whose error against the FP64 sweep is reported beside it."""
+    whose error against the FP64 sweep is reported beside it.  Returns the JSON line (dict)."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    cpu_sweeps = args.cpu_sweeps if cpu_sweeps is None else cpu_sweeps
     import torch
     import tsw
     from tsw import problems
@@ -241,11 +248,11 @@ def bench_batched(args):
     res, outs = {}, {}
     for name, prec in (("fp32", tsw.FP32), ("mixed", tsw.MIXED), ("fp64", tsw.FP64)):
         work = [s.clone() for s in srcs]
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             ctx.batch_sweep(work, precision=prec)
         ctx.sweep_timing(reset=True)
         wall = 0.0
-        for _ in range(args.steps):
+        for _ in range(steps):
             for w, s in zip(work, srcs):   # restore the sources (outside the timed call)
                 w.copy_(s)
             torch.cuda.synchronize()
@@ -253,10 +260,11 @@ def bench_batched(args):
             ctx.batch_sweep(work, precision=prec)   # returns when the device work is done
             wall += time.perf_counter() - t0
         n, ms = ctx.sweep_timing(reset=True)
-        res[name] = (wall / args.steps, ms / max(n, 1), n // max(args.steps, 1))
+        res[name] = (wall / steps, ms / max(n, 1), n // max(steps, 1))
         outs[name] = work
-        print(f"bench: batched {name}: call {wall / args.steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
+        print(f"bench: batched {name}: call {wall / steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
               f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
+    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch")}
     ctx.close()
 
     def rel(a_list, b_list):
@@ -268,10 +276,11 @@ def bench_batched(args):
     cm, c32, c64 = res["mixed"][0], res["fp32"][0], res["fp64"][0]   # whole tsw_batch_sweep call (s)
     t32, t64 = res["fp32"][1] * 1e-3, res["fp64"][1] * 1e-3          # one batched sweep's kernels (s)
     b32 = U * 8 + ncell * 4
+    traffic, traffic_as = traffic_from_profiles("batched256 fp32")
     line = {
         "metric": "angular unknowns swept/s, mixed vs FP64 sweep at 1e-8 rel err; %HBM roofline",
-        "value": U / cm, "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": cm * 1e3, "higher_is_better": True, "scaling": "weak",
+        "value": U / cm, "unit": "angular unknowns swept/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": cm * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 sweep / f64 refinement",
         "data": "synthetic (seeded batch generated in-process)",
         "config": {"workload": "batched sweep: 256 uneven problems, N 32..256, Q1..Q3, S4..S12, 8x8 block "
@@ -283,13 +292,15 @@ def bench_batched(args):
                  "hbm_gbs": (U * 16 + ncell * 8) / t64 / 1e9},
         "speedup_mixed_vs_fp64": c64 / cm,
         "roofline": {"bound": "hbm", "achieved": b32 / t32 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b32 / t32 / 1e9 / HBM_PEAK_GBS, "kernel": "k_diag_sweep batched launches (FP32)",
-                     "algorithmic_bytes_per_launch": b32, "traffic": None},
+                     "frac": b32 / t32 / 1e9 / HBM_PEAK_GBS,
+                     "kernel": "k_diag_sweep batched launch groups (FP32), one batched sweep",
+                     "algorithmic_bytes_per_launch": b32, "traffic": traffic, "traffic_measured_as": traffic_as,
+                     "tuning": tuned},
     }
-    if args.cpu_sweeps > 0:
+    if cpu_sweeps > 0:
         mid = sorted(batch, key=lambda P: P.unknowns)[len(batch) // 2]
-        line["cpu_baseline"] = cpu_baseline(mid, args.cpu_sweeps)
-    print(json.dumps(line), flush=True)
+        line["cpu_baseline"] = cpu_baseline(mid, cpu_sweeps)
+    return line
 
 
 def traffic_from_profiles(family, band_rows=None):
@@ -321,6 +332,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
     ap.add_argument("--workload", default="lattice", choices=["lattice", "batched", "si", "idc"])
+    ap.add_argument("--batched-steps", type=int, default=2,
+                    help="lattice workload: timed calls per precision of the 256-problem batch reported in its "
+                         "'batched' sub-object (0: skip)")
     ap.add_argument("--idc-steps", type=int, default=10, help="IDC steps timed by --workload idc (100 reach t=1)")
     ap.add_argument("--band-rows", type=int, default=0, help="rows per workgroup band (0 automatic, -1 none)")
     ap.add_argument("--prefetch", type=int, default=0, help="prefetch depth of the staggered sweep (0 default)")
@@ -339,7 +353,7 @@ def main():
               f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
-        bench_batched(args)
+        print(json.dumps(bench_batched(args)), flush=True)
         return
     if args.workload == "si":
         bench_si(args)
@@ -501,6 +515,15 @@ def main():
         ref = cpu_baseline_reference(P.nx, args.cpu_sweeps)
         line["cpu_baseline"] = ref if ref else cpu_baseline(P, args.cpu_sweeps)
     ctx.close()
+    del work, d_src, d_st, d_ss, phi64, phi32
+    if args.batched_steps > 0:
+        # the batched configuration, where the mixed-precision targets are stated: the 256-problem batch
+        # timed in FP32, mixed and FP64 (its own sub-object; `value` stays the lattice solve's)
+        torch.cuda.empty_cache()
+        b = bench_batched(args, steps=args.batched_steps, warmup=1, cpu_sweeps=0)
+        line["batched"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "steps", "warmup", "dtype", "config",
+                                              "mixed_rel_err_vs_fp64", "fp32", "fp64", "speedup_mixed_vs_fp64",
+                                              "roofline")}
     print(json.dumps(line), flush=True)
 
 

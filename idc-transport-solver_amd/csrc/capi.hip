@@ -124,6 +124,16 @@ struct tsw_ctx {
     int tuned_br = INT_MIN, tuned_pf = INT_MIN, tuned_w1 = INT_MIN, tuned_sl = INT_MIN;
     int last_br = 0, last_pf = 0, last_w1 = 0, last_sl = 0;
     int one_wave_opt = 0;  // 1: one wave per band of 64 rows, lane-to-lane traces (tuning key one_wave)
+    // Ordinate-lane engine (sweep_lanes.hpp) on the reference layout: tuning key "lanes" (-1 off, 0
+    // automatic, 1 on where built), "lanes_block" (ordinates per row group, 0 automatic) and "lanes_waves"
+    // (waves per workgroup, 0 automatic); the ordinate-block table for the block size in use, the inflow
+    // traces and the per-block moment partials.
+    int lanes_opt = 0, lanes_J_opt = 0, lanes_W_opt = 0;
+    struct Lanes {
+        DBuf blk, mom, tinx64, tiny64, tinx32, tiny32, psi_keep;
+        int J = 0, nblk = 0;
+    } lanes;
+    int last_lanes = 0, last_lanes_J = 0, last_lanes_W = 0;
     int split_opt = 0;     // 1: 2 or 4 lanes per row on the material path (tuning key lane_split)
     // batch launch groups: -1 storer wave unless a table only fits without it, else (0, 1) direct stores
     // whenever every member has element-inverse tables (tuning key batch_direct)
@@ -149,8 +159,8 @@ struct tsw_ctx {
     // integration matrix W (row 0 over [0,1], row n over [t_{n-1}, t_n]) and the stage, residual and
     // correction vectors in the reference layout, FP64.
     struct Idc {
-        int num_nodes = 4, num_corr = 3, kind = TSW_IDC_LOBATTO;
-        bool collocation = false, left = true;
+        int num_nodes = 4, num_corr = 3, kind = TSW_IDC_LOBATTO, form = TSW_IDC_ERROR;
+        bool collocation = false, left = true, right = true;
         std::vector<double> nodes, W;  // S and S*S
         DBuf stages, res, err, phi;
         // adjoint: saved forward intermediates, stage adjoints, residual adjoints, work vectors,
@@ -1209,6 +1219,179 @@ bool staggered_ok(tsw_ctx *c) {
     return diag_sweep_lds_bytes<double>(probe, c->g.p) != 0;
 }
 
+// ---------------------------------------------------------------------------------------------------
+// Ordinate-lane engine: k_lanes_sweep on the caller's reference-layout arrays (no layout conversion).
+// ---------------------------------------------------------------------------------------------------
+// Most ordinates of one quadrant.
+int lanes_quadrant_max(const tsw_ctx *c) {
+    int m = 0;
+    for (int Q = 0; Q < 4; ++Q) m = std::max(m, c->g.quad_off[Q + 1] - c->g.quad_off[Q]);
+    return m;
+}
+
+// Row-group width J (ordinates per row group, one block of a quadrant each) and waves per workgroup W.
+// Automatic: each quadrant split into the fewest equal blocks of at most 7 ordinates (a wavefront then
+// holds 9 or more rows, so a band of four waves is 36 rows or more, and a row group still reads and
+// writes runs of J N values); four waves per workgroup, fewer when the mesh has fewer rows.
+std::pair<int, int> lanes_shape(const tsw_ctx *c) {
+    const int nQ = std::max(lanes_quadrant_max(c), 1);
+    int J = c->lanes_J_opt;
+    if (J <= 0) {
+        const int parts = (nQ + 6) / 7;
+        J = (nQ + parts - 1) / parts;
+    }
+    J = std::min(std::max(J, 1), 64);
+    const int RW = 64 / J;
+    int W = c->lanes_W_opt > 0 ? c->lanes_W_opt : 4;
+    W = std::min(W, std::max(1, (c->g.ny + RW - 1) / RW));
+    return {J, W};
+}
+
+// True when the ordinate-lane engine runs this context's sweeps: built for DG degrees 0-2 on the
+// material path of a staggered pattern, selected by the "lanes" tuning key.
+bool lanes_ok(tsw_ctx *c) {
+    if (c->lanes_opt <= 0 || !is_staggered(c->pattern) || c->pattern == Pattern::StaggeredGE || c->xs_deg > 0)
+        return false;
+    return c->g.p <= 2;
+}
+
+// Ordinate blocks of at most J ordinates, each quadrant split into equal parts (uploaded once per J).
+void lanes_blocks(tsw_ctx *c, int J) {
+    auto &L = c->lanes;
+    if (L.J == J && L.nblk > 0) return;
+    std::vector<LaneBlock> b;
+    for (int Q = 0; Q < 4; ++Q) {
+        const int q0 = c->g.quad_off[Q], n = c->g.quad_off[Q + 1] - q0;
+        if (n == 0) continue;
+        const int parts = (n + J - 1) / J;
+        for (int k = 0; k < parts; ++k) {
+            const int a = q0 + (int)((long long)n * k / parts), e = q0 + (int)((long long)n * (k + 1) / parts);
+            b.push_back({a, e - a, Q, 0});
+        }
+    }
+    TSW_HIP(hipMemcpyAsync(L.blk.reserve(sizeof(LaneBlock) * b.size()), b.data(), sizeof(LaneBlock) * b.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    TSW_HIP(hipStreamSynchronize(c->stream));  // b goes out of scope
+    L.J = J;
+    L.nblk = (int)b.size();
+}
+
+// Exchange records for `nbands` bands of the lane engine (shared with the staggered kernels, whose records
+// are all back to the sentinel after every sweep).
+template <typename T>
+T *lanes_records(tsw_ctx *c, int nbands) {
+    const Geom &g = c->g;
+    const size_t n = (size_t)g.nq * std::max(nbands - 1, 1) * g.nx * (g.p + 1);
+    DBuf &buf = sizeof(T) == 8 ? c->xtr64 : c->xtr32;
+    size_t &have = sizeof(T) == 8 ? c->xtr64_n : c->xtr32_n;
+    if (have < n) {
+        buf.reserve(n * sizeof(T));
+        have = n;
+        launch_exchange_fill<T>(buf.as<T>(), n, c->stream);
+    }
+    if (!c->abort_host) {
+        TSW_HIP(hipHostMalloc((void **)&c->abort_host, sizeof(int), hipHostMallocDefault));
+        *c->abort_host = 0;
+        c->abort_dev.reserve(sizeof(int));
+        TSW_HIP(hipMemsetAsync(c->abort_dev.ptr, 0, sizeof(int), c->stream));
+    }
+    if (nbands > 1) c->bands_used = true;
+    return buf.as<T>();
+}
+
+// One ordinate-lane sweep, timed: psi_out <- L^{-1}(src + sigma_s phi / (4 pi)) or, with mom, the
+// per-block moment partials of that solution.  src (reference layout with the inflow in its halos), phi and
+// sig_s (cell arrays) may each be null (not src and phi both); the material table must be valid.
+template <typename T>
+void lanes_sweep(tsw_ctx *c, double dt, const double *src, const double *phi, const double *sig_s, double *psi_out,
+                 double *mom) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    const bool f64 = sizeof(T) == 8;
+    const std::pair<int, int> shape = lanes_shape(c);
+    const int J = shape.first, W = shape.second;
+    lanes_blocks(c, J);
+    LanesParams<T> sp{};
+    sp.xi = f64 ? (const T *)c->xi64.ptr : (const T *)c->xi32.ptr;
+    sp.eta = f64 ? (const T *)c->eta64.ptr : (const T *)c->eta32.ptr;
+    sp.nx = g.nx; sp.ny = g.ny; sp.nq = g.nq;
+    sp.dx = T(g.dx); sp.dy = T(g.dy);
+    sp.blk = c->lanes.blk.as<LaneBlock>();
+    sp.nblk = c->lanes.nblk;
+    sp.J = J;
+    sp.RW = 64 / J;
+    sp.src = src;
+    if (src) {  // inflow traces of src's halos
+        const size_t nt_x = (size_t)g.nq * g.ny * (g.p + 1), nt_y = (size_t)g.nq * g.nx * (g.p + 1);
+        auto &L = c->lanes;
+        T *tx = dbuf<T>(f64 ? L.tinx64 : L.tinx32, nt_x), *ty = dbuf<T>(f64 ? L.tiny64 : L.tiny32, nt_y);
+        launch_inflow<T>(g, src, tx, ty, s);
+        sp.tin_x = tx;
+        sp.tin_y = ty;
+    }
+    sp.psi = psi_out;
+    require(c->mat.valid, TSW_E_STATE, "ordinate-lane sweep without a material table");
+    sp.mat = c->mat.idx.as<int32_t>();
+    sp.ainv = diag_inverses<T>(c, dt);
+    require(sp.ainv != nullptr, TSW_E_STATE, "ordinate-lane sweep without element inverses");
+    sp.M = c->mat.M;
+    sp.phi = phi;
+    sp.sig_s = phi ? sig_s : nullptr;
+    sp.mom = mom;
+    sp.w = c->w64.as<double>();
+    const int rows = W * sp.RW;
+    sp.band_rows = rows;
+    sp.nbands = (g.ny + rows - 1) / rows;
+    sp.xtr = lanes_records<T>(c, sp.nbands);
+    sp.abort_flag = c->abort_dev.as<int>();
+    sp.skip = c->skip_flag;
+    const int N = g.N();
+    auto lds_of = [&](bool table) {
+        const size_t vw = 16 / sizeof(T), ps = (g.p + 1 + vw - 1) / vw * vw;  // trace slot, 16-byte rows
+        size_t b = ((size_t)2 * rows * J * ps * sizeof(T) + 15) / 16 * 16;
+        if (mom) b += (size_t)2 * rows * N * J * sizeof(double);
+        if (table) b += ((size_t)sp.M * J * N * N * sizeof(T) + 15) / 16 * 16;
+        return b;
+    };
+    sp.inv_lds = lds_of(true) <= 96 * 1024 ? 1 : 0;
+    const size_t lds = lds_of(sp.inv_lds != 0);
+    require(lds <= 160 * 1024, TSW_E_UNSUPPORTED, "ordinate-lane sweep: LDS layout too large");
+    if (c->stall_test && sp.nbands > 1 && !c->skip_flag) {
+        c->stall_test = false;
+        *c->abort_host = 1;
+        TSW_HIP(hipMemcpyAsync(c->abort_dev.ptr, c->abort_host, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    c->last_lanes = 1;
+    c->last_lanes_J = J;
+    c->last_lanes_W = W;
+    c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
+    auto launch = [&] {
+        switch (g.p) {
+            case 0: launch_lanes_sweep_p<T, 1>(sp, W * 64, lds, s); break;
+            case 1: launch_lanes_sweep_p<T, 2>(sp, W * 64, lds, s); break;
+            case 2: launch_lanes_sweep_p<T, 3>(sp, W * 64, lds, s); break;
+            default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
+        }
+    };
+    if (c->skip_flag) {  // inside a graph capture: the replay is timed as a whole
+        launch();
+        return;
+    }
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, s));
+    launch();
+    TSW_HIP(hipEventRecord(e1, s));
+    ++c->sweeps_done;
+}
+
+// In-place sweep of a reference-layout device array on the ordinate-lane engine, computing in precision T.
+template <typename T>
+void apply_lanes(tsw_ctx *c, double *psi_ref, const double *sigma_t, double dt) {
+    prepare_materials(c, sigma_t, dt);
+    require(c->mat.valid, TSW_E_UNSUPPORTED, "ordinate-lane sweep needs at most 64 distinct sigma_t values");
+    lanes_sweep<T>(c, dt, psi_ref, nullptr, nullptr, psi_ref, nullptr);
+}
+
 // In-place sweep of a reference-layout device array in precision T.
 template <typename T>
 void apply_staggered(tsw_ctx *c, double *psi_ref, const double *sigma_t, double dt) {
@@ -1645,9 +1828,99 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
     return converged;
 }
 
+// Moment partials of the ordinate-lane engine for its current block shape.
+double *lanes_mom(tsw_ctx *c) {
+    lanes_blocks(c, lanes_shape(c).first);
+    return dbuf<double>(c->lanes.mom, (size_t)c->lanes.nblk * c->g.nphi());
+}
+
+// Peierls GMRES on the ordinate-lane engine (the material table must be valid).  The same algorithm as
+// gmres_staggered on the reference layouts themselves: the Arnoldi products sweep the scattering source
+// of x straight into per-block moments (no angular flux is written), b = P L^{-1} Q likewise, and the final
+// psi = L^{-1}(Q + S phi) is swept in place over Q.  FP32 (relaxed) as in gmres_staggered.
+bool gmres_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, double dt, double abs_tol, int max_its,
+                 int restart, int precision, double *phi_out, int &its_total, double &res_final) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t n = g.nphi();
+    DBuf vecs;
+    double *b = dbuf<double>(vecs, 3 * n);
+    double *x = b + n, *r = x + n;
+    TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * 3 * n, s));
+    Gmres solver(c, n, restart);
+    bool converged = false;
+    its_total = 0;
+    const bool dsa = c->use_dsa;
+    if (dsa) dsa_build(c, st, ss, dt);
+    double *mom = lanes_mom(c);
+    const int nblk = c->lanes.nblk;
+    auto apply64 = [&](const double *xv, double *yv) {  // (I - P L^{-1} S) x, zero inflow (:548-555)
+        lanes_sweep<double>(c, dt, nullptr, xv, ss, nullptr, mom);
+        launch_mom_sum(g, nblk, mom, xv, yv, -1.0, 1.0, s);
+        if (dsa) dsa_apply(c, yv, yv);
+    };
+    auto apply32 = [&](const double *xv, double *yv) {
+        lanes_sweep<float>(c, dt, nullptr, xv, ss, nullptr, mom);
+        launch_mom_sum(g, nblk, mom, xv, yv, -1.0, 1.0, s);
+        if (dsa) dsa_apply(c, yv, yv);
+    };
+    lanes_sweep<double>(c, dt, Q, nullptr, nullptr, nullptr, mom);  // b = P L^{-1} Q (:461-466)
+    launch_mom_sum(g, nblk, mom, b, b, 1.0, 0.0, s);
+    if (dsa) dsa_apply(c, b, b);
+    if (precision == TSW_FP64) {
+        converged = solver.solve(apply64, b, x, r, abs_tol, max_its, its_total, res_final, true);
+        lanes_sweep<double>(c, dt, Q, x, ss, Q, nullptr);  // psi = L^{-1}(Q + S phi) (:504-505)
+    } else {
+        auto &fe = c->fp32_err;
+        if (!(fe.value >= 0.0 && fe.mat_build == c->mat.build && fe.inv_dt == inverse_dt(dt) && fe.dsa == dsa)) {
+            const double nb = solver.norm(b);
+            double d = kFp32ErrFloor;
+            if (nb > 0.0) {
+                apply64(b, x);
+                apply32(b, r);
+                launch_axpby(n, -1.0, x, 1.0, r, s);  // r = A32 b - A64 b
+                d = solver.norm(r) / nb;
+                TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+            }
+            fe = {std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d)), c->mat.build, inverse_dt(dt), dsa};
+        }
+        const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
+        converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
+        // the true residual M (P psi - x) from the final FP64 psi; resume while it exceeds abs_tol
+        const double *q_src = Q;
+        if (solver.used_lo) {  // keep Q: the check may need further sweeps of it
+            double *keep = dbuf<double>(c->lanes.psi_keep, g.npsi());
+            TSW_HIP(hipMemcpyAsync(keep, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+            q_src = keep;
+        }
+        for (int check = 0;; ++check) {
+            lanes_sweep<double>(c, dt, q_src, x, ss, Q, nullptr);
+            if (!solver.used_lo) break;
+            launch_pmv<double, double>(g, Q, r, c->w64.as<double>(), 1.0, 0.0, 0, s);
+            launch_axpby(n, -1.0, x, 1.0, r, s);
+            if (dsa) dsa_apply(c, r, r);
+            res_final = solver.norm(r);
+            converged = res_final <= abs_tol;
+            if (converged || its_total >= max_its || check >= 3) break;
+            int its = 0;
+            double rr = 0.0;
+            converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its - its_total, its, rr, false);
+            its_total += its;
+        }
+    }
+    if (phi_out) TSW_HIP(hipMemcpyAsync(phi_out, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    return converged;
+}
+
 // Peierls GMRES on device arrays with the context's engine.
 bool gmres_device(tsw_ctx *ctx, double *Q, const double *st, const double *ss, double dt, double abs_tol, int max_its,
                   int restart, int precision, double *pd, int &its_total, double &res_final) {
+    ctx->last_lanes = 0;
+    if (lanes_ok(ctx) && (precision == TSW_FP64 || !ctx->gmres_refine)) {
+        prepare_materials(ctx, st, dt);
+        if (ctx->mat.valid)
+            return gmres_lanes(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
+    }
     if (staggered_ok(ctx))
         return gmres_staggered(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
     return gmres_wavefront(ctx, Q, st, ss, dt, abs_tol, max_its, restart, precision, pd, its_total, res_final);
@@ -1682,7 +1955,11 @@ void sweep_wavefront_mixed(tsw_ctx *ctx, double *Q, const double *st, double dt)
 
 void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int precision) {
     require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG, "bad precision");
-    if (staggered_ok(ctx)) {
+    ctx->last_lanes = 0;
+    if (lanes_ok(ctx) && precision != TSW_MIXED) {
+        if (precision == TSW_FP64) apply_lanes<double>(ctx, psi, sig, dt);
+        else apply_lanes<float>(ctx, psi, sig, dt);
+    } else if (staggered_ok(ctx)) {
         if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
         else if (precision == TSW_FP32) apply_staggered<float>(ctx, psi, sig, dt);
         else apply_staggered_mixed(ctx, psi, sig, dt);
@@ -1755,68 +2032,16 @@ Tpi tpi_table(int p) {
     return t;
 }
 
-// Gauss-Lobatto nodes on [-1, 1]: the endpoints and the roots of P'_{m-1} (Newton with P'' from the
-// Legendre equation)
-std::vector<double> gauss_lobatto(int m) {
-    std::vector<double> x(m);
-    x[0] = -1.0;
-    x[m - 1] = 1.0;
-    const int n = m - 1;
-    for (int k = 1; k < m - 1; ++k) {
-        double t = -std::cos(M_PI * k / n);
-        for (int it = 0; it < 100; ++it) {
-            double p, dp;
-            legendre_pd(n, t, p, dp);
-            const double d2p = (2.0 * t * dp - n * (n + 1.0) * p) / (1.0 - t * t);
-            const double dt = dp / d2p;
-            t -= dt;
-            if (std::fabs(dt) < 1e-16) break;
-        }
-        x[k] = t;
-    }
-    return x;
-}
-
-// integral over [a, b] of the Lagrange basis polynomials of `nodes`
-std::vector<double> lagrange_integrals(const std::vector<double> &nodes, double a, double b) {
-    const int m = (int)nodes.size();
-    std::vector<double> xg, wg, out(m, 0.0);
-    gauss_legendre(m + 2, xg, wg);
-    for (size_t k = 0; k < xg.size(); ++k) {
-        const double t = 0.5 * (b - a) * (xg[k] + 1.0) + a;
-        for (int l = 0; l < m; ++l) {
-            double v = 1.0;
-            for (int j = 0; j < m; ++j)
-                if (j != l) v *= (t - nodes[j]) / (nodes[l] - nodes[j]);
-            out[l] += 0.5 * (b - a) * wg[k] * v;
-        }
-    }
-    return out;
-}
-
-// Nodes and integration matrix (SetNodes, idc.cpp:2305-2376)
+// Nodes and integration matrix (SetNodes, idc.cpp:2305-2376), from tsw_idc_nodes
 void idc_tables(tsw_ctx::Idc &I) {
-    std::vector<double> nodes;
-    if (I.kind == TSW_IDC_LOBATTO) {
-        nodes = gauss_lobatto(I.num_nodes);
-        I.left = true;
-    } else {
-        std::vector<double> w;
-        gauss_legendre(I.num_nodes, nodes, w);
-        I.left = false;
-    }
-    for (double &t : nodes) t = 0.5 * (t + 1.0);
-    I.nodes.clear();
-    if (!I.left) I.nodes.push_back(0.0);
-    I.nodes.insert(I.nodes.end(), nodes.begin(), nodes.end());
-    const int S = (int)I.nodes.size(), off = I.left ? 0 : 1;
+    int S = 0;
+    require(tsw_idc_nodes(I.kind, I.num_nodes, 0, &S, nullptr, nullptr) == TSW_OK, TSW_E_ARG, "bad IDC nodes");
+    I.nodes.assign(S, 0.0);
     I.W.assign((size_t)S * S, 0.0);
-    std::vector<double> row = lagrange_integrals(nodes, 0.0, 1.0);
-    for (size_t l = 0; l < nodes.size(); ++l) I.W[off + l] = row[l];
-    for (int n = 1; n < S; ++n) {
-        row = lagrange_integrals(nodes, I.nodes[n - 1], I.nodes[n]);
-        for (size_t l = 0; l < nodes.size(); ++l) I.W[(size_t)n * S + off + l] = row[l];
-    }
+    require(tsw_idc_nodes(I.kind, I.num_nodes, S, &S, I.nodes.data(), I.W.data()) == TSW_OK, TSW_E_ARG,
+            "bad IDC nodes");
+    I.left = I.kind == TSW_IDC_LOBATTO;
+    I.right = I.kind == TSW_IDC_LOBATTO || I.kind == TSW_IDC_RADAU || I.kind == TSW_IDC_CHEBYSHEV_RADAU;
 }
 
 // R = L x - S P x on the interior (ComputeResiduals, idc.cpp:1658-1667); R's halos stay zero
@@ -1868,23 +2093,35 @@ bool idc_step_device(tsw_ctx *c, double *psi, const double *src, const double *s
         launch_lincomb(g, K, L, stage(k), s);
         ok = idc_solve(c, stage(k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
     }
-    // corrections in error form (:882-930, ComputeSource :1686-1740)
+    // corrections (:882-930, ComputeSource :1686-1740)
     for (int p = 0; p < I.num_corr; ++p) {
         for (int l = l0; l < S; ++l) idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
         for (int k = 1; k < S; ++k) {
             const double h = I.nodes[k] - I.nodes[k - 1];
-            LinComb L;  // e = -stage(k)/(h dt) - sum_l W(k,l) R_l / h + stage(k-1)/(h dt) + Q, zero halos
-            L.add(-1.0 / (h * dt), stage(k));
-            for (int l = l0; l < S; ++l) L.add(-I.W[(size_t)k * S + l] / h, R + (size_t)l * n);
-            L.add(1.0 / (h * dt), stage(k - 1));
-            L.add(1.0, src);
-            launch_lincomb(g, K, L, e, s);
-            ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
-            launch_axpby(n, 1.0, e, 1.0, stage(k), s);
+            LinComb L;
+            if (I.form == TSW_IDC_ERROR) {
+                // e = -stage(k)/(h dt) - sum_l W(k,l) R_l / h + stage(k-1)/(h dt) + Q, zero halos; stage(k) += e
+                L.add(-1.0 / (h * dt), stage(k));
+                for (int l = l0; l < S; ++l) L.add(-I.W[(size_t)k * S + l] / h, R + (size_t)l * n);
+                L.add(1.0 / (h * dt), stage(k - 1));
+                L.add(1.0, src);
+                launch_lincomb(g, K, L, e, s);
+                ok = idc_solve(c, e, st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+                launch_axpby(n, 1.0, e, 1.0, stage(k), s);
+            } else {
+                // stage(k) = sum_l (delta_kl - W(k,l)/h) R_l + stage(k-1)/(h dt) on the interior, + Q with
+                // its inflow halos, then solved in place (:1711-1730, :909-911)
+                for (int l = l0; l < S; ++l)
+                    L.add((l == k ? 1.0 : 0.0) - I.W[(size_t)k * S + l] / h, R + (size_t)l * n);
+                L.add(1.0 / (h * dt), stage(k - 1));
+                L.add(1.0, src, true);
+                launch_lincomb(g, K, L, stage(k), s);
+                ok = idc_solve(c, stage(k), st, ss, h * dt, tol, max_its, precision, its_total) && ok;
+            }
         }
     }
     // result: the last node, or the collocation formula (:933-939, ComputeCollocation_Nonhybrid :1541-1566)
-    if (!I.left || I.collocation) {
+    if (!I.right || I.collocation) {
         // psi = psi0 + dt Q + dt sum_l W(0,l) (S P - L) psi_l (its halos are zeroed below)
         for (int l = l0; l < S; ++l) idc_residual(c, stage(l), R + (size_t)l * n, st, ss, phi);
         LinComb L;
@@ -1924,6 +2161,7 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     ctx->dsa.keyed = false;
     ctx->mat.valid = ctx->mat.keyed = ctx->mat.have64 = ctx->mat.have32 = false;
     ++ctx->mat.build;
+    ctx->lanes.J = ctx->lanes.nblk = 0;
     ctx->xs_deg = 0;
     ctx->xs.key[0] = ctx->xs.key[1] = nullptr;
     Geom &g = ctx->g;
@@ -2109,7 +2347,7 @@ bool idc_step_vjp_device(tsw_ctx *c, const double *psi_n, const double *src, con
     const int S = (int)I.nodes.size(), P = I.num_corr, l0 = I.left ? 0 : 1;
     const size_t n = g.npsi(), K = (size_t)g.nq * g.N();
     const bool grads = gst || gss;
-    const bool colloc = !I.left || I.collocation;
+    const bool colloc = !I.right || I.collocation;
     bool ok = true;
     its_total = 0;
     // saved forward values: levels 0..P of all stages, then the corrections e[p][k], p = 1..P
@@ -2297,6 +2535,15 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
             require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
                     "prefetch must be 0, 2, 3, 4 or 6");
             ctx->prefetch_opt = value;
+        } else if (k == "lanes") {
+            require(value >= -1 && value <= 1, TSW_E_ARG, "lanes must be -1, 0 or 1");
+            ctx->lanes_opt = value;
+        } else if (k == "lanes_block") {
+            require(value >= 0 && value <= 64, TSW_E_ARG, "lanes_block must be 0..64");
+            ctx->lanes_J_opt = value;
+        } else if (k == "lanes_waves") {
+            require(value >= 0 && value <= 4, TSW_E_ARG, "lanes_waves must be 0..4");
+            ctx->lanes_W_opt = value;
         } else if (k == "stall_test") {
             require(value == 0 || value == 1, TSW_E_ARG, "stall_test must be 0 or 1");
             ctx->stall_test = value != 0;
@@ -2321,6 +2568,9 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "one_wave") *value = ctx->last_w1;
         else if (k == "lane_split") *value = ctx->last_sl;
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;
+        else if (k == "lanes") *value = ctx->last_lanes;
+        else if (k == "lanes_block") *value = ctx->last_lanes_J;
+        else if (k == "lanes_waves") *value = ctx->last_lanes_W;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }
@@ -2599,7 +2849,8 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
 
 int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type, int use_collocation) {
     return guarded(ctx, [&] {
-        require(nodes_type == TSW_IDC_LOBATTO || nodes_type == TSW_IDC_LEGENDRE, TSW_E_ARG, "unknown IDC node type");
+        require(nodes_type >= TSW_IDC_LOBATTO && nodes_type <= TSW_IDC_CHEBYSHEV_RADAU, TSW_E_ARG,
+                "unknown IDC node type");
         require(num_nodes >= (nodes_type == TSW_IDC_LOBATTO ? 2 : 1) && num_nodes <= 12, TSW_E_ARG,
                 "IDC node count out of range");
         require(num_corrections >= 0 && num_corrections <= 32, TSW_E_ARG, "IDC correction count out of range");
@@ -2609,6 +2860,13 @@ int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type
         I.kind = nodes_type;
         I.collocation = use_collocation != 0;
         idc_tables(I);
+    });
+}
+
+int tsw_idc_set_type(tsw_ctx *ctx, int idc_type) {
+    return guarded(ctx, [&] {
+        require(idc_type == TSW_IDC_ERROR || idc_type == TSW_IDC_UPDATE, TSW_E_ARG, "unknown IDC type");
+        ctx->idc.form = idc_type;
     });
 }
 

@@ -25,6 +25,32 @@ inline Domain domain(const Geom &g, int all_cells) {
 }
 }  // namespace
 
+// ---- Moments from the ordinate-lane sweep's block partials: y = beta xb + alpha sum_b mom[b] on the
+// interior (blocks summed in order), halos of y untouched.  The Pmv of a GMRES matvec whose sweep wrote
+// the partials instead of psi.
+__global__ __launch_bounds__(kBlock) void k_mom_sum(const double *__restrict__ mom, int nblk, size_t stride,
+                                                    const double *xb, double *y, double alpha, double beta, int nx,
+                                                    int ny, int N, const int *skip) {
+    if (skip && *skip) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nx * ny * N) return;
+    const size_t cell = t / N;
+    const int k = (int)(t - cell * N);
+    const int i = (int)(cell / ny) + 1, j = (int)(cell % ny) + 1;
+    const size_t at = ((size_t)i * (ny + 2) + j) * N + k;
+    double acc = mom[at];
+    for (int b = 1; b < nblk; ++b) acc += mom[(size_t)b * stride + at];
+    y[at] = beta * xb[at] + alpha * acc;
+}
+
+void launch_mom_sum(const Geom &g, int nblk, const double *mom, const double *xb, double *y, double alpha,
+                    double beta, hipStream_t s, const int *skip) {
+    const size_t n = (size_t)g.nx * g.ny * g.N();
+    hipLaunchKernelGGL(k_mom_sum, dim3(grid_for(n)), dim3(kBlock), 0, s, mom, nblk, g.nphi(), xb, y, alpha, beta,
+                       g.nx, g.ny, g.N(), skip);
+    TSW_HIP(hipGetLastError());
+}
+
 // ---- Pmv: phi = beta*phi + alpha*sum_q w_q psi_q (TransportOperator.cpp:140-158) ----
 // TI: storage type of psi; TO: type of phi, w and the accumulation.
 template <typename TI, typename TO>

@@ -402,3 +402,122 @@ extern "C" int tsw_ordinates(const char *type, int order, int symmetric_reduce, 
         return TSW_E_ARG;
     }
 }
+
+// ---------------------------------------------------------------------------------------------------
+// IDC quadrature nodes and integration matrices (IDCIntegrator::SetNodes, idc.cpp:2305-2376, over
+// Quadrule::ComputeQuadratureNodes, Quadrule.cpp:1077-1145, and ComputeQuadratureWeights).
+namespace {
+
+using qreal = __float128;
+
+// Newton's method in quadruple precision: f(t, v, dv) evaluates the function and its derivative.
+template <typename F>
+qreal newton_q(F &&f, qreal r) {
+    for (int it = 0; it < 100; ++it) {
+        qreal v, dv;
+        f(r, v, dv);
+        const qreal dx = v / dv;
+        r -= dx;
+        if (std::fabs((double)dx) < 1e-30) break;
+    }
+    return r;
+}
+
+// The m nodes of a family on [-1, 1], ascending, in quadruple precision:
+//   Gauss-Lobatto: -1, the roots of P'_{m-1}, 1 (P'' from the Legendre equation);
+//   Gauss-Legendre: the roots of P_m;
+//   Gauss-Radau IIA: the roots of P_m - P_{m-1} (Quadrule::Radau, Quadrule.cpp:412-446, is this
+//     polynomial up to a factor), the last of which is 1; Newton from the Chebyshev-Radau points;
+//   Chebyshev cos((2i+1) pi / 2m) and Chebyshev-Radau cos(2 i pi / (2m - 1)), i = m-1..0, evaluated in
+//     double as the reference does.
+std::vector<qreal> idc_family(int kind, int m) {
+    std::vector<qreal> x(m);
+    switch (kind) {
+    case TSW_IDC_LOBATTO: {
+        const int n = m - 1;
+        x[0] = -1;
+        x[m - 1] = 1;
+        for (int k = 1; k < m - 1; ++k)
+            x[k] = newton_q([n](qreal t, qreal &v, qreal &dv) {
+                qreal p, dp;
+                legendre_q(n, t, p, dp);
+                v = dp;
+                dv = (2 * t * dp - qreal(n) * (n + 1) * p) / (1 - t * t);
+            }, -std::cos(M_PI * k / n));
+        break;
+    }
+    case TSW_IDC_LEGENDRE:
+        for (int k = 0; k < m; ++k)
+            x[k] = newton_q([m](qreal t, qreal &v, qreal &dv) { legendre_q(m, t, v, dv); },
+                            -std::cos(M_PI * (k + 0.75) / (m + 0.5)));
+        break;
+    case TSW_IDC_RADAU:
+        x[m - 1] = 1;
+        for (int k = 0; k < m - 1; ++k)
+            x[k] = newton_q([m](qreal t, qreal &v, qreal &dv) {
+                qreal p1, dp1, p0, dp0;
+                legendre_q(m, t, p1, dp1);
+                legendre_q(m - 1, t, p0, dp0);
+                v = p1 - p0;
+                dv = dp1 - dp0;
+            }, std::cos(2.0 * M_PI * (m - 1 - k) / (2 * m - 1)));
+        break;
+    case TSW_IDC_CHEBYSHEV:
+        for (int i = 0; i < m; ++i) x[m - 1 - i] = std::cos((double)(2 * i + 1) * M_PI / (2 * m));
+        break;
+    case TSW_IDC_CHEBYSHEV_RADAU:
+        for (int i = 0; i < m; ++i) x[m - 1 - i] = std::cos(2 * i * M_PI / (2 * m - 1));
+        break;
+    default:
+        throw std::invalid_argument("node type");
+    }
+    return x;
+}
+
+// integrals over [a, b] of the Lagrange basis polynomials of `t` (degree m-1), by an (m+1)-point
+// Gauss-Legendre rule (exact for degree 2m+1), in quadruple precision
+std::vector<qreal> lagrange_integrals_q(const std::vector<qreal> &t, qreal a, qreal b) {
+    const int m = (int)t.size(), ng = m + 1;
+    const std::vector<qreal> xg = idc_family(TSW_IDC_LEGENDRE, ng);
+    std::vector<qreal> out(m, 0);
+    for (int k = 0; k < ng; ++k) {
+        qreal p, dp;
+        legendre_q(ng, xg[k], p, dp);
+        const qreal wg = 2 / ((1 - xg[k] * xg[k]) * dp * dp);
+        const qreal s = (b - a) / 2 * (xg[k] + 1) + a;
+        for (int l = 0; l < m; ++l) {
+            qreal v = 1;
+            for (int j = 0; j < m; ++j)
+                if (j != l) v *= (s - t[j]) / (t[l] - t[j]);
+            out[l] += (b - a) / 2 * wg * v;
+        }
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" int tsw_idc_nodes(int nodes_type, int num_nodes, int capacity, int *num_stages, double *nodes, double *W) {
+    const bool left = nodes_type == TSW_IDC_LOBATTO;
+    if (nodes_type < TSW_IDC_LOBATTO || nodes_type > TSW_IDC_CHEBYSHEV_RADAU || !num_stages) return TSW_E_ARG;
+    if (num_nodes < (left ? 2 : 1) || num_nodes > 12) return TSW_E_ARG;
+    const int S = num_nodes + (left ? 0 : 1), off = left ? 0 : 1;
+    *num_stages = S;
+    if (!nodes) return TSW_OK;
+    if (!W || capacity < S) return TSW_E_ARG;
+    try {
+        std::vector<qreal> t = idc_family(nodes_type, num_nodes);
+        for (qreal &v : t) v = (v + 1) / 2;  // to [0, 1]
+        std::vector<qreal> st(S, 0);
+        std::copy(t.begin(), t.end(), st.begin() + off);  // stage 0 is t = 0 where no node lies there
+        for (int i = 0; i < S; ++i) nodes[i] = (double)st[i];
+        std::fill(W, W + (size_t)S * S, 0.0);
+        for (int n = 0; n < S; ++n) {
+            const std::vector<qreal> row = lagrange_integrals_q(t, n == 0 ? qreal(0) : st[n - 1], n == 0 ? qreal(1) : st[n]);
+            for (int l = 0; l < num_nodes; ++l) W[(size_t)n * S + off + l] = (double)row[l];
+        }
+        return TSW_OK;
+    } catch (...) {
+        return TSW_E_ARG;
+    }
+}

@@ -136,6 +136,38 @@ template <typename T> struct DiagSweepParams {
     const int *skip;       // device flag: the launch does nothing while *skip != 0 (null: always runs)
 };
 
+// Ordinate-lane sweep on the reference layout (sweep_lanes.hpp).  An ordinate block is ordinates
+// [q0, q0 + n) of quadrant `quad`; a row group of a wavefront holds one block's ordinates of one row.
+struct LaneBlock {
+    int q0, n, quad, pad;
+};
+template <typename T> struct LanesParams {
+    const T *xi, *eta;          // [nq]
+    int nx, ny, nq;
+    T dx, dy;
+    const LaneBlock *blk;       // [nblk]
+    int nblk, J, RW;            // lanes per row group (>= every block's n), row groups per wave (RW * J <= 64)
+    const double *src;          // bit 1: Q [nx+2][ny+2][nq][N]
+    const T *tin_x, *tin_y;     // with src: inflow traces [nq][ny][P], [nq][nx][P] of its halos
+    double *psi;                // out unless mom: interior of [nx+2][ny+2][nq][N] (may alias src)
+    const int32_t *mat;         // [nx+2][ny+2]
+    const T *ainv;              // [M][nq][N][N]: dx dy A^{-1}
+    int M, inv_lds;             // inv_lds: stage the blocks' tables in LDS
+    const double *phi;          // bit 2: [nx+2][ny+2][N]
+    const double *sig_s;        // bit 2: [nx+2][ny+2]
+    double *mom;                // bit 4: [nblk][nx+2][ny+2][N], sum over the block of w_q psi_q (interior)
+    const double *w;            // bit 4: [nq]
+    int band_rows, nbands;      // rows per band (a workgroup of band_rows / RW waves) and bands per block
+    T *xtr;                     // [nq][nbands-1][nx][P] band exchange records
+    int *abort_flag;
+    const int *skip;            // device flag: the launch does nothing while *skip != 0 (null: always runs)
+};
+template <typename T, int P>
+void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s);
+// y = beta xb + alpha sum_b mom[b] on the interior of phi-layout arrays (mom: nblk partials of g.nphi())
+void launch_mom_sum(const Geom &g, int nblk, const double *mom, const double *xb, double *y, double alpha,
+                    double beta, hipStream_t s, const int *skip = nullptr);
+
 // Device-side state of a source iteration replayed from a graph (SISolver.cpp:219-241).
 struct SiControl {
     int done;     // set when the loop test fails; every kernel of an iteration then does nothing

@@ -3,6 +3,7 @@
 
 #include <array>
 #include <cmath>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 
@@ -242,14 +243,28 @@ IDCIntegrator::IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &p
     if (p.Has("max_its")) max_its_ = p.GetInt("max_its");
     const int nodes = p.Has("idc_num_nodes") ? (int)p.GetInt("idc_num_nodes") : 4;
     const int corr = p.Has("idc_num_corr") ? (int)p.GetInt("idc_num_corr") : 3;
-    int type = TSW_IDC_LOBATTO;
+    // the reference's String_to_NodesType (Quadrule.hpp:62-70) and String_to_IDCType (idc.cpp:44-50)
+    static const std::map<std::string, int> kNodes = {
+        {"gauss-lobatto", TSW_IDC_LOBATTO}, {"gauss-legendre", TSW_IDC_LEGENDRE}, {"gauss-radau", TSW_IDC_RADAU},
+        {"chebyshev", TSW_IDC_CHEBYSHEV}, {"chebyshev-radau", TSW_IDC_CHEBYSHEV_RADAU}};
+    static const std::map<std::string, int> kTypes = {{"error", TSW_IDC_ERROR}, {"update", TSW_IDC_UPDATE}};
+    int type = TSW_IDC_LOBATTO, form = TSW_IDC_ERROR;
     if (p.Has("idc_nodes_type")) {
         const std::string t = p.GetString("idc_nodes_type");
-        if (t == "gauss-lobatto") type = TSW_IDC_LOBATTO;
-        else if (t == "gauss-legendre") type = TSW_IDC_LEGENDRE;
-        else throw std::out_of_range("idc_nodes_type '" + t + "' is not built");
+        auto it = kNodes.find(t);
+        if (it == kNodes.end()) throw std::out_of_range("idc_nodes_type '" + t + "' is not built");
+        type = it->second;
     }
-    const int rc = tsw_idc_set(dev_->ctx(), nodes, corr, type, flag_from(p, "idc_use_collocation", false) ? 1 : 0);
+    if (p.Has("idc_type")) {
+        const std::string t = p.GetString("idc_type");
+        auto it = kTypes.find(t);
+        if (it == kTypes.end()) throw std::out_of_range("idc_type '" + t + "' is not built");
+        form = it->second;
+    }
+    // idc_use_collocation is read only for node families holding the right end (idc.cpp:225-229); the
+    // others always use collocation
+    int rc = tsw_idc_set(dev_->ctx(), nodes, corr, type, flag_from(p, "idc_use_collocation", false) ? 1 : 0);
+    if (rc == TSW_OK) rc = tsw_idc_set_type(dev_->ctx(), form);
     if (rc == TSW_E_ARG) throw std::invalid_argument(tsw_last_error(dev_->ctx()));
     dev_->Check(rc, "tsw_idc_set");
 }

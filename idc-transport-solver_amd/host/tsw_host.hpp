@@ -278,9 +278,10 @@ class PeierlsGMRESSolver : public ImplicitSolver {
     int precision_ = TSW_FP64;
 };
 
-// IDC time integrator, the non-hybrid error-form IDCIntegrator (time_integrators/RKDG/idc.cpp:846-940)
-// over the same device.  Parameters: idc_num_nodes (4), idc_num_corr (3), idc_nodes_type
-// ("gauss-lobatto" default, "gauss-legendre"), idc_use_collocation (false), abs_tol (1e-10) and max_its
+// IDC time integrator, the non-hybrid IDCIntegrator (time_integrators/RKDG/idc.cpp:846-940) over the same
+// device.  Parameters: idc_num_nodes (4), idc_num_corr (3), idc_type ("error" default, "update"),
+// idc_nodes_type ("gauss-lobatto" default, "gauss-legendre", "gauss-radau", "chebyshev",
+// "chebyshev-radau"), idc_use_collocation (false), abs_tol (1e-10) and max_its
 // (3000) of the implicit solves, sweep_precision.  Step advances initial_condition by dt in place
 // (halos zero), with the material source and boundary inflow in `source`, constant over the step.
 class IDCIntegrator {

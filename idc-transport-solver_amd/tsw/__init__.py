@@ -32,6 +32,15 @@ MEM_HOST = 0
 MEM_DEVICE = 1
 IDC_LOBATTO = 0
 IDC_LEGENDRE = 1
+IDC_RADAU = 2
+IDC_CHEBYSHEV = 3
+IDC_CHEBYSHEV_RADAU = 4
+IDC_ERROR = 0
+IDC_UPDATE = 1
+# the reference's idc_nodes_type / idc_type strings (Quadrule.hpp:47-70, idc.cpp:44-50)
+IDC_NODES_TYPES = {"gauss-lobatto": IDC_LOBATTO, "gauss-legendre": IDC_LEGENDRE, "gauss-radau": IDC_RADAU,
+                   "chebyshev": IDC_CHEBYSHEV, "chebyshev-radau": IDC_CHEBYSHEV_RADAU}
+IDC_TYPES = {"error": IDC_ERROR, "update": IDC_UPDATE}
 
 
 class TswProblem(ctypes.Structure):
@@ -90,6 +99,8 @@ def lib():
         L.tsw_sweep_vjp.argtypes = [c, _vp, _vp, d, _vp, _vp, _vp, i, i]
         L.tsw_solve_vjp.argtypes = [c, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, d, i, i, i, i, _ip, _dp]
         L.tsw_idc_set.argtypes = [c, i, i, i, i]
+        L.tsw_idc_set_type.argtypes = [c, i]
+        L.tsw_idc_nodes.argtypes = [i, i, i, _ip, _dp, _dp]
         L.tsw_idc_step.argtypes = [c, _vp, _vp, _vp, _vp, d, d, i, i, i]
         L.tsw_idc_run.argtypes = [c, _vp, _vp, _vp, _vp, d, i, d, i, i, i]
         L.tsw_idc_adjoint.argtypes = [c, _vp, _vp, _vp, _vp, d, i, i, _vp, _vp, _vp, _vp, d, i, i, i]
@@ -119,6 +130,20 @@ def level_symmetric(order):
     if rc != TSW_OK:
         raise TswError(rc, f"no level-symmetric set of order {order}")
     return xi, eta, w, off
+
+
+def idc_nodes(nodes_type, num_nodes):
+    """(stage nodes on [0, 1], integration matrix W) of an IDC configuration (host-only)."""
+    S = ctypes.c_int(0)
+    rc = lib().tsw_idc_nodes(nodes_type, num_nodes, 0, ctypes.byref(S), None, None)
+    if rc != TSW_OK:
+        raise TswError(rc, f"no IDC nodes of type {nodes_type} and count {num_nodes}")
+    nodes, W = np.zeros(S.value), np.zeros((S.value, S.value))
+    rc = lib().tsw_idc_nodes(nodes_type, num_nodes, S.value, ctypes.byref(S), nodes.ctypes.data_as(_dp),
+                             W.ctypes.data_as(_dp))
+    if rc != TSW_OK:
+        raise TswError(rc, f"no IDC nodes of type {nodes_type} and count {num_nodes}")
+    return nodes, W
 
 
 def ordinates(ordinate_type, order, symmetric_reduce=True):
@@ -327,8 +352,10 @@ class Context:
                                           mem, ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
 
-    def idc_set(self, num_nodes=4, num_corrections=3, nodes_type=IDC_LOBATTO, use_collocation=False):
+    def idc_set(self, num_nodes=4, num_corrections=3, nodes_type=IDC_LOBATTO, use_collocation=False, form=None):
         self._check(self._L.tsw_idc_set(self._h, num_nodes, num_corrections, nodes_type, int(use_collocation)))
+        if form is not None:
+            self._check(self._L.tsw_idc_set_type(self._h, form))
 
     def idc_step(self, psi, source, sigma_t, sigma_s, dt, abs_tol=1e-12, max_its=3000, precision=FP64):
         """psi^n -> psi^{n+1} in place (zero halos); source holds Q and the inflow."""

@@ -203,17 +203,43 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
                   const double *seed, double *grad_src, double *grad_sigma_t, double *grad_sigma_s, double abs_tol,
                   int max_its, int restart, int precision, int mem, int *iters, double *resid);
 
-#define TSW_IDC_LOBATTO 0  /* Gauss-Lobatto nodes (both endpoints) */
-#define TSW_IDC_LEGENDRE 1 /* Gauss-Legendre nodes (interior), output by collocation */
+/* IDC node families, the reference's idc_nodes_type (Quadrule::NodesType, Quadrule.hpp:30-70) */
+#define TSW_IDC_LOBATTO 0         /* "gauss-lobatto": both ends of the step are nodes */
+#define TSW_IDC_LEGENDRE 1        /* "gauss-legendre": interior nodes; output by collocation */
+#define TSW_IDC_RADAU 2           /* "gauss-radau": Radau IIA, the right end is a node */
+#define TSW_IDC_CHEBYSHEV 3       /* "chebyshev": interior nodes; output by collocation */
+#define TSW_IDC_CHEBYSHEV_RADAU 4 /* "chebyshev-radau": the right end is a node */
+
+/* IDC forms, the reference's idc_type (IDCIntegrator::IDCType, idc.cpp:33-50) */
+#define TSW_IDC_ERROR 0  /* "error": corrections solve for the error and add it to the stage */
+#define TSW_IDC_UPDATE 1 /* "update": corrections solve for the updated stage directly */
 
 /* IDC integrator configuration, the reference's idc_num_nodes, idc_num_corr, idc_nodes_type and
  * idc_use_collocation parameters (idc.cpp:223, SetNodes :2305-2376): num_nodes quadrature nodes on each
- * step (2..12 Lobatto, 1..12 Legendre), num_corrections correction sweeps (0..32).  Defaults: 4 Lobatto
- * nodes, 3 corrections, no collocation.  Only the non-hybrid error-form integrator is built. */
+ * step (2..12 Lobatto, 1..12 for the other families), num_corrections correction sweeps (0..32).
+ * Collocation forms the result from the integration matrix; it is always used for families without the
+ * right end (as the reference's Step_Nonhybrid does, :933-936).  Defaults: 4 Lobatto nodes, 3
+ * corrections, no collocation, error form.  Only the non-hybrid integrator is built. */
 int tsw_idc_set(tsw_ctx *ctx, int num_nodes, int num_corrections, int nodes_type, int use_collocation);
 
-/* One IDC step psi^n -> psi^{n+1} in place, IDCIntegrator::Step_Nonhybrid (idc.cpp:846-940) with
- * IDCType::Error: implicit-Euler prediction over the nodes, corrections whose sources integrate the
+/* IDC form (TSW_IDC_ERROR default, TSW_IDC_UPDATE) for the following steps, the reference's idc_type.
+ * Both forms give the same step up to the implicit solves' tolerance; the update form's solves start
+ * from the stage's previous iterate in the reference (SetInitialGuess, idc.cpp:893-901), which changes
+ * only their iteration counts.  tsw_idc_adjoint transposes the step, which is the same linear map in
+ * both forms, so it ignores this setting.  Other values fail with TSW_E_ARG. */
+int tsw_idc_set_type(tsw_ctx *ctx, int idc_type);
+
+/* Stage nodes on [0, 1] and integration matrix of an IDC configuration (host-only, needs no device):
+ * num_stages = num_nodes + 1 for families without the left end (t = 0 becomes stage 0, as
+ * num_stages = num_nodes + !left_endpoint in SetNodes), num_nodes otherwise; nodes[S] ascending; W[S*S]
+ * row-major, row 0 the integrals over [0, 1] of the nodes' Lagrange basis, row n > 0 those over
+ * [t_{n-1}, t_n] (ComputeQuadratureWeights, columns of non-node stages zero).  Roots are found by Newton
+ * in quadruple precision and rounded once.  Writes *num_stages; fills nodes and W when nodes is non-NULL
+ * (capacity >= num_stages).  Bad types or counts fail with TSW_E_ARG. */
+int tsw_idc_nodes(int nodes_type, int num_nodes, int capacity, int *num_stages, double *nodes, double *W);
+
+/* One IDC step psi^n -> psi^{n+1} in place, IDCIntegrator::Step_Nonhybrid (idc.cpp:846-940) in the form
+ * tsw_idc_set_type chose: implicit-Euler prediction over the nodes, corrections whose sources integrate the
  * residuals (L - S P) psi with the nodes' integration matrix (ComputeSource :1686-1740), result at the
  * last node or by collocation.  psi's halos are taken as zero (as the reference assumes) and returned
  * zero; source holds Q with the inflow in its halos, constant over the step.  Each implicit solve is the

@@ -24,6 +24,7 @@
 #include "linear_solvers/Abstract/SweepOperator.hpp"
 #include "objects/RKDG/CrossSection.hpp"
 #include "objects/RKDG/OrdinateFlux.hpp"
+#include "time_integrators/RKDG/idc.hpp"
 #include "utils/CLog.hpp"
 #include "utils/global.hpp"
 #include "utils/init.hpp"
@@ -206,6 +207,17 @@ int ref_solve(double dt, const char *log_path) {
         }
         Global::logfp = old;
         if (lf) std::fclose(lf);
+    });
+}
+
+/* One step of the deck's IDC integrator (IDCIntegrator::Create and Step, idc.cpp:98-110,521-600 over
+ * Step_Nonhybrid :846-940: idc_type, idc_nodes_type, idc_num_nodes, idc_num_corr, idc_use_collocation, and
+ * the deck's solve_type for the implicit solves): psi (ghost cells zero, as the step assumes) advanced by
+ * dt in place with the source held constant. */
+int ref_idc_step(double dt) {
+    return guarded([&] {
+        std::unique_ptr<IDCIntegrator> idc(IDCIntegrator::Create(Global::input_list));
+        idc->Step(*g->psi, *g->source, g->sigma_t, g->sigma_s, dt);
     });
 }
 

@@ -38,6 +38,7 @@ def lib(fast=False):
         L.ref_set_xs.argtypes = [ctypes.c_int, _dp]
         L.ref_sweep.argtypes = [ctypes.c_double]
         L.ref_solve.argtypes = [ctypes.c_double, ctypes.c_char_p]
+        L.ref_idc_step.argtypes = [ctypes.c_double]
         _lib = L
     return _lib
 
@@ -116,3 +117,10 @@ class Deck:
                     parts = line.split()
                     its, err = int(parts[1]), float(parts[4])
         return self.get_flux(1), its, err
+
+    def idc_step(self, psi0, src, dt):
+        """One step of the deck's IDC integrator from psi0 (ghost cells zero) with source src."""
+        self.set_flux(0, psi0)
+        self.set_flux(1, src)
+        _check(lib().ref_idc_step(dt))
+        return self.get_flux(0)

@@ -124,15 +124,34 @@ int main(int argc, char **argv) {
     idc.Step(psi_t, src0, st, ss, 0.05);
     idc.Step(psi_t, src0, st, ss, 0.05);
     dump(out + "/idc.bin", psi_t.data(), psi_t.size());
+    // the reference's SetNodes rejects the equispaced families (idc.cpp:2344-2352), and idc_type has two values
     bool threw_idc = false;
     try {
         ParameterList badi;
-        badi.Set("idc_nodes_type", "gauss-radau");
+        badi.Set("idc_nodes_type", "equispaced-both");
         IDCIntegrator bad_idc(dev, badi);
     } catch (const std::out_of_range &) {
         threw_idc = true;
     }
     check(threw_idc, "unbuilt idc_nodes_type throws std::out_of_range");
+    bool threw_type = false;
+    try {
+        ParameterList badt;
+        badt.Set("idc_type", "none");
+        IDCIntegrator bad_idc(dev, badt);
+    } catch (const std::out_of_range &) {
+        threw_type = true;
+    }
+    check(threw_type, "unbuilt idc_type throws std::out_of_range");
+    {
+        ParameterList ok;
+        ok.Set("idc_type", "update");
+        ok.Set("idc_nodes_type", "gauss-radau");
+        ok.Set("idc_num_nodes", "2");
+        ok.Set("idc_num_corr", "2");
+        IDCIntegrator radau(dev, ok);  // the reference input deck's IDC settings construct
+        check(radau.GetDescriptor() == "idc", "gauss-radau update-form integrator constructs");
+    }
 
     bool threw = false;
     try {

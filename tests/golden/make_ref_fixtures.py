@@ -9,7 +9,10 @@ oracle/build_ref.sh from the unmodified reference sources; run in the container 
   sources with inflow halos, for decks whose cross sections Init projects at degree 0 and at the flux
   degree (piecewise_cross false), steady and with dt;
 * source iteration: SISolver::Solve on two lattice decks (steady, abs_tol 1e-10 and 1e-12): the angular
-  flux, the iteration count and the last change (printed with four decimals).
+  flux, the iteration count and the last change (printed with four decimals);
+* IDC steps: IDCIntegrator::Step on a lattice deck from a seeded initial condition, for the input deck's
+  configuration (idc_type update, gauss-radau, 2 nodes, 2 corrections) and the other node families and
+  forms, each implicit solve a source iteration to 1e-13.
 Each case also stores the inputs the reference's Init built (cross-section coefficients, ordinates, mesh),
 so the tests need only the fixture.  Usage: python tests/golden/make_ref_fixtures.py
 """
@@ -34,6 +37,17 @@ SWEEPS = {  # name: deck parameters, seed
     "sweep_ho": (dict(n=8, m=9, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-10"), 12),
     "sweep_ho3": (dict(n=6, m=5, p=3, pwc="false", otype="spherical-triangle", order=2, tol="1e-10"), 13),
 }
+IDC_DECK = LATTICE.format(n=10, m=9, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-13") + \
+    "cfl 0.5\nidc_type {form}\nidc_nodes_type {nodes}\nidc_num_nodes {nn}\nidc_num_corr {nc}\n" \
+    "idc_use_collocation {colloc}\n"
+IDC_STEPS = {  # name: (form, nodes, number of nodes, corrections, collocation)
+    "idc_radau_update": ("update", "gauss-radau", 2, 2, "false"),
+    "idc_lobatto_error": ("error", "gauss-lobatto", 3, 2, "false"),
+    "idc_legendre_update": ("update", "gauss-legendre", 3, 1, "false"),
+    "idc_chebradau_error_colloc": ("error", "chebyshev-radau", 3, 2, "true"),
+    "idc_chebyshev_update": ("update", "chebyshev", 2, 1, "false"),
+}
+IDC_DT = 0.1
 SOLVES = {
     "si_pwc": dict(n=14, m=14, p=1, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"),
     "si_ho": dict(n=10, m=10, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-12"),
@@ -76,6 +90,23 @@ def main():
         out[name + "/its"] = np.array(its)
         out[name + "/err"] = np.array(err)
         print(name, "iterations", its, "last change", err)
+    for name, (form, nodes, nn, nc, colloc) in IDC_STEPS.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(IDC_DECK.format(form=form, nodes=nodes, nn=nn, nc=nc, colloc=colloc))
+        d = _ref.Deck(path)
+        _store_inputs(out, name, d)
+        rng = np.random.default_rng(77)
+        psi0 = np.zeros(d.psi_shape())
+        psi0[1:-1, 1:-1] = rng.uniform(0.0, 1.0, size=psi0[1:-1, 1:-1].shape)
+        src = d.get_flux(1)
+        out[name + "/psi0"] = psi0
+        out[name + "/src"] = src
+        out[name + "/config"] = np.array([1 if form == "update" else 0, nn, nc, 1 if colloc == "true" else 0])
+        out[name + "/nodes"] = np.array(nodes)
+        out[name + "/dt"] = np.array(IDC_DT)
+        out[name + "/psi1"] = d.idc_step(psi0, src, IDC_DT)
+        print(name, "step norm", float(np.linalg.norm(out[name + "/psi1"])))
     np.savez_compressed(os.path.join(HERE, "ref_fixtures.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_fixtures.npz"), len(out), "arrays")
 

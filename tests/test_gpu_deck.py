@@ -75,12 +75,21 @@ def test_tsw_solve_steady_deck_matches_oracle(tmp_path):
     assert l2 <= 1e-9 * fields.l2_norm(phi_ref, m2) and li <= 1e-8 * fields.linf_norm(phi_ref, m2)
 
 
-def test_tsw_solve_idc_deck_matches_oracle(tmp_path):
+# the IDC lines of the reference's input.deck (idc_type update, gauss-radau, 2 nodes, 2 corrections)
+INPUT_DECK_IDC = "idc_type update\nidc_use_collocation false\nidc_nodes_type gauss-radau\nidc_num_nodes 2\n" \
+                 "idc_num_corr 2\n"
+
+
+@pytest.mark.parametrize("idc_lines,kw", [("", {}),
+                                          (INPUT_DECK_IDC, dict(num_nodes=2, num_corr=2, kind="radau",
+                                                                form="update"))])
+def test_tsw_solve_idc_deck_matches_oracle(tmp_path, idc_lines, kw):
     """Three IDC steps (0.1, 0.1 and a final 0.05 to reach tfinal = 0.25) from the deck's initial
-    condition, against the IDC restatement over the oracle's solves."""
+    condition, against the IDC restatement over the oracle's solves: the defaults (error form, 4 Lobatto
+    nodes, 3 corrections) and the reference input deck's IDC settings."""
     import idc
     text = LATTICE.replace("nx 28", "nx 14").replace("ny 28", "ny 14").replace("abs_tol 1e-12", "abs_tol 1e-13") + \
-        "time_integrator idc\ntfinal 0.25\ncfl 0.2\noutput_af true\n"
+        "time_integrator idc\ntfinal 0.25\ncfl 0.2\noutput_af true\n" + idc_lines
     d = tmp_path / "t.deck"
     d.write_text(text)
     print("running tsw_solve with time_integrator idc", flush=True)
@@ -94,7 +103,7 @@ def test_tsw_solve_idc_deck_matches_oracle(tmp_path):
     psi = np.zeros_like(P.source)
     src = P.source.copy()
     for dt in (0.1, 0.1, 0.25 - 2 * 0.1):
-        psi = idc.idc_step(ops, psi, src, dt)
+        psi = idc.idc_step(ops, psi, src, dt, **kw)
     got, _, _ = fields.read_ordinate_flux(str(tmp_path / "t.af2"))
     e = rel_err(got[1:-1, 1:-1], psi[1:-1, 1:-1])
     print("IDC rel err vs oracle", e, flush=True)

@@ -1,8 +1,9 @@
 """IDC time steps on the GPU (tsw_idc_step) against the oracle's restatement of the reference's
-non-hybrid error-form IDC (oracle/idc.py, pinned by order-of-accuracy tests): several steps with
-Lobatto nodes (FP64 to 1e-10 relative, FP32 sweeps with FP64 Krylov vectors and correction sums to
-1e-8), Legendre nodes with collocation, sources with and without inflow halos, and the largest node
-counts (12 Lobatto or Legendre nodes, whose correction sources combine 15 arrays).  The adjoint of
+non-hybrid IDC (oracle/idc.py, pinned by order-of-accuracy tests): several steps with Lobatto nodes
+(FP64 to 1e-10 relative, FP32 sweeps with FP64 Krylov vectors and correction sums to 1e-8), Legendre
+nodes with collocation, Radau, Chebyshev and Chebyshev-Radau nodes in the error and update forms,
+sources with and without inflow halos, and the largest node counts (12 Lobatto or Legendre nodes, whose
+correction sources combine 15 arrays).  The adjoint of
 several steps is pinned by the dot-product identity and by central differences of the cross-section
 gradients, with and without inflow halos.  The line-source configuration itself is in
 test_gpu_line_source.py."""
@@ -53,6 +54,39 @@ def test_idc_steps_match_oracle(kind, num_nodes, num_corr, colloc, pattern, halo
             assert not psi[0].any() and not psi[:, -1].any()
 
 
+NODE_NAMES = {tsw.IDC_LOBATTO: "lobatto", tsw.IDC_LEGENDRE: "legendre", tsw.IDC_RADAU: "radau",
+              tsw.IDC_CHEBYSHEV: "chebyshev", tsw.IDC_CHEBYSHEV_RADAU: "chebyshev-radau"}
+
+
+@pytest.mark.parametrize("kind,num_nodes,num_corr,colloc", [(tsw.IDC_RADAU, 3, 2, False),
+                                                             (tsw.IDC_RADAU, 2, 1, True),
+                                                             (tsw.IDC_CHEBYSHEV, 3, 2, False),
+                                                             (tsw.IDC_CHEBYSHEV_RADAU, 4, 3, False),
+                                                             (tsw.IDC_LOBATTO, 4, 3, False)])
+@pytest.mark.parametrize("form", [tsw.IDC_ERROR, tsw.IDC_UPDATE])
+def test_idc_node_families_and_forms_match_oracle(kind, num_nodes, num_corr, colloc, form):
+    """The reference deck's configuration (idc_type update, idc_nodes_type gauss-radau) and the other node
+    families, in both forms, with inflow halos, against the oracle's step of the same form."""
+    g, o, st, ss, src, psi0 = _vacuum_problem(7, 9, 2, 4, seed=29, halo_inflow=True)
+    ops = OracleIdcOps(g, o, st, ss, tol=1e-14)
+    name = NODE_NAMES[kind]
+    fname = "update" if form == tsw.IDC_UPDATE else "error"
+    dt = 0.05
+    ref = psi0.copy()
+    for _ in range(2):
+        ref = idc.idc_step(ops, ref, src, dt, num_nodes=num_nodes, num_corr=num_corr, kind=name,
+                           collocation=colloc, form=fname)
+    with tsw.Context(0) as c:
+        c.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
+        c.idc_set(num_nodes, num_corr, kind, colloc, form=form)
+        for prec, tol in ((tsw.FP64, 1e-10), (tsw.FP32, 1e-8)):
+            psi = psi0.copy()
+            for _ in range(2):
+                c.idc_step(psi, src, st, ss, dt, abs_tol=1e-13 if prec == tsw.FP64 else 1e-11, precision=prec)
+            assert rel_err(psi, ref) <= tol, prec
+            assert not psi[0].any() and not psi[:, -1].any()
+
+
 @pytest.mark.parametrize("kind,colloc", [(tsw.IDC_LOBATTO, False), (tsw.IDC_LOBATTO, True),
                                          (tsw.IDC_LEGENDRE, True)])
 def test_idc_twelve_nodes_match_oracle(kind, colloc):
@@ -95,17 +129,20 @@ def _ctx_for(g, o, num_nodes=3, num_corr=2):
 
 
 @pytest.mark.parametrize("halo_inflow", [False, True])
-@pytest.mark.parametrize("kind,colloc", [(tsw.IDC_LOBATTO, False), (tsw.IDC_LEGENDRE, True)])
-def test_idc_adjoint_dot_product_over_steps(halo_inflow, kind, colloc):
+@pytest.mark.parametrize("kind,colloc,form", [(tsw.IDC_LOBATTO, False, tsw.IDC_ERROR),
+                                              (tsw.IDC_LEGENDRE, True, tsw.IDC_ERROR),
+                                              (tsw.IDC_RADAU, False, tsw.IDC_UPDATE)])
+def test_idc_adjoint_dot_product_over_steps(halo_inflow, kind, colloc, form):
     """The IDC map is affine in psi^0: <g, run(x) - run(0)> = <dJ/dpsi^0, x> pins the transposed step
     (adjoint solves, (L - S P)^T residual terms, integration-matrix weights) over several steps; with
-    inflow halos the affine part changes and the linear part must not."""
+    inflow halos the affine part changes and the linear part must not.  The update form's forward runs
+    check that the one transposed step serves both forms."""
     g, o, st, ss, src, psi0 = _vacuum_problem(8, 7, 1, 4, seed=90, halo_inflow=halo_inflow)
     rng = np.random.default_rng(91)
     seed = np.zeros(g.psi_shape(o.nq))
     seed[1:-1, 1:-1] = rng.normal(size=seed[1:-1, 1:-1].shape)
     with _ctx_for(g, o) as c:
-        c.idc_set(3, 2, kind, colloc)
+        c.idc_set(3, 2, kind, colloc, form=form)
         dt, nsteps = 0.07, 4
         a = psi0.copy()
         c.idc_run(a, src, st, ss, dt, nsteps, abs_tol=1e-14)

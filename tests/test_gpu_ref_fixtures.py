@@ -1,7 +1,9 @@
 """The HIP library against the reference's own outputs (tests/golden/ref_fixtures.npz, written from the
 reference sources compiled unmodified): sweeps on every pattern with degree-0 and degree-p cross sections
-(FP64 to 1e-12, FP32 to 2e-6), and source iteration (the reference's iteration count, its angular flux to
-1e-11) on lattice decks with both kinds of cross section."""
+(FP64 to 1e-12, FP32 to 2e-6), source iteration (the reference's iteration count, its angular flux to
+1e-11) on lattice decks with both kinds of cross section, and IDC steps (IDCIntegrator::Step) for the input
+deck's configuration (update form, 2 Gauss-Radau nodes, 2 corrections) and the other node families and
+forms (FP64 to 1e-11: the reference's implicit solves stop at 1e-13)."""
 import math
 import os
 
@@ -60,3 +62,22 @@ def test_source_iteration_matches_reference(fx, name, graph):
     assert its == int(fx[name + "/its"])
     assert abs(err - float(fx[name + "/err"])) <= 1e-4 * float(fx[name + "/err"])
     assert rel_err(out, fx[name + "/psi"]) <= 1e-11
+
+
+IDC_NODES = {"gauss-radau": tsw.IDC_RADAU, "gauss-lobatto": tsw.IDC_LOBATTO, "gauss-legendre": tsw.IDC_LEGENDRE,
+             "chebyshev-radau": tsw.IDC_CHEBYSHEV_RADAU, "chebyshev": tsw.IDC_CHEBYSHEV}
+
+
+@pytest.mark.parametrize("name", ["idc_radau_update", "idc_lobatto_error", "idc_legendre_update",
+                                  "idc_chebradau_error_colloc", "idc_chebyshev_update"])
+def test_idc_step_matches_reference(fx, name):
+    print(f"reference fixture {name}", flush=True)
+    update, nn, nc, colloc = (int(v) for v in fx[name + "/config"])
+    c, st, ss = _ctx(fx, name)
+    with c:
+        c.idc_set(nn, nc, IDC_NODES[str(fx[name + "/nodes"])], bool(colloc),
+                  form=tsw.IDC_UPDATE if update else tsw.IDC_ERROR)
+        for prec, tol in ((tsw.FP64, 1e-11), (tsw.FP32, 1e-9)):
+            out = fx[name + "/psi0"].copy()
+            c.idc_step(out, fx[name + "/src"], st, ss, float(fx[name + "/dt"]), abs_tol=1e-13, precision=prec)
+            assert rel_err(out, fx[name + "/psi1"]) <= tol, (name, prec)

@@ -9,7 +9,10 @@ compiled unmodified, oracle/build_ref.sh):
 * the oracle's sweep equals the reference's SweepOperator::Apply on the same inputs (degree-0 and
   degree-p cross sections, steady and with dt);
 * the oracle's source iteration takes the reference's iteration count and reaches its angular flux;
-* the numpy deck path builds the cross sections the reference's Init builds.
+* the numpy deck path builds the cross sections the reference's Init builds;
+* the oracle's IDC step (oracle/idc.py over the oracle's GMRES) equals the reference's IDCIntegrator::Step
+  for the input deck's configuration (update form, 2 Gauss-Radau nodes, 2 corrections) and the other node
+  families and forms (the reference's implicit solves are source iterations to 1e-13).
 Where oracle/_ref/libref.so is present the live reference is also run on fresh seeded inputs."""
 import math
 import os
@@ -76,6 +79,25 @@ def test_oracle_source_iteration_equals_reference(fx, name):
     assert its == int(fx[name + "/its"])
     assert abs(err - float(fx[name + "/err"])) <= 1e-4 * float(fx[name + "/err"])   # printed with 4 decimals
     assert rel_err(psi, fx[name + "/psi"]) <= 1e-12
+
+
+IDC_CASES = ["idc_radau_update", "idc_lobatto_error", "idc_legendre_update", "idc_chebradau_error_colloc",
+             "idc_chebyshev_update"]
+IDC_KINDS = {"gauss-radau": "radau", "gauss-lobatto": "lobatto", "gauss-legendre": "legendre",
+             "chebyshev-radau": "chebyshev-radau", "chebyshev": "chebyshev"}
+
+
+@pytest.mark.parametrize("name", IDC_CASES)
+def test_oracle_idc_step_equals_reference(fx, name):
+    import idc
+    from _common import OracleIdcOps
+    g, o, _, st, ss = _case(fx, name)
+    update, nn, nc, colloc = (int(v) for v in fx[name + "/config"])
+    ops = OracleIdcOps(g, o, st, ss, tol=1e-14)
+    out = idc.idc_step(ops, fx[name + "/psi0"], fx[name + "/src"], float(fx[name + "/dt"]), num_nodes=nn,
+                       num_corr=nc, kind=IDC_KINDS[str(fx[name + "/nodes"])], collocation=bool(colloc),
+                       form="update" if update else "error")
+    assert rel_err(out, fx[name + "/psi1"]) <= 1e-12, name
 
 
 def test_deck_cross_sections_equal_reference_init(fx):
