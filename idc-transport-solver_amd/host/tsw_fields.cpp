@@ -112,10 +112,58 @@ ScalarFlux ScalarFlux::ReadFromDisk(const std::string &filename) {
 void OrdinateFlux::WriteToDisk(const std::string &filename) const {
     File f(filename, "wb");
     if (!f.fp) throw std::runtime_error("Failed to open file '" + filename + "' in 'WriteToDisk'.");
-    put_mesh(f.fp, dd_);
-    put_string(f.fp, os_.type() + (os_.symmetric_reduce() ? "#R" : ""));
-    put<int64_t>(f.fp, os_.ang_order());
-    put_density(f.fp, DG_degree, density_);
+    WriteToDisk(f.fp);
+}
+
+void OrdinateFlux::WriteToDisk(std::FILE *fp) const {
+    put_mesh(fp, dd_);
+    put_string(fp, os_.type() + (os_.symmetric_reduce() ? "#R" : ""));
+    put<int64_t>(fp, os_.ang_order());
+    put_density(fp, DG_degree, density_);
+}
+
+bool OrdinateFlux::ReadValues(std::FILE *fp) {
+    DomainDecomposition dd;
+    std::string name;
+    int64_t order, degree, bytes, count;
+    if (!get_mesh(fp, dd) || !get_string(fp, name) || !get(fp, order) || !get(fp, degree) || !get(fp, bytes) ||
+        !get(fp, count))
+        return false;
+    const std::string mine = os_.type() + (os_.symmetric_reduce() ? "#R" : "");
+    if (dd.nx != dd_.nx || dd.ny != dd_.ny || dd.ax != dd_.ax || dd.bx != dd_.bx || dd.ay != dd_.ay ||
+        dd.by != dd_.by || name != mine || order != os_.ang_order() || degree != DG_degree ||
+        count != (int64_t)density_.size() || bytes != count * (int64_t)sizeof(double))
+        return false;
+    return std::fread(density_.data(), sizeof(double), density_.size(), fp) == density_.size();
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Checkpoint files (TimeIntegrator::WriteCheckpoint / ResumeCheckpoint, TimeIntegrator.cpp:745-1111,
+// single rank): int64 step, double time, double solve time, the angular flux as OrdinateFlux::WriteToDisk
+// writes it, then the 20 timer values the reference keeps (TMR_AF_Lsv .. TMR_io).
+void WriteCheckpoint(const std::string &filename, const CheckpointState &st, const OrdinateFlux &psi) {
+    File f(filename, "wb");
+    if (!f.fp) throw std::runtime_error("Failed to open file '" + filename + "' in 'WriteCheckpoint'.");
+    put<int64_t>(f.fp, st.step);
+    put<double>(f.fp, st.time);
+    put<double>(f.fp, st.solve_time);
+    psi.WriteToDisk(f.fp);
+    for (double v : st.timers) put<double>(f.fp, v);
+}
+
+CheckpointState ResumeCheckpoint(const std::string &filename, OrdinateFlux &psi) {
+    File f(filename, "rb");
+    const std::string err = "Failed to resume from checkpoint file '" + filename + "'.";
+    if (!f.fp) throw std::runtime_error("Failed to open file '" + filename + "' in 'ResumeCheckpoint'.");
+    CheckpointState st;
+    OrdinateFlux tmp = psi;  // psi is left as it was when the file does not fit it
+    if (!get(f.fp, st.step) || !get(f.fp, st.time) || !get(f.fp, st.solve_time) || !tmp.ReadValues(f.fp))
+        throw std::runtime_error(err);
+    for (double &v : st.timers)
+        if (!get(f.fp, v)) throw std::runtime_error(err);
+    if (st.step < 0 || !(st.time >= 0.0)) throw std::runtime_error(err);
+    std::copy(tmp.data(), tmp.data() + tmp.size(), psi.data());
+    return st;
 }
 
 OrdinateFlux OrdinateFlux::ReadFromDisk(const std::string &filename) {

@@ -2,6 +2,7 @@
 #include "tsw_host.hpp"
 
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <map>
 #include <sstream>
@@ -261,6 +262,9 @@ IDCIntegrator::IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &p
         if (it == kTypes.end()) throw std::out_of_range("idc_type '" + t + "' is not built");
         form = it->second;
     }
+    // checkpoint settings (TimeIntegrator.cpp:145-159): negative values are taken as zero
+    if (p.Has("checkpoint_steps")) checkpoint_steps_ = std::max<int64_t>(0, p.GetInt("checkpoint_steps"));
+    if (p.Has("checkpoint_walltime")) checkpoint_walltime_ = std::max(0.0, p.GetDouble("checkpoint_walltime"));
     // idc_use_collocation is read only for node families holding the right end (idc.cpp:225-229); the
     // others always use collocation
     int rc = tsw_idc_set(dev_->ctx(), nodes, corr, type, flag_from(p, "idc_use_collocation", false) ? 1 : 0);
@@ -276,6 +280,78 @@ void IDCIntegrator::Step(OrdinateFlux &initial_condition, const OrdinateFlux &so
                                 dt, abs_tol_, (int)max_its_, precision_, TSW_MEM_HOST);
     if (rc == TSW_E_NOCONV) throw std::runtime_error("an implicit solve of the IDC step failed to converge");
     dev_->Check(rc, "IDCIntegrator::Step");
+}
+
+IDCIntegrator::Outcome IDCIntegrator::Integrate(OrdinateFlux &psi, const OrdinateFlux &source,
+                                                const CrossSection &sigma_t, const CrossSection &sigma_s,
+                                                double final_time, double max_step_size, const std::string &prefix,
+                                                const volatile std::sig_atomic_t *stop, std::FILE *log) {
+    if (!(final_time > 0.0) || !(max_step_size > 0.0))
+        throw std::invalid_argument("Integrate needs a positive final time and step size");
+    const double TOL = 1.0e-14;
+    const int64_t num_steps =
+        (int64_t)(final_time / max_step_size) + (std::fmod(final_time, max_step_size) > TOL ? 1 : 0);
+    const bool checkpointing = checkpoint_steps_ > 0 || checkpoint_walltime_ > 0.0;
+    state_ = CheckpointState{};
+    last_checkpoint_.clear();
+    double wtime_limit = checkpoint_walltime_, last_wtime = 0.0, est_wtime = 0.0;
+    bool first_step = true;
+    if (checkpointing) {
+        try {
+            state_ = ResumeCheckpoint(prefix + ".chk", psi);
+            if (log) std::fprintf(log, "resumed from %s.chk at step %lld, t = %.4e\n", prefix.c_str(),
+                                  (long long)state_.step, state_.time);
+        } catch (const std::exception &) {
+            std::fprintf(stderr, "Warning: Failed to resume from checkpoint file: Starting from beginning.\n");
+            state_ = CheckpointState{};
+        }
+        // the walltime limit counts from the resumed solve time (ResumeCheckpoint, :1079-1089)
+        if (checkpoint_walltime_ > 0.0) {
+            last_wtime = state_.solve_time;
+            wtime_limit += last_wtime;
+        }
+    }
+    long long n0 = 0;
+    double ms0 = 0.0;
+    dev_->Check(tsw_sweep_timing(dev_->ctx(), 0, &n0, &ms0), "tsw_sweep_timing");
+    while (state_.step < num_steps) {
+        double h = max_step_size;
+        if ((state_.step + 1) * max_step_size > final_time + TOL) h = final_time - state_.step * max_step_size;
+        const auto t0 = std::chrono::steady_clock::now();
+        Step(psi, source, sigma_t, sigma_s, h);
+        state_.solve_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        state_.time += h;
+        ++state_.step;
+        long long n1 = 0;
+        double ms1 = 0.0;
+        dev_->Check(tsw_sweep_timing(dev_->ctx(), 0, &n1, &ms1), "tsw_sweep_timing");
+        state_.timers[0] += (ms1 - ms0) * 1e-3;  // TMR_AF_Lsv: time in sweeps
+        ms0 = ms1;
+        if (log) {
+            std::fprintf(log, "% 4d%%    t = % .4e    dt = % .4e\n", (int)(100 * state_.step / num_steps), state_.time, h);
+            std::fflush(log);
+        }
+        if (state_.step >= num_steps) break;
+        bool write = checkpoint_steps_ > 0 && state_.step % checkpoint_steps_ == 0;
+        if (checkpoint_walltime_ > 0.0) {  // CheckWalltimeCheckpoint: exponential average of the step time
+            const double step_wtime = state_.solve_time - last_wtime;
+            est_wtime = first_step ? step_wtime : 0.3 * step_wtime + 0.7 * est_wtime;
+            first_step = false;
+            last_wtime = state_.solve_time;
+            const double left = wtime_limit - last_wtime;
+            if (left <= 0.0 || (int64_t)(left / est_wtime) <= 0) write = true;
+        }
+        if (checkpointing && stop && *stop) write = true;
+        if (write) {
+            char tag[32];
+            std::snprintf(tag, sizeof tag, "%04llX", (unsigned long long)state_.step);
+            last_checkpoint_ = prefix + "." + tag + ".chk";
+            WriteCheckpoint(last_checkpoint_, state_, psi);
+            return Outcome::Checkpointed;
+        }
+    }
+    if (state_.step == num_steps) state_.time = final_time;
+    return Outcome::Done;
 }
 
 }  // namespace tsw_host

@@ -19,6 +19,8 @@
 #pragma once
 
 #include <array>
+#include <csignal>
+#include <cstdio>
 #include <cstdint>
 #include <limits>
 #include <map>
@@ -110,8 +112,12 @@ class OrdinateFlux {
     // header, the ordinate-set name with "#R" (the 2D sets are symmetry-reduced) as a null-terminated
     // string, the angular order, then DG degree, byte count, value count and every value with halos.
     void WriteToDisk(const std::string &filename) const;
+    void WriteToDisk(std::FILE *fp) const;
     // Reads such a file; the ordinate set is rebuilt from its name and order (level-symmetric only).
     static OrdinateFlux ReadFromDisk(const std::string &filename);
+    // Reads such a record from fp into this flux; false (values partly read or untouched) when the record's
+    // mesh, ordinate set, degree or size differ from this flux's.
+    bool ReadValues(std::FILE *fp);
     const int DG_degree;
 
   private:
@@ -284,6 +290,20 @@ class PeierlsGMRESSolver : public ImplicitSolver {
 // "chebyshev-radau"), idc_use_collocation (false), abs_tol (1e-10) and max_its
 // (3000) of the implicit solves, sweep_precision.  Step advances initial_condition by dt in place
 // (halos zero), with the material source and boundary inflow in `source`, constant over the step.
+// Checkpoint of a time integration (TimeIntegrator::WriteCheckpoint / ResumeCheckpoint,
+// TimeIntegrator.cpp:745-1111): completed steps, time reached, accumulated solve time and the reference's
+// 20 timer slots (this build fills the first, the sweep time TMR_AF_Lsv, in seconds).
+struct CheckpointState {
+    int64_t step = 0;
+    double time = 0.0, solve_time = 0.0;
+    std::array<double, 20> timers{};
+};
+// Writes `filename`; throws std::runtime_error when it cannot be written.
+void WriteCheckpoint(const std::string &filename, const CheckpointState &st, const OrdinateFlux &psi);
+// Reads `filename` into psi (same mesh, ordinates and degree); throws std::runtime_error, leaving psi as it
+// was, when the file is missing, short or written for another problem.
+CheckpointState ResumeCheckpoint(const std::string &filename, OrdinateFlux &psi);
+
 class IDCIntegrator {
   public:
     IDCIntegrator(std::shared_ptr<Device> dev, const ParameterList &params);
@@ -291,11 +311,31 @@ class IDCIntegrator {
               const CrossSection &sigma_s, double dt);
     std::string GetDescriptor() const { return "idc"; }
 
+    // TimeIntegrator::Integrate (TimeIntegrator.cpp:325-433): steps of max_step_size up to final_time, the
+    // last one shortened to land on it.  With checkpoint_steps or checkpoint_walltime (deck keys of the same
+    // names) it first resumes from `<prefix>.chk` when that file fits (a warning otherwise), and after a step
+    // writes `<prefix>.<step as %04X>.chk` and returns Checkpointed when: the step count is a multiple of
+    // checkpoint_steps (and steps remain), the walltime estimate says the next step would pass
+    // checkpoint_walltime seconds of solve time (CheckWalltimeCheckpoint :1141-1199), or *stop is set (the
+    // driver's SIGINT/SIGTERM flag, CheckpointSignalHandler :67-103).  `log`, when set, receives a line per
+    // step.  The reference's driver exits with status 150 after a checkpoint (global.hpp:179).
+    enum class Outcome { Done, Checkpointed };
+    Outcome Integrate(OrdinateFlux &psi, const OrdinateFlux &source, const CrossSection &sigma_t,
+                      const CrossSection &sigma_s, double final_time, double max_step_size,
+                      const std::string &prefix, const volatile std::sig_atomic_t *stop = nullptr,
+                      std::FILE *log = nullptr);
+    const CheckpointState &state() const { return state_; }
+    std::string last_checkpoint() const { return last_checkpoint_; }
+
   private:
     std::shared_ptr<Device> dev_;
     double abs_tol_ = 1e-10;
     int64_t max_its_ = 3000;
     int precision_ = TSW_FP64;
+    int64_t checkpoint_steps_ = 0;
+    double checkpoint_walltime_ = 0.0;
+    CheckpointState state_;
+    std::string last_checkpoint_;
 };
 
 // ---------------------------------------------------------------------------------------------------

@@ -136,6 +136,62 @@ static void test_ordinate_format(const std::string &dir) {
     check(OrdinateFlux::ReadFromDisk(dir + "/psi2.af2").nq() == 32, "unreduced Chebyshev-Legendre N=4 has 32 ordinates");
 }
 
+// Checkpoint files (TimeIntegrator::WriteCheckpoint, TimeIntegrator.cpp:745-870): int64 step, double time,
+// double solve time, the OrdinateFlux record, 20 timer doubles; ResumeCheckpoint reads them back and refuses
+// files of another problem or cut short, leaving psi unchanged.
+static void test_checkpoint_format(const std::string &dir) {
+    const DomainDecomposition dd = mesh(3, 2, 0.0, 1.5, 0.0, 1.0);
+    const OrdinateSet os = OrdinateSet::Named("chebyshev-legendre", 4, true);
+    OrdinateFlux psi(dd, os, 1);
+    for (size_t k = 0; k < psi.size(); ++k) psi.data()[k] = std::cos(0.3 * (double)k);
+    CheckpointState st;
+    st.step = 0x2A;
+    st.time = 4.2;
+    st.solve_time = 1.25;
+    for (int k = 0; k < 20; ++k) st.timers[k] = 0.5 * k;
+    const std::string path = dir + "/run.002A.chk";
+    WriteCheckpoint(path, st, psi);
+    const std::vector<unsigned char> b = slurp(path);
+    const size_t rec = 48 + std::strlen("chebyshev-legendre#R") + 1 + 32 + psi.size() * 8;
+    check(at<int64_t>(b, 0) == 0x2A && at<double>(b, 8) == 4.2 && at<double>(b, 16) == 1.25 &&
+              b.size() == 24 + rec + 20 * 8 && at<double>(b, 24 + rec + 8) == 0.5,
+          "checkpoint layout: step, time, solve time, flux record, 20 timers");
+    OrdinateFlux back(dd, os, 1);
+    const CheckpointState r = ResumeCheckpoint(path, back);
+    check(r.step == 0x2A && r.time == 4.2 && r.solve_time == 1.25 && r.timers[19] == 9.5 &&
+              std::memcmp(back.data(), psi.data(), psi.size() * 8) == 0,
+          "checkpoint round trip is bitwise");
+    OrdinateFlux other(mesh(3, 3, 0.0, 1.5, 0.0, 1.0), os, 1);
+    bool threw = false;
+    try {
+        ResumeCheckpoint(path, other);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    bool untouched = true;
+    for (size_t k = 0; k < other.size(); ++k) untouched &= other.data()[k] == 0.0;
+    check(threw && untouched, "a checkpoint of another mesh is refused and psi left as it was");
+    {
+        std::FILE *f = std::fopen((dir + "/short.chk").c_str(), "wb");
+        std::fwrite(b.data(), 1, b.size() - 9, f);
+        std::fclose(f);
+    }
+    threw = false;
+    try {
+        ResumeCheckpoint(dir + "/short.chk", back);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    check(threw, "a checkpoint cut short is refused");
+    threw = false;
+    try {
+        ResumeCheckpoint(dir + "/missing.chk", back);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    check(threw, "a missing checkpoint throws std::runtime_error");
+}
+
 static void test_norms() {
     // constant c on [0,2] x [0,3]: L1 = 6|c|, L2 = sqrt(6)|c|, Linf = |c|
     const DomainDecomposition d1 = mesh(4, 5, 0.0, 2.0, 0.0, 3.0);
@@ -385,6 +441,7 @@ int main(int argc, char **argv) {
     try {
         test_density_format(dir);
         test_ordinate_format(dir);
+        test_checkpoint_format(dir);
         test_norms();
         test_difference();
         test_projection();

@@ -178,3 +178,79 @@ def test_deck_high_order_cross_sections_match_oracle(tmp_path):
     assert r.returncode == 0
     psi, _, _ = fields.read_ordinate_flux(str(tmp_path / "ho.af2"))
     assert rel_err(psi, ref) <= 1e-9
+
+
+def _run_solve(deck_path, cwd):
+    return subprocess.run([os.path.join(PKG, "tsw_solve"), str(deck_path)], capture_output=True, text=True,
+                          timeout=300, cwd=str(cwd))
+
+
+def test_tsw_solve_checkpoint_steps_and_resume(tmp_path):
+    """checkpoint_steps 2 on a five-step IDC run (TimeIntegrator.cpp:325-433): the driver writes
+    <prefix>.0002.chk and exits with status 150, resumes from <prefix>.chk, checkpoints again at step 4 and
+    then finishes; the result equals an uninterrupted run's."""
+    import shutil
+    text = LATTICE.replace("nx 28", "nx 14").replace("ny 28", "ny 14") + \
+        "time_integrator idc\ntfinal 0.45\ncfl 0.2\noutput_af true\n"
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    (tmp_path / "a" / "t.deck").write_text(text)
+    (tmp_path / "b" / "t.deck").write_text(text + "checkpoint_steps 2\n")
+    print("uninterrupted run", flush=True)
+    r = _run_solve(tmp_path / "a" / "t.deck", tmp_path / "a")
+    assert r.returncode == 0 and "5 IDC steps" in r.stdout, r.stdout + r.stderr
+    b = tmp_path / "b"
+    for k, chk in ((1, "t.0002.chk"), (2, "t.0004.chk")):
+        print(f"checkpointed run {k}", flush=True)
+        r = _run_solve(b / "t.deck", b)
+        assert r.returncode == 150, r.stdout + r.stderr
+        assert (b / chk).exists() and "checkpoint" in r.stdout
+        if k == 2:
+            assert "resumed from" in r.stdout
+        shutil.copy(b / chk, b / "t.chk")
+    r = _run_solve(b / "t.deck", b)
+    assert r.returncode == 0 and "5 IDC steps" in r.stdout, r.stdout + r.stderr
+    got, _, _ = fields.read_ordinate_flux(str(b / "t.af2"))
+    ref, _, _ = fields.read_ordinate_flux(str(tmp_path / "a" / "t.af2"))
+    assert rel_err(got, ref) <= 1e-14
+
+
+def test_tsw_solve_sigterm_checkpoint_and_resume(tmp_path):
+    """SIGTERM during a checkpointing run (CheckpointSignalHandler, TimeIntegrator.cpp:67-103): the driver
+    checkpoints after the step in progress and exits with status 150; resuming from that file reaches the
+    uninterrupted run's result.  A run without checkpoint keys installs no handler (the signal ends it)."""
+    import shutil
+    import signal
+    text = LATTICE.replace("nx 28", "nx 14").replace("ny 28", "ny 14") + \
+        "time_integrator idc\ntfinal 18.75\ncfl 0.25\noutput_af true\nidc_num_nodes 2\nidc_num_corr 1\n"  # 150 steps
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    (tmp_path / "a" / "t.deck").write_text(text)
+    (tmp_path / "b" / "t.deck").write_text(text + "checkpoint_steps 100000\n")
+    b = tmp_path / "b"
+    print("checkpointing run, SIGTERM after its first step", flush=True)
+    proc = subprocess.Popen([os.path.join(PKG, "tsw_solve"), str(b / "t.deck")], cwd=str(b), stdout=subprocess.PIPE,
+                            stderr=subprocess.PIPE, text=True)
+    try:
+        for line in proc.stdout:
+            if "t =" in line and "dt =" in line:
+                proc.send_signal(signal.SIGTERM)
+                break
+        out, err = proc.communicate(timeout=300)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+    assert proc.returncode == 150, (out, err)
+    chks = sorted(p for p in b.iterdir() if p.name.startswith("t.") and p.name.endswith(".chk"))
+    assert len(chks) == 1
+    step = int(chks[0].name.split(".")[1], 16)
+    assert 1 <= step < 150, step
+    shutil.copy(chks[0], b / "t.chk")
+    print(f"resume from step {step}", flush=True)
+    r = _run_solve(b / "t.deck", b)
+    assert r.returncode == 0 and "150 IDC steps" in r.stdout, r.stdout + r.stderr
+    r = _run_solve(tmp_path / "a" / "t.deck", tmp_path / "a")
+    assert r.returncode == 0
+    got, _, _ = fields.read_ordinate_flux(str(b / "t.af2"))
+    ref, _, _ = fields.read_ordinate_flux(str(tmp_path / "a" / "t.af2"))
+    assert rel_err(got, ref) <= 1e-14
