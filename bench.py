@@ -242,6 +242,7 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
     ctx = tsw.Context(0)
     ctx.set_tuning("prefetch", args.prefetch)
     ctx.set_tuning("batch_direct", args.batch_direct)
+    ctx.set_tuning("lanes", args.lanes)
     ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
     U = sum(P.unknowns for P in batch)
     ncell = sum(P.nx * P.ny for P in batch)
@@ -264,7 +265,7 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
         outs[name] = work
         print(f"bench: batched {name}: call {wall / steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
               f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
-    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch")}
+    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes")}
     ctx.close()
 
     def rel(a_list, b_list):
@@ -342,6 +343,8 @@ def main():
                     help="1: lane-split sweep kernel; 0: the autotuner chooses (with automatic bands and prefetch)")
     ap.add_argument("--batch-direct", type=int, default=0,
                     help="batch groups: -1 storer wave, 0 or 1 direct stores where the tables allow")
+    ap.add_argument("--lanes", type=int, default=-1,
+                    help="batched workload: sweep members on the ordinate-lane engine (1) or the staggered kernels (-1)")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
     ap.add_argument("--dsa-cycles", type=int, default=1, help="multigrid V-cycles per DSA application")
     ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "

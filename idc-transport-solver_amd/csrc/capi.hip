@@ -105,6 +105,18 @@ struct tsw_ctx {
         size_t lds64 = 0, lds32 = 0, lds32c = 0;
     };
     std::vector<Group> groups;
+    // Batch members swept by the ordinate-lane engine (tuning key "lanes" = 1 when tsw_batch_set ran), in
+    // place on the caller's arrays: one launch per DG degree over (member, ordinate block, band) items,
+    // band-major so that every band's upstream band is dispatched first.
+    struct LaneGroup {
+        int p = 0;
+        std::vector<int> members;
+        std::vector<int4> items;
+        std::vector<unsigned char> host;  // the parameter table on its way to the device
+        DBuf params, dev_items;
+    };
+    std::vector<LaneGroup> lane_groups;
+    std::vector<char> lane_member;
     // Adjoint problem: the same mesh with the negated ordinate set in reverse order (its sweeps are the
     // transposed sweeps of this problem up to the Legendre mass scaling), sharing this stream; built on
     // the first vector-Jacobian product.  Staging of the vector-Jacobian products' arrays.
@@ -129,6 +141,7 @@ struct tsw_ctx {
     // (waves per workgroup, 0 automatic); the ordinate-block table for the block size in use, the inflow
     // traces and the per-block moment partials.
     int lanes_opt = 0, lanes_J_opt = 0, lanes_W_opt = 0;
+    bool lanes_fixed_W = false;  // batch members: every member's workgroup has lanes_W_opt waves
     struct Lanes {
         DBuf blk, mom, tinx64, tiny64, tinx32, tiny32, psi_keep;
         int J = 0, nblk = 0;
@@ -1243,7 +1256,7 @@ std::pair<int, int> lanes_shape(const tsw_ctx *c) {
     J = std::min(std::max(J, 1), 64);
     const int RW = 64 / J;
     int W = c->lanes_W_opt > 0 ? c->lanes_W_opt : 4;
-    W = std::min(W, std::max(1, (c->g.ny + RW - 1) / RW));
+    if (!c->lanes_fixed_W) W = std::min(W, std::max(1, (c->g.ny + RW - 1) / RW));
     return {J, W};
 }
 
@@ -1252,7 +1265,7 @@ std::pair<int, int> lanes_shape(const tsw_ctx *c) {
 bool lanes_ok(tsw_ctx *c) {
     if (c->lanes_opt <= 0 || !is_staggered(c->pattern) || c->pattern == Pattern::StaggeredGE || c->xs_deg > 0)
         return false;
-    return c->g.p <= 2;
+    return c->g.p <= 2 && c->g.npsi() * sizeof(double) < (size_t(1) << 31);  // 32-bit byte offsets
 }
 
 // Ordinate blocks of at most J ordinates, each quadrant split into equal parts (uploaded once per J).
@@ -1299,22 +1312,50 @@ T *lanes_records(tsw_ctx *c, int nbands) {
     return buf.as<T>();
 }
 
-// One ordinate-lane sweep, timed: psi_out <- L^{-1}(src + sigma_s phi / (4 pi)) or, with mom, the
-// per-block moment partials of that solution.  src (reference layout with the inflow in its halos), phi and
-// sig_s (cell arrays) may each be null (not src and phi both); the material table must be valid.
+// Moment partials of the ordinate-lane engine for its current block shape.
+double *lanes_mom(tsw_ctx *c) {
+    lanes_blocks(c, lanes_shape(c).first);
+    return dbuf<double>(c->lanes.mom, (size_t)c->lanes.nblk * c->g.nphi());
+}
+
+// Everything an ordinate-lane sweep of this context allocates or reads back (block table, element
+// inverses, exchange records, trace and moment buffers), so that later sweeps only launch kernels (and can
+// be captured in a graph).
 template <typename T>
-void lanes_sweep(tsw_ctx *c, double dt, const double *src, const double *phi, const double *sig_s, double *psi_out,
-                 double *mom) {
+void lanes_prepare(tsw_ctx *c, double dt, const double *src, bool mom) {
     const Geom &g = c->g;
-    hipStream_t s = c->stream;
+    const std::pair<int, int> shape = lanes_shape(c);
+    lanes_blocks(c, shape.first);
+    require(diag_inverses<T>(c, dt) != nullptr, TSW_E_STATE, "ordinate-lane sweep without element inverses");
+    const int rows = (mom ? shape.second : std::min(shape.second, kLanesStoreWaves)) * (64 / shape.first);
+    (void)lanes_records<T>(c, (g.ny + rows - 1) / rows);
+    if (src) {
+        auto &L = c->lanes;
+        const bool f64 = sizeof(T) == 8;
+        dbuf<T>(f64 ? L.tinx64 : L.tinx32, (size_t)g.nq * g.ny * (g.p + 1));
+        dbuf<T>(f64 ? L.tiny64 : L.tiny32, (size_t)g.nq * g.nx * (g.p + 1));
+    }
+    if (mom) (void)lanes_mom(c);
+}
+
+// Parameters of one ordinate-lane sweep: psi_out <- L^{-1}(src + sigma_s phi / (4 pi)) or, with mom, the
+// per-block moment partials of that solution.  src (reference layout with the inflow in its halos), phi and
+// sig_s (cell arrays) may each be null (not src and phi both); the material table must be valid.  With
+// `inflow` the traces of src's halos are computed here (a batch computes every member's in one launch);
+// threads and lds receive the launch shape.
+template <typename T>
+LanesParams<T> lanes_params(tsw_ctx *c, double dt, const double *src, const double *phi, const double *sig_s,
+                            double *psi_out, double *mom, bool inflow, int &threads, size_t &lds) {
+    const Geom &g = c->g;
     const bool f64 = sizeof(T) == 8;
     const std::pair<int, int> shape = lanes_shape(c);
-    const int J = shape.first, W = shape.second;
+    const int J = shape.first, W = mom ? shape.second : std::min(shape.second, kLanesStoreWaves);  // + storer
     lanes_blocks(c, J);
     LanesParams<T> sp{};
     sp.xi = f64 ? (const T *)c->xi64.ptr : (const T *)c->xi32.ptr;
     sp.eta = f64 ? (const T *)c->eta64.ptr : (const T *)c->eta32.ptr;
     sp.nx = g.nx; sp.ny = g.ny; sp.nq = g.nq;
+    sp.qo1 = g.quad_off[1]; sp.qo2 = g.quad_off[2]; sp.qo3 = g.quad_off[3];
     sp.dx = T(g.dx); sp.dy = T(g.dy);
     sp.blk = c->lanes.blk.as<LaneBlock>();
     sp.nblk = c->lanes.nblk;
@@ -1325,7 +1366,7 @@ void lanes_sweep(tsw_ctx *c, double dt, const double *src, const double *phi, co
         const size_t nt_x = (size_t)g.nq * g.ny * (g.p + 1), nt_y = (size_t)g.nq * g.nx * (g.p + 1);
         auto &L = c->lanes;
         T *tx = dbuf<T>(f64 ? L.tinx64 : L.tinx32, nt_x), *ty = dbuf<T>(f64 ? L.tiny64 : L.tiny32, nt_y);
-        launch_inflow<T>(g, src, tx, ty, s);
+        if (inflow) launch_inflow<T>(g, src, tx, ty, c->stream);
         sp.tin_x = tx;
         sp.tin_y = ty;
     }
@@ -1348,38 +1389,56 @@ void lanes_sweep(tsw_ctx *c, double dt, const double *src, const double *phi, co
     const int N = g.N();
     auto lds_of = [&](bool table) {
         const size_t vw = 16 / sizeof(T), ps = (g.p + 1 + vw - 1) / vw * vw;  // trace slot, 16-byte rows
-        size_t b = ((size_t)2 * rows * J * ps * sizeof(T) + 15) / 16 * 16;
-        if (mom) b += (size_t)2 * rows * N * J * sizeof(double);
+        size_t b = ((size_t)2 * rows * J * ps * sizeof(T) + 15) / 16 * 16;  // LanesLds: traces
+        if (mom) b += (size_t)2 * rows * N * J * sizeof(double);              // moment reduction
+        b += ((size_t)W * 64 * N * sizeof(T) + 15) / 16 * 16;                // per-wave source stage
+        if (!mom) b += ((size_t)2 * W * 64 * N * sizeof(T) + 15) / 16 * 16;  // storer stage
         if (table) b += ((size_t)sp.M * J * N * N * sizeof(T) + 15) / 16 * 16;
         return b;
     };
     sp.inv_lds = lds_of(true) <= 96 * 1024 ? 1 : 0;
-    const size_t lds = lds_of(sp.inv_lds != 0);
+    lds = lds_of(sp.inv_lds != 0);
     require(lds <= 160 * 1024, TSW_E_UNSUPPORTED, "ordinate-lane sweep: LDS layout too large");
+    threads = (W + (mom ? 0 : 1)) * 64;
+    c->last_lanes = 1;
+    c->last_lanes_J = J;
+    c->last_lanes_W = W;
+    c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
+    return sp;
+}
+
+// Launch of a lane sweep (or of a batch of them, all of DG degree p).
+template <typename T>
+void lanes_launch(int p, const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s,
+                  const LanesParams<T> *batch = nullptr, const int4 *items = nullptr, int nitems = 0) {
+    switch (p) {
+        case 0: launch_lanes_sweep_p<T, 1>(sp, threads, lds, s, batch, items, nitems); break;
+        case 1: launch_lanes_sweep_p<T, 2>(sp, threads, lds, s, batch, items, nitems); break;
+        case 2: launch_lanes_sweep_p<T, 3>(sp, threads, lds, s, batch, items, nitems); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
+    }
+}
+
+// One ordinate-lane sweep (lanes_params), timed.
+template <typename T>
+void lanes_sweep(tsw_ctx *c, double dt, const double *src, const double *phi, const double *sig_s, double *psi_out,
+                 double *mom) {
+    hipStream_t s = c->stream;
+    int threads = 0;
+    size_t lds = 0;
+    const LanesParams<T> sp = lanes_params<T>(c, dt, src, phi, sig_s, psi_out, mom, true, threads, lds);
     if (c->stall_test && sp.nbands > 1 && !c->skip_flag) {
         c->stall_test = false;
         *c->abort_host = 1;
         TSW_HIP(hipMemcpyAsync(c->abort_dev.ptr, c->abort_host, sizeof(int), hipMemcpyHostToDevice, s));
     }
-    c->last_lanes = 1;
-    c->last_lanes_J = J;
-    c->last_lanes_W = W;
-    c->last_br = sp.nbands > 1 ? sp.band_rows : 0;
-    auto launch = [&] {
-        switch (g.p) {
-            case 0: launch_lanes_sweep_p<T, 1>(sp, W * 64, lds, s); break;
-            case 1: launch_lanes_sweep_p<T, 2>(sp, W * 64, lds, s); break;
-            case 2: launch_lanes_sweep_p<T, 3>(sp, W * 64, lds, s); break;
-            default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
-        }
-    };
     if (c->skip_flag) {  // inside a graph capture: the replay is timed as a whole
-        launch();
+        lanes_launch<T>(c->g.p, sp, threads, lds, s);
         return;
     }
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     TSW_HIP(hipEventRecord(e0, s));
-    launch();
+    lanes_launch<T>(c->g.p, sp, threads, lds, s);
     TSW_HIP(hipEventRecord(e1, s));
     ++c->sweeps_done;
 }
@@ -1568,6 +1627,111 @@ void si_staggered_graph(tsw_ctx *c, double *Q, const double *st, const double *s
         diag_store<T>(c, psi_d, psi_state);
     }
     TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    iters = h.iters;
+    err = h.err;
+}
+
+// Source iteration on the ordinate-lane engine (the material table must be valid): every iteration
+// sweeps Q + S phi straight into per-block moments, so no angular flux is written until the end, where one
+// more sweep of the last iteration's source writes psi (the same values that iteration's sweep computed).
+// With `graph`, kSiChunk iterations are captured and replayed with the device-side loop test, as in
+// si_staggered_graph.
+template <typename T>
+void si_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, double dt, double abs_tol, int max_its,
+              bool graph, int &iters, double &err) {
+    const Geom &g = c->g;
+    hipStream_t s = c->stream;
+    double *psi_state = c->si_state.as<double>();
+    double *bufA = dbuf<double>(c->phi64a, g.nphi());
+    double *bufB = dbuf<double>(c->phi64b, g.nphi());
+    TSW_HIP(hipMemsetAsync(bufA, 0, sizeof(double) * g.nphi(), s));
+    TSW_HIP(hipMemsetAsync(bufB, 0, sizeof(double) * g.nphi(), s));
+    launch_pmv<double, double>(g, psi_state, bufA, c->w64.as<double>(), 1.0, 0.0, 0, s);  // :214
+    double *mom = lanes_mom(c);
+    const int nblk = c->lanes.nblk;
+    lanes_prepare<T>(c, dt, Q, true);  // tables, records and trace buffers exist before any capture
+    auto iteration = [&](double *cur, const double *last) {
+        lanes_sweep<T>(c, dt, Q, last, ss, nullptr, mom);           // L^{-1}(Q + S phi) (:226-232) ...
+        launch_mom_sum(g, nblk, mom, cur, cur, 1.0, 0.0, s, c->skip_flag);  // ... its moments (:236)
+    };
+    int it = 0;
+    double e = 1.79769313486231570e+308;
+    if (!graph) {
+        double *cur = bufA, *last = bufB;
+        while (it < max_its && e > abs_tol) {
+            ++it;
+            std::swap(cur, last);
+            iteration(cur, last);
+            e = device_norm2_interior_diff(c, cur, last);  // :239
+        }
+        if (it > 0) {
+            lanes_sweep<T>(c, dt, Q, last, ss, Q, nullptr);  // psi of the last iteration
+            TSW_HIP(hipMemcpyAsync(psi_state, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+        } else {
+            TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+        }
+        iters = it;
+        err = e;
+        return;
+    }
+    const int nb = reduction_blocks();
+    double *part = dbuf<double>(c->red_part, (size_t)nb * 64);
+    SiControl *ctl = static_cast<SiControl *>(c->si_ctl.reserve(sizeof(SiControl)));
+    if (!c->si_ctl_host) TSW_HIP(hipHostMalloc((void **)&c->si_ctl_host, sizeof(SiControl), hipHostMallocDefault));
+    SiControl init{};
+    init.done = max_its <= 0 ? 1 : 0;
+    init.max_its = max_its;
+    init.tol = abs_tol;
+    init.err = 1.79769313486231570e+308;
+    TSW_HIP(hipMemcpyAsync(ctl, &init, sizeof(SiControl), hipMemcpyHostToDevice, s));
+    TSW_HIP(hipStreamSynchronize(s));
+    // iteration k (1-based) writes phi into bufB when k is odd, bufA when even
+    hipGraph_t gr;
+    c->skip_flag = &ctl->done;
+    TSW_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int j = 0; j < tsw_ctx::kSiChunk; ++j) {
+            double *cur = (j & 1) ? bufA : bufB, *last = (j & 1) ? bufB : bufA;
+            iteration(cur, last);
+            launch_si_check(g, cur, last, part, nb, ctl, s);  // :239 and the loop test
+        }
+    } catch (...) {
+        c->skip_flag = nullptr;
+        (void)hipStreamEndCapture(s, &gr);
+        throw;
+    }
+    c->skip_flag = nullptr;
+    TSW_HIP(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t exec;
+    const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    TSW_HIP(ie);
+    try {
+        SiControl &h = *c->si_ctl_host;
+        h = init;
+        while (!h.done) {
+            const int before = h.iters;
+            hipEvent_t e0 = next_event(c), e1 = next_event(c);
+            TSW_HIP(hipEventRecord(e0, s));
+            TSW_HIP(hipGraphLaunch(exec, s));
+            TSW_HIP(hipEventRecord(e1, s));
+            TSW_HIP(hipMemcpyAsync(&h, ctl, sizeof(SiControl), hipMemcpyDeviceToHost, s));
+            sync(c);
+            c->sweeps_done += h.iters - before;
+        }
+    } catch (...) {
+        (void)hipGraphExecDestroy(exec);
+        throw;
+    }
+    (void)hipGraphExecDestroy(exec);
+    const SiControl &h = *c->si_ctl_host;
+    if (h.iters > 0) {
+        const double *last = (h.iters & 1) ? bufA : bufB;  // the scalar flux the last iteration swept
+        lanes_sweep<T>(c, dt, Q, last, ss, Q, nullptr);
+        TSW_HIP(hipMemcpyAsync(psi_state, Q, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    } else {
+        TSW_HIP(hipMemcpyAsync(Q, psi_state, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));
+    }
     iters = h.iters;
     err = h.err;
 }
@@ -1826,12 +1990,6 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
     }
     if (pd) TSW_HIP(hipMemcpyAsync(pd, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     return converged;
-}
-
-// Moment partials of the ordinate-lane engine for its current block shape.
-double *lanes_mom(tsw_ctx *c) {
-    lanes_blocks(c, lanes_shape(c).first);
-    return dbuf<double>(c->lanes.mom, (size_t)c->lanes.nblk * c->g.nphi());
 }
 
 // Peierls GMRES on the ordinate-lane engine (the material table must be valid).  The same algorithm as
@@ -2704,6 +2862,23 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
+        bool lanes = false;
+        if (lanes_ok(ctx)) {
+            prepare_materials(ctx, st, dt);
+            lanes = ctx->mat.valid;
+        }
+        ctx->last_lanes = lanes ? 1 : 0;
+        if (lanes) {
+            int it = 0;
+            double e = 0.0;
+            if (precision == TSW_FP64) si_lanes<double>(ctx, Q, st, ss, dt, abs_tol, max_its, ctx->si_graph, it, e);
+            else si_lanes<float>(ctx, Q, st, ss, dt, abs_tol, max_its, ctx->si_graph, it, e);
+            if (mem == TSW_MEM_HOST || Q != source_inout) stage_out(ctx, source_inout, Q, g.npsi(), mem);
+            sync(ctx);
+            if (iters) *iters = it;
+            if (err) *err = e;
+            return;
+        }
         if (staggered_ok(ctx)) {
             int it = 0;
             double e = 0.0;
@@ -3012,6 +3187,8 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
         require(nprob >= 1 && probs && sigma_t, TSW_E_ARG, "empty batch");
         ctx->batch.clear();
         ctx->groups.clear();
+        ctx->lane_groups.clear();
+        ctx->lane_member.assign(nprob, 0);
         ctx->batch_dt = dt;
         for (int b = 0; b < nprob; ++b) {
             const tsw_problem &pr = probs[b];
@@ -3029,12 +3206,25 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             double *sig = static_cast<double *>(sub->sig_t64.reserve(sizeof(double) * g.ncell_h()));
             TSW_HIP(hipMemcpyAsync(sig, sigma_t[b], sizeof(double) * g.ncell_h(),
                                    mem == TSW_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
-            diag_setup(sub.get(), sig, nullptr, nullptr, dt, true, true);
+            bool lane = false;
+            if (ctx->lanes_opt > 0) {  // the ordinate-lane engine where it is built for the member
+                sub->lanes_opt = 1;
+                sub->lanes_J_opt = ctx->lanes_J_opt;
+                sub->lanes_W_opt = kLanesStoreWaves;
+                sub->lanes_fixed_W = true;
+                if (lanes_ok(sub.get())) {
+                    prepare_materials(sub.get(), sig, dt);
+                    lane = sub->mat.valid && diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt);
+                }
+            }
+            ctx->lane_member[b] = lane;
+            if (!lane) diag_setup(sub.get(), sig, nullptr, nullptr, dt, true, true);
             ctx->batch.push_back(std::move(sub));
         }
         // launch groups
         std::map<std::tuple<int, int, bool>, int> gid;
         for (int b = 0; b < nprob; ++b) {
+            if (ctx->lane_member[b]) continue;
             tsw_ctx *sub = ctx->batch[b].get();
             const auto key = std::make_tuple(sub->g.p, (sub->g.ny + 63) / 64, sub->mat.valid);
             auto it = gid.find(key);
@@ -3063,6 +3253,43 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             }
             TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
                                    sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        // lane groups by DG degree; items band-major, heaviest members first within a band
+        std::map<int, int> lid;
+        for (int b = 0; b < nprob; ++b) {
+            if (!ctx->lane_member[b]) continue;
+            const int p = ctx->batch[b]->g.p;
+            auto it = lid.find(p);
+            if (it == lid.end()) {
+                it = lid.emplace(p, (int)ctx->lane_groups.size()).first;
+                ctx->lane_groups.emplace_back();
+                ctx->lane_groups.back().p = p;
+            }
+            ctx->lane_groups[it->second].members.push_back(b);
+        }
+        for (auto &LG : ctx->lane_groups) {
+            std::vector<std::tuple<long long, int>> order;  // (-work, slot)
+            std::vector<int> nblk(LG.members.size()), nbands(LG.members.size());
+            int max_bands = 0;
+            for (int slot = 0; slot < (int)LG.members.size(); ++slot) {
+                tsw_ctx *sub = ctx->batch[LG.members[slot]].get();
+                const std::pair<int, int> shape = lanes_shape(sub);
+                lanes_blocks(sub, shape.first);
+                const int rows = shape.second * (64 / shape.first);
+                nblk[slot] = sub->lanes.nblk;
+                nbands[slot] = (sub->g.ny + rows - 1) / rows;
+                max_bands = std::max(max_bands, nbands[slot]);
+                order.emplace_back(-(long long)(sub->g.nx + sub->g.ny) * sub->g.nq * sub->g.N(), slot);
+            }
+            std::sort(order.begin(), order.end());
+            for (int band = 0; band < max_bands; ++band)
+                for (auto &o : order) {
+                    const int slot = std::get<1>(o);
+                    if (band < nbands[slot])
+                        for (int bi = 0; bi < nblk[slot]; ++bi) LG.items.push_back(int4{slot, bi, band, 0});
+                }
+            TSW_HIP(hipMemcpyAsync(LG.dev_items.reserve(sizeof(int4) * LG.items.size()), LG.items.data(),
+                                   sizeof(int4) * LG.items.size(), hipMemcpyHostToDevice, ctx->stream));
         }
         for (int k = 0; k < tsw_ctx::kSideStreams; ++k) {
             if (!ctx->side[k]) TSW_HIP(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
@@ -3141,13 +3368,68 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
     built = true;
 }
 
+// The lane groups' parameter tables for one batched sweep of lane members b in place over arr[b] (their
+// inflow from arr[b]'s halos), uploaded on the context's stream; threads / lds per group.
 template <typename T>
-void batch_sweep(tsw_ctx *ctx, bool corr = false) {
+void lanes_batch_params(tsw_ctx *ctx, const std::vector<double *> &arr, std::vector<int> &threads,
+                        std::vector<size_t> &lds) {
+    threads.assign(ctx->lane_groups.size(), 0);
+    lds.assign(ctx->lane_groups.size(), 0);
+    for (size_t k = 0; k < ctx->lane_groups.size(); ++k) {
+        auto &LG = ctx->lane_groups[k];
+        LG.host.resize(sizeof(LanesParams<T>) * LG.members.size());
+        LanesParams<T> *ps = reinterpret_cast<LanesParams<T> *>(LG.host.data());
+        for (size_t slot = 0; slot < LG.members.size(); ++slot) {
+            const int b = LG.members[slot];
+            int th = 0;
+            size_t l = 0;
+            ps[slot] = lanes_params<T>(ctx->batch[b].get(), ctx->batch_dt, arr[b], nullptr, nullptr, arr[b], nullptr,
+                                       false, th, l);
+            require(threads[k] == 0 || threads[k] == th, TSW_E_STATE, "lane batch members of unequal workgroups");
+            threads[k] = th;
+            lds[k] = std::max(lds[k], l);
+        }
+        TSW_HIP(hipMemcpyAsync(LG.params.reserve(LG.host.size()), LG.host.data(), LG.host.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+}
+
+template <typename T>
+void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStream_t s) {
+    auto &LG = ctx->lane_groups[k];
+    const LanesParams<T> *dp = LG.params.as<LanesParams<T>>();
+    const int n = (int)LG.members.size();
+    switch (LG.p) {
+        case 0: launch_lanes_inflow_batch_p<T, 1>(dp, n, s); break;
+        case 1: launch_lanes_inflow_batch_p<T, 2>(dp, n, s); break;
+        case 2: launch_lanes_inflow_batch_p<T, 3>(dp, n, s); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
+    }
+    const LanesParams<T> &rep = reinterpret_cast<const LanesParams<T> *>(LG.host.data())[0];  // the mode
+    lanes_launch<T>(LG.p, rep, threads, lds, s, dp, LG.dev_items.as<int4>(), (int)LG.items.size());
+}
+
+// One batched sweep: the staggered launch groups (diagonal-layout buffers of their members) and, with
+// `lanes`, the lane groups over lanes[b] in place; all on the side streams, timed as one sweep.
+template <typename T>
+void batch_sweep(tsw_ctx *ctx, bool corr = false, const std::vector<double *> *lanes = nullptr, bool lanes64 = false) {
+    if (ctx->groups.empty() && !lanes) return;
     for (auto &G : ctx->groups) batch_group_params<T>(ctx, G, corr);
+    std::vector<int> lthreads;
+    std::vector<size_t> llds;
+    if (lanes && lanes64) lanes_batch_params<double>(ctx, *lanes, lthreads, llds);
+    else if (lanes) lanes_batch_params<float>(ctx, *lanes, lthreads, llds);
     hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
     TSW_HIP(hipEventRecord(e0, ctx->stream));
     TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     int k = 0;
+    if (lanes)
+        for (size_t j = 0; j < ctx->lane_groups.size(); ++j, ++k) {
+            hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
+            TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
+            if (lanes64) lanes_batch_launch<double>(ctx, j, lthreads[j], llds[j], sd);
+            else lanes_batch_launch<float>(ctx, j, lthreads[j], llds[j], sd);
+        }
     for (auto &G : ctx->groups) {
         hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
         TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
@@ -3206,10 +3488,14 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
         // inputs per member on the side streams: inflow traces and the source in the diagonal layout
         on_side_streams([&](size_t b, tsw_ctx *sub) {
             dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
-            diag_load_source(sub, dev[b], f64 || mixed, !f64);
+            if (!ctx->lane_member[b]) diag_load_source(sub, dev[b], f64 || mixed, !f64);
         });
-        if (f64) batch_sweep<double>(ctx);
-        else batch_sweep<float>(ctx);
+        // Lane members are swept in place on the caller's layout.  Under TSW_MIXED they take the FP64
+        // sweep: that engine reads and writes FP64 in both precisions and its FP32 arithmetic is not
+        // faster, so a refinement would only add a residual and a second sweep.
+        const bool any_lanes = !ctx->lane_groups.empty();
+        if (f64) batch_sweep<double>(ctx, false, any_lanes ? &dev : nullptr, true);
+        else batch_sweep<float>(ctx, false, any_lanes ? &dev : nullptr, mixed);
         // the refinement of mixed_refine_store, its FP32 sweeps batched across members
         const bool fused_store = mixed && ctx->refine_steps == 1;
         auto residual = [&](tsw_ctx *sub, const auto *psi) {
@@ -3220,26 +3506,31 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
                                         dbuf<float>(d.res32, nd_of(sub)), sub->stream);
         };
         if (fused_store) {
-            on_side_streams([&](size_t, tsw_ctx *sub) { residual(sub, sub->diag.q32.as<float>()); });
+            on_side_streams([&](size_t b, tsw_ctx *sub) {
+                if (!ctx->lane_member[b]) residual(sub, sub->diag.q32.as<float>());
+            });
             batch_sweep<float>(ctx, true);
         } else if (mixed) {
-            on_side_streams([&](size_t, tsw_ctx *sub) {
+            on_side_streams([&](size_t b, tsw_ctx *sub) {
+                if (ctx->lane_member[b]) return;
                 launch_convert_f2d(sub->diag.q32.as<float>(), dbuf<double>(sub->diag.ir64, nd_of(sub)), nd_of(sub),
                                    sub->stream);
             });
             for (int k = 0; k < ctx->refine_steps; ++k) {
-                on_side_streams([&](size_t, tsw_ctx *sub) {
-                    residual(sub, static_cast<const double *>(sub->diag.ir64.as<double>()));
+                on_side_streams([&](size_t b, tsw_ctx *sub) {
+                    if (!ctx->lane_member[b]) residual(sub, static_cast<const double *>(sub->diag.ir64.as<double>()));
                 });
                 batch_sweep<float>(ctx, true);
-                on_side_streams([&](size_t, tsw_ctx *sub) {
+                on_side_streams([&](size_t b, tsw_ctx *sub) {
+                    if (ctx->lane_member[b]) return;
                     launch_accumulate_f2d(sub->diag.res32.as<float>(), sub->diag.ir64.as<double>(), nd_of(sub),
                                           sub->stream);
                 });
             }
         }
         on_side_streams([&](size_t b, tsw_ctx *sub) {
-            if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
+            if (ctx->lane_member[b]) {
+            } else if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
             else if (fused_store)
                 launch_psi_store_sum(sub->g, dev[b], sub->diag.q32.as<float>(), sub->diag.res32.as<float>(), sub->stream);
             else if (mixed) diag_store<double>(sub, sub->diag.ir64.as<double>(), dev[b]);

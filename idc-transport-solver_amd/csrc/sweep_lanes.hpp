@@ -10,6 +10,14 @@
 // no layout conversion.  The x-upwind trace stays in registers, the y-upwind trace passes through LDS
 // (between row groups) or through the band exchange records (between bands, as in k_diag_sweep).
 //
+// Cell I/O is coalesced through LDS stages: lane l of a wave loads elements l, l + 64, ... of the
+// concatenation of its row groups' runs, so consecutive lanes touch consecutive values, and a per-wave
+// stage hands each value to the lane of its ordinate and coefficient.  Loaded per lane instead, the 64
+// lanes of an access sit N values apart and the address unit, not HBM, sets the step time.  Results go the
+// other way through a double-buffered stage to a storer wave (the workgroup's last wave), which writes step
+// t-1 during step t: compute waves then issue no stores, so their prefetch loads are not queued behind
+// stores (vector memory operations complete in issue order).
+//
 // Element solve: the material table dx dy A^{-1} per (material, ordinate); a lane keeps its ordinate's
 // matrix for the current material in registers and reloads it (from LDS when the block's table fits,
 // else from global memory) when its row enters another material.  The right-hand side, the matrix-vector
@@ -33,26 +41,49 @@ template <typename T, int P>
 struct LanesLds {  // LDS carve-up in bytes, every array 16-byte aligned
     static constexpr int N = P * P;
     static constexpr int PS = vround<T>(P);
-    size_t ex, red, inv, total;
-    // slots = rows of the band * J; M, J: material table [M][J][N*N] when `table`
-    __host__ __device__ LanesLds(int slots, int rows, int J, int M, bool mom, bool table) {
-        ex = 0;                                                     // [2][slots][PS] T
+    size_t ex, red, stage, out, inv, total;
+    // slots = rows of the band * J; compute waves of the workgroup; M, J: material table [M][J][N*N] when
+    // `table`; mom: moments instead of psi (no storer stage)
+    __host__ __device__ LanesLds(int slots, int rows, int waves, int J, int M, bool mom, bool table) {
+        ex = 0;                                                          // [2][slots][PS] T
         red = ex + ((size_t)2 * slots * PS * sizeof(T) + 15) / 16 * 16;  // [2][rows][N][J] double
-        inv = red + (mom ? (size_t)2 * rows * N * J * sizeof(double) : 0);
+        stage = red + (mom ? (size_t)2 * rows * N * J * sizeof(double) : 0);  // [waves][64 N] T
+        out = stage + ((size_t)waves * 64 * N * sizeof(T) + 15) / 16 * 16;    // [2][waves][64 N] T
+        inv = out + (mom ? 0 : ((size_t)2 * waves * 64 * N * sizeof(T) + 15) / 16 * 16);
         total = inv + (table ? ((size_t)M * J * N * N * sizeof(T) + 15) / 16 * 16 : 0);
     }
 };
 
-template <typename T, int P, int MODE, int D>
-__global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
+// The parameters of a workgroup: the kernel argument, or a batch member's entry (a uniform load from a
+// read-only table, so its fields stay in scalar registers).
+template <bool BATCH, typename T>
+__device__ __forceinline__ LanesParams<T> lanes_params_of(const LanesParams<T> &sp1,
+                                                          const LanesParams<T> *__restrict__ batch, int member) {
+    if constexpr (BATCH) return batch[member];
+    else return sp1;
+}
+
+// vmcnt(0) (expcnt, lgkmcnt left at their maxima): wait for every outstanding vector memory operation.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// One problem (sp1; workgroup = band x ordinate block) or, with BATCH, a batch (items[workgroup] = {problem,
+// block, band}, parameters batch[problem]; items are band-major, so a band's upstream neighbour is dispatched
+// before it).
+template <typename T, int P, int MODE, int D, bool BATCH>
+__global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp1, const LanesParams<T> *__restrict__ batch,
+                                                     const int4 *__restrict__ items) {
     constexpr int N = P * P, NN = N * N;
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0, MOM = (MODE & 4) != 0;
     using Lds = LanesLds<T, P>;
     constexpr int PS = Lds::PS;
     using U = typename ExchangeSentinel<T>::U;
+    int4 item = {0, 0, 0, 0};
+    if constexpr (BATCH) item = items[blockIdx.x];
+    const LanesParams<T> sp = lanes_params_of<BATCH>(sp1, batch, item.x);
     if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
     const int nb = sp.nbands < 2 ? 1 : sp.nbands;
-    const int bi = (int)(blockIdx.x % (unsigned)sp.nblk), band = (int)(blockIdx.x / (unsigned)sp.nblk);
+    const int bi = BATCH ? item.y : (int)(blockIdx.x % (unsigned)sp.nblk);
+    const int band = BATCH ? item.z : (int)(blockIdx.x / (unsigned)sp.nblk);
     const LaneBlock bk = sp.blk[bi];
     const int nx = sp.nx, ny = sp.ny, nq = sp.nq, J = sp.J, RW = sp.RW;
     const bool xneg = bk.quad >= 2, yneg = (bk.quad & 1) != 0;
@@ -67,14 +98,17 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
     const int q = bk.q0 + (j < bk.n ? j : 0);  // spare lanes mirror the block's first ordinate
     const int rr = R0 + (rl < nrows ? rl : nrows - 1);
     const int jc = yneg ? ny - rr : rr + 1;  // reference-layout j of the row
-    const int rows = (int)(blockDim.x >> 6) * RW;
+    const int waves = (int)(blockDim.x >> 6) - (MOM ? 0 : 1);  // compute waves (then the storer wave)
+    const int rows = waves * RW;
     const int slots = rows * J;
     const int slot = gi < RW ? rl * J + j : j;            // LDS trace slot of the lane
     const int slot_up = gi < RW && rl > 0 ? slot - J : slot;  // the row below's slot
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const Lds L(slots, rows, J, sp.M, MOM, sp.inv_lds != 0);
+    const Lds L(slots, rows, waves, J, sp.M, MOM, sp.inv_lds != 0);
     T *ex = reinterpret_cast<T *>(smem_raw + L.ex);
     double *red = reinterpret_cast<double *>(smem_raw + L.red);
+    T *stage = reinterpret_cast<T *>(smem_raw + L.stage) + (size_t)wv * 64 * N;  // this wave's stage
+    T *outst = reinterpret_cast<T *>(smem_raw + L.out);                          // [2][waves][64 N]
     T *linv = reinterpret_cast<T *>(smem_raw + L.inv);
 
     if (sp.inv_lds)  // the block's slice of the material table: [M][J][N*N]
@@ -84,31 +118,89 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
             linv[k] = G(sp.ainv)[((size_t)m * nq + qq) * NN + e];
         }
 
+    const int steps = nx + nrows - 1;
+    const int nsteps = ((steps + D - 1) / D) * D;
+    const int dstep_b = (int)((xneg ? -1LL : 1LL) * (ny + 2) * nq * N * (long long)sizeof(double));
+    const unsigned long long psi_bytes = (unsigned long long)(nx + 2) * (ny + 2) * nq * N * sizeof(double);
+    // Coalesced I/O: element m of lane l of compute wave w is element l + 64 m of w's runs (run g = row group
+    // g's cell, J N values from the block's first ordinate; elements past the block's n N values, past the
+    // band's rows or past RW runs are not moved).  Its byte offset in the cell of column c is
+    // io_off + c * dstep_b (io_off < 2^31, checked by the host), its row of the band io_row.
+    const int run = J * N, nrun = bk.n * N;
+    auto io_elem = [&](int w, int m, int &off, int &row, bool &ok) {
+        const int e = lane + 64 * m, g = e / run, o = e - g * run, rg = w * RW + g;
+        ok = g < RW && o < nrun && rg < nrows;
+        row = rg;
+        const int rrg = R0 + (rg < nrows ? rg : nrows - 1);
+        const long long c0 = (long long)(xneg ? nx : 1) * (ny + 2) + (yneg ? ny - rrg : rrg + 1);
+        off = (int)(((c0 * nq + bk.q0) * N + o) * (long long)sizeof(double));
+    };
+    if (!MOM && wv == waves) {  // the storer wave (at most kLanesStoreWaves compute waves): step t-1 during step t
+        const __amdgpu_buffer_rsrc_t rpsi = buf_rsrc(sp.psi, psi_bytes);
+        int soff[kLanesStoreWaves][N], srow[kLanesStoreWaves][N];
+        bool sok[kLanesStoreWaves][N];
+#pragma unroll
+        for (int w = 0; w < kLanesStoreWaves; ++w)
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                io_elem(w, m, soff[w][m], srow[w][m], sok[w][m]);
+                sok[w][m] = sok[w][m] && w < waves;
+            }
+        lds_barrier();  // the compute waves' first barrier (after their first prefetch)
+        auto store = [&](int t) {
+            const T *sb = outst + (size_t)(t & 1) * waves * 64 * N;
+#pragma unroll
+            for (int w = 0; w < kLanesStoreWaves; ++w)
+#pragma unroll
+                for (int m = 0; m < N; ++m) {
+                    const int cm = t - srow[w][m];
+                    if (sok[w][m] && (unsigned)cm < (unsigned)nx)
+                        buf_st<double>(rpsi, (unsigned)soff[w][m] + (unsigned)cm * (unsigned)dstep_b, 0,
+                                       (double)sb[(size_t)w * 64 * N + lane + 64 * m]);
+                }
+        };
+        for (int t = 0; t < nsteps; ++t) {
+            if (t > 0) store(t - 1);
+            lds_barrier();
+        }
+        store(nsteps - 1);
+        return;
+    }
+
     const gptr<U> xin = (nb > 1 && band > 0) ? G(reinterpret_cast<U *>(sp.xtr)) + ((size_t)q * (nb - 1) + band - 1) * nx * P : nullptr;
     const gptr<U> xout = (nb > 1 && band < nb - 1) ? G(reinterpret_cast<U *>(sp.xtr)) + ((size_t)q * (nb - 1) + band) * nx * P : nullptr;
     const gptr<const U> tiny = (HASQ && band == 0 && sp.tin_y) ? G(reinterpret_cast<const U *>(sp.tin_y)) + (size_t)q * nx * P : nullptr;
     const bool yrow = rl == 0;  // takes its y-inflow from tin_y (band 0) or the record of the band below
     const bool ywave = __builtin_amdgcn_readfirstlane(wv) == 0;
 
-    // per-lane addressing: cell (c, row) sits at reference cell (ic, jc), ic = c + 1 or nx - c
-    const long long cstride = (long long)(ny + 2) * nq * N;  // doubles between cells i and i+1
-    const long long dstep = xneg ? -cstride : cstride;
+    // this lane's row: cell (c, row) sits at reference cell (ic, jc), ic = c + 1 or nx - c
     const long long cell0 = ((long long)(xneg ? nx : 1) * (ny + 2) + jc);  // reference cell of c = 0
-    const long long base0 = (cell0 * nq + q) * N;
     const int mstep = xneg ? -(ny + 2) : (ny + 2);
 
-    const int steps = nx + nrows - 1;
-    const int nsteps = ((steps + D - 1) / D) * D;
     const int w0 = __builtin_amdgcn_readfirstlane(wv * RW);  // first row of this wave
     const int wrows = w0 < nrows ? (nrows - w0 < RW ? nrows - w0 : RW) : 0;
     const int live_n = wrows > 0 ? wrows - 1 + nx : 0;  // steps in [w0, w0 + live_n) touch this wave's rows
+
+    int io_off[HASQ ? N : 1], io_row[HASQ ? N : 1];  // this lane's source elements
+    if (HASQ) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            bool ok;
+            io_elem(wv, m, io_off[m], io_row[m], ok);
+        }
+    }
+    const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(HASQ ? sp.src : nullptr, HASQ ? psi_bytes : 0);
+    // this lane's values in the stage: lane = gi J + j, so the row groups' runs are the first RW J N entries
+    // and lanes past them (which compute nothing useful) write to the unused tail
+    const int own = lane * N;
 
     T tx[P];  // x-inflow trace of the row (tin_x), then the outflow trace of the last cell
 #pragma unroll
     for (int d = 0; d < P; ++d)
         tx[d] = (HASQ && sp.tin_x) ? G(sp.tin_x)[((size_t)q * ny + rr) * P + d] : T(0);
 
-    double pq[D][HASQ ? N : 1], pf[D][HASPHI ? N : 1], ps[D];
+    double pq[D][HASQ ? N : 1];  // the source, in the coalesced arrangement (lane element m)
+    double pf[D][HASPHI ? N : 1], ps[D];
     int pm[D];
     U py[D][P];
     int ft = 0;
@@ -116,10 +208,12 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
         const int c = ft - rl;
         const int cc = c < 0 ? 0 : (c >= nx ? nx - 1 : c);
         const long long cell = cell0 + (long long)cc * mstep;
-        if (HASQ) {
-            const gptr<const double> p = G(sp.src) + base0 + (long long)cc * dstep;
+        if (HASQ) {  // elements of cells outside the mesh read a valid address or, past the array, zero
 #pragma unroll
-            for (int k = 0; k < N; ++k) pq[u][k] = p[k];
+            for (int m = 0; m < N; ++m) {
+                const int cm = ft - io_row[m];
+                pq[u][m] = buf_ld<double>(rsrc, (unsigned)io_off[m] + (unsigned)cm * (unsigned)dstep_b, 0);
+            }
         }
         if (HASPHI) {
             const gptr<const double> p = G(sp.phi) + cell * N;
@@ -188,7 +282,10 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
                 T ty[P];
                 if (yrow) {
                     if (xin) {
-                        if (active && py[u][0] == ExchangeSentinel<T>::bits) {
+                        bool empty = false;  // the record is filled value by value: wait for all P
+#pragma unroll
+                        for (int d = 0; d < P; ++d) empty = empty || py[u][d] == ExchangeSentinel<T>::bits;
+                        if (active && empty) {
                             Record<U, P> rec;
 #pragma unroll
                             for (int d = 0; d < P; ++d) rec.v[d] = py[u][d];
@@ -215,7 +312,22 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
                         const gptr<const T> src = G(sp.ainv) + ((size_t)cur_mat * nq + q) * NN;
 #pragma unroll
                         for (int k = 0; k < NN; ++k) A[k] = src[k];
+                        // Wait here, on the rare reload: vector memory operations complete in issue order,
+                        // so a wait for A after the branch (where the compiler would put it) would also
+                        // wait for the prefetches in flight on every step, reloaded or not.
+                        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                     }
+                }
+                T q_own[HASQ ? N : 1];  // this lane's source values, through the stage
+                if (HASQ) {
+#pragma unroll
+                    for (int m = 0; m < N; ++m) stage[lane + 64 * m] = T(pq[u][m]);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+                    for (int k = 0; k < N; ++k) q_own[k] = stage[own + k];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
                 }
                 T B[N];
 #pragma unroll
@@ -226,9 +338,9 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
                         T a;
                         if (HASPHI) {
                             const T wsc = T(1) * T(ps[u]) * T(kInv4Pi);
-                            a = HASQ ? fmadd(wsc, T(pf[u][k]), T(pq[u][k])) : wsc * T(pf[u][k]);
+                            a = HASQ ? fmadd(wsc, T(pf[u][k]), q_own[k]) : wsc * T(pf[u][k]);
                         } else {
-                            a = T(pq[u][k]);
+                            a = q_own[k];
                         }
                         B[k] = fmadd(ky[e], ty[d], fmadd(kx[d], tx[e], a));
                     }
@@ -246,10 +358,10 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
 #pragma unroll
                         for (int k = 0; k < N; ++k) rb[k * J + j] = wq * (double)x[k];
                     }
-                } else if (active) {
-                    const gptr<double> po = G(sp.psi) + base0 + (long long)cc * dstep;
+                } else {  // to the storer wave (stored during the next step)
+                    T *ob = outst + ((size_t)(t & 1) * waves + wv) * 64 * N;
 #pragma unroll
-                    for (int k = 0; k < N; ++k) po[k] = (double)x[k];
+                    for (int k = 0; k < N; ++k) ob[own + k] = x[k];
                 }
 #pragma unroll
                 for (int e = 0; e < P; ++e) {  // outflow x-trace: sum_a sx^a psi(a, e)
@@ -286,30 +398,76 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp) {
 }
 
 template <typename T, int P, int MODE>
-void launch_lanes_d(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s) {
-    auto kern = k_lanes_sweep<T, P, MODE, sizeof(T) == 8 ? 2 : 3>;
-    if (lds > 64 * 1024)
-        TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int blocks = sp.nblk * (sp.nbands > 1 ? sp.nbands : 1);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, sp);
-    TSW_HIP(hipGetLastError());
+void launch_lanes_d(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s, const LanesParams<T> *batch,
+                    const int4 *items, int nitems) {
+    constexpr int D = sizeof(T) == 8 ? 2 : 3;
+    auto go = [&](auto kern, int blocks) {
+        if (lds > 64 * 1024)
+            TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, sp, batch, items);
+        TSW_HIP(hipGetLastError());
+    };
+    if (!batch) {
+        go(k_lanes_sweep<T, P, MODE, D, false>, sp.nblk * (sp.nbands > 1 ? sp.nbands : 1));
+    } else if constexpr (MODE == 1) {  // batches sweep sources in place
+        go(k_lanes_sweep<T, P, MODE, D, true>, nitems);
+    } else {
+        throw Error(TSW_E_ARG, "ordinate-lane batch: sources only");
+    }
 }
 
-// Entry point per (precision, DG order P); MODE from the params (src, phi, mom).
+// Entry point per (precision, DG order P); MODE from the params (src, phi, mom), those of `sp` for a batch
+// (whose members share the mode).
 template <typename T, int P>
-void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s) {
+void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s,
+                          const LanesParams<T> *batch, const int4 *items, int nitems) {
     const int mode = (sp.src ? 1 : 0) | (sp.phi ? 2 : 0) | (sp.mom ? 4 : 0);
     switch (mode) {
-        case 1: launch_lanes_d<T, P, 1>(sp, threads, lds, s); break;
-        case 3: launch_lanes_d<T, P, 3>(sp, threads, lds, s); break;
-        case 5: launch_lanes_d<T, P, 5>(sp, threads, lds, s); break;
-        case 6: launch_lanes_d<T, P, 6>(sp, threads, lds, s); break;
-        case 7: launch_lanes_d<T, P, 7>(sp, threads, lds, s); break;
+        case 1: launch_lanes_d<T, P, 1>(sp, threads, lds, s, batch, items, nitems); break;
+        case 3: launch_lanes_d<T, P, 3>(sp, threads, lds, s, batch, items, nitems); break;
+        case 5: launch_lanes_d<T, P, 5>(sp, threads, lds, s, batch, items, nitems); break;
+        case 6: launch_lanes_d<T, P, 6>(sp, threads, lds, s, batch, items, nitems); break;
+        case 7: launch_lanes_d<T, P, 7>(sp, threads, lds, s, batch, items, nitems); break;
         default: throw Error(TSW_E_ARG, "ordinate-lane sweep: unsupported source / output combination");
     }
 }
 
-#define TSW_LANES_INSTANTIATE(T, P) \
-    template void launch_lanes_sweep_p<T, P>(const LanesParams<T> &, int, size_t, hipStream_t);
+// Inflow traces of a batch's sources (k_inflow per member): item y = member, x over ordinates x rows or columns.
+template <typename T, int P>
+__global__ void k_lanes_inflow_batch(const LanesParams<T> *__restrict__ batch) {
+    constexpr int N = P * P;
+    const LanesParams<T> sp = batch[blockIdx.y];
+    const int nx = sp.nx, ny = sp.ny, nq = sp.nq, len = nx > ny ? nx : ny;
+    for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < nq * len; t += (int)(gridDim.x * blockDim.x)) {
+        const int q = t / len, r = t - q * len;
+        const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
+        const bool xneg = Q >= 2, yneg = (Q & 1) != 0;
+        if (r < ny) {  // row r: the x-inflow halo cell
+            const int j = yneg ? ny - r : r + 1, i = xneg ? nx + 1 : 0;
+            double tr[P];
+            trace_x<double, P>(sp.src + (((size_t)i * (ny + 2) + j) * nq + q) * N, xneg, tr);
+#pragma unroll
+            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_x)[((size_t)q * ny + r) * P + d] = T(tr[d]);
+        }
+        if (r < nx) {  // column r: the y-inflow halo cell
+            const int i = xneg ? nx - r : r + 1, j = yneg ? ny + 1 : 0;
+            double tr[P];
+            trace_y<double, P>(sp.src + (((size_t)i * (ny + 2) + j) * nq + q) * N, yneg, tr);
+#pragma unroll
+            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_y)[((size_t)q * nx + r) * P + d] = T(tr[d]);
+        }
+    }
+}
+
+template <typename T, int P>
+void launch_lanes_inflow_batch_p(const LanesParams<T> *batch, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL((k_lanes_inflow_batch<T, P>), dim3(64, nprob), dim3(256), 0, s, batch);
+    TSW_HIP(hipGetLastError());
+}
+
+#define TSW_LANES_INSTANTIATE(T, P)                                                                          \
+    template void launch_lanes_sweep_p<T, P>(const LanesParams<T> &, int, size_t, hipStream_t,               \
+                                             const LanesParams<T> *, const int4 *, int);                     \
+    template void launch_lanes_inflow_batch_p<T, P>(const LanesParams<T> *, int, hipStream_t);
 
 }  // namespace tsw

@@ -143,7 +143,7 @@ struct LaneBlock {
 };
 template <typename T> struct LanesParams {
     const T *xi, *eta;          // [nq]
-    int nx, ny, nq;
+    int nx, ny, nq, qo1, qo2, qo3;
     T dx, dy;
     const LaneBlock *blk;       // [nblk]
     int nblk, J, RW;            // lanes per row group (>= every block's n), row groups per wave (RW * J <= 64)
@@ -162,8 +162,13 @@ template <typename T> struct LanesParams {
     int *abort_flag;
     const int *skip;            // device flag: the launch does nothing while *skip != 0 (null: always runs)
 };
+constexpr int kLanesStoreWaves = 3;  // compute waves per workgroup when a storer wave writes psi
 template <typename T, int P>
-void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s);
+void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s,
+                          const LanesParams<T> *batch = nullptr, const int4 *items = nullptr, int nitems = 0);
+// tin_x / tin_y of every member of a batch from its src (device parameter table of nprob members)
+template <typename T, int P>
+void launch_lanes_inflow_batch_p(const LanesParams<T> *batch, int nprob, hipStream_t s);
 // y = beta xb + alpha sum_b mom[b] on the interior of phi-layout arrays (mom: nblk partials of g.nphi())
 void launch_mom_sum(const Geom &g, int nblk, const double *mom, const double *xb, double *y, double alpha,
                     double beta, hipStream_t s, const int *skip = nullptr);
