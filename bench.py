@@ -243,6 +243,7 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
     ctx.set_tuning("prefetch", args.prefetch)
     ctx.set_tuning("batch_direct", args.batch_direct)
     ctx.set_tuning("lanes", args.lanes)
+    ctx.set_tuning("batch_ref", args.batch_ref)
     ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
     U = sum(P.unknowns for P in batch)
     ncell = sum(P.nx * P.ny for P in batch)
@@ -265,7 +266,8 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
         outs[name] = work
         print(f"bench: batched {name}: call {wall / steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
               f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
-    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes")}
+    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes", "batch_ref", "batch_engine_p1",
+                                             "batch_engine_p2", "batch_engine_p3")}
     ctx.close()
 
     def rel(a_list, b_list):
@@ -343,8 +345,11 @@ def main():
                     help="1: lane-split sweep kernel; 0: the autotuner chooses (with automatic bands and prefetch)")
     ap.add_argument("--batch-direct", type=int, default=0,
                     help="batch groups: -1 storer wave, 0 or 1 direct stores where the tables allow")
-    ap.add_argument("--lanes", type=int, default=-1,
-                    help="batched workload: sweep members on the ordinate-lane engine (1) or the staggered kernels (-1)")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="batched workload: ordinate-lane engine forced (1), forbidden (-1) or autotuned (0)")
+    ap.add_argument("--batch-ref", type=int, default=0,
+                    help="batched workload: staggered kernels on the caller's layout forced (1), forbidden (-1) "
+                         "or autotuned (0)")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
     ap.add_argument("--dsa-cycles", type=int, default=1, help="multigrid V-cycles per DSA application")
     ap.add_argument("--band-scan", default="", help="comma-separated band_rows values: time contract sweeps "

@@ -103,11 +103,25 @@ struct tsw_ctx {
         bool built64 = false, built32 = false, built32c = false, mat64 = false, mat32 = false;
         bool direct64 = false, direct32 = false;  // direct-store layout (set with the parameter table)
         size_t lds64 = 0, lds32 = 0, lds32c = 0;
+        // reference-layout I/O: members swept in place on the caller's arrays; the table is rebuilt for
+        // each call's arrays
+        bool ref = false;
+        std::vector<unsigned char> ref_host;
+        DBuf ref_params;
+        size_t ref_lds = 0;
+        bool ref_mat = false, ref_direct = false;
     };
+    // Engine of each batch member, chosen per DG degree (tsw_batch_set): the staggered kernels through
+    // layout conversions, the staggered kernels on the caller's layout, or the ordinate-lane engine.
+    enum : char { kEngConverted = 0, kEngRef = 1, kEngLanes = 2 };
+    std::vector<char> engine, lanes_able;
+    bool batch_ref_ok = false;  // every member's array admits 32-bit byte offsets
+    int batch_engine[4] = {-1, -1, -1, -1};  // per DG degree present in the batch
+    std::vector<int> batch_all;              // 0 .. nprob-1
     std::vector<Group> groups;
-    // Batch members swept by the ordinate-lane engine (tuning key "lanes" = 1 when tsw_batch_set ran), in
-    // place on the caller's arrays: one launch per DG degree over (member, ordinate block, band) items,
-    // band-major so that every band's upstream band is dispatched first.
+    // Batch members on the ordinate-lane engine, swept in place on the caller's arrays: one launch per DG
+    // degree over (member, ordinate block, band) items, band-major so that every band's upstream band is
+    // dispatched first.
     struct LaneGroup {
         int p = 0;
         std::vector<int> members;
@@ -116,7 +130,6 @@ struct tsw_ctx {
         DBuf params, dev_items;
     };
     std::vector<LaneGroup> lane_groups;
-    std::vector<char> lane_member;
     // Adjoint problem: the same mesh with the negated ordinate set in reverse order (its sweeps are the
     // transposed sweeps of this problem up to the Legendre mass scaling), sharing this stream; built on
     // the first vector-Jacobian product.  Staging of the vector-Jacobian products' arrays.
@@ -151,6 +164,8 @@ struct tsw_ctx {
     // batch launch groups: -1 storer wave unless a table only fits without it, else (0, 1) direct stores
     // whenever every member has element-inverse tables (tuning key batch_direct)
     int batch_direct_opt = 0;
+    int batch_ref_opt = 0;   // batch engines: 1 / -1 force / forbid the reference-layout staggered groups
+    bool last_batch_ref = false;
     DBuf tune_out;
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
     // device (SiControl); skip_flag points at the control's done flag while such a graph is captured.
@@ -2680,6 +2695,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
             require(value >= -1 && value <= 1, TSW_E_ARG, "batch_direct must be -1, 0 or 1");
             ctx->batch_direct_opt = value;
             for (auto &G : ctx->groups) G.built32 = G.built32c = G.built64 = false;
+        } else if (k == "batch_ref") {
+            require(value == -1 || value == 0 || value == 1, TSW_E_ARG, "batch_ref must be -1, 0 or 1");
+            ctx->batch_ref_opt = value;
         } else if (k == "gmres_refine") {
             require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
             ctx->gmres_refine = value != 0;
@@ -2726,7 +2744,10 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "one_wave") *value = ctx->last_w1;
         else if (k == "lane_split") *value = ctx->last_sl;
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;
-        else if (k == "lanes") *value = ctx->last_lanes;
+        else if (k == "batch_ref") *value = ctx->last_batch_ref ? 1 : 0;
+        else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
+            *value = ctx->batch_engine[k[14] - '0'];
+        else if (k == "lanes") *value = ctx->batch.empty() ? ctx->last_lanes : (ctx->lane_groups.empty() ? 0 : 1);
         else if (k == "lanes_block") *value = ctx->last_lanes_J;
         else if (k == "lanes_waves") *value = ctx->last_lanes_W;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
@@ -3181,6 +3202,15 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
     return rc;
 }
 
+}  // extern "C"
+
+namespace {
+void batch_choose_engines(tsw_ctx *ctx);
+void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members);
+}  // namespace
+
+extern "C" {
+
 int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const double *const *sigma_t, double dt,
                   int mem) {
     return guarded(ctx, [&] {
@@ -3188,7 +3218,10 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
         ctx->batch.clear();
         ctx->groups.clear();
         ctx->lane_groups.clear();
-        ctx->lane_member.assign(nprob, 0);
+        ctx->engine.assign(nprob, tsw_ctx::kEngConverted);
+        ctx->lanes_able.assign(nprob, 0);
+        ctx->batch_all.resize(nprob);
+        for (int b = 0; b < nprob; ++b) ctx->batch_all[b] = b;
         ctx->batch_dt = dt;
         for (int b = 0; b < nprob; ++b) {
             const tsw_problem &pr = probs[b];
@@ -3206,96 +3239,26 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             double *sig = static_cast<double *>(sub->sig_t64.reserve(sizeof(double) * g.ncell_h()));
             TSW_HIP(hipMemcpyAsync(sig, sigma_t[b], sizeof(double) * g.ncell_h(),
                                    mem == TSW_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
-            bool lane = false;
-            if (ctx->lanes_opt > 0) {  // the ordinate-lane engine where it is built for the member
-                sub->lanes_opt = 1;
-                sub->lanes_J_opt = ctx->lanes_J_opt;
-                sub->lanes_W_opt = kLanesStoreWaves;
-                sub->lanes_fixed_W = true;
-                if (lanes_ok(sub.get())) {
-                    prepare_materials(sub.get(), sig, dt);
-                    lane = sub->mat.valid && diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt);
-                }
-            }
-            ctx->lane_member[b] = lane;
-            if (!lane) diag_setup(sub.get(), sig, nullptr, nullptr, dt, true, true);
+            diag_setup(sub.get(), sig, nullptr, nullptr, dt, true, true);
+            // the ordinate-lane engine where it is built for the member (three compute waves and a storer)
+            sub->lanes_opt = 1;
+            sub->lanes_J_opt = ctx->lanes_J_opt;
+            sub->lanes_W_opt = ctx->lanes_W_opt > 0 ? std::min(ctx->lanes_W_opt, kLanesStoreWaves) : kLanesStoreWaves;
+            sub->lanes_fixed_W = true;
+            ctx->lanes_able[b] = ctx->lanes_opt >= 0 && lanes_ok(sub.get()) && sub->mat.valid &&
+                                 diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt);
             ctx->batch.push_back(std::move(sub));
         }
-        // launch groups
-        std::map<std::tuple<int, int, bool>, int> gid;
-        for (int b = 0; b < nprob; ++b) {
-            if (ctx->lane_member[b]) continue;
-            tsw_ctx *sub = ctx->batch[b].get();
-            const auto key = std::make_tuple(sub->g.p, (sub->g.ny + 63) / 64, sub->mat.valid);
-            auto it = gid.find(key);
-            if (it == gid.end()) {
-                it = gid.emplace(key, (int)ctx->groups.size()).first;
-                ctx->groups.emplace_back();
-                auto &G = ctx->groups.back();
-                G.p = sub->g.p;
-                G.waves = (sub->g.ny + 63) / 64;
-                G.mat = sub->mat.valid;
-            }
-            auto &G = ctx->groups[it->second];
-            G.members.push_back(b);
-            G.max_ny = std::max(G.max_ny, sub->g.ny);
-        }
-        for (auto &G : ctx->groups) {  // work items, heaviest (most diagonal steps) first
-            std::vector<std::tuple<long long, int, int>> w;
-            for (int slot = 0; slot < (int)G.members.size(); ++slot) {
-                const Geom &g = ctx->batch[G.members[slot]]->g;
-                for (int q = 0; q < g.nq; ++q) w.emplace_back(-(long long)(g.nx + g.ny) * g.ny, slot, q);
-            }
-            std::sort(w.begin(), w.end());
-            for (auto &t : w) {
-                G.items.push_back(std::get<1>(t));
-                G.items.push_back(std::get<2>(t));
-            }
-            TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
-                                   sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));
-        }
-        // lane groups by DG degree; items band-major, heaviest members first within a band
-        std::map<int, int> lid;
-        for (int b = 0; b < nprob; ++b) {
-            if (!ctx->lane_member[b]) continue;
-            const int p = ctx->batch[b]->g.p;
-            auto it = lid.find(p);
-            if (it == lid.end()) {
-                it = lid.emplace(p, (int)ctx->lane_groups.size()).first;
-                ctx->lane_groups.emplace_back();
-                ctx->lane_groups.back().p = p;
-            }
-            ctx->lane_groups[it->second].members.push_back(b);
-        }
-        for (auto &LG : ctx->lane_groups) {
-            std::vector<std::tuple<long long, int>> order;  // (-work, slot)
-            std::vector<int> nblk(LG.members.size()), nbands(LG.members.size());
-            int max_bands = 0;
-            for (int slot = 0; slot < (int)LG.members.size(); ++slot) {
-                tsw_ctx *sub = ctx->batch[LG.members[slot]].get();
-                const std::pair<int, int> shape = lanes_shape(sub);
-                lanes_blocks(sub, shape.first);
-                const int rows = shape.second * (64 / shape.first);
-                nblk[slot] = sub->lanes.nblk;
-                nbands[slot] = (sub->g.ny + rows - 1) / rows;
-                max_bands = std::max(max_bands, nbands[slot]);
-                order.emplace_back(-(long long)(sub->g.nx + sub->g.ny) * sub->g.nq * sub->g.N(), slot);
-            }
-            std::sort(order.begin(), order.end());
-            for (int band = 0; band < max_bands; ++band)
-                for (auto &o : order) {
-                    const int slot = std::get<1>(o);
-                    if (band < nbands[slot])
-                        for (int bi = 0; bi < nblk[slot]; ++bi) LG.items.push_back(int4{slot, bi, band, 0});
-                }
-            TSW_HIP(hipMemcpyAsync(LG.dev_items.reserve(sizeof(int4) * LG.items.size()), LG.items.data(),
-                                   sizeof(int4) * LG.items.size(), hipMemcpyHostToDevice, ctx->stream));
-        }
+        ctx->batch_ref_ok = ctx->batch_ref_opt >= 0;  // 32-bit byte offsets into every member's array
+        for (auto &sub : ctx->batch)
+            ctx->batch_ref_ok = ctx->batch_ref_ok && sub->g.npsi() * sizeof(double) < (size_t(1) << 32);
         for (int k = 0; k < tsw_ctx::kSideStreams; ++k) {
             if (!ctx->side[k]) TSW_HIP(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
             if (!ctx->join_ev[k]) TSW_HIP(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
         }
         if (!ctx->fork_ev) TSW_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+        batch_choose_engines(ctx);
+        build_batch_groups(ctx, ctx->batch_all);
         sync(ctx);
     });
 }
@@ -3305,19 +3268,34 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
 namespace {
 // Builds (once) the device parameter table of a launch group for precision T: the sweep of each member's
 // source in place with its inflow, or (corr, FP32 only) the zero-inflow sweep of its residual buffer.
+// With `ref`, the table sweeps each member's reference-layout array ref[b] in place (inflow traces into the
+// member's trace buffers by launch_diag_inflow_batch) and is rebuilt and uploaded for every call, without
+// a host wait; G.ref_* receive the launch choices.
 template <typename T>
-void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
+void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::vector<double *> *ref = nullptr) {
     bool &built = sizeof(T) == 8 ? G.built64 : (corr ? G.built32c : G.built32);
-    if (built) return;
+    if (built && !ref) return;
     std::vector<DiagSweepParams<T>> ps;
     size_t lds = 0;
     for (int b : G.members) {
         tsw_ctx *sub = ctx->batch[b].get();
-        const size_t nd = (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny;
-        T *q = corr ? (T *)dbuf<float>(sub->diag.res32, nd)
-                    : (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
-        DiagSweepParams<T> sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, !corr, q);
-        lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, sub->g.p));
+        const Geom &g = sub->g;
+        const size_t nd = (size_t)g.nq * g.N() * g.nx * g.ny;
+        DiagSweepParams<T> sp;
+        if (ref) {
+            sp = make_diag_params<T>(sub, ctx->batch_dt, nullptr, nullptr, false, nullptr);
+            sp.qref = (*ref)[b];
+            sp.psi_ref = (*ref)[b];
+            auto &d = sub->diag;
+            const bool f64 = sizeof(T) == 8;
+            sp.tin_x = dbuf<T>(f64 ? d.tinx64 : d.tinx32, (size_t)g.nq * g.ny * (g.p + 1));
+            sp.tin_y = dbuf<T>(f64 ? d.tiny64 : d.tiny32, (size_t)g.nq * g.nx * (g.p + 1));
+        } else {
+            T *q = corr ? (T *)dbuf<float>(sub->diag.res32, nd)
+                        : (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
+            sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, !corr, q);
+        }
+        lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, g.p));
         ps.push_back(sp);
     }
     // one launch serves the group: if any member needs the direct-store layout every member uses it;
@@ -3345,8 +3323,10 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
     }
     bool all_mat = G.mat;
     for (auto &sp : ps) all_mat = all_mat && sp.ainv != nullptr;
-    (sizeof(T) == 8 ? G.direct64 : G.direct32) = all_mat && any_direct;
-    (sizeof(T) == 8 ? G.mat64 : G.mat32) = all_mat;
+    if (!ref) {
+        (sizeof(T) == 8 ? G.direct64 : G.direct32) = all_mat && any_direct;
+        (sizeof(T) == 8 ? G.mat64 : G.mat32) = all_mat;
+    }
     if (!all_mat) {
         lds = 0;
         for (size_t k = 0; k < ps.size(); ++k) {
@@ -3360,6 +3340,16 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr) {
         }
     }
     require(lds != 0, TSW_E_UNSUPPORTED, "batch group does not fit the staggered sweep");
+    if (ref) {
+        G.ref_host.resize(sizeof(DiagSweepParams<T>) * ps.size());
+        std::memcpy(G.ref_host.data(), ps.data(), G.ref_host.size());
+        TSW_HIP(hipMemcpyAsync(G.ref_params.reserve(G.ref_host.size()), G.ref_host.data(), G.ref_host.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+        G.ref_lds = lds;
+        G.ref_mat = all_mat;
+        G.ref_direct = all_mat && any_direct;
+        return;
+    }
     DBuf &buf = sizeof(T) == 8 ? G.params64 : (corr ? G.params32c : G.params32);
     TSW_HIP(hipMemcpyAsync(buf.reserve(sizeof(DiagSweepParams<T>) * ps.size()), ps.data(),
                            sizeof(DiagSweepParams<T>) * ps.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -3409,16 +3399,136 @@ void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStre
     lanes_launch<T>(LG.p, rep, threads, lds, s, dp, LG.dev_items.as<int4>(), (int)LG.items.size());
 }
 
-// One batched sweep: the staggered launch groups (diagonal-layout buffers of their members) and, with
-// `lanes`, the lane groups over lanes[b] in place; all on the side streams, timed as one sweep.
+// Launch groups of the members `members` by their engines: staggered groups by (DG degree, compute waves,
+// material path, layout), work items {member slot, ordinate} heaviest first; lane groups by DG degree, items
+// {member slot, ordinate block, band} band-major, heaviest members first within a band.
+void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
+    ctx->groups.clear();
+    ctx->lane_groups.clear();
+    std::map<std::tuple<int, int, bool, bool>, int> gid;
+    std::map<int, int> lid;
+    for (int b : members) {
+        tsw_ctx *sub = ctx->batch[b].get();
+        if (ctx->engine[b] == tsw_ctx::kEngLanes) {
+            auto it = lid.find(sub->g.p);
+            if (it == lid.end()) {
+                it = lid.emplace(sub->g.p, (int)ctx->lane_groups.size()).first;
+                ctx->lane_groups.emplace_back();
+                ctx->lane_groups.back().p = sub->g.p;
+            }
+            ctx->lane_groups[it->second].members.push_back(b);
+            continue;
+        }
+        const bool ref = ctx->engine[b] == tsw_ctx::kEngRef;
+        const auto key = std::make_tuple(sub->g.p, (sub->g.ny + 63) / 64, sub->mat.valid, ref);
+        auto it = gid.find(key);
+        if (it == gid.end()) {
+            it = gid.emplace(key, (int)ctx->groups.size()).first;
+            ctx->groups.emplace_back();
+            auto &G = ctx->groups.back();
+            G.p = sub->g.p;
+            G.waves = (sub->g.ny + 63) / 64;
+            G.mat = sub->mat.valid;
+            G.ref = ref;
+        }
+        auto &G = ctx->groups[it->second];
+        G.members.push_back(b);
+        G.max_ny = std::max(G.max_ny, sub->g.ny);
+    }
+    for (auto &G : ctx->groups) {
+        std::vector<std::tuple<long long, int, int>> w;
+        for (int slot = 0; slot < (int)G.members.size(); ++slot) {
+            const Geom &g = ctx->batch[G.members[slot]]->g;
+            for (int q = 0; q < g.nq; ++q) w.emplace_back(-(long long)(g.nx + g.ny) * g.ny, slot, q);
+        }
+        std::sort(w.begin(), w.end());
+        for (auto &t : w) {
+            G.items.push_back(std::get<1>(t));
+            G.items.push_back(std::get<2>(t));
+        }
+        TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
+                               sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    for (auto &LG : ctx->lane_groups) {
+        std::vector<std::tuple<long long, int>> order;  // (-work, slot)
+        std::vector<int> nblk(LG.members.size()), nbands(LG.members.size());
+        int max_bands = 0;
+        for (int slot = 0; slot < (int)LG.members.size(); ++slot) {
+            tsw_ctx *sub = ctx->batch[LG.members[slot]].get();
+            const std::pair<int, int> shape = lanes_shape(sub);
+            lanes_blocks(sub, shape.first);
+            const int rows = shape.second * (64 / shape.first);
+            nblk[slot] = sub->lanes.nblk;
+            nbands[slot] = (sub->g.ny + rows - 1) / rows;
+            max_bands = std::max(max_bands, nbands[slot]);
+            order.emplace_back(-(long long)(sub->g.nx + sub->g.ny) * sub->g.nq * sub->g.N(), slot);
+        }
+        std::sort(order.begin(), order.end());
+        for (int band = 0; band < max_bands; ++band)
+            for (auto &o : order) {
+                const int slot = std::get<1>(o);
+                if (band < nbands[slot])
+                    for (int bi = 0; bi < nblk[slot]; ++bi) LG.items.push_back(int4{slot, bi, band, 0});
+            }
+        TSW_HIP(hipMemcpyAsync(LG.dev_items.reserve(sizeof(int4) * LG.items.size()), LG.items.data(),
+                               sizeof(int4) * LG.items.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    TSW_HIP(hipStreamSynchronize(ctx->stream));  // the item vectors may be rebuilt by the next call
+}
+
+// Launch of one staggered group in precision T, whose parameter table batch_group_params has built and
+// uploaded on the context's stream before the side streams forked from it: `ref` sweeps the members'
+// arrays dev[b] in place.
 template <typename T>
-void batch_sweep(tsw_ctx *ctx, bool corr = false, const std::vector<double *> *lanes = nullptr, bool lanes64 = false) {
-    if (ctx->groups.empty() && !lanes) return;
-    for (auto &G : ctx->groups) batch_group_params<T>(ctx, G, corr);
+void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::vector<double *> *ref, hipStream_t sd) {
+    DiagSweepParams<T> rep{};  // representative parameters for the host-side dispatch
+    const DiagSweepParams<T> *probs;
+    size_t lds;
+    if (ref) {
+        probs = G.ref_params.as<DiagSweepParams<T>>();
+        rep.qref = reinterpret_cast<const double *>(1);
+        rep.ainv = G.ref_mat ? reinterpret_cast<const T *>(1) : nullptr;
+        rep.direct = G.ref_direct ? 1 : 0;
+        lds = G.ref_lds;
+        launch_diag_inflow_batch<T>(probs, (int)G.members.size(), G.p, sd);
+    } else {
+        probs = (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : (corr ? G.params32c.ptr : G.params32.ptr));
+        rep.qsrc = reinterpret_cast<const T *>(1);
+        rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
+        rep.direct = (sizeof(T) == 8 ? G.direct64 : G.direct32) ? 1 : 0;
+        lds = sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32);
+    }
+    rep.prefetch = ctx->prefetch_opt;
+    launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny, lds, G.p, sd);
+}
+
+// One batched sweep over the current groups, all on the side streams, timed as one sweep.  Converted
+// groups sweep their diagonal-layout buffers in FP64 (conv64) or FP32 (corr: the FP32 correction sweeps of
+// TSW_MIXED, converted groups only); groups on the caller's layout (lane groups, and staggered groups
+// with `ref_call`) sweep dev[b] in place in FP64 (inplace64) or FP32.
+void batch_sweep(tsw_ctx *ctx, bool conv64, bool inplace64, bool corr, const std::vector<double *> &dev, bool ref_call) {
     std::vector<int> lthreads;
     std::vector<size_t> llds;
-    if (lanes && lanes64) lanes_batch_params<double>(ctx, *lanes, lthreads, llds);
-    else if (lanes) lanes_batch_params<float>(ctx, *lanes, lthreads, llds);
+    const bool lanes = !corr && !ctx->lane_groups.empty();
+    if (lanes && inplace64) lanes_batch_params<double>(ctx, dev, lthreads, llds);
+    else if (lanes) lanes_batch_params<float>(ctx, dev, lthreads, llds);
+    std::vector<std::pair<tsw_ctx::Group *, bool>> diag;  // (group, on the caller's layout)
+    for (auto &G : ctx->groups) {
+        const bool in_place = G.ref && ref_call;
+        if (!corr || !in_place) diag.emplace_back(&G, in_place);
+    }
+    if (diag.empty() && !lanes) return;
+    // every parameter table is on the device (uploads on ctx->stream) before the fork below
+    for (auto &gd : diag) {
+        if (gd.second) {
+            if (inplace64) batch_group_params<double>(ctx, *gd.first, false, &dev);
+            else batch_group_params<float>(ctx, *gd.first, false, &dev);
+        } else if (conv64 && !corr) {
+            batch_group_params<double>(ctx, *gd.first, false);
+        } else {
+            batch_group_params<float>(ctx, *gd.first, corr);
+        }
+    }
     hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
     TSW_HIP(hipEventRecord(e0, ctx->stream));
     TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
@@ -3427,22 +3537,20 @@ void batch_sweep(tsw_ctx *ctx, bool corr = false, const std::vector<double *> *l
         for (size_t j = 0; j < ctx->lane_groups.size(); ++j, ++k) {
             hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
             TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
-            if (lanes64) lanes_batch_launch<double>(ctx, j, lthreads[j], llds[j], sd);
+            if (inplace64) lanes_batch_launch<double>(ctx, j, lthreads[j], llds[j], sd);
             else lanes_batch_launch<float>(ctx, j, lthreads[j], llds[j], sd);
         }
-    for (auto &G : ctx->groups) {
+    for (auto &gd : diag) {
         hipStream_t sd = ctx->side[k % tsw_ctx::kSideStreams];
         TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
-        const DiagSweepParams<T> *probs =
-            (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : (corr ? G.params32c.ptr : G.params32.ptr));
-        // representative parameters for the host-side dispatch (mode and material path)
-        DiagSweepParams<T> rep{};
-        rep.qsrc = reinterpret_cast<const T *>(1);
-        rep.ainv = (sizeof(T) == 8 ? G.mat64 : G.mat32) ? reinterpret_cast<const T *>(1) : nullptr;
-        rep.direct = (sizeof(T) == 8 ? G.direct64 : G.direct32) ? 1 : 0;
-        rep.prefetch = ctx->prefetch_opt;
-        launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny,
-                                   sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32), G.p, sd);
+        if (gd.second) {
+            if (inplace64) batch_group_launch<double>(ctx, *gd.first, false, &dev, sd);
+            else batch_group_launch<float>(ctx, *gd.first, false, &dev, sd);
+        } else if (conv64 && !corr) {
+            batch_group_launch<double>(ctx, *gd.first, false, nullptr, sd);
+        } else {
+            batch_group_launch<float>(ctx, *gd.first, corr, nullptr, sd);
+        }
         ++k;
     }
     for (int j = 0; j < std::min<int>(k, tsw_ctx::kSideStreams); ++j) {
@@ -3451,6 +3559,177 @@ void batch_sweep(tsw_ctx *ctx, bool corr = false, const std::vector<double *> *l
     }
     TSW_HIP(hipEventRecord(e1, ctx->stream));
     ++ctx->sweeps_done;
+}
+
+// One batched sweep of the members `members` over their device arrays dev[b] (reference layout, in place),
+// by the current groups.  Members on the caller's layout (lane members; reference-layout members when
+// every array is 16-byte aligned, else they go through the conversions for this call) take the FP64 sweep
+// under TSW_MIXED: those sweeps read and write FP64 in either precision and their FP32 arithmetic is not
+// faster, so a refinement would only add a residual and a second sweep.  Converted members: inflow traces
+// and the source to the diagonal layout, the sweep (FP32 with the FP64 refinement of mixed_refine_store
+// under TSW_MIXED), the result back.
+void batch_apply(tsw_ctx *ctx, const std::vector<int> &members, std::vector<double *> &dev, int precision) {
+    bool aligned = true;
+    for (int b : members) aligned = aligned && ((uintptr_t)dev[b] & 15) == 0;
+    bool any_ref = false;
+    for (auto &G : ctx->groups) any_ref = any_ref || G.ref;
+    const bool ref_call = aligned && any_ref;
+    ctx->last_batch_ref = ref_call;
+    auto converted = [&](int b) {
+        const char e = ctx->engine[b];
+        return e == tsw_ctx::kEngConverted || (e == tsw_ctx::kEngRef && !ref_call);
+    };
+    const bool conv64 = precision == TSW_FP64, mixed = precision == TSW_MIXED;
+    const bool inplace64 = precision != TSW_FP32;
+    // per-member layout conversions spread over the side streams, joined before the next phase
+    auto on_side_streams = [&](auto &&fn) {
+        bool any = false;
+        for (int b : members) any = any || converted(b);
+        if (!any) return;
+        TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+        for (auto sd : ctx->side) TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
+        int k = 0;
+        for (int b : members) {
+            if (!converted(b)) continue;
+            tsw_ctx *sub = ctx->batch[b].get();
+            sub->stream = ctx->side[k++ % tsw_ctx::kSideStreams];
+            try {
+                fn(b, sub);
+            } catch (...) {
+                sub->stream = ctx->stream;
+                throw;
+            }
+            sub->stream = ctx->stream;
+        }
+        for (int j = 0; j < tsw_ctx::kSideStreams; ++j) {
+            TSW_HIP(hipEventRecord(ctx->join_ev[j], ctx->side[j]));
+            TSW_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0));
+        }
+    };
+    auto nd_of = [](const tsw_ctx *sub) { return (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny; };
+    on_side_streams([&](int b, tsw_ctx *sub) { diag_load_source(sub, dev[b], conv64 || mixed, !conv64); });
+    batch_sweep(ctx, conv64, inplace64, false, dev, ref_call);
+    const bool fused_store = mixed && ctx->refine_steps == 1;
+    auto residual = [&](tsw_ctx *sub, const auto *psi) {
+        auto &d = sub->diag;
+        launch_residual_diag<float>(sub->g, d.q64.as<double>(), nullptr, nullptr, d.sigt64.as<double>(), psi,
+                                    d.tinx64.as<double>(), d.tiny64.as<double>(), sub->xi64.as<double>(),
+                                    sub->eta64.as<double>(), inverse_dt(ctx->batch_dt),
+                                    dbuf<float>(d.res32, nd_of(sub)), sub->stream);
+    };
+    if (fused_store) {
+        on_side_streams([&](int, tsw_ctx *sub) { residual(sub, sub->diag.q32.as<float>()); });
+        batch_sweep(ctx, false, inplace64, true, dev, ref_call);
+    } else if (mixed) {
+        on_side_streams([&](int, tsw_ctx *sub) {
+            launch_convert_f2d(sub->diag.q32.as<float>(), dbuf<double>(sub->diag.ir64, nd_of(sub)), nd_of(sub),
+                               sub->stream);
+        });
+        for (int k = 0; k < ctx->refine_steps; ++k) {
+            on_side_streams([&](int, tsw_ctx *sub) {
+                residual(sub, static_cast<const double *>(sub->diag.ir64.as<double>()));
+            });
+            batch_sweep(ctx, false, inplace64, true, dev, ref_call);
+            on_side_streams([&](int, tsw_ctx *sub) {
+                launch_accumulate_f2d(sub->diag.res32.as<float>(), sub->diag.ir64.as<double>(), nd_of(sub), sub->stream);
+            });
+        }
+    }
+    on_side_streams([&](int b, tsw_ctx *sub) {
+        if (conv64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
+        else if (fused_store)
+            launch_psi_store_sum(sub->g, dev[b], sub->diag.q32.as<float>(), sub->diag.res32.as<float>(), sub->stream);
+        else if (mixed) diag_store<double>(sub, sub->diag.ir64.as<double>(), dev[b]);
+        else diag_store<float>(sub, sub->diag.q32.as<float>(), dev[b]);
+    });
+}
+
+// Autotuning of the batch engines, per DG degree: each candidate (staggered kernels through the layout
+// conversions or on the caller's layout, one ordinate per wavefront; the ordinate-lane engine, J ordinates
+// per wavefront) sweeps the degree's members, or the largest ones that fit a 4 GiB scratch budget, twice
+// in FP64 on zero-source scratch arrays, and the second sweep is timed.  Every engine gives the same bits,
+// so the choice only affects speed.  Without the autotuner (or with an engine forced by the tuning keys
+// "lanes" and "batch_ref") the lane engine takes the degrees it is built for and the staggered kernels on
+// the caller's layout the others.
+void batch_choose_engines(tsw_ctx *ctx) {
+    const int nprob = (int)ctx->batch.size();
+    for (int p = 0; p < 4; ++p) ctx->batch_engine[p] = -1;
+    for (int p = 0; p < 4; ++p) {
+        std::vector<int> mem;
+        bool all_lanes = true;
+        for (int b = 0; b < nprob; ++b)
+            if (ctx->batch[b]->g.p == p) {
+                mem.push_back(b);
+                all_lanes = all_lanes && ctx->lanes_able[b];
+            }
+        if (mem.empty()) continue;
+        std::vector<char> cand;
+        if (ctx->lanes_opt > 0 && all_lanes) cand = {tsw_ctx::kEngLanes};
+        else if (ctx->batch_ref_opt > 0 && ctx->batch_ref_ok) cand = {tsw_ctx::kEngRef};
+        else {
+            if (ctx->lanes_opt == 0 && all_lanes) cand.push_back(tsw_ctx::kEngLanes);
+            if (ctx->batch_ref_ok) cand.push_back(tsw_ctx::kEngRef);
+            if (ctx->batch_ref_opt <= 0 && ctx->lanes_opt <= 0) cand.push_back(tsw_ctx::kEngConverted);
+            if (cand.empty()) cand.push_back(tsw_ctx::kEngConverted);
+        }
+        char best = cand[0];
+        if (cand.size() > 1 && ctx->autotune) {
+            // the heaviest members up to the scratch budget
+            std::sort(mem.begin(), mem.end(),
+                      [&](int a, int b) { return ctx->batch[a]->g.npsi() > ctx->batch[b]->g.npsi(); });
+            std::vector<int> sub_members;
+            size_t bytes = 0;
+            for (int b : mem) {
+                const size_t nb = ctx->batch[b]->g.npsi() * sizeof(double);
+                if (!sub_members.empty() && bytes + nb > (size_t(4) << 30)) continue;
+                sub_members.push_back(b);
+                bytes += nb;
+            }
+            DBuf scratch;
+            char *base = static_cast<char *>(scratch.reserve(bytes + 256 * sub_members.size()));
+            TSW_HIP(hipMemsetAsync(base, 0, bytes + 256 * sub_members.size(), ctx->stream));
+            std::vector<double *> dev(nprob, nullptr);
+            size_t off = 0;
+            for (int b : sub_members) {
+                dev[b] = reinterpret_cast<double *>(base + off);
+                off += (ctx->batch[b]->g.npsi() * sizeof(double) + 255) / 256 * 256;
+            }
+            double best_ms = 0.0;
+            for (char e : cand) {
+                for (int b : sub_members) ctx->engine[b] = e;
+                build_batch_groups(ctx, sub_members);
+                batch_apply(ctx, sub_members, dev, TSW_FP64);  // first touch of tables and buffers
+                hipEvent_t t0 = next_event(ctx), t1 = next_event(ctx);
+                TSW_HIP(hipEventRecord(t0, ctx->stream));
+                batch_apply(ctx, sub_members, dev, TSW_FP64);
+                TSW_HIP(hipEventRecord(t1, ctx->stream));
+                TSW_HIP(hipEventSynchronize(t1));
+                float ms = 0.f;
+                TSW_HIP(hipEventElapsedTime(&ms, t0, t1));
+                if (e == cand[0] || ms < best_ms) {
+                    best_ms = ms;
+                    best = e;
+                }
+            }
+            sync(ctx);
+            ctx->ev_used = 0;  // the tuning sweeps are not counted
+            ctx->sweeps_done = 0;
+            ctx->ms_folded = 0.0;
+        }
+        for (int b : mem) {
+            ctx->engine[b] = best;
+            if (best != tsw_ctx::kEngConverted) {  // diagonal-layout buffers of the tuning sweeps
+                auto &d = ctx->batch[b]->diag;
+                d.q64.release();
+                d.q32.release();
+                d.res32.release();
+                d.ir64.release();
+            }
+        }
+        ctx->batch_engine[p] = best;
+    }
+    ctx->groups.clear();
+    ctx->lane_groups.clear();
 }
 }  // namespace
 
@@ -3462,81 +3741,18 @@ int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem) {
         require(psi != nullptr, TSW_E_ARG, "null psi array list");
         require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
                 "bad precision");
-        const bool f64 = precision == TSW_FP64, mixed = precision == TSW_MIXED;
         std::vector<double *> dev(ctx->batch.size());
-        // per-member layout conversions spread over the side streams, joined before the next phase
-        auto on_side_streams = [&](auto &&fn) {
-            TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-            for (auto sd : ctx->side) TSW_HIP(hipStreamWaitEvent(sd, ctx->fork_ev, 0));
+        for (size_t b = 0; b < ctx->batch.size(); ++b) {
+            require(psi[b] != nullptr, TSW_E_ARG, "null psi array");
+            tsw_ctx *sub = ctx->batch[b].get();
+            dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
+        }
+        batch_apply(ctx, ctx->batch_all, dev, precision);
+        if (mem == TSW_MEM_HOST)
             for (size_t b = 0; b < ctx->batch.size(); ++b) {
                 tsw_ctx *sub = ctx->batch[b].get();
-                sub->stream = ctx->side[b % tsw_ctx::kSideStreams];
-                try {
-                    fn(b, sub);
-                } catch (...) {
-                    sub->stream = ctx->stream;
-                    throw;
-                }
-                sub->stream = ctx->stream;
+                stage_out(sub, psi[b], dev[b], sub->g.npsi(), mem);
             }
-            for (int j = 0; j < tsw_ctx::kSideStreams; ++j) {
-                TSW_HIP(hipEventRecord(ctx->join_ev[j], ctx->side[j]));
-                TSW_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[j], 0));
-            }
-        };
-        auto nd_of = [](const tsw_ctx *sub) { return (size_t)sub->g.nq * sub->g.N() * sub->g.nx * sub->g.ny; };
-        // inputs per member on the side streams: inflow traces and the source in the diagonal layout
-        on_side_streams([&](size_t b, tsw_ctx *sub) {
-            dev[b] = stage_in(sub, psi[b], sub->g.npsi(), mem, sub->psi64);
-            if (!ctx->lane_member[b]) diag_load_source(sub, dev[b], f64 || mixed, !f64);
-        });
-        // Lane members are swept in place on the caller's layout.  Under TSW_MIXED they take the FP64
-        // sweep: that engine reads and writes FP64 in both precisions and its FP32 arithmetic is not
-        // faster, so a refinement would only add a residual and a second sweep.
-        const bool any_lanes = !ctx->lane_groups.empty();
-        if (f64) batch_sweep<double>(ctx, false, any_lanes ? &dev : nullptr, true);
-        else batch_sweep<float>(ctx, false, any_lanes ? &dev : nullptr, mixed);
-        // the refinement of mixed_refine_store, its FP32 sweeps batched across members
-        const bool fused_store = mixed && ctx->refine_steps == 1;
-        auto residual = [&](tsw_ctx *sub, const auto *psi) {
-            auto &d = sub->diag;
-            launch_residual_diag<float>(sub->g, d.q64.as<double>(), nullptr, nullptr, d.sigt64.as<double>(), psi,
-                                        d.tinx64.as<double>(), d.tiny64.as<double>(), sub->xi64.as<double>(),
-                                        sub->eta64.as<double>(), inverse_dt(ctx->batch_dt),
-                                        dbuf<float>(d.res32, nd_of(sub)), sub->stream);
-        };
-        if (fused_store) {
-            on_side_streams([&](size_t b, tsw_ctx *sub) {
-                if (!ctx->lane_member[b]) residual(sub, sub->diag.q32.as<float>());
-            });
-            batch_sweep<float>(ctx, true);
-        } else if (mixed) {
-            on_side_streams([&](size_t b, tsw_ctx *sub) {
-                if (ctx->lane_member[b]) return;
-                launch_convert_f2d(sub->diag.q32.as<float>(), dbuf<double>(sub->diag.ir64, nd_of(sub)), nd_of(sub),
-                                   sub->stream);
-            });
-            for (int k = 0; k < ctx->refine_steps; ++k) {
-                on_side_streams([&](size_t b, tsw_ctx *sub) {
-                    if (!ctx->lane_member[b]) residual(sub, static_cast<const double *>(sub->diag.ir64.as<double>()));
-                });
-                batch_sweep<float>(ctx, true);
-                on_side_streams([&](size_t b, tsw_ctx *sub) {
-                    if (ctx->lane_member[b]) return;
-                    launch_accumulate_f2d(sub->diag.res32.as<float>(), sub->diag.ir64.as<double>(), nd_of(sub),
-                                          sub->stream);
-                });
-            }
-        }
-        on_side_streams([&](size_t b, tsw_ctx *sub) {
-            if (ctx->lane_member[b]) {
-            } else if (f64) diag_store<double>(sub, sub->diag.q64.as<double>(), dev[b]);
-            else if (fused_store)
-                launch_psi_store_sum(sub->g, dev[b], sub->diag.q32.as<float>(), sub->diag.res32.as<float>(), sub->stream);
-            else if (mixed) diag_store<double>(sub, sub->diag.ir64.as<double>(), dev[b]);
-            else diag_store<float>(sub, sub->diag.q32.as<float>(), dev[b]);
-            if (mem == TSW_MEM_HOST) stage_out(sub, psi[b], dev[b], sub->g.npsi(), mem);
-        });
         sync(ctx);
     });
 }

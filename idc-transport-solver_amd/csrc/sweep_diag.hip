@@ -112,6 +112,18 @@ void launch_diag_sweep_batch(const DiagSweepParams<T> &rep, const DiagSweepParam
         default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
     }
 }
+template <typename T>
+void launch_diag_inflow_batch(const DiagSweepParams<T> *probs, int nprob, int p, hipStream_t s) {
+    switch (p) {
+        case 0: launch_diag_inflow_batch_p<T, 1>(probs, nprob, s); break;
+        case 1: launch_diag_inflow_batch_p<T, 2>(probs, nprob, s); break;
+        case 2: launch_diag_inflow_batch_p<T, 3>(probs, nprob, s); break;
+        case 3: launch_diag_inflow_batch_p<T, 4>(probs, nprob, s); break;
+        default: throw Error(TSW_E_UNSUPPORTED, "DG degree not built");
+    }
+}
+template void launch_diag_inflow_batch<double>(const DiagSweepParams<double> *, int, int, hipStream_t);
+template void launch_diag_inflow_batch<float>(const DiagSweepParams<float> *, int, int, hipStream_t);
 template void launch_diag_sweep_batch<double>(const DiagSweepParams<double> &, const DiagSweepParams<double> *,
                                               const int2 *, int, int, size_t, int, hipStream_t);
 template void launch_diag_sweep_batch<float>(const DiagSweepParams<float> &, const DiagSweepParams<float> *,

@@ -143,6 +143,46 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, 
     }
 }
 
+// A cell's N coefficients of one ordinate in the reference layout: N contiguous doubles at byte offset
+// `off` (16-byte aligned when N is even: 16-byte accesses, else 8-byte ones).
+template <int N>
+__device__ __forceinline__ void ref_ld(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[N]) {
+    if constexpr (N % 2 == 0) {
+        typedef unsigned V4 __attribute__((ext_vector_type(4)));
+        typedef unsigned V2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            const V4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 16 * i, 0);
+            v[2 * i] = __builtin_bit_cast(double, V2{x[0], x[1]});
+            v[2 * i + 1] = __builtin_bit_cast(double, V2{x[2], x[3]});
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = buf_ld<double>(r, off, 8 * k);
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void ref_st(__amdgpu_buffer_rsrc_t r, unsigned off, const T (&v)[N]) {
+    typedef unsigned V2 __attribute__((ext_vector_type(2)));
+    if constexpr (N % 2 == 0) {
+        typedef unsigned V4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            const V2 a = __builtin_bit_cast(V2, (double)v[2 * i]), b = __builtin_bit_cast(V2, (double)v[2 * i + 1]);
+            __builtin_amdgcn_raw_buffer_store_b128(V4{a[0], a[1], b[0], b[1]}, r, (int)off, 16 * i, 0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) buf_st<double>(r, off, 8 * k, (double)v[k]);
+    }
+}
+// Byte offset of cell (c, r) (sweep coordinates of an ordinate's quadrant) of ordinate q in the reference
+// layout [nx+2][ny+2][nq][N] of doubles.
+__device__ __forceinline__ unsigned ref_off(int c, int r, int nx, int ny, int nq, int q, int N, bool xneg, bool yneg) {
+    const int i = xneg ? nx - c : c + 1, j = yneg ? ny - r : r + 1;
+    return (unsigned)((((long long)i * (ny + 2) + j) * nq + q) * N * 8);
+}
+
 template <typename U>
 __device__ __forceinline__ U xload(gptr<const U> p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -192,7 +232,9 @@ struct DiagWalk {
     }
 };
 
-// MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi).
+// MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi); bit 4 (with bit 1): Q from `qref` and
+// psi to `psi_ref`, both in the reference layout (a cell's N values of the ordinate are one contiguous run
+// per lane, read and written by 16-byte accesses where N is even), so no layout conversion is needed.
 // Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
 // upstream neighbour is dispatched before it and never waits on it).  Store variant ST: 0, a storer wave
 // writes the cells staged in LDS; 1 (DIRECT, one band only), no storer wave, each compute lane stores its
@@ -216,6 +258,9 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
     constexpr bool DIRECT = ST != 0, W1 = ST == 2;
+    constexpr bool REF = (MODE & 4) != 0;
+    static_assert(!(REF && W1), "reference-layout I/O is built for the storer-wave and direct-store variants");
+    using PQ = typename std::conditional<REF, double, T>::type;  // prefetched source (converted at use)
     using Lds = DiagLds<T, P>;
     constexpr int PS = Lds::PS, NS = Lds::NS, MS = Lds::MS;
     using U = typename ExchangeSentinel<T>::U;
@@ -269,6 +314,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     const int steps = nx + nrows - 1;              // local step t is global diagonal R0 + t
     const int nsteps = ((steps + D - 1) / D) * D;  // padded steps have no active cell
     const unsigned plane = (unsigned)(ncell * sizeof(T));  // bytes of one coefficient plane (host-checked)
+    const unsigned long long ref_bytes = REF ? (unsigned long long)(nx + 2) * (ny + 2) * sp.nq * N * 8 : 0;
     const DiagWalk walk0{R0, (int)(dm.base(R0) - dm.rlo(R0))};
 
     if (!DIRECT && tid >= cthreads) {
@@ -276,7 +322,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         // the compute waves fill the other ring slot; it also hands the band's top trace on and empties
         // the records this band has consumed
         const int lane = tid - cthreads;
-        const __amdgpu_buffer_rsrc_t rpsi = buf_rsrc(sp.psi + (size_t)q * N * ncell, (unsigned long long)N * plane);
+        const __amdgpu_buffer_rsrc_t rpsi = REF ? buf_rsrc(sp.psi_ref, ref_bytes)
+                                                : buf_rsrc(sp.psi + (size_t)q * N * ncell, (unsigned long long)N * plane);
         DiagWalk w = walk0;
         lds_barrier();
         for (int t = 0; t < nsteps; ++t) {
@@ -295,9 +342,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             for (int r = lo + lane; r <= hi; r += 64) {
                 T v[N];
                 lds_get<T, N>(slot + (size_t)(r - R0) * NS, v);
-                const unsigned boff = (unsigned)(w.off + r) * (unsigned)sizeof(T);
+                if constexpr (REF) {
+                    ref_st<T, N>(rpsi, ref_off(s - r, r, nx, ny, sp.nq, q, N, xneg, yneg), v);
+                } else {
+                    const unsigned boff = (unsigned)(w.off + r) * (unsigned)sizeof(T);
 #pragma unroll
-                for (int k = 0; k < N; ++k) buf_st<T>(rpsi, boff, k * plane, v[k]);
+                    for (int k = 0; k < N; ++k) buf_st<T>(rpsi, boff, k * plane, v[k]);
+                }
             }
             w.next(nx, ny);
         }
@@ -312,8 +363,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     for (int d = 0; d < P; ++d) tx[d] = (row_ok && sp.tin_x) ? G(sp.tin_x)[((size_t)q * ny + r) * P + d] : T(0);
 
     const unsigned long long nplanes = (unsigned long long)N * plane;
-    const __amdgpu_buffer_rsrc_t rq = buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
-    const __amdgpu_buffer_rsrc_t rout = buf_rsrc(DIRECT ? sp.psi + (size_t)q * N * ncell : nullptr, DIRECT ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rq = REF ? buf_rsrc(sp.qref, ref_bytes)
+                                          : buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rout = REF ? buf_rsrc(DIRECT ? sp.psi_ref : nullptr, DIRECT ? ref_bytes : 0)
+                                            : buf_rsrc(DIRECT ? sp.psi + (size_t)q * N * ncell : nullptr, DIRECT ? nplanes : 0);
     DiagWalk cw = walk0;  // diagonal of the current step (DIRECT stores)
     const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr, HASPHI ? nplanes : 0);
     const __amdgpu_buffer_rsrc_t rsigs = buf_rsrc(HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr, HASPHI ? plane : 0);
@@ -330,7 +383,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     // prefetch ring: slot u holds the inputs this row meets at a step t with t % D == u.  Lanes whose row
     // has no cell on the fetched diagonal read the nearest cell of the band on it (so a wavefront's loads
     // stay within one contiguous run); padded steps read position 0.
-    T pq[D][HASQ ? N : 1], pf[D][HASPHI ? N : 1], ps[D], pm_sig[D];
+    PQ pq[D][HASQ ? N : 1];
+    T pf[D][HASPHI ? N : 1], ps[D], pm_sig[D];
     int pm[D];
     Record<U, P> px[D];
     DiagWalk fw = walk0;  // diagonal of the next fetch
@@ -342,7 +396,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         const int rc = r < lo ? lo : (r > hi ? hi : r);
         const unsigned p = (unsigned)((live ? fw.off : 0) + rc);
         const unsigned bt = p * (unsigned)sizeof(T), bi = p * 4u;
-        if (HASQ) {
+        if constexpr (REF) {  // the cell (clamped into the mesh on padded steps) in the reference layout
+            const int cl = fw.s - rc < 0 ? 0 : (fw.s - rc >= nx ? nx - 1 : fw.s - rc);
+            ref_ld<N>(rq, ref_off(cl, rc, nx, ny, sp.nq, q, N, xneg, yneg), pq[u]);
+        } else if (HASQ) {
 #pragma unroll
             for (int k = 0; k < N; ++k) pq[u][k] = buf_ld<T>(rq, bt, k * plane);
         }
@@ -440,10 +497,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                     const T wsc = bsc * ps[u] * T(kInv4Pi);
 #pragma unroll
                     for (int k = 0; k < N; ++k)
-                        a[k] = HASQ ? fmadd(wsc, pf[u][k], MAT ? pq[u][k] : bsc * pq[u][k]) : wsc * pf[u][k];
+                        a[k] = HASQ ? fmadd(wsc, pf[u][k], MAT ? T(pq[u][k]) : bsc * T(pq[u][k])) : wsc * pf[u][k];
                 } else {
 #pragma unroll
-                    for (int k = 0; k < N; ++k) a[k] = MAT ? pq[u][k] : bsc * pq[u][k];
+                    for (int k = 0; k < N; ++k) a[k] = MAT ? T(pq[u][k]) : bsc * T(pq[u][k]);
                 }
 #pragma unroll
                 for (int d = 0; d < P; ++d)
@@ -473,9 +530,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             }
             if constexpr (DIRECT) {
                 if (active) {
-                    const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T);
+                    if constexpr (REF) {
+                        ref_st<T, N>(rout, ref_off(c, r, nx, ny, sp.nq, q, N, xneg, yneg), x);
+                    } else {
+                        const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T);
 #pragma unroll
-                    for (int k = 0; k < N; ++k) buf_st<T>(rout, boff, k * plane, x[k]);
+                        for (int k = 0; k < N; ++k) buf_st<T>(rout, boff, k * plane, x[k]);
+                    }
                 }
             } else {
                 lds_put<T, N>(lout + ((size_t)(t & 1) * cthreads + rl) * NS, x);
@@ -541,7 +602,7 @@ void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const 
                         const int2 *items, int nitems, int max_ny) {
     const int rows = batch ? max_ny : band_rows_of(sp);
     if (sp.split) {
-        if constexpr (MAT && split_supported<T, P>()) {
+        if constexpr (MAT && split_supported<T, P>() && (MODE & 4) == 0) {
             constexpr int LG = split_lanes<P>(), RPW = 64 / LG;  // lanes per row, rows per wavefront
             const int threads = LG * ((rows + RPW - 1) / RPW * RPW);  // LG lanes for each row slot
             if (batch || threads > 512) throw Error(TSW_E_ARG, "lane-split sweep needs one problem and bands of at most 512 / lanes rows");
@@ -559,7 +620,9 @@ void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const 
     if (direct && !batch && sp.nbands > 1) throw Error(TSW_E_ARG, "direct-store sweep with row bands");
     if (wave1 && (batch || rows > 64)) throw Error(TSW_E_ARG, "one-wave sweep needs bands of at most 64 rows");
     const int threads = wave1 ? 64 : ((rows + 63) / 64) * 64 + (direct ? 0 : 64);
-    auto kern = wave1 ? k_diag_sweep<T, P, MAT, MODE, D, 2>
+    constexpr bool REF = (MODE & 4) != 0;
+    if (REF && wave1) throw Error(TSW_E_ARG, "reference-layout sweep without a one-wave variant");
+    auto kern = (wave1 && !REF) ? k_diag_sweep<T, P, MAT, MODE, D, REF ? 0 : 2>
                       : (direct ? k_diag_sweep<T, P, MAT, MODE, D, MAT ? 1 : 0> : k_diag_sweep<T, P, MAT, MODE, D, 0>);
     if (lds > 64 * 1024)
         TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -608,6 +671,13 @@ void launch_diag_sweep_p(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s
 template <typename T, int P>
 void launch_diag_sweep_batch_p(const DiagSweepParams<T> &rep, const DiagSweepParams<T> *probs, const int2 *items,
                                int nitems, int max_ny, size_t lds, hipStream_t s) {
+    if (rep.qref) {  // reference-layout I/O: sources in place, the default prefetch depth
+        constexpr int D = sizeof(T) == 8 ? 2 : 3;
+        if (rep.phi) throw Error(TSW_E_ARG, "reference-layout batch sweep: sources only");
+        if (rep.ainv) launch_ds_d<T, P, true, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
+        else launch_ds_d<T, P, false, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
+        return;
+    }
     const int mode = (rep.qsrc ? 1 : 0) | (rep.phi ? 2 : 0);
     if (rep.ainv) {
         if (mode == 1) launch_ds<T, P, true, 1>(rep, lds, s, probs, items, nitems, max_ny);
@@ -620,12 +690,52 @@ void launch_diag_sweep_batch_p(const DiagSweepParams<T> &rep, const DiagSweepPar
     }
 }
 
+// Inflow traces of a batch's sources from their reference-layout halos, as launch_inflow computes them
+// for one problem: tin_x [nq][ny][P] from the x-inflow halo cell of each row, tin_y [nq][nx][P] from the
+// y-inflow halo cell of each column.  Member blockIdx.y; x strides over ordinates x rows (columns).
+inline __device__ const double *inflow_source(const DiagSweepParams<double> &p) { return p.qref; }
+inline __device__ const double *inflow_source(const DiagSweepParams<float> &p) { return p.qref; }
+template <typename T, int P, typename Params>
+__global__ void k_inflow_batch(const Params *__restrict__ batch) {
+    constexpr int N = P * P;
+    const Params sp = batch[blockIdx.y];
+    const double *src = inflow_source(sp);
+    const int nx = sp.nx, ny = sp.ny, nq = sp.nq, len = nx > ny ? nx : ny;
+    for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < nq * len; t += (int)(gridDim.x * blockDim.x)) {
+        const int q = t / len, r = t - q * len;
+        const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
+        const bool xneg = Q >= 2, yneg = (Q & 1) != 0;
+        if (r < ny) {  // row r: the x-inflow halo cell
+            const int j = yneg ? ny - r : r + 1, i = xneg ? nx + 1 : 0;
+            double tr[P];
+            trace_x<double, P>(src + (((size_t)i * (ny + 2) + j) * nq + q) * N, xneg, tr);
+#pragma unroll
+            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_x)[((size_t)q * ny + r) * P + d] = T(tr[d]);
+        }
+        if (r < nx) {  // column r: the y-inflow halo cell
+            const int i = xneg ? nx - r : r + 1, j = yneg ? ny + 1 : 0;
+            double tr[P];
+            trace_y<double, P>(src + (((size_t)i * (ny + 2) + j) * nq + q) * N, yneg, tr);
+#pragma unroll
+            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_y)[((size_t)q * nx + r) * P + d] = T(tr[d]);
+        }
+    }
+}
+
+template <typename T, int P>
+void launch_diag_inflow_batch_p(const DiagSweepParams<T> *batch, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL((k_inflow_batch<T, P, DiagSweepParams<T>>), dim3(64, nprob), dim3(256), 0, s, batch);
+    TSW_HIP(hipGetLastError());
+}
+
 #define TSW_SWEEP_INSTANTIATE(T, P)                                                                         \
+    template void launch_diag_inflow_batch_p<T, P>(const DiagSweepParams<T> *, int, hipStream_t);          \
     template void launch_diag_sweep_p<T, P>(const DiagSweepParams<T> &, size_t, hipStream_t);              \
     template void launch_diag_sweep_batch_p<T, P>(const DiagSweepParams<T> &, const DiagSweepParams<T> *,   \
                                                   const int2 *, int, int, size_t, hipStream_t);
 
 #define TSW_SWEEP_EXTERN(T, P)                                                                              \
+    extern template void launch_diag_inflow_batch_p<T, P>(const DiagSweepParams<T> *, int, hipStream_t);   \
     extern template void launch_diag_sweep_p<T, P>(const DiagSweepParams<T> &, size_t, hipStream_t);       \
     extern template void launch_diag_sweep_batch_p<T, P>(const DiagSweepParams<T> &, const DiagSweepParams<T> *, \
                                                          const int2 *, int, int, size_t, hipStream_t);

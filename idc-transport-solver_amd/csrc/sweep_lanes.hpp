@@ -432,36 +432,12 @@ void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hip
     }
 }
 
-// Inflow traces of a batch's sources (k_inflow per member): item y = member, x over ordinates x rows or columns.
-template <typename T, int P>
-__global__ void k_lanes_inflow_batch(const LanesParams<T> *__restrict__ batch) {
-    constexpr int N = P * P;
-    const LanesParams<T> sp = batch[blockIdx.y];
-    const int nx = sp.nx, ny = sp.ny, nq = sp.nq, len = nx > ny ? nx : ny;
-    for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < nq * len; t += (int)(gridDim.x * blockDim.x)) {
-        const int q = t / len, r = t - q * len;
-        const int Q = q >= sp.qo3 ? 3 : (q >= sp.qo2 ? 2 : (q >= sp.qo1 ? 1 : 0));
-        const bool xneg = Q >= 2, yneg = (Q & 1) != 0;
-        if (r < ny) {  // row r: the x-inflow halo cell
-            const int j = yneg ? ny - r : r + 1, i = xneg ? nx + 1 : 0;
-            double tr[P];
-            trace_x<double, P>(sp.src + (((size_t)i * (ny + 2) + j) * nq + q) * N, xneg, tr);
-#pragma unroll
-            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_x)[((size_t)q * ny + r) * P + d] = T(tr[d]);
-        }
-        if (r < nx) {  // column r: the y-inflow halo cell
-            const int i = xneg ? nx - r : r + 1, j = yneg ? ny + 1 : 0;
-            double tr[P];
-            trace_y<double, P>(sp.src + (((size_t)i * (ny + 2) + j) * nq + q) * N, yneg, tr);
-#pragma unroll
-            for (int d = 0; d < P; ++d) const_cast<T *>(sp.tin_y)[((size_t)q * nx + r) * P + d] = T(tr[d]);
-        }
-    }
-}
+inline __device__ const double *inflow_source(const LanesParams<double> &p) { return p.src; }
+inline __device__ const double *inflow_source(const LanesParams<float> &p) { return p.src; }
 
 template <typename T, int P>
 void launch_lanes_inflow_batch_p(const LanesParams<T> *batch, int nprob, hipStream_t s) {
-    hipLaunchKernelGGL((k_lanes_inflow_batch<T, P>), dim3(64, nprob), dim3(256), 0, s, batch);
+    hipLaunchKernelGGL((k_inflow_batch<T, P, LanesParams<T>>), dim3(64, nprob), dim3(256), 0, s, batch);
     TSW_HIP(hipGetLastError());
 }
 

@@ -134,6 +134,10 @@ template <typename T> struct DiagSweepParams {
     int wave1;             // 1: one wave per band of at most 64 rows, traces passed between lanes
     int split;             // 1: lane-split kernel (2 or 4 lanes per row, material path, direct stores)
     const int *skip;       // device flag: the launch does nothing while *skip != 0 (null: always runs)
+    // Reference-layout I/O (batch members): Q from qref and psi to psi_ref ([nx+2][ny+2][nq][N] doubles,
+    // may alias, 16-byte aligned) instead of qsrc / psi
+    const double *qref;
+    double *psi_ref;
 };
 
 // Ordinate-lane sweep on the reference layout (sweep_lanes.hpp).  An ordinate block is ordinates
@@ -224,6 +228,9 @@ size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p);
 template <typename T>
 void launch_diag_sweep_batch(const DiagSweepParams<T> &rep, const DiagSweepParams<T> *probs, const int2 *items,
                              int nitems, int max_ny, size_t lds, int p, hipStream_t s);
+// tin_x / tin_y of every member of a batch from the halos of its qref (device table of nprob members)
+template <typename T>
+void launch_diag_inflow_batch(const DiagSweepParams<T> *probs, int nprob, int p, hipStream_t s);
 // dir 0: reference psi -> diagonal psi; dir 1: diagonal -> reference (interior cells only)
 template <typename TR, typename TD>
 void launch_psi_convert(const Geom &g, TR *ref, TD *diag, int dir, hipStream_t s);

@@ -74,3 +74,122 @@ def test_full_batch_fp32_vs_fp64_and_oracle_members():
     for k in order:
         ref = oracle_sweep(batch[k], batch[k].source)
         assert rel_err(o64[k].cpu().numpy(), ref) <= 1e-12
+
+
+def _random_halo_sources(batch, seed):
+    rng = np.random.default_rng(seed)
+    srcs = []
+    for P in batch:
+        s = P.source.copy()
+        s[0] = rng.uniform(0, 1, size=s[0].shape)
+        s[:, -1] = rng.uniform(0, 1, size=s[:, -1].shape)
+        srcs.append(s)
+    return srcs
+
+
+@pytest.mark.parametrize("batch_direct", [-1, 1])
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
+def test_reference_layout_batch_matches_oracle(precision, batch_direct):
+    """batch_ref = 1: the staggered batch kernels read each member's source from, and write psi to, its
+    own reference-layout array (storer-wave and direct-store variants); TSW_MIXED takes the FP64 sweep."""
+    batch = problems.batched(seed=7, count=10, n_range=(3, 40), degrees=(0, 1, 2, 3), orders=(2, 4, 6, 8))
+    srcs = _random_halo_sources(batch, 3)
+    print("ref-layout batch", precision, batch_direct, flush=True)
+    tol = {tsw.FP64: 1e-12, tsw.FP32: 2e-6, tsw.MIXED: 1e-12}[precision]
+    with tsw.Context(0) as ctx:
+        ctx.set_tuning("batch_direct", batch_direct)
+        ctx.set_tuning("lanes", -1)
+        ctx.set_tuning("batch_ref", 1)
+        ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
+        for rep in range(2):
+            out = [s.copy() for s in srcs]
+            ctx.batch_sweep(out, precision=precision)
+            assert ctx.get_tuning("batch_ref") == 1
+            for P, s, o in zip(batch, srcs, out):
+                e = rel_err(o, oracle_sweep(P, s))
+                assert e <= tol, (rep, P.name, P.nx, P.p, P.nq, e)
+                np.testing.assert_array_equal(o[0], s[0])
+                np.testing.assert_array_equal(o[:, -1], s[:, -1])
+
+
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
+def test_reference_layout_batch_bitwise_equals_converted_batch(precision):
+    """Reading and writing the caller's layout changes where values move, not how they are computed."""
+    batch = problems.batched(seed=8, count=12, n_range=(5, 70), degrees=(0, 1, 2, 3), orders=(4, 8, 12))
+    srcs = _random_halo_sources(batch, 4)
+    outs = {}
+    for ref in (-1, 1):
+        with tsw.Context(0) as ctx:
+            ctx.set_tuning("lanes", -1)
+            ctx.set_tuning("batch_ref", ref)
+            ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
+            out = [s.copy() for s in srcs]
+            ctx.batch_sweep(out, precision=precision)
+            assert ctx.get_tuning("batch_ref") == (1 if ref == 1 else 0)
+            outs[ref] = out
+    for P, a, b in zip(batch, outs[-1], outs[1]):
+        np.testing.assert_array_equal(a, b, err_msg=P.name)
+
+
+def test_reference_layout_full_batch_extremes_match_oracle():
+    """The 256-problem batch through the reference-layout kernels in FP64; the thickest member (largest
+    sigma_t dx) and the smallest Q3 S12 member against the oracle, every member against the converted path."""
+    import torch
+    batch = problems.batched(seed=2026)
+    dev = torch.device("cuda:0")
+    srcs = [P.source_device(dev) for P in batch]
+    sig = [torch.from_numpy(P.sigma_t).to(dev) for P in batch]
+    outs = {}
+    for ref in (-1, 1):
+        with tsw.Context(0) as ctx:
+            ctx.set_tuning("lanes", -1)
+            ctx.set_tuning("batch_ref", ref)
+            ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
+            o = [s.clone() for s in srcs]
+            ctx.batch_sweep(o, precision=tsw.FP64)
+            assert ctx.get_tuning("batch_ref") == (1 if ref == 1 else 0)
+            outs[ref] = o
+    torch.cuda.synchronize()
+    for k in range(len(batch)):
+        assert torch.equal(outs[-1][k], outs[1][k]), batch[k].name
+    thick = max(range(len(batch)), key=lambda k: batch[k].sigma_t.max() / batch[k].nx)
+    q3 = min((k for k in range(len(batch)) if batch[k].p == 3 and batch[k].nq == 84), key=lambda k: batch[k].nx)
+    for k in (thick, q3):
+        P = batch[k]
+        print("extreme member", P.name, P.nx, P.p, P.nq, flush=True)
+        assert rel_err(outs[1][k].cpu().numpy(), oracle_sweep(P, P.source)) <= 1e-12, P.name
+
+
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
+def test_batch_engines_give_the_same_bits(precision):
+    """The three batch engines (staggered kernels through the layout conversions, staggered kernels on the
+    caller's layout, ordinate-lane engine for DG degrees 0-2) and the autotuner's choice agree bit for bit."""
+    batch = problems.batched(seed=9, count=12, n_range=(5, 60), degrees=(0, 1, 2, 3), orders=(2, 6, 12))
+    srcs = _random_halo_sources(batch, 6)
+    outs = {}
+    for name, lanes, ref in (("converted", -1, -1), ("ref", -1, 1), ("lanes", 1, -1), ("auto", 0, 0)):
+        with tsw.Context(0) as ctx:
+            ctx.set_tuning("lanes", lanes)
+            ctx.set_tuning("batch_ref", ref)
+            ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
+            engines = [ctx.get_tuning(f"batch_engine_p{p}") for p in range(4)]
+            print(name, "engines per DG degree", engines, flush=True)
+            present = {P.p for P in batch}
+            assert [p for p in range(4) if engines[p] != -1] == sorted(present)
+            for p in present:
+                if name == "converted":
+                    assert engines[p] == 0
+                elif name == "ref":
+                    assert engines[p] == 1
+                elif name == "lanes":  # degree 3 through the conversions (the lane engine is built for 0-2)
+                    assert engines[p] == (2 if p <= 2 else 0)
+                else:
+                    assert engines[p] in (0, 1, 2)
+            out = [s.copy() for s in srcs]
+            ctx.batch_sweep(out, precision=precision)
+            outs[name] = out
+    for name in ("ref", "lanes", "auto"):
+        for P, a, b in zip(batch, outs["converted"], outs[name]):
+            np.testing.assert_array_equal(a, b, err_msg=f"{name} {P.name}")
+    for P, s, o in zip(batch, srcs, outs["auto"]):
+        assert rel_err(o, oracle_sweep(P, s)) <= (1e-12 if precision == tsw.FP64 else 2e-6), P.name
