@@ -118,6 +118,13 @@ struct tsw_ctx {
     bool batch_ref_ok = false;  // every member's array admits 32-bit byte offsets
     int batch_engine[4] = {-1, -1, -1, -1};  // per DG degree present in the batch
     std::vector<int> batch_all;              // 0 .. nprob-1
+    // Many right-hand sides of this problem (tsw_sweep_apply_multi): one staggered launch over
+    // (source, ordinate) items sharing the material table and element inverses; per-source inflow traces.
+    struct Multi {
+        std::vector<unsigned char> host;
+        std::vector<int> items;
+        DBuf params, dev_items, traces, stage;
+    } multi;
     std::vector<Group> groups;
     // Batch members on the ordinate-lane engine, swept in place on the caller's arrays: one launch per DG
     // degree over (member, ordinate block, band) items, band-major so that every band's upstream band is
@@ -2126,6 +2133,78 @@ void sweep_wavefront_mixed(tsw_ctx *ctx, double *Q, const double *st, double dt)
     TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
 }
 
+// Sweeps of n right-hand sides arr[r] of this problem (reference layout, in place, 16-byte aligned) in
+// one launch of the staggered kernel on the caller's layout: items (source, ordinate), one band per
+// ordinate (ny <= 448), the material table and element inverses shared by every source; counted as n
+// sweeps.  The launch shape follows the batch: stores from the compute lanes when the material table
+// fits beside them, else the storer wave, else per-cell elimination.
+template <typename T>
+void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt) {
+    const Geom &g = c->g;
+    const bool f64 = sizeof(T) == 8;
+    const int n = (int)arr.size();
+    diag_setup(c, sig, nullptr, nullptr, dt, f64, !f64);
+    DiagSweepParams<T> base = make_diag_params<T>(c, dt, nullptr, nullptr, false, nullptr);
+    base.nbands = 1;
+    base.band_rows = g.ny;
+    base.xtr = nullptr;
+    base.wave1 = 0;
+    base.split = 0;
+    size_t lds = 0;
+    if (base.ainv) {
+        base.direct = 1;
+        lds = diag_sweep_lds_bytes<T>(base, g.p);
+        if (lds == 0) {
+            base.direct = 0;
+            lds = diag_sweep_lds_bytes<T>(base, g.p);
+        }
+    }
+    if (lds == 0) {  // per-cell elimination
+        base.ainv = nullptr;
+        base.mat = nullptr;
+        base.M = 0;
+        base.direct = 0;
+        base.sig_t = (const T *)(f64 ? c->diag.sigt64.ptr : c->diag.sigt32.ptr);
+        lds = diag_sweep_lds_bytes<T>(base, g.p);
+    }
+    require(lds != 0, TSW_E_UNSUPPORTED, "the mesh does not fit the staggered sweep");
+    const size_t ntx = (size_t)g.nq * g.ny * (g.p + 1), nty = (size_t)g.nq * g.nx * (g.p + 1);
+    T *tr = static_cast<T *>(c->multi.traces.reserve(sizeof(T) * (ntx + nty) * n));
+    auto &M = c->multi;
+    M.host.resize(sizeof(DiagSweepParams<T>) * n);
+    DiagSweepParams<T> *ps = reinterpret_cast<DiagSweepParams<T> *>(M.host.data());
+    for (int r = 0; r < n; ++r) {
+        ps[r] = base;
+        ps[r].qref = arr[r];
+        ps[r].psi_ref = arr[r];
+        ps[r].tin_x = tr + (ntx + nty) * r;
+        ps[r].tin_y = tr + (ntx + nty) * r + ntx;
+    }
+    M.items.resize((size_t)2 * n * g.nq);
+    for (int r = 0; r < n; ++r)  // the ordinates of one source side by side (they share cache lines)
+        for (int q = 0; q < g.nq; ++q) {
+            M.items[2 * ((size_t)r * g.nq + q)] = r;
+            M.items[2 * ((size_t)r * g.nq + q) + 1] = q;
+        }
+    hipStream_t s = c->stream;
+    TSW_HIP(hipMemcpyAsync(M.params.reserve(M.host.size()), M.host.data(), M.host.size(), hipMemcpyHostToDevice, s));
+    TSW_HIP(hipMemcpyAsync(M.dev_items.reserve(sizeof(int) * M.items.size()), M.items.data(),
+                           sizeof(int) * M.items.size(), hipMemcpyHostToDevice, s));
+    DiagSweepParams<T> rep{};
+    rep.qref = reinterpret_cast<const double *>(1);
+    rep.ainv = base.ainv;
+    rep.direct = base.direct;
+    rep.prefetch = 0;
+    const DiagSweepParams<T> *dp = M.params.as<DiagSweepParams<T>>();
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, s));
+    launch_diag_inflow_batch<T>(dp, n, g.p, s);
+    launch_diag_sweep_batch<T>(rep, dp, M.dev_items.as<int2>(), n * g.nq, g.ny, lds, g.p, s);
+    TSW_HIP(hipEventRecord(e1, s));
+    c->sweeps_done += n;
+    TSW_HIP(hipStreamSynchronize(s));  // the host tables may be rebuilt by the next call
+}
+
 void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int precision) {
     require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG, "bad precision");
     ctx->last_lanes = 0;
@@ -2764,6 +2843,45 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
         const double *sig = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         sweep_device(ctx, psi, sig, dt, precision);
         if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
+int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, const double *sigma_t, double dt,
+                          int precision, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        require(n_rhs >= 1 && psi_inout != nullptr, TSW_E_ARG, "empty right-hand-side list");
+        require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
+                "bad precision");
+        for (int r = 0; r < n_rhs; ++r) require(psi_inout[r] != nullptr, TSW_E_ARG, "null psi array");
+        const double *sig = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
+        std::vector<double *> dev(n_rhs);
+        const size_t npsi = g.npsi(), stride = (npsi * sizeof(double) + 255) / 256 * 256;
+        if (mem == TSW_MEM_HOST) {
+            char *base = static_cast<char *>(ctx->multi.stage.reserve(stride * n_rhs));
+            for (int r = 0; r < n_rhs; ++r) {
+                dev[r] = reinterpret_cast<double *>(base + stride * r);
+                TSW_HIP(hipMemcpyAsync(dev[r], psi_inout[r], npsi * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            }
+        } else {
+            require(mem == TSW_MEM_DEVICE, TSW_E_ARG, "mem must be TSW_MEM_HOST or TSW_MEM_DEVICE");
+            for (int r = 0; r < n_rhs; ++r) dev[r] = psi_inout[r];
+        }
+        bool one_launch = staggered_ok(ctx) && ctx->pattern == Pattern::Staggered && ctx->xs_deg == 0 &&
+                          npsi * sizeof(double) < (size_t(1) << 32) && g.ny <= 448;
+        for (int r = 0; r < n_rhs; ++r) one_launch = one_launch && ((uintptr_t)dev[r] & 15) == 0;
+        if (one_launch) {
+            // TSW_MIXED: the FP64 sweep (this engine reads and writes the FP64 arrays in either precision)
+            if (precision == TSW_FP32) multi_sweep<float>(ctx, dev, sig, dt);
+            else multi_sweep<double>(ctx, dev, sig, dt);
+        } else {
+            for (int r = 0; r < n_rhs; ++r) sweep_device(ctx, dev[r], sig, dt, precision);
+        }
+        if (mem == TSW_MEM_HOST)
+            for (int r = 0; r < n_rhs; ++r)
+                TSW_HIP(hipMemcpyAsync(psi_inout[r], dev[r], npsi * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         sync(ctx);
     });
 }

@@ -91,6 +91,7 @@ def lib():
         L.tsw_get_tuning.argtypes = [c, ctypes.c_char_p, _ip]
         L.tsw_sweep_apply.argtypes = [c, _vp, _vp, d, i, i]
         L.tsw_sweep_apply_noalloc.argtypes = [c, _vp, _vp, d, i]
+        L.tsw_sweep_apply_multi.argtypes = [c, i, ctypes.POINTER(_vp), _vp, d, i, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
         L.tsw_smv.argtypes = [c, d, d, _vp, _vp, _vp, i, i]
         L.tsw_lmv.argtypes = [c, d, d, _vp, _vp, _vp, i]
@@ -289,6 +290,15 @@ class Context:
         (pp, sp), mem = _addrs(psi, sigma_t)
         self._check(self._L.tsw_sweep_apply(self._h, pp, sp, dt, precision, mem))
         return psi
+
+    def sweep_multi(self, psi_list, sigma_t, dt=math.inf, precision=FP64):
+        """tsw_sweep_apply_multi: every array of psi_list swept in place (same layouts as sweep)."""
+        for a in psi_list:
+            self._expect(psi=a, xs=sigma_t)
+        ptrs, mem = _addrs(*psi_list, sigma_t)
+        parr = (_vp * len(psi_list))(*ptrs[:-1])
+        self._check(self._L.tsw_sweep_apply_multi(self._h, len(psi_list), parr, ptrs[-1], dt, precision, mem))
+        return psi_list
 
     def sweep_noalloc(self, psi, sigma_t, dt=math.inf, precision=FP64):
         """tsw_sweep_apply_noalloc: device tensors only; no device allocation inside the call."""

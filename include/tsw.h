@@ -129,6 +129,18 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
  * to single precision for the sweep and back; TSW_MIXED refines that FP32 sweep to FP64 accuracy. */
 int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem);
 
+/* Many right-hand sides: psi_inout[r] <- L^{-1} psi_inout[r] for r < n_rhs, each as tsw_sweep_apply would
+ * sweep it (same sigma_t and dt).  It has no counterpart in the reference, whose SweepOperator::Apply
+ * (SweepOperator.cpp:118-135) takes one source: a caller with several sources of one problem (sources of
+ * several detectors, the stages of an implicit step) calls it once instead of Apply per source.  On the
+ * "hip-staggered" pattern with ny <= 448 and 16-byte aligned arrays the sweeps run as one launch over
+ * (source, ordinate) work items that share the material table and element inverses, reading and writing
+ * the caller's arrays directly; results are bitwise those of tsw_sweep_apply with the tuning key "lanes"
+ * at -1 in TSW_FP64 and TSW_FP32, and TSW_MIXED is the FP64 sweep there (it moves the same FP64 arrays and
+ * its FP32 arithmetic is not faster).  Elsewhere the sources are swept one after the other. */
+int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, const double *sigma_t, double dt,
+                          int precision, int mem);
+
 /* tsw_sweep_apply without any device allocation, for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It has no counterpart in the
  * reference.  psi_inout and sigma_t are device arrays (TSW_MEM_DEVICE layouts, cell-mean cross sections);
