@@ -109,7 +109,7 @@ struct tsw_ctx {
         std::vector<unsigned char> ref_host;
         DBuf ref_params;
         size_t ref_lds = 0;
-        bool ref_mat = false, ref_direct = false;
+        bool ref_mat = false, ref_direct = false, ref_stage = false;
     };
     // Engine of each batch member, chosen per DG degree (tsw_batch_set): the staggered kernels through
     // layout conversions, the staggered kernels on the caller's layout, or the ordinate-lane engine.
@@ -172,6 +172,7 @@ struct tsw_ctx {
     // whenever every member has element-inverse tables (tuning key batch_direct)
     int batch_direct_opt = 0;
     int batch_ref_opt = 0;   // batch engines: 1 / -1 force / forbid the reference-layout staggered groups
+    int ref_stage_opt = 0;   // 1: reference-layout sweeps stage their source loads through LDS where it fits
     bool last_batch_ref = false;
     DBuf tune_out;
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
@@ -2133,6 +2134,37 @@ void sweep_wavefront_mixed(tsw_ctx *ctx, double *Q, const double *st, double dt)
     TSW_HIP(hipMemcpyAsync(Q, psi64, sizeof(double) * npsi, hipMemcpyDeviceToDevice, s));
 }
 
+// Launch shape of reference-layout sweeps: staged (coalesced) source loads with the storer wave, else
+// with stores from the compute lanes (material path only), when the stages fit every member's LDS beside
+// its tables; otherwise `ps` keep their shape (direct) unstaged.  Returns whether the sources are staged;
+// lds and direct receive the launch's values.
+template <typename T>
+bool ref_stage_layout(std::vector<DiagSweepParams<T>> &ps, int p, bool mat, size_t &lds, bool &direct) {
+    for (int d = 0; d <= (mat ? 1 : 0); ++d) {
+        size_t need = 0;
+        bool fits = true;
+        for (auto &sp : ps) {
+            DiagSweepParams<T> t = sp;
+            t.direct = d;
+            t.ref_stage = 1;
+            const size_t b = diag_sweep_lds_bytes<T>(t, p);
+            fits = fits && b != 0;
+            need = std::max(need, b);
+        }
+        if (fits) {
+            for (auto &sp : ps) {
+                sp.direct = d;
+                sp.ref_stage = 1;
+            }
+            lds = need;
+            direct = d != 0;
+            return true;
+        }
+    }
+    for (auto &sp : ps) sp.ref_stage = 0;
+    return false;
+}
+
 // Sweeps of n right-hand sides arr[r] of this problem (reference layout, in place, 16-byte aligned) in
 // one launch of the staggered kernel on the caller's layout: items (source, ordinate), one band per
 // ordinate (ny <= 448), the material table and element inverses shared by every source; counted as n
@@ -2168,6 +2200,14 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
         lds = diag_sweep_lds_bytes<T>(base, g.p);
     }
     require(lds != 0, TSW_E_UNSUPPORTED, "the mesh does not fit the staggered sweep");
+    if (c->ref_stage_opt > 0) {
+        std::vector<DiagSweepParams<T>> one{base};
+        one[0].qref = reinterpret_cast<const double *>(1);
+        bool direct = base.direct != 0;
+        ref_stage_layout<T>(one, g.p, base.ainv != nullptr, lds, direct);
+        base.direct = one[0].direct;
+        base.ref_stage = one[0].ref_stage;
+    }
     const size_t ntx = (size_t)g.nq * g.ny * (g.p + 1), nty = (size_t)g.nq * g.nx * (g.p + 1);
     T *tr = static_cast<T *>(c->multi.traces.reserve(sizeof(T) * (ntx + nty) * n));
     auto &M = c->multi;
@@ -2194,6 +2234,7 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
     rep.qref = reinterpret_cast<const double *>(1);
     rep.ainv = base.ainv;
     rep.direct = base.direct;
+    rep.ref_stage = base.ref_stage;
     rep.prefetch = 0;
     const DiagSweepParams<T> *dp = M.params.as<DiagSweepParams<T>>();
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
@@ -2774,6 +2815,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
             require(value >= -1 && value <= 1, TSW_E_ARG, "batch_direct must be -1, 0 or 1");
             ctx->batch_direct_opt = value;
             for (auto &G : ctx->groups) G.built32 = G.built32c = G.built64 = false;
+        } else if (k == "ref_stage") {
+            require(value == 0 || value == 1, TSW_E_ARG, "ref_stage must be 0 or 1");
+            ctx->ref_stage_opt = value;
         } else if (k == "batch_ref") {
             require(value == -1 || value == 0 || value == 1, TSW_E_ARG, "batch_ref must be -1, 0 or 1");
             ctx->batch_ref_opt = value;
@@ -2824,6 +2868,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "lane_split") *value = ctx->last_sl;
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;
         else if (k == "batch_ref") *value = ctx->last_batch_ref ? 1 : 0;
+        else if (k == "ref_stage") *value = ctx->ref_stage_opt;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
             *value = ctx->batch_engine[k[14] - '0'];
         else if (k == "lanes") *value = ctx->batch.empty() ? ctx->last_lanes : (ctx->lane_groups.empty() ? 0 : 1);
@@ -3459,13 +3504,14 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
     }
     require(lds != 0, TSW_E_UNSUPPORTED, "batch group does not fit the staggered sweep");
     if (ref) {
+        G.ref_direct = all_mat && any_direct;
+        G.ref_stage = ctx->ref_stage_opt > 0 && ref_stage_layout<T>(ps, G.p, all_mat, lds, G.ref_direct);
         G.ref_host.resize(sizeof(DiagSweepParams<T>) * ps.size());
         std::memcpy(G.ref_host.data(), ps.data(), G.ref_host.size());
         TSW_HIP(hipMemcpyAsync(G.ref_params.reserve(G.ref_host.size()), G.ref_host.data(), G.ref_host.size(),
                                hipMemcpyHostToDevice, ctx->stream));
         G.ref_lds = lds;
         G.ref_mat = all_mat;
-        G.ref_direct = all_mat && any_direct;
         return;
     }
     DBuf &buf = sizeof(T) == 8 ? G.params64 : (corr ? G.params32c : G.params32);
@@ -3607,6 +3653,7 @@ void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
         rep.qref = reinterpret_cast<const double *>(1);
         rep.ainv = G.ref_mat ? reinterpret_cast<const T *>(1) : nullptr;
         rep.direct = G.ref_direct ? 1 : 0;
+        rep.ref_stage = G.ref_stage ? 1 : 0;
         lds = G.ref_lds;
         launch_diag_inflow_batch<T>(probs, (int)G.members.size(), G.p, sd);
     } else {

@@ -51,13 +51,14 @@ size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     const int cthreads = ((BR + 63) / 64) * 64;
     const int M = sp.ainv ? sp.M : 0;
     const bool direct = (sp.ainv && sp.direct) || sp.wave1;
+    const int stage_w = sp.qref && sp.ref_stage ? cthreads / 64 : 0;  // compute waves with a source stage
     if (sp.ainv && sp.direct && sp.nbands > 1) return 0;
     size_t total;
     switch (p) {
-        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, M, direct).total; break;
-        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, M, direct).total; break;
-        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, M, direct).total; break;
-        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, M, direct).total; break;
+        case 0: total = DiagLds<T, 1>(cthreads, sp.nx, M, direct, stage_w).total; break;
+        case 1: total = DiagLds<T, 2>(cthreads, sp.nx, M, direct, stage_w).total; break;
+        case 2: total = DiagLds<T, 3>(cthreads, sp.nx, M, direct, stage_w).total; break;
+        case 3: total = DiagLds<T, 4>(cthreads, sp.nx, M, direct, stage_w).total; break;
         default: return 0;
     }
     const size_t bytes = total * sizeof(T);

@@ -90,13 +90,15 @@ template <typename T, int P>
 struct DiagLds {  // LDS carve-up in units of T; every array and row is 16-byte aligned
     static constexpr int N = P * P;
     static constexpr int PS = vround<T>(P), NS = vround<T>(N), MS = vround<T>(N * N);
-    size_t ex, in, out, inv, total;
-    __host__ __device__ DiagLds(int cthreads, int nx, int M, bool direct) {
+    size_t ex, in, out, inv, stage, total;
+    // stage_waves: compute waves with a source stage of 64 N doubles each (staged reference-layout loads)
+    __host__ __device__ DiagLds(int cthreads, int nx, int M, bool direct, int stage_waves = 0) {
         ex = 0;                                 // [2][cthreads][PS] y-traces between rows
         in = ex + (size_t)2 * cthreads * PS;    // [nx][PS] y-inflow traces of row 0
         out = in + (size_t)nx * PS;             // [2][cthreads][NS] solved cells awaiting the store
         inv = out + (direct ? 0 : (size_t)2 * cthreads * NS);  // [M][MS] element inverses (material path)
-        total = inv + (size_t)M * MS;
+        stage = inv + (size_t)M * MS;           // [stage_waves][64 N] double
+        total = stage + (size_t)stage_waves * 64 * N * (8 / sizeof(T));
     }
 };
 
@@ -233,8 +235,13 @@ struct DiagWalk {
 };
 
 // MODE bit 1: add Q from `qsrc`; bit 2: add sigma_s * phi / (4 pi); bit 4 (with bit 1): Q from `qref` and
-// psi to `psi_ref`, both in the reference layout (a cell's N values of the ordinate are one contiguous run
-// per lane, read and written by 16-byte accesses where N is even), so no layout conversion is needed.
+// psi to `psi_ref`, both in the reference layout, so no layout conversion is needed.  There a cell's N
+// values of the ordinate are one contiguous run, and the runs of a wavefront's cells lie far apart; the
+// storer wave writes them as consecutive elements of the concatenated runs (lane l element l + 64 m), so
+// the lanes of a store touch a few cache lines rather than one line each.  Bit 8 (with bit 4) loads the
+// sources the same way into a per-wave LDS stage that hands each value to the lane of its cell (the
+// address unit, not HBM, otherwise sets the step time); without it each lane reads its own run (16-byte
+// accesses where N is even).
 // Workgroup b*nq + q sweeps rows [b*band_rows, ...) of ordinate q (band-major numbering, so a band's
 // upstream neighbour is dispatched before it and never waits on it).  Store variant ST: 0, a storer wave
 // writes the cells staged in LDS; 1 (DIRECT, one band only), no storer wave, each compute lane stores its
@@ -258,7 +265,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                                                     const int2 *__restrict__ items) {
     constexpr int N = P * P;
     constexpr bool DIRECT = ST != 0, W1 = ST == 2;
-    constexpr bool REF = (MODE & 4) != 0;
+    constexpr bool REF = (MODE & 4) != 0, RSTAGE = REF && (MODE & 8) != 0;
     static_assert(!(REF && W1), "reference-layout I/O is built for the storer-wave and direct-store variants");
     using PQ = typename std::conditional<REF, double, T>::type;  // prefetched source (converted at use)
     using Lds = DiagLds<T, P>;
@@ -286,8 +293,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
     // every compute thread owns one LDS row slot; threads past the band's rows compute on clamped
     // inputs and their results are never read
     const int cthreads = DIRECT ? (int)blockDim.x : (int)blockDim.x - 64;
-    const Lds L(cthreads, nx, MAT ? sp.M : 0, DIRECT);
+    const Lds L(cthreads, nx, MAT ? sp.M : 0, DIRECT, RSTAGE ? cthreads / 64 : 0);
     T *ex = lds + L.ex, *lin = lds + L.in, *lout = lds + L.out, *linv = lds + L.inv;
+    // this compute wave's source stage (staged reference-layout loads)
+    double *rstage = RSTAGE ? reinterpret_cast<double *>(lds + L.stage) + (size_t)(threadIdx.x >> 6) * 64 * N : nullptr;
     const DiagMap dm{nx, ny};
     const size_t ncell = (size_t)dm.cells();
     // exchange records: read from the band below (xin), written for the band above (xout)
@@ -339,12 +348,17 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
             const int s = w.s;
             const int lo = s - nx + 1 > R0 ? s - nx + 1 : R0;
             const int hi = s < R0 + nrows - 1 ? s : R0 + nrows - 1;
-            for (int r = lo + lane; r <= hi; r += 64) {
-                T v[N];
-                lds_get<T, N>(slot + (size_t)(r - R0) * NS, v);
-                if constexpr (REF) {
-                    ref_st<T, N>(rpsi, ref_off(s - r, r, nx, ny, sp.nq, q, N, xneg, yneg), v);
-                } else {
+            if constexpr (REF) {  // element e of the runs of rows lo.. (coefficient e % N of row lo + e / N)
+                const int E = (hi - lo + 1) * N;
+                for (int e = lane; e < E; e += 64) {
+                    const int g = e / N, o = e - g * N, r = lo + g;
+                    buf_st<double>(rpsi, ref_off(s - r, r, nx, ny, sp.nq, q, N, xneg, yneg) + 8u * (unsigned)o, 0,
+                                   (double)slot[(size_t)(r - R0) * NS + o]);
+                }
+            } else {
+                for (int r = lo + lane; r <= hi; r += 64) {
+                    T v[N];
+                    lds_get<T, N>(slot + (size_t)(r - R0) * NS, v);
                     const unsigned boff = (unsigned)(w.off + r) * (unsigned)sizeof(T);
 #pragma unroll
                     for (int k = 0; k < N; ++k) buf_st<T>(rpsi, boff, k * plane, v[k]);
@@ -396,7 +410,16 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
         const int rc = r < lo ? lo : (r > hi ? hi : r);
         const unsigned p = (unsigned)((live ? fw.off : 0) + rc);
         const unsigned bt = p * (unsigned)sizeof(T), bi = p * 4u;
-        if constexpr (REF) {  // the cell (clamped into the mesh on padded steps) in the reference layout
+        if constexpr (RSTAGE) {  // element lane + 64 m of the runs of this wave's rows (clamped as above)
+            const int wl = tid & 63, wrow = R0 + (tid & ~63);
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                const int e = wl + 64 * m, g = e / N, o = e - g * N;
+                const int rg = wrow + g < lo ? lo : (wrow + g > hi ? hi : wrow + g);
+                const int cl = fw.s - rg < 0 ? 0 : (fw.s - rg >= nx ? nx - 1 : fw.s - rg);
+                pq[u][m] = buf_ld<double>(rq, ref_off(cl, rg, nx, ny, sp.nq, q, N, xneg, yneg) + 8u * (unsigned)o, 0);
+            }
+        } else if constexpr (REF) {  // the cell (clamped into the mesh on padded steps) in the reference layout
             const int cl = fw.s - rc < 0 ? 0 : (fw.s - rc >= nx ? nx - 1 : fw.s - rc);
             ref_ld<N>(rq, ref_off(cl, rc, nx, ny, sp.nq, q, N, xneg, yneg), pq[u]);
         } else if (HASQ) {
@@ -491,6 +514,17 @@ __global__ __launch_bounds__(512) void k_diag_sweep(DiagSweepParams<T> sp1, cons
                     }
 #pragma unroll
                     for (int d = 0; d < P; ++d) ty[d] = xrow ? (active ? from_bits<T>(px[u].v[d]) : T(0)) : ty[d];
+                }
+                if constexpr (RSTAGE) {  // the stage hands each lane its own cell's run
+                    const int wl = tid & 63;
+#pragma unroll
+                    for (int m = 0; m < N; ++m) rstage[wl + 64 * m] = pq[u][m];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+                    for (int k = 0; k < N; ++k) pq[u][k] = rstage[wl * N + k];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
                 }
                 T a[N];  // bsc * source
                 if (HASPHI) {
@@ -674,8 +708,13 @@ void launch_diag_sweep_batch_p(const DiagSweepParams<T> &rep, const DiagSweepPar
     if (rep.qref) {  // reference-layout I/O: sources in place, the default prefetch depth
         constexpr int D = sizeof(T) == 8 ? 2 : 3;
         if (rep.phi) throw Error(TSW_E_ARG, "reference-layout batch sweep: sources only");
-        if (rep.ainv) launch_ds_d<T, P, true, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
-        else launch_ds_d<T, P, false, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
+        if (rep.ref_stage) {
+            if (rep.ainv) launch_ds_d<T, P, true, 13, D>(rep, lds, s, probs, items, nitems, max_ny);
+            else launch_ds_d<T, P, false, 13, D>(rep, lds, s, probs, items, nitems, max_ny);
+        } else {
+            if (rep.ainv) launch_ds_d<T, P, true, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
+            else launch_ds_d<T, P, false, 5, D>(rep, lds, s, probs, items, nitems, max_ny);
+        }
         return;
     }
     const int mode = (rep.qsrc ? 1 : 0) | (rep.phi ? 2 : 0);

@@ -138,6 +138,7 @@ template <typename T> struct DiagSweepParams {
     // may alias, 16-byte aligned) instead of qsrc / psi
     const double *qref;
     double *psi_ref;
+    int ref_stage;         // 1: reference-layout sources loaded through a per-wave LDS stage (coalesced)
 };
 
 // Ordinate-lane sweep on the reference layout (sweep_lanes.hpp).  An ordinate block is ordinates

@@ -87,11 +87,13 @@ def _random_halo_sources(batch, seed):
     return srcs
 
 
+@pytest.mark.parametrize("ref_stage", [0, 1])
 @pytest.mark.parametrize("batch_direct", [-1, 1])
 @pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
-def test_reference_layout_batch_matches_oracle(precision, batch_direct):
+def test_reference_layout_batch_matches_oracle(precision, batch_direct, ref_stage):
     """batch_ref = 1: the staggered batch kernels read each member's source from, and write psi to, its
-    own reference-layout array (storer-wave and direct-store variants); TSW_MIXED takes the FP64 sweep."""
+    own reference-layout array (storer-wave and direct-store variants, sources loaded by each lane or
+    through the LDS stage); TSW_MIXED takes the FP64 sweep."""
     batch = problems.batched(seed=7, count=10, n_range=(3, 40), degrees=(0, 1, 2, 3), orders=(2, 4, 6, 8))
     srcs = _random_halo_sources(batch, 3)
     print("ref-layout batch", precision, batch_direct, flush=True)
@@ -100,6 +102,7 @@ def test_reference_layout_batch_matches_oracle(precision, batch_direct):
         ctx.set_tuning("batch_direct", batch_direct)
         ctx.set_tuning("lanes", -1)
         ctx.set_tuning("batch_ref", 1)
+        ctx.set_tuning("ref_stage", ref_stage)
         ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
         for rep in range(2):
             out = [s.copy() for s in srcs]
@@ -118,17 +121,19 @@ def test_reference_layout_batch_bitwise_equals_converted_batch(precision):
     batch = problems.batched(seed=8, count=12, n_range=(5, 70), degrees=(0, 1, 2, 3), orders=(4, 8, 12))
     srcs = _random_halo_sources(batch, 4)
     outs = {}
-    for ref in (-1, 1):
+    for ref, stage in ((-1, 0), (1, 0), (1, 1)):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", -1)
             ctx.set_tuning("batch_ref", ref)
+            ctx.set_tuning("ref_stage", stage)
             ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
             out = [s.copy() for s in srcs]
             ctx.batch_sweep(out, precision=precision)
             assert ctx.get_tuning("batch_ref") == (1 if ref == 1 else 0)
-            outs[ref] = out
-    for P, a, b in zip(batch, outs[-1], outs[1]):
-        np.testing.assert_array_equal(a, b, err_msg=P.name)
+            outs[ref, stage] = out
+    for key in ((1, 0), (1, 1)):
+        for P, a, b in zip(batch, outs[-1, 0], outs[key]):
+            np.testing.assert_array_equal(a, b, err_msg=f"{key} {P.name}")
 
 
 def test_reference_layout_full_batch_extremes_match_oracle():

@@ -31,14 +31,16 @@ def _sources(src, n, seed):
     return out
 
 
+@pytest.mark.parametrize("ref_stage", [0, 1])
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
 @pytest.mark.parametrize("order", [4, 8])
-def test_multi_bitwise_equals_single_sweeps(p, order):
+def test_multi_bitwise_equals_single_sweeps(p, order, ref_stage):
     o = ls_ordinates(order)
     g, st, _, src = random_problem(29, 23, p, o, seed=3100 + 10 * p + order, materials=5)
     srcs = _sources(src, 5, 7 + p)
-    print(f"multi p={p} S{order}", flush=True)
+    print(f"multi p={p} S{order} ref_stage={ref_stage}", flush=True)
     with _ctx(g, o) as c:
+        c.set_tuning("ref_stage", ref_stage)
         for dt in (math.inf, 0.2):
             for prec in (tsw.FP64, tsw.FP32):
                 one = []
