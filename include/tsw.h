@@ -37,7 +37,11 @@ extern "C" {
 #define TSW_FP32 1 /* sweep and element solves in single precision (outer loops stay FP64) */
 #define TSW_MIXED 2 /* sweeps (tsw_sweep_apply, tsw_batch_sweep): FP32 sweeps with FP64 iterative refinement,
                      * psi_0 = L32^{-1} src, then refine_steps times psi += L32^{-1}(src - L psi) with the
-                     * residual in FP64; solvers treat it as TSW_FP32 */
+                     * residual in FP64; on the engines that sweep the caller's FP64 arrays in place
+                     * (batch members on the ordinate-lane engine or the staggered kernels on the
+                     * caller's layout, tsw_sweep_apply_multi) the FP64 sweep, which moves the same
+                     * bytes and reaches the accuracy without a second sweep; solvers treat it as
+                     * TSW_FP32 */
 
 #define TSW_MEM_HOST 0
 #define TSW_MEM_DEVICE 1
@@ -101,7 +105,17 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * off): while band_rows, prefetch, one_wave and lane_split are all 0, the first sweep of each kind
  * (mesh, degree, ordinates, precision, source terms, material path) times every candidate (kernel, band
  * rows, prefetch) into scratch memory and keeps the fastest for later sweeps; inside a HIP graph capture
- * the defaults apply.  Results do not depend on any of these settings (bit for bit).
+ * the defaults apply.  Key "lanes" (0 default, 1, -1): the ordinate-lane engine (J ordinates of a quadrant
+ * block times 64/J rows per wavefront, reading and writing the caller's layout; DG degrees 0-2, material
+ * path) runs single-problem sweeps, GMRES and source iteration with 1; for tsw_batch_set, 1 / -1 force /
+ * forbid it per DG degree and 0 leaves the choice to the batch autotuner.  "lanes_block" (0 automatic,
+ * 1..64): ordinates per block; "lanes_waves" (0 automatic, 1..4): row-group waves per workgroup.  Key
+ * "batch_ref" (0 default, 1, -1): tsw_batch_set's staggered groups on the caller's layout (no layout
+ * conversions) forced, forbidden, or left to the autotuner, which times, per DG degree, the staggered
+ * kernels through conversions, on the caller's layout and the ordinate-lane engine, and keeps the
+ * fastest.  Key "ref_stage" (0 default, 1): sweeps on the caller's layout (batch groups,
+ * tsw_sweep_apply_multi) stage their source loads through LDS where it fits.  Results do not depend on
+ * any of these settings (bit for bit).
  * Solver keys: "use_dsa" (0 default, 1): diffusion synthetic acceleration of tsw_gmres_solve and
  * tsw_solve_vjp, the reference's `use_dsa` parameter (PETScPeierlsSolver.cpp:151, shell preconditioner
  * :180-182, apply :588-640), applied on the left as PETSc does, so the residual test is on
@@ -119,8 +133,12 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
 
 /* Reads back "band_rows" (rows per band of the last staggered sweep, 0 = one band per ordinate),
  * "prefetch" (its prefetch depth), "one_wave" (1 if it ran the one-wavefront kernel), "lane_split" (1 if
- * it ran the lane-split kernel), or the setting of
- * "batch_direct", "autotune", "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
+ * it ran the lane-split kernel), "lanes" (1 if the last sweep, or for a batch context any member, ran on
+ * the ordinate-lane engine), "lanes_block" / "lanes_waves" (its shape), "batch_ref" (1 if the last
+ * tsw_batch_sweep ran staggered groups on the caller's layout), "batch_engine_p0" .. "batch_engine_p3"
+ * (the engine tsw_batch_set chose for that DG degree: 0 staggered through conversions, 1 staggered on the
+ * caller's layout, 2 ordinate lanes, -1 none in the batch), or the setting of "batch_direct", "ref_stage",
+ * "autotune", "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
@@ -323,9 +341,13 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
                   int mem);
 
 /* Sweeps every member of the batch in place (psi[b] in the reference layout, halos = inflow), in
- * one set of concurrent launches (one launch per DG degree and row count, workgroups = ordinates of
- * all members).  SweepOperator::Apply semantics per member.  Timed as one sweep by tsw_sweep_timing
- * (TSW_MIXED: each of its FP32 batched sweeps is timed as one). */
+ * one set of concurrent launches on four streams: per DG degree the engine tsw_batch_set chose (the
+ * staggered kernels, one launch per degree and row count over (member, ordinate) items, through layout
+ * conversions or on the caller's arrays; or the ordinate-lane engine, one launch per degree over (member,
+ * ordinate block, band) items on the caller's arrays).  The engines give the same bits.  SweepOperator::
+ * Apply semantics per member.  Timed as one sweep by tsw_sweep_timing (TSW_MIXED on converted members:
+ * each of its FP32 batched sweeps is timed as one).  Device arrays that are not 16-byte aligned send the
+ * call's staggered groups through the conversions. */
 int tsw_batch_sweep(tsw_ctx *ctx, double *const *psi, int precision, int mem);
 
 /* Timing of the sweeps run on ctx since the last reset, measured with HIP events on the context's
