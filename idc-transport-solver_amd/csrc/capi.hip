@@ -160,7 +160,7 @@ struct tsw_ctx {
     // automatic, 1 on where built), "lanes_block" (ordinates per row group, 0 automatic) and "lanes_waves"
     // (waves per workgroup, 0 automatic); the ordinate-block table for the block size in use, the inflow
     // traces and the per-block moment partials.
-    int lanes_opt = 0, lanes_J_opt = 0, lanes_W_opt = 0;
+    int lanes_opt = 0, lanes_J_opt = 0, lanes_W_opt = 0, lanes_pf_opt = 0;
     bool lanes_fixed_W = false;  // batch members: every member's workgroup has lanes_W_opt waves
     struct Lanes {
         DBuf blk, mom, tinx64, tiny64, tinx32, tiny32, psi_keep;
@@ -1382,6 +1382,9 @@ LanesParams<T> lanes_params(tsw_ctx *c, double dt, const double *src, const doub
     sp.dx = T(g.dx); sp.dy = T(g.dy);
     sp.blk = c->lanes.blk.as<LaneBlock>();
     sp.nblk = c->lanes.nblk;
+    sp.prefetch = c->lanes_pf_opt;
+    require(sp.prefetch == 0 || sp.prefetch == (f64 ? 2 : 3) || sp.prefetch == (f64 ? 4 : 6), TSW_E_ARG,
+            f64 ? "ordinate-lane prefetch depth must be 2 or 4 in FP64" : "ordinate-lane prefetch depth must be 3 or 6 in FP32");
     sp.J = J;
     sp.RW = 64 / J;
     sp.src = src;
@@ -2843,6 +2846,10 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "lanes_waves") {
             require(value >= 0 && value <= 4, TSW_E_ARG, "lanes_waves must be 0..4");
             ctx->lanes_W_opt = value;
+        } else if (k == "lanes_prefetch") {
+            require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
+                    "lanes_prefetch must be 0, 2, 3, 4 or 6");
+            ctx->lanes_pf_opt = value;
         } else if (k == "stall_test") {
             require(value == 0 || value == 1, TSW_E_ARG, "stall_test must be 0 or 1");
             ctx->stall_test = value != 0;
@@ -2874,6 +2881,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "lanes") *value = ctx->batch.empty() ? ctx->last_lanes : (ctx->lane_groups.empty() ? 0 : 1);
         else if (k == "lanes_block") *value = ctx->last_lanes_J;
         else if (k == "lanes_waves") *value = ctx->last_lanes_W;
+        else if (k == "lanes_prefetch") *value = ctx->lanes_pf_opt;
         else throw Error(TSW_E_ARG, "unknown tuning key '" + k + "'");
     });
 }
@@ -3408,6 +3416,7 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             sub->lanes_J_opt = ctx->lanes_J_opt;
             sub->lanes_W_opt = ctx->lanes_W_opt > 0 ? std::min(ctx->lanes_W_opt, kLanesStoreWaves) : kLanesStoreWaves;
             sub->lanes_fixed_W = true;
+            sub->lanes_pf_opt = ctx->lanes_pf_opt;
             ctx->lanes_able[b] = ctx->lanes_opt >= 0 && lanes_ok(sub.get()) && sub->mat.valid &&
                                  diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt);
             ctx->batch.push_back(std::move(sub));

@@ -397,10 +397,9 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp1, const L
     if (MOM) reduce(nsteps - 1);
 }
 
-template <typename T, int P, int MODE>
-void launch_lanes_d(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s, const LanesParams<T> *batch,
-                    const int4 *items, int nitems) {
-    constexpr int D = sizeof(T) == 8 ? 2 : 3;
+template <typename T, int P, int MODE, int D>
+void launch_lanes_dd(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s, const LanesParams<T> *batch,
+                     const int4 *items, int nitems) {
     auto go = [&](auto kern, int blocks) {
         if (lds > 64 * 1024)
             TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -414,6 +413,20 @@ void launch_lanes_d(const LanesParams<T> &sp, int threads, size_t lds, hipStream
     } else {
         throw Error(TSW_E_ARG, "ordinate-lane batch: sources only");
     }
+}
+
+// Prefetch depth D (steps of cell inputs in flight per lane): sp.prefetch (FP64 2 or 4, FP32 3 or 6), by
+// default 2 in FP64 and 3 in FP32 (the deeper rings measured 3-9% slower on the 280x280 Q2 S12 lattice:
+// the step is set by the lane's own instruction stream, not by memory latency).
+template <typename T, int P, int MODE>
+void launch_lanes_d(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s, const LanesParams<T> *batch,
+                    const int4 *items, int nitems) {
+    constexpr bool F64 = sizeof(T) == 8;
+    const int D = sp.prefetch > 0 ? sp.prefetch : (F64 ? 2 : 3);
+    if (D == (F64 ? 2 : 3)) launch_lanes_dd<T, P, MODE, F64 ? 2 : 3>(sp, threads, lds, s, batch, items, nitems);
+    else if (D == (F64 ? 4 : 6)) launch_lanes_dd<T, P, MODE, F64 ? 4 : 6>(sp, threads, lds, s, batch, items, nitems);
+    else throw Error(TSW_E_ARG, F64 ? "ordinate-lane prefetch depth must be 2 or 4 in FP64"
+                                    : "ordinate-lane prefetch depth must be 3 or 6 in FP32");
 }
 
 // Entry point per (precision, DG order P); MODE from the params (src, phi, mom), those of `sp` for a batch

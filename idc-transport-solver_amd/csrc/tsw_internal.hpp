@@ -152,6 +152,7 @@ template <typename T> struct LanesParams {
     T dx, dy;
     const LaneBlock *blk;       // [nblk]
     int nblk, J, RW;            // lanes per row group (>= every block's n), row groups per wave (RW * J <= 64)
+    int prefetch;               // steps of inputs in flight per lane (0: default for the precision)
     const double *src;          // bit 1: Q [nx+2][ny+2][nq][N]
     const T *tin_x, *tin_y;     // with src: inflow traces [nq][ny][P], [nq][nx][P] of its halos
     double *psi;                // out unless mom: interior of [nx+2][ny+2][nq][N] (may alias src)

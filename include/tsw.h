@@ -109,7 +109,8 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * block times 64/J rows per wavefront, reading and writing the caller's layout; DG degrees 0-2, material
  * path) runs single-problem sweeps, GMRES and source iteration with 1; for tsw_batch_set, 1 / -1 force /
  * forbid it per DG degree and 0 leaves the choice to the batch autotuner.  "lanes_block" (0 automatic,
- * 1..64): ordinates per block; "lanes_waves" (0 automatic, 1..4): row-group waves per workgroup.  Key
+ * 1..64): ordinates per block; "lanes_waves" (0 automatic, 1..4): row-group waves per workgroup; "lanes_prefetch" (0
+ * default = 2 in FP64, 3 in FP32; or 4 in FP64, 6 in FP32): steps of inputs in flight per lane.  Key
  * "batch_ref" (0 default, 1, -1): tsw_batch_set's staggered groups on the caller's layout (no layout
  * conversions) forced, forbidden, or left to the autotuner, which times, per DG degree, the staggered
  * kernels through conversions, on the caller's layout and the ordinate-lane engine, and keeps the

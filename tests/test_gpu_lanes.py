@@ -105,9 +105,31 @@ def test_lanes_tuning_keys():
     o = ls_ordinates(4)
     g, st, _, src = random_problem(5, 5, 1, o, seed=1)
     with _ctx(g, o, 0) as c:
-        for key, bad in (("lanes", 2), ("lanes_block", 65), ("lanes_waves", 5)):
+        for key, bad in (("lanes", 2), ("lanes_block", 65), ("lanes_waves", 5), ("lanes_prefetch", 5)):
             with pytest.raises(tsw.TswError):
                 c.set_tuning(key, bad)
+
+
+@pytest.mark.parametrize("p", [0, 1, 2])
+def test_lanes_prefetch_depths_give_the_same_bits(p):
+    """The deeper prefetch rings (FP64 4, FP32 6) move when inputs arrive, not what is computed."""
+    o = ls_ordinates(8)
+    g, st, _, src = random_problem(37, 45, p, o, seed=2400 + p, materials=3)
+    with _ctx(g, o, 1) as c:
+        for prec, deep in ((tsw.FP64, 4), (tsw.FP32, 6)):
+            outs = []
+            for pf in (0, deep):
+                c.set_tuning("lanes_prefetch", pf)
+                out = src.copy()
+                c.sweep(out, st, math.inf, precision=prec)
+                assert c.get_tuning("lanes") == 1
+                outs.append(out)
+            np.testing.assert_array_equal(outs[0], outs[1])
+            if prec == tsw.FP32:  # a depth of the other precision is refused at the launch
+                c.set_tuning("lanes_prefetch", 4)
+                with pytest.raises(tsw.TswError):
+                    c.sweep(src.copy(), st, math.inf, precision=prec)
+            c.set_tuning("lanes_prefetch", 0)
 
 
 @pytest.mark.parametrize("dsa", [0, 1])
