@@ -173,6 +173,10 @@ struct tsw_ctx {
     int batch_direct_opt = 0;
     int batch_ref_opt = 0;   // batch engines: 1 / -1 force / forbid the reference-layout staggered groups
     int ref_stage_opt = 0;   // 1: reference-layout sweeps stage their source loads through LDS where it fits
+    // Reflecting edges (tsw_set_boundaries): mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max; the mirror map
+    // [2][nq] (OrdinateSet::ReflectMap) on the device
+    int reflect_mask = 0;
+    DBuf reflect_map_d;
     bool last_batch_ref = false;
     DBuf tune_out;
     // Source iteration replayed from a captured graph of kSiChunk iterations, with the loop test on the
@@ -2168,16 +2172,14 @@ bool ref_stage_layout(std::vector<DiagSweepParams<T>> &ps, int p, bool mat, size
     return false;
 }
 
-// Sweeps of n right-hand sides arr[r] of this problem (reference layout, in place, 16-byte aligned) in
-// one launch of the staggered kernel on the caller's layout: items (source, ordinate), one band per
-// ordinate (ny <= 448), the material table and element inverses shared by every source; counted as n
-// sweeps.  The launch shape follows the batch: stores from the compute lanes when the material table
-// fits beside them, else the storer wave, else per-cell elimination.
+// Launch shape of the staggered kernel on the caller's layout for this problem (one band per ordinate,
+// ny <= 448): stores from the compute lanes when the material table fits beside them, else the storer
+// wave, else per-cell elimination; staged source loads with "ref_stage".  The material table, element
+// inverses and per-cell arrays are set up here.
 template <typename T>
-void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt) {
+DiagSweepParams<T> ref_plan(tsw_ctx *c, const double *sig, double dt, size_t &lds) {
     const Geom &g = c->g;
     const bool f64 = sizeof(T) == 8;
-    const int n = (int)arr.size();
     diag_setup(c, sig, nullptr, nullptr, dt, f64, !f64);
     DiagSweepParams<T> base = make_diag_params<T>(c, dt, nullptr, nullptr, false, nullptr);
     base.nbands = 1;
@@ -2185,7 +2187,7 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
     base.xtr = nullptr;
     base.wave1 = 0;
     base.split = 0;
-    size_t lds = 0;
+    lds = 0;
     if (base.ainv) {
         base.direct = 1;
         lds = diag_sweep_lds_bytes<T>(base, g.p);
@@ -2211,6 +2213,17 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
         base.direct = one[0].direct;
         base.ref_stage = one[0].ref_stage;
     }
+    return base;
+}
+
+// One launch sweeping ordinates [q_lo, q_hi) of each array arr[r] in place with `base` (ref_plan): items
+// (source, ordinate), the ordinates of one source side by side (they share cache lines); the inflow
+// traces of every ordinate come first from the arrays' current halos.
+template <typename T>
+void ref_launch(tsw_ctx *c, const DiagSweepParams<T> &base, size_t lds, const std::vector<double *> &arr, int q_lo,
+                int q_hi) {
+    const Geom &g = c->g;
+    const int n = (int)arr.size(), nqs = q_hi - q_lo;
     const size_t ntx = (size_t)g.nq * g.ny * (g.p + 1), nty = (size_t)g.nq * g.nx * (g.p + 1);
     T *tr = static_cast<T *>(c->multi.traces.reserve(sizeof(T) * (ntx + nty) * n));
     auto &M = c->multi;
@@ -2223,11 +2236,11 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
         ps[r].tin_x = tr + (ntx + nty) * r;
         ps[r].tin_y = tr + (ntx + nty) * r + ntx;
     }
-    M.items.resize((size_t)2 * n * g.nq);
-    for (int r = 0; r < n; ++r)  // the ordinates of one source side by side (they share cache lines)
-        for (int q = 0; q < g.nq; ++q) {
-            M.items[2 * ((size_t)r * g.nq + q)] = r;
-            M.items[2 * ((size_t)r * g.nq + q) + 1] = q;
+    M.items.resize((size_t)2 * n * nqs);
+    for (int r = 0; r < n; ++r)
+        for (int k = 0; k < nqs; ++k) {
+            M.items[2 * ((size_t)r * nqs + k)] = r;
+            M.items[2 * ((size_t)r * nqs + k) + 1] = q_lo + k;
         }
     hipStream_t s = c->stream;
     TSW_HIP(hipMemcpyAsync(M.params.reserve(M.host.size()), M.host.data(), M.host.size(), hipMemcpyHostToDevice, s));
@@ -2240,18 +2253,62 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
     rep.ref_stage = base.ref_stage;
     rep.prefetch = 0;
     const DiagSweepParams<T> *dp = M.params.as<DiagSweepParams<T>>();
-    hipEvent_t e0 = next_event(c), e1 = next_event(c);
-    TSW_HIP(hipEventRecord(e0, s));
     launch_diag_inflow_batch<T>(dp, n, g.p, s);
-    launch_diag_sweep_batch<T>(rep, dp, M.dev_items.as<int2>(), n * g.nq, g.ny, lds, g.p, s);
-    TSW_HIP(hipEventRecord(e1, s));
-    c->sweeps_done += n;
-    TSW_HIP(hipStreamSynchronize(s));  // the host tables may be rebuilt by the next call
+    launch_diag_sweep_batch<T>(rep, dp, M.dev_items.as<int2>(), n * nqs, g.ny, lds, g.p, s);
+    TSW_HIP(hipStreamSynchronize(s));  // the host tables may be rebuilt by the next launch
+}
+
+// Sweeps of n right-hand sides arr[r] of this problem (reference layout, in place, 16-byte aligned) in
+// one launch of the staggered kernel on the caller's layout, the material table and element inverses
+// shared by every source; counted as n sweeps.
+template <typename T>
+void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt) {
+    size_t lds = 0;
+    const DiagSweepParams<T> base = ref_plan<T>(c, sig, dt, lds);
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, c->stream));
+    ref_launch<T>(c, base, lds, arr, 0, c->g.nq);
+    TSW_HIP(hipEventRecord(e1, c->stream));
+    c->sweeps_done += (long long)arr.size();
+}
+
+// Whether this context's sweeps can run the staggered kernel on the caller's layout for `psi`.
+bool ref_sweep_ok(tsw_ctx *c, const double *psi) {
+    return staggered_ok(c) && c->pattern == Pattern::Staggered && c->xs_deg == 0 &&
+           c->g.npsi() * sizeof(double) < (size_t(1) << 32) && c->g.ny <= 448 && ((uintptr_t)psi & 15) == 0;
+}
+
+// SweepOperator::Apply with reflecting edges (SweepPatternUniCartWavefront.cpp:180-245): the quadrants
+// one after the other in the order 3, 2, 1, 0 (SweepPatternUniCart.cpp:33-40), in place, and
+// OrdinateFlux::ReflectBoundaries on the result after each quadrant, so a quadrant takes its reflected
+// inflow from whatever the mirrored ordinates hold at that moment, as the reference does.  One sweep.
+template <typename T>
+void reflect_sweep(tsw_ctx *c, double *psi, const double *sig, double dt) {
+    const Geom &g = c->g;
+    require(ref_sweep_ok(c, psi), TSW_E_UNSUPPORTED,
+            "reflecting boundaries need the hip-staggered pattern, cell-mean cross sections, ny <= 448 and a "
+            "16-byte aligned psi");
+    size_t lds = 0;
+    const DiagSweepParams<T> base = ref_plan<T>(c, sig, dt, lds);
+    const std::vector<double *> one{psi};
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, c->stream));
+    for (int quad : {3, 2, 1, 0}) {
+        if (g.quad_off[quad + 1] > g.quad_off[quad]) ref_launch<T>(c, base, lds, one, g.quad_off[quad], g.quad_off[quad + 1]);
+        launch_reflect_boundaries(g, psi, c->reflect_map_d.as<int>(), c->reflect_mask, c->stream);
+    }
+    TSW_HIP(hipEventRecord(e1, c->stream));
+    ++c->sweeps_done;
 }
 
 void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int precision) {
     require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG, "bad precision");
     ctx->last_lanes = 0;
+    if (ctx->reflect_mask) {  // TSW_MIXED: the FP64 sweep (this engine moves the same FP64 arrays in FP32)
+        if (precision == TSW_FP32) reflect_sweep<float>(ctx, psi, sig, dt);
+        else reflect_sweep<double>(ctx, psi, sig, dt);
+        return;
+    }
     if (lanes_ok(ctx) && precision != TSW_MIXED) {
         if (precision == TSW_FP64) apply_lanes<double>(ctx, psi, sig, dt);
         else apply_lanes<float>(ctx, psi, sig, dt);
@@ -2483,6 +2540,7 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     put32(ctx->xi32, xi); put32(ctx->eta32, eta); put32(ctx->w32, w);
     double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
     TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
+    ctx->reflect_mask = 0;
     ctx->have_problem = true;
 }
 
@@ -2922,9 +2980,8 @@ int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, con
             require(mem == TSW_MEM_DEVICE, TSW_E_ARG, "mem must be TSW_MEM_HOST or TSW_MEM_DEVICE");
             for (int r = 0; r < n_rhs; ++r) dev[r] = psi_inout[r];
         }
-        bool one_launch = staggered_ok(ctx) && ctx->pattern == Pattern::Staggered && ctx->xs_deg == 0 &&
-                          npsi * sizeof(double) < (size_t(1) << 32) && g.ny <= 448;
-        for (int r = 0; r < n_rhs; ++r) one_launch = one_launch && ((uintptr_t)dev[r] & 15) == 0;
+        bool one_launch = ctx->reflect_mask == 0;  // reflecting sweeps run one source after the other
+        for (int r = 0; r < n_rhs; ++r) one_launch = one_launch && ref_sweep_ok(ctx, dev[r]);
         if (one_launch) {
             // TSW_MIXED: the FP64 sweep (this engine reads and writes the FP64 arrays in either precision)
             if (precision == TSW_FP32) multi_sweep<float>(ctx, dev, sig, dt);
@@ -2939,8 +2996,57 @@ int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, con
     });
 }
 
+int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const int nq = ctx->g.nq;
+        require(reflect_mask >= 0 && reflect_mask <= 15, TSW_E_ARG, "reflect_mask must be 0..15");
+        ctx->reflect_mask = 0;
+        if (reflect_mask == 0) return;
+        std::vector<int> map(2 * nq, -1);
+        if (reflect_map) {
+            for (int k = 0; k < 2 * nq; ++k) {
+                require(reflect_map[k] >= 0 && reflect_map[k] < nq, TSW_E_ARG, "reflect_map entry out of range");
+                map[k] = reflect_map[k];
+            }
+        } else {  // ReflectMap (OrdinateSet.cpp:1093-1140) from (xi, eta): the first ordinate within 1e-14
+            const double TOL = 1.0e-14;
+            const auto &xi = ctx->h_xi, &eta = ctx->h_eta;
+            for (int q = 0; q < nq; ++q)
+                for (int k = 0; k < q; ++k)
+                    require(!(std::fabs(xi[q] - xi[k]) < TOL && std::fabs(eta[q] - eta[k]) < TOL), TSW_E_ARG,
+                            "ordinates share (xi, eta): pass reflect_map");
+            for (int dim = 0; dim < 2; ++dim)
+                for (int q = 0; q < nq; ++q) {
+                    for (int k = 0; k < nq && map[dim * nq + q] < 0; ++k) {
+                        const double cx = dim == 0 ? -xi[k] : xi[k], cy = dim == 1 ? -eta[k] : eta[k];
+                        if (std::fabs(xi[q] - cx) < TOL && std::fabs(eta[q] - cy) < TOL) map[dim * nq + q] = k;
+                    }
+                    require(map[dim * nq + q] >= 0, TSW_E_ARG,
+                            "ordinate " + std::to_string(q) + " has no mirror in direction " + std::to_string(dim));
+                }
+        }
+        TSW_HIP(hipMemcpy(ctx->reflect_map_d.reserve(sizeof(int) * map.size()), map.data(), sizeof(int) * map.size(),
+                          hipMemcpyHostToDevice));
+        ctx->reflect_mask = reflect_mask;
+    });
+}
+
+int tsw_reflect_boundaries(tsw_ctx *ctx, double *psi_inout, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
+        if (ctx->reflect_mask) launch_reflect_boundaries(g, psi, ctx->reflect_map_d.as<int>(), ctx->reflect_mask, ctx->stream);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
 int tsw_sweep_apply_noalloc(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision) {
     return guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_sweep_apply_noalloc: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
                 "bad precision");
@@ -3046,6 +3152,8 @@ int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, cons
 int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
                  double abs_tol, int max_its, int precision, int mem, int *iters, double *err) {
     return guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_si_solve: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
@@ -3121,6 +3229,8 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
     double res_final = 0.0;
     bool converged = false;
     const int rc = guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_gmres_solve: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3151,6 +3261,8 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
 int tsw_sweep_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, double dt, const double *seed,
                   double *grad_src, double *grad_sigma_t, int precision, int mem) {
     return guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_sweep_vjp: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");
@@ -3179,6 +3291,8 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
     double res = 0.0;
     bool converged = false;
     const int rc = guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_solve_vjp: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");
@@ -3241,6 +3355,8 @@ int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const do
                  const double *sigma_s, double dt, double abs_tol, int max_its, int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_step: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt), TSW_E_ARG, "IDC needs a finite time step");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3267,6 +3383,8 @@ int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const dou
                 double dt, int nsteps, double abs_tol, int max_its, int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_run: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3297,6 +3415,8 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
                     int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
+        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_adjoint: reflecting boundaries are built for tsw_sweep_apply, "
+                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");

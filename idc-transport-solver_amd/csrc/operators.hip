@@ -573,4 +573,50 @@ void launch_sqdist_interior(const Geom &g, const double *a, const double *b, dou
     TSW_HIP(hipGetLastError());
 }
 
+
+// ---------------------------------------------------------------------------------------------------
+// Reflecting boundaries: OrdinateFlux::ReflectBoundaries (RKDG/OrdinateFlux.cpp:1181-1420).  The ghost
+// cells of edge `edge` (0 x-min, 1 x-max, 2 y-min, 3 y-max) take the mirrored ordinate's values of the
+// adjacent interior cell, the coefficients odd in the reflected direction negated; x-min covers
+// quadrants 0 and 1, x-max 2 and 3, y-min 0 and 2, y-max all four (as the reference does).
+// ---------------------------------------------------------------------------------------------------
+namespace {
+__global__ void k_reflect_edge(double *__restrict__ psi, int nx, int ny, int nq, int P, int qo1, int qo2, int qo3,
+                               const int *__restrict__ map, int edge) {
+    const int N = P * P, dim = edge / 2, len = dim == 0 ? ny : nx;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)nq * len * N;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % N), r = (int)((t / N) % len), q = (int)(t / ((long long)N * len));
+        const int Q = q >= qo3 ? 3 : (q >= qo2 ? 2 : (q >= qo1 ? 1 : 0));
+        const bool on = edge == 0 ? Q <= 1 : (edge == 1 ? Q >= 2 : (edge == 2 ? (Q & 1) == 0 : true));
+        if (!on) continue;
+        const int d = k / P, e = k - d * P;
+        int gi, gj, si, sj;
+        if (dim == 0) {
+            gi = edge == 0 ? 0 : nx + 1;
+            si = edge == 0 ? 1 : nx;
+            gj = sj = r + 1;
+        } else {
+            gj = edge == 2 ? 0 : ny + 1;
+            sj = edge == 2 ? 1 : ny;
+            gi = si = r + 1;
+        }
+        const int m = map[dim * nq + q];
+        const double v = psi[(((size_t)si * (ny + 2) + sj) * nq + m) * N + k];
+        psi[(((size_t)gi * (ny + 2) + gj) * nq + q) * N + k] = ((dim == 0 ? d : e) & 1) ? -v : v;
+    }
+}
+}  // namespace
+
+void launch_reflect_boundaries(const Geom &g, double *psi, const int *map, int mask, hipStream_t s) {
+    for (int edge = 0; edge < 4; ++edge) {
+        if (!(mask & (1 << edge))) continue;
+        const long long n = (long long)g.nq * (edge < 2 ? g.ny : g.nx) * g.N();
+        const int blocks = (int)std::min<long long>((n + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_reflect_edge, dim3(blocks), dim3(256), 0, s, psi, g.nx, g.ny, g.nq, g.p + 1,
+                           g.quad_off[1], g.quad_off[2], g.quad_off[3], map, edge);
+        TSW_HIP(hipGetLastError());
+    }
+}
+
 }  // namespace tsw

@@ -175,6 +175,8 @@ void launch_lanes_sweep_p(const LanesParams<T> &sp, int threads, size_t lds, hip
 // tin_x / tin_y of every member of a batch from its src (device parameter table of nprob members)
 template <typename T, int P>
 void launch_lanes_inflow_batch_p(const LanesParams<T> *batch, int nprob, hipStream_t s);
+// OrdinateFlux::ReflectBoundaries on the edges of `mask` (1 x-min, 2 x-max, 4 y-min, 8 y-max); map [2][nq]
+void launch_reflect_boundaries(const Geom &g, double *psi, const int *map, int mask, hipStream_t s);
 // y = beta xb + alpha sum_b mom[b] on the interior of phi-layout arrays (mom: nblk partials of g.nphi())
 void launch_mom_sum(const Geom &g, int nblk, const double *mom, const double *xb, double *y, double alpha,
                     double beta, hipStream_t s, const int *skip = nullptr);

@@ -28,6 +28,7 @@ TSW_E_UNSUPPORTED = -5
 FP64 = 0
 FP32 = 1
 MIXED = 2  # sweeps: FP32 with FP64 iterative refinement; solvers: as FP32
+REFLECT_X_MIN, REFLECT_X_MAX, REFLECT_Y_MIN, REFLECT_Y_MAX = 1, 2, 4, 8
 MEM_HOST = 0
 MEM_DEVICE = 1
 IDC_LOBATTO = 0
@@ -92,6 +93,8 @@ def lib():
         L.tsw_sweep_apply.argtypes = [c, _vp, _vp, d, i, i]
         L.tsw_sweep_apply_noalloc.argtypes = [c, _vp, _vp, d, i]
         L.tsw_sweep_apply_multi.argtypes = [c, i, ctypes.POINTER(_vp), _vp, d, i, i]
+        L.tsw_set_boundaries.argtypes = [c, i, _vp]
+        L.tsw_reflect_boundaries.argtypes = [c, _vp, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
         L.tsw_smv.argtypes = [c, d, d, _vp, _vp, _vp, i, i]
         L.tsw_lmv.argtypes = [c, d, d, _vp, _vp, _vp, i]
@@ -289,6 +292,21 @@ class Context:
         self._expect(psi=psi, xs=sigma_t)
         (pp, sp), mem = _addrs(psi, sigma_t)
         self._check(self._L.tsw_sweep_apply(self._h, pp, sp, dt, precision, mem))
+        return psi
+
+    def set_boundaries(self, reflect_mask, reflect_map=None):
+        """tsw_set_boundaries: reflecting edges (REFLECT_X_MIN | ...), optional mirror map [2][nq]."""
+        if reflect_map is None:
+            self._check(self._L.tsw_set_boundaries(self._h, int(reflect_mask), None))
+        else:
+            m = np.ascontiguousarray(reflect_map, dtype=np.int32).ravel()
+            self._check(self._L.tsw_set_boundaries(self._h, int(reflect_mask), m.ctypes.data))
+
+    def reflect_boundaries(self, psi):
+        """tsw_reflect_boundaries: the ghost cells of the reflecting edges from psi's interior."""
+        self._expect(psi=psi)
+        (pp,), mem = _addrs(psi)
+        self._check(self._L.tsw_reflect_boundaries(self._h, pp, mem))
         return psi
 
     def sweep_multi(self, psi_list, sigma_t, dt=math.inf, precision=FP64):

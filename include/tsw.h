@@ -160,6 +160,31 @@ int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, doub
 int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, const double *sigma_t, double dt,
                           int precision, int mem);
 
+#define TSW_REFLECT_X_MIN 0x01 /* BoundaryEdge::X_Min (global.hpp:104) */
+#define TSW_REFLECT_X_MAX 0x02 /* BoundaryEdge::X_Max */
+#define TSW_REFLECT_Y_MIN 0x04 /* BoundaryEdge::Y_Min */
+#define TSW_REFLECT_Y_MAX 0x08 /* BoundaryEdge::Y_Max */
+
+/* Reflecting edges of the current problem: the deck keys reflect_x_min / _x_max / _y_min / _y_max that
+ * Init turns into Global::reflecting_boundaries (init.cpp:448-477); 0 restores vacuum/inflow edges
+ * everywhere (tsw_problem_set also does).  reflect_map [2][nq] gives, per direction (0 x, 1 y) and
+ * ordinate, the index of its mirror (OrdinateSet::ReflectMap, OrdinateSet.cpp:1093-1140); null computes it
+ * from xi and eta as the reference does (the first ordinate matching within 1e-14), which fails with
+ * TSW_E_ARG when two ordinates share (xi, eta) (an unreduced set's +-mu pairs) or one has no mirror.
+ * With reflecting edges tsw_sweep_apply sweeps the quadrants one after the other in the order 3, 2, 1, 0
+ * (SweepPatternUniCart.cpp:33-40) and reflects the ghost cells after each one
+ * (SweepPatternUniCartWavefront.cpp:180-245), on the "hip-staggered" pattern with cell-mean cross sections,
+ * ny <= 448 and a 16-byte aligned psi (else TSW_E_UNSUPPORTED); its result, ghost cells included, is that
+ * of the reference's sweep.  The solvers, the IDC integrator and the vector-Jacobian products refuse
+ * reflecting edges with TSW_E_UNSUPPORTED. */
+int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map);
+
+/* OrdinateFlux::ReflectBoundaries (RKDG/OrdinateFlux.cpp:1181-1420) on psi for the edges set by
+ * tsw_set_boundaries: ghost cells of x-min (quadrants 0, 1), x-max (2, 3), y-min (0, 2) and y-max (all
+ * quadrants, as the reference does) take the mirrored ordinate's values of the adjacent interior cell,
+ * coefficients odd in the reflected direction negated.  Nothing happens without reflecting edges. */
+int tsw_reflect_boundaries(tsw_ctx *ctx, double *psi_inout, int mem);
+
 /* tsw_sweep_apply without any device allocation, for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It has no counterpart in the
  * reference.  psi_inout and sigma_t are device arrays (TSW_MEM_DEVICE layouts, cell-mean cross sections);

@@ -281,6 +281,107 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
 }
 
 /* ---------------------------------------------------------------------------------------------- */
+/* Reflecting boundaries.                                                                         */
+/* ---------------------------------------------------------------------------------------------- */
+/* ReflectMap (OrdinateSet.cpp:1093-1140): map[dim * nq + q] = the ordinate k whose coordinate dim is
+ * the negative of q's and whose other coordinates equal q's (to 1e-14), searched from k = 0.  Only xi
+ * and eta are known here, so a set in which two ordinates share (xi, eta) (an unreduced set's +-mu
+ * pairs) is refused: returns -1 (also when an ordinate has no mirror), else 0. */
+int orc_reflect_map(int nq, const double *xi, const double *eta, int *map)
+{
+    const double TOL = 1.0e-14;
+    for (int q = 0; q < nq; ++q)
+        for (int k = 0; k < q; ++k)
+            if (fabs(xi[q] - xi[k]) < TOL && fabs(eta[q] - eta[k]) < TOL) return -1;
+    for (int dim = 0; dim < 2; ++dim)
+        for (int q = 0; q < nq; ++q) {
+            map[dim * nq + q] = -1;
+            for (int k = 0; k < nq; ++k) {
+                const double cx = dim == 0 ? -xi[k] : xi[k], cy = dim == 1 ? -eta[k] : eta[k];
+                if (fabs(xi[q] - cx) < TOL && fabs(eta[q] - cy) < TOL) {
+                    map[dim * nq + q] = k;
+                    break;
+                }
+            }
+            if (map[dim * nq + q] < 0) return -1;
+        }
+    return 0;
+}
+
+/* OrdinateFlux::ReflectBoundaries (RKDG/OrdinateFlux.cpp:1181-1420): the ghost cells of every edge in
+ * `mask` (1 x-min, 2 x-max, 4 y-min, 8 y-max, BoundaryEdge in global.hpp:93-120) take the mirrored
+ * ordinate's values of the adjacent interior cell, the coefficients odd in the reflected direction
+ * negated.  As there, x-min covers quadrants 0 and 1, x-max 2 and 3, y-min 0 and 2, and y-max all four. */
+void orc_reflect_boundaries(int nx, int ny, int p, int nq, const int *quad_off, const int *map, int mask, double *psi)
+{
+    static const int quads[4][4] = {{0, 1, -1, -1}, {2, 3, -1, -1}, {0, 2, -1, -1}, {0, 1, 2, 3}};
+    for (int edge = 0; edge < 4; ++edge) {
+        if (!(mask & (1 << edge))) continue;
+        const int dim = edge / 2;
+        for (int k = 0; k < 4 && quads[edge][k] >= 0; ++k) {
+            const int quad = quads[edge][k];
+            for (int q = quad_off[quad]; q < quad_off[quad + 1]; ++q) {
+                const int m = map[dim * nq + q];
+                const int len = dim == 0 ? ny : nx;
+                for (int t = 1; t <= len; ++t)
+                    for (int d = 0; d <= p; ++d)
+                        for (int e = 0; e <= p; ++e) {
+                            int gi, gj, si, sj;
+                            if (dim == 0) {
+                                gi = edge == 0 ? 0 : nx + 1;
+                                si = edge == 0 ? 1 : nx;
+                                gj = sj = t;
+                            } else {
+                                gj = edge == 2 ? 0 : ny + 1;
+                                sj = edge == 2 ? 1 : ny;
+                                gi = si = t;
+                            }
+                            psi[psi_idx(ny, nq, p, q, gi, gj, d, e)] =
+                                sgn_pow(dim == 0 ? d : e) * psi[psi_idx(ny, nq, p, m, si, sj, d, e)];
+                        }
+            }
+        }
+    }
+}
+
+/* SweepOperator::Apply with reflecting edges (SweepPatternUniCartWavefront.cpp:180-245): the quadrants
+ * one after the other in the order 3, 2, 1, 0 (SweepPatternUniCart.cpp:33-40), each a wavefront over
+ * its diagonals, and ReflectBoundaries on the in-place result after each quadrant.  A quadrant swept
+ * before the mirror of its reflected inflow takes that inflow from the mirror's values at that moment
+ * (the source, or the halo the caller gave), as the reference does. */
+void orc_sweep_reflect(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                       double dx, double dy, const double *sigma_t, double dt, int mask, const int *map, double *psi)
+{
+    const int P = p + 1, N = P * P;
+    double *Aq = (double *)malloc(sizeof(double) * (size_t)nq * N * N);
+    double *A = (double *)malloc(sizeof(double) * N * N);
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *W0 = (double *)malloc(sizeof(double) * N);
+    double *W1 = (double *)malloc(sizeof(double) * N);
+    orc_aq(p, nq, xi, eta, dx, dy, Aq);
+    const double inv_dt = (dt > 0.0 && !isinf(dt)) ? 1.0 / dt : 0.0;
+    double *tensor = g_xs_deg > 0 ? xs_tensor(nx, ny, p, sigma_t) : NULL;
+    static const int order[4] = {3, 2, 1, 0};
+    for (int k = 0; k < 4; ++k) {
+        const int quad = order[k];
+        for (int diag = 1; diag <= nx + ny - 1; ++diag) {
+            const int i_hi = diag < nx ? diag : nx;
+            const int i_lo = diag - ny + 1 > 1 ? diag - ny + 1 : 1;
+            for (int cell = 1; cell <= i_hi - i_lo + 1; ++cell)
+                for (int q = quad_off[quad]; q < quad_off[quad + 1]; ++q) {
+                    int i = i_hi - cell + 1;
+                    int j = diag - i + 1;
+                    if (quad >= 2) i = nx - i + 1;
+                    if (quad & 1) j = ny - j + 1;
+                    cell_solve(nx, ny, p, nq, xi, eta, dx, dy, Aq, sigma_t, tensor, inv_dt, q, i, j, psi, A, B, W0, W1);
+                }
+        }
+        if (mask) orc_reflect_boundaries(nx, ny, p, nq, quad_off, map, mask, psi);
+    }
+    free(Aq); free(A); free(B); free(W0); free(W1); free(tensor);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
 /* The same wavefront with the cells x ordinates of each diagonal shared among OpenMP threads, one   */
 /* barrier per diagonal, as SweepPatternUniCartWavefront::Sweep distributes them with its `omp for`  */
 /* (SweepPatternUniCartWavefront.cpp:146,253-296).  Per-cell arithmetic is that of orc_sweep, so the */

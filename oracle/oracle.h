@@ -59,6 +59,14 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
 
 /* orc_sweep with each diagonal's cells x ordinates shared among `nthreads` OpenMP threads (one barrier
  * per diagonal, SweepPatternUniCartWavefront.cpp:146,253-296); bitwise equal to orc_sweep. */
+/* Reflecting edges (mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max): the mirror map [2][nq] from (xi, eta)
+ * (0, or -1 when an ordinate has none or two ordinates share (xi, eta)); OrdinateFlux::ReflectBoundaries;
+ * SweepOperator::Apply with reflecting edges (quadrants in the order 3, 2, 1, 0, ghosts reflected after
+ * each). */
+int orc_reflect_map(int nq, const double *xi, const double *eta, int *map);
+void orc_reflect_boundaries(int nx, int ny, int p, int nq, const int *quad_off, const int *map, int mask, double *psi);
+void orc_sweep_reflect(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
+                       double dx, double dy, const double *sigma_t, double dt, int mask, const int *map, double *psi);
 void orc_sweep_omp(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
                    double dx, double dy, const double *sigma_t, double dt, double *psi, int nthreads);
 

@@ -64,13 +64,16 @@ extern "C" {
 const char *ref_last_error(void) { return g_err.c_str(); }
 
 /* Reads `deck` (ParameterList::ReadFromFile) and runs Init (init.cpp:422-946), as main_solver.cpp does
- * before integrating.  Replaces the state of an earlier call. */
+ * before integrating.  Replaces the state of an earlier call, boundary conditions included. */
 int ref_init(const char *deck) {
     return guarded([&] {
         delete g;
         g = new State();
         Global::input_list = ParameterList();
         Global::input_list.ReadFromFile(deck);
+        // Init only adds the deck's boundary conditions to these globals (init.cpp:448-477)
+        Global::reflecting_boundaries = BoundaryEdge::None;
+        Global::periodic = false;
         Global::DG_degree = GetDGDegreeX(Global::input_list);
         g->psi = new RKDG::OrdinateFlux();
         g->source = new RKDG::OrdinateFlux();

@@ -62,6 +62,11 @@ def oracle():
         lib.orc_peierls_gmres.argtypes = [i, i, i, i, P, P, P, IP, d, d, P, P, d, d, i, i, P, P, IP, P]
         lib.orc_sweep_adjoint.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, P]
         lib.orc_sweep_omp.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, P, i]
+        lib.orc_reflect_map.argtypes = [i, P, P, IP]
+        lib.orc_reflect_boundaries.argtypes = [i, i, i, i, IP, IP, i, P]
+        lib.orc_reflect_boundaries.restype = None
+        lib.orc_sweep_reflect.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, i, IP, P]
+        lib.orc_sweep_reflect.restype = None
         lib.orc_set_xs_degree.argtypes = [i]
         lib.orc_set_xs_degree.restype = None
         lib.orc_xs_tensor.argtypes = [i, i, i, P, P]
@@ -143,6 +148,30 @@ def orc_sweep(g: Grid, o: Ordinates, sigma_t, psi, dt=math.inf):
     s = np.ascontiguousarray(sigma_t, dtype=np.float64)
     oracle().orc_sweep(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy, ptr(s),
                        dt, ptr(out))
+    return out
+
+
+def orc_reflect_map(o: Ordinates):
+    m = np.zeros(2 * o.nq, dtype=np.int32)
+    if oracle().orc_reflect_map(o.nq, ptr(o.xi), ptr(o.eta), m.ctypes.data_as(_ip)) != 0:
+        raise ValueError("ordinate set has no unique mirror map")
+    return m
+
+
+def orc_reflect_boundaries(g: Grid, o: Ordinates, psi, mask, rmap=None):
+    out = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    m = orc_reflect_map(o) if rmap is None else np.ascontiguousarray(rmap, dtype=np.int32)
+    oracle().orc_reflect_boundaries(g.nx, g.ny, g.p, o.nq, ptr(o.quad_off), m.ctypes.data_as(_ip), mask, ptr(out))
+    return out
+
+
+def orc_sweep_reflect(g: Grid, o: Ordinates, sigma_t, psi, mask, dt=math.inf, rmap=None):
+    """SweepOperator::Apply with reflecting edges `mask` (1 x-min, 2 x-max, 4 y-min, 8 y-max)."""
+    out = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    s = np.ascontiguousarray(sigma_t, dtype=np.float64)
+    m = orc_reflect_map(o) if rmap is None else np.ascontiguousarray(rmap, dtype=np.int32)
+    oracle().orc_sweep_reflect(g.nx, g.ny, g.p, o.nq, ptr(o.xi), ptr(o.eta), ptr(o.quad_off), g.dx, g.dy, ptr(s),
+                               dt, mask, m.ctypes.data_as(_ip), ptr(out))
     return out
 
 

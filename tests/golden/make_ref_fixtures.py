@@ -8,6 +8,9 @@ oracle/build_ref.sh from the unmodified reference sources; run in the container 
 * sweeps: SweepOperator::Apply (sweep_pattern wavefront, sweep_solve gaussian-elimination) on seeded random
   sources with inflow halos, for decks whose cross sections Init projects at degree 0 and at the flux
   degree (piecewise_cross false), steady and with dt;
+* reflecting boundaries: SweepOperator::Apply on decks with reflect_x_min / _x_max / _y_min / _y_max
+  (quadrants in sequence with OrdinateFlux::ReflectBoundaries after each, SweepPatternUniCartWavefront.cpp:
+  180-245), the mask (1 x-min, 2 x-max, 4 y-min, 8 y-max) stored beside them;
 * source iteration: SISolver::Solve on two lattice decks (steady, abs_tol 1e-10 and 1e-12): the angular
   flux, the iteration count and the last change (printed with four decimals);
 * IDC steps: IDCIntegrator::Step on a lattice deck from a seeded initial condition, for the input deck's
@@ -37,6 +40,14 @@ SWEEPS = {  # name: deck parameters, seed
     "sweep_ho": (dict(n=8, m=9, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-10"), 12),
     "sweep_ho3": (dict(n=6, m=5, p=3, pwc="false", otype="spherical-triangle", order=2, tol="1e-10"), 13),
 }
+REFLECT_SWEEPS = {  # name: deck parameters, reflecting-edge mask, seed
+    "reflect_xmin": (dict(n=9, m=7, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-10"), 1, 21),
+    "reflect_xmax_ymin": (dict(n=8, m=10, p=2, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"), 6, 22),
+    "reflect_all": (dict(n=11, m=9, p=1, pwc="true", otype="chebyshev-legendre", order=6, tol="1e-10"), 15, 23),
+    "reflect_ymax_q3": (dict(n=6, m=7, p=3, pwc="true", otype="spherical-triangle", order=1, tol="1e-10"), 8, 24),
+    "reflect_xmin_ymax_ho": (dict(n=7, m=6, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-10"), 9, 25),
+}
+REFLECT_KEYS = ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max")
 IDC_DECK = LATTICE.format(n=10, m=9, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-13") + \
     "cfl 0.5\nidc_type {form}\nidc_nodes_type {nodes}\nidc_num_nodes {nn}\nidc_num_corr {nc}\n" \
     "idc_use_collocation {colloc}\n"
@@ -58,13 +69,30 @@ def main():
     if not _ref.available():
         sys.exit("oracle/_ref/libref.so is missing: run oracle/build_ref.sh")
     out = {}
+    tmp = tempfile.mkdtemp()
+    # Reflecting decks first: the reference builds an ordinate set's mirror map only when the set is first
+    # constructed while reflecting boundaries are on (OrdinateSet.cpp:499-504) and reuses built sets.
+    for name, (params, mask, seed) in REFLECT_SWEEPS.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(LATTICE.format(**params))
+            for b, key in enumerate(REFLECT_KEYS):
+                if mask & (1 << b):
+                    f.write(key + " true\n")
+        d = _ref.Deck(path)
+        rng = np.random.default_rng(seed)
+        src = rng.uniform(-0.5, 1.0, size=d.psi_shape())
+        _store_inputs(out, name, d)
+        out[name + "/mask"] = np.array(mask)
+        out[name + "/src"] = src
+        out[name + "/steady"] = d.sweep(src)
+        out[name + "/dt03"] = d.sweep(src, dt=0.3)
     for kind, orders in (("spherical-triangle", range(1, 9)), ("chebyshev-legendre", (2, 4, 6, 8))):
         for order in orders:
             for red in (0, 1):
                 xi, eta, w, quad = _ref.ordinate_set(kind, order, red)
                 key = f"ords/{kind}/{order}/{red}"
                 out[key + "/xi"], out[key + "/eta"], out[key + "/w"], out[key + "/quad"] = xi, eta, w, quad
-    tmp = tempfile.mkdtemp()
     for name, (params, seed) in SWEEPS.items():
         path = os.path.join(tmp, name + ".deck")
         with open(path, "w") as f:

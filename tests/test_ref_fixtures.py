@@ -22,7 +22,8 @@ import pytest
 
 import tsw
 import _ref
-from _common import REPO, Grid, Ordinates, orc_si, orc_sweep, rel_err, xs_degree
+from _common import (REPO, Grid, Ordinates, orc_reflect_boundaries, orc_reflect_map, orc_si, orc_sweep,
+                     orc_sweep_reflect, rel_err, xs_degree)
 
 FIX = os.path.join(REPO, "tests", "golden", "ref_fixtures.npz")
 
@@ -69,6 +70,48 @@ def test_oracle_sweep_equals_reference(fx, name):
         for key, dt in (("steady", math.inf), ("dt03", 0.3)):
             out = orc_sweep(g, o, st, src, dt)
             assert rel_err(out, fx[name + "/" + key]) <= 1e-14, (name, key)
+
+
+REFLECT = ["reflect_xmin", "reflect_xmax_ymin", "reflect_all", "reflect_ymax_q3", "reflect_xmin_ymax_ho"]
+
+
+@pytest.mark.parametrize("name", REFLECT)
+def test_oracle_reflecting_sweep_equals_reference(fx, name):
+    """Reflecting edges: the quadrants one after the other with the ghosts reflected after each, bit for
+    bit the reference's SweepOperator::Apply (ghost cells included) with cell-mean cross sections."""
+    g, o, sd, st, _ = _case(fx, name)
+    src, mask = fx[name + "/src"], int(fx[name + "/mask"])
+    with xs_degree(sd):
+        for key, dt in (("steady", math.inf), ("dt03", 0.3)):
+            out = orc_sweep_reflect(g, o, st, src, mask, dt)
+            if sd == 0:
+                np.testing.assert_array_equal(out, fx[name + "/" + key], err_msg=f"{name} {key}")
+            else:  # degree-p cross sections: the sweep itself agrees to rounding (test_oracle_sweep_equals_reference)
+                assert rel_err(out, fx[name + "/" + key]) <= 1e-14, (name, key)
+
+
+def test_oracle_reflect_map_and_ghosts_closed_forms(fx):
+    """The mirror map flips one direction cosine and is an involution; reflected ghosts of a field that is
+    itself mirror-symmetric reproduce the field's own halo values."""
+    g, o, _, _, _ = _case(fx, "reflect_all")
+    m = orc_reflect_map(o).reshape(2, o.nq)
+    for dim, (a, b) in enumerate(((o.xi, o.eta), (o.eta, o.xi))):
+        np.testing.assert_allclose(a[m[dim]], -a, rtol=0, atol=1e-14)  # the reference's matching tolerance
+        np.testing.assert_allclose(b[m[dim]], b, rtol=0, atol=1e-14)
+        np.testing.assert_array_equal(m[dim][m[dim]], np.arange(o.nq))
+    # an unreduced set has +-mu pairs with equal (xi, eta): no unique map from (xi, eta)
+    xi, eta, w, off = tsw.ordinates("chebyshev-legendre", 4, False)
+    with pytest.raises(ValueError):
+        orc_reflect_map(Ordinates(xi, eta, w, off))
+    # psi(q, i, j) = f(i) with f(0) = f(1) (x-min mirror plane), coefficient d odd in x zero: x-min ghosts
+    # reflect onto themselves
+    rng = np.random.default_rng(3)
+    psi = np.zeros(g.psi_shape(o.nq))
+    col = rng.uniform(size=(g.ny + 2, o.nq))
+    col = 0.5 * (col + col[:, m[0]])  # symmetric under the x mirror
+    psi[:, :, :, 0, :] = col[None, :, :, None]
+    out = orc_reflect_boundaries(g, o, psi, 1, m.ravel())
+    np.testing.assert_array_equal(out[0, 1:-1], psi[0, 1:-1])
 
 
 @pytest.mark.parametrize("name", ["si_pwc", "si_ho"])
