@@ -3152,8 +3152,6 @@ int tsw_lmv(tsw_ctx *ctx, double alpha, double beta, const double *sigma_t, cons
 int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, const double *sigma_s, double dt,
                  double abs_tol, int max_its, int precision, int mem, int *iters, double *err) {
     return guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_si_solve: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
         need_problem(ctx);
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
         require(precision == TSW_FP64 || precision == TSW_FP32, TSW_E_ARG, "bad precision");
@@ -3162,6 +3160,36 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
+        if (ctx->reflect_mask) {  // SISolver::Solve with reflecting edges (SISolver.cpp:190-254)
+            double *psi = ctx->si_state.as<double>();
+            double *cur = static_cast<double *>(ctx->phi64a.reserve(sizeof(double) * g.nphi()));
+            double *last = static_cast<double *>(ctx->phi64b.reserve(sizeof(double) * g.nphi()));
+            TSW_HIP(hipMemsetAsync(cur, 0, sizeof(double) * g.nphi(), s));
+            TSW_HIP(hipMemsetAsync(last, 0, sizeof(double) * g.nphi(), s));
+            const double *w = ctx->w64.as<double>();
+            const int *map = ctx->reflect_map_d.as<int>();
+            launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);  // :214
+            int it = 0;
+            double e = 1.79769313486231570e+308;
+            while (it < max_its && e > abs_tol) {
+                ++it;
+                std::swap(cur, last);
+                // the Smv of every iteration leaves psi's halos dirty: SynchronizeHalos reflects them (:222)
+                launch_reflect_boundaries(g, psi, map, ctx->reflect_mask, s);
+                launch_copy_outside_reflect(g, Q, psi, ctx->reflect_mask, s);                       // :226
+                launch_smv<double>(g, ss, last, psi, 1.0, 1.0, 0, s, xs_tensor<double>(ctx, ss));  // :229
+                if (precision == TSW_FP64) reflect_sweep<double>(ctx, psi, st, dt);                // :232
+                else reflect_sweep<float>(ctx, psi, st, dt);
+                launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);  // :236
+                e = device_norm2_interior_diff(ctx, cur, last);             // :239
+            }
+            TSW_HIP(hipMemcpyAsync(Q, psi, sizeof(double) * g.npsi(), hipMemcpyDeviceToDevice, s));  // :249
+            if (mem == TSW_MEM_HOST || Q != source_inout) stage_out(ctx, source_inout, Q, g.npsi(), mem);
+            sync(ctx);
+            if (iters) *iters = it;
+            if (err) *err = e;
+            return;
+        }
         bool lanes = false;
         if (lanes_ok(ctx)) {
             prepare_materials(ctx, st, dt);

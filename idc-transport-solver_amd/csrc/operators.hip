@@ -608,6 +608,30 @@ __global__ void k_reflect_edge(double *__restrict__ psi, int nx, int ny, int nq,
 }
 }  // namespace
 
+namespace {
+__global__ void k_copy_outside_reflect(const double *__restrict__ src, double *__restrict__ dst, int nx, int ny,
+                                       size_t per, int mask) {
+    const size_t n = (size_t)(nx + 2) * (ny + 2) * per;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t cell = t / per;
+        const int i = (int)(cell / (ny + 2)), j = (int)(cell % (ny + 2));
+        const bool skip = (i == 0 && (mask & 1)) || (i == nx + 1 && (mask & 2)) || (j == 0 && (mask & 4)) ||
+                          (j == ny + 1 && (mask & 8));
+        if (!skip) dst[t] = src[t];
+    }
+}
+}  // namespace
+
+// dst <- src on the interior and the ghost cells of the edges that do not reflect (AngularFlux::Copy over
+// OpDomain::Interior | OpDomain::Boundary, whose edge rule is DomainDecomposition.cpp:285-336).
+void launch_copy_outside_reflect(const Geom &g, const double *src, double *dst, int mask, hipStream_t s) {
+    const size_t n = g.npsi();
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_copy_outside_reflect, dim3(blocks), dim3(256), 0, s, src, dst, g.nx, g.ny,
+                       (size_t)g.nq * g.N(), mask);
+    TSW_HIP(hipGetLastError());
+}
+
 void launch_reflect_boundaries(const Geom &g, double *psi, const int *map, int mask, hipStream_t s) {
     for (int edge = 0; edge < 4; ++edge) {
         if (!(mask & (1 << edge))) continue;

@@ -175,8 +175,10 @@ int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, con
  * (SweepPatternUniCart.cpp:33-40) and reflects the ghost cells after each one
  * (SweepPatternUniCartWavefront.cpp:180-245), on the "hip-staggered" pattern with cell-mean cross sections,
  * ny <= 448 and a 16-byte aligned psi (else TSW_E_UNSUPPORTED); its result, ghost cells included, is that
- * of the reference's sweep.  The solvers, the IDC integrator and the vector-Jacobian products refuse
- * reflecting edges with TSW_E_UNSUPPORTED. */
+ * of the reference's sweep.  tsw_si_solve then runs SISolver::Solve's loop with the ghost cells of the
+ * iterate reflected before each source copy (SynchronizeHalos, SISolver.cpp:222) and the source copied
+ * onto the interior and the other edges' ghost cells (OpDomain::Interior | OpDomain::Boundary).  GMRES,
+ * the IDC integrator and the vector-Jacobian products refuse reflecting edges with TSW_E_UNSUPPORTED. */
 int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map);
 
 /* OrdinateFlux::ReflectBoundaries (RKDG/OrdinateFlux.cpp:1181-1420) on psi for the edges set by

@@ -11,6 +11,7 @@ oracle/build_ref.sh from the unmodified reference sources; run in the container 
 * reflecting boundaries: SweepOperator::Apply on decks with reflect_x_min / _x_max / _y_min / _y_max
   (quadrants in sequence with OrdinateFlux::ReflectBoundaries after each, SweepPatternUniCartWavefront.cpp:
   180-245), the mask (1 x-min, 2 x-max, 4 y-min, 8 y-max) stored beside them;
+* source iteration with reflecting edges: SISolver::Solve on a lattice deck reflecting on three edges;
 * source iteration: SISolver::Solve on two lattice decks (steady, abs_tol 1e-10 and 1e-12): the angular
   flux, the iteration count and the last change (printed with four decimals);
 * IDC steps: IDCIntegrator::Step on a lattice deck from a seeded initial condition, for the input deck's
@@ -48,6 +49,9 @@ REFLECT_SWEEPS = {  # name: deck parameters, reflecting-edge mask, seed
     "reflect_xmin_ymax_ho": (dict(n=7, m=6, p=2, pwc="false", otype="chebyshev-legendre", order=4, tol="1e-10"), 9, 25),
 }
 REFLECT_KEYS = ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max")
+REFLECT_SOLVES = {  # source iteration with reflecting edges: deck parameters, mask
+    "si_reflect": (dict(n=12, m=10, p=1, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"), 13),
+}
 IDC_DECK = LATTICE.format(n=10, m=9, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-13") + \
     "cfl 0.5\nidc_type {form}\nidc_nodes_type {nodes}\nidc_num_nodes {nn}\nidc_num_corr {nc}\n" \
     "idc_use_collocation {colloc}\n"
@@ -87,6 +91,24 @@ def main():
         out[name + "/src"] = src
         out[name + "/steady"] = d.sweep(src)
         out[name + "/dt03"] = d.sweep(src, dt=0.3)
+    for name, (params, mask) in REFLECT_SOLVES.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(LATTICE.format(**params))
+            for b, key in enumerate(REFLECT_KEYS):
+                if mask & (1 << b):
+                    f.write(key + " true\n")
+        d = _ref.Deck(path)
+        _store_inputs(out, name, d)
+        src = d.get_flux(1)
+        out[name + "/mask"] = np.array(mask)
+        out[name + "/src"] = src
+        out[name + "/tol"] = np.array(float(params["tol"]))
+        psi, its, err = d.solve(src, os.path.join(tmp, name + ".log"))
+        out[name + "/psi"] = psi
+        out[name + "/its"] = np.array(its)
+        out[name + "/err"] = np.array(err)
+        print(name, "iterations", its, "last change", err)
     for kind, orders in (("spherical-triangle", range(1, 9)), ("chebyshev-legendre", (2, 4, 6, 8))):
         for order in orders:
             for red in (0, 1):

@@ -1,8 +1,9 @@
 """Reflecting boundaries (tsw_set_boundaries, tsw_reflect_boundaries): the sweep with reflecting edges
 against the reference's own SweepOperator::Apply on reflecting decks (tests/golden/ref_fixtures.npz) and
 against the oracle's restatement on random problems (every edge combination, DG degrees 0-3, FP64 to 1e-12,
-FP32 to 2e-6, TSW_MIXED = the FP64 sweep), ghost cells included; the ghost update alone bit for bit; the
-configurations refused."""
+FP32 to 2e-6, TSW_MIXED = the FP64 sweep), ghost cells included; source iteration with reflecting edges
+against the reference's SISolver (its iteration count, its angular flux) and the oracle; the ghost update
+alone bit for bit; the configurations refused."""
 import math
 import os
 
@@ -10,7 +11,8 @@ import numpy as np
 import pytest
 
 import tsw
-from _common import (REPO, ls_ordinates, orc_reflect_boundaries, orc_sweep_reflect, random_problem, rel_err)
+from _common import (REPO, ls_ordinates, orc_reflect_boundaries, orc_si_reflect, orc_sweep_reflect, random_problem,
+                     rel_err)
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(REPO, "tests", "golden", "ref_fixtures.npz")
@@ -120,13 +122,11 @@ def test_reflecting_configurations_refused():
         with pytest.raises(tsw.TswError):
             c.set_boundaries(16)
         c.set_boundaries(15)
-        with pytest.raises(tsw.TswError):  # the solvers are not built for reflecting edges
-            c.si_solve(src.copy(), st, ss)
-        with pytest.raises(tsw.TswError):
+        with pytest.raises(tsw.TswError):  # GMRES is not built for reflecting edges
             c.gmres_solve(src.copy(), st, ss)
         # tsw_problem_set restores inflow edges
         c.problem_set(g.nx, g.ny, g.p, o.xi, o.eta, o.w, o.quad_off, g.ax, g.bx, g.ay, g.by)
-        c.si_solve(src.copy(), st, ss)
+        c.gmres_solve(src.copy(), st, ss)
     # an unreduced set has ordinates sharing (xi, eta): the map must come from the caller
     xi, eta, w, off = tsw.ordinates("chebyshev-legendre", 4, False)
     with tsw.Context(0) as c:
@@ -143,3 +143,42 @@ def test_reflecting_configurations_refused():
         c.set_boundaries(4)
         with pytest.raises(tsw.TswError):
             c.sweep(src.copy(), st, math.inf)
+
+
+def test_reflecting_source_iteration_matches_reference(fx):
+    name = "si_reflect"
+    nx, ny, p, nq, sd = (int(v) for v in fx[name + "/dims"])
+    ax, bx, ay, by = (float(v) for v in fx[name + "/box"])
+    st = fx[name + "/sigma_t"][..., 0, 0].copy()
+    ss = fx[name + "/sigma_s"][..., 0, 0].copy()
+    print("reference reflecting SI fixture", flush=True)
+    with tsw.Context(0) as c:
+        c.problem_set(nx, ny, p, fx[name + "/xi"], fx[name + "/eta"], fx[name + "/w"], fx[name + "/quad"], ax, bx, ay, by)
+        c.set_pattern("hip-staggered")
+        c.set_boundaries(int(fx[name + "/mask"]))
+        out = fx[name + "/src"].copy()
+        its, err = c.si_solve(out, st, ss, abs_tol=float(fx[name + "/tol"]), max_its=5000)
+        assert its == int(fx[name + "/its"])
+        assert rel_err(out, fx[name + "/psi"]) <= 1e-11
+
+
+@pytest.mark.parametrize("mask", [2, 12, 15])
+def test_reflecting_source_iteration_matches_oracle(mask):
+    o = ls_ordinates(6)
+    g, st, ss, src = random_problem(14, 12, 2, o, seed=4300 + mask, materials=3)
+    ref, rits, _ = orc_si_reflect(g, o, st, ss, src, mask, abs_tol=1e-11, max_its=2000)
+    with _ctx(g, o) as c:
+        c.set_boundaries(mask)
+        out = src.copy()
+        its, _ = c.si_solve(out, st, ss, abs_tol=1e-11, max_its=2000)
+        assert its == rits
+        assert rel_err(out, ref) <= 1e-10
+        # the solver state persists: a second solve starts from it, as SISolver::psi_ptr does
+        ref2, rits2, _ = orc_si_reflect(g, o, st, ss, src, mask, abs_tol=1e-11, max_its=2000, psi0=ref)
+        out2 = src.copy()
+        its2, _ = c.si_solve(out2, st, ss, abs_tol=1e-11, max_its=2000)
+        assert its2 == rits2
+        assert rel_err(out2, ref2) <= 1e-10
+        out32 = src.copy()
+        c.si_solve(out32, st, ss, abs_tol=1e-11, max_its=2000, precision=tsw.FP32)
+        assert rel_err(out32, ref) <= 1e-5

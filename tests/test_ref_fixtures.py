@@ -23,7 +23,7 @@ import pytest
 import tsw
 import _ref
 from _common import (REPO, Grid, Ordinates, orc_reflect_boundaries, orc_reflect_map, orc_si, orc_sweep,
-                     orc_sweep_reflect, rel_err, xs_degree)
+                     orc_si_reflect, orc_sweep_reflect, rel_err, xs_degree)
 
 FIX = os.path.join(REPO, "tests", "golden", "ref_fixtures.npz")
 
@@ -88,6 +88,18 @@ def test_oracle_reflecting_sweep_equals_reference(fx, name):
                 np.testing.assert_array_equal(out, fx[name + "/" + key], err_msg=f"{name} {key}")
             else:  # degree-p cross sections: the sweep itself agrees to rounding (test_oracle_sweep_equals_reference)
                 assert rel_err(out, fx[name + "/" + key]) <= 1e-14, (name, key)
+
+
+def test_oracle_reflecting_source_iteration_equals_reference(fx):
+    """SISolver::Solve on a lattice deck reflecting on three edges: the reference's iteration count and
+    last change, its angular flux bit for bit."""
+    name = "si_reflect"
+    g, o, _, st, ss = _case(fx, name)
+    psi, its, err = orc_si_reflect(g, o, st, ss, fx[name + "/src"], int(fx[name + "/mask"]),
+                                   abs_tol=float(fx[name + "/tol"]), max_its=5000)
+    assert its == int(fx[name + "/its"])
+    assert float(f"{err:.4e}") == float(fx[name + "/err"])
+    np.testing.assert_array_equal(psi, fx[name + "/psi"])
 
 
 def test_oracle_reflect_map_and_ghosts_closed_forms(fx):
