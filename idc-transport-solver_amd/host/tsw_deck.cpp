@@ -342,12 +342,14 @@ ProblemInputs Init(ParameterList &params, const std::string &directory) {
     prob->OverrideOptions(params);
     if (optional_bool(params, "periodic", false))
         throw std::invalid_argument("periodic boundaries are not available (the sweeps take inflow halos)");
-    for (const char *k : {"reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"})
-        if (optional_bool(params, k, false))
-            throw std::invalid_argument(std::string("reflecting boundaries are not available: ") + k);
+    int reflect_mask = 0;  // init.cpp:448-477
+    const char *reflect_keys[4] = {"reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"};
+    for (int b = 0; b < 4; ++b)
+        if (optional_bool(params, reflect_keys[b], false)) reflect_mask |= 1 << b;
     const bool pwc_cross_sections = optional_bool(params, "piecewise_cross", false);
 
     DomainDecomposition dd;
+    dd.reflect_mask = reflect_mask;
     dd.nx = params.GetInt("nx");
     dd.ax = params.GetDouble("ax");
     dd.bx = params.GetDouble("bx");

@@ -104,6 +104,7 @@ Device::Device(const DomainDecomposition &dd, const OrdinateSet &os, int degree,
         Check(tsw_problem_set(ctx_, (int)dd.nx, (int)dd.ny, dd.ax, dd.bx, dd.ay, dd.by, degree, (int)os.nq(),
                               os.xi().data(), os.eta().data(), os.w().data(), off.data()),
               "tsw_problem_set");
+        if (dd.reflect_mask) Check(tsw_set_boundaries(ctx_, dd.reflect_mask, nullptr), "tsw_set_boundaries");
     } catch (...) {
         tsw_destroy(ctx_);
         throw;

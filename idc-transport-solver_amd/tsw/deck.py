@@ -5,8 +5,9 @@ condition projected into the DG space, inflow halos, and the named ordinate set.
 Problems: "lattice" (LatticeProblem.cpp:100-324), "hohlraum" (HohlraumProblem.cpp:60-272),
 "hohlraum2" (Hohlraum2Problem.cpp:14-148), "bump" (BumpProblem.cpp) and "inhomogeneous-sphere"
 (InhomogeneousSphereProblem.cpp).  Cross sections are projected at DG degree 0 when piecewise_cross is
-true and at the flux's degree otherwise (init.cpp:741); periodic and reflecting boundaries and mollified
-problem functions are not available (ValueError)."""
+true and at the flux's degree otherwise (init.cpp:741); the reflect_* keys give the problem's reflecting
+edges (init.cpp:448-477); periodic boundaries and mollified problem functions are not available
+(ValueError)."""
 from __future__ import annotations
 
 import math
@@ -252,9 +253,10 @@ def init(params):
                 params[k] = "false"
     if _bool(params, "periodic", False):
         raise ValueError("periodic boundaries are not available (the sweeps take inflow halos)")
-    for k in ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"):
+    reflect_mask = 0  # init.cpp:448-477
+    for b, k in enumerate(("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max")):
         if _bool(params, k, False):
-            raise ValueError(f"reflecting boundaries are not available: {k}")
+            reflect_mask |= 1 << b
     sdeg_pwc = _bool(params, "piecewise_cross", False)
     mesh = Mesh(int(params["nx"]), int(params["ny"]), float(params["ax"]), float(params["bx"]),
                 float(params["ay"]), float(params["by"]))
@@ -275,7 +277,7 @@ def init(params):
     prob.inflow(q)
     P = Problem(name, mesh.nx, mesh.ny, p, mesh.ax, mesh.bx, mesh.ay, mesh.by, xi, eta, w, off,
                 st[:, :, 0, 0].copy(), ss[:, :, 0, 0].copy(), q[:, :, 0, 0].copy(), q_dg=q,
-                sigma_t_dg=st if sdeg else None, sigma_s_dg=ss if sdeg else None)
+                sigma_t_dg=st if sdeg else None, sigma_s_dg=ss if sdeg else None, reflect_mask=reflect_mask)
     psi0 = np.zeros((mesh.nx + 2, mesh.ny + 2, xi.size, p + 1, p + 1))
     psi0[1:-1, 1:-1] = ic[1:-1, 1:-1, None] / (4.0 * math.pi)
     nxs = f"{mesh.nx:07d}"

@@ -37,6 +37,7 @@ class Problem:
     # optional degree-p cross sections [nx+2][ny+2][p+1][p+1] (sigma_t / sigma_s hold their cell means)
     sigma_t_dg: np.ndarray = None
     sigma_s_dg: np.ndarray = None
+    reflect_mask: int = 0  # reflecting edges (REFLECT_X_MIN | ...), from the deck keys reflect_*
 
     @property
     def xs_degree(self):
@@ -84,6 +85,8 @@ class Problem:
                         self.ay, self.by)
         if self.xs_degree:
             ctx.set_xs_degree(self.xs_degree)
+        if self.reflect_mask:
+            ctx.set_boundaries(self.reflect_mask)
 
 
 def _centres(n, a, b):

@@ -401,7 +401,13 @@ static void test_decks(const std::string &dir) {
               "piecewise_cross false: degree-1 cross sections with higher moments");
     }
     throws(base + "piecewise_cross true\nperiodic true\n", "periodic boundaries are refused");
-    throws(base + "piecewise_cross true\nreflect_y_min true\n", "reflecting boundaries are refused");
+    {  // reflecting edges: the mask Init derives from the reflect_* keys (init.cpp:448-477)
+        write_text(dir + "/r.deck", base + "piecewise_cross true\nreflect_y_min true\nreflect_x_max true\n");
+        ParameterList r;
+        r.ReadFromFile(dir + "/r.deck");
+        ProblemInputs in = Init(r);
+        check(in.dd.reflect_mask == (TSW_REFLECT_Y_MIN | TSW_REFLECT_X_MAX), "reflect_* keys give the edge mask");
+    }
     throws("nx 4\nny 4\nax 0\nbx 7\nay 0\nby 7\ndg_degree 1\nproblem reeds\nmoll_radius 0\nang_order 2\npiecewise_cross true\n",
            "unknown problem is refused");
     throws(base + "piecewise_cross true\nordinate_type lebedev\n", "unavailable ordinate type is refused");

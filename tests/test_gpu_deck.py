@@ -10,7 +10,7 @@ import pytest
 
 import tsw
 from tsw import deck, fields
-from _common import REPO, Grid, Ordinates, OracleIdcOps, orc_gmres, orc_pmv, rel_err
+from _common import REPO, Grid, Ordinates, OracleIdcOps, orc_gmres, orc_pmv, orc_si_reflect, rel_err
 
 pytestmark = pytest.mark.gpu
 PKG = os.path.join(REPO, "idc-transport-solver_amd")
@@ -73,6 +73,34 @@ def test_tsw_solve_steady_deck_matches_oracle(tmp_path):
     assert c.returncode == 0, c.stderr
     l1, l2, li = (float(v) for v in c.stdout.splitlines()[1].split()[1:])
     assert l2 <= 1e-9 * fields.l2_norm(phi_ref, m2) and li <= 1e-8 * fields.linf_norm(phi_ref, m2)
+
+
+def test_reflecting_si_deck_matches_oracle(tmp_path):
+    """A lattice deck with reflect_x_min and reflect_y_max solved by source iteration, through the Python
+    deck path and through tsw_solve, against the oracle's reflecting source iteration on the same inputs."""
+    text = LATTICE.replace("nx 28", "nx 16").replace("ny 28", "ny 14").replace(
+        "solve_type petsc-peierls-gmres", "solve_type source-iteration").replace("sweep_pattern wavefront",
+                                                                                 "sweep_pattern wavefront") + \
+        "reflect_x_min true\nreflect_y_max true\noutput_af true\n"
+    d = tmp_path / "refl.deck"
+    d.write_text(text)
+    P, _ = deck.init(deck.read_deck(str(d)))
+    assert P.reflect_mask == tsw.REFLECT_X_MIN | tsw.REFLECT_Y_MAX
+    ref, its, _ = orc_si_reflect(_grid(P), _ords(P), P.sigma_t, P.sigma_s, P.source, P.reflect_mask,
+                                 abs_tol=1e-12, max_its=3000)
+    print("reflecting SI deck", P.nx, P.ny, P.p, P.nq, "oracle iterations", its, flush=True)
+    with tsw.Context(0) as c:
+        P.set_on(c)
+        c.set_pattern("hip-staggered")
+        out = P.source.copy()
+        gits, _ = c.si_solve(out, P.sigma_t, P.sigma_s, abs_tol=1e-12, max_its=3000)
+    assert gits == its
+    assert rel_err(out, ref) <= 1e-10
+    r = _run_solve(d, tmp_path)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    psi, _, _ = fields.read_ordinate_flux(str(tmp_path / "refl.af2"))
+    assert rel_err(psi, ref) <= 1e-10
 
 
 # the IDC lines of the reference's input.deck (idc_type update, gauss-radau, 2 nodes, 2 corrections)
