@@ -176,6 +176,7 @@ struct tsw_ctx {
     // Reflecting edges (tsw_set_boundaries): mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max; the mirror map
     // [2][nq] (OrdinateSet::ReflectMap) on the device
     int reflect_mask = 0;
+    bool periodic = false;  // periodic boundaries (the deck key periodic, Global::periodic)
     DBuf reflect_map_d;
     bool last_batch_ref = false;
     DBuf tune_out;
@@ -2541,6 +2542,7 @@ void set_problem(tsw_ctx *ctx, int nx, int ny, double ax, double bx, double ay, 
     double *st = static_cast<double *>(ctx->si_state.reserve(sizeof(double) * g.npsi()));
     TSW_HIP(hipMemset(st, 0, sizeof(double) * g.npsi()));
     ctx->reflect_mask = 0;
+    ctx->periodic = false;
     ctx->have_problem = true;
 }
 
@@ -3000,8 +3002,13 @@ int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map) {
     return guarded(ctx, [&] {
         need_problem(ctx);
         const int nq = ctx->g.nq;
-        require(reflect_mask >= 0 && reflect_mask <= 15, TSW_E_ARG, "reflect_mask must be 0..15");
+        require(reflect_mask >= 0 && (reflect_mask & ~(15 | TSW_PERIODIC)) == 0, TSW_E_ARG,
+                "reflect_mask takes TSW_REFLECT_* edges or TSW_PERIODIC");
+        require(!((reflect_mask & TSW_PERIODIC) && (reflect_mask & 15)), TSW_E_ARG,
+                "combined reflecting and periodic boundary conditions are not supported (init.cpp:498-510)");
         ctx->reflect_mask = 0;
+        ctx->periodic = (reflect_mask & TSW_PERIODIC) != 0;
+        reflect_mask &= 15;
         if (reflect_mask == 0) return;
         std::vector<int> map(2 * nq, -1);
         if (reflect_map) {
@@ -3032,6 +3039,18 @@ int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map) {
     });
 }
 
+int tsw_synchronize_halos(tsw_ctx *ctx, double *psi_inout, int mem) {
+    return guarded(ctx, [&] {
+        need_problem(ctx);
+        const Geom &g = ctx->g;
+        double *psi = stage_in(ctx, psi_inout, g.npsi(), mem, ctx->psi64);
+        if (ctx->reflect_mask) launch_reflect_boundaries(g, psi, ctx->reflect_map_d.as<int>(), ctx->reflect_mask, ctx->stream);
+        if (ctx->periodic) launch_periodic_halos(g, psi, ctx->stream);
+        if (mem == TSW_MEM_HOST) stage_out(ctx, psi_inout, psi, g.npsi(), mem);
+        sync(ctx);
+    });
+}
+
 int tsw_reflect_boundaries(tsw_ctx *ctx, double *psi_inout, int mem) {
     return guarded(ctx, [&] {
         need_problem(ctx);
@@ -3045,8 +3064,8 @@ int tsw_reflect_boundaries(tsw_ctx *ctx, double *psi_inout, int mem) {
 
 int tsw_sweep_apply_noalloc(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision) {
     return guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_sweep_apply_noalloc: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_sweep_apply_noalloc: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(precision == TSW_FP64 || precision == TSW_FP32 || precision == TSW_MIXED, TSW_E_ARG,
                 "bad precision");
@@ -3160,7 +3179,7 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
         double *Q = stage_in(ctx, source_inout, g.npsi(), mem, ctx->q64);
         const double *st = stage_in(ctx, sigma_t, xs_n(ctx), mem, ctx->sig_t64);
         const double *ss = stage_in(ctx, sigma_s, xs_n(ctx), mem, ctx->sig_s64);
-        if (ctx->reflect_mask) {  // SISolver::Solve with reflecting edges (SISolver.cpp:190-254)
+        if (ctx->reflect_mask || ctx->periodic) {  // SISolver::Solve, reflecting or periodic (SISolver.cpp:190-254)
             double *psi = ctx->si_state.as<double>();
             double *cur = static_cast<double *>(ctx->phi64a.reserve(sizeof(double) * g.nphi()));
             double *last = static_cast<double *>(ctx->phi64b.reserve(sizeof(double) * g.nphi()));
@@ -3174,12 +3193,13 @@ int tsw_si_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, cons
             while (it < max_its && e > abs_tol) {
                 ++it;
                 std::swap(cur, last);
-                // the Smv of every iteration leaves psi's halos dirty: SynchronizeHalos reflects them (:222)
-                launch_reflect_boundaries(g, psi, map, ctx->reflect_mask, s);
-                launch_copy_outside_reflect(g, Q, psi, ctx->reflect_mask, s);                       // :226
+                // the Smv of every iteration leaves psi's halos dirty: SynchronizeHalos reflects them, or
+                // copies them across the periodic boundaries (:222); the source copy then skips those halos
+                if (ctx->reflect_mask) launch_reflect_boundaries(g, psi, map, ctx->reflect_mask, s);
+                if (ctx->periodic) launch_periodic_halos(g, psi, s);
+                launch_copy_outside_reflect(g, Q, psi, ctx->periodic ? 15 : ctx->reflect_mask, s);  // :226
                 launch_smv<double>(g, ss, last, psi, 1.0, 1.0, 0, s, xs_tensor<double>(ctx, ss));  // :229
-                if (precision == TSW_FP64) reflect_sweep<double>(ctx, psi, st, dt);                // :232
-                else reflect_sweep<float>(ctx, psi, st, dt);
+                sweep_device(ctx, psi, st, dt, precision);                                          // :232
                 launch_pmv<double, double>(g, psi, cur, w, 1.0, 0.0, 0, s);  // :236
                 e = device_norm2_interior_diff(ctx, cur, last);             // :239
             }
@@ -3257,8 +3277,8 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
     double res_final = 0.0;
     bool converged = false;
     const int rc = guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_gmres_solve: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_gmres_solve: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(restart >= 1 && max_its >= 0, TSW_E_ARG, "bad GMRES parameters");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3289,8 +3309,8 @@ int tsw_gmres_solve(tsw_ctx *ctx, double *source_inout, const double *sigma_t, c
 int tsw_sweep_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, double dt, const double *seed,
                   double *grad_src, double *grad_sigma_t, int precision, int mem) {
     return guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_sweep_vjp: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_sweep_vjp: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");
@@ -3319,8 +3339,8 @@ int tsw_solve_vjp(tsw_ctx *ctx, const double *psi, const double *sigma_t, const 
     double res = 0.0;
     bool converged = false;
     const int rc = guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_solve_vjp: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_solve_vjp: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");
@@ -3383,8 +3403,8 @@ int tsw_idc_step(tsw_ctx *ctx, double *psi_inout, const double *source, const do
                  const double *sigma_s, double dt, double abs_tol, int max_its, int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_step: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_idc_step: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt), TSW_E_ARG, "IDC needs a finite time step");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3411,8 +3431,8 @@ int tsw_idc_run(tsw_ctx *ctx, double *psi_inout, const double *source, const dou
                 double dt, int nsteps, double abs_tol, int max_its, int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_run: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_idc_run: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(dt > 0.0 && std::isfinite(dt) && nsteps >= 0, TSW_E_ARG, "IDC needs a finite time step and nsteps >= 0");
         if (precision == TSW_MIXED) precision = TSW_FP32;  // solvers refine their FP32 sweeps themselves
@@ -3443,8 +3463,8 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
                     int precision, int mem) {
     bool ok = true;
     const int rc = guarded(ctx, [&] {
-        require(ctx->reflect_mask == 0, TSW_E_UNSUPPORTED, "tsw_idc_adjoint: reflecting boundaries are built for tsw_sweep_apply, "
-                "tsw_sweep_apply_multi and tsw_reflect_boundaries");
+        require(ctx->reflect_mask == 0 && !ctx->periodic, TSW_E_UNSUPPORTED, "tsw_idc_adjoint: reflecting and periodic boundaries are built for the "
+                "sweeps, tsw_si_solve and the halo updates");
         need_problem(ctx);
         require(ctx->xs_deg == 0, TSW_E_UNSUPPORTED, "vector-Jacobian products take cell-mean cross sections "
                                                      "(tsw_set_xs_degree 0)");

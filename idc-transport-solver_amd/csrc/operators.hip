@@ -632,6 +632,36 @@ void launch_copy_outside_reflect(const Geom &g, const double *src, double *dst, 
     TSW_HIP(hipGetLastError());
 }
 
+namespace {
+// DensityFunction::SynchronizeHalos across periodic boundaries (Abstract/DensityFunction.cpp:927-956):
+// phase 0 the y-halos of columns 1..nx, phase 1 the x-halos of rows 1..ny, every value of the cell.
+__global__ void k_periodic_halos(double *__restrict__ psi, int nx, int ny, size_t per, int phase) {
+    const int len = phase == 0 ? nx : ny;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)len * per;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t / per) + 1;
+        const size_t l = t % per;
+        if (phase == 0) {
+            psi[((size_t)k * (ny + 2) + 0) * per + l] = psi[((size_t)k * (ny + 2) + ny) * per + l];
+            psi[((size_t)k * (ny + 2) + ny + 1) * per + l] = psi[((size_t)k * (ny + 2) + 1) * per + l];
+        } else {
+            psi[((size_t)0 * (ny + 2) + k) * per + l] = psi[((size_t)nx * (ny + 2) + k) * per + l];
+            psi[((size_t)(nx + 1) * (ny + 2) + k) * per + l] = psi[((size_t)1 * (ny + 2) + k) * per + l];
+        }
+    }
+}
+}  // namespace
+
+void launch_periodic_halos(const Geom &g, double *psi, hipStream_t s) {
+    const size_t per = (size_t)g.nq * g.N();
+    for (int phase = 0; phase < 2; ++phase) {
+        const size_t n = (size_t)(phase == 0 ? g.nx : g.ny) * per;
+        const int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_periodic_halos, dim3(blocks), dim3(256), 0, s, psi, g.nx, g.ny, per, phase);
+        TSW_HIP(hipGetLastError());
+    }
+}
+
 void launch_reflect_boundaries(const Geom &g, double *psi, const int *map, int mask, hipStream_t s) {
     for (int edge = 0; edge < 4; ++edge) {
         if (!(mask & (1 << edge))) continue;

@@ -177,6 +177,8 @@ template <typename T, int P>
 void launch_lanes_inflow_batch_p(const LanesParams<T> *batch, int nprob, hipStream_t s);
 // OrdinateFlux::ReflectBoundaries on the edges of `mask` (1 x-min, 2 x-max, 4 y-min, 8 y-max); map [2][nq]
 void launch_reflect_boundaries(const Geom &g, double *psi, const int *map, int mask, hipStream_t s);
+// DensityFunction::SynchronizeHalos across periodic boundaries (the x- and y-halos of rows and columns 1..n)
+void launch_periodic_halos(const Geom &g, double *psi, hipStream_t s);
 // dst <- src except the ghost cells of the reflecting edges in `mask`
 void launch_copy_outside_reflect(const Geom &g, const double *src, double *dst, int mask, hipStream_t s);
 // y = beta xb + alpha sum_b mom[b] on the interior of phi-layout arrays (mom: nblk partials of g.nphi())

@@ -340,12 +340,14 @@ void projection_or_file(const Problem &prob, ScalarFlux &out, ProblemEvalFcn f, 
 ProblemInputs Init(ParameterList &params, const std::string &directory) {
     std::unique_ptr<Problem> prob = Problem::Create(params);
     prob->OverrideOptions(params);
-    if (optional_bool(params, "periodic", false))
-        throw std::invalid_argument("periodic boundaries are not available (the sweeps take inflow halos)");
-    int reflect_mask = 0;  // init.cpp:448-477
+    int reflect_mask = 0;  // init.cpp:448-510
     const char *reflect_keys[4] = {"reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max"};
     for (int b = 0; b < 4; ++b)
         if (optional_bool(params, reflect_keys[b], false)) reflect_mask |= 1 << b;
+    if (optional_bool(params, "periodic", false)) {
+        if (reflect_mask) throw std::runtime_error("Combined reflecting and periodic boundary conditions not supported.");
+        reflect_mask = TSW_PERIODIC;
+    }
     const bool pwc_cross_sections = optional_bool(params, "piecewise_cross", false);
 
     DomainDecomposition dd;

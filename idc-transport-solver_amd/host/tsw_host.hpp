@@ -60,8 +60,9 @@ class ParameterList {
 struct DomainDecomposition {
     int64_t nx = 0, ny = 0;
     double ax = 0, bx = 1, ay = 0, by = 1;
-    // reflecting edges (TSW_REFLECT_X_MIN | ...): the reference's Global::reflecting_boundaries, which Init
-    // sets from the deck keys reflect_x_min / _x_max / _y_min / _y_max (init.cpp:448-477)
+    // reflecting edges (TSW_REFLECT_X_MIN | ...) or TSW_PERIODIC: the reference's Global::reflecting_boundaries
+    // and Global::periodic, which Init sets from the deck keys reflect_x_min / _x_max / _y_min / _y_max and
+    // periodic (init.cpp:448-510)
     int reflect_mask = 0;
     double dx(int dim) const { return dim == 0 ? (bx - ax) / nx : (by - ay) / ny; }
 };
@@ -394,8 +395,8 @@ struct ProblemInputs {
 // from nx, ny, ax, bx, ay, by; the DG degree from dg_degree (or dg_degree_x, global.cpp:198-215); the
 // ordinate set from ordinate_type (default "chebyshev-legendre"), ang_order and ordinate_sym_reduce
 // (default false).  The cross sections are projected at DG degree 0 when piecewise_cross is true and at
-// the flux's degree otherwise (init.cpp:741).  Reflecting edges go to dd.reflect_mask (a Device on that
-// decomposition sets them); periodic boundaries throw std::invalid_argument.  With output_init true each projection is read from, or else computed and
+// the flux's degree otherwise (init.cpp:741).  Reflecting edges and periodic boundaries go to
+// dd.reflect_mask (a Device on that decomposition sets them); both at once throw std::runtime_error.  With output_init true each projection is read from, or else computed and
 // written to, the file the reference names (sigma_t_dg<0 or p>_nx0000224.cs2, sigma_s_dg..., 
 // initial_dg<p>_..., source_dg<p>_....sd2) in `directory`.
 ProblemInputs Init(ParameterList &params, const std::string &directory = ".");

@@ -29,6 +29,7 @@ FP64 = 0
 FP32 = 1
 MIXED = 2  # sweeps: FP32 with FP64 iterative refinement; solvers: as FP32
 REFLECT_X_MIN, REFLECT_X_MAX, REFLECT_Y_MIN, REFLECT_Y_MAX = 1, 2, 4, 8
+PERIODIC = 0x100
 MEM_HOST = 0
 MEM_DEVICE = 1
 IDC_LOBATTO = 0
@@ -95,6 +96,7 @@ def lib():
         L.tsw_sweep_apply_multi.argtypes = [c, i, ctypes.POINTER(_vp), _vp, d, i, i]
         L.tsw_set_boundaries.argtypes = [c, i, _vp]
         L.tsw_reflect_boundaries.argtypes = [c, _vp, i]
+        L.tsw_synchronize_halos.argtypes = [c, _vp, i]
         L.tsw_pmv.argtypes = [c, d, d, _vp, _vp, i, i]
         L.tsw_smv.argtypes = [c, d, d, _vp, _vp, _vp, i, i]
         L.tsw_lmv.argtypes = [c, d, d, _vp, _vp, _vp, i]
@@ -301,6 +303,13 @@ class Context:
         else:
             m = np.ascontiguousarray(reflect_map, dtype=np.int32).ravel()
             self._check(self._L.tsw_set_boundaries(self._h, int(reflect_mask), m.ctypes.data))
+
+    def synchronize_halos(self, psi):
+        """tsw_synchronize_halos: reflecting edges, then the periodic copy."""
+        self._expect(psi=psi)
+        (pp,), mem = _addrs(psi)
+        self._check(self._L.tsw_synchronize_halos(self._h, pp, mem))
+        return psi
 
     def reflect_boundaries(self, psi):
         """tsw_reflect_boundaries: the ghost cells of the reflecting edges from psi's interior."""

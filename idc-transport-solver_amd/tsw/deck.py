@@ -5,8 +5,8 @@ condition projected into the DG space, inflow halos, and the named ordinate set.
 Problems: "lattice" (LatticeProblem.cpp:100-324), "hohlraum" (HohlraumProblem.cpp:60-272),
 "hohlraum2" (Hohlraum2Problem.cpp:14-148), "bump" (BumpProblem.cpp) and "inhomogeneous-sphere"
 (InhomogeneousSphereProblem.cpp).  Cross sections are projected at DG degree 0 when piecewise_cross is
-true and at the flux's degree otherwise (init.cpp:741); the reflect_* keys give the problem's reflecting
-edges (init.cpp:448-477); periodic boundaries and mollified problem functions are not available
+true and at the flux's degree otherwise (init.cpp:741); the reflect_* keys and periodic give the problem's
+reflecting edges or periodic boundaries (init.cpp:448-510); mollified problem functions are not available
 (ValueError)."""
 from __future__ import annotations
 
@@ -251,12 +251,14 @@ def init(params):
         for k in ("periodic", "reflect_x_min"):
             if _bool(params, k, False):
                 params[k] = "false"
-    if _bool(params, "periodic", False):
-        raise ValueError("periodic boundaries are not available (the sweeps take inflow halos)")
-    reflect_mask = 0  # init.cpp:448-477
+    reflect_mask = 0  # init.cpp:448-510
     for b, k in enumerate(("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max")):
         if _bool(params, k, False):
             reflect_mask |= 1 << b
+    if _bool(params, "periodic", False):
+        if reflect_mask:
+            raise RuntimeError("Combined reflecting and periodic boundary conditions not supported.")
+        reflect_mask = 0x100  # TSW_PERIODIC
     sdeg_pwc = _bool(params, "piecewise_cross", False)
     mesh = Mesh(int(params["nx"]), int(params["ny"]), float(params["ax"]), float(params["bx"]),
                 float(params["ay"]), float(params["by"]))

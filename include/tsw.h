@@ -164,6 +164,7 @@ int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, con
 #define TSW_REFLECT_X_MAX 0x02 /* BoundaryEdge::X_Max */
 #define TSW_REFLECT_Y_MIN 0x04 /* BoundaryEdge::Y_Min */
 #define TSW_REFLECT_Y_MAX 0x08 /* BoundaryEdge::Y_Max */
+#define TSW_PERIODIC 0x100     /* Global::periodic (the deck key periodic) */
 
 /* Reflecting edges of the current problem: the deck keys reflect_x_min / _x_max / _y_min / _y_max that
  * Init turns into Global::reflecting_boundaries (init.cpp:448-477); 0 restores vacuum/inflow edges
@@ -177,9 +178,19 @@ int tsw_sweep_apply_multi(tsw_ctx *ctx, int n_rhs, double *const *psi_inout, con
  * ny <= 448 and a 16-byte aligned psi (else TSW_E_UNSUPPORTED); its result, ghost cells included, is that
  * of the reference's sweep.  tsw_si_solve then runs SISolver::Solve's loop with the ghost cells of the
  * iterate reflected before each source copy (SynchronizeHalos, SISolver.cpp:222) and the source copied
- * onto the interior and the other edges' ghost cells (OpDomain::Interior | OpDomain::Boundary).  GMRES,
- * the IDC integrator and the vector-Jacobian products refuse reflecting edges with TSW_E_UNSUPPORTED. */
+ * onto the interior and the other edges' ghost cells (OpDomain::Interior | OpDomain::Boundary).
+ * reflect_mask = TSW_PERIODIC (alone: the reference refuses the combination, init.cpp:498-510) sets periodic
+ * boundaries: sweeps take the halos as given, as SweepOperator::Apply does, and tsw_si_solve copies the
+ * iterate's halos across the periodic boundaries before each iteration and the source onto the interior
+ * only.  GMRES, the IDC integrator and the vector-Jacobian products refuse reflecting and periodic
+ * boundaries with TSW_E_UNSUPPORTED. */
 int tsw_set_boundaries(tsw_ctx *ctx, int reflect_mask, const int *reflect_map);
+
+/* OrdinateFlux::SynchronizeHalos (Abstract/OrdinateFlux.cpp:200-205): ReflectBoundaries on the reflecting
+ * edges, then with TSW_PERIODIC the periodic copy of DensityFunction::SynchronizeHalos
+ * (Abstract/DensityFunction.cpp:927-956: the y-halos of columns 1..nx from rows ny and 1, then the x-halos
+ * of rows 1..ny from columns nx and 1, every value of the cell; corners untouched). */
+int tsw_synchronize_halos(tsw_ctx *ctx, double *psi_inout, int mem);
 
 /* OrdinateFlux::ReflectBoundaries (RKDG/OrdinateFlux.cpp:1181-1420) on psi for the edges set by
  * tsw_set_boundaries: ghost cells of x-min (quadrants 0, 1), x-max (2, 3), y-min (0, 2) and y-max (all

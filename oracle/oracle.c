@@ -344,6 +344,23 @@ void orc_reflect_boundaries(int nx, int ny, int p, int nq, const int *quad_off, 
     }
 }
 
+/* DensityFunction::SynchronizeHalos across periodic boundaries (Abstract/DensityFunction.cpp:927-956):
+ * y-halos of columns 1..nx first, then x-halos of rows 1..ny, every value of the cell (corners untouched). */
+void orc_periodic_halos(int nx, int ny, int p, int nq, double *psi)
+{
+    const size_t per = (size_t)nq * (p + 1) * (p + 1);
+    for (int i = 1; i <= nx; ++i)
+        for (size_t l = 0; l < per; ++l) {
+            psi[((size_t)i * (ny + 2) + 0) * per + l] = psi[((size_t)i * (ny + 2) + ny) * per + l];
+            psi[((size_t)i * (ny + 2) + ny + 1) * per + l] = psi[((size_t)i * (ny + 2) + 1) * per + l];
+        }
+    for (int j = 1; j <= ny; ++j)
+        for (size_t l = 0; l < per; ++l) {
+            psi[((size_t)0 * (ny + 2) + j) * per + l] = psi[((size_t)nx * (ny + 2) + j) * per + l];
+            psi[((size_t)(nx + 1) * (ny + 2) + j) * per + l] = psi[((size_t)1 * (ny + 2) + j) * per + l];
+        }
+}
+
 /* SweepOperator::Apply with reflecting edges (SweepPatternUniCartWavefront.cpp:180-245): the quadrants
  * one after the other in the order 3, 2, 1, 0 (SweepPatternUniCart.cpp:33-40), each a wavefront over
  * its diagonals, and ReflectBoundaries on the in-place result after each quadrant.  A quadrant swept

@@ -64,6 +64,8 @@ void orc_sweep(int nx, int ny, int p, int nq, const double *xi, const double *et
  * SweepOperator::Apply with reflecting edges (quadrants in the order 3, 2, 1, 0, ghosts reflected after
  * each). */
 int orc_reflect_map(int nq, const double *xi, const double *eta, int *map);
+/* DensityFunction::SynchronizeHalos across periodic boundaries (every value of the cell, corners untouched). */
+void orc_periodic_halos(int nx, int ny, int p, int nq, double *psi);
 void orc_reflect_boundaries(int nx, int ny, int p, int nq, const int *quad_off, const int *map, int mask, double *psi);
 void orc_sweep_reflect(int nx, int ny, int p, int nq, const double *xi, const double *eta, const int *quad_off,
                        double dx, double dy, const double *sigma_t, double dt, int mask, const int *map, double *psi);

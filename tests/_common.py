@@ -67,6 +67,8 @@ def oracle():
         lib.orc_reflect_boundaries.restype = None
         lib.orc_sweep_reflect.argtypes = [i, i, i, i, P, P, IP, d, d, P, d, i, IP, P]
         lib.orc_sweep_reflect.restype = None
+        lib.orc_periodic_halos.argtypes = [i, i, i, i, P]
+        lib.orc_periodic_halos.restype = None
         lib.orc_set_xs_degree.argtypes = [i]
         lib.orc_set_xs_degree.restype = None
         lib.orc_xs_tensor.argtypes = [i, i, i, P, P]
@@ -175,29 +177,39 @@ def orc_sweep_reflect(g: Grid, o: Ordinates, sigma_t, psi, mask, dt=math.inf, rm
     return out
 
 
+def orc_periodic_halos(g: Grid, o: Ordinates, psi):
+    out = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    oracle().orc_periodic_halos(g.nx, g.ny, g.p, o.nq, ptr(out))
+    return out
+
+
 def orc_si_reflect(g: Grid, o: Ordinates, sigma_t, sigma_s, source, mask, dt=math.inf, abs_tol=1e-8,
-                   max_its=500, psi0=None):
-    """SISolver::Solve (SISolver.cpp:190-254) with reflecting edges, from the oracle's pieces: each iteration
-    reflects the ghost cells of the iterate (SynchronizeHalos: the Smv before left them dirty), copies the
-    source onto the interior and the ghost cells of the other edges (Copy over Interior | Boundary,
-    DomainDecomposition.cpp:285-336), adds the scattering source and sweeps with the reflecting edges.
-    Returns (psi, iterations, last change)."""
+                   max_its=500, psi0=None, periodic=False):
+    """SISolver::Solve (SISolver.cpp:190-254) with reflecting edges `mask` or periodic boundaries, from the
+    oracle's pieces: each iteration synchronizes the iterate's halos (OrdinateFlux::SynchronizeHalos:
+    ReflectBoundaries, then the periodic copy; the Smv before left them dirty), copies the source onto the
+    interior and the ghost cells of the edges that are neither reflecting nor periodic (Copy over
+    Interior | Boundary, DomainDecomposition.cpp:285-336), adds the scattering source and sweeps (reflecting
+    edges: quadrants in sequence).  Returns (psi, iterations, last change)."""
     psi = np.zeros(g.psi_shape(o.nq)) if psi0 is None else np.ascontiguousarray(psi0, dtype=np.float64).copy()
     src = np.ascontiguousarray(source, dtype=np.float64)
-    keep = np.zeros(src.shape[:2], dtype=bool)  # ghost cells of the reflecting edges
-    keep[0, :] |= bool(mask & 1)
-    keep[-1, :] |= bool(mask & 2)
-    keep[:, 0] |= bool(mask & 4)
-    keep[:, -1] |= bool(mask & 8)
+    keep = np.zeros(src.shape[:2], dtype=bool)  # halos the copy leaves alone
+    keep[0, :] |= bool(mask & 1) or periodic
+    keep[-1, :] |= bool(mask & 2) or periodic
+    keep[:, 0] |= bool(mask & 4) or periodic
+    keep[:, -1] |= bool(mask & 8) or periodic
     cur = orc_pmv(g, o, psi)
     it, err = 0, math.inf
     while it < max_its and err > abs_tol:
         it += 1
         last = cur
-        psi = orc_reflect_boundaries(g, o, psi, mask)
+        if mask:
+            psi = orc_reflect_boundaries(g, o, psi, mask)
+        if periodic:
+            psi = orc_periodic_halos(g, o, psi)
         psi[~keep] = src[~keep]
         psi = orc_smv(g, o, sigma_s, last, psi)
-        psi = orc_sweep_reflect(g, o, sigma_t, psi, mask, dt)
+        psi = orc_sweep_reflect(g, o, sigma_t, psi, mask, dt) if mask else orc_sweep(g, o, sigma_t, psi, dt)
         cur = orc_pmv(g, o, psi)
         err = oracle().orc_l2dist(g.nx, g.ny, g.p, ptr(cur), ptr(last))
     return psi, it, err

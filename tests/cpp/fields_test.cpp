@@ -400,7 +400,8 @@ static void test_decks(const std::string &dir) {
                   ho.sigma_t(2, 2, 1, 1) == ho.sigma_t_density(2, 2, 1, 1),
               "piecewise_cross false: degree-1 cross sections with higher moments");
     }
-    throws(base + "piecewise_cross true\nperiodic true\n", "periodic boundaries are refused");
+    throws(base + "piecewise_cross true\nperiodic true\nreflect_x_min true\n",
+           "periodic boundaries combined with reflecting ones are refused");
     {  // reflecting edges: the mask Init derives from the reflect_* keys (init.cpp:448-477)
         write_text(dir + "/r.deck", base + "piecewise_cross true\nreflect_y_min true\nreflect_x_max true\n");
         ParameterList r;

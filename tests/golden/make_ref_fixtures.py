@@ -11,7 +11,8 @@ oracle/build_ref.sh from the unmodified reference sources; run in the container 
 * reflecting boundaries: SweepOperator::Apply on decks with reflect_x_min / _x_max / _y_min / _y_max
   (quadrants in sequence with OrdinateFlux::ReflectBoundaries after each, SweepPatternUniCartWavefront.cpp:
   180-245), the mask (1 x-min, 2 x-max, 4 y-min, 8 y-max) stored beside them;
-* source iteration with reflecting edges: SISolver::Solve on a lattice deck reflecting on three edges;
+* source iteration with reflecting edges: SISolver::Solve on a lattice deck reflecting on three edges, and
+  with periodic boundaries (deck key periodic);
 * source iteration: SISolver::Solve on two lattice decks (steady, abs_tol 1e-10 and 1e-12): the angular
   flux, the iteration count and the last change (printed with four decimals);
 * IDC steps: IDCIntegrator::Step on a lattice deck from a seeded initial condition, for the input deck's
@@ -51,6 +52,9 @@ REFLECT_SWEEPS = {  # name: deck parameters, reflecting-edge mask, seed
 REFLECT_KEYS = ("reflect_x_min", "reflect_x_max", "reflect_y_min", "reflect_y_max")
 REFLECT_SOLVES = {  # source iteration with reflecting edges: deck parameters, mask
     "si_reflect": (dict(n=12, m=10, p=1, pwc="true", otype="spherical-triangle", order=2, tol="1e-10"), 13),
+}
+PERIODIC_SOLVES = {  # source iteration with periodic boundaries (deck key periodic)
+    "si_periodic": dict(n=12, m=11, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-10"),
 }
 IDC_DECK = LATTICE.format(n=10, m=9, p=1, pwc="true", otype="chebyshev-legendre", order=4, tol="1e-13") + \
     "cfl 0.5\nidc_type {form}\nidc_nodes_type {nodes}\nidc_num_nodes {nn}\nidc_num_corr {nc}\n" \
@@ -102,6 +106,20 @@ def main():
         _store_inputs(out, name, d)
         src = d.get_flux(1)
         out[name + "/mask"] = np.array(mask)
+        out[name + "/src"] = src
+        out[name + "/tol"] = np.array(float(params["tol"]))
+        psi, its, err = d.solve(src, os.path.join(tmp, name + ".log"))
+        out[name + "/psi"] = psi
+        out[name + "/its"] = np.array(its)
+        out[name + "/err"] = np.array(err)
+        print(name, "iterations", its, "last change", err)
+    for name, params in PERIODIC_SOLVES.items():
+        path = os.path.join(tmp, name + ".deck")
+        with open(path, "w") as f:
+            f.write(LATTICE.format(**params) + "periodic true\n")
+        d = _ref.Deck(path)
+        _store_inputs(out, name, d)
+        src = d.get_flux(1)
         out[name + "/src"] = src
         out[name + "/tol"] = np.array(float(params["tol"]))
         psi, its, err = d.solve(src, os.path.join(tmp, name + ".log"))

@@ -2,8 +2,9 @@
 against the reference's own SweepOperator::Apply on reflecting decks (tests/golden/ref_fixtures.npz) and
 against the oracle's restatement on random problems (every edge combination, DG degrees 0-3, FP64 to 1e-12,
 FP32 to 2e-6, TSW_MIXED = the FP64 sweep), ghost cells included; source iteration with reflecting edges
-against the reference's SISolver (its iteration count, its angular flux) and the oracle; the ghost update
-alone bit for bit; the configurations refused."""
+against the reference's SISolver (its iteration count, its angular flux) and the oracle; periodic
+boundaries (deck key periodic) the same way; the halo updates alone bit for bit; the configurations
+refused."""
 import math
 import os
 
@@ -11,8 +12,8 @@ import numpy as np
 import pytest
 
 import tsw
-from _common import (REPO, ls_ordinates, orc_reflect_boundaries, orc_si_reflect, orc_sweep_reflect, random_problem,
-                     rel_err)
+from _common import (REPO, ls_ordinates, orc_periodic_halos, orc_reflect_boundaries, orc_si_reflect,
+                     orc_sweep_reflect, random_problem, rel_err)
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(REPO, "tests", "golden", "ref_fixtures.npz")
@@ -182,3 +183,58 @@ def test_reflecting_source_iteration_matches_oracle(mask):
         out32 = src.copy()
         c.si_solve(out32, st, ss, abs_tol=1e-11, max_its=2000, precision=tsw.FP32)
         assert rel_err(out32, ref) <= 1e-5
+
+
+def test_periodic_source_iteration_matches_reference(fx):
+    name = "si_periodic"
+    nx, ny, p, nq, sd = (int(v) for v in fx[name + "/dims"])
+    ax, bx, ay, by = (float(v) for v in fx[name + "/box"])
+    st = fx[name + "/sigma_t"][..., 0, 0].copy()
+    ss = fx[name + "/sigma_s"][..., 0, 0].copy()
+    print("reference periodic SI fixture", flush=True)
+    with tsw.Context(0) as c:
+        c.problem_set(nx, ny, p, fx[name + "/xi"], fx[name + "/eta"], fx[name + "/w"], fx[name + "/quad"], ax, bx, ay, by)
+        c.set_pattern("hip-staggered")
+        c.set_boundaries(tsw.PERIODIC)
+        out = fx[name + "/src"].copy()
+        its, _ = c.si_solve(out, st, ss, abs_tol=float(fx[name + "/tol"]), max_its=5000)
+        assert its == int(fx[name + "/its"])
+        assert rel_err(out, fx[name + "/psi"]) <= 1e-11
+        with pytest.raises(tsw.TswError):  # GMRES is not built for periodic boundaries
+            c.gmres_solve(fx[name + "/src"].copy(), st, ss)
+        with pytest.raises(tsw.TswError):  # the reference refuses the combination
+            c.set_boundaries(tsw.PERIODIC | tsw.REFLECT_X_MIN)
+
+
+@pytest.mark.parametrize("bc", [tsw.PERIODIC, 5, 15])
+def test_synchronize_halos_bitwise(bc):
+    o = ls_ordinates(4)
+    g, _, _, src = random_problem(10, 7, 1, o, seed=4500 + bc)
+    psi = np.random.default_rng(bc).uniform(-1.0, 1.0, size=src.shape)
+    with _ctx(g, o) as c:
+        c.set_boundaries(bc)
+        out = psi.copy()
+        c.synchronize_halos(out)
+        ref = orc_periodic_halos(g, o, psi) if bc == tsw.PERIODIC else orc_reflect_boundaries(g, o, psi, bc)
+        np.testing.assert_array_equal(out, ref)
+
+
+def test_periodic_deck_source_iteration_matches_oracle(tmp_path):
+    from tsw import deck
+    from _common import Grid, Ordinates
+    text = ("ax 0.0\nbx 7.0\nay 0.0\nby 7.0\nnx 14\nny 12\ndg_degree 2\npiecewise_cross true\nproblem lattice\n"
+            "moll_radius 0.0\nordinate_type spherical-triangle\nang_order 2\nordinate_sym_reduce true\n"
+            "solve_type source-iteration\nsweep_pattern wavefront\nabs_tol 1e-11\nmax_its 3000\nperiodic true\n")
+    (tmp_path / "per.deck").write_text(text)
+    P, _ = deck.init(deck.read_deck(str(tmp_path / "per.deck")))
+    assert P.reflect_mask == tsw.PERIODIC
+    g = Grid(P.nx, P.ny, P.p, P.ax, P.bx, P.ay, P.by)
+    o = Ordinates(P.xi, P.eta, P.w, P.quad_off)
+    ref, its, _ = orc_si_reflect(g, o, P.sigma_t, P.sigma_s, P.source, 0, abs_tol=1e-11, max_its=3000, periodic=True)
+    with tsw.Context(0) as c:
+        P.set_on(c)
+        c.set_pattern("hip-staggered")
+        out = P.source.copy()
+        gits, _ = c.si_solve(out, P.sigma_t, P.sigma_s, abs_tol=1e-11, max_its=3000)
+    assert gits == its
+    assert rel_err(out, ref) <= 1e-10

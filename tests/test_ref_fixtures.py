@@ -102,6 +102,19 @@ def test_oracle_reflecting_source_iteration_equals_reference(fx):
     np.testing.assert_array_equal(psi, fx[name + "/psi"])
 
 
+def test_oracle_periodic_source_iteration_equals_reference(fx):
+    """SISolver::Solve on a lattice deck with periodic boundaries (halos synchronized from the opposite side
+    before each iteration's source copy, which then skips them): iteration count, last change and angular
+    flux bit for bit."""
+    name = "si_periodic"
+    g, o, _, st, ss = _case(fx, name)
+    psi, its, err = orc_si_reflect(g, o, st, ss, fx[name + "/src"], 0, abs_tol=float(fx[name + "/tol"]),
+                                   max_its=5000, periodic=True)
+    assert its == int(fx[name + "/its"])
+    assert float(f"{err:.4e}") == float(fx[name + "/err"])
+    np.testing.assert_array_equal(psi, fx[name + "/psi"])
+
+
 def test_oracle_reflect_map_and_ghosts_closed_forms(fx):
     """The mirror map flips one direction cosine and is an involution; reflected ghosts of a field that is
     itself mirror-symmetric reproduce the field's own halo values."""
