@@ -3988,7 +3988,8 @@ void batch_apply(tsw_ctx *ctx, const std::vector<int> &members, std::vector<doub
 
 // Autotuning of the batch engines, per DG degree: each candidate (staggered kernels through the layout
 // conversions or on the caller's layout, one ordinate per wavefront; the ordinate-lane engine, J ordinates
-// per wavefront) sweeps the degree's members, or the largest ones that fit a 4 GiB scratch budget, twice
+// per wavefront) sweeps the degree's members, or the largest ones that fit a scratch budget (4 GiB, at most
+// a quarter of the free device memory), twice
 // in FP64 on zero-source scratch arrays, and the second sweep is timed.  Every engine gives the same bits,
 // so the choice only affects speed.  Without the autotuner (or with an engine forced by the tuning keys
 // "lanes" and "batch_ref") the lane engine takes the degrees it is built for and the staggered kernels on
@@ -4020,10 +4021,12 @@ void batch_choose_engines(tsw_ctx *ctx) {
             std::sort(mem.begin(), mem.end(),
                       [&](int a, int b) { return ctx->batch[a]->g.npsi() > ctx->batch[b]->g.npsi(); });
             std::vector<int> sub_members;
-            size_t bytes = 0;
+            size_t bytes = 0, free_b = 0, total_b = 0;
+            TSW_HIP(hipMemGetInfo(&free_b, &total_b));
+            const size_t budget = std::min(size_t(4) << 30, free_b / 4);  // leave room for the engines' buffers
             for (int b : mem) {
                 const size_t nb = ctx->batch[b]->g.npsi() * sizeof(double);
-                if (!sub_members.empty() && bytes + nb > (size_t(4) << 30)) continue;
+                if (!sub_members.empty() && bytes + nb > budget) continue;
                 sub_members.push_back(b);
                 bytes += nb;
             }
