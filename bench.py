@@ -228,7 +228,8 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
     """Batched sweep of the seeded 256-problem uneven batch (N 32..256, Q1..Q3, S4..S12, 8x8 block
     materials): one step = one tsw_batch_sweep call sweeping every member in place, inputs resident in HBM.
     Timed in FP32, mixed (TSW_MIXED) and FP64.  The batch engine of each DG degree (staggered kernels
-    through layout conversions or on the caller's layout, or the ordinate-lane engine) is the autotuner's
+    through layout conversions or on the caller's layout, the ordinate-lane engine, or the lane-split
+    staggered kernel on the caller's layout) is the autotuner's
     choice at tsw_batch_set unless forced.  TSW_MIXED runs the FP64 sweep on the engines that work on the
     caller's layout (their FP32 sweep reads and writes the same FP64 arrays and computes no faster), and
     FP32 sweeps with one FP64 refinement on converted members.  value = unknowns/s of the faster of the
@@ -249,6 +250,7 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
     ctx.set_tuning("batch_direct", args.batch_direct)
     ctx.set_tuning("lanes", args.lanes)
     ctx.set_tuning("batch_ref", args.batch_ref)
+    ctx.set_tuning("batch_split", args.batch_split)
     t0 = time.perf_counter()
     ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
     t_set = time.perf_counter() - t0
@@ -276,10 +278,12 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
         outs[name] = work
         print(f"bench: batched {name}: call {wall / steps * 1e3:.3f} ms, {res[name][2]} batched sweeps per call "
               f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
-    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes", "batch_ref", "batch_engine_p1",
-                                             "batch_engine_p2", "batch_engine_p3")}
+    tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes", "batch_ref", "batch_split",
+                                             "batch_split_prefetch", "batch_engine_p1", "batch_engine_p2",
+                                             "batch_engine_p3")}
     ctx.close()
-    engine_names = {0: "staggered, layout conversions", 1: "staggered, caller's layout", 2: "ordinate lanes"}
+    engine_names = {0: "staggered, layout conversions", 1: "staggered, caller's layout", 2: "ordinate lanes",
+                    3: "lane-split staggered, caller's layout"}
     engines = {f"Q{p}": engine_names.get(tuned[f"batch_engine_p{p}"], "none") for p in (1, 2, 3)}
     print(f"bench: batched engines {engines} (tsw_batch_set {t_set * 1e3:.0f} ms)",
           file=sys.stderr, flush=True)
@@ -378,6 +382,9 @@ def main():
                     help="batched workload: the precisions timed (comma-separated fp32, mixed, fp64)")
     ap.add_argument("--batch-ref", type=int, default=0,
                     help="batched workload: staggered kernels on the caller's layout forced (1), forbidden (-1) "
+                         "or autotuned (0)")
+    ap.add_argument("--batch-split", type=int, default=0,
+                    help="batched workload: lane-split kernel on the caller's layout forced (1), forbidden (-1) "
                          "or autotuned (0)")
     ap.add_argument("--dsa", type=int, default=1, help="diffusion synthetic acceleration of the GMRES (1 on)")
     ap.add_argument("--dsa-cycles", type=int, default=1, help="multigrid V-cycles per DSA application")

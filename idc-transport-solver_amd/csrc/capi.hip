@@ -110,11 +110,17 @@ struct tsw_ctx {
         DBuf ref_params;
         size_t ref_lds = 0;
         bool ref_mat = false, ref_direct = false, ref_stage = false;
+        // lane-split kernel (on the caller's layout, or on the diagonal layout when a call's arrays are not
+        // 16-byte aligned): members in row bands of at most `rows` rows, items {slot, band * nq + ordinate}
+        bool split = false;
+        int rows = 0;
+        std::vector<int> band_rows;  // per member slot, fixed when the items are built
     };
     // Engine of each batch member, chosen per DG degree (tsw_batch_set): the staggered kernels through
-    // layout conversions, the staggered kernels on the caller's layout, or the ordinate-lane engine.
-    enum : char { kEngConverted = 0, kEngRef = 1, kEngLanes = 2 };
-    std::vector<char> engine, lanes_able;
+    // layout conversions, the staggered kernels on the caller's layout, the ordinate-lane engine, or the
+    // lane-split staggered kernel on the caller's layout.
+    enum : char { kEngConverted = 0, kEngRef = 1, kEngLanes = 2, kEngSplit = 3 };
+    std::vector<char> engine, lanes_able, split_able;
     bool batch_ref_ok = false;  // every member's array admits 32-bit byte offsets
     int batch_engine[4] = {-1, -1, -1, -1};  // per DG degree present in the batch
     std::vector<int> batch_all;              // 0 .. nprob-1
@@ -172,6 +178,9 @@ struct tsw_ctx {
     // whenever every member has element-inverse tables (tuning key batch_direct)
     int batch_direct_opt = 0;
     int batch_ref_opt = 0;   // batch engines: 1 / -1 force / forbid the reference-layout staggered groups
+    // batch engines: 1 / -1 force / forbid the lane-split groups (tuning key batch_split); their rows per
+    // band (batch_split_rows, 0: the most one workgroup holds) and prefetch depth (batch_split_prefetch)
+    int batch_split_opt = 0, batch_split_rows_opt = 0, batch_split_pf_opt = 0;
     int ref_stage_opt = 0;   // 1: reference-layout sweeps stage their source loads through LDS where it fits
     // Reflecting edges (tsw_set_boundaries): mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max; the mirror map
     // [2][nq] (OrdinateSet::ReflectMap) on the device
@@ -264,6 +273,7 @@ struct tsw_ctx {
 constexpr size_t kScratchBytes = size_t(128) << 20;  // 128 MiB per GPU
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 constexpr int kMaxScratchDevices = 64;
+constexpr int kSplitBatchPrefetch = 4;  // prefetch depth of the batch's lane-split groups
 static std::mutex g_scratch_mu;
 static void *g_scratch_ptr[kMaxScratchDevices] = {};
 
@@ -396,6 +406,7 @@ void sync(tsw_ctx *c) {
             *k->abort_host = 0;
             TSW_HIP(hipMemsetAsync(k->abort_dev.ptr, 0, sizeof(int), c->stream));
             refill_exchange(k);
+            for (auto &m : k->batch) refill_exchange(m.get());  // batch members' bands report through k
         }
     }
     if (stalled) {
@@ -1041,6 +1052,18 @@ int band_rows_for(const tsw_ctx *c) {
     return br;
 }
 
+// The context's device stall flag (allocated and cleared on first use); sync(c) reports a stall.
+int *abort_flag_of(tsw_ctx *c) {
+    if (!c->abort_host) {
+        TSW_HIP(hipHostMalloc((void **)&c->abort_host, sizeof(int), hipHostMallocDefault));
+        *c->abort_host = 0;
+        c->abort_dev.reserve(sizeof(int));
+        TSW_HIP(hipMemsetAsync(c->abort_dev.ptr, 0, sizeof(int), c->stream));
+    }
+    c->bands_used = true;
+    return c->abort_dev.as<int>();
+}
+
 // Fills the band fields of sp and makes sure the exchange records exist (filled with the sentinel).
 template <typename T>
 void configure_bands(tsw_ctx *c, DiagSweepParams<T> &sp) {
@@ -1061,15 +1084,8 @@ void configure_bands(tsw_ctx *c, DiagSweepParams<T> &sp) {
         have = n;
         launch_exchange_fill<T>(buf.as<T>(), n, c->stream);
     }
-    if (!c->abort_host) {
-        TSW_HIP(hipHostMalloc((void **)&c->abort_host, sizeof(int), hipHostMallocDefault));
-        *c->abort_host = 0;
-        c->abort_dev.reserve(sizeof(int));
-        TSW_HIP(hipMemsetAsync(c->abort_dev.ptr, 0, sizeof(int), c->stream));
-    }
     sp.xtr = buf.as<T>();
-    sp.abort_flag = c->abort_dev.as<int>();
-    c->bands_used = true;
+    sp.abort_flag = abort_flag_of(c);
 }
 
 // Kernel parameters of one staggered sweep: psi_out <- L^{-1}(qsrc + sigma_s phi_d / (4 pi)), inflow from
@@ -2884,6 +2900,16 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "batch_ref") {
             require(value == -1 || value == 0 || value == 1, TSW_E_ARG, "batch_ref must be -1, 0 or 1");
             ctx->batch_ref_opt = value;
+        } else if (k == "batch_split") {
+            require(value == -1 || value == 0 || value == 1, TSW_E_ARG, "batch_split must be -1, 0 or 1");
+            ctx->batch_split_opt = value;
+        } else if (k == "batch_split_rows") {
+            require(value == 0 || (value >= 16 && value <= 256), TSW_E_ARG, "batch_split_rows must be 0 or 16..256");
+            ctx->batch_split_rows_opt = value;
+        } else if (k == "batch_split_prefetch") {
+            require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
+                    "batch_split_prefetch must be 0, 2, 3, 4 or 6");
+            ctx->batch_split_pf_opt = value;
         } else if (k == "gmres_refine") {
             require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
             ctx->gmres_refine = value != 0;
@@ -2936,6 +2962,10 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;
         else if (k == "batch_ref") *value = ctx->last_batch_ref ? 1 : 0;
         else if (k == "ref_stage") *value = ctx->ref_stage_opt;
+        else if (k == "batch_split") *value = ctx->batch_split_opt;
+        else if (k == "batch_split_rows") *value = ctx->batch_split_rows_opt;
+        else if (k == "batch_split_prefetch")
+            *value = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
             *value = ctx->batch_engine[k[14] - '0'];
         else if (k == "lanes") *value = ctx->batch.empty() ? ctx->last_lanes : (ctx->lane_groups.empty() ? 0 : 1);
@@ -3546,6 +3576,18 @@ int tsw_idc_adjoint(tsw_ctx *ctx, const double *psi0, const double *source, cons
 namespace {
 void batch_choose_engines(tsw_ctx *ctx);
 void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members);
+
+// Lanes per mesh row of the lane-split kernel for DG degree p (split_lanes<p + 1>).
+int split_lanes_of(int p) { return p == 1 ? 2 : 4; }
+
+// Rows per band of a batch member on the lane-split engine: the fewest bands of at most R rows (R: the
+// tuning key batch_split_rows, else the rows one workgroup of 512 threads holds), split evenly.
+int split_band_rows(const tsw_ctx *ctx, int p, int ny) {
+    int R = 512 / split_lanes_of(p);
+    if (ctx->batch_split_rows_opt > 0) R = std::min(R, ctx->batch_split_rows_opt);
+    const int nb = (ny + R - 1) / R;
+    return (ny + nb - 1) / nb;
+}
 }  // namespace
 
 extern "C" {
@@ -3559,6 +3601,7 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
         ctx->lane_groups.clear();
         ctx->engine.assign(nprob, tsw_ctx::kEngConverted);
         ctx->lanes_able.assign(nprob, 0);
+        ctx->split_able.assign(nprob, 0);
         ctx->batch_all.resize(nprob);
         for (int b = 0; b < nprob; ++b) ctx->batch_all[b] = b;
         ctx->batch_dt = dt;
@@ -3587,9 +3630,15 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
             sub->lanes_pf_opt = ctx->lanes_pf_opt;
             ctx->lanes_able[b] = ctx->lanes_opt >= 0 && lanes_ok(sub.get()) && sub->mat.valid &&
                                  diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt);
+            // the lane-split kernel where it is built for the degree and the member's tables fit its LDS
+            const int sbr = g.p >= 1 ? split_band_rows(ctx, g.p, g.ny) : 0;
+            ctx->split_able[b] = ctx->batch_split_opt >= 0 && g.p >= 1 && sub->mat.valid &&
+                                 diag_inverses<double>(sub.get(), dt) && diag_inverses<float>(sub.get(), dt) &&
+                                 split_sweep_lds_bytes<double>(g.p, sbr, g.nx, sub->mat.M) != 0 &&
+                                 split_sweep_lds_bytes<float>(g.p, sbr, g.nx, sub->mat.M) != 0;
             ctx->batch.push_back(std::move(sub));
         }
-        ctx->batch_ref_ok = ctx->batch_ref_opt >= 0;  // 32-bit byte offsets into every member's array
+        ctx->batch_ref_ok = true;  // 32-bit byte offsets into every member's array
         for (auto &sub : ctx->batch)
             ctx->batch_ref_ok = ctx->batch_ref_ok && sub->g.npsi() * sizeof(double) < (size_t(1) << 32);
         for (int k = 0; k < tsw_ctx::kSideStreams; ++k) {
@@ -3635,14 +3684,43 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
                         : (T *)(sizeof(T) == 8 ? sub->diag.q64.ptr : sub->diag.q32.ptr);
             sp = make_diag_params<T>(sub, ctx->batch_dt, q, nullptr, !corr, q);
         }
-        lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, g.p));
+        if (!G.split) lds = std::max(lds, diag_sweep_lds_bytes<T>(sp, g.p));
         ps.push_back(sp);
+    }
+    if (G.split) {  // lane-split kernel: material path, row bands of split_band_rows rows
+        for (size_t k = 0; k < ps.size(); ++k) {
+            tsw_ctx *sub = ctx->batch[G.members[k]].get();
+            const Geom &g = sub->g;
+            DiagSweepParams<T> &sp = ps[k];
+            require(sp.ainv != nullptr, TSW_E_STATE, "lane-split batch member without element inverses");
+            sp.split = 1;
+            sp.direct = 0;
+            sp.wave1 = 0;
+            const int br = G.band_rows[k];
+            sp.nbands = (g.ny + br - 1) / br;
+            sp.band_rows = sp.nbands > 1 ? br : g.ny;
+            if (sp.nbands > 1) {
+                sp.xtr = lanes_records<T>(sub, sp.nbands);
+                sp.abort_flag = abort_flag_of(ctx);  // a stall is reported (and the records refilled) by sync(ctx)
+            }
+            const size_t b = split_sweep_lds_bytes<T>(g.p, G.rows, g.nx, sp.M);
+            require(b != 0, TSW_E_UNSUPPORTED, "lane-split batch group does not fit one CU's LDS");
+            lds = std::max(lds, b);
+        }
+        if (ref) {
+            G.ref_direct = false;
+            G.ref_stage = false;
+            G.ref_mat = true;
+        } else {
+            (sizeof(T) == 8 ? G.direct64 : G.direct32) = false;
+            (sizeof(T) == 8 ? G.mat64 : G.mat32) = true;
+        }
     }
     // one launch serves the group: if any member needs the direct-store layout every member uses it;
     // members whose tables do not fit fall back to elimination, and a mixed group eliminates everywhere
     bool any_direct = false;
     for (auto &sp : ps) any_direct = any_direct || (sp.ainv && sp.direct);
-    if (!any_direct && ctx->batch_direct_opt != -1) {
+    if (!G.split && !any_direct && ctx->batch_direct_opt != -1) {
         // Without the storer wave and its staging rows a workgroup is one wave and 2*rows*NS values
         // smaller, so more of them share a CU: the batched sweep is limited by how many item sweeps run
         // at once (on the 256-problem batch the FP32 kernels take 6.5 ms direct, 8.6 ms with the storer).
@@ -3650,7 +3728,7 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
         for (auto &sp : ps) all_tables = all_tables && sp.ainv != nullptr;
         any_direct = all_tables;
     }
-    if (any_direct) {
+    if (!G.split && any_direct) {
         lds = 0;
         for (size_t k = 0; k < ps.size(); ++k) {
             tsw_ctx *sub = ctx->batch[G.members[k]].get();
@@ -3663,7 +3741,7 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
     }
     bool all_mat = G.mat;
     for (auto &sp : ps) all_mat = all_mat && sp.ainv != nullptr;
-    if (!ref) {
+    if (!ref && !G.split) {
         (sizeof(T) == 8 ? G.direct64 : G.direct32) = all_mat && any_direct;
         (sizeof(T) == 8 ? G.mat64 : G.mat32) = all_mat;
     }
@@ -3681,8 +3759,10 @@ void batch_group_params(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
     }
     require(lds != 0, TSW_E_UNSUPPORTED, "batch group does not fit the staggered sweep");
     if (ref) {
-        G.ref_direct = all_mat && any_direct;
-        G.ref_stage = ctx->ref_stage_opt > 0 && ref_stage_layout<T>(ps, G.p, all_mat, lds, G.ref_direct);
+        if (!G.split) {
+            G.ref_direct = all_mat && any_direct;
+            G.ref_stage = ctx->ref_stage_opt > 0 && ref_stage_layout<T>(ps, G.p, all_mat, lds, G.ref_direct);
+        }
         G.ref_host.resize(sizeof(DiagSweepParams<T>) * ps.size());
         std::memcpy(G.ref_host.data(), ps.data(), G.ref_host.size());
         TSW_HIP(hipMemcpyAsync(G.ref_params.reserve(G.ref_host.size()), G.ref_host.data(), G.ref_host.size(),
@@ -3716,6 +3796,7 @@ void lanes_batch_params(tsw_ctx *ctx, const std::vector<double *> &arr, std::vec
             size_t l = 0;
             ps[slot] = lanes_params<T>(ctx->batch[b].get(), ctx->batch_dt, arr[b], nullptr, nullptr, arr[b], nullptr,
                                        false, th, l);
+            if (ps[slot].nbands > 1) ps[slot].abort_flag = abort_flag_of(ctx);  // reported by sync(ctx)
             require(threads[k] == 0 || threads[k] == th, TSW_E_STATE, "lane batch members of unequal workgroups");
             threads[k] = th;
             lds[k] = std::max(lds[k], l);
@@ -3746,7 +3827,7 @@ void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStre
 void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
     ctx->groups.clear();
     ctx->lane_groups.clear();
-    std::map<std::tuple<int, int, bool, bool>, int> gid;
+    std::map<std::tuple<int, int, bool, int>, int> gid;
     std::map<int, int> lid;
     for (int b : members) {
         tsw_ctx *sub = ctx->batch[b].get();
@@ -3760,32 +3841,44 @@ void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
             ctx->lane_groups[it->second].members.push_back(b);
             continue;
         }
-        const bool ref = ctx->engine[b] == tsw_ctx::kEngRef;
-        const auto key = std::make_tuple(sub->g.p, (sub->g.ny + 63) / 64, sub->mat.valid, ref);
+        const int e = ctx->engine[b];
+        const bool split = e == tsw_ctx::kEngSplit;
+        // staggered groups hold members of equal waves per workgroup, split groups of equal row slots
+        const int rpw = split ? 64 / split_lanes_of(sub->g.p) : 64;
+        const int br = split ? split_band_rows(ctx, sub->g.p, sub->g.ny) : sub->g.ny;
+        const auto key = std::make_tuple(sub->g.p, (br + rpw - 1) / rpw, sub->mat.valid, e);
         auto it = gid.find(key);
         if (it == gid.end()) {
             it = gid.emplace(key, (int)ctx->groups.size()).first;
             ctx->groups.emplace_back();
             auto &G = ctx->groups.back();
             G.p = sub->g.p;
-            G.waves = (sub->g.ny + 63) / 64;
+            G.waves = (br + rpw - 1) / rpw;
             G.mat = sub->mat.valid;
-            G.ref = ref;
+            G.ref = e == tsw_ctx::kEngRef || split;
+            G.split = split;
         }
         auto &G = ctx->groups[it->second];
         G.members.push_back(b);
+        G.band_rows.push_back(br);
         G.max_ny = std::max(G.max_ny, sub->g.ny);
+        G.rows = std::max(G.rows, br);
     }
     for (auto &G : ctx->groups) {
-        std::vector<std::tuple<long long, int, int>> w;
+        // (-work, slot, band * nq + ordinate), band-major: every band's upstream band is dispatched first
+        std::vector<std::tuple<int, long long, int, int>> w;
         for (int slot = 0; slot < (int)G.members.size(); ++slot) {
             const Geom &g = ctx->batch[G.members[slot]]->g;
-            for (int q = 0; q < g.nq; ++q) w.emplace_back(-(long long)(g.nx + g.ny) * g.ny, slot, q);
+            const int br = G.band_rows[slot];
+            const int nb = (g.ny + br - 1) / br;
+            for (int band = 0; band < nb; ++band)
+                for (int q = 0; q < g.nq; ++q)
+                    w.emplace_back(band, -(long long)(g.nx + g.ny) * g.ny, slot, band * g.nq + q);
         }
         std::sort(w.begin(), w.end());
         for (auto &t : w) {
-            G.items.push_back(std::get<1>(t));
             G.items.push_back(std::get<2>(t));
+            G.items.push_back(std::get<3>(t));
         }
         TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
                                sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -3841,7 +3934,12 @@ void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
         lds = sizeof(T) == 8 ? G.lds64 : (corr ? G.lds32c : G.lds32);
     }
     rep.prefetch = ctx->prefetch_opt;
-    launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2, G.max_ny, lds, G.p, sd);
+    if (G.split) {
+        rep.split = 1;
+        rep.prefetch = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
+    }
+    launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2,
+                               G.split ? G.rows : G.max_ny, lds, G.p, sd);
 }
 
 // One batched sweep over the current groups, all on the side streams, timed as one sweep.  Converted
@@ -3869,6 +3967,18 @@ void batch_sweep(tsw_ctx *ctx, bool conv64, bool inplace64, bool corr, const std
             batch_group_params<double>(ctx, *gd.first, false);
         } else {
             batch_group_params<float>(ctx, *gd.first, corr);
+        }
+    }
+    if (ctx->stall_test) {  // tuning key stall_test: every bounded wait of a banded lane-split group gives up
+        bool banded = false;
+        for (auto &gd : diag)
+            for (size_t m = 0; gd.first->split && m < gd.first->members.size(); ++m)
+                banded = banded || gd.first->band_rows[m] < ctx->batch[gd.first->members[m]]->g.ny;
+        if (banded) {
+            ctx->stall_test = false;
+            int *flag = abort_flag_of(ctx);
+            *ctx->abort_host = 1;
+            TSW_HIP(hipMemcpyAsync(flag, ctx->abort_host, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         }
     }
     hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
@@ -3919,7 +4029,7 @@ void batch_apply(tsw_ctx *ctx, const std::vector<int> &members, std::vector<doub
     ctx->last_batch_ref = ref_call;
     auto converted = [&](int b) {
         const char e = ctx->engine[b];
-        return e == tsw_ctx::kEngConverted || (e == tsw_ctx::kEngRef && !ref_call);
+        return e == tsw_ctx::kEngConverted || ((e == tsw_ctx::kEngRef || e == tsw_ctx::kEngSplit) && !ref_call);
     };
     const bool conv64 = precision == TSW_FP64, mixed = precision == TSW_MIXED;
     const bool inplace64 = precision != TSW_FP32;
@@ -3999,20 +4109,24 @@ void batch_choose_engines(tsw_ctx *ctx) {
     for (int p = 0; p < 4; ++p) ctx->batch_engine[p] = -1;
     for (int p = 0; p < 4; ++p) {
         std::vector<int> mem;
-        bool all_lanes = true;
+        bool all_lanes = true, all_split = ctx->batch_ref_ok;
         for (int b = 0; b < nprob; ++b)
             if (ctx->batch[b]->g.p == p) {
                 mem.push_back(b);
                 all_lanes = all_lanes && ctx->lanes_able[b];
+                all_split = all_split && ctx->split_able[b];
             }
         if (mem.empty()) continue;
         std::vector<char> cand;
-        if (ctx->lanes_opt > 0 && all_lanes) cand = {tsw_ctx::kEngLanes};
+        if (ctx->batch_split_opt > 0 && all_split) cand = {tsw_ctx::kEngSplit};
+        else if (ctx->lanes_opt > 0 && all_lanes) cand = {tsw_ctx::kEngLanes};
         else if (ctx->batch_ref_opt > 0 && ctx->batch_ref_ok) cand = {tsw_ctx::kEngRef};
         else {
+            if (ctx->batch_split_opt == 0 && all_split) cand.push_back(tsw_ctx::kEngSplit);
             if (ctx->lanes_opt == 0 && all_lanes) cand.push_back(tsw_ctx::kEngLanes);
-            if (ctx->batch_ref_ok) cand.push_back(tsw_ctx::kEngRef);
-            if (ctx->batch_ref_opt <= 0 && ctx->lanes_opt <= 0) cand.push_back(tsw_ctx::kEngConverted);
+            if (ctx->batch_ref_opt == 0 && ctx->batch_ref_ok) cand.push_back(tsw_ctx::kEngRef);
+            if (ctx->batch_ref_opt <= 0 && ctx->lanes_opt <= 0 && ctx->batch_split_opt <= 0)
+                cand.push_back(tsw_ctx::kEngConverted);
             if (cand.empty()) cand.push_back(tsw_ctx::kEngConverted);
         }
         char best = cand[0];

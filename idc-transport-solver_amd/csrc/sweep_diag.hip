@@ -30,6 +30,21 @@ size_t split_lds_total(int BR, int nx, int M) {
 }
 
 template <typename T>
+size_t split_sweep_lds_bytes(int p, int rows, int nx, int M) {
+    size_t total = 0;
+    switch (p) {
+        case 1: total = split_lds_total<T, 2>(rows, nx, M); break;
+        case 2: total = split_lds_total<T, 3>(rows, nx, M); break;
+        case 3: total = split_lds_total<T, 4>(rows, nx, M); break;
+        default: break;
+    }
+    const size_t bytes = total * sizeof(T);
+    return bytes <= max_lds_per_block() ? bytes : 0;
+}
+template size_t split_sweep_lds_bytes<double>(int, int, int, int);
+template size_t split_sweep_lds_bytes<float>(int, int, int, int);
+
+template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p) {
     const int BR = band_rows_of(sp);
     if (BR > 448) return 0;

@@ -634,16 +634,26 @@ inline int band_rows_of(const DiagSweepParams<T> &sp) {
 template <typename T, int P, bool MAT, int MODE, int D>
 void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const DiagSweepParams<T> *batch,
                         const int2 *items, int nitems, int max_ny) {
+    // rows of a workgroup: the band (one problem) or the largest band of the batch's members (max_ny)
     const int rows = batch ? max_ny : band_rows_of(sp);
     if (sp.split) {
-        if constexpr (MAT && split_supported<T, P>() && (MODE & 4) == 0) {
+        if constexpr (MAT && split_supported<T, P>() && ((MODE & 4) == 0 || MODE == 5)) {
             constexpr int LG = split_lanes<P>(), RPW = 64 / LG;  // lanes per row, rows per wavefront
             const int threads = LG * ((rows + RPW - 1) / RPW * RPW);  // LG lanes for each row slot
-            if (batch || threads > 512) throw Error(TSW_E_ARG, "lane-split sweep needs one problem and bands of at most 512 / lanes rows");
-            auto kern = k_diag_sweep_split<T, P, MODE, D>;
-            if (lds > 64 * 1024)
-                TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(kern, dim3(sp.nq * (sp.nbands > 1 ? sp.nbands : 1)), dim3(threads), lds, s, sp);
+            if (threads > 512) throw Error(TSW_E_ARG, "lane-split sweep needs bands of at most 512 / lanes rows");
+            if (batch) {
+                auto kern = k_diag_sweep_split<T, P, MODE, D, true>;
+                if (lds > 64 * 1024)
+                    TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, dim3(nitems), dim3(threads), lds, s, sp, batch, items);
+            } else {
+                if constexpr ((MODE & 4) != 0) throw Error(TSW_E_ARG, "reference-layout lane-split sweep is built for batches");
+                auto kern = k_diag_sweep_split<T, P, MODE, D, false>;
+                if (lds > 64 * 1024)
+                    TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, dim3(sp.nq * (sp.nbands > 1 ? sp.nbands : 1)), dim3(threads), lds, s, sp,
+                                   nullptr, nullptr);
+            }
             TSW_HIP(hipGetLastError());
             return;
         } else {
@@ -705,6 +715,11 @@ void launch_diag_sweep_p(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s
 template <typename T, int P>
 void launch_diag_sweep_batch_p(const DiagSweepParams<T> &rep, const DiagSweepParams<T> *probs, const int2 *items,
                                int nitems, int max_ny, size_t lds, hipStream_t s) {
+    if (rep.qref && rep.split) {  // lane-split kernel on the caller's layout (rep.prefetch or the default depth)
+        if (rep.phi || !rep.ainv) throw Error(TSW_E_ARG, "reference-layout lane-split batch sweep: sources, material path");
+        launch_ds<T, P, true, 5>(rep, lds, s, probs, items, nitems, max_ny);
+        return;
+    }
     if (rep.qref) {  // reference-layout I/O: sources in place, the default prefetch depth
         constexpr int D = sizeof(T) == 8 ? 2 : 3;
         if (rep.phi) throw Error(TSW_E_ARG, "reference-layout batch sweep: sources only");

@@ -14,6 +14,13 @@
 // Bands, exchange records, prefetch ring and the wave-uniform skip of diagonals without a cell follow
 // k_diag_sweep; stores are direct (no storer wave), and the band's first and last rows consume and fill
 // the exchange records themselves, as the one-wave variant does.
+//
+// MODE bit 4 (with bit 1, sources only): the source is read from `qref` and psi written to `psi_ref`, both in
+// the reference layout [i][j][q][d][e], where lane g's coefficients are one contiguous run of P doubles
+// (whole 16-byte accesses for even P) and the LG lanes of a row cover the cell's run, so no layout
+// conversion is needed and a wavefront's accesses are RPW whole cell runs.  BATCH: the workgroup's
+// parameters are batch[items[b].x], and items[b].y = band * nq + ordinate, numbered band-major so that a
+// band's upstream band is always dispatched first.
 #pragma once
 
 namespace tsw {
@@ -24,10 +31,10 @@ __host__ __device__ constexpr int split_lanes() {
     return P == 2 ? 2 : 4;
 }
 // The lane-split sweep keeps a lane's P rows of the inverse in registers: built where they fit
-// (FP32 Q1-Q3, FP64 Q1-Q2).
+// (Q1-Q3 in both precisions; an FP64 Q3 lane holds 64 doubles, about 230 VGPRs in all, two waves per SIMD).
 template <typename T, int P>
 __host__ __device__ constexpr bool split_supported() {
-    return P >= 2 && P <= 4 && P * P * P * (int)sizeof(T) <= 64 * 4;
+    return P >= 2 && P <= 4 && P * P * P * (int)sizeof(T) <= 64 * 8;
 }
 
 template <typename T, int P>
@@ -73,17 +80,46 @@ __device__ __forceinline__ T group_bcast(T v, int K) {
     }
 }
 
-template <typename T, int P, int MODE, int D>
-__global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp) {
+// P contiguous doubles of a reference-layout run at byte offset off (16-byte accesses for even P)
+template <int P>
+__device__ __forceinline__ void ref_run_ld(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[P]) {
+    if constexpr (P % 2 == 0) {
+        typedef unsigned V4 __attribute__((ext_vector_type(4)));
+        typedef unsigned V2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int i = 0; i < P / 2; ++i) {
+            const V4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 16 * i, 0);
+            v[2 * i] = __builtin_bit_cast(double, V2{x[0], x[1]});
+            v[2 * i + 1] = __builtin_bit_cast(double, V2{x[2], x[3]});
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = buf_ld<double>(r, off, 8 * k);
+    }
+}
+
+template <typename T, int P, int MODE, int D, bool BATCH>
+__global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
+                                                          const int2 *__restrict__ items) {
     constexpr int N = P * P, LG = split_lanes<P>(), RPW = 64 / LG;  // rows per wavefront
     using Lds = SplitLds<T, P>;
     constexpr int PS = Lds::PS, RS = Lds::RS;
     using U = typename ExchangeSentinel<T>::U;
+    constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0, REF = (MODE & 4) != 0;
+    static_assert(!REF || MODE == 5, "reference-layout lane-split sweep: sources only");
+    using PQ = typename std::conditional<REF, double, T>::type;  // prefetched source (converted at use)
+    int item_p = 0, item_v = 0;
+    if constexpr (BATCH) {
+        const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)blockIdx.x;
+        item_p = it[0];
+        item_v = it[1];
+    }
+    const DiagSweepParams<T> sp = BATCH ? batch[item_p] : sp1;
     if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
     const int nb = sp.nbands < 2 ? 1 : sp.nbands;
-    const int q = (int)(blockIdx.x % (unsigned)sp.nq);
-    const int band = (int)(blockIdx.x / (unsigned)sp.nq);
-    constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0;
+    const unsigned wid = BATCH ? (unsigned)item_v : blockIdx.x;  // band * nq + ordinate
+    const int q = (int)(wid % (unsigned)sp.nq);
+    const int band = (int)(wid / (unsigned)sp.nq);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int nx = sp.nx, ny = sp.ny;
@@ -130,8 +166,11 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp)
 
     const unsigned long long nplanes = (unsigned long long)N * plane;
     const unsigned dplane = (unsigned)(dd * P) * plane;  // byte offset of this lane's first plane (< N * plane)
-    const __amdgpu_buffer_rsrc_t rq = buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
-    const __amdgpu_buffer_rsrc_t rout = buf_rsrc(sp.psi + (size_t)q * N * ncell, nplanes);
+    const unsigned long long ref_bytes = REF ? (unsigned long long)(nx + 2) * (ny + 2) * sp.nq * N * 8 : 0;
+    const unsigned dref = 8u * (unsigned)(dd * P);  // byte offset of this lane's run within a cell's run
+    const __amdgpu_buffer_rsrc_t rq = REF ? buf_rsrc(sp.qref, ref_bytes)
+                                          : buf_rsrc(HASQ ? sp.qsrc + (size_t)q * N * ncell : nullptr, HASQ ? nplanes : 0);
+    const __amdgpu_buffer_rsrc_t rout = REF ? buf_rsrc(sp.psi_ref, ref_bytes) : buf_rsrc(sp.psi + (size_t)q * N * ncell, nplanes);
     DiagWalk cw = walk0;
     const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(HASPHI ? sp.phi + (size_t)Q * N * ncell : nullptr, HASPHI ? nplanes : 0);
     const __amdgpu_buffer_rsrc_t rsigs = buf_rsrc(HASPHI ? sp.sig_s + (size_t)Q * ncell : nullptr, HASPHI ? plane : 0);
@@ -141,7 +180,8 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp)
     const bool xwave = xin != nullptr && w0 == 0;
     const bool xrow = xwave && rl == 0;
 
-    T pq[D][HASQ ? P : 1], pf[D][HASPHI ? P : 1], ps[D];
+    PQ pq[D][HASQ ? P : 1];
+    T pf[D][HASPHI ? P : 1], ps[D];
     int pm[D];
     U px[D];
     DiagWalk fw = walk0;
@@ -153,7 +193,10 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp)
         const int rc = r < lo ? lo : (r > hi ? hi : r);
         const unsigned p = (unsigned)((live ? fw.off : 0) + rc);
         const unsigned bt = p * (unsigned)sizeof(T), bi = p * 4u;
-        if (HASQ) {
+        if constexpr (REF) {  // the cell (clamped into the mesh on padded steps) in the reference layout
+            const int cl = fw.s - rc < 0 ? 0 : (fw.s - rc >= nx ? nx - 1 : fw.s - rc);
+            ref_run_ld<P>(rq, ref_off(cl, rc, nx, ny, sp.nq, q, N, xneg, yneg) + dref, pq[u]);
+        } else if (HASQ) {
 #pragma unroll
             for (int e = 0; e < P; ++e) pq[u][e] = buf_ld<T>(rq, bt + dplane, e * plane);
         }
@@ -212,9 +255,9 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp)
                     T a;
                     if (HASPHI) {
                         const T wsc = T(1) * ps[u] * T(kInv4Pi);
-                        a = HASQ ? fmadd(wsc, pf[u][e], pq[u][e]) : wsc * pf[u][e];
+                        a = HASQ ? fmadd(wsc, pf[u][e], T(pq[u][e])) : wsc * pf[u][e];
                     } else {
-                        a = pq[u][e];
+                        a = T(pq[u][e]);
                     }
                     Bo[e] = fmadd(kye[e], ty, fmadd(kxo, tx[e], a));
                 }
@@ -232,9 +275,13 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp)
                     xo[e] = acc;
                 }
                 if (active && owner) {
-                    const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T) + dplane;
+                    if constexpr (REF) {
+                        ref_st<T, P>(rout, ref_off(c, r, nx, ny, sp.nq, q, N, xneg, yneg) + dref, xo);
+                    } else {
+                        const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T) + dplane;
 #pragma unroll
-                    for (int e = 0; e < P; ++e) buf_st<T>(rout, boff, e * plane, xo[e]);
+                        for (int e = 0; e < P; ++e) buf_st<T>(rout, boff, e * plane, xo[e]);
+                    }
                 }
                 T x[N];
 #pragma unroll

@@ -231,6 +231,10 @@ template <typename T>
 void launch_diag_sweep(const DiagSweepParams<T> &sp, int p, hipStream_t s);
 template <typename T>
 size_t diag_sweep_lds_bytes(const DiagSweepParams<T> &sp, int p);
+// LDS of a lane-split sweep workgroup of `rows` row slots over an nx-column mesh with M materials (0: the
+// lane-split kernel is not built for (T, p) or does not fit)
+template <typename T>
+size_t split_sweep_lds_bytes(int p, int rows, int nx, int M);
 // Batched sweep over work items {problem, ordinate}; `rep` fixes degree-independent choices (mode,
 // material path), probs/items are device arrays, lds >= diag_sweep_lds_bytes of every problem.
 template <typename T>

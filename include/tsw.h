@@ -99,7 +99,7 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * (0 default, 1): bands of at most 64 rows, each swept by a single wavefront that passes the y-traces
  * lane to lane instead of through LDS.  Key "lane_split" (0 default, 1): on the material-table path,
  * 2 (DG degree 1) or 4 (degrees 2, 3) lanes per row, each solving its share of the cell's coefficients
- * (FP32 degrees 1-3, FP64 degrees 1-2; elsewhere the one-lane kernel runs).  Key "batch_direct" (0 default or 1, -1): launch groups of
+ * (degrees 1-3; degree 0 runs the one-lane kernel).  Key "batch_direct" (0 default or 1, -1): launch groups of
  * tsw_batch_sweep store from the compute lanes instead of a storer wave (smaller workgroups, more of
  * them per CU), or keep the storer wave where the tables fit with it.  Key "autotune" (1 default, 0
  * off): while band_rows, prefetch, one_wave and lane_split are all 0, the first sweep of each kind
@@ -113,8 +113,12 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * default = 2 in FP64, 3 in FP32; or 4 in FP64, 6 in FP32): steps of inputs in flight per lane.  Key
  * "batch_ref" (0 default, 1, -1): tsw_batch_set's staggered groups on the caller's layout (no layout
  * conversions) forced, forbidden, or left to the autotuner, which times, per DG degree, the staggered
- * kernels through conversions, on the caller's layout and the ordinate-lane engine, and keeps the
- * fastest.  Key "ref_stage" (0 default, 1): sweeps on the caller's layout (batch groups,
+ * kernels through conversions, on the caller's layout, the ordinate-lane engine and the lane-split
+ * kernel on the caller's layout, and keeps the fastest.  Key "batch_split" (0 default, 1, -1): that
+ * lane-split engine (DG degrees 1-3, row bands with exchange records, a band's workgroup at most 512
+ * threads) forced, forbidden or autotuned; "batch_split_rows" (0 default = the most rows one workgroup
+ * holds: 256 at degree 1, 128 at degrees 2-3; or 16..256): rows per band; "batch_split_prefetch" (0
+ * default = 4, or 2, 3, 6): its prefetch depth.  Key "ref_stage" (0 default, 1): sweeps on the caller's layout (batch groups,
  * tsw_sweep_apply_multi) stage their source loads through LDS where it fits.  Results do not depend on
  * any of these settings (bit for bit).
  * Solver keys: "use_dsa" (0 default, 1): diffusion synthetic acceleration of tsw_gmres_solve and
@@ -136,10 +140,12 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
  * "prefetch" (its prefetch depth), "one_wave" (1 if it ran the one-wavefront kernel), "lane_split" (1 if
  * it ran the lane-split kernel), "lanes" (1 if the last sweep, or for a batch context any member, ran on
  * the ordinate-lane engine), "lanes_block" / "lanes_waves" (its shape), "batch_ref" (1 if the last
- * tsw_batch_sweep ran staggered groups on the caller's layout), "batch_engine_p0" .. "batch_engine_p3"
- * (the engine tsw_batch_set chose for that DG degree: 0 staggered through conversions, 1 staggered on the
- * caller's layout, 2 ordinate lanes, -1 none in the batch), or the setting of "batch_direct", "ref_stage",
- * "autotune", "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
+ * tsw_batch_sweep ran staggered or lane-split groups on the caller's layout), "batch_engine_p0" ..
+ * "batch_engine_p3" (the engine tsw_batch_set chose for that DG degree: 0 staggered through conversions,
+ * 1 staggered on the caller's layout, 2 ordinate lanes, 3 lane-split on the caller's layout, -1 none in
+ * the batch), or the setting of "batch_direct", "ref_stage", "batch_split", "batch_split_rows",
+ * "batch_split_prefetch" (the depth in use), "autotune", "use_dsa", "dsa_cycles", "si_graph",
+ * "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior

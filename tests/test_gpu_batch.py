@@ -124,6 +124,7 @@ def test_reference_layout_batch_bitwise_equals_converted_batch(precision):
     for ref, stage in ((-1, 0), (1, 0), (1, 1)):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", -1)
+            ctx.set_tuning("batch_split", -1)
             ctx.set_tuning("batch_ref", ref)
             ctx.set_tuning("ref_stage", stage)
             ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
@@ -148,6 +149,7 @@ def test_reference_layout_full_batch_extremes_match_oracle():
     for ref in (-1, 1):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", -1)
+            ctx.set_tuning("batch_split", -1)
             ctx.set_tuning("batch_ref", ref)
             ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
             o = [s.clone() for s in srcs]
@@ -167,15 +169,21 @@ def test_reference_layout_full_batch_extremes_match_oracle():
 
 @pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32])
 def test_batch_engines_give_the_same_bits(precision):
-    """The three batch engines (staggered kernels through the layout conversions, staggered kernels on the
-    caller's layout, ordinate-lane engine for DG degrees 0-2) and the autotuner's choice agree bit for bit."""
+    """The four batch engines (staggered kernels through the layout conversions, staggered kernels on the
+    caller's layout, ordinate-lane engine for DG degrees 0-2, lane-split kernel on the caller's layout for
+    DG degrees 1-3, in one band per member and in bands of 16 rows) and the autotuner's choice agree bit
+    for bit."""
     batch = problems.batched(seed=9, count=12, n_range=(5, 60), degrees=(0, 1, 2, 3), orders=(2, 6, 12))
     srcs = _random_halo_sources(batch, 6)
     outs = {}
-    for name, lanes, ref in (("converted", -1, -1), ("ref", -1, 1), ("lanes", 1, -1), ("auto", 0, 0)):
+    for name, lanes, ref, split, rows in (("converted", -1, -1, -1, 0), ("ref", -1, 1, -1, 0),
+                                          ("lanes", 1, -1, -1, 0), ("split", -1, -1, 1, 0),
+                                          ("split16", -1, -1, 1, 16), ("auto", 0, 0, 0, 0)):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", lanes)
             ctx.set_tuning("batch_ref", ref)
+            ctx.set_tuning("batch_split", split)
+            ctx.set_tuning("batch_split_rows", rows)
             ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
             engines = [ctx.get_tuning(f"batch_engine_p{p}") for p in range(4)]
             print(name, "engines per DG degree", engines, flush=True)
@@ -188,13 +196,111 @@ def test_batch_engines_give_the_same_bits(precision):
                     assert engines[p] == 1
                 elif name == "lanes":  # degree 3 through the conversions (the lane engine is built for 0-2)
                     assert engines[p] == (2 if p <= 2 else 0)
+                elif name.startswith("split"):  # degree 0 through the conversions (lane split: 1-3)
+                    assert engines[p] == (3 if p >= 1 else 0)
                 else:
-                    assert engines[p] in (0, 1, 2)
+                    assert engines[p] in (0, 1, 2, 3)
             out = [s.copy() for s in srcs]
             ctx.batch_sweep(out, precision=precision)
             outs[name] = out
-    for name in ("ref", "lanes", "auto"):
+    for name in ("ref", "lanes", "split", "split16", "auto"):
         for P, a, b in zip(batch, outs["converted"], outs[name]):
             np.testing.assert_array_equal(a, b, err_msg=f"{name} {P.name}")
     for P, s, o in zip(batch, srcs, outs["auto"]):
         assert rel_err(o, oracle_sweep(P, s)) <= (1e-12 if precision == tsw.FP64 else 2e-6), P.name
+
+
+def _split_ctx(rows):
+    ctx = tsw.Context(0)
+    ctx.set_tuning("lanes", -1)
+    ctx.set_tuning("batch_ref", -1)
+    ctx.set_tuning("batch_split", 1)
+    ctx.set_tuning("batch_split_rows", rows)
+    return ctx
+
+
+@pytest.mark.parametrize("rows", [0, 16])
+@pytest.mark.parametrize("precision", [tsw.FP64, tsw.FP32, tsw.MIXED])
+def test_split_batch_matches_oracle(precision, rows):
+    """batch_split = 1: the lane-split kernel sweeps each member's array in place (DG degrees 1-3; degree 0
+    members go through the conversions), in one band per member or in bands of at most 16 rows with
+    exchange records between them; two calls, so the records are reused.  TSW_MIXED takes the FP64 sweep."""
+    batch = problems.batched(seed=7, count=10, n_range=(3, 40), degrees=(0, 1, 2, 3), orders=(2, 4, 6, 8))
+    srcs = _random_halo_sources(batch, 3)
+    print("split batch", precision, rows, [(P.nx, P.p, P.nq) for P in batch], flush=True)
+    tol = {tsw.FP64: 1e-12, tsw.FP32: 2e-6, tsw.MIXED: 1e-12}[precision]
+    with _split_ctx(rows) as ctx:
+        ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
+        for p in {P.p for P in batch}:
+            assert ctx.get_tuning(f"batch_engine_p{p}") == (3 if p >= 1 else 0)
+        for rep in range(2):
+            out = [s.copy() for s in srcs]
+            ctx.batch_sweep(out, precision=precision)
+            for P, s, o in zip(batch, srcs, out):
+                e = rel_err(o, oracle_sweep(P, s))
+                assert e <= tol, (rep, P.name, P.nx, P.p, P.nq, e)
+                np.testing.assert_array_equal(o[0], s[0])
+                np.testing.assert_array_equal(o[:, -1], s[:, -1])
+
+
+def test_split_batch_misaligned_array_takes_the_conversions():
+    """A device array 8 bytes off a 16-byte boundary sends the call through the layout conversions, where
+    the lane-split groups sweep the diagonal layout: the same bits as the in-place call."""
+    import torch
+    batch = problems.batched(seed=14, count=5, n_range=(20, 50), degrees=(1, 2, 3), orders=(4, 8))
+    srcs = _random_halo_sources(batch, 9)
+    dev = torch.device("cuda:0")
+    sig = [torch.from_numpy(P.sigma_t).to(dev) for P in batch]
+    with _split_ctx(16) as ctx:
+        ctx.batch_set([problems.as_batch_member(P) for P in batch], sig)
+        for prec in (tsw.FP64, tsw.FP32):
+            aligned = [torch.from_numpy(s).to(dev) for s in srcs]
+            ctx.batch_sweep(aligned, precision=prec)
+            assert ctx.get_tuning("batch_ref") == 1
+            big = torch.zeros(srcs[0].size + 1, dtype=torch.float64, device=dev)
+            shifted = big[1:].view(srcs[0].shape)
+            shifted.copy_(torch.from_numpy(srcs[0]))
+            arrs = [shifted] + [torch.from_numpy(s).to(dev) for s in srcs[1:]]
+            ctx.batch_sweep(arrs, precision=prec)
+            assert ctx.get_tuning("batch_ref") == 0
+            torch.cuda.synchronize()
+            for P, a, b in zip(batch, aligned, arrs):
+                assert torch.equal(a, b), (prec, P.name)
+
+
+def test_split_batch_band_stall_is_reported_and_recovered():
+    """The stall path of the banded lane-split groups, forced from the host (tuning key stall_test raises
+    the stall flag before the sweep, so every bounded wait gives up within one poll interval and every
+    workgroup still exits): the call fails with TSW_E_HIP, and the next calls, after every member's
+    exchange records are refilled, match the oracle again."""
+    batch = problems.batched(seed=13, count=6, n_range=(40, 70), degrees=(1, 2, 3), orders=(4, 8))
+    srcs = _random_halo_sources(batch, 7)
+    refs = [oracle_sweep(P, s) for P, s in zip(batch, srcs)]
+    with _split_ctx(16) as ctx:
+        ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
+        for prec, tol in ((tsw.FP64, 1e-12), (tsw.FP32, 2e-6)):
+            out = [s.copy() for s in srcs]
+            ctx.batch_sweep(out, precision=prec)
+            for P, o, r in zip(batch, out, refs):
+                assert rel_err(o, r) <= tol, P.name
+            ctx.set_tuning("stall_test", 1)
+            with pytest.raises(tsw.TswError) as ei:
+                ctx.batch_sweep([s.copy() for s in srcs], precision=prec)
+            assert ei.value.code == tsw.TSW_E_HIP and "stalled" in str(ei.value)
+            for _ in range(2):
+                out = [s.copy() for s in srcs]
+                ctx.batch_sweep(out, precision=prec)
+                for P, o, r in zip(batch, out, refs):
+                    assert rel_err(o, r) <= tol, (prec, P.name)
+
+
+def test_split_batch_tuning_keys():
+    with tsw.Context(0) as ctx:
+        for key, bad in (("batch_split", 2), ("batch_split_rows", 8), ("batch_split_rows", 300),
+                         ("batch_split_prefetch", 5)):
+            with pytest.raises(tsw.TswError):
+                ctx.set_tuning(key, bad)
+        ctx.set_tuning("batch_split_prefetch", 2)
+        assert ctx.get_tuning("batch_split_prefetch") == 2
+        ctx.set_tuning("batch_split_prefetch", 0)
+        assert ctx.get_tuning("batch_split_prefetch") == 4

@@ -27,8 +27,9 @@ def _ctx(g, o, band_rows, split, prefetch=0):
 
 
 def _built(p, prec):
-    """Where the lane-split kernel exists: FP32 Q1-Q3, FP64 Q1-Q2 (a lane's rows of the inverse in registers)."""
-    return p >= 1 and (p <= 2 or prec == tsw.FP32)
+    """Where the lane-split kernel exists: Q1-Q3 in both precisions (a lane's rows of the inverse in
+    registers: FP64 Q3 holds 64 doubles per lane)."""
+    return p >= 1
 
 
 @pytest.mark.parametrize("p", [1, 2, 3])
