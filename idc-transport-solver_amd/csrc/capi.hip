@@ -18,6 +18,8 @@ enum class Pattern { WavefrontGraph, WavefrontEager, Staggered, StaggeredGE };
 
 static bool is_staggered(Pattern p) { return p == Pattern::Staggered || p == Pattern::StaggeredGE; }
 
+constexpr int kXcds = 8;  // MI355X: workgroup b runs on XCD b mod 8, each XCD with its own L2
+
 struct tsw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,7 +55,19 @@ struct tsw_ctx {
         bool keyed = false;  // key holds that copy
         unsigned long long build = 0;  // counts builds and cross-section changes
     } mat;
-    DBuf same_flag;  // device flag of the input change checks
+    DBuf same_flag;  // device flags of the input change checks
+    // input checks made ahead in one round trip (precheck_keys), each consumed by the same_as_key call on
+    // the same key, array and length within the call that made them
+    struct KeyCheck {
+        const DBuf *key;
+        const double *a;
+        size_t n;
+        bool same;
+    };
+    std::vector<KeyCheck> key_checks;
+    // Krylov workspace kept across solves: basis, Gram-Schmidt coefficients, partial sums, the update
+    // coefficients, and the solver's right-hand side, iterate and residual
+    DBuf gm_v, gm_h, gm_part, gm_y, gm_vecs;
     // Cross-section degree (tsw_set_xs_degree): 0, cell means; p, Legendre coefficients per cell, used
     // through their triple-product tensors, built once per call and per coefficient array (two slots,
     // keyed by device pointer, cleared at every entry point), plus their cell means for DSA.
@@ -181,6 +195,9 @@ struct tsw_ctx {
     // batch engines: 1 / -1 force / forbid the lane-split groups (tuning key batch_split); their rows per
     // band (batch_split_rows, 0: the most one workgroup holds) and prefetch depth (batch_split_prefetch)
     int batch_split_opt = 0, batch_split_rows_opt = 0, batch_split_pf_opt = 0;
+    // lane-split groups: runs of this many consecutive items go to one XCD (tuning key batch_xcd_tile;
+    // 0 or 1: dispatch order)
+    int batch_xcd_tile_opt = 0;
     int ref_stage_opt = 0;   // 1: reference-layout sweeps stage their source loads through LDS where it fits
     // Reflecting edges (tsw_set_boundaries): mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max; the mirror map
     // [2][nq] (OrdinateSet::ReflectMap) on the device
@@ -458,9 +475,18 @@ SweepParams<float> params(tsw_ctx *c, float *psi, const float *sigma, double dt)
     return sp;
 }
 
-// True when device array a (n doubles) equals the copy held in key, checked on the device; otherwise
-// key receives a copy of a and the result is false.
+// True when device array a (n doubles) equals the copy held in key, checked on the device (or by an
+// earlier precheck_keys of this call); otherwise key receives a copy of a and the result is false.
 bool same_as_key(tsw_ctx *c, const double *a, size_t n, DBuf &key, bool keyed) {
+    for (size_t k = 0; k < c->key_checks.size(); ++k) {
+        const auto kc = c->key_checks[k];
+        if (kc.key != &key || kc.a != a || kc.n != n) continue;
+        c->key_checks.erase(c->key_checks.begin() + k);
+        if (kc.same) return true;
+        TSW_HIP(hipMemcpyAsync(key.reserve(n * sizeof(double)), a, n * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+        return false;
+    }
     if (keyed && key.bytes >= n * sizeof(double)) {
         int *flag = static_cast<int *>(c->same_flag.reserve(sizeof(int)));
         launch_differs(n, a, key.as<double>(), flag, c->stream);
@@ -471,6 +497,35 @@ bool same_as_key(tsw_ctx *c, const double *a, size_t n, DBuf &key, bool keyed) {
     }
     TSW_HIP(hipMemcpyAsync(key.reserve(n * sizeof(double)), a, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     return false;
+}
+
+// The checks of same_as_key for several (array, length, key, keyed) at once, with one host round trip;
+// the results wait in key_checks for the same_as_key calls of the current solve (KeyCheckScope clears
+// what is left when it ends).
+struct KeyCheckScope {
+    tsw_ctx *c;
+    explicit KeyCheckScope(tsw_ctx *c_) : c(c_) { c->key_checks.clear(); }
+    ~KeyCheckScope() { c->key_checks.clear(); }
+};
+void precheck_keys(tsw_ctx *c, const std::vector<std::tuple<const double *, size_t, DBuf *, bool>> &items) {
+    std::vector<int> run;
+    for (size_t k = 0; k < items.size(); ++k) {
+        const auto &[a, n, key, keyed] = items[k];
+        if (keyed && key->bytes >= n * sizeof(double)) run.push_back((int)k);
+    }
+    if (run.empty()) return;
+    int *flags = static_cast<int *>(c->same_flag.reserve(sizeof(int) * items.size()));
+    for (int k : run) {
+        const auto &[a, n, key, keyed] = items[k];
+        launch_differs(n, a, key->as<double>(), flags + k, c->stream);
+    }
+    std::vector<int> h(items.size(), 1);
+    TSW_HIP(hipMemcpyAsync(h.data(), flags, sizeof(int) * items.size(), hipMemcpyDeviceToHost, c->stream));
+    TSW_HIP(hipStreamSynchronize(c->stream));
+    for (int k : run) {
+        const auto &[a, n, key, keyed] = items[k];
+        c->key_checks.push_back({key, a, n, h[k] == 0});
+    }
 }
 
 // Builds the material table from device sigma_t (double cell array) for time step dt: reads sigma
@@ -836,7 +891,6 @@ struct Gmres {
     tsw_ctx *c;
     size_t n;
     int m;
-    DBuf vbuf, hbuf, pbuf;
     double *V;     // (m+1) x n, halos kept zero
     double *hdev;  // [2][m+2]: the two Gram-Schmidt passes' coefficients; slot j+1 of each holds
                    // ||w|| and ||w||^2 of step j
@@ -844,13 +898,16 @@ struct Gmres {
     double *part;
     int nb;
 
+    double *ydev;  // the update coefficients, negated, for one multi_axpy
+    // the workspace is the context's (kept across solves, so a solve allocates nothing once warm)
     Gmres(tsw_ctx *ctx, size_t n_, int m_) : c(ctx), n(n_), m(m_) {
-        V = static_cast<double *>(vbuf.reserve(sizeof(double) * n * (m + 1)));
+        V = static_cast<double *>(c->gm_v.reserve(sizeof(double) * n * (m + 1)));
         TSW_HIP(hipMemsetAsync(V, 0, sizeof(double) * n * (m + 1), c->stream));
-        hdev = static_cast<double *>(hbuf.reserve(sizeof(double) * 2 * (m + 2)));
+        hdev = static_cast<double *>(c->gm_h.reserve(sizeof(double) * 2 * (m + 2)));
         nb = reduction_blocks();
-        part = static_cast<double *>(pbuf.reserve(sizeof(double) * nb * (m + 2)));
-        const size_t need = 2 * (size_t)(m + 2);
+        part = static_cast<double *>(c->gm_part.reserve(sizeof(double) * nb * (m + 2)));
+        ydev = static_cast<double *>(c->gm_y.reserve(sizeof(double) * m));
+        const size_t need = 3 * (size_t)(m + 2);
         if (c->gm_host_n < need) {
             if (c->gm_host) TSW_HIP(hipHostFree(c->gm_host));
             c->gm_host = nullptr;
@@ -957,7 +1014,12 @@ struct Gmres {
                 for (int k = i + 1; k < jdone; ++k) s -= H[i + (m + 1) * k] * y[k];
                 y[i] = s / H[i + (m + 1) * i];
             }
-            for (int i = 0; i < jdone; ++i) launch_axpby(n, y[i], V + (size_t)i * n, 1.0, x, c->stream);
+            if (jdone > 0) {  // x += V y in one pass (multi_axpy subtracts: it takes -y)
+                double *yh = hh + 2 * (m + 2);
+                for (int i = 0; i < jdone; ++i) yh[i] = -y[i];
+                TSW_HIP(hipMemcpyAsync(ydev, yh, sizeof(double) * jdone, hipMemcpyHostToDevice, c->stream));
+                launch_multi_axpy(n, jdone, V, n, ydev, x, c->stream);
+            }
             if (conv) return true;
             if (its >= max_its) {
                 sync(c);
@@ -1793,8 +1855,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
     hipStream_t s = c->stream;
     const size_t n = g.nphi();
     const size_t npsi_d = (size_t)g.nq * g.N() * g.nx * g.ny;
-    DBuf vecs;
-    double *b = dbuf<double>(vecs, 3 * n);
+    double *b = dbuf<double>(c->gm_vecs, 3 * n);
     double *x = b + n, *r = x + n;
     TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * 3 * n, s));
     Gmres solver(c, n, restart);
@@ -1803,6 +1864,18 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
     // left DSA preconditioning, as PETSc applies the reference's shell preconditioner (the stop test is
     // then on the preconditioned residual, PETSc's default norm)
     const bool dsa = c->use_dsa;
+    // the unchanged-input checks of the material table and the DSA hierarchy in one round trip
+    KeyCheckScope checks(c);
+    {
+        const size_t nh = g.ncell_h();
+        std::vector<std::tuple<const double *, size_t, DBuf *, bool>> items;
+        if (c->pattern == Pattern::Staggered) items.emplace_back(st, nh, &c->mat.key, c->mat.keyed);
+        if (dsa) {
+            items.emplace_back(st, nh, &c->dsa.key_st, c->dsa.keyed);
+            items.emplace_back(ss, nh, &c->dsa.key_ss, c->dsa.keyed);
+        }
+        precheck_keys(c, items);
+    }
     if (dsa) dsa_build(c, st, ss, dt);
     if (precision == TSW_FP64) {
         diag_setup(c, st, ss, Q, dt, true, false);
@@ -1942,8 +2015,7 @@ bool gmres_wavefront(tsw_ctx *ctx, double *Q, const double *st, const double *ss
     const double *xt64 = xs_tensor<double>(ctx, st), *xss64 = xs_tensor<double>(ctx, ss);
     double *psi = static_cast<double *>(ctx->psi64.reserve(sizeof(double) * npsi));
     // vectors: b, x, r (phi-shaped, zero halos)
-    DBuf vecs;
-    double *b = static_cast<double *>(vecs.reserve(sizeof(double) * n * 3));
+    double *b = static_cast<double *>(ctx->gm_vecs.reserve(sizeof(double) * n * 3));
     double *x = b + n, *r = x + n;
     TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * n * 3, s));
     Gmres solver(ctx, n, restart);
@@ -2048,8 +2120,7 @@ bool gmres_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, doub
     const Geom &g = c->g;
     hipStream_t s = c->stream;
     const size_t n = g.nphi();
-    DBuf vecs;
-    double *b = dbuf<double>(vecs, 3 * n);
+    double *b = dbuf<double>(c->gm_vecs, 3 * n);
     double *x = b + n, *r = x + n;
     TSW_HIP(hipMemsetAsync(b, 0, sizeof(double) * 3 * n, s));
     Gmres solver(c, n, restart);
@@ -2906,6 +2977,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "batch_split_rows") {
             require(value == 0 || (value >= 16 && value <= 256), TSW_E_ARG, "batch_split_rows must be 0 or 16..256");
             ctx->batch_split_rows_opt = value;
+        } else if (k == "batch_xcd_tile") {
+            require(value >= 0 && value <= 64, TSW_E_ARG, "batch_xcd_tile must be 0..64");
+            ctx->batch_xcd_tile_opt = value;
         } else if (k == "batch_split_prefetch") {
             require(value == 0 || value == 2 || value == 3 || value == 4 || value == 6, TSW_E_ARG,
                     "batch_split_prefetch must be 0, 2, 3, 4 or 6");
@@ -2964,6 +3038,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "ref_stage") *value = ctx->ref_stage_opt;
         else if (k == "batch_split") *value = ctx->batch_split_opt;
         else if (k == "batch_split_rows") *value = ctx->batch_split_rows_opt;
+        else if (k == "batch_xcd_tile") *value = ctx->batch_xcd_tile_opt;
         else if (k == "batch_split_prefetch")
             *value = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
@@ -3876,9 +3951,28 @@ void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
                     w.emplace_back(band, -(long long)(g.nx + g.ny) * g.ny, slot, band * g.nq + q);
         }
         std::sort(w.begin(), w.end());
-        for (auto &t : w) {
-            G.items.push_back(std::get<2>(t));
-            G.items.push_back(std::get<3>(t));
+        const int T = G.split ? ctx->batch_xcd_tile_opt : 0;
+        if (T <= 1) {
+            for (auto &t : w) {
+                G.items.push_back(std::get<2>(t));
+                G.items.push_back(std::get<3>(t));
+            }
+        } else {
+            // Within each band's run of items (so the order stays band-major), runs of T consecutive items
+            // (neighbouring ordinates of one member, whose cells share cache lines in the caller's layout)
+            // go to one XCD and the runs go round the XCDs: workgroup k of the run (padded to a multiple
+            // of kXcds * T with empty items) takes item ((m / T) * kXcds + k % kXcds) * T + m % T, m = k / kXcds.
+            for (size_t s0 = 0; s0 < w.size();) {
+                size_t s1 = s0;
+                while (s1 < w.size() && std::get<0>(w[s1]) == std::get<0>(w[s0])) ++s1;
+                const size_t L = s1 - s0, C = (size_t)kXcds * T, Lp = (L + C - 1) / C * C;
+                for (size_t k = 0; k < Lp; ++k) {
+                    const size_t m = k / kXcds, r = k % kXcds, j = ((m / T) * kXcds + r) * T + m % T;
+                    G.items.push_back(j < L ? std::get<2>(w[s0 + j]) : -1);
+                    G.items.push_back(j < L ? std::get<3>(w[s0 + j]) : 0);
+                }
+                s0 = s1;
+            }
         }
         TSW_HIP(hipMemcpyAsync(G.dev_items.reserve(sizeof(int) * G.items.size()), G.items.data(),
                                sizeof(int) * G.items.size(), hipMemcpyHostToDevice, ctx->stream));

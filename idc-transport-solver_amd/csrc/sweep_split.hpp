@@ -20,7 +20,7 @@
 // (whole 16-byte accesses for even P) and the LG lanes of a row cover the cell's run, so no layout
 // conversion is needed and a wavefront's accesses are RPW whole cell runs.  BATCH: the workgroup's
 // parameters are batch[items[b].x], and items[b].y = band * nq + ordinate, numbered band-major so that a
-// band's upstream band is always dispatched first.
+// band's upstream band is always dispatched first; items with a negative problem index are padding.
 #pragma once
 
 namespace tsw {
@@ -113,6 +113,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1
         const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)blockIdx.x;
         item_p = it[0];
         item_v = it[1];
+        if (item_p < 0) return;  // padding of an XCD-ordered item list
     }
     const DiagSweepParams<T> sp = BATCH ? batch[item_p] : sp1;
     if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
