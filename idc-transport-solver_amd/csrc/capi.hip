@@ -78,14 +78,16 @@ struct tsw_ctx {
         bool have32[2] = {false, false}, have_mean[2] = {false, false};
         int next = 0;
     } xs;
-    // relative error of an FP32-sweep GMRES product, measured for the material table build `mat_build`,
-    // 1/dt and the DSA setting (value < 0: not measured)
+    // relative errors of an FP32-sweep GMRES product, measured per material table build `mat_build`, 1/dt
+    // and DSA setting (an IDC step solves with a few different substep lengths), the latest kFp32ErrKeep
     struct Fp32Err {
         double value = -1.0;
         unsigned long long mat_build = 0;
         double inv_dt = -1.0;
         bool dsa = false;
-    } fp32_err;
+    };
+    static constexpr size_t kFp32ErrKeep = 16;
+    std::vector<Fp32Err> fp32_err;
     static constexpr int kMaxMaterials = 64;
 
     // Inputs and work arrays of the staggered (diagonal-layout) sweep, per precision where needed:
@@ -886,6 +888,17 @@ void dsa_apply(tsw_ctx *c, const double *x, double *y) {
 constexpr double kFp32ErrSafety = 10.0;
 constexpr double kFp32ErrFloor = 1e-7;
 constexpr double kFp32ErrCeil = 1e-4;
+
+// The measured-error entry of (material table build, dt, DSA setting), created unmeasured (value < 0) when
+// absent; the oldest entry goes when kFp32ErrKeep are held.
+tsw_ctx::Fp32Err &fp32_err_entry(tsw_ctx *c, double dt, bool dsa) {
+    const double idt = inverse_dt(dt);
+    for (auto &e : c->fp32_err)
+        if (e.mat_build == c->mat.build && e.inv_dt == idt && e.dsa == dsa) return e;
+    if (c->fp32_err.size() >= tsw_ctx::kFp32ErrKeep) c->fp32_err.erase(c->fp32_err.begin());
+    c->fp32_err.push_back({-1.0, c->mat.build, idt, dsa});
+    return c->fp32_err.back();
+}
 
 struct Gmres {
     tsw_ctx *c;
@@ -1916,8 +1929,8 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
         diag_pmv<double>(c, psi_d, b, 1.0, 0.0);
         if (dsa) dsa_apply(c, b, b);
         // measured FP32 product error, once per material table, dt and DSA setting
-        auto &fe = c->fp32_err;
-        if (!(fe.value >= 0.0 && fe.mat_build == c->mat.build && fe.inv_dt == inverse_dt(dt) && fe.dsa == dsa)) {
+        tsw_ctx::Fp32Err &fe = fp32_err_entry(c, dt, dsa);
+        if (!(fe.value >= 0.0)) {
             const double nb = solver.norm(b);
             double d = kFp32ErrFloor;
             if (nb > 0.0) {
@@ -1927,7 +1940,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
                 d = solver.norm(r) / nb;
                 TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
             }
-            fe = {std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d)), c->mat.build, inverse_dt(dt), dsa};
+            fe.value = std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d));
         }
         const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
         converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
@@ -2147,8 +2160,8 @@ bool gmres_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, doub
         converged = solver.solve(apply64, b, x, r, abs_tol, max_its, its_total, res_final, true);
         lanes_sweep<double>(c, dt, Q, x, ss, Q, nullptr);  // psi = L^{-1}(Q + S phi) (:504-505)
     } else {
-        auto &fe = c->fp32_err;
-        if (!(fe.value >= 0.0 && fe.mat_build == c->mat.build && fe.inv_dt == inverse_dt(dt) && fe.dsa == dsa)) {
+        tsw_ctx::Fp32Err &fe = fp32_err_entry(c, dt, dsa);
+        if (!(fe.value >= 0.0)) {
             const double nb = solver.norm(b);
             double d = kFp32ErrFloor;
             if (nb > 0.0) {
@@ -2158,7 +2171,7 @@ bool gmres_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, doub
                 d = solver.norm(r) / nb;
                 TSW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
             }
-            fe = {std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d)), c->mat.build, inverse_dt(dt), dsa};
+            fe.value = std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d));
         }
         const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
         converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
