@@ -130,6 +130,7 @@ struct tsw_ctx {
         // 16-byte aligned): members in row bands of at most `rows` rows, items {slot, band * nq + ordinate}
         bool split = false;
         int rows = 0;
+        DBuf work;  // persistent launch: the item counter
         std::vector<int> band_rows;  // per member slot, fixed when the items are built
     };
     // Engine of each batch member, chosen per DG degree (tsw_batch_set): the staggered kernels through
@@ -200,6 +201,9 @@ struct tsw_ctx {
     // lane-split groups: runs of this many consecutive items go to one XCD (tuning key batch_xcd_tile;
     // 0 or 1: dispatch order)
     int batch_xcd_tile_opt = 0;
+    // lane-split groups: one persistent launch per group, workgroups taking items from a device counter
+    // (tuning key batch_persistent, 1 default; 0: one workgroup per item)
+    int batch_persistent_opt = 1;
     int ref_stage_opt = 0;   // 1: reference-layout sweeps stage their source loads through LDS where it fits
     // Reflecting edges (tsw_set_boundaries): mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max; the mirror map
     // [2][nq] (OrdinateSet::ReflectMap) on the device
@@ -2990,6 +2994,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "batch_split_rows") {
             require(value == 0 || (value >= 16 && value <= 256), TSW_E_ARG, "batch_split_rows must be 0 or 16..256");
             ctx->batch_split_rows_opt = value;
+        } else if (k == "batch_persistent") {
+            require(value == 0 || value == 1, TSW_E_ARG, "batch_persistent must be 0 or 1");
+            ctx->batch_persistent_opt = value;
         } else if (k == "batch_xcd_tile") {
             require(value >= 0 && value <= 64, TSW_E_ARG, "batch_xcd_tile must be 0..64");
             ctx->batch_xcd_tile_opt = value;
@@ -3052,6 +3059,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "batch_split") *value = ctx->batch_split_opt;
         else if (k == "batch_split_rows") *value = ctx->batch_split_rows_opt;
         else if (k == "batch_xcd_tile") *value = ctx->batch_xcd_tile_opt;
+        else if (k == "batch_persistent") *value = ctx->batch_persistent_opt;
         else if (k == "batch_split_prefetch")
             *value = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
@@ -3945,6 +3953,7 @@ void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
             G.mat = sub->mat.valid;
             G.ref = e == tsw_ctx::kEngRef || split;
             G.split = split;
+            if (split) G.work.reserve(sizeof(int));
         }
         auto &G = ctx->groups[it->second];
         G.members.push_back(b);
@@ -4044,6 +4053,10 @@ void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
     if (G.split) {
         rep.split = 1;
         rep.prefetch = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
+        if (ctx->batch_persistent_opt > 0 && G.work.ptr) {  // the counter starts each launch at zero
+            rep.work = G.work.as<int>();
+            TSW_HIP(hipMemsetAsync(rep.work, 0, sizeof(int), sd));
+        }
     }
     launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2,
                                G.split ? G.rows : G.max_ny, lds, G.p, sd);

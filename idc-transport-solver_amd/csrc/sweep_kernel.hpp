@@ -645,14 +645,22 @@ void launch_ds_d(const DiagSweepParams<T> &sp, size_t lds, hipStream_t s, const 
                 auto kern = k_diag_sweep_split<T, P, MODE, D, true>;
                 if (lds > 64 * 1024)
                     TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(kern, dim3(nitems), dim3(threads), lds, s, sp, batch, items);
+                int grid = nitems;
+                if (sp.work) {  // persistent: as many workgroups as the device holds at once (the counter is zeroed)
+                    int per_cu = 0, dev = 0, cus = 0;
+                    TSW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, threads, lds));
+                    TSW_HIP(hipGetDevice(&dev));
+                    TSW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+                    grid = std::max(1, std::min(nitems, std::max(per_cu, 1) * cus));
+                }
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, sp, batch, items, sp.work, nitems);
             } else {
                 if constexpr ((MODE & 4) != 0) throw Error(TSW_E_ARG, "reference-layout lane-split sweep is built for batches");
                 auto kern = k_diag_sweep_split<T, P, MODE, D, false>;
                 if (lds > 64 * 1024)
                     TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 hipLaunchKernelGGL(kern, dim3(sp.nq * (sp.nbands > 1 ? sp.nbands : 1)), dim3(threads), lds, s, sp,
-                                   nullptr, nullptr);
+                                   nullptr, nullptr, nullptr, 0);
             }
             TSW_HIP(hipGetLastError());
             return;

@@ -98,9 +98,10 @@ __device__ __forceinline__ void ref_run_ld(__amdgpu_buffer_rsrc_t r, unsigned of
     }
 }
 
-template <typename T, int P, int MODE, int D, bool BATCH>
-__global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
-                                                          const int2 *__restrict__ items) {
+// One item of the lane-split sweep: band wid / nq of ordinate wid % nq of the problem sp, by the whole
+// workgroup (every thread calls it; it ends with a workgroup barrier).
+template <typename T, int P, int MODE, int D>
+__device__ __forceinline__ void split_item(const DiagSweepParams<T> &sp, unsigned wid) {
     constexpr int N = P * P, LG = split_lanes<P>(), RPW = 64 / LG;  // rows per wavefront
     using Lds = SplitLds<T, P>;
     constexpr int PS = Lds::PS, RS = Lds::RS;
@@ -108,17 +109,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0, REF = (MODE & 4) != 0;
     static_assert(!REF || MODE == 5, "reference-layout lane-split sweep: sources only");
     using PQ = typename std::conditional<REF, double, T>::type;  // prefetched source (converted at use)
-    int item_p = 0, item_v = 0;
-    if constexpr (BATCH) {
-        const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)blockIdx.x;
-        item_p = it[0];
-        item_v = it[1];
-        if (item_p < 0) return;  // padding of an XCD-ordered item list
-    }
-    const DiagSweepParams<T> sp = BATCH ? batch[item_p] : sp1;
-    if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
     const int nb = sp.nbands < 2 ? 1 : sp.nbands;
-    const unsigned wid = BATCH ? (unsigned)item_v : blockIdx.x;  // band * nq + ordinate
     const int q = (int)(wid % (unsigned)sp.nq);
     const int band = (int)(wid / (unsigned)sp.nq);
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -308,6 +299,41 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1
             fetch(u);
             __builtin_amdgcn_sched_barrier(0);
             lds_barrier();
+        }
+    }
+}
+
+// One problem (sp1, workgroup = band x ordinate), a batch with one item per workgroup (items[b] = {problem,
+// band * nq + ordinate}, a negative problem marking padding), or a persistent batch launch (work != null):
+// each workgroup takes the next item index from *work until nwork are taken, so items start in list
+// order (band-major: a band's upstream band has always been taken, by a running workgroup, before it).
+template <typename T, int P, int MODE, int D, bool BATCH>
+__global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1, const DiagSweepParams<T> *__restrict__ batch,
+                                                          const int2 *__restrict__ items, int *__restrict__ work, int nwork) {
+    if constexpr (!BATCH) {
+        if (sp1.skip && *G(sp1.skip)) return;  // a replayed iteration after the loop has stopped
+        split_item<T, P, MODE, D>(sp1, blockIdx.x);
+    } else if (work == nullptr) {
+        const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)blockIdx.x;
+        const int item_p = it[0], item_v = it[1];
+        if (item_p < 0) return;  // padding of an XCD-ordered item list
+        const DiagSweepParams<T> sp = batch[item_p];
+        if (sp.skip && *G(sp.skip)) return;
+        split_item<T, P, MODE, D>(sp, (unsigned)item_v);
+    } else {
+        __shared__ int next;
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(work, 1);
+            __syncthreads();
+            const int k = next;
+            __syncthreads();  // every thread has read `next` before it is taken again
+            if (k >= nwork) break;  // workgroup-uniform: every wave leaves
+            const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)k;
+            const int item_p = it[0], item_v = it[1];
+            if (item_p < 0) continue;
+            const DiagSweepParams<T> sp = batch[item_p];
+            if (sp.skip && *G(sp.skip)) continue;
+            split_item<T, P, MODE, D>(sp, (unsigned)item_v);
         }
     }
 }

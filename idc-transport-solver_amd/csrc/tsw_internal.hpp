@@ -139,6 +139,9 @@ template <typename T> struct DiagSweepParams {
     const double *qref;
     double *psi_ref;
     int ref_stage;         // 1: reference-layout sources loaded through a per-wave LDS stage (coalesced)
+    // Batched lane-split launch only (host-side dispatch): work counter of a persistent launch, whose
+    // workgroups take items in order from *work until all are taken (null: one item per workgroup)
+    int *work;
 };
 
 // Ordinate-lane sweep on the reference layout (sweep_lanes.hpp).  An ordinate block is ordinates

@@ -171,21 +171,23 @@ def test_reference_layout_full_batch_extremes_match_oracle():
 def test_batch_engines_give_the_same_bits(precision):
     """The four batch engines (staggered kernels through the layout conversions, staggered kernels on the
     caller's layout, ordinate-lane engine for DG degrees 0-2, lane-split kernel on the caller's layout for
-    DG degrees 1-3, in one band per member, in bands of 16 rows, and with runs of 3 items per XCD) and the
-    autotuner's choice agree bit for bit."""
+    DG degrees 1-3, in one band per member, in bands of 16 rows, and with runs of 3 items per XCD; in a
+    persistent launch taking items from a counter and with one workgroup per item) and the autotuner's
+    choice agree bit for bit."""
     batch = problems.batched(seed=9, count=12, n_range=(5, 60), degrees=(0, 1, 2, 3), orders=(2, 6, 12))
     srcs = _random_halo_sources(batch, 6)
     outs = {}
-    for name, lanes, ref, split, rows, xcd in (("converted", -1, -1, -1, 0, 0), ("ref", -1, 1, -1, 0, 0),
-                                               ("lanes", 1, -1, -1, 0, 0), ("split", -1, -1, 1, 0, 0),
-                                               ("split16", -1, -1, 1, 16, 0), ("split16xcd", -1, -1, 1, 16, 3),
-                                               ("auto", 0, 0, 0, 0, 0)):
+    for name, lanes, ref, split, rows, xcd, pers in (
+            ("converted", -1, -1, -1, 0, 0, 1), ("ref", -1, 1, -1, 0, 0, 1), ("lanes", 1, -1, -1, 0, 0, 1),
+            ("split", -1, -1, 1, 0, 0, 1), ("split16", -1, -1, 1, 16, 0, 1), ("split16xcd", -1, -1, 1, 16, 3, 1),
+            ("split16np", -1, -1, 1, 16, 0, 0), ("split16xcdnp", -1, -1, 1, 16, 3, 0), ("auto", 0, 0, 0, 0, 0, 1)):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", lanes)
             ctx.set_tuning("batch_ref", ref)
             ctx.set_tuning("batch_split", split)
             ctx.set_tuning("batch_split_rows", rows)
             ctx.set_tuning("batch_xcd_tile", xcd)
+            ctx.set_tuning("batch_persistent", pers)
             ctx.batch_set([problems.as_batch_member(P) for P in batch], [P.sigma_t for P in batch])
             engines = [ctx.get_tuning(f"batch_engine_p{p}") for p in range(4)]
             print(name, "engines per DG degree", engines, flush=True)
@@ -205,7 +207,7 @@ def test_batch_engines_give_the_same_bits(precision):
             out = [s.copy() for s in srcs]
             ctx.batch_sweep(out, precision=precision)
             outs[name] = out
-    for name in ("ref", "lanes", "split", "split16", "split16xcd", "auto"):
+    for name in ("ref", "lanes", "split", "split16", "split16xcd", "split16np", "split16xcdnp", "auto"):
         for P, a, b in zip(batch, outs["converted"], outs[name]):
             np.testing.assert_array_equal(a, b, err_msg=f"{name} {P.name}")
     for P, s, o in zip(batch, srcs, outs["auto"]):
@@ -299,7 +301,8 @@ def test_split_batch_band_stall_is_reported_and_recovered():
 def test_split_batch_tuning_keys():
     with tsw.Context(0) as ctx:
         for key, bad in (("batch_split", 2), ("batch_split_rows", 8), ("batch_split_rows", 300),
-                         ("batch_split_prefetch", 5), ("batch_xcd_tile", -1), ("batch_xcd_tile", 65)):
+                         ("batch_split_prefetch", 5), ("batch_xcd_tile", -1), ("batch_xcd_tile", 65),
+                         ("batch_persistent", 2)):
             with pytest.raises(tsw.TswError):
                 ctx.set_tuning(key, bad)
         ctx.set_tuning("batch_split_prefetch", 2)
