@@ -147,7 +147,13 @@ struct tsw_ctx {
         std::vector<unsigned char> host;
         std::vector<int> items;
         DBuf params, dev_items, traces, stage;
+        DBuf xtr64, xtr32, work;  // lane-split sweeps: exchange records per source, persistent-launch counter
+        size_t xtr64_n = 0, xtr32_n = 0;
     } multi;
+    // Sweeps of this problem on the caller's layout by the lane-split kernel where it is built and its
+    // tables fit (tuning key ref_split: 1 default, -1 off): tsw_sweep_apply (device arrays, 16-byte aligned)
+    // and tsw_sweep_apply_multi
+    int ref_split_opt = 1;
     std::vector<Group> groups;
     // Batch members on the ordinate-lane engine, swept in place on the caller's arrays: one launch per DG
     // degree over (member, ordinate block, band) items, band-major so that every band's upstream band is
@@ -413,6 +419,9 @@ void stage_out(tsw_ctx *c, T *p, const T *d, size_t n, int mem) {
 void refill_exchange(tsw_ctx *c) {
     if (c->xtr64_n) launch_exchange_fill<double>(c->xtr64.as<double>(), c->xtr64_n, c->stream);
     if (c->xtr32_n) launch_exchange_fill<float>(c->xtr32.as<float>(), c->xtr32_n, c->stream);
+    auto &M = c->multi;  // the per-source records of lane-split sweeps on the caller's layout
+    if (M.xtr64_n) launch_exchange_fill<double>(M.xtr64.as<double>(), M.xtr64_n, c->stream);
+    if (M.xtr32_n) launch_exchange_fill<float>(M.xtr32.as<float>(), M.xtr32_n, c->stream);
 }
 
 // Waits for the context's stream; reports (and clears) a row-band stall of this context or of its
@@ -2367,7 +2376,10 @@ void ref_launch(tsw_ctx *c, const DiagSweepParams<T> &base, size_t lds, const st
 // one launch of the staggered kernel on the caller's layout, the material table and element inverses
 // shared by every source; counted as n sweeps.
 template <typename T>
+bool try_split_ref(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt);
+template <typename T>
 void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt) {
+    if (try_split_ref<T>(c, arr, sig, dt)) return;
     size_t lds = 0;
     const DiagSweepParams<T> base = ref_plan<T>(c, sig, dt, lds);
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
@@ -2377,10 +2389,135 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
     c->sweeps_done += (long long)arr.size();
 }
 
+// Lanes per mesh row of the lane-split kernel for DG degree p (split_lanes<p + 1>).
+int split_lanes_of(int p) { return p == 1 ? 2 : 4; }
+
+// Rows per band of a problem (a batch member, or a single problem) on the lane-split kernel: the fewest bands of at most R rows (R: the
+// tuning key batch_split_rows, else the rows one workgroup of 512 threads holds), split evenly.
+int split_band_rows(const tsw_ctx *ctx, int p, int ny) {
+    int R = 512 / split_lanes_of(p);
+    if (ctx->batch_split_rows_opt > 0) R = std::min(R, ctx->batch_split_rows_opt);
+    const int nb = (ny + R - 1) / R;
+    return (ny + nb - 1) / nb;
+}
+
 // Whether this context's sweeps can run the staggered kernel on the caller's layout for `psi`.
 bool ref_sweep_ok(tsw_ctx *c, const double *psi) {
     return staggered_ok(c) && c->pattern == Pattern::Staggered && c->xs_deg == 0 &&
            c->g.npsi() * sizeof(double) < (size_t(1) << 32) && c->g.ny <= 448 && ((uintptr_t)psi & 15) == 0;
+}
+
+// Plan of a lane-split sweep on the caller's layout (the batch engine's kernel for one problem): false
+// when it is not built for the degree, the element-inverse tables are missing or the LDS does not fit.
+// Sets up the material table and inverses of sig for dt; base has the bands of split_band_rows rows.
+template <typename T>
+bool split_ref_plan(tsw_ctx *c, const double *sig, double dt, DiagSweepParams<T> &base, size_t &lds) {
+    const Geom &g = c->g;
+    if (c->ref_split_opt < 0 || g.p < 1) return false;
+    // a set band_rows, prefetch, one_wave or lane_split asks for the diagonal-layout kernels
+    if (c->band_rows_opt != 0 || c->prefetch_opt != 0 || c->one_wave_opt != 0 || c->split_opt != 0) return false;
+    const bool f64 = sizeof(T) == 8;
+    diag_setup(c, sig, nullptr, nullptr, dt, f64, !f64);
+    base = make_diag_params<T>(c, dt, nullptr, nullptr, false, nullptr);
+    if (!base.ainv) return false;
+    const int br = split_band_rows(c, g.p, g.ny);
+    lds = split_sweep_lds_bytes<T>(g.p, br, g.nx, base.M);
+    if (lds == 0) return false;
+    base.split = 1;
+    base.direct = 0;
+    base.wave1 = 0;
+    base.ref_stage = 0;
+    base.nbands = (g.ny + br - 1) / br;
+    base.band_rows = base.nbands > 1 ? br : g.ny;
+    base.xtr = nullptr;
+    base.abort_flag = nullptr;
+    return true;
+}
+
+// One persistent launch of the lane-split kernel sweeping every ordinate of each array arr[r] in place
+// (reference layout, 16-byte aligned), with base from split_ref_plan: items {source, band * nq + ordinate},
+// band-major; per-source inflow traces from the arrays' halos and per-source exchange records.  Counted
+// as arr.size() sweeps.
+template <typename T>
+void split_ref_sweep(tsw_ctx *c, const DiagSweepParams<T> &base, size_t lds, const std::vector<double *> &arr) {
+    const Geom &g = c->g;
+    const int n = (int)arr.size(), nb = base.nbands;
+    const size_t ntx = (size_t)g.nq * g.ny * (g.p + 1), nty = (size_t)g.nq * g.nx * (g.p + 1);
+    T *tr = static_cast<T *>(c->multi.traces.reserve(sizeof(T) * (ntx + nty) * n));
+    auto &M = c->multi;
+    hipStream_t s = c->stream;
+    const size_t per = nb > 1 ? (size_t)g.nq * (nb - 1) * g.nx * (g.p + 1) : 0;  // records of one source
+    T *xtr = nullptr;
+    if (per) {
+        DBuf &xb = sizeof(T) == 8 ? M.xtr64 : M.xtr32;
+        size_t &have = sizeof(T) == 8 ? M.xtr64_n : M.xtr32_n;
+        if (have < per * n) {  // every record empty (the sentinel); a sweep leaves them empty again
+            xb.reserve(sizeof(T) * per * n);
+            have = per * n;
+            launch_exchange_fill<T>(xb.as<T>(), have, s);
+        }
+        xtr = xb.as<T>();
+    }
+    M.host.resize(sizeof(DiagSweepParams<T>) * n);
+    DiagSweepParams<T> *ps = reinterpret_cast<DiagSweepParams<T> *>(M.host.data());
+    for (int r = 0; r < n; ++r) {
+        ps[r] = base;
+        ps[r].qref = arr[r];
+        ps[r].psi_ref = arr[r];
+        ps[r].tin_x = tr + (ntx + nty) * r;
+        ps[r].tin_y = tr + (ntx + nty) * r + ntx;
+        if (per) {
+            ps[r].xtr = xtr + per * r;
+            ps[r].abort_flag = abort_flag_of(c);
+        }
+    }
+    M.items.clear();
+    for (int band = 0; band < nb; ++band)
+        for (int r = 0; r < n; ++r)
+            for (int q = 0; q < g.nq; ++q) {
+                M.items.push_back(r);
+                M.items.push_back(band * g.nq + q);
+            }
+    TSW_HIP(hipMemcpyAsync(M.params.reserve(M.host.size()), M.host.data(), M.host.size(), hipMemcpyHostToDevice, s));
+    TSW_HIP(hipMemcpyAsync(M.dev_items.reserve(sizeof(int) * M.items.size()), M.items.data(),
+                           sizeof(int) * M.items.size(), hipMemcpyHostToDevice, s));
+    DiagSweepParams<T> rep{};
+    rep.qref = reinterpret_cast<const double *>(1);
+    rep.ainv = base.ainv;
+    rep.split = 1;
+    rep.prefetch = c->batch_split_pf_opt > 0 ? c->batch_split_pf_opt : kSplitBatchPrefetch;
+    if (c->batch_persistent_opt > 0) {
+        rep.work = static_cast<int *>(M.work.reserve(sizeof(int)));
+        TSW_HIP(hipMemsetAsync(rep.work, 0, sizeof(int), s));
+    }
+    const DiagSweepParams<T> *dp = M.params.as<DiagSweepParams<T>>();
+    if (c->stall_test && per) {  // tuning key stall_test: every bounded wait of the banded sweep gives up
+        c->stall_test = false;
+        int *flag = abort_flag_of(c);
+        *c->abort_host = 1;
+        TSW_HIP(hipMemcpyAsync(flag, c->abort_host, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    TSW_HIP(hipEventRecord(e0, s));
+    launch_diag_inflow_batch<T>(dp, n, g.p, s);
+    launch_diag_sweep_batch<T>(rep, dp, M.dev_items.as<int2>(), (int)M.items.size() / 2, base.band_rows, lds, g.p, s);
+    TSW_HIP(hipEventRecord(e1, s));
+    c->sweeps_done += n;
+    TSW_HIP(hipStreamSynchronize(s));  // the host tables may be rebuilt by the next launch
+}
+
+// The lane-split sweep on the caller's layout of arrays arr (16-byte aligned, ref_sweep_ok) when it is
+// built for this problem, in precision T; false (nothing done) otherwise.
+template <typename T>
+bool try_split_ref(tsw_ctx *c, const std::vector<double *> &arr, const double *sig, double dt) {
+    for (double *a : arr)
+        if (!ref_sweep_ok(c, a)) return false;
+    DiagSweepParams<T> base;
+    size_t lds = 0;
+    if (!split_ref_plan<T>(c, sig, dt, base, lds)) return false;
+    split_ref_sweep<T>(c, base, lds, arr);
+    c->last_sl = 1;
+    return true;
 }
 
 // SweepOperator::Apply with reflecting edges (SweepPatternUniCartWavefront.cpp:180-245): the quadrants
@@ -2417,6 +2554,9 @@ void sweep_device(tsw_ctx *ctx, double *psi, const double *sig, double dt, int p
     if (lanes_ok(ctx) && precision != TSW_MIXED) {
         if (precision == TSW_FP64) apply_lanes<double>(ctx, psi, sig, dt);
         else apply_lanes<float>(ctx, psi, sig, dt);
+    } else if (precision == TSW_FP32 ? try_split_ref<float>(ctx, {psi}, sig, dt)
+                                     : try_split_ref<double>(ctx, {psi}, sig, dt)) {
+        // in place on the caller's layout; TSW_MIXED: the FP64 sweep (the same FP64 arrays either way)
     } else if (staggered_ok(ctx)) {
         if (precision == TSW_FP64) apply_staggered<double>(ctx, psi, sig, dt);
         else if (precision == TSW_FP32) apply_staggered<float>(ctx, psi, sig, dt);
@@ -2994,6 +3134,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "batch_split_rows") {
             require(value == 0 || (value >= 16 && value <= 256), TSW_E_ARG, "batch_split_rows must be 0 or 16..256");
             ctx->batch_split_rows_opt = value;
+        } else if (k == "ref_split") {
+            require(value == -1 || value == 1, TSW_E_ARG, "ref_split must be -1 or 1");
+            ctx->ref_split_opt = value;
         } else if (k == "batch_persistent") {
             require(value == 0 || value == 1, TSW_E_ARG, "batch_persistent must be 0 or 1");
             ctx->batch_persistent_opt = value;
@@ -3060,6 +3203,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "batch_split_rows") *value = ctx->batch_split_rows_opt;
         else if (k == "batch_xcd_tile") *value = ctx->batch_xcd_tile_opt;
         else if (k == "batch_persistent") *value = ctx->batch_persistent_opt;
+        else if (k == "ref_split") *value = ctx->ref_split_opt;
         else if (k == "batch_split_prefetch")
             *value = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
@@ -3673,17 +3817,6 @@ namespace {
 void batch_choose_engines(tsw_ctx *ctx);
 void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members);
 
-// Lanes per mesh row of the lane-split kernel for DG degree p (split_lanes<p + 1>).
-int split_lanes_of(int p) { return p == 1 ? 2 : 4; }
-
-// Rows per band of a batch member on the lane-split engine: the fewest bands of at most R rows (R: the
-// tuning key batch_split_rows, else the rows one workgroup of 512 threads holds), split evenly.
-int split_band_rows(const tsw_ctx *ctx, int p, int ny) {
-    int R = 512 / split_lanes_of(p);
-    if (ctx->batch_split_rows_opt > 0) R = std::min(R, ctx->batch_split_rows_opt);
-    const int nb = (ny + R - 1) / R;
-    return (ny + nb - 1) / nb;
-}
 }  // namespace
 
 extern "C" {

@@ -148,14 +148,18 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
  * "batch_engine_p3" (the engine tsw_batch_set chose for that DG degree: 0 staggered through conversions,
  * 1 staggered on the caller's layout, 2 ordinate lanes, 3 lane-split on the caller's layout, -1 none in
  * the batch), or the setting of "batch_direct", "ref_stage", "batch_split", "batch_split_rows",
- * "batch_split_prefetch" (the depth in use), "autotune", "use_dsa", "dsa_cycles", "si_graph",
- * "refine_steps", "gmres_refine". */
+ * "batch_split_prefetch" (the depth in use), "batch_persistent", "batch_xcd_tile", "ref_split", "autotune",
+ * "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
  * cells are overwritten.  dt <= 0 or +inf is steady state.  SweepOperator::Apply
  * (SweepOperator.hpp:126-131, SweepOperator.cpp:118-135).  precision TSW_FP32 converts psi and sigma
- * to single precision for the sweep and back; TSW_MIXED refines that FP32 sweep to FP64 accuracy. */
+ * to single precision for the sweep and back; TSW_MIXED refines that FP32 sweep to FP64 accuracy.
+ * With the hip-staggered pattern, DG degree 1-3, the staggered tuning keys at their defaults and a
+ * 16-byte aligned device array (host arrays are staged into one), the lane-split kernel sweeps psi in
+ * place on this layout with no layout conversion (tuning key "ref_split", 1 default, -1 off); FP32 then
+ * computes in single precision from and into the FP64 array, and TSW_MIXED is the FP64 sweep. */
 int tsw_sweep_apply(tsw_ctx *ctx, double *psi_inout, const double *sigma_t, double dt, int precision, int mem);
 
 /* Many right-hand sides: psi_inout[r] <- L^{-1} psi_inout[r] for r < n_rhs, each as tsw_sweep_apply would

@@ -139,6 +139,7 @@ def test_autotuner_choice_is_valid_and_result_identical():
     outs = []
     for autotune in (1, 0):
         with _ctx(g, o, 0) as c:
+            c.set_tuning("ref_split", -1)  # the diagonal-layout kernels (not the lane split on the caller's layout)
             c.set_tuning("autotune", autotune)
             for prec in (tsw.FP64, tsw.FP32):
                 for _ in range(2):
