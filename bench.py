@@ -280,7 +280,8 @@ def bench_batched(args, steps=None, warmup=None, cpu_sweeps=None):
               f"of {ms / max(n, 1):.3f} ms kernels each", file=sys.stderr, flush=True)
     tuned = {k: ctx.get_tuning(k) for k in ("batch_direct", "prefetch", "lanes", "batch_ref", "batch_split",
                                              "batch_split_prefetch", "batch_engine_p1", "batch_engine_p2",
-                                             "batch_engine_p3")}
+                                             "batch_engine_p3", "batch_split_rows_p1", "batch_split_rows_p2",
+                                             "batch_split_rows_p3")}
     ctx.close()
     engine_names = {0: "staggered, layout conversions", 1: "staggered, caller's layout", 2: "ordinate lanes",
                     3: "lane-split staggered, caller's layout"}

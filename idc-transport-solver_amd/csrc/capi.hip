@@ -204,6 +204,7 @@ struct tsw_ctx {
     // batch engines: 1 / -1 force / forbid the lane-split groups (tuning key batch_split); their rows per
     // band (batch_split_rows, 0: the most one workgroup holds) and prefetch depth (batch_split_prefetch)
     int batch_split_opt = 0, batch_split_rows_opt = 0, batch_split_pf_opt = 0;
+    int split_rows_p[4] = {0, 0, 0, 0};  // rows per band of the lane-split engine per DG degree (autotuned; 0: most)
     // lane-split groups: runs of this many consecutive items go to one XCD (tuning key batch_xcd_tile;
     // 0 or 1: dispatch order)
     int batch_xcd_tile_opt = 0;
@@ -2392,11 +2393,13 @@ void multi_sweep(tsw_ctx *c, const std::vector<double *> &arr, const double *sig
 // Lanes per mesh row of the lane-split kernel for DG degree p (split_lanes<p + 1>).
 int split_lanes_of(int p) { return p == 1 ? 2 : 4; }
 
-// Rows per band of a problem (a batch member, or a single problem) on the lane-split kernel: the fewest bands of at most R rows (R: the
-// tuning key batch_split_rows, else the rows one workgroup of 512 threads holds), split evenly.
+// Rows per band of a problem (a batch member, or a single problem) on the lane-split kernel: the fewest
+// bands of at most R rows (R: the tuning key batch_split_rows, else the batch autotuner's choice for the
+// degree, else the rows one workgroup of 512 threads holds), split evenly.
 int split_band_rows(const tsw_ctx *ctx, int p, int ny) {
     int R = 512 / split_lanes_of(p);
-    if (ctx->batch_split_rows_opt > 0) R = std::min(R, ctx->batch_split_rows_opt);
+    const int want = ctx->batch_split_rows_opt > 0 ? ctx->batch_split_rows_opt : ctx->split_rows_p[p & 3];
+    if (want > 0) R = std::min(R, want);
     const int nb = (ny + R - 1) / R;
     return (ny + nb - 1) / nb;
 }
@@ -3208,6 +3211,8 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
             *value = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
         else if (k.rfind("batch_engine_p", 0) == 0 && k.size() == 15 && k[14] >= '0' && k[14] <= '3')
             *value = ctx->batch_engine[k[14] - '0'];
+        else if (k.rfind("batch_split_rows_p", 0) == 0 && k.size() == 19 && k[18] >= '0' && k[18] <= '3')
+            *value = ctx->split_rows_p[k[18] - '0'];
         else if (k == "lanes") *value = ctx->batch.empty() ? ctx->last_lanes : (ctx->lane_groups.empty() ? 0 : 1);
         else if (k == "lanes_block") *value = ctx->last_lanes_J;
         else if (k == "lanes_waves") *value = ctx->last_lanes_W;
@@ -3831,6 +3836,7 @@ int tsw_batch_set(tsw_ctx *ctx, int nprob, const tsw_problem *probs, const doubl
         ctx->engine.assign(nprob, tsw_ctx::kEngConverted);
         ctx->lanes_able.assign(nprob, 0);
         ctx->split_able.assign(nprob, 0);
+        for (int &r : ctx->split_rows_p) r = 0;
         ctx->batch_all.resize(nprob);
         for (int b = 0; b < nprob; ++b) ctx->batch_all[b] = b;
         ctx->batch_dt = dt;
@@ -4382,8 +4388,17 @@ void batch_choose_engines(tsw_ctx *ctx) {
                 cand.push_back(tsw_ctx::kEngConverted);
             if (cand.empty()) cand.push_back(tsw_ctx::kEngConverted);
         }
+        // the lane-split engine is tried with the most rows one workgroup holds and with 64-row bands
+        // (smaller workgroups, more of them per CU where the tables leave room), unless rows are set
+        std::vector<std::pair<char, int>> tries;
+        for (char e : cand) {
+            tries.emplace_back(e, 0);
+            if (e == tsw_ctx::kEngSplit && ctx->batch_split_rows_opt == 0) tries.emplace_back(e, 64);
+        }
         char best = cand[0];
-        if (cand.size() > 1 && ctx->autotune) {
+        int best_rows = 0;
+        ctx->split_rows_p[p] = 0;
+        if (tries.size() > 1 && ctx->autotune) {
             // the heaviest members up to the scratch budget
             std::sort(mem.begin(), mem.end(),
                       [&](int a, int b) { return ctx->batch[a]->g.npsi() > ctx->batch[b]->g.npsi(); });
@@ -4407,7 +4422,9 @@ void batch_choose_engines(tsw_ctx *ctx) {
                 off += (ctx->batch[b]->g.npsi() * sizeof(double) + 255) / 256 * 256;
             }
             double best_ms = 0.0;
-            for (char e : cand) {
+            for (size_t t = 0; t < tries.size(); ++t) {
+                const char e = tries[t].first;
+                ctx->split_rows_p[p] = tries[t].second;
                 for (int b : sub_members) ctx->engine[b] = e;
                 build_batch_groups(ctx, sub_members);
                 batch_apply(ctx, sub_members, dev, TSW_FP64);  // first touch of tables and buffers
@@ -4418,11 +4435,13 @@ void batch_choose_engines(tsw_ctx *ctx) {
                 TSW_HIP(hipEventSynchronize(t1));
                 float ms = 0.f;
                 TSW_HIP(hipEventElapsedTime(&ms, t0, t1));
-                if (e == cand[0] || ms < best_ms) {
+                if (t == 0 || ms < best_ms) {
                     best_ms = ms;
                     best = e;
+                    best_rows = tries[t].second;
                 }
             }
+            ctx->split_rows_p[p] = best_rows;
             sync(ctx);
             ctx->ev_used = 0;  // the tuning sweeps are not counted
             ctx->sweeps_done = 0;

@@ -17,7 +17,7 @@
 //
 // MODE bit 4 (with bit 1, sources only): the source is read from `qref` and psi written to `psi_ref`, both in
 // the reference layout [i][j][q][d][e], where lane g's coefficients are one contiguous run of P doubles
-// (whole 16-byte accesses for even P) and the LG lanes of a row cover the cell's run, so no layout
+// (16-byte accesses, and an 8-byte one for odd P) and the LG lanes of a row cover the cell's run, so no layout
 // conversion is needed and a wavefront's accesses are RPW whole cell runs.  BATCH: the workgroup's
 // parameters are batch[items[b].x], and items[b].y = band * nq + ordinate, numbered band-major so that a
 // band's upstream band is always dispatched first; items with a negative problem index are padding.
@@ -80,22 +80,31 @@ __device__ __forceinline__ T group_bcast(T v, int K) {
     }
 }
 
-// P contiguous doubles of a reference-layout run at byte offset off (16-byte accesses for even P)
+// P contiguous doubles of a reference-layout run at byte offset off: 16-byte accesses, then an 8-byte one
+// for odd P (off is a multiple of 8; buffer accesses of several dwords need only dword alignment)
 template <int P>
 __device__ __forceinline__ void ref_run_ld(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[P]) {
-    if constexpr (P % 2 == 0) {
-        typedef unsigned V4 __attribute__((ext_vector_type(4)));
-        typedef unsigned V2 __attribute__((ext_vector_type(2)));
+    typedef unsigned V4 __attribute__((ext_vector_type(4)));
+    typedef unsigned V2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-        for (int i = 0; i < P / 2; ++i) {
-            const V4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 16 * i, 0);
-            v[2 * i] = __builtin_bit_cast(double, V2{x[0], x[1]});
-            v[2 * i + 1] = __builtin_bit_cast(double, V2{x[2], x[3]});
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = buf_ld<double>(r, off, 8 * k);
+    for (int i = 0; i < P / 2; ++i) {
+        const V4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 16 * i, 0);
+        v[2 * i] = __builtin_bit_cast(double, V2{x[0], x[1]});
+        v[2 * i + 1] = __builtin_bit_cast(double, V2{x[2], x[3]});
     }
+    if constexpr (P % 2 != 0) v[P - 1] = buf_ld<double>(r, off, 8 * (P - 1));
+}
+// v (as doubles) to P contiguous doubles at byte offset off, in the same pieces as ref_run_ld
+template <typename T, int P>
+__device__ __forceinline__ void ref_run_st(__amdgpu_buffer_rsrc_t r, unsigned off, const T (&v)[P]) {
+    typedef unsigned V4 __attribute__((ext_vector_type(4)));
+    typedef unsigned V2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int i = 0; i < P / 2; ++i) {
+        const V2 a = __builtin_bit_cast(V2, (double)v[2 * i]), b = __builtin_bit_cast(V2, (double)v[2 * i + 1]);
+        __builtin_amdgcn_raw_buffer_store_b128(V4{a[0], a[1], b[0], b[1]}, r, (int)off, 16 * i, 0);
+    }
+    if constexpr (P % 2 != 0) buf_st<double>(r, off, 8 * (P - 1), (double)v[P - 1]);
 }
 
 // One item of the lane-split sweep: band wid / nq of ordinate wid % nq of the problem sp, by the whole
@@ -268,7 +277,7 @@ __device__ __forceinline__ void split_item(const DiagSweepParams<T> &sp, unsigne
                 }
                 if (active && owner) {
                     if constexpr (REF) {
-                        ref_st<T, P>(rout, ref_off(c, r, nx, ny, sp.nq, q, N, xneg, yneg) + dref, xo);
+                        ref_run_st<T, P>(rout, ref_off(c, r, nx, ny, sp.nq, q, N, xneg, yneg) + dref, xo);
                     } else {
                         const unsigned boff = (unsigned)(cw.off + r) * (unsigned)sizeof(T) + dplane;
 #pragma unroll

@@ -117,7 +117,8 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * kernel on the caller's layout, and keeps the fastest.  Key "batch_split" (0 default, 1, -1): that
  * lane-split engine (DG degrees 1-3, row bands with exchange records, a band's workgroup at most 512
  * threads) forced, forbidden or autotuned; "batch_split_rows" (0 default = the most rows one workgroup
- * holds: 256 at degree 1, 128 at degrees 2-3; or 16..256): rows per band; "batch_split_prefetch" (0
+ * holds: 256 at degree 1, 128 at degrees 2-3, or 64 where the batch autotuner finds that faster; or
+ * 16..256): rows per band; "batch_split_prefetch" (0
  * default = 4, or 2, 3, 6): its prefetch depth; "batch_xcd_tile" (0 default = dispatch order, or
  * 2..64): runs of that many consecutive items (neighbouring ordinates of a member) go to one XCD (with
  * "batch_persistent" 0); "batch_persistent" (1 default, 0): each lane-split group is one persistent launch
@@ -147,7 +148,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
  * tsw_batch_sweep ran staggered or lane-split groups on the caller's layout), "batch_engine_p0" ..
  * "batch_engine_p3" (the engine tsw_batch_set chose for that DG degree: 0 staggered through conversions,
  * 1 staggered on the caller's layout, 2 ordinate lanes, 3 lane-split on the caller's layout, -1 none in
- * the batch), or the setting of "batch_direct", "ref_stage", "batch_split", "batch_split_rows",
+ * the batch), "batch_split_rows_p0" .. "batch_split_rows_p3" (the lane-split engine's rows per band the
+ * batch autotuner chose for that degree, 0: the most a workgroup holds), or the setting of
+ * "batch_direct", "ref_stage", "batch_split", "batch_split_rows",
  * "batch_split_prefetch" (the depth in use), "batch_persistent", "batch_xcd_tile", "ref_split", "autotune",
  * "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
