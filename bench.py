@@ -449,16 +449,26 @@ def main():
     U = P.unknowns
     ncell = P.nx * P.ny
 
-    def contract_sweeps(precision):
-        """Average kernel time (ms) of in-place operator-contract sweeps of the lattice source."""
+    def contract_sweeps(precision, ref_split=-1, wall=False):
+        """Average kernel time (ms) of in-place operator-contract sweeps of the lattice source: the kernel on
+        the diagonal layout (ref_split -1, the roofline kernel; the call converts layouts around it) or the
+        lane-split kernel on the caller's layout (ref_split 1, the default of tsw_sweep_apply).  With `wall`,
+        also the average wall time of the whole call."""
+        ctx.set_tuning("ref_split", ref_split)
         ctx.sweep(work, d_st, precision=precision)  # warm-up (builds tables)
         ctx.sweep_timing(reset=True)
+        t_call = 0.0
         for _ in range(args.roofline_sweeps):
             work.copy_(d_src)
             torch.cuda.synchronize()
+            t0 = time.perf_counter()
             ctx.sweep(work, d_st, precision=precision)
+            torch.cuda.synchronize()
+            t_call += time.perf_counter() - t0
         n, ms = ctx.sweep_timing(reset=True)
-        return ms / max(n, 1)
+        ctx.set_tuning("ref_split", 1)
+        k = ms / max(n, 1)
+        return (k, t_call / args.roofline_sweeps * 1e3) if wall else k
 
     if args.band_scan or args.prefetch_scan:
         brs = [int(v) for v in args.band_scan.split(",")] if args.band_scan else [args.band_rows]
@@ -482,6 +492,16 @@ def main():
     print(f"bench: sweep tuning (band_rows, prefetch): fp32 {tuned32} fp64 {tuned64}; lane split fp32 {split32} "
           f"fp64 {split64}", file=sys.stderr, flush=True)
     print(f"bench: contract sweep fp32 {t_sw32:.3f} ms fp64 {t_sw64:.3f} ms", file=sys.stderr, flush=True)
+    # whole tsw_sweep_apply calls (SweepOperator::Apply) on device arrays: in place on the caller's layout by
+    # the lane-split kernel (the default) and through the layout conversions
+    apply_calls = {}
+    for name, prec in (("fp64", tsw.FP64), ("fp32", tsw.FP32), ("mixed", tsw.MIXED)):
+        k1, w1 = contract_sweeps(prec, 1, wall=True)
+        k0, w0 = contract_sweeps(prec, -1, wall=True)
+        apply_calls[name] = {"call_ms": w1, "kernel_ms": k1, "through_conversions_call_ms": w0}
+    print("bench: tsw_sweep_apply calls (caller's layout / through conversions, ms): " +
+          ", ".join(f"{n} {v['call_ms']:.3f} / {v['through_conversions_call_ms']:.3f}" for n, v in apply_calls.items()),
+          file=sys.stderr, flush=True)
     if args.copy_probe:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         b = torch.empty_like(a)
@@ -549,6 +569,7 @@ def main():
                   "solve_sweep_ms": ms32 / max(n32, 1), "contract_sweep_ms": t_sw32},
         "speedup_mixed_vs_fp64": (w64 / w32),
         "mixed_rel_err_vs_fp64": err,
+        "sweep_apply_calls": apply_calls,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "kernel": f"{kernel32} (FP32 in-place operator-contract sweep, band_rows {tuned32[0]})",
