@@ -249,7 +249,8 @@ def test_split_batch_matches_oracle(precision, rows):
 
 def test_split_batch_misaligned_array_takes_the_conversions():
     """A device array 8 bytes off a 16-byte boundary sends the call through the layout conversions, where
-    the lane-split groups sweep the diagonal layout: the same bits as the in-place call."""
+    the lane-split groups sweep the diagonal layout: the same bits as the in-place call (FP64, FP32), and
+    TSW_MIXED (FP32 sweeps refined in FP64 there) within 1e-10 of the oracle."""
     import torch
     batch = problems.batched(seed=14, count=5, n_range=(20, 50), degrees=(1, 2, 3), orders=(4, 8))
     srcs = _random_halo_sources(batch, 9)
@@ -270,6 +271,14 @@ def test_split_batch_misaligned_array_takes_the_conversions():
             torch.cuda.synchronize()
             for P, a, b in zip(batch, aligned, arrs):
                 assert torch.equal(a, b), (prec, P.name)
+        # TSW_MIXED: FP64 in place when aligned; through the conversions, FP32 sweeps of the lane-split groups
+        # on the diagonal layout with one FP64 refinement (their correction sweeps)
+        arrs = [shifted] + [torch.from_numpy(s).to(dev) for s in srcs[1:]]
+        shifted.copy_(torch.from_numpy(srcs[0]))
+        ctx.batch_sweep(arrs, precision=tsw.MIXED)
+        assert ctx.get_tuning("batch_ref") == 0
+        for P, s_, a in zip(batch, srcs, arrs):
+            assert rel_err(a.cpu().numpy(), oracle_sweep(P, s_)) <= 1e-10, P.name
 
 
 def test_split_batch_band_stall_is_reported_and_recovered():
