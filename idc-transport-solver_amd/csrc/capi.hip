@@ -163,7 +163,7 @@ struct tsw_ctx {
         std::vector<int> members;
         std::vector<int4> items;
         std::vector<unsigned char> host;  // the parameter table on its way to the device
-        DBuf params, dev_items;
+        DBuf params, dev_items, work;     // work: the item counter of a persistent launch
     };
     std::vector<LaneGroup> lane_groups;
     // Adjoint problem: the same mesh with the negated ordinate set in reverse order (its sweeps are the
@@ -4052,7 +4052,12 @@ void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStre
         case 2: launch_lanes_inflow_batch_p<T, 3>(dp, n, s); break;
         default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
     }
-    const LanesParams<T> &rep = reinterpret_cast<const LanesParams<T> *>(LG.host.data())[0];  // the mode
+    LanesParams<T> rep = reinterpret_cast<const LanesParams<T> *>(LG.host.data())[0];  // the mode
+    rep.work = nullptr;
+    if (ctx->batch_persistent_opt > 0 && LG.work.ptr) {  // persistent launch, the counter starting at zero
+        rep.work = LG.work.as<int>();
+        TSW_HIP(hipMemsetAsync(rep.work, 0, sizeof(int), s));
+    }
     lanes_launch<T>(LG.p, rep, threads, lds, s, dp, LG.dev_items.as<int4>(), (int)LG.items.size());
 }
 
@@ -4072,6 +4077,7 @@ void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
                 it = lid.emplace(sub->g.p, (int)ctx->lane_groups.size()).first;
                 ctx->lane_groups.emplace_back();
                 ctx->lane_groups.back().p = sub->g.p;
+                ctx->lane_groups.back().work.reserve(sizeof(int));
             }
             ctx->lane_groups[it->second].members.push_back(b);
             continue;

@@ -54,36 +54,19 @@ struct LanesLds {  // LDS carve-up in bytes, every array 16-byte aligned
     }
 };
 
-// The parameters of a workgroup: the kernel argument, or a batch member's entry (a uniform load from a
-// read-only table, so its fields stay in scalar registers).
-template <bool BATCH, typename T>
-__device__ __forceinline__ LanesParams<T> lanes_params_of(const LanesParams<T> &sp1,
-                                                          const LanesParams<T> *__restrict__ batch, int member) {
-    if constexpr (BATCH) return batch[member];
-    else return sp1;
-}
-
 // vmcnt(0) (expcnt, lgkmcnt left at their maxima): wait for every outstanding vector memory operation.
 constexpr int kWaitVmcnt0 = 0x0F70;
 
-// One problem (sp1; workgroup = band x ordinate block) or, with BATCH, a batch (items[workgroup] = {problem,
-// block, band}, parameters batch[problem]; items are band-major, so a band's upstream neighbour is dispatched
-// before it).
-template <typename T, int P, int MODE, int D, bool BATCH>
-__global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp1, const LanesParams<T> *__restrict__ batch,
-                                                     const int4 *__restrict__ items) {
+// One item of the ordinate-lane sweep: band `band` of ordinate block `bi` of the problem sp, by the whole
+// workgroup (the storer wave returns after its last store, the compute waves after their last step).
+template <typename T, int P, int MODE, int D>
+__device__ __forceinline__ void lanes_item(const LanesParams<T> &sp, int bi, int band) {
     constexpr int N = P * P, NN = N * N;
     constexpr bool HASQ = (MODE & 1) != 0, HASPHI = (MODE & 2) != 0, MOM = (MODE & 4) != 0;
     using Lds = LanesLds<T, P>;
     constexpr int PS = Lds::PS;
     using U = typename ExchangeSentinel<T>::U;
-    int4 item = {0, 0, 0, 0};
-    if constexpr (BATCH) item = items[blockIdx.x];
-    const LanesParams<T> sp = lanes_params_of<BATCH>(sp1, batch, item.x);
-    if (sp.skip && *G(sp.skip)) return;  // a replayed iteration after the loop has stopped
     const int nb = sp.nbands < 2 ? 1 : sp.nbands;
-    const int bi = BATCH ? item.y : (int)(blockIdx.x % (unsigned)sp.nblk);
-    const int band = BATCH ? item.z : (int)(blockIdx.x / (unsigned)sp.nblk);
     const LaneBlock bk = sp.blk[bi];
     const int nx = sp.nx, ny = sp.ny, nq = sp.nq, J = sp.J, RW = sp.RW;
     const bool xneg = bk.quad >= 2, yneg = (bk.quad & 1) != 0;
@@ -397,13 +380,54 @@ __global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp1, const L
     if (MOM) reduce(nsteps - 1);
 }
 
+// One problem (sp1; workgroup = band x ordinate block) or, with BATCH, a batch: items[k] = {problem, block,
+// band} with parameters batch[problem], band-major (a band's upstream neighbour comes first); one item per
+// workgroup, or (work != null) a persistent launch whose workgroups take item indices in order from *work
+// until nwork are taken, so an item's upstream band has always been taken by a running workgroup.
+template <typename T, int P, int MODE, int D, bool BATCH>
+__global__ __launch_bounds__(256) void k_lanes_sweep(LanesParams<T> sp1, const LanesParams<T> *__restrict__ batch,
+                                                     const int4 *__restrict__ items, int *__restrict__ work,
+                                                     int nwork) {
+    if constexpr (!BATCH) {
+        if (sp1.skip && *G(sp1.skip)) return;  // a replayed iteration after the loop has stopped
+        lanes_item<T, P, MODE, D>(sp1, (int)(blockIdx.x % (unsigned)sp1.nblk), (int)(blockIdx.x / (unsigned)sp1.nblk));
+    } else if (work == nullptr) {
+        const int4 item = items[blockIdx.x];
+        const LanesParams<T> sp = batch[item.x];
+        if (sp.skip && *G(sp.skip)) return;
+        lanes_item<T, P, MODE, D>(sp, item.y, item.z);
+    } else {
+        __shared__ int next;
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(work, 1);
+            __syncthreads();
+            const int k = __builtin_amdgcn_readfirstlane(next);  // workgroup-uniform: scalar item loads
+            __syncthreads();  // every thread has read `next` before it is taken again
+            if (k >= nwork) break;  // workgroup-uniform: every wave leaves
+            const int4 item = items[k];
+            const LanesParams<T> sp = batch[item.x];
+            if (sp.skip && *G(sp.skip)) continue;
+            lanes_item<T, P, MODE, D>(sp, item.y, item.z);
+        }
+    }
+}
+
 template <typename T, int P, int MODE, int D>
 void launch_lanes_dd(const LanesParams<T> &sp, int threads, size_t lds, hipStream_t s, const LanesParams<T> *batch,
                      const int4 *items, int nitems) {
     auto go = [&](auto kern, int blocks) {
         if (lds > 64 * 1024)
             TSW_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, sp, batch, items);
+        int grid = blocks;
+        if (batch && sp.work) {  // persistent: as many workgroups as the device holds at once (counter zeroed)
+            int per_cu = 0, dev = 0, cus = 0;
+            TSW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, threads, lds));
+            TSW_HIP(hipGetDevice(&dev));
+            TSW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            grid = std::max(1, std::min(blocks, std::max(per_cu, 1) * cus));
+        }
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, sp, batch, items, batch ? sp.work : nullptr,
+                           blocks);
         TSW_HIP(hipGetLastError());
     };
     if (!batch) {

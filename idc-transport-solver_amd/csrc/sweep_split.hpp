@@ -334,7 +334,7 @@ __global__ __launch_bounds__(512) void k_diag_sweep_split(DiagSweepParams<T> sp1
         for (;;) {
             if (threadIdx.x == 0) next = atomicAdd(work, 1);
             __syncthreads();
-            const int k = next;
+            const int k = __builtin_amdgcn_readfirstlane(next);  // workgroup-uniform: scalar item loads
             __syncthreads();  // every thread has read `next` before it is taken again
             if (k >= nwork) break;  // workgroup-uniform: every wave leaves
             const gptr<const int> it = G(reinterpret_cast<const int *>(items)) + 2 * (size_t)k;

@@ -170,6 +170,7 @@ template <typename T> struct LanesParams {
     T *xtr;                     // [nq][nbands-1][nx][P] band exchange records
     int *abort_flag;
     const int *skip;            // device flag: the launch does nothing while *skip != 0 (null: always runs)
+    int *work;                  // batch launches (host-side dispatch): counter of a persistent launch, or null
 };
 constexpr int kLanesStoreWaves = 3;  // compute waves per workgroup when a storer wave writes psi
 template <typename T, int P>

@@ -121,9 +121,9 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * 16..256): rows per band; "batch_split_prefetch" (0
  * default = 4, or 2, 3, 6): its prefetch depth; "batch_xcd_tile" (0 default = dispatch order, or
  * 2..64): runs of that many consecutive items (neighbouring ordinates of a member) go to one XCD (with
- * "batch_persistent" 0); "batch_persistent" (1 default, 0): each lane-split group is one persistent launch
- * of as many workgroups as the device holds, taking items in order from a device counter (0: one
- * workgroup per item).  Key "ref_stage" (0 default, 1): sweeps on the caller's layout (batch groups,
+ * "batch_persistent" 0); "batch_persistent" (1 default, 0): each lane-split and ordinate-lane group is one
+ * persistent launch of as many workgroups as the device holds, taking items in order from a device counter
+ * (0: one workgroup per item).  Key "ref_stage" (0 default, 1): sweeps on the caller's layout (batch groups,
  * tsw_sweep_apply_multi) stage their source loads through LDS where it fits.  Results do not depend on
  * any of these settings (bit for bit).
  * Solver keys: "use_dsa" (0 default, 1): diffusion synthetic acceleration of tsw_gmres_solve and

@@ -180,7 +180,8 @@ def test_batch_engines_give_the_same_bits(precision):
     for name, lanes, ref, split, rows, xcd, pers in (
             ("converted", -1, -1, -1, 0, 0, 1), ("ref", -1, 1, -1, 0, 0, 1), ("lanes", 1, -1, -1, 0, 0, 1),
             ("split", -1, -1, 1, 0, 0, 1), ("split16", -1, -1, 1, 16, 0, 1), ("split16xcd", -1, -1, 1, 16, 3, 1),
-            ("split16np", -1, -1, 1, 16, 0, 0), ("split16xcdnp", -1, -1, 1, 16, 3, 0), ("auto", 0, 0, 0, 0, 0, 1)):
+            ("split16np", -1, -1, 1, 16, 0, 0), ("split16xcdnp", -1, -1, 1, 16, 3, 0), ("lanesnp", 1, -1, -1, 0, 0, 0),
+            ("auto", 0, 0, 0, 0, 0, 1)):
         with tsw.Context(0) as ctx:
             ctx.set_tuning("lanes", lanes)
             ctx.set_tuning("batch_ref", ref)
@@ -198,7 +199,7 @@ def test_batch_engines_give_the_same_bits(precision):
                     assert engines[p] == 0
                 elif name == "ref":
                     assert engines[p] == 1
-                elif name == "lanes":  # degree 3 through the conversions (the lane engine is built for 0-2)
+                elif name.startswith("lanes"):  # degree 3 through the conversions (lanes are built for 0-2)
                     assert engines[p] == (2 if p <= 2 else 0)
                 elif name.startswith("split"):  # degree 0 through the conversions (lane split: 1-3)
                     assert engines[p] == (3 if p >= 1 else 0)
@@ -207,7 +208,7 @@ def test_batch_engines_give_the_same_bits(precision):
             out = [s.copy() for s in srcs]
             ctx.batch_sweep(out, precision=precision)
             outs[name] = out
-    for name in ("ref", "lanes", "split", "split16", "split16xcd", "split16np", "split16xcdnp", "auto"):
+    for name in ("ref", "lanes", "split", "split16", "split16xcd", "split16np", "split16xcdnp", "lanesnp", "auto"):
         for P, a, b in zip(batch, outs["converted"], outs[name]):
             np.testing.assert_array_equal(a, b, err_msg=f"{name} {P.name}")
     for P, s, o in zip(batch, srcs, outs["auto"]):
