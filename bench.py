@@ -367,7 +367,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--sweeps-only", action="store_true", help="skip the solves (profiling runs)")
     ap.add_argument("--workload", default="lattice", choices=["lattice", "batched", "si", "idc"])
-    ap.add_argument("--batched-steps", type=int, default=2,
+    ap.add_argument("--batched-steps", type=int, default=5,
                     help="lattice workload: timed calls per precision of the 256-problem batch reported in its "
                          "'batched' sub-object (0: skip)")
     ap.add_argument("--idc-steps", type=int, default=10, help="IDC steps timed by --workload idc (100 reach t=1)")
