@@ -224,6 +224,7 @@ struct tsw_ctx {
     bool si_graph = true;
     int refine_steps = 1;  // FP64 refinement steps of a TSW_MIXED sweep
     bool gmres_refine = false;  // mixed GMRES by iterative refinement instead of the relaxed strategy
+    double fp32_safety = 10.0;  // relaxed GMRES: FP32 products once the estimate is <= abs_tol / (fp32_safety d)
     static constexpr int kSiChunk = 8;
     DBuf si_ctl;
     SiControl *si_ctl_host = nullptr;
@@ -899,7 +900,7 @@ void dsa_apply(tsw_ctx *c, const double *x, double *y) {
 // floor keeps an exactly representable problem (d = 0) from admitting FP32 products at the first step,
 // the ceiling keeps a badly conditioned one from never admitting them (the final FP64 residual check
 // guards the result either way).
-constexpr double kFp32ErrSafety = 10.0;
+constexpr double kFp32ErrSafety = 10.0;  // default of the tuning key fp32_safety
 constexpr double kFp32ErrFloor = 1e-7;
 constexpr double kFp32ErrCeil = 1e-4;
 
@@ -1956,7 +1957,7 @@ bool gmres_staggered(tsw_ctx *c, double *Q, const double *st, const double *ss, 
             }
             fe.value = std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d));
         }
-        const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
+        const double lo_below = abs_tol / (c->fp32_safety * fe.value);
         converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
         // psi = L^{-1}(Q + S phi) in FP64; with FP32 products in the solve, the true residual
         // b - A x = M (P psi - x) comes from the same sweep, and the solve resumes from x while it exceeds
@@ -2187,7 +2188,7 @@ bool gmres_lanes(tsw_ctx *c, double *Q, const double *st, const double *ss, doub
             }
             fe.value = std::min(kFp32ErrCeil, std::max(kFp32ErrFloor, d));
         }
-        const double lo_below = abs_tol / (kFp32ErrSafety * fe.value);
+        const double lo_below = abs_tol / (c->fp32_safety * fe.value);
         converged = solver.solve(apply64, apply32, lo_below, b, x, r, abs_tol, max_its, its_total, res_final, true);
         // the true residual M (P psi - x) from the final FP64 psi; resume while it exceeds abs_tol
         const double *q_src = Q;
@@ -3153,6 +3154,9 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value) {
         } else if (k == "gmres_refine") {
             require(value == 0 || value == 1, TSW_E_ARG, "gmres_refine must be 0 or 1");
             ctx->gmres_refine = value != 0;
+        } else if (k == "fp32_safety") {
+            require(value >= 1 && value <= 1000, TSW_E_ARG, "fp32_safety must be 1..1000");
+            ctx->fp32_safety = value;
         } else if (k == "refine_steps") {
             require(value >= 1 && value <= 8, TSW_E_ARG, "refine_steps must be 1..8");
             ctx->refine_steps = value;
@@ -3197,6 +3201,7 @@ int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value) {
         else if (k == "dsa_cycles") *value = ctx->dsa_cycles;
         else if (k == "refine_steps") *value = ctx->refine_steps;
         else if (k == "gmres_refine") *value = ctx->gmres_refine ? 1 : 0;
+        else if (k == "fp32_safety") *value = (int)ctx->fp32_safety;
         else if (k == "one_wave") *value = ctx->last_w1;
         else if (k == "lane_split") *value = ctx->last_sl;
         else if (k == "batch_direct") *value = ctx->batch_direct_opt;

@@ -134,7 +134,9 @@ int tsw_set_xs_degree(tsw_ctx *ctx, int degree);
  * "si_graph" (1 default, 0): staggered source iteration replays captured graphs of 8 iterations with
  * the loop test on the device (same iterates and counts as the eager loop).  "refine_steps" (1..8,
  * default 1): FP64 refinement steps of a TSW_MIXED sweep.  "gmres_refine" (0 default, 1): mixed-precision
- * GMRES by iterative refinement instead of the relaxed strategy.
+ * GMRES by iterative refinement instead of the relaxed strategy.  "fp32_safety" (1..1000, default 10): the
+ * relaxed strategy admits FP32 products once the residual estimate is <= abs_tol / (fp32_safety d), d the
+ * measured relative error of an FP32 product (the final FP64 residual check guards the result).
  * Test key "stall_test" (1): the next banded staggered sweep starts with the row-band stall flag raised,
  * so its bounded waits give up at once and the call fails with TSW_E_HIP as a real stall would (the
  * exchange records are refilled and later sweeps are unaffected).
@@ -152,7 +154,7 @@ int tsw_set_tuning(tsw_ctx *ctx, const char *key, int value);
  * batch autotuner chose for that degree, 0: the most a workgroup holds), or the setting of
  * "batch_direct", "ref_stage", "batch_split", "batch_split_rows",
  * "batch_split_prefetch" (the depth in use), "batch_persistent", "batch_xcd_tile", "ref_split", "autotune",
- * "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine". */
+ * "use_dsa", "dsa_cycles", "si_graph", "refine_steps", "gmres_refine", "fp32_safety". */
 int tsw_get_tuning(tsw_ctx *ctx, const char *key, int *value);
 
 /* psi <- L^{-1} psi in place; halo cells of psi hold the inflow and are left unchanged; interior
