@@ -4046,8 +4046,9 @@ void lanes_batch_params(tsw_ctx *ctx, const std::vector<double *> &arr, std::vec
     }
 }
 
+// The inflow traces of lane group k's members from their arrays' halos (before the side streams fork).
 template <typename T>
-void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStream_t s) {
+void lanes_batch_inflow(tsw_ctx *ctx, size_t k, hipStream_t s) {
     auto &LG = ctx->lane_groups[k];
     const LanesParams<T> *dp = LG.params.as<LanesParams<T>>();
     const int n = (int)LG.members.size();
@@ -4057,6 +4058,12 @@ void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStre
         case 2: launch_lanes_inflow_batch_p<T, 3>(dp, n, s); break;
         default: throw Error(TSW_E_UNSUPPORTED, "ordinate-lane sweep built for DG degrees 0-2");
     }
+}
+
+template <typename T>
+void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStream_t s) {
+    auto &LG = ctx->lane_groups[k];
+    const LanesParams<T> *dp = LG.params.as<LanesParams<T>>();
     LanesParams<T> rep = reinterpret_cast<const LanesParams<T> *>(LG.host.data())[0];  // the mode
     rep.work = nullptr;
     if (ctx->batch_persistent_opt > 0 && LG.work.ptr) {  // persistent launch, the counter starting at zero
@@ -4178,7 +4185,14 @@ void build_batch_groups(tsw_ctx *ctx, const std::vector<int> &members) {
 
 // Launch of one staggered group in precision T, whose parameter table batch_group_params has built and
 // uploaded on the context's stream before the side streams forked from it: `ref` sweeps the members'
-// arrays dev[b] in place.
+// arrays dev[b] in place (their inflow traces computed before the fork, batch_group_inflow).
+// The inflow traces of a group on the caller's layout from its members' halos, on the context's stream
+// before the side streams fork (so no side stream waits on them behind another stream's sweeps).
+template <typename T>
+void batch_group_inflow(tsw_ctx *ctx, tsw_ctx::Group &G) {
+    launch_diag_inflow_batch<T>(G.ref_params.as<DiagSweepParams<T>>(), (int)G.members.size(), G.p, ctx->stream);
+}
+
 template <typename T>
 void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::vector<double *> *ref, hipStream_t sd) {
     DiagSweepParams<T> rep{};  // representative parameters for the host-side dispatch
@@ -4191,7 +4205,6 @@ void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
         rep.direct = G.ref_direct ? 1 : 0;
         rep.ref_stage = G.ref_stage ? 1 : 0;
         lds = G.ref_lds;
-        launch_diag_inflow_batch<T>(probs, (int)G.members.size(), G.p, sd);
     } else {
         probs = (const DiagSweepParams<T> *)(sizeof(T) == 8 ? G.params64.ptr : (corr ? G.params32c.ptr : G.params32.ptr));
         rep.qsrc = reinterpret_cast<const T *>(1);
@@ -4253,6 +4266,17 @@ void batch_sweep(tsw_ctx *ctx, bool conv64, bool inplace64, bool corr, const std
     }
     hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
     TSW_HIP(hipEventRecord(e0, ctx->stream));
+    // every inflow trace of the sweeps on the caller's layout, before the fork
+    if (lanes)
+        for (size_t j = 0; j < ctx->lane_groups.size(); ++j) {
+            if (inplace64) lanes_batch_inflow<double>(ctx, j, ctx->stream);
+            else lanes_batch_inflow<float>(ctx, j, ctx->stream);
+        }
+    for (auto &gd : diag)
+        if (gd.second) {
+            if (inplace64) batch_group_inflow<double>(ctx, *gd.first);
+            else batch_group_inflow<float>(ctx, *gd.first);
+        }
     TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     int k = 0;
     if (lanes)
