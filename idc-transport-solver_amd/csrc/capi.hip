@@ -4065,11 +4065,8 @@ void lanes_batch_launch(tsw_ctx *ctx, size_t k, int threads, size_t lds, hipStre
     auto &LG = ctx->lane_groups[k];
     const LanesParams<T> *dp = LG.params.as<LanesParams<T>>();
     LanesParams<T> rep = reinterpret_cast<const LanesParams<T> *>(LG.host.data())[0];  // the mode
-    rep.work = nullptr;
-    if (ctx->batch_persistent_opt > 0 && LG.work.ptr) {  // persistent launch, the counter starting at zero
-        rep.work = LG.work.as<int>();
-        TSW_HIP(hipMemsetAsync(rep.work, 0, sizeof(int), s));
-    }
+    // persistent launch: the counter was zeroed on the context's stream before the fork
+    rep.work = ctx->batch_persistent_opt > 0 && LG.work.ptr ? LG.work.as<int>() : nullptr;
     lanes_launch<T>(LG.p, rep, threads, lds, s, dp, LG.dev_items.as<int4>(), (int)LG.items.size());
 }
 
@@ -4216,10 +4213,8 @@ void batch_group_launch(tsw_ctx *ctx, tsw_ctx::Group &G, bool corr, const std::v
     if (G.split) {
         rep.split = 1;
         rep.prefetch = ctx->batch_split_pf_opt > 0 ? ctx->batch_split_pf_opt : kSplitBatchPrefetch;
-        if (ctx->batch_persistent_opt > 0 && G.work.ptr) {  // the counter starts each launch at zero
-            rep.work = G.work.as<int>();
-            TSW_HIP(hipMemsetAsync(rep.work, 0, sizeof(int), sd));
-        }
+        // persistent launch: the counter was zeroed on the context's stream before the fork
+        if (ctx->batch_persistent_opt > 0 && G.work.ptr) rep.work = G.work.as<int>();
     }
     launch_diag_sweep_batch<T>(rep, probs, G.dev_items.as<int2>(), (int)G.items.size() / 2,
                                G.split ? G.rows : G.max_ny, lds, G.p, sd);
@@ -4277,6 +4272,14 @@ void batch_sweep(tsw_ctx *ctx, bool conv64, bool inplace64, bool corr, const std
             if (inplace64) batch_group_inflow<double>(ctx, *gd.first);
             else batch_group_inflow<float>(ctx, *gd.first);
         }
+    if (ctx->batch_persistent_opt > 0) {  // the persistent launches' item counters start at zero
+        if (lanes)
+            for (auto &LG : ctx->lane_groups)
+                if (LG.work.ptr) TSW_HIP(hipMemsetAsync(LG.work.ptr, 0, sizeof(int), ctx->stream));
+        for (auto &gd : diag)
+            if (gd.first->split && gd.first->work.ptr)
+                TSW_HIP(hipMemsetAsync(gd.first->work.ptr, 0, sizeof(int), ctx->stream));
+    }
     TSW_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     int k = 0;
     if (lanes)
